@@ -1,0 +1,870 @@
+// C-ABI layer: contexts, box tables, descriptor tables, workspace, ghost-fill
+// task lists, stage dispatch (fused or pass-structured kernels), CNS::advance
+// sequencing, hipGraph capture and reductions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "amrsolver_hip.h"
+#include "kernels.h"
+
+using namespace amr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& m) {
+  g_err = m;
+  return code;
+}
+
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess) return fail(AMR_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int64_t cells(const int lo[3], const int hi[3]) {
+  return int64_t(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+}
+
+// amr_fab_t for a compact array of `nc` components on [lo-ng, hi+ng]
+amr_fab_t compact_fab(const int lo[3], const int hi[3], int ng, double* base) {
+  amr_fab_t f;
+  std::memset(&f, 0, sizeof f);
+  for (int d = 0; d < 3; ++d) {
+    f.lo[d] = lo[d];
+    f.hi[d] = hi[d];
+  }
+  f.ng = ng;
+  f.data = base;
+  int64_t nx = hi[0] - lo[0] + 1 + 2 * ng, ny = hi[1] - lo[1] + 1 + 2 * ng, nz = hi[2] - lo[2] + 1 + 2 * ng;
+  f.jstride = nx;
+  f.kstride = nx * ny;
+  f.nstride = nx * ny * nz;
+  return f;
+}
+int64_t compact_len(const int lo[3], const int hi[3], int ng, int nc) {
+  int64_t nx = hi[0] - lo[0] + 1 + 2 * ng, ny = hi[1] - lo[1] + 1 + 2 * ng, nz = hi[2] - lo[2] + 1 + 2 * ng;
+  return nx * ny * nz * nc;
+}
+
+// Address of cell (0,0,0) comp 0 plus strides: two descriptors with the same
+// mapping address the same memory.
+struct Mapping {
+  intptr_t origin;
+  int64_t js, ks, ns;
+  bool operator==(const Mapping& o) const {
+    return origin == o.origin && js == o.js && ks == o.ks && ns == o.ns;
+  }
+};
+Mapping mapping_of(const amr_fab_t& f) {
+  Fab v = make_fab(f);
+  return Mapping{intptr_t(f.data) + intptr_t(v.off * 8), f.jstride, f.kstride, f.nstride};
+}
+
+}  // namespace
+
+struct amr_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  // autotuned fused-kernel configuration per (scheme key, box-shape key)
+  std::map<std::string, FusedConfig> tuned;
+};
+
+namespace {
+
+struct DescSet {
+  std::vector<amr_fab_t> host;
+  Fab* d = nullptr;
+  bool fill_built = false;
+  CopyTask* d_copy = nullptr;
+  int ncopy = 0;
+  int64_t ncopy_cells = 0;
+  BcTask* d_bc[3] = {nullptr, nullptr, nullptr};
+  int nbc[3] = {0, 0, 0};
+  int64_t nbc_cells[3] = {0, 0, 0};
+  int64_t last_use = 0;
+  ~DescSet() {
+    if (d) (void)hipFree(d);
+    if (d_copy) (void)hipFree(d_copy);
+    for (int i = 0; i < 3; ++i)
+      if (d_bc[i]) (void)hipFree(d_bc[i]);
+  }
+};
+
+struct TileTable {
+  TileDev* d = nullptr;
+  int n = 0;
+  ~TileTable() {
+    if (d) (void)hipFree(d);
+  }
+};
+
+struct GraphEntry {
+  std::vector<char> key;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  ~GraphEntry() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+}  // namespace
+
+struct amr_boxtab_s {
+  amr_ctx_t ctx = nullptr;
+  amr_geom_t geom;
+  BoxTable bt;
+  std::vector<std::unique_ptr<DescSet>> descs;
+  int64_t use_clock = 0;
+  // pass-structured workspace
+  double* uw_mem = nullptr;
+  int uw_kinds = 0;  // bitmask of allocated kinds
+  UnfusedWork uw;
+  std::vector<void*> uw_tables;
+  // advance workspace (two ghosted state copies mirroring the caller layout)
+  double* ws_mem[2] = {nullptr, nullptr};
+  std::vector<amr_fab_t> ws_fabs[2];
+  std::vector<amr_fab_t> ws_key;
+  // reductions
+  double* d_partial = nullptr;
+  double* d_result = nullptr;
+  double* h_result = nullptr;
+  // fused tiles per config
+  std::map<std::tuple<int, int, int>, std::unique_ptr<TileTable>> tiles;
+  std::vector<std::unique_ptr<GraphEntry>> graphs;
+};
+
+namespace {
+
+int check_scheme(const amr_scheme_t& s) {
+  if (s.art_diss != 0) return fail(AMR_ERR_ARG, "art_diss=1 (JST dissipation) is not supported");
+  if (s.rhs_source != 0) return fail(AMR_ERR_ARG, "rhs_source=1: user_source is problem code and is not supported");
+  if (!s.rhs_euler && s.rhs_visc)
+    return fail(AMR_ERR_ARG, "rhs_euler=0 with rhs_visc=1 is not supported (the reference accumulates into an "
+                             "uninitialised flux array)");
+  if (s.rhs_euler && (s.flux_euler < 0 || s.flux_euler > 2)) return fail(AMR_ERR_ARG, "flux_euler must be 0, 1 or 2");
+  if (s.rhs_euler && s.flux_euler == 1 && !(s.order_keep == 2 || s.order_keep == 4 || s.order_keep == 6))
+    return fail(AMR_ERR_ARG, "order_keep must be 2, 4 or 6");
+  return AMR_OK;
+}
+
+int check_closures(const amr_closures_t& c) {
+  if (!(c.gamma > 1.0) || !(c.Rspec > 0.0) || !(c.cv > 0.0))
+    return fail(AMR_ERR_ARG, "closures: gamma > 1, Rspec > 0 and cv > 0 required");
+  if (c.visc_law < 0 || c.visc_law > 1) return fail(AMR_ERR_ARG, "closures: visc_law must be 0 or 1");
+  if (c.cond_law < 0 || c.cond_law > 2) return fail(AMR_ERR_ARG, "closures: cond_law must be 0, 1 or 2");
+  if (c.cond_law == 2 && !(c.Pr > 0.0)) return fail(AMR_ERR_ARG, "closures: Pr > 0 required for cond_law 2");
+  return AMR_OK;
+}
+
+// Descriptor set for `fabs` (nbox entries), uploaded once and cached.
+int get_desc(amr_boxtab_s* bt, const amr_fab_t* fabs, int min_ng, DescSet** out) {
+  int nb = bt->bt.nbox;
+  size_t bytes = sizeof(amr_fab_t) * nb;
+  for (auto& d : bt->descs) {
+    if (std::memcmp(d->host.data(), fabs, bytes) == 0) {
+      d->last_use = ++bt->use_clock;
+      *out = d.get();
+      return AMR_OK;
+    }
+  }
+  for (int b = 0; b < nb; ++b) {
+    const amr_fab_t& f = fabs[b];
+    const BoxDev& B = bt->bt.h_boxes[b];
+    for (int d = 0; d < 3; ++d)
+      if (f.lo[d] != B.lo[d] || f.hi[d] != B.hi[d])
+        return fail(AMR_ERR_ARG, "fab " + std::to_string(b) + " does not match box " + std::to_string(b));
+    if (f.ng < min_ng)
+      return fail(AMR_ERR_ARG, "fab " + std::to_string(b) + " has " + std::to_string(f.ng) + " ghost cells, " +
+                                   std::to_string(min_ng) + " required");
+    if (!f.data) return fail(AMR_ERR_ARG, "fab " + std::to_string(b) + " has a null data pointer");
+    int64_t nx = f.hi[0] - f.lo[0] + 1 + 2 * f.ng, ny = f.hi[1] - f.lo[1] + 1 + 2 * f.ng;
+    int64_t nz = f.hi[2] - f.lo[2] + 1 + 2 * f.ng;
+    if (f.jstride < nx || f.kstride < f.jstride * ny || f.nstride < f.kstride * nz)
+      return fail(AMR_ERR_ARG, "fab " + std::to_string(b) + " strides do not cover its box");
+  }
+  if (bt->descs.size() >= 24) {
+    auto it = std::min_element(bt->descs.begin(), bt->descs.end(),
+                               [](const auto& a, const auto& b2) { return a->last_use < b2->last_use; });
+    bt->descs.erase(it);
+  }
+  auto ds = std::make_unique<DescSet>();
+  ds->host.assign(fabs, fabs + nb);
+  std::vector<Fab> h(nb);
+  for (int b = 0; b < nb; ++b) h[b] = make_fab(fabs[b]);
+  HIPCHK(hipMalloc(&ds->d, sizeof(Fab) * nb));
+  HIPCHK(hipMemcpy(ds->d, h.data(), sizeof(Fab) * nb, hipMemcpyHostToDevice));
+  ds->last_use = ++bt->use_clock;
+  *out = ds.get();
+  bt->descs.push_back(std::move(ds));
+  return AMR_OK;
+}
+
+// BC code -> boundary type per component (src/set/CNS_setup.cpp:18-31):
+// 0 int_dir (nothing), 1 ext_dir, 2 foextrap, 3 reflect_even, 4 reflect_odd
+int bc_type(int code, int comp, int dir) {
+  static const int scalar_bc[7] = {0, 1, 2, 3, 3, 3, 1};
+  static const int norm_vel_bc[7] = {0, 1, 2, 4, 4, 4, 1};
+  static const int tang_vel_bc[7] = {0, 1, 2, 3, 3, 4, 1};
+  if (code < 0 || code > 6) return -1;
+  if (comp == URHO || comp == UET) return scalar_bc[code];
+  return (comp - UMX == dir) ? norm_vel_bc[code] : tang_vel_bc[code];
+}
+
+int build_fill(amr_boxtab_s* bt, DescSet* ds) {
+  if (ds->fill_built) return AMR_OK;
+  const amr_geom_t& g = bt->geom;
+  int nb = bt->bt.nbox;
+  int len[3];
+  for (int d = 0; d < 3; ++d) len[d] = g.dom_hi[d] - g.dom_lo[d] + 1;
+  std::vector<Mapping> map(nb);
+  for (int b = 0; b < nb; ++b) map[b] = mapping_of(ds->host[b]);
+  std::vector<CopyTask> copies;
+  int64_t ncell = 0;
+  for (int b = 0; b < nb; ++b) {
+    const amr_fab_t& fb = ds->host[b];
+    int G[2][3];
+    for (int d = 0; d < 3; ++d) {
+      G[0][d] = fb.lo[d] - fb.ng;
+      G[1][d] = fb.hi[d] + fb.ng;
+    }
+    for (int s = 0; s < nb; ++s) {
+      const BoxDev& S = bt->bt.h_boxes[s];
+      for (int sx = -1; sx <= 1; ++sx)
+        for (int sy = -1; sy <= 1; ++sy)
+          for (int sz = -1; sz <= 1; ++sz) {
+            int sv[3] = {sx, sy, sz};
+            bool ok = true;
+            int sh[3];
+            for (int d = 0; d < 3; ++d) {
+              if (sv[d] != 0 && !g.periodic[d]) ok = false;
+              sh[d] = sv[d] * len[d];
+            }
+            if (!ok) continue;
+            bool zero = sx == 0 && sy == 0 && sz == 0;
+            if (zero && (s == b || map[s] == map[b])) continue;
+            CopyTask t;
+            bool empty = false;
+            for (int d = 0; d < 3; ++d) {
+              t.lo[d] = std::max(G[0][d], S.lo[d] + sh[d]);
+              t.hi[d] = std::min(G[1][d], S.hi[d] + sh[d]);
+              if (t.lo[d] > t.hi[d]) empty = true;
+              t.sh[d] = sh[d];
+            }
+            if (empty) continue;
+            t.dst = b;
+            t.src = s;
+            t.start = ncell;
+            ncell += cells(t.lo, t.hi);
+            copies.push_back(t);
+          }
+    }
+  }
+  ds->ncopy = int(copies.size());
+  ds->ncopy_cells = ncell;
+  if (!copies.empty()) {
+    HIPCHK(hipMalloc(&ds->d_copy, sizeof(CopyTask) * copies.size()));
+    HIPCHK(hipMemcpy(ds->d_copy, copies.data(), sizeof(CopyTask) * copies.size(), hipMemcpyHostToDevice));
+  }
+  for (int d = 0; d < 3; ++d) {
+    if (g.periodic[d]) continue;
+    std::vector<BcTask> tasks;
+    int64_t nc = 0;
+    for (int b = 0; b < nb; ++b) {
+      const amr_fab_t& fb = ds->host[b];
+      for (int side = 0; side < 2; ++side) {
+        BcTask t;
+        for (int e = 0; e < 3; ++e) {
+          t.lo[e] = fb.lo[e] - fb.ng;
+          t.hi[e] = fb.hi[e] + fb.ng;
+        }
+        if (side == 0) {
+          if (t.lo[d] >= g.dom_lo[d]) continue;
+          t.hi[d] = g.dom_lo[d] - 1;
+          t.bnd = g.dom_lo[d];
+        } else {
+          if (t.hi[d] <= g.dom_hi[d]) continue;
+          t.lo[d] = g.dom_hi[d] + 1;
+          t.bnd = g.dom_hi[d];
+        }
+        int code = side == 0 ? g.lo_bc[d] : g.hi_bc[d];
+        bool any = false;
+        for (int n = 0; n < 5; ++n) {
+          int ty = bc_type(code, n, d);
+          if (ty < 0) return fail(AMR_ERR_ARG, "invalid physical boundary code");
+          if (ty == 1)
+            return fail(AMR_ERR_ARG, "ext_dir boundaries (inflow / user) need a problem bcnormal; not supported");
+          t.type[n] = ty;
+          any = any || ty != 0;
+        }
+        if (!any) continue;
+        t.box = b;
+        t.dir = d;
+        t.side = side;
+        t.start = nc;
+        nc += cells(t.lo, t.hi);
+        tasks.push_back(t);
+      }
+    }
+    ds->nbc[d] = int(tasks.size());
+    ds->nbc_cells[d] = nc;
+    if (!tasks.empty()) {
+      HIPCHK(hipMalloc(&ds->d_bc[d], sizeof(BcTask) * tasks.size()));
+      HIPCHK(hipMemcpy(ds->d_bc[d], tasks.data(), sizeof(BcTask) * tasks.size(), hipMemcpyHostToDevice));
+    }
+  }
+  ds->fill_built = true;
+  return AMR_OK;
+}
+
+int enqueue_fill(amr_boxtab_s* bt, DescSet* ds) {
+  hipStream_t st = bt->ctx->stream;
+  HIPCHK(launch_fill_copy(ds->d_copy, ds->ncopy, ds->ncopy_cells, ds->d, st));
+  for (int d = 0; d < 3; ++d) HIPCHK(launch_fill_bc(ds->d_bc[d], ds->nbc[d], ds->nbc_cells[d], ds->d, st));
+  return AMR_OK;
+}
+
+enum { UW_PRIMS = 1, UW_VISC = 2, UW_WENO = 4, UW_SLOPE = 8 };
+
+int ensure_unfused(amr_boxtab_s* bt, const amr_scheme_t& s) {
+  int need = UW_PRIMS;
+  if (s.rhs_visc) need |= UW_VISC;
+  if (s.rhs_euler && s.flux_euler == 2) need |= UW_WENO;
+  if (s.rhs_euler && s.flux_euler == 0) need |= UW_SLOPE;
+  need |= bt->uw_kinds;
+  if ((need & ~bt->uw_kinds) == 0) return AMR_OK;
+  // (re)allocate everything needed in one block
+  HIPCHK(hipStreamSynchronize(bt->ctx->stream));
+  if (bt->uw_mem) HIPCHK(hipFree(bt->uw_mem));
+  for (void* p : bt->uw_tables) HIPCHK(hipFree(p));
+  bt->uw_tables.clear();
+  bt->uw_mem = nullptr;
+  int nb = bt->bt.nbox;
+  struct Arr {
+    int ng, nc, nodal;
+  };
+  std::vector<Arr> kinds;  // order: prims, nf0..2, [pv0..2], [pe0..2, lam], [slope]
+  kinds.push_back({3, NPRIM, 0});
+  for (int d = 0; d < 3; ++d) kinds.push_back({0, NCONS, 1});
+  if (need & UW_VISC)
+    for (int d = 0; d < 3; ++d) kinds.push_back({1, NCONS, 0});
+  if (need & UW_WENO) {
+    for (int d = 0; d < 3; ++d) kinds.push_back({3, NCONS, 0});
+    kinds.push_back({3, 3, 0});
+  }
+  if (need & UW_SLOPE) kinds.push_back({1, NCONS, 0});
+  int64_t total = 0;
+  for (auto& k : kinds)
+    for (int b = 0; b < nb; ++b) {
+      BoxDev B = bt->bt.h_boxes[b];
+      if (k.nodal)
+        for (int d = 0; d < 3; ++d) B.hi[d] += 1;
+      total += compact_len(B.lo, B.hi, k.ng, k.nc);
+    }
+  HIPCHK(hipMalloc(&bt->uw_mem, sizeof(double) * std::max<int64_t>(total, 1)));
+  HIPCHK(hipMemset(bt->uw_mem, 0, sizeof(double) * std::max<int64_t>(total, 1)));
+  int64_t off = 0;
+  std::vector<Fab*> tables;
+  for (auto& k : kinds) {
+    std::vector<Fab> h(nb);
+    for (int b = 0; b < nb; ++b) {
+      BoxDev B = bt->bt.h_boxes[b];
+      if (k.nodal)
+        for (int d = 0; d < 3; ++d) B.hi[d] += 1;
+      h[b] = make_fab(compact_fab(B.lo, B.hi, k.ng, bt->uw_mem + off));
+      off += compact_len(B.lo, B.hi, k.ng, k.nc);
+    }
+    Fab* d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(Fab) * nb));
+    HIPCHK(hipMemcpy(d, h.data(), sizeof(Fab) * nb, hipMemcpyHostToDevice));
+    bt->uw_tables.push_back(d);
+    tables.push_back(d);
+  }
+  size_t t = 0;
+  bt->uw = UnfusedWork();
+  bt->uw.prims = tables[t++];
+  for (int d = 0; d < 3; ++d) bt->uw.nf[d] = tables[t++];
+  if (need & UW_VISC)
+    for (int d = 0; d < 3; ++d) bt->uw.pv[d] = tables[t++];
+  if (need & UW_WENO) {
+    for (int d = 0; d < 3; ++d) bt->uw.pe[d] = tables[t++];
+    bt->uw.lam = tables[t++];
+  }
+  if (need & UW_SLOPE) bt->uw.slope = tables[t++];
+  bt->uw_kinds = need;
+  return AMR_OK;
+}
+
+std::string scheme_key(const amr_scheme_t& s) {
+  char buf[96];
+  std::snprintf(buf, sizeof buf, "e%d v%d f%d k%d", s.rhs_euler, s.rhs_visc, s.flux_euler, s.order_keep);
+  return buf;
+}
+
+// Tile table for the fused kernel; blockIdx -> tile is permuted so that
+// consecutive tiles of a box land on one XCD (blocks are dealt round-robin
+// over the 8 XCDs, each with its own L2).
+int get_tiles(amr_boxtab_s* bt, const FusedConfig& cfg, TileTable** out) {
+  auto key = std::make_tuple(cfg.tx, cfg.ty, cfg.tz);
+  auto it = bt->tiles.find(key);
+  if (it != bt->tiles.end()) {
+    *out = it->second.get();
+    return AMR_OK;
+  }
+  std::vector<TileDev> logical;
+  for (int b = 0; b < bt->bt.nbox; ++b) {
+    const BoxDev& B = bt->bt.h_boxes[b];
+    for (int k0 = B.lo[2]; k0 <= B.hi[2]; k0 += cfg.tz)
+      for (int j0 = B.lo[1]; j0 <= B.hi[1]; j0 += cfg.ty)
+        for (int i0 = B.lo[0]; i0 <= B.hi[0]; i0 += cfg.tx) {
+          TileDev t;
+          t.box = b;
+          t.i0 = i0;
+          t.j0 = j0;
+          t.k0 = k0;
+          t.nz = std::min(cfg.tz, B.hi[2] - k0 + 1);
+          logical.push_back(t);
+        }
+  }
+  const int NXCD = 8;
+  int n = int(logical.size());
+  std::vector<TileDev> table(n);
+  int per = (n + NXCD - 1) / NXCD;
+  // block t runs on XCD t % 8; give XCD x the contiguous logical range
+  // [x*per, (x+1)*per)
+  std::vector<int> used(n, 0);
+  std::vector<int> order;
+  order.reserve(n);
+  for (int t = 0; t < per * NXCD; ++t) {
+    int x = t % NXCD, r = t / NXCD;
+    int l = x * per + r;
+    if (l < n) order.push_back(l);
+  }
+  for (int t = 0; t < n; ++t) table[t] = logical[order[t]];
+  auto tt = std::make_unique<TileTable>();
+  tt->n = n;
+  HIPCHK(hipMalloc(&tt->d, sizeof(TileDev) * std::max(n, 1)));
+  HIPCHK(hipMemcpy(tt->d, table.data(), sizeof(TileDev) * n, hipMemcpyHostToDevice));
+  *out = tt.get();
+  bt->tiles[key] = std::move(tt);
+  return AMR_OK;
+}
+
+FusedConfig default_config(const amr_boxtab_s* bt) {
+  // pick the largest tile that does not exceed the smallest box extent
+  int mnx = 1 << 30, mny = 1 << 30;
+  for (auto& B : bt->bt.h_boxes) {
+    mnx = std::min(mnx, B.hi[0] - B.lo[0] + 1);
+    mny = std::min(mny, B.hi[1] - B.lo[1] + 1);
+  }
+  FusedConfig c{16, 16, 32};
+  if (mnx < 16 || mny < 16) c = FusedConfig{8, 8, 32};
+  return c;
+}
+
+int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageCoef& sc,
+             DescSet* ds1, DescSet* dsin, DescSet* dout) {
+  Closures c = make_closures(cls);
+  double dxinv[3];
+  for (int d = 0; d < 3; ++d) dxinv[d] = 1.0 / bt->geom.dx[d];
+  hipStream_t st = bt->ctx->stream;
+  const Fab* s1 = ds1 ? ds1->d : nullptr;
+  FusedConfig cfg = default_config(bt);
+  auto tk = bt->ctx->tuned.find(scheme_key(sch));
+  if (tk != bt->ctx->tuned.end()) cfg = tk->second;
+  if (fused_supported(sch, cfg.tx, cfg.ty)) {
+    TileTable* tt = nullptr;
+    int rc = get_tiles(bt, cfg, &tt);
+    if (rc) return rc;
+    HIPCHK(launch_fused_stage(sch, cfg, tt->d, tt->n, bt->bt.d_boxes, c, dxinv, sc, s1, dsin->d, dout->d, st));
+    return AMR_OK;
+  }
+  int rc = ensure_unfused(bt, sch);
+  if (rc) return rc;
+  HIPCHK(launch_unfused_stage(bt->bt, bt->uw, c, sch, dxinv, sc, s1, dsin->d, dout->d, st));
+  return AMR_OK;
+}
+
+// Workspace state arrays mirroring the layout of `state` (a level-wide
+// allocation stays level-wide, so its ghost fill stays cheap).
+int ensure_ws(amr_boxtab_s* bt, const amr_fab_t* state) {
+  int nb = bt->bt.nbox;
+  if (int(bt->ws_key.size()) == nb && std::memcmp(bt->ws_key.data(), state, sizeof(amr_fab_t) * nb) == 0)
+    return AMR_OK;
+  HIPCHK(hipStreamSynchronize(bt->ctx->stream));
+  for (int w = 0; w < 2; ++w) {
+    if (bt->ws_mem[w]) HIPCHK(hipFree(bt->ws_mem[w]));
+    bt->ws_mem[w] = nullptr;
+    bt->ws_fabs[w].assign(nb, amr_fab_t());
+  }
+  // span of the caller's storage
+  intptr_t lo = INTPTR_MAX, hi = 0;
+  int64_t sum = 0;
+  for (int b = 0; b < nb; ++b) {
+    const amr_fab_t& f = state[b];
+    int64_t nx = f.hi[0] - f.lo[0] + 1 + 2 * f.ng, ny = f.hi[1] - f.lo[1] + 1 + 2 * f.ng;
+    int64_t nz = f.hi[2] - f.lo[2] + 1 + 2 * f.ng;
+    int64_t last = (nx - 1) + (ny - 1) * f.jstride + (nz - 1) * f.kstride + (NCONS - 1) * f.nstride;
+    intptr_t a = intptr_t(f.data), e = intptr_t(f.data + last + 1);
+    lo = std::min(lo, a);
+    hi = std::max(hi, e);
+    sum += compact_len(f.lo, f.hi, f.ng, NCONS);
+  }
+  int64_t span = int64_t(hi - lo) / 8;
+  bool mirror = span <= sum + sum / 2;
+  for (int w = 0; w < 2; ++w) {
+    if (mirror) {
+      HIPCHK(hipMalloc(&bt->ws_mem[w], sizeof(double) * span));
+      for (int b = 0; b < nb; ++b) {
+        bt->ws_fabs[w][b] = state[b];
+        bt->ws_fabs[w][b].data = bt->ws_mem[w] + (intptr_t(state[b].data) - lo) / 8;
+      }
+    } else {
+      HIPCHK(hipMalloc(&bt->ws_mem[w], sizeof(double) * sum));
+      int64_t off = 0;
+      for (int b = 0; b < nb; ++b) {
+        const amr_fab_t& f = state[b];
+        bt->ws_fabs[w][b] = compact_fab(f.lo, f.hi, f.ng, bt->ws_mem[w] + off);
+        off += compact_len(f.lo, f.hi, f.ng, NCONS);
+      }
+    }
+  }
+  bt->ws_key.assign(state, state + nb);
+  return AMR_OK;
+}
+
+int stage_coeffs(int order_rk, int stages_rk, double dt, int stage, int* nst, StageCoef* sc) {
+  // CNS::advance, src/tim/CNS_advance.cpp:51-181
+  StageCoef B0{1, 0.0, 0.0, 1.0, dt};  // Sin + d*L
+  int n = 0;
+  StageCoef r = B0;
+  if (order_rk == -2) {
+    n = 2;
+    if (stage == 1) r = StageCoef{1, 0.5, 0.0, 0.5, 0.5 * dt};
+  } else if (order_rk == 0) {
+    n = 1;
+    r = StageCoef{2, 0, 0, 0, 0};
+  } else if (order_rk == 1) {
+    n = 1;
+  } else if (order_rk == 2) {
+    int m = stages_rk;
+    if (m < 2) return fail(AMR_ERR_ARG, "SSPRK2 number of stages must be >= 2");
+    n = m;
+    if (stage < m - 1) r = StageCoef{1, 0.0, 0.0, 1.0, dt / double(m - 1)};
+    else r = StageCoef{0, 1.0 / m, 1.0 / m, double(m - 1), dt};
+  } else if (order_rk == 3 && stages_rk == 3) {
+    n = 3;
+    if (stage == 1) r = StageCoef{0, 3.0 / 4, 1.0 / 4, 1.0, dt};
+    if (stage == 2) r = StageCoef{0, 1.0 / 3, 2.0 / 3, 1.0, dt};
+  } else if (order_rk == 3 && stages_rk == 4) {
+    n = 4;
+    r = StageCoef{1, 0.0, 0.0, 1.0, dt / 2};
+    if (stage == 2) r = StageCoef{1, 2.0 / 3, 0.0, 1.0 / 3, dt / 6};
+  } else {
+    return fail(AMR_ERR_ARG, "unsupported order_rk/stages_rk combination");
+  }
+  if (stage < 0 || stage >= n) return fail(AMR_ERR_ARG, "stage index out of range");
+  *nst = n;
+  *sc = r;
+  return AMR_OK;
+}
+
+int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const amr_fab_t* state,
+                 double dt) {
+  int rc;
+  int nst = 0;
+  StageCoef sc;
+  if ((rc = stage_coeffs(sch.order_rk, sch.stages_rk, dt, 0, &nst, &sc))) return rc;
+  if ((rc = ensure_ws(bt, state))) return rc;
+  DescSet *dst = nullptr, *dw[2] = {nullptr, nullptr};
+  if ((rc = get_desc(bt, state, NGHOST, &dst))) return rc;
+  for (int w = 0; w < 2; ++w)
+    if ((rc = get_desc(bt, bt->ws_fabs[w].data(), NGHOST, &dw[w]))) return rc;
+  if ((rc = build_fill(bt, dst))) return rc;
+  for (int w = 0; w < 2; ++w)
+    if ((rc = build_fill(bt, dw[w]))) return rc;
+  for (int s = 0; s < nst; ++s) {
+    if ((rc = stage_coeffs(sch.order_rk, sch.stages_rk, dt, s, &nst, &sc))) return rc;
+    DescSet* sin = s == 0 ? dst : dw[(s - 1) % 2];
+    DescSet* out = (s == nst - 1 && s > 0) ? dst : dw[s % 2];
+    if ((rc = enqueue_fill(bt, sin))) return rc;
+    if ((rc = do_stage(bt, cls, sch, sc, dst, sin, out))) return rc;
+  }
+  if (nst == 1) {
+    // single-stage schemes wrote into workspace 0; copy its valid cells back
+    HIPCHK(launch_copy_valid(bt->bt, dw[0]->d, dst->d, bt->ctx->stream));
+  }
+  return AMR_OK;
+}
+
+}  // namespace
+
+// ======================================================================= ABI
+extern "C" {
+
+const char* amr_last_error(void) { return g_err.c_str(); }
+
+int amr_ctx_create(int device, void* stream, amr_ctx_t* out) {
+  if (!out) return fail(AMR_ERR_ARG, "null output pointer");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(AMR_ERR_ARG, "device index out of range");
+  HIPCHK(hipSetDevice(device));
+  auto* c = new amr_ctx_s();
+  c->device = device;
+  if (stream) {
+    c->stream = reinterpret_cast<hipStream_t>(stream);
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      return fail(AMR_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    c->own_stream = true;
+  }
+  *out = c;
+  return AMR_OK;
+}
+
+int amr_ctx_destroy(amr_ctx_t ctx) {
+  if (!ctx) return AMR_OK;
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return AMR_OK;
+}
+
+int amr_ctx_synchronize(amr_ctx_t ctx) {
+  if (!ctx) return fail(AMR_ERR_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return AMR_OK;
+}
+
+int amr_boxtab_create(amr_ctx_t ctx, const amr_geom_t* geom, int nbox, const int32_t* boxes, amr_boxtab_t* out) {
+  if (!ctx || !geom || !boxes || !out || nbox <= 0) return fail(AMR_ERR_ARG, "invalid box table arguments");
+  for (int d = 0; d < 3; ++d) {
+    if (geom->dom_hi[d] < geom->dom_lo[d]) return fail(AMR_ERR_ARG, "empty domain");
+    if (!(geom->dx[d] > 0.0)) return fail(AMR_ERR_ARG, "dx must be positive");
+    if (geom->periodic[d] && geom->dom_hi[d] - geom->dom_lo[d] + 1 < NGHOST)
+      return fail(AMR_ERR_ARG, "periodic direction shorter than the ghost width");
+  }
+  for (int b = 0; b < nbox; ++b)
+    for (int d = 0; d < 3; ++d) {
+      int lo = boxes[6 * b + d], hi = boxes[6 * b + 3 + d];
+      if (hi < lo) return fail(AMR_ERR_ARG, "box " + std::to_string(b) + " is empty");
+      if (lo < geom->dom_lo[d] || hi > geom->dom_hi[d])
+        return fail(AMR_ERR_ARG, "box " + std::to_string(b) + " leaves the domain");
+    }
+  HIPCHK(hipSetDevice(ctx->device));
+  auto* bt = new amr_boxtab_s();
+  bt->ctx = ctx;
+  bt->geom = *geom;
+  bt->bt.nbox = nbox;
+  bt->bt.h_boxes.resize(nbox);
+  for (int b = 0; b < nbox; ++b)
+    for (int d = 0; d < 3; ++d) {
+      bt->bt.h_boxes[b].lo[d] = boxes[6 * b + d];
+      bt->bt.h_boxes[b].hi[d] = boxes[6 * b + 3 + d];
+    }
+  int rc = AMR_OK;
+  hipError_t e = hipMalloc(&bt->bt.d_boxes, sizeof(BoxDev) * nbox);
+  if (e == hipSuccess)
+    e = hipMemcpy(bt->bt.d_boxes, bt->bt.h_boxes.data(), sizeof(BoxDev) * nbox, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&bt->d_partial, sizeof(double) * 5 * 1024);
+  if (e == hipSuccess) e = hipMalloc(&bt->d_result, sizeof(double) * 8);
+  if (e == hipSuccess) e = hipHostMalloc(&bt->h_result, sizeof(double) * 8);
+  if (e != hipSuccess) rc = fail(AMR_ERR_HIP, std::string("box table allocation: ") + hipGetErrorString(e));
+  if (rc) {
+    amr_boxtab_destroy(bt);
+    return rc;
+  }
+  *out = bt;
+  return AMR_OK;
+}
+
+int amr_boxtab_destroy(amr_boxtab_t bt) {
+  if (!bt) return AMR_OK;
+  (void)hipStreamSynchronize(bt->ctx->stream);
+  bt->graphs.clear();
+  bt->tiles.clear();
+  bt->descs.clear();
+  if (bt->uw_mem) (void)hipFree(bt->uw_mem);
+  for (void* p : bt->uw_tables) (void)hipFree(p);
+  for (int w = 0; w < 2; ++w)
+    if (bt->ws_mem[w]) (void)hipFree(bt->ws_mem[w]);
+  if (bt->bt.d_boxes) (void)hipFree(bt->bt.d_boxes);
+  if (bt->d_partial) (void)hipFree(bt->d_partial);
+  if (bt->d_result) (void)hipFree(bt->d_result);
+  if (bt->h_result) (void)hipHostFree(bt->h_result);
+  delete bt;
+  return AMR_OK;
+}
+
+int amr_fill_boundary(amr_boxtab_t bt, const amr_fab_t* fabs) {
+  if (!bt || !fabs) return fail(AMR_ERR_ARG, "null argument");
+  DescSet* ds = nullptr;
+  int rc = get_desc(bt, fabs, 0, &ds);
+  if (rc) return rc;
+  if ((rc = build_fill(bt, ds))) return rc;
+  return enqueue_fill(bt, ds);
+}
+
+int amr_stage_coeffs(int order_rk, int stages_rk, double dt, int stage, int* nstages, int* mode, double* a,
+                     double* b, double* c, double* d) {
+  StageCoef sc;
+  int n = 0;
+  int rc = stage_coeffs(order_rk, stages_rk, dt, stage, &n, &sc);
+  if (rc) return rc;
+  if (nstages) *nstages = n;
+  if (mode) *mode = sc.mode;
+  if (a) *a = sc.a;
+  if (b) *b = sc.b;
+  if (c) *c = sc.c;
+  if (d) *d = sc.d;
+  return AMR_OK;
+}
+
+int amr_rk_stage(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, int mode, double a, double b,
+                 double c, double d, const amr_fab_t* s1, const amr_fab_t* sin, const amr_fab_t* out) {
+  if (!bt || !cls || !sch || !sin || !out) return fail(AMR_ERR_ARG, "null argument");
+  int rc;
+  if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls))) return rc;
+  if (mode < 0 || mode > 2) return fail(AMR_ERR_ARG, "mode must be 0, 1 or 2");
+  StageCoef sc{mode, a, b, c, d};
+  bool need_s1 = mode != 2 && a != 0.0;
+  if (need_s1 && !s1) return fail(AMR_ERR_ARG, "s1 is required when a != 0");
+  DescSet *ds1 = nullptr, *dsin = nullptr, *dout = nullptr;
+  if (need_s1 && (rc = get_desc(bt, s1, 0, &ds1))) return rc;
+  if ((rc = get_desc(bt, sin, NGHOST, &dsin))) return rc;
+  if ((rc = get_desc(bt, out, 0, &dout))) return rc;
+  // out must not share memory with sin
+  for (int i = 0; i < bt->bt.nbox; ++i)
+    if (mapping_of(out[i]) == mapping_of(sin[i]))
+      return fail(AMR_ERR_ARG, "out must not alias sin");
+  return do_stage(bt, *cls, *sch, sc, ds1, dsin, dout);
+}
+
+int amr_rhs(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
+            const amr_fab_t* dsdt) {
+  return amr_rk_stage(bt, cls, sch, 2, 0.0, 0.0, 0.0, 0.0, nullptr, state, dsdt);
+}
+
+int amr_advance(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
+                double time, double dt) {
+  (void)time;
+  if (!bt || !cls || !sch || !state) return fail(AMR_ERR_ARG, "null argument");
+  int rc;
+  if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls))) return rc;
+  return advance_impl(bt, *cls, *sch, state, dt);
+}
+
+int amr_advance_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
+                      double dt, int nsteps) {
+  if (!bt || !cls || !sch || !state || nsteps <= 0) return fail(AMR_ERR_ARG, "invalid argument");
+  int rc;
+  if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls))) return rc;
+  int nb = bt->bt.nbox;
+  std::vector<char> key(sizeof(amr_fab_t) * nb + sizeof(amr_closures_t) + sizeof(amr_scheme_t) + sizeof(double) +
+                        sizeof(int));
+  char* p = key.data();
+  std::memcpy(p, state, sizeof(amr_fab_t) * nb);
+  p += sizeof(amr_fab_t) * nb;
+  std::memcpy(p, cls, sizeof(amr_closures_t));
+  p += sizeof(amr_closures_t);
+  std::memcpy(p, sch, sizeof(amr_scheme_t));
+  p += sizeof(amr_scheme_t);
+  std::memcpy(p, &dt, sizeof(double));
+  p += sizeof(double);
+  std::memcpy(p, &nsteps, sizeof(int));
+  GraphEntry* ge = nullptr;
+  for (auto& g : bt->graphs)
+    if (g->key == key) ge = g.get();
+  hipStream_t st = bt->ctx->stream;
+  if (!ge) {
+    // Prepare every table and workspace outside the capture: one plain step
+    // on a scratch copy would change the state, so build tables directly.
+    if ((rc = ensure_ws(bt, state))) return rc;
+    DescSet* ds = nullptr;
+    if ((rc = get_desc(bt, state, NGHOST, &ds)) || (rc = build_fill(bt, ds))) return rc;
+    for (int w = 0; w < 2; ++w) {
+      DescSet* dw = nullptr;
+      if ((rc = get_desc(bt, bt->ws_fabs[w].data(), NGHOST, &dw)) || (rc = build_fill(bt, dw))) return rc;
+    }
+    FusedConfig cfg = default_config(bt);
+    auto tk = bt->ctx->tuned.find(scheme_key(*sch));
+    if (tk != bt->ctx->tuned.end()) cfg = tk->second;
+    if (fused_supported(*sch, cfg.tx, cfg.ty)) {
+      TileTable* tt = nullptr;
+      if ((rc = get_tiles(bt, cfg, &tt))) return rc;
+    } else if ((rc = ensure_unfused(bt, *sch))) {
+      return rc;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    auto g = std::make_unique<GraphEntry>();
+    g->key = key;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < nsteps; ++s) {
+      rc = advance_impl(bt, *cls, *sch, state, dt);
+      if (rc) {
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(st, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        return rc;
+      }
+    }
+    HIPCHK(hipStreamEndCapture(st, &g->graph));
+    HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    ge = g.get();
+    if (bt->graphs.size() >= 8) bt->graphs.erase(bt->graphs.begin());
+    bt->graphs.push_back(std::move(g));
+  }
+  HIPCHK(hipGraphLaunch(ge->exec, st));
+  return AMR_OK;
+}
+
+int amr_est_dt(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, double cfl, double* dt_out) {
+  if (!bt || !cls || !state || !dt_out) return fail(AMR_ERR_ARG, "null argument");
+  int rc = check_closures(*cls);
+  if (rc) return rc;
+  DescSet* ds = nullptr;
+  if ((rc = get_desc(bt, state, 0, &ds))) return rc;
+  Closures c = make_closures(*cls);
+  hipStream_t st = bt->ctx->stream;
+  HIPCHK(launch_estdt(bt->bt, ds->d, c, bt->geom.dx, bt->d_partial, bt->d_result, st));
+  HIPCHK(hipMemcpyAsync(bt->h_result, bt->d_result, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double dt = bt->h_result[0];
+  if (!(dt > 0.0) || std::isnan(dt))
+    return fail(AMR_ERR_STATE, "non-physical state: CFL time step is not positive (negative pressure or NaN)");
+  *dt_out = dt * cfl;  // src/CNS.cpp:391
+  return AMR_OK;
+}
+
+int amr_totals(amr_boxtab_t bt, const amr_fab_t* state, double out[5]) {
+  if (!bt || !state || !out) return fail(AMR_ERR_ARG, "null argument");
+  DescSet* ds = nullptr;
+  int rc = get_desc(bt, state, 0, &ds);
+  if (rc) return rc;
+  hipStream_t st = bt->ctx->stream;
+  HIPCHK(launch_totals(bt->bt, ds->d, bt->d_partial, bt->d_result, st));
+  HIPCHK(hipMemcpyAsync(bt->h_result, bt->d_result, sizeof(double) * 5, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double probsize = 1.0;
+  for (int d = 0; d < 3; ++d) probsize *= (bt->geom.dom_hi[d] - bt->geom.dom_lo[d] + 1) * bt->geom.dx[d];
+  for (int n = 0; n < 5; ++n) out[n] = bt->h_result[n] * probsize;  // src/CNS.cpp:541
+  return AMR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,286 @@
+// Device point functions of the CNS right-hand side (FP64).
+//
+// Each function restates one reference point kernel in the reference's
+// operation order; the build compiles device code with -ffp-contract=off so
+// results round exactly as the reference's CPU build does.  max/min follow
+// std::max/std::min semantics ((a<b)?b:a).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "amrsolver_hip.h"
+
+namespace amr {
+
+constexpr int URHO = 0, UMX = 1, UMY = 2, UMZ = 3, UET = 4, NCONS = 5;
+constexpr int QRHO = 0, QU = 1, QV = 2, QW = 3, QT = 4, QPRES = 5, NPRIM = 6;
+constexpr int NGHOST = 3;
+
+__host__ __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+__host__ __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+
+// Array view with the AMReX Array4 index convention.  `off` folds the lower
+// corner of the allocation into one offset so that index = off + i + j*js +
+// k*ks + n*ns.
+struct Fab {
+  double* p;
+  int64_t off, js, ks, ns;
+  __host__ __device__ __forceinline__ double& operator()(int i, int j, int k, int n) const {
+    return p[off + i + j * js + k * ks + n * ns];
+  }
+  __host__ __device__ __forceinline__ int64_t idx(int i, int j, int k) const {
+    return off + i + j * js + k * ks;
+  }
+};
+
+inline Fab make_fab(const amr_fab_t& f) {
+  Fab r;
+  r.p = f.data;
+  r.js = f.jstride;
+  r.ks = f.kstride;
+  r.ns = f.nstride;
+  r.off = -(int64_t(f.lo[0] - f.ng) + int64_t(f.lo[1] - f.ng) * f.jstride +
+            int64_t(f.lo[2] - f.ng) * f.kstride);
+  return r;
+}
+
+// Closures with the law switches resolved into plain fields.
+struct Closures {
+  double gamma, gm1, cv, cp, Rspec;
+  int visc_law, cond_law;
+  double visc_ref, Tvisc_ref, cond_ref, Tcond_ref, Pr;
+};
+
+inline Closures make_closures(const amr_closures_t& c) {
+  Closures r;
+  r.gamma = c.gamma;
+  r.gm1 = c.gamma - 1.0;
+  r.cv = c.cv;
+  r.cp = c.cp;
+  r.Rspec = c.Rspec;
+  r.visc_law = c.visc_law;
+  r.cond_law = c.cond_law;
+  r.visc_ref = c.visc_ref;
+  r.Tvisc_ref = c.Tvisc_ref;
+  r.cond_ref = c.cond_ref;
+  r.Tcond_ref = c.Tcond_ref;
+  r.Pr = c.Pr;
+  return r;
+}
+
+// src/cls/Closures.h:108-109, :118-119
+__device__ __forceinline__ double visc(const Closures& c, double T) {
+  if (c.visc_law == 0) return c.visc_ref;
+  return c.visc_ref * T * sqrt(T) / (c.Tvisc_ref + T);
+}
+// src/cls/Closures.h:135-136, :161; law 2 = mu*cp/Pr (mu passed in)
+__device__ __forceinline__ double cond(const Closures& c, double T, double mu) {
+  if (c.cond_law == 0) return c.cond_ref;
+  if (c.cond_law == 1) return c.cond_ref * T * sqrt(T) / (c.Tcond_ref + T);
+  return mu * c.cp / c.Pr;
+}
+
+// src/rhs/CNS_hydro_K.h:15-34.  q[] = rho, u, v, w, T, p
+__device__ __forceinline__ void cons2prim(const double u0, const double u1, const double u2,
+                                          const double u3, const double u4, const Closures& c,
+                                          double q[6]) {
+  double rho = dmax(1e-40, u0);
+  double rhoinv = 1.0 / rho;
+  double ux = u1 * rhoinv;
+  double uy = u2 * rhoinv;
+  double uz = u3 * rhoinv;
+  double rhoke = 0.5 * rho * (ux * ux + uy * uy + uz * uz);
+  double rhoei = (u4 - rhoke);
+  double p = c.gm1 * rhoei;
+  q[QRHO] = rho;
+  q[QU] = ux;
+  q[QV] = uy;
+  q[QW] = uz;
+  q[QPRES] = p;
+  q[QT] = p / (rho * c.Rspec);
+}
+
+// src/rhs/CentralKEEP.h:19-36
+__device__ __forceinline__ double fDiv(double f, double fl) { return 0.5 * (f + fl); }
+__device__ __forceinline__ double fgDiv(double f, double fl, double g, double gl) {
+  return 0.5 * (f * gl + fl * g);
+}
+__device__ __forceinline__ double fgQuad(double f, double fl, double g, double gl) {
+  return 0.25 * (f + fl) * (g + gl);
+}
+__device__ __forceinline__ double fghCubic(double f, double fl, double g, double gl, double h,
+                                           double hl) {
+  return 0.1250 * (f + fl) * (g + gl) * (h + hl);
+}
+
+// One (l,m) pair of the KEEP flux in direction `nrm` (0,1,2), accumulated
+// into f[5] with coefficient co (CentralKEEP.h:58-84 and the y/z copies).
+// Cell 1 = (rho1,u1,v1,w1,ie1,p1), cell 2 likewise.
+template <int NRM>
+__device__ __forceinline__ void keep_pair(double co, double rho1, double ux1, double uy1,
+                                          double uz1, double ie1, double p1, double rho2,
+                                          double ux2, double uy2, double uz2, double ie2,
+                                          double p2, double f[5]) {
+  const double un1 = NRM == 0 ? ux1 : (NRM == 1 ? uy1 : uz1);
+  const double un2 = NRM == 0 ? ux2 : (NRM == 1 ? uy2 : uz2);
+  double massflx = fgQuad(rho1, rho2, un1, un2);
+  double ke = 0.5 * (ux1 * ux2 + uy1 * uy2 + uz1 * uz2);
+  f[URHO] += co * massflx;
+  if (NRM == 0)
+    f[UMX] += co * (fghCubic(rho1, rho2, un1, un2, un1, un2) + fDiv(p1, p2));
+  else
+    f[UMX] += co * fghCubic(rho1, rho2, un1, un2, ux1, ux2);
+  if (NRM == 1)
+    f[UMY] += co * (fghCubic(rho1, rho2, un1, un2, un1, un2) + fDiv(p1, p2));
+  else
+    f[UMY] += co * fghCubic(rho1, rho2, un1, un2, uy1, uy2);
+  if (NRM == 2)
+    f[UMZ] += co * (fghCubic(rho1, rho2, un1, un2, un1, un2) + fDiv(p1, p2));
+  else
+    f[UMZ] += co * fghCubic(rho1, rho2, un1, un2, uz1, uz2);
+  f[UET] += co * (massflx * ke + fghCubic(rho1, rho2, un1, un2, ie1, ie2) + fgDiv(p1, p2, un1, un2));
+}
+
+// src/rhs/High_resolution.h:124-159 (beta1 with the 4th power of (s1-s3),
+// eps = 1e-40; pow(x,2) written as x*x)
+__device__ __forceinline__ double weno5js(double s0, double s1, double s2, double s3, double s4) {
+  const double eps = 1e-40;
+  double b0 = (13.0 / 12.0) * (s0 - 2.0 * s1 + s2) * (s0 - 2.0 * s1 + s2) +
+              0.25 * (s0 - 4.0 * s1 + 3.0 * s2) * (s0 - 4.0 * s1 + 3.0 * s2);
+  double b1 = (13.0 / 12.0) * (s1 - 2.0 * s2 + s3) * (s1 - 2.0 * s2 + s3) +
+              0.25 * (s1 - s3) * (s1 - s3) * (s1 - s3) * (s1 - s3);
+  double b2 = (13.0 / 12.0) * (s2 - 2.0 * s3 + s4) * (s2 - 2.0 * s3 + s4) +
+              0.25 * (3.0 * s2 - 4.0 * s3 + s4) * (3.0 * s2 - 4.0 * s3 + s4);
+  double t0 = eps + b0, t1 = eps + b1, t2 = eps + b2;
+  double a0 = 0.1 / (t0 * t0);
+  double a1 = 0.6 / (t1 * t1);
+  double a2 = 0.3 / (t2 * t2);
+  double sum = 1.0 / (a0 + a1 + a2);
+  double v0 = 2.0 * s0 - 7.0 * s1 + 11.0 * s2;
+  double v1 = -s1 + 5.0 * s2 + 2.0 * s3;
+  double v2 = 2.0 * s2 + 5.0 * s3 - s4;
+  return (1.0 / 6.0) * sum * (a0 * v0 + a1 * v1 + a2 * v2);
+}
+
+// src/rhs/Riemann.h:9-17
+__device__ __forceinline__ double limiter(double dlft, double drgt) {
+  double dcen = 0.5 * (dlft + drgt);
+  double dsgn = copysign(1.0, dcen);
+  double slop = 2.0 * dmin(fabs(dlft), fabs(drgt));
+  double dlim = (dlft * drgt >= 0.0) ? slop : 0.0;
+  return dsgn * dmin(dlim, fabs(dcen));
+}
+
+// src/rhs/Riemann.h:123-249; out = (rho, normal mom, t1 mom, t2 mom, energy) flux
+__device__ __forceinline__ void riemann_prob(double gamma, double smallp, double rl, double ul,
+                                             double pl, double ut1l, double ut2l, double rr,
+                                             double ur, double pr, double ut1r, double ut2r,
+                                             double out[5]) {
+  const double weakwv = 1.e-3;
+  const double small = 1.e-6;
+  double clsql = gamma * pl * rl;
+  double clsqr = gamma * pr * rr;
+  double wl = sqrt(clsql);
+  double wr = sqrt(clsqr);
+  double cleft = wl / rl;
+  double cright = wr / rr;
+  double ccsmall = small * (cleft + cright);
+  double pstar = (wl * pr + wr * pl - wr * wl * (ur - ul)) / (wl + wr);
+  pstar = dmax(pstar, smallp);
+  double pstnm1 = pstar;
+  double wlsq = (0.5 * (gamma - 1.) * (pstar + pl) + pstar) * rl;
+  double wrsq = (0.5 * (gamma - 1.) * (pstar + pr) + pstar) * rr;
+  wl = sqrt(wlsq);
+  wr = sqrt(wrsq);
+  double ustarp = ul - (pstar - pl) / wl;
+  double ustarm = ur + (pstar - pr) / wr;
+  pstar = (wl * pr + wr * pl - wr * wl * (ur - ul)) / (wl + wr);
+  pstar = dmax(pstar, smallp);
+  double ustar = 0.0;
+#pragma unroll
+  for (int iter = 0; iter < 3; ++iter) {
+    wlsq = (0.5 * (gamma - 1.) * (pstar + pl) + pstar) * rl;
+    wrsq = (0.5 * (gamma - 1.) * (pstar + pr) + pstar) * rr;
+    wl = 1. / sqrt(wlsq);
+    wr = 1. / sqrt(wrsq);
+    double ustnm1 = ustarm;
+    double ustnp1 = ustarp;
+    ustarm = ur - (pr - pstar) * wr;
+    ustarp = ul + (pl - pstar) * wl;
+    double dpditer = fabs(pstnm1 - pstar);
+    double zp = fabs(ustarp - ustnp1);
+    if (zp - weakwv * cleft < 0.0) zp = dpditer * wl;
+    double zm = fabs(ustarm - ustnm1);
+    if (zm - weakwv * cright < 0.0) zm = dpditer * wr;
+    double zz = zp + zm;
+    double denom = dpditer / dmax(zz, ccsmall);
+    pstnm1 = pstar;
+    pstar = pstar - denom * (ustarm - ustarp);
+    pstar = dmax(pstar, smallp);
+    ustar = 0.5 * (ustarm + ustarp);
+  }
+  double ro, uo, po, sgnm, utrans1, utrans2;
+  if (ustar > 0.) {
+    ro = rl; uo = ul; po = pl; sgnm = 1.; utrans1 = ut1l; utrans2 = ut2l;
+  } else if (ustar < 0.) {
+    ro = rr; uo = ur; po = pr; sgnm = -1.; utrans1 = ut1r; utrans2 = ut2r;
+  } else {
+    uo = 0.5 * (ur + ul);
+    po = 0.5 * (pr + pl);
+    ro = 2. * (rl * rr) / (rl + rr);
+    sgnm = 1.;
+    utrans1 = 0.5 * (ut1l + ut1r);
+    utrans2 = 0.5 * (ut2l + ut2r);
+  }
+  double wosq = (0.5 * (gamma - 1.) * (pstar + po) + pstar) * ro;
+  double co = sqrt(gamma * po / ro);
+  double wo = sqrt(wosq);
+  double dpjmp = pstar - po;
+  double rstar = ro / (1. - ro * dpjmp / wosq);
+  double cstar = sqrt(gamma * pstar / rstar);
+  double spout = co - sgnm * uo;
+  double spin = cstar - sgnm * uo;
+  if (pstar >= po) {
+    spin = wo / ro - sgnm * uo;
+    spout = spin;
+  }
+  double ss = dmax(spout - spin, spout + spin);
+  double frac = 0.5 * (1. + (spin + spout) / dmax(ss, ccsmall));
+  double rgdnv, ugdnv, pgdnv;
+  if (spout < 0.) {
+    rgdnv = ro; ugdnv = uo; pgdnv = po;
+  } else if (spin >= 0.) {
+    rgdnv = rstar; ugdnv = ustar; pgdnv = pstar;
+  } else {
+    rgdnv = frac * rstar + (1. - frac) * ro;
+    ugdnv = frac * ustar + (1. - frac) * uo;
+    pgdnv = frac * pstar + (1. - frac) * po;
+  }
+  out[0] = rgdnv * ugdnv;
+  out[1] = rgdnv * ugdnv * ugdnv + pgdnv;
+  out[2] = rgdnv * ugdnv * utrans1;
+  out[3] = rgdnv * ugdnv * utrans2;
+  out[4] = ugdnv * (0.5 * rgdnv * (ugdnv * ugdnv + utrans1 * utrans1 + utrans2 * utrans2) +
+                    pgdnv / (gamma - 1.) + pgdnv);
+}
+
+// Stage-combination epilogue (CNS::advance MultiFab ops, src/tim/CNS_advance.cpp:51-181).
+struct StageCoef {
+  int mode;  // 0: a*s1 + b*(c*sin + d*L); 1: (a*s1 + c*sin) + d*L; 2: L
+  double a, b, c, d;
+};
+
+// With a == 0 the s1 term is left out entirely (s1 is then not read), which
+// reproduces LinComb(dst, 1.0, x, dt, y) = x + dt*y bit for bit.
+__device__ __forceinline__ double stage_combine(const StageCoef& sc, double s1, double sin,
+                                                double L) {
+  if (sc.mode == 0) {
+    double t = sc.c * sin + sc.d * L;
+    return sc.a == 0.0 ? sc.b * t : sc.a * s1 + sc.b * t;
+  }
+  if (sc.mode == 1) return (sc.a == 0.0 ? sc.c * sin : sc.a * s1 + sc.c * sin) + sc.d * L;
+  return L;
+}
+
+}  // namespace amr

@@ -1,0 +1,246 @@
+// Ghost fill (single-level FillPatch) and valid-cell reductions (CFL dt,
+// conserved totals).
+#include <algorithm>
+
+#include "kernels.h"
+
+namespace amr {
+
+namespace {
+
+constexpr int FBLK = 256;
+
+template <class T>
+__device__ __forceinline__ int find_task(const T* tasks, int ntask, int64_t t) {
+  int lo = 0, hi = ntask - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].start <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// FillBoundary + periodic images: copy valid cells of `src` into ghost
+// cells of `dst` (AMReX FillBoundary semantics restated).
+__global__ void __launch_bounds__(FBLK) k_fill_copy(const CopyTask* tasks, int ntask, int64_t ncell,
+                                                    const Fab* fabs) {
+  int64_t t = int64_t(blockIdx.x) * FBLK + threadIdx.x;
+  if (t >= ncell) return;
+  const CopyTask& T = tasks[find_task(tasks, ntask, t)];
+  int64_t r = t - T.start;
+  int nx = T.hi[0] - T.lo[0] + 1, ny = T.hi[1] - T.lo[1] + 1;
+  int i = T.lo[0] + int(r % nx);
+  int64_t r2 = r / nx;
+  int j = T.lo[1] + int(r2 % ny);
+  int k = T.lo[2] + int(r2 / ny);
+  const Fab d = fabs[T.dst], s = fabs[T.src];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) d(i, j, k, n) = s(i - T.sh[0], j - T.sh[1], k - T.sh[2], n);
+}
+
+// Physical boundaries (src/set/CNS_setup.cpp:18-31 BC map; AMReX foextrap
+// and reflect_even/odd): ghost value from the boundary cell or its mirror.
+__global__ void __launch_bounds__(FBLK) k_fill_bc(const BcTask* tasks, int ntask, int64_t ncell,
+                                                  const Fab* fabs) {
+  int64_t t = int64_t(blockIdx.x) * FBLK + threadIdx.x;
+  if (t >= ncell) return;
+  const BcTask& T = tasks[find_task(tasks, ntask, t)];
+  int64_t r = t - T.start;
+  int nx = T.hi[0] - T.lo[0] + 1, ny = T.hi[1] - T.lo[1] + 1;
+  int c[3];
+  c[0] = T.lo[0] + int(r % nx);
+  int64_t r2 = r / nx;
+  c[1] = T.lo[1] + int(r2 % ny);
+  c[2] = T.lo[2] + int(r2 / ny);
+  const Fab f = fabs[T.box];
+  const int cd = c[T.dir];
+  for (int n = 0; n < NCONS; ++n) {
+    int ty = T.type[n];
+    if (ty == 0) continue;
+    int s[3] = {c[0], c[1], c[2]};
+    if (ty == 2) s[T.dir] = T.bnd;
+    else s[T.dir] = T.side == 0 ? 2 * T.bnd - 1 - cd : 2 * T.bnd + 1 - cd;
+    double v = f(s[0], s[1], s[2], n);
+    f(c[0], c[1], c[2], n) = (ty == 4) ? -v : v;
+  }
+}
+
+__global__ void __launch_bounds__(FBLK) k_copy_valid(const BoxDev* boxes, const Fab* src, const Fab* dst) {
+  const int b = blockIdx.y;
+  const BoxDev bx = boxes[b];
+  int nx = bx.hi[0] - bx.lo[0] + 1, ny = bx.hi[1] - bx.lo[1] + 1, nz = bx.hi[2] - bx.lo[2] + 1;
+  int64_t t = int64_t(blockIdx.x) * FBLK + threadIdx.x;
+  if (t >= int64_t(nx) * ny * nz) return;
+  int i = bx.lo[0] + int(t % nx);
+  int64_t r = t / nx;
+  int j = bx.lo[1] + int(r % ny);
+  int k = bx.lo[2] + int(r / ny);
+  const Fab s = src[b], d = dst[b];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) d(i, j, k, n) = s(i, j, k, n);
+}
+
+constexpr int RBLK = 256;
+
+// cns_estdt (src/cls/CNS_K.h:12-47; z momentum read from UMY, CNS_K.h:29):
+// per-block minimum of the cell time steps.
+__global__ void __launch_bounds__(RBLK) k_estdt(const BoxDev* boxes, int nbox, const Fab* st,
+                                                Closures c, double dx0, double dx1, double dx2,
+                                                double* partial) {
+  __shared__ double sm[RBLK];
+  double dt = 1.79769313486231570815e+308;
+  for (int b = 0; b < nbox; ++b) {
+    const BoxDev bx = boxes[b];
+    const Fab u = st[b];
+    int nx = bx.hi[0] - bx.lo[0] + 1, ny = bx.hi[1] - bx.lo[1] + 1, nz = bx.hi[2] - bx.lo[2] + 1;
+    int64_t n = int64_t(nx) * ny * nz;
+    for (int64_t t = int64_t(blockIdx.x) * RBLK + threadIdx.x; t < n; t += int64_t(gridDim.x) * RBLK) {
+      int i = bx.lo[0] + int(t % nx);
+      int64_t r = t / nx;
+      int j = bx.lo[1] + int(r % ny);
+      int k = bx.lo[2] + int(r / ny);
+      double rho = u(i, j, k, URHO);
+      double mx = u(i, j, k, UMX);
+      double my = u(i, j, k, UMY);
+      double mz = u(i, j, k, UMY);
+      double rhoinv = 1.0 / dmax(rho, 1.0e-40);
+      double vx = mx * rhoinv, vy = my * rhoinv, vz = mz * rhoinv;
+      double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+      double rhoei = (u(i, j, k, UET) - ke);
+      double p = c.gm1 * rhoei;
+      double cs = sqrt(c.gamma * p * rhoinv);
+      double dtx = dx0 / (fabs(vx) + cs);
+      double dty = dx1 / (fabs(vy) + cs);
+      double dtz = dx2 / (fabs(vz) + cs);
+      dt = dmin(dt, dmin(dtx, dmin(dty, dtz)));
+    }
+  }
+  sm[threadIdx.x] = dt;
+  __syncthreads();
+  for (int s = RBLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = dmin(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+}
+
+__global__ void __launch_bounds__(RBLK) k_min_final(const double* partial, int n, double* result) {
+  __shared__ double sm[RBLK];
+  double v = 1.79769313486231570815e+308;
+  for (int t = threadIdx.x; t < n; t += RBLK) v = dmin(v, partial[t]);
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = RBLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = dmin(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) result[0] = sm[0];
+}
+
+// Conserved totals: per-block sums of the five components (fixed reduction
+// tree, so the result is reproducible run to run).
+__global__ void __launch_bounds__(RBLK) k_totals(const BoxDev* boxes, int nbox, const Fab* st,
+                                                 double* partial) {
+  __shared__ double sm[5][RBLK];
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int b = 0; b < nbox; ++b) {
+    const BoxDev bx = boxes[b];
+    const Fab u = st[b];
+    int nx = bx.hi[0] - bx.lo[0] + 1, ny = bx.hi[1] - bx.lo[1] + 1, nz = bx.hi[2] - bx.lo[2] + 1;
+    int64_t n = int64_t(nx) * ny * nz;
+    for (int64_t t = int64_t(blockIdx.x) * RBLK + threadIdx.x; t < n; t += int64_t(gridDim.x) * RBLK) {
+      int i = bx.lo[0] + int(t % nx);
+      int64_t r = t / nx;
+      int j = bx.lo[1] + int(r % ny);
+      int k = bx.lo[2] + int(r / ny);
+#pragma unroll
+      for (int c = 0; c < 5; ++c) acc[c] += u(i, j, k, c);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 5; ++c) sm[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int s = RBLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) sm[c][threadIdx.x] += sm[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) partial[5 * blockIdx.x + c] = sm[c][0];
+}
+
+__global__ void __launch_bounds__(RBLK) k_sum_final(const double* partial, int n, double* result) {
+  __shared__ double sm[5][RBLK];
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int t = threadIdx.x; t < n; t += RBLK)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc[c] += partial[5 * t + c];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) sm[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int s = RBLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) sm[c][threadIdx.x] += sm[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) result[c] = sm[c][0];
+}
+
+}  // namespace
+
+hipError_t launch_fill_copy(const CopyTask* tasks, int ntask, int64_t ncell, const Fab* fabs,
+                            hipStream_t st) {
+  if (ntask == 0 || ncell == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_copy, dim3(unsigned((ncell + FBLK - 1) / FBLK)), dim3(FBLK), 0, st, tasks, ntask,
+                     ncell, fabs);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_bc(const BcTask* tasks, int ntask, int64_t ncell, const Fab* fabs, hipStream_t st) {
+  if (ntask == 0 || ncell == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_bc, dim3(unsigned((ncell + FBLK - 1) / FBLK)), dim3(FBLK), 0, st, tasks, ntask,
+                     ncell, fabs);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_valid(const BoxTable& bt, const Fab* src, const Fab* dst, hipStream_t st) {
+  int64_t mx = 1;
+  for (auto& b : bt.h_boxes)
+    mx = std::max<int64_t>(mx, int64_t(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1));
+  hipLaunchKernelGGL(k_copy_valid, dim3(unsigned((mx + FBLK - 1) / FBLK), unsigned(bt.nbox)), dim3(FBLK), 0, st,
+                     bt.d_boxes, src, dst);
+  return hipGetLastError();
+}
+
+int reduce_blocks(const BoxTable& bt) {
+  int64_t n = 0;
+  for (auto& b : bt.h_boxes)
+    n += int64_t(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
+  int64_t blocks = (n / bt.nbox + RBLK - 1) / RBLK;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 1024) blocks = 1024;
+  return int(blocks);
+}
+
+hipError_t launch_estdt(const BoxTable& bt, const Fab* st, const Closures& c, const double dx[3], double* partial,
+                        double* result, hipStream_t s) {
+  int nb = reduce_blocks(bt);
+  hipLaunchKernelGGL(k_estdt, dim3(nb), dim3(RBLK), 0, s, bt.d_boxes, bt.nbox, st, c, dx[0], dx[1], dx[2], partial);
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(RBLK), 0, s, partial, nb, result);
+  return hipGetLastError();
+}
+
+hipError_t launch_totals(const BoxTable& bt, const Fab* st, double* partial, double* result, hipStream_t s) {
+  int nb = reduce_blocks(bt);
+  hipLaunchKernelGGL(k_totals, dim3(nb), dim3(RBLK), 0, s, bt.d_boxes, bt.nbox, st, partial);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(RBLK), 0, s, partial, nb, result);
+  return hipGetLastError();
+}
+
+}  // namespace amr

@@ -1,0 +1,100 @@
+// Internal interface between the C-ABI layer (capi.cpp) and the kernel
+// translation units.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "amrsolver_hip.h"
+#include "cns_point.h"
+
+namespace amr {
+
+struct BoxDev {
+  int lo[3], hi[3];
+};
+
+// The level's box list on host and device.
+struct BoxTable {
+  int nbox = 0;
+  std::vector<BoxDev> h_boxes;
+  BoxDev* d_boxes = nullptr;
+};
+
+// Per-box workspace views of the unfused pass structure (device arrays of
+// nbox Fab each).
+struct UnfusedWork {
+  Fab* prims = nullptr;     // NPRIM comps on grow(bx,3)
+  Fab* nf[3] = {};          // NCONS comps on [lo, hi+1]^3
+  Fab* pv[3] = {};          // NCONS comps on grow(bx,1)   (viscous)
+  Fab* pe[3] = {};          // NCONS comps on grow(bx,3)   (WENO point fluxes)
+  Fab* lam = nullptr;       // 3 comps on grow(bx,3)       (WENO eigenvalues)
+  Fab* slope = nullptr;     // NCONS comps on grow(bx,1)   (MUSCL slopes)
+};
+
+// Ghost-fill tasks.  Copy: dst(i,j,k) = src(i - sh) for (i,j,k) in [lo,hi].
+struct CopyTask {
+  int dst, src;
+  int lo[3], hi[3];
+  int sh[3];
+  int64_t start;  // exclusive prefix of cell counts
+};
+// Physical BC: for cells [lo,hi] of box `box` in direction `dir`, side
+// `side` (0 lo, 1 hi) with domain boundary index `bnd`; per-component type
+// 0 nothing, 2 foextrap, 3 reflect_even, 4 reflect_odd.
+struct BcTask {
+  int box, dir, side, bnd;
+  int lo[3], hi[3];
+  int type[5];
+  int64_t start;
+};
+
+inline void keep_coefficients(int order, double co[3]) {
+  // src/CNS.cpp:243-251; src/rhs/CentralKEEP.h:196-202
+  if (order == 6) {
+    co[0] = 6.0 / 4; co[1] = -6.0 / 20; co[2] = 2.0 / 60;
+  } else if (order == 4) {
+    co[0] = 4.0 / 3; co[1] = -2.0 / 12; co[2] = 0.0;
+  } else {
+    co[0] = 1.0; co[1] = 0.0; co[2] = 0.0;
+  }
+}
+
+hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const Closures& c,
+                                const amr_scheme_t& s, const double dxinv[3], const StageCoef& sc,
+                                const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st);
+
+hipError_t launch_fill_copy(const CopyTask* tasks, int ntask, int64_t ncell, const Fab* fabs,
+                            hipStream_t st);
+hipError_t launch_fill_bc(const BcTask* tasks, int ntask, int64_t ncell, const Fab* fabs,
+                          hipStream_t st);
+
+// dst = src on the valid cells of every box.
+hipError_t launch_copy_valid(const BoxTable& bt, const Fab* src, const Fab* dst, hipStream_t st);
+
+// Reductions over valid cells.  `partial` holds at least reduce_partials()
+// doubles (x5 for totals); the result lands in result[0] (dt) or result[0..4].
+int reduce_blocks(const BoxTable& bt);
+hipError_t launch_estdt(const BoxTable& bt, const Fab* st, const Closures& c, const double dx[3],
+                        double* partial, double* result, hipStream_t s);
+hipError_t launch_totals(const BoxTable& bt, const Fab* st, double* partial, double* result,
+                         hipStream_t s);
+
+// Fused KEEP (order 2/4) + viscous stage kernel over a tile table.
+struct TileDev {
+  int box;
+  int i0, j0, k0;  // tile origin (absolute index)
+  int nz;          // planes in this tile
+};
+struct FusedConfig {
+  int tx, ty, tz;  // tile extents
+};
+bool fused_supported(const amr_scheme_t& s, int tx, int ty);
+hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const TileDev* tiles,
+                              int ntiles, const BoxDev* boxes, const Closures& c,
+                              const double dxinv[3], const StageCoef& sc, const Fab* s1,
+                              const Fab* sin, const Fab* out, hipStream_t st);
+
+}  // namespace amr

@@ -1,0 +1,394 @@
+// Box-table kernels with the reference's pass structure: one launch per
+// reference ParallelFor, covering every box of the level at once
+// (blockIdx.y = box).  These serve every scheme (KEEP 2/4/6, WENO5,
+// MUSCL+Riemann, viscous) and are the fallback where no fused stage kernel
+// exists; intermediates live in box-table workspace arrays.
+#include "cns_point.h"
+#include "kernels.h"
+
+namespace amr {
+
+namespace {
+
+constexpr int BLK = 256;
+
+// Region of box b: [lo + glo, hi + ghi] per direction.
+struct Region {
+  int glo[3], ghi[3];
+};
+
+__device__ __forceinline__ bool region_cell(const BoxDev& bx, const Region& r, int64_t t, int& i,
+                                            int& j, int& k) {
+  int lo0 = bx.lo[0] + r.glo[0], lo1 = bx.lo[1] + r.glo[1], lo2 = bx.lo[2] + r.glo[2];
+  int nx = bx.hi[0] + r.ghi[0] - lo0 + 1;
+  int ny = bx.hi[1] + r.ghi[1] - lo1 + 1;
+  int nz = bx.hi[2] + r.ghi[2] - lo2 + 1;
+  if (t >= int64_t(nx) * ny * nz) return false;
+  i = lo0 + int(t % nx);
+  int64_t r2 = t / nx;
+  j = lo1 + int(r2 % ny);
+  k = lo2 + int(r2 / ny);
+  return true;
+}
+
+#define REGION_LOOP_BEGIN                                                   \
+  const int b = blockIdx.y;                                                 \
+  const BoxDev bx = boxes[b];                                               \
+  int i, j, k;                                                              \
+  if (!region_cell(bx, reg, int64_t(blockIdx.x) * BLK + threadIdx.x, i, j, k)) return;
+
+// ---------------------------------------------------------------- prims
+// CNS_advance.cpp:200-208: cons2prim on growntilebox(NGHOST)
+__global__ void __launch_bounds__(BLK) k_prims(const BoxDev* boxes, Region reg, const Fab* st,
+                                               const Fab* pr, Closures c) {
+  REGION_LOOP_BEGIN
+  const Fab u = st[b], q = pr[b];
+  double qq[6];
+  cons2prim(u(i, j, k, 0), u(i, j, k, 1), u(i, j, k, 2), u(i, j, k, 3), u(i, j, k, 4), c, qq);
+#pragma unroll
+  for (int n = 0; n < NPRIM; ++n) q(i, j, k, n) = qq[n];
+}
+
+// ---------------------------------------------------------------- KEEP
+// CentralKEEP.h:40-145 on grownnodaltilebox(-1,0)
+template <int H>
+__global__ void __launch_bounds__(BLK) k_keep(const BoxDev* boxes, Region reg, const Fab* pr,
+                                              const Fab* nfx, const Fab* nfy, const Fab* nfz,
+                                              Closures c, double c0, double c1, double c2) {
+  REGION_LOOP_BEGIN
+  const Fab q = pr[b];
+  const double co[3] = {c0, c1, c2};
+  double fx[5] = {0, 0, 0, 0, 0}, fy[5] = {0, 0, 0, 0, 0}, fz[5] = {0, 0, 0, 0, 0};
+#define KEEP_DIR(F, NRM, DI, DJ, DK)                                                        \
+  _Pragma("unroll") for (int l = 1; l <= H; ++l) {                                         \
+    _Pragma("unroll") for (int m = 0; m <= l - 1; ++m) {                                   \
+      int i1 = i + m * DI, j1 = j + m * DJ, k1 = k + m * DK;                               \
+      int i2 = i + (m - l) * DI, j2 = j + (m - l) * DJ, k2 = k + (m - l) * DK;             \
+      keep_pair<NRM>(co[l - 1], q(i1, j1, k1, QRHO), q(i1, j1, k1, QU), q(i1, j1, k1, QV),   \
+                     q(i1, j1, k1, QW), c.cv * q(i1, j1, k1, QT), q(i1, j1, k1, QPRES),      \
+                     q(i2, j2, k2, QRHO), q(i2, j2, k2, QU), q(i2, j2, k2, QV),              \
+                     q(i2, j2, k2, QW), c.cv * q(i2, j2, k2, QT), q(i2, j2, k2, QPRES), F);  \
+    }                                                                                       \
+  }
+  KEEP_DIR(fx, 0, 1, 0, 0)
+  KEEP_DIR(fy, 1, 0, 1, 0)
+  KEEP_DIR(fz, 2, 0, 0, 1)
+#undef KEEP_DIR
+  const Fab ox = nfx[b], oy = nfy[b], oz = nfz[b];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) {
+    ox(i, j, k, n) = fx[n];
+    oy(i, j, k, n) = fy[n];
+    oz(i, j, k, n) = fz[n];
+  }
+}
+
+// ---------------------------------------------------------------- WENO5
+// High_resolution.h:83-121 cons2eulerflux_lambda on growntilebox(NGHOST)
+__global__ void __launch_bounds__(BLK) k_eulerflux(const BoxDev* boxes, Region reg, const Fab* st,
+                                                   const Fab* pfx, const Fab* pfy, const Fab* pfz,
+                                                   const Fab* lam, Closures c) {
+  REGION_LOOP_BEGIN
+  const Fab u = st[b], fx = pfx[b], fy = pfy[b], fz = pfz[b], L = lam[b];
+  double rhoinv = 1.0 / u(i, j, k, URHO);
+  double momx = u(i, j, k, UMX), momy = u(i, j, k, UMY), momz = u(i, j, k, UMZ);
+  double rhoet = u(i, j, k, UET);
+  double ux = momx * rhoinv, uy = momy * rhoinv, uz = momz * rhoinv;
+  double rhoekin = 0.5 * rhoinv * (momx * momx + momy * momy + momz * momz);
+  double rhoeint = rhoet - rhoekin;
+  double P = c.gm1 * rhoeint;
+  fx(i, j, k, URHO) = momx;
+  fx(i, j, k, UMX) = momx * ux + P;
+  fx(i, j, k, UMY) = momy * ux;
+  fx(i, j, k, UMZ) = momz * ux;
+  fx(i, j, k, UET) = (rhoet + P) * ux;
+  fy(i, j, k, URHO) = momy;
+  fy(i, j, k, UMX) = momx * uy;
+  fy(i, j, k, UMY) = momy * uy + P;
+  fy(i, j, k, UMZ) = momz * uy;
+  fy(i, j, k, UET) = (rhoet + P) * uy;
+  fz(i, j, k, URHO) = momz;
+  fz(i, j, k, UMX) = momx * uz;
+  fz(i, j, k, UMY) = momy * uz;
+  fz(i, j, k, UMZ) = momz * uz + P;
+  fz(i, j, k, UET) = (rhoet + P) * uz;
+  double cs = sqrt(c.gamma * P * rhoinv);
+  L(i, j, k, 0) = fabs(dmax(dmax(ux + cs, ux - cs), ux));
+  L(i, j, k, 1) = fabs(dmax(dmax(uy + cs, uy - cs), uy));
+  L(i, j, k, 2) = fabs(dmax(dmax(uz + cs, uz - cs), uz));
+}
+
+// High_resolution.h:162-246 numericalflux_globallaxsplit for one direction
+template <int D>
+__device__ __forceinline__ double weno_face(int i, int j, int k, int n, const Fab& u, const Fab& pf,
+                                            const Fab& lam) {
+  constexpr int DI = D == 0, DJ = D == 1, DK = D == 2;
+  double me = lam(i - 3 * DI, j - 3 * DJ, k - 3 * DK, D);
+#pragma unroll
+  for (int o = -2; o <= 2; ++o) me = dmax(me, lam(i + o * DI, j + o * DJ, k + o * DK, D));
+  double df[6], P[6];
+#pragma unroll
+  for (int o = -3; o <= 2; ++o) {
+    df[o + 3] = me * u(i + o * DI, j + o * DJ, k + o * DK, n);
+    P[o + 3] = pf(i + o * DI, j + o * DJ, k + o * DK, n);
+  }
+  double flx = weno5js(0.5 * (P[0] + df[0]), 0.5 * (P[1] + df[1]), 0.5 * (P[2] + df[2]),
+                       0.5 * (P[3] + df[3]), 0.5 * (P[4] + df[4]));
+  flx += weno5js(0.5 * (P[5] - df[5]), 0.5 * (P[4] - df[4]), 0.5 * (P[3] - df[3]),
+                 0.5 * (P[2] - df[2]), 0.5 * (P[1] - df[1]));
+  return flx;
+}
+
+__global__ void __launch_bounds__(BLK) k_weno(const BoxDev* boxes, Region reg, const Fab* st,
+                                              const Fab* pfx, const Fab* pfy, const Fab* pfz,
+                                              const Fab* lam, const Fab* nfx, const Fab* nfy,
+                                              const Fab* nfz) {
+  REGION_LOOP_BEGIN
+  const Fab u = st[b], L = lam[b];
+  const Fab ox = nfx[b], oy = nfy[b], oz = nfz[b];
+  for (int n = 0; n < NCONS; ++n) {
+    ox(i, j, k, n) = weno_face<0>(i, j, k, n, u, pfx[b], L);
+    oy(i, j, k, n) = weno_face<1>(i, j, k, n, u, pfy[b], L);
+    oz(i, j, k, n) = weno_face<2>(i, j, k, n, u, pfz[b], L);
+  }
+}
+
+// ---------------------------------------------------------------- Riemann
+// Riemann.h:19-120 cns_slope_{x,y,z} on grow(bx,dir,1)
+template <int D>
+__global__ void __launch_bounds__(BLK) k_slope(const BoxDev* boxes, Region reg, const Fab* pr,
+                                               const Fab* slope, Closures c) {
+  REGION_LOOP_BEGIN
+  constexpr int DI = D == 0, DJ = D == 1, DK = D == 2;
+  constexpr int qn = QU + D;
+  constexpr int qt1 = D == 0 ? QV : QU;
+  constexpr int qt2 = D == 2 ? QV : QW;
+  const Fab q = pr[b], dq = slope[b];
+  auto Q = [&](int o, int comp) { return q(i + o * DI, j + o * DJ, k + o * DK, comp); };
+  const double add = D == 0 ? 2.e-40 : 1.e-40;
+  double cspeed = sqrt(c.gamma * c.Rspec * Q(0, QT)) + add;
+  double dlft = 0.5 * (Q(0, QPRES) - Q(-1, QPRES)) / cspeed - 0.5 * Q(0, QRHO) * (Q(0, qn) - Q(-1, qn));
+  double drgt = 0.5 * (Q(1, QPRES) - Q(0, QPRES)) / cspeed - 0.5 * Q(0, QRHO) * (Q(1, qn) - Q(0, qn));
+  double d0 = limiter(dlft, drgt);
+  double cs2 = cspeed * cspeed;
+  dlft = (Q(0, QRHO) - Q(-1, QRHO)) - (Q(0, QPRES) - Q(-1, QPRES)) / cs2;
+  drgt = (Q(1, QRHO) - Q(0, QRHO)) - (Q(1, QPRES) - Q(0, QPRES)) / cs2;
+  double d1 = limiter(dlft, drgt);
+  dlft = 0.5 * (Q(0, QPRES) - Q(-1, QPRES)) / cspeed + 0.5 * Q(0, QRHO) * (Q(0, qn) - Q(-1, qn));
+  drgt = 0.5 * (Q(1, QPRES) - Q(0, QPRES)) / cspeed + 0.5 * Q(0, QRHO) * (Q(1, qn) - Q(0, qn));
+  double d2 = limiter(dlft, drgt);
+  double d3 = limiter(Q(0, qt1) - Q(-1, qt1), Q(1, qt1) - Q(0, qt1));
+  double d4 = limiter(Q(0, qt2) - Q(-1, qt2), Q(1, qt2) - Q(0, qt2));
+  dq(i, j, k, 0) = d0;
+  dq(i, j, k, 1) = d1;
+  dq(i, j, k, 2) = d2;
+  dq(i, j, k, 3) = d3;
+  dq(i, j, k, 4) = d4;
+}
+
+// Riemann.h:251-337 cns_riemann_{x,y,z} on surroundingNodes(bx,dir).  The z
+// left state takes its sound speed from T at k+1 (Riemann.h:316).
+template <int D>
+__global__ void __launch_bounds__(BLK) k_riemann(const BoxDev* boxes, Region reg, const Fab* pr,
+                                                 const Fab* slope, const Fab* nf, Closures c) {
+  REGION_LOOP_BEGIN
+  constexpr int DI = D == 0, DJ = D == 1, DK = D == 2;
+  constexpr int qn = QU + D;
+  constexpr int qt1 = D == 0 ? QV : QU;
+  constexpr int qt2 = D == 2 ? QV : QW;
+  const Fab q = pr[b], dq = slope[b], f = nf[b];
+  const int il = i - DI, jl = j - DJ, kl = k - DK;
+  double Tl = D == 2 ? q(i, j, k + 1, QT) : q(il, jl, kl, QT);
+  double cspeed = sqrt(c.gamma * c.Rspec * Tl) + 1.e-40;
+  double rl = q(il, jl, kl, QRHO) + 0.5 * ((dq(il, jl, kl, 0) + dq(il, jl, kl, 2)) / cspeed + dq(il, jl, kl, 1));
+  rl = dmax(rl, 1.0e-40);
+  double ul = q(il, jl, kl, qn) + 0.5 * ((dq(il, jl, kl, 2) - dq(il, jl, kl, 0)) / q(il, jl, kl, QRHO));
+  double pl = q(il, jl, kl, QPRES) + 0.5 * (dq(il, jl, kl, 0) + dq(il, jl, kl, 2)) * cspeed;
+  pl = dmax(pl, 1.0e-40);
+  double ut1l = q(il, jl, kl, qt1) + 0.5 * dq(il, jl, kl, 3);
+  double ut2l = q(il, jl, kl, qt2) + 0.5 * dq(il, jl, kl, 4);
+  cspeed = sqrt(c.gamma * c.Rspec * q(i, j, k, QT)) + 1.e-40;
+  double rr = q(i, j, k, QRHO) - 0.5 * ((dq(i, j, k, 0) + dq(i, j, k, 2)) / cspeed + dq(i, j, k, 1));
+  rr = dmax(rr, 1.0e-40);
+  double ur = q(i, j, k, qn) - 0.5 * ((dq(i, j, k, 2) - dq(i, j, k, 0)) / q(i, j, k, QRHO));
+  double pr_ = q(i, j, k, QPRES) - 0.5 * (dq(i, j, k, 0) + dq(i, j, k, 2)) * cspeed;
+  pr_ = dmax(pr_, 1.0e-40);
+  double ut1r = q(i, j, k, qt1) - 0.5 * dq(i, j, k, 3);
+  double ut2r = q(i, j, k, qt2) - 0.5 * dq(i, j, k, 4);
+  double o[5];
+  riemann_prob(c.gamma, 1.0e-40, rl, ul, pl, ut1l, ut2l, rr, ur, pr_, ut1r, ut2r, o);
+  constexpr int cn = UMX + D;
+  constexpr int ct1 = D == 0 ? UMY : UMX;
+  constexpr int ct2 = D == 2 ? UMY : UMZ;
+  f(i, j, k, URHO) = o[0];
+  f(i, j, k, cn) = o[1];
+  f(i, j, k, ct1) = o[2];
+  f(i, j, k, ct2) = o[3];
+  f(i, j, k, UET) = o[4];
+}
+
+// ---------------------------------------------------------------- viscous
+// CNS_hydro_K.h:108-172 ViscousFluxes on growntilebox(1)
+__global__ void __launch_bounds__(BLK) k_visc_pf(const BoxDev* boxes, Region reg, const Fab* pr,
+                                                 const Fab* pfx, const Fab* pfy, const Fab* pfz,
+                                                 Closures c, double dx0, double dx1, double dx2) {
+  REGION_LOOP_BEGIN
+  const Fab q = pr[b], fx = pfx[b], fy = pfy[b], fz = pfz[b];
+  double ux = q(i, j, k, QU);
+  double dudx = (q(i + 1, j, k, QU) - q(i - 1, j, k, QU)) * 0.5 * dx0;
+  double dvdx = (q(i + 1, j, k, QV) - q(i - 1, j, k, QV)) * 0.5 * dx0;
+  double dwdx = (q(i + 1, j, k, QW) - q(i - 1, j, k, QW)) * 0.5 * dx0;
+  double dTdx = (q(i + 1, j, k, QT) - q(i - 1, j, k, QT)) * 0.5 * dx0;
+  double uy = q(i, j, k, QV);
+  double dudy = (q(i, j + 1, k, QU) - q(i, j - 1, k, QU)) * 0.5 * dx1;
+  double dvdy = (q(i, j + 1, k, QV) - q(i, j - 1, k, QV)) * 0.5 * dx1;
+  double dwdy = (q(i, j + 1, k, QW) - q(i, j - 1, k, QW)) * 0.5 * dx1;
+  double dTdy = (q(i, j + 1, k, QT) - q(i, j - 1, k, QT)) * 0.5 * dx1;
+  double uz = q(i, j, k, QW);
+  double dudz = (q(i, j, k + 1, QU) - q(i, j, k - 1, QU)) * 0.5 * dx2;
+  double dvdz = (q(i, j, k + 1, QV) - q(i, j, k - 1, QV)) * 0.5 * dx2;
+  double dwdz = (q(i, j, k + 1, QW) - q(i, j, k - 1, QW)) * 0.5 * dx2;
+  double dTdz = (q(i, j, k + 1, QT) - q(i, j, k - 1, QT)) * 0.5 * dx2;
+  double div = dudx + dvdy + dwdz;
+  double T = q(i, j, k, QT);
+  double mu = visc(c, T);
+  double lambda = cond(c, T, mu);
+  double r1_3 = 1.0 / 3.0;
+  double tauxx = 2.0 * mu * (dudx - r1_3 * div);
+  double tauxy = mu * (dudy + dvdx);
+  double tauxz = mu * (dudz + dwdx);
+  double tauyy = 2.0 * mu * (dvdy - r1_3 * div);
+  double tauyz = mu * (dvdz + dwdy);
+  double tauzz = 2.0 * mu * (dwdz - r1_3 * div);
+  fx(i, j, k, URHO) = 0.0;
+  fx(i, j, k, UMX) = -tauxx;
+  fx(i, j, k, UMY) = -tauxy;
+  fx(i, j, k, UMZ) = -tauxz;
+  fx(i, j, k, UET) = -lambda * dTdx - tauxx * ux - tauxy * uy - tauxz * uz;
+  fy(i, j, k, URHO) = 0.0;
+  fy(i, j, k, UMX) = -tauxy;
+  fy(i, j, k, UMY) = -tauyy;
+  fy(i, j, k, UMZ) = -tauyz;
+  fy(i, j, k, UET) = -lambda * dTdy - tauxy * ux - tauyy * uy - tauyz * uz;
+  fz(i, j, k, URHO) = 0.0;
+  fz(i, j, k, UMX) = -tauxz;
+  fz(i, j, k, UMY) = -tauyz;
+  fz(i, j, k, UMZ) = -tauzz;
+  fz(i, j, k, UET) = -lambda * dTdz - tauxz * ux - tauyz * uy - tauzz * uz;
+}
+
+// CNS_hydro_K.h:254-261 ViscousNumericalFluxes on grownnodaltilebox(-1,0)
+__global__ void __launch_bounds__(BLK) k_visc_add(const BoxDev* boxes, Region reg, const Fab* pfx,
+                                                  const Fab* pfy, const Fab* pfz, const Fab* nfx,
+                                                  const Fab* nfy, const Fab* nfz) {
+  REGION_LOOP_BEGIN
+  const Fab px = pfx[b], py = pfy[b], pz = pfz[b], ox = nfx[b], oy = nfy[b], oz = nfz[b];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) {
+    ox(i, j, k, n) += 0.5 * (px(i - 1, j, k, n) + px(i, j, k, n));
+    oy(i, j, k, n) += 0.5 * (py(i, j - 1, k, n) + py(i, j, k, n));
+    oz(i, j, k, n) += 0.5 * (pz(i, j, k - 1, n) + pz(i, j, k, n));
+  }
+}
+
+// ---------------------------------------------------------------- assembly
+// CNS_advance.cpp:376-391 plus the stage combination of CNS::advance
+__global__ void __launch_bounds__(BLK) k_assemble(const BoxDev* boxes, Region reg, const Fab* nfx,
+                                                  const Fab* nfy, const Fab* nfz, double dx0,
+                                                  double dx1, double dx2, StageCoef sc,
+                                                  const Fab* s1, const Fab* sin, const Fab* out) {
+  REGION_LOOP_BEGIN
+  const Fab fx = nfx[b], fy = nfy[b], fz = nfz[b], o = out[b];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) {
+    double L = dx0 * (fx(i, j, k, n) - fx(i + 1, j, k, n)) + dx1 * (fy(i, j, k, n) - fy(i, j + 1, k, n)) +
+               dx2 * (fz(i, j, k, n) - fz(i, j, k + 1, n));
+    double v;
+    if (sc.mode == 2) {
+      v = L;
+    } else {
+      double a = (sc.a != 0.0) ? s1[b](i, j, k, n) : 0.0;
+      v = stage_combine(sc, a, sin[b](i, j, k, n), L);
+    }
+    o(i, j, k, n) = v;
+  }
+}
+
+dim3 grid_for(const BoxTable& bt, const Region& r) {
+  int64_t mx = 1;
+  for (int b = 0; b < bt.nbox; ++b) {
+    int64_t n = 1;
+    for (int d = 0; d < 3; ++d) n *= int64_t(bt.h_boxes[b].hi[d] + r.ghi[d] - bt.h_boxes[b].lo[d] - r.glo[d] + 1);
+    mx = n > mx ? n : mx;
+  }
+  return dim3(unsigned((mx + BLK - 1) / BLK), unsigned(bt.nbox), 1);
+}
+
+Region grown(int g) { return Region{{-g, -g, -g}, {g, g, g}}; }
+Region nodal() { return Region{{0, 0, 0}, {1, 1, 1}}; }
+
+}  // namespace
+
+// Flux phase + assembly + stage combination, reference pass structure.
+hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const Closures& c,
+                                const amr_scheme_t& s, const double dxinv[3], const StageCoef& sc,
+                                const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
+  const BoxDev* B = bt.d_boxes;
+  {
+    Region r = grown(NGHOST);
+    hipLaunchKernelGGL(k_prims, grid_for(bt, r), dim3(BLK), 0, st, B, r, sin, w.prims, c);
+  }
+  if (s.rhs_euler) {
+    if (s.flux_euler == 1) {
+      double co[3];
+      keep_coefficients(s.order_keep, co);
+      Region r = nodal();
+      dim3 g = grid_for(bt, r);
+      if (s.order_keep == 6)
+        hipLaunchKernelGGL(k_keep<3>, g, dim3(BLK), 0, st, B, r, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
+      else if (s.order_keep == 4)
+        hipLaunchKernelGGL(k_keep<2>, g, dim3(BLK), 0, st, B, r, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
+      else
+        hipLaunchKernelGGL(k_keep<1>, g, dim3(BLK), 0, st, B, r, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
+    } else if (s.flux_euler == 2) {
+      Region r = grown(NGHOST);
+      hipLaunchKernelGGL(k_eulerflux, grid_for(bt, r), dim3(BLK), 0, st, B, r, sin, w.pe[0], w.pe[1], w.pe[2], w.lam, c);
+      Region rn = nodal();
+      hipLaunchKernelGGL(k_weno, grid_for(bt, rn), dim3(BLK), 0, st, B, rn, sin, w.pe[0], w.pe[1], w.pe[2], w.lam,
+                         w.nf[0], w.nf[1], w.nf[2]);
+    } else {
+      for (int d = 0; d < 3; ++d) {
+        Region rs{{0, 0, 0}, {0, 0, 0}};
+        rs.glo[d] = -1;
+        rs.ghi[d] = 1;
+        Region rf{{0, 0, 0}, {0, 0, 0}};
+        rf.ghi[d] = 1;
+        dim3 gs = grid_for(bt, rs), gf = grid_for(bt, rf);
+        if (d == 0) {
+          hipLaunchKernelGGL(k_slope<0>, gs, dim3(BLK), 0, st, B, rs, w.prims, w.slope, c);
+          hipLaunchKernelGGL(k_riemann<0>, gf, dim3(BLK), 0, st, B, rf, w.prims, w.slope, w.nf[0], c);
+        } else if (d == 1) {
+          hipLaunchKernelGGL(k_slope<1>, gs, dim3(BLK), 0, st, B, rs, w.prims, w.slope, c);
+          hipLaunchKernelGGL(k_riemann<1>, gf, dim3(BLK), 0, st, B, rf, w.prims, w.slope, w.nf[1], c);
+        } else {
+          hipLaunchKernelGGL(k_slope<2>, gs, dim3(BLK), 0, st, B, rs, w.prims, w.slope, c);
+          hipLaunchKernelGGL(k_riemann<2>, gf, dim3(BLK), 0, st, B, rf, w.prims, w.slope, w.nf[2], c);
+        }
+      }
+    }
+  }
+  if (s.rhs_visc) {
+    Region r1 = grown(1);
+    hipLaunchKernelGGL(k_visc_pf, grid_for(bt, r1), dim3(BLK), 0, st, B, r1, w.prims, w.pv[0], w.pv[1], w.pv[2], c,
+                       dxinv[0], dxinv[1], dxinv[2]);
+    Region rn = nodal();
+    hipLaunchKernelGGL(k_visc_add, grid_for(bt, rn), dim3(BLK), 0, st, B, rn, w.pv[0], w.pv[1], w.pv[2], w.nf[0],
+                       w.nf[1], w.nf[2]);
+  }
+  Region r0 = grown(0);
+  hipLaunchKernelGGL(k_assemble, grid_for(bt, r0), dim3(BLK), 0, st, B, r0, w.nf[0], w.nf[1], w.nf[2], dxinv[0],
+                     dxinv[1], dxinv[2], sc, s1, sin, out);
+  return hipGetLastError();
+}
+
+}  // namespace amr

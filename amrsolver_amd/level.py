@@ -1,0 +1,220 @@
+"""Host-side mirror of the reference's CNS level interface over the C ABI.
+
+``Level`` plays the role of the CNS AmrLevel (src/CNS.h): it owns the box
+table (BoxArray + post_regrid workspace) and offers ``advance`` (CNS::advance),
+``compute_rhs`` (CNS::compute_rhs), ``fill_patch`` (FillPatch), ``est_time_step``
+(CNS::estTimeStep) and ``totals`` (CNS::printTotal sums).  Device memory
+comes from torch (plumbing only); all arithmetic runs in the HIP library.
+"""
+import ctypes
+
+import numpy as np
+
+from . import capi
+from .capi import Fab, call
+
+_torch = None
+
+
+def _torch_mod():
+    global _torch
+    if _torch is None:
+        import torch
+        _torch = torch
+    return _torch
+
+
+class Context:
+    """amr_ctx_t bound to torch's current stream on `device`."""
+
+    def __init__(self, device=0, stream=None):
+        torch = _torch_mod()
+        if not torch.cuda.is_available():
+            raise capi.AmrError("no GPU visible: the HIP path needs a device")
+        torch.cuda.set_device(device)
+        self.device = device
+        self.stream = stream if stream is not None else torch.cuda.current_stream(device)
+        h = ctypes.c_void_p()
+        call("amr_ctx_create", device, ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h))
+        self.h = h
+
+    def synchronize(self):
+        call("amr_ctx_synchronize", self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            capi.load().amr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class StateArrays:
+    """Device arrays of one MultiFab: either one level-wide array (boxes are
+    windows into it) or one array per box."""
+
+    def __init__(self, level, ncomp=5, ng=3, layout="level"):
+        torch = _torch_mod()
+        self.level = level
+        self.ng = ng
+        self.layout = layout
+        self.tensors = []
+        nb = len(level.boxes)
+        self.fabs = (Fab * nb)()
+        dev = torch.device("cuda", level.ctx.device)
+        if layout == "level":
+            N = level.n
+            shape = (ncomp, N[2] + 2 * ng, N[1] + 2 * ng, N[0] + 2 * ng)
+            t = torch.zeros(shape, dtype=torch.float64, device=dev)
+            self.tensors.append(t)
+            JS, KS, NS = shape[3], shape[3] * shape[2], shape[3] * shape[2] * shape[1]
+            base = t.data_ptr()
+            for b, (lo, hi) in enumerate(level.boxes):
+                f = self.fabs[b]
+                for d in range(3):
+                    f.lo[d], f.hi[d] = lo[d], hi[d]
+                f.ng = ng
+                f.pad_ = 0
+                off = lo[0] + lo[1] * JS + lo[2] * KS  # (lo-ng)+ng in every direction
+                f.data = base + 8 * off
+                f.jstride, f.kstride, f.nstride = JS, KS, NS
+        elif layout == "boxes":
+            for b, (lo, hi) in enumerate(level.boxes):
+                n = [hi[d] - lo[d] + 1 for d in range(3)]
+                shape = (ncomp, n[2] + 2 * ng, n[1] + 2 * ng, n[0] + 2 * ng)
+                t = torch.zeros(shape, dtype=torch.float64, device=dev)
+                self.tensors.append(t)
+                f = self.fabs[b]
+                for d in range(3):
+                    f.lo[d], f.hi[d] = lo[d], hi[d]
+                f.ng = ng
+                f.pad_ = 0
+                f.data = t.data_ptr()
+                f.jstride = shape[3]
+                f.kstride = shape[3] * shape[2]
+                f.nstride = shape[3] * shape[2] * shape[1]
+        else:
+            raise ValueError(layout)
+
+    def upload(self, u):
+        """u: (ncomp, nz, ny, nx) valid-cell array of the whole domain."""
+        torch = _torch_mod()
+        g = self.ng
+        if self.layout == "level":
+            t = self.tensors[0]
+            t[:, g:t.shape[1] - g, g:t.shape[2] - g, g:t.shape[3] - g] = torch.from_numpy(np.ascontiguousarray(u)).to(t.device)
+        else:
+            for (lo, hi), t in zip(self.level.boxes, self.tensors):
+                sub = u[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+                t[:, g:t.shape[1] - g, g:t.shape[2] - g, g:t.shape[3] - g] = torch.from_numpy(np.ascontiguousarray(sub)).to(t.device)
+        torch.cuda.synchronize()
+
+    def download(self):
+        """Valid cells of every box as one (ncomp, nz, ny, nx) numpy array."""
+        torch = _torch_mod()
+        self.level.ctx.synchronize()
+        torch.cuda.synchronize()
+        g = self.ng
+        N = self.level.n
+        if self.layout == "level":
+            t = self.tensors[0]
+            return t[:, g:t.shape[1] - g, g:t.shape[2] - g, g:t.shape[3] - g].cpu().numpy().copy()
+        out = np.zeros((self.tensors[0].shape[0], N[2], N[1], N[0]))
+        for (lo, hi), t in zip(self.level.boxes, self.tensors):
+            out[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = \
+                t[:, g:t.shape[1] - g, g:t.shape[2] - g, g:t.shape[3] - g].cpu().numpy()
+        return out
+
+    def download_box(self, b):
+        """Whole ghosted array of box b, (ncomp, nz+2g, ny+2g, nx+2g)."""
+        torch = _torch_mod()
+        self.level.ctx.synchronize()
+        torch.cuda.synchronize()
+        g = self.ng
+        lo, hi = self.level.boxes[b]
+        if self.layout == "boxes":
+            return self.tensors[b].cpu().numpy().copy()
+        t = self.tensors[0]
+        return t[:, lo[2]:hi[2] + 1 + 2 * g, lo[1]:hi[1] + 1 + 2 * g, lo[0]:hi[0] + 1 + 2 * g].cpu().numpy().copy()
+
+
+class Level:
+    """One AMR level on one GPU (mirror of the CNS AmrLevel's hot path)."""
+
+    def __init__(self, geom, boxes, closures, scheme, ctx=None, device=0):
+        self.ctx = ctx if ctx is not None else Context(device)
+        self.geom = geom
+        self.boxes = [(tuple(lo), tuple(hi)) for lo, hi in boxes]
+        self.n = [geom.dom_hi[d] - geom.dom_lo[d] + 1 for d in range(3)]
+        self.closures = closures
+        self.scheme = scheme
+        arr = (ctypes.c_int32 * (6 * len(self.boxes)))()
+        for b, (lo, hi) in enumerate(self.boxes):
+            for d in range(3):
+                arr[6 * b + d] = lo[d]
+                arr[6 * b + 3 + d] = hi[d]
+        h = ctypes.c_void_p()
+        call("amr_boxtab_create", self.ctx.h, ctypes.byref(geom), len(self.boxes), arr, ctypes.byref(h))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            capi.load().amr_boxtab_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def new_state(self, layout="level", ng=3, ncomp=5):
+        return StateArrays(self, ncomp=ncomp, ng=ng, layout=layout)
+
+    # --- reference interface -------------------------------------------
+    def fill_patch(self, S):
+        call("amr_fill_boundary", self.h, S.fabs)
+
+    def compute_rhs(self, Sborder, dSdt):
+        call("amr_rhs", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), Sborder.fabs, dSdt.fabs)
+
+    def rk_stage(self, mode, a, b, c, d, S1, Sin, Sout):
+        call("amr_rk_stage", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), mode, a, b, c, d,
+             S1.fabs if S1 is not None else None, Sin.fabs, Sout.fabs)
+
+    def advance(self, S, time, dt):
+        call("amr_advance", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, time, dt)
+
+    def advance_graph(self, S, dt, nsteps):
+        call("amr_advance_graph", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, dt,
+             nsteps)
+
+    def est_time_step(self, S, cfl):
+        out = ctypes.c_double()
+        call("amr_est_dt", self.h, ctypes.byref(self.closures), S.fabs, cfl, ctypes.byref(out))
+        return out.value
+
+    def totals(self, S):
+        out = (ctypes.c_double * 5)()
+        call("amr_totals", self.h, S.fabs, out)
+        return list(out)
+
+    def synchronize(self):
+        self.ctx.synchronize()
+
+
+def compute_new_dt(est, dt_prev, cur_time, stop_time, first):
+    """CNS::computeInitialDt / computeNewDt dt policy (src/CNS.cpp:282-377)
+    for one level: growth capped at 1.1x, clipped to stop_time with
+    eps = 0.001 dt."""
+    if first:
+        return est
+    dt0 = min(est, 1.1 * dt_prev)
+    eps = 0.001 * dt0
+    if stop_time >= 0.0 and (cur_time + dt0) > (stop_time - eps):
+        dt0 = stop_time - cur_time
+    return dt0

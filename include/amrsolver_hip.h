@@ -1,0 +1,157 @@
+/*
+ * amrsolver_hip: MI355X (gfx950) implementation of amrsolver's per-box
+ * right-hand side and SSPRK stage update, behind a C ABI.
+ *
+ * Every entry point returns AMR_OK (0) or a negative status; amr_last_error()
+ * gives the message of the last failing call on the calling thread.  All
+ * device work is enqueued on the context's stream and is asynchronous unless
+ * stated otherwise.  A context (and the box tables made from it) must not be
+ * used from two host threads at once.
+ *
+ * Array layout ("FAB", AMReX FArrayBox/Array4 convention): Fortran order, i
+ * fastest, then j, then k, component outermost.  An amr_fab_t describes one
+ * box's array: `data` points at element (lo-ng, lo-ng, lo-ng, 0), strides are
+ * in elements.  Several boxes may share one allocation (a level-wide array)
+ * by pointing into it with the level's strides; their ghost cells are then
+ * their neighbours' valid cells.
+ *
+ * Conserved components: 0 rho, 1 rho*u, 2 rho*v, 3 rho*w, 4 rho*E
+ * (exm/Sods/prob.h:16-21).  Ghost width NGHOST = 3 (exm/Sods/prob.h:39).
+ */
+#ifndef AMRSOLVER_HIP_H_
+#define AMRSOLVER_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMR_OK 0
+#define AMR_ERR_ARG (-1)      /* invalid argument / unsupported combination */
+#define AMR_ERR_HIP (-2)      /* HIP runtime error                         */
+#define AMR_ERR_STATE (-3)    /* non-physical state detected (rho<=0, NaN)  */
+
+#define AMR_NCONS 5
+#define AMR_NGHOST 3
+
+/* Closure object: the POD form of PROB::ProbClosures =
+ * closures_derived_t<Visc,Cond,Thermo> (src/cls/Closures.h:172-182) with
+ * calorifically_perfect_gas_t (Closures.h:72-82), visc_const_t/visc_suth_t
+ * (:103-120), cond_const_t/cond_suth_t (:130-166).  cond_law 2 is the
+ * constant-Prandtl conductivity k = mu*cp/Pr. */
+typedef struct {
+  double gamma, mw, Ru, cv, cp, Rspec, Cshock, Cdamp;
+  int32_t visc_law; /* 0 constant, 1 Sutherland */
+  double visc_ref, Tvisc_ref;
+  int32_t cond_law; /* 0 constant, 1 Sutherland, 2 mu*cp/Pr */
+  double cond_ref, Tcond_ref, Pr;
+} amr_closures_t;
+
+/* The cns.* scheme switches read by CNS::read_params (src/CNS.cpp:85-153). */
+typedef struct {
+  int32_t rhs_euler, rhs_visc, rhs_source;
+  int32_t flux_euler; /* 0 MUSCL+Riemann, 1 KEEP, 2 WENO5-JS with LF split */
+  int32_t order_keep; /* 2, 4, 6 */
+  int32_t art_diss;   /* must be 0 */
+  int32_t order_rk, stages_rk;
+} amr_scheme_t;
+
+/* Level geometry (AMReX Geometry + the cns.lo_bc/hi_bc codes 0..6). */
+typedef struct {
+  int32_t dom_lo[3], dom_hi[3];
+  int32_t periodic[3];
+  int32_t lo_bc[3], hi_bc[3];
+  double dx[3];
+  double prob_lo[3];
+} amr_geom_t;
+
+/* One box's array (FArrayBox / Array4). */
+typedef struct {
+  int32_t lo[3], hi[3]; /* valid box, inclusive */
+  int32_t ng;           /* ghost cells the array holds around the valid box */
+  int32_t pad_;
+  double* data;         /* device pointer to element (lo-ng, lo-ng, lo-ng, comp 0) */
+  int64_t jstride, kstride, nstride; /* element strides */
+} amr_fab_t;
+
+typedef struct amr_ctx_s* amr_ctx_t;
+typedef struct amr_boxtab_s* amr_boxtab_t;
+
+/* Last error message of the calling thread ("" if none). */
+const char* amr_last_error(void);
+
+/* Device, stream and tuning table.  `stream` is a hipStream_t (NULL: the
+ * context creates its own).  Replaces the AMReX runtime state the CNS level
+ * relies on (amrex::Initialize, The_Arena, Gpu::gpuStream). */
+int amr_ctx_create(int device, void* stream, amr_ctx_t* out);
+int amr_ctx_destroy(amr_ctx_t ctx);
+int amr_ctx_synchronize(amr_ctx_t ctx);
+
+/* Box table of one level: the BoxArray plus the per-level workspace that
+ * CNS::post_regrid allocates (src/CNS.cpp:432-467).  `boxes` holds
+ * lo[3],hi[3] per box.  Rebuild after a regrid. */
+int amr_boxtab_create(amr_ctx_t ctx, const amr_geom_t* geom, int nbox, const int32_t* boxes,
+                      amr_boxtab_t* out);
+int amr_boxtab_destroy(amr_boxtab_t bt);
+
+/* Ghost fill of one level from its own valid data (single-level FillPatch:
+ * FillBoundary + periodic images + foextrap/reflect physical BCs,
+ * src/set/bcs.cpp:95-106, src/set/CNS_setup.cpp:18-31).  fabs[nbox] need
+ * ng >= 3. */
+int amr_fill_boundary(amr_boxtab_t bt, const amr_fab_t* fabs);
+
+/* dSdt = L(state) for every box: CNS::compute_rhs (src/tim/CNS_advance.cpp:186-427)
+ * with the per-scheme Flux functions (Riemann::Flux, Riemann.h:339;
+ * HiRes::FluxWENO, High_resolution.h:279; CentralKEEP::FluxKEEP,
+ * CentralKEEP.h:189).  `state` must be ghost-filled (ng >= 3); dsdt needs
+ * ng >= 0.  This is the order_rk = 0 unit. */
+int amr_rhs(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch,
+            const amr_fab_t* state, const amr_fab_t* dsdt);
+
+/* One fused RHS + stage combination (the compute_rhs call plus the
+ * MultiFab::LinComb/Saxpy/Xpay that follow it in CNS::advance,
+ * src/tim/CNS_advance.cpp:51-181):
+ *   mode 0: out = a*s1 + b*(c*sin + d*L(sin))
+ *   mode 1: out = (a*s1 + c*sin) + d*L(sin)
+ *   mode 2: out = L(sin)
+ * `sin` must be ghost-filled.  `out` may alias `s1` (same cell is read then
+ * written by one thread) but must not alias `sin`.  s1 is not read when
+ * a == 0.  amr_stage_coeffs() gives (mode,a,b,c,d) for each stage of the
+ * reference's schemes. */
+int amr_rk_stage(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, int mode,
+                 double a, double b, double c, double d, const amr_fab_t* s1, const amr_fab_t* sin,
+                 const amr_fab_t* out);
+
+/* Stage count and coefficients of stage `stage` (0-based) of the scheme
+ * (order_rk, stages_rk) for step size dt, in the form amr_rk_stage takes. */
+int amr_stage_coeffs(int order_rk, int stages_rk, double dt, int stage, int* nstages, int* mode,
+                     double* a, double* b, double* c, double* d);
+
+/* One time step of the level: CNS::advance (src/tim/CNS_advance.cpp:14-183)
+ * = per stage FillPatch + compute_rhs + stage update.  `state` (ng >= 3)
+ * holds U^n on entry and U^{n+1} on exit (valid cells; ghosts are left as
+ * filled for the last stage).  Uses two ghosted workspace arrays owned by
+ * the box table. */
+int amr_advance(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch,
+                const amr_fab_t* state, double time, double dt);
+
+/* nsteps steps captured once into a hipGraph and replayed (fixed dt).
+ * Capture happens on the first call for a given (state, scheme, dt,
+ * nsteps); later calls replay. */
+int amr_advance_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch,
+                      const amr_fab_t* state, double dt, int nsteps);
+
+/* CFL time step: cfl * min over valid cells of cns_estdt (src/cls/CNS_K.h:12-47,
+ * CNS::estTimeStep src/CNS.cpp:380-395).  Synchronous. */
+int amr_est_dt(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, double cfl,
+               double* dt_out);
+
+/* Conserved totals sum(U_n) * ProbSize (CNS::printTotal, src/CNS.cpp:537-542).
+ * Synchronous. */
+int amr_totals(amr_boxtab_t bt, const amr_fab_t* state, double out[5]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

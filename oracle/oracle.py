@@ -1,0 +1,262 @@
+"""ctypes binding of the CPU oracle (oracle/cns_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "_build", "libcns_oracle.so")
+
+
+class OClosures(ctypes.Structure):
+    _fields_ = [
+        ("gamma", ctypes.c_double), ("mw", ctypes.c_double), ("Ru", ctypes.c_double),
+        ("cv", ctypes.c_double), ("cp", ctypes.c_double), ("Rspec", ctypes.c_double),
+        ("Cshock", ctypes.c_double), ("Cdamp", ctypes.c_double),
+        ("visc_law", ctypes.c_int), ("visc_ref", ctypes.c_double), ("Tvisc_ref", ctypes.c_double),
+        ("cond_law", ctypes.c_int), ("cond_ref", ctypes.c_double), ("Tcond_ref", ctypes.c_double),
+        ("Pr", ctypes.c_double),
+    ]
+
+
+class OScheme(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "rhs_euler", "rhs_visc", "rhs_source", "flux_euler", "order_keep", "art_diss",
+        "order_rk", "stages_rk")]
+
+
+class OGeom(ctypes.Structure):
+    _fields_ = [
+        ("dom_lo", ctypes.c_int * 3), ("dom_hi", ctypes.c_int * 3), ("periodic", ctypes.c_int * 3),
+        ("lo_bc", ctypes.c_int * 3), ("hi_bc", ctypes.c_int * 3),
+        ("dx", ctypes.c_double * 3), ("prob_lo", ctypes.c_double * 3),
+    ]
+
+
+def _copy_struct(dst_cls, src):
+    d = dst_cls()
+    for name, _ in dst_cls._fields_:
+        v = getattr(src, name)
+        if isinstance(v, ctypes.Array):
+            arr = getattr(d, name)
+            for i in range(len(v)):
+                arr[i] = v[i]
+        else:
+            setattr(d, name, v)
+    return d
+
+
+def closures(c):
+    return _copy_struct(OClosures, c)
+
+
+def scheme(s):
+    return _copy_struct(OScheme, s)
+
+
+def geom(g):
+    return _copy_struct(OGeom, g)
+
+
+_lib = None
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        build()
+    lib = ctypes.CDLL(LIBPATH)
+    I3 = ctypes.c_int * 3
+    lib.orc_last_error.restype = ctypes.c_char_p
+    lib.orc_visc.restype = ctypes.c_double
+    lib.orc_visc.argtypes = [ctypes.POINTER(OClosures), ctypes.c_double]
+    lib.orc_cond.restype = ctypes.c_double
+    lib.orc_cond.argtypes = [ctypes.POINTER(OClosures), ctypes.c_double]
+    lib.orc_cons2prim_point.argtypes = [ctypes.POINTER(OClosures), _DP, _DP]
+    lib.orc_weno5js.restype = ctypes.c_double
+    lib.orc_weno5js.argtypes = [_DP]
+    lib.orc_limiter.restype = ctypes.c_double
+    lib.orc_limiter.argtypes = [ctypes.c_double, ctypes.c_double]
+    lib.orc_riemann_prob.argtypes = [ctypes.c_double] * 12 + [_DP]
+    lib.orc_box_prims.argtypes = [ctypes.POINTER(OClosures), I3, I3, ctypes.c_int, _DP, _DP]
+    lib.orc_box_numflux.argtypes = [ctypes.POINTER(OClosures), ctypes.POINTER(OScheme), I3, I3, ctypes.c_int,
+                                    _DP, _DP, _DP, _DP, _DP]
+    lib.orc_box_rhs.argtypes = [ctypes.POINTER(OClosures), ctypes.POINTER(OScheme), I3, I3, ctypes.c_int,
+                                _DP, _DP, _DP]
+    PP = ctypes.POINTER(_DP)
+    lib.orc_fill_patch.argtypes = [ctypes.POINTER(OGeom), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.c_int, PP]
+    lib.orc_advance.argtypes = [ctypes.POINTER(OGeom), ctypes.POINTER(OClosures), ctypes.POINTER(OScheme),
+                                ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, PP,
+                                ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    lib.orc_est_dt.restype = ctypes.c_double
+    lib.orc_est_dt.argtypes = [ctypes.POINTER(OGeom), ctypes.POINTER(OClosures), ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_int), ctypes.c_int, PP, ctypes.c_double]
+    lib.orc_totals.argtypes = [ctypes.POINTER(OGeom), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                               ctypes.c_int, PP, _DP]
+    lib.orc_rk_linear.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_int, _DP]
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_DP)
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("oracle: " + load().orc_last_error().decode())
+
+
+# ------------------------------------------------------------------ points
+
+
+def visc(c, T):
+    return load().orc_visc(ctypes.byref(closures(c)), T)
+
+
+def cond(c, T):
+    return load().orc_cond(ctypes.byref(closures(c)), T)
+
+
+def cons2prim(c, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    q = np.zeros(6)
+    load().orc_cons2prim_point(ctypes.byref(closures(c)), _ptr(u), _ptr(q))
+    return q
+
+
+def weno5js(s):
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    return load().orc_weno5js(_ptr(s))
+
+
+def limiter(a, b):
+    return load().orc_limiter(a, b)
+
+
+def riemann_prob(gamma, rl, ul, pl, ut1l, ut2l, rr, ur, pr, ut1r, ut2r, smallp=1.0e-40):
+    out = np.zeros(5)
+    load().orc_riemann_prob(gamma, smallp, rl, ul, pl, ut1l, ut2l, rr, ur, pr, ut1r, ut2r, _ptr(out))
+    return out
+
+
+# ------------------------------------------------------------------ boxes
+# A box FAB is a numpy array (ncomp, nz+2ng, ny+2ng, nx+2ng) (i fastest).
+
+
+def box_rhs(c, s, lo, hi, ng, state, dxinv):
+    n = [hi[d] - lo[d] + 1 for d in range(3)]
+    out = np.zeros((5, n[2], n[1], n[0]))
+    I3 = ctypes.c_int * 3
+    dx = np.ascontiguousarray(dxinv, dtype=np.float64)
+    _check(load().orc_box_rhs(ctypes.byref(closures(c)), ctypes.byref(scheme(s)), I3(*lo), I3(*hi), ng,
+                              _ptr(np.ascontiguousarray(state)), _ptr(dx), _ptr(out)))
+    return out
+
+
+def box_numflux(c, s, lo, hi, ng, state, dxinv):
+    n = [hi[d] - lo[d] + 2 + 2 * ng for d in range(3)]
+    f = [np.zeros((5, n[2], n[1], n[0])) for _ in range(3)]
+    I3 = ctypes.c_int * 3
+    dx = np.ascontiguousarray(dxinv, dtype=np.float64)
+    _check(load().orc_box_numflux(ctypes.byref(closures(c)), ctypes.byref(scheme(s)), I3(*lo), I3(*hi), ng,
+                                  _ptr(np.ascontiguousarray(state)), _ptr(dx), _ptr(f[0]), _ptr(f[1]),
+                                  _ptr(f[2])))
+    return f
+
+
+def box_prims(c, lo, hi, ng, state):
+    q = np.zeros((6,) + state.shape[1:])
+    I3 = ctypes.c_int * 3
+    _check(load().orc_box_prims(ctypes.byref(closures(c)), I3(*lo), I3(*hi), ng,
+                                _ptr(np.ascontiguousarray(state)), _ptr(q)))
+    return q
+
+
+# ------------------------------------------------------------------ level
+
+
+class OracleLevel:
+    """Boxes with one ghosted FAB each (AMReX MultiFab layout) on the CPU."""
+
+    def __init__(self, g, boxes, ng=3):
+        self.g = geom(g)
+        self.boxes = [(tuple(lo), tuple(hi)) for lo, hi in boxes]
+        self.ng = ng
+        self.n = [g.dom_hi[d] - g.dom_lo[d] + 1 for d in range(3)]
+        self.barr = (ctypes.c_int * (6 * len(boxes)))()
+        for b, (lo, hi) in enumerate(self.boxes):
+            for d in range(3):
+                self.barr[6 * b + d] = lo[d]
+                self.barr[6 * b + 3 + d] = hi[d]
+        self.fabs = []
+        for lo, hi in self.boxes:
+            m = [hi[d] - lo[d] + 1 + 2 * ng for d in range(3)]
+            self.fabs.append(np.zeros((5, m[2], m[1], m[0])))
+
+    def _pp(self):
+        arr = (_DP * len(self.fabs))()
+        for b, f in enumerate(self.fabs):
+            arr[b] = _ptr(f)
+        return arr
+
+    def set_state(self, u):
+        """u: (5, nz, ny, nx) domain array of valid cells."""
+        g = self.ng
+        for (lo, hi), f in zip(self.boxes, self.fabs):
+            f[:, g:f.shape[1] - g, g:f.shape[2] - g, g:f.shape[3] - g] = \
+                u[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+
+    def get_state(self):
+        g = self.ng
+        out = np.zeros((5, self.n[2], self.n[1], self.n[0]))
+        for (lo, hi), f in zip(self.boxes, self.fabs):
+            out[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = \
+                f[:, g:f.shape[1] - g, g:f.shape[2] - g, g:f.shape[3] - g]
+        return out
+
+    def fill_patch(self):
+        _check(load().orc_fill_patch(ctypes.byref(self.g), len(self.boxes), self.barr, self.ng, self._pp()))
+
+    def advance(self, c, s, dt, time=0.0, nthreads=1):
+        _check(load().orc_advance(ctypes.byref(self.g), ctypes.byref(closures(c)), ctypes.byref(scheme(s)),
+                                  len(self.boxes), self.barr, self.ng, self._pp(), time, dt, nthreads))
+
+    def est_dt(self, c, cfl):
+        return load().orc_est_dt(ctypes.byref(self.g), ctypes.byref(closures(c)), len(self.boxes), self.barr,
+                                 self.ng, self._pp(), cfl)
+
+    def totals(self):
+        out = np.zeros(5)
+        load().orc_totals(ctypes.byref(self.g), len(self.boxes), self.barr, self.ng, self._pp(), _ptr(out))
+        return out
+
+    def rhs(self, c, s):
+        """dSdt of every box after a FillPatch of the current state."""
+        self.fill_patch()
+        dx = [1.0 / self.g.dx[d] for d in range(3)]
+        out = np.zeros((5, self.n[2], self.n[1], self.n[0]))
+        for (lo, hi), f in zip(self.boxes, self.fabs):
+            out[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = box_rhs(c, s, lo, hi, self.ng, f, dx)
+        return out
+
+
+def rk_linear(order_rk, stages_rk, lam, dt, u):
+    u = np.array(u, dtype=np.float64)
+    _check(load().orc_rk_linear(order_rk, stages_rk, lam, dt, len(u), _ptr(u)))
+    return u
