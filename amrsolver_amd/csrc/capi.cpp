@@ -469,9 +469,23 @@ FusedConfig default_config(const amr_boxtab_s* bt) {
     mnx = std::min(mnx, B.hi[0] - B.lo[0] + 1);
     mny = std::min(mny, B.hi[1] - B.lo[1] + 1);
   }
-  FusedConfig c{16, 16, 32};
-  if (mnx < 16 || mny < 16) c = FusedConfig{8, 8, 32};
+  FusedConfig c{16, 16, 64};
+  if (mnx < 16 || mny < 16) c = FusedConfig{8, 8, 64};
   return c;
+}
+
+// The fused kernel needs every box to be a whole number of tiles in x and y
+// and a state array with at least three ghost cells.
+bool fused_fits(const amr_boxtab_s* bt, const FusedConfig& cfg) {
+  for (auto& B : bt->bt.h_boxes)
+    if ((B.hi[0] - B.lo[0] + 1) % cfg.tx != 0 || (B.hi[1] - B.lo[1] + 1) % cfg.ty != 0) return false;
+  return true;
+}
+
+FusedConfig choose_config(const amr_boxtab_s* bt, const amr_scheme_t& sch) {
+  auto tk = bt->ctx->tuned.find(scheme_key(sch));
+  if (tk != bt->ctx->tuned.end()) return tk->second;
+  return default_config(bt);
 }
 
 int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageCoef& sc,
@@ -481,10 +495,8 @@ int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sc
   for (int d = 0; d < 3; ++d) dxinv[d] = 1.0 / bt->geom.dx[d];
   hipStream_t st = bt->ctx->stream;
   const Fab* s1 = ds1 ? ds1->d : nullptr;
-  FusedConfig cfg = default_config(bt);
-  auto tk = bt->ctx->tuned.find(scheme_key(sch));
-  if (tk != bt->ctx->tuned.end()) cfg = tk->second;
-  if (fused_supported(sch, cfg.tx, cfg.ty)) {
+  FusedConfig cfg = choose_config(bt, sch);
+  if (cfg.tx > 0 && fused_supported(sch, cfg.tx, cfg.ty) && fused_fits(bt, cfg)) {
     TileTable* tt = nullptr;
     int rc = get_tiles(bt, cfg, &tt);
     if (rc) return rc;
@@ -802,10 +814,8 @@ int amr_advance_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_sche
       DescSet* dw = nullptr;
       if ((rc = get_desc(bt, bt->ws_fabs[w].data(), NGHOST, &dw)) || (rc = build_fill(bt, dw))) return rc;
     }
-    FusedConfig cfg = default_config(bt);
-    auto tk = bt->ctx->tuned.find(scheme_key(*sch));
-    if (tk != bt->ctx->tuned.end()) cfg = tk->second;
-    if (fused_supported(*sch, cfg.tx, cfg.ty)) {
+    FusedConfig cfg = choose_config(bt, *sch);
+    if (cfg.tx > 0 && fused_supported(*sch, cfg.tx, cfg.ty) && fused_fits(bt, cfg)) {
       TileTable* tt = nullptr;
       if ((rc = get_tiles(bt, cfg, &tt))) return rc;
     } else if ((rc = ensure_unfused(bt, *sch))) {
@@ -831,6 +841,84 @@ int amr_advance_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_sche
     bt->graphs.push_back(std::move(g));
   }
   HIPCHK(hipGraphLaunch(ge->exec, st));
+  return AMR_OK;
+}
+
+int amr_set_tiling(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, int tz) {
+  if (!ctx || !sch) return fail(AMR_ERR_ARG, "null argument");
+  std::string key = scheme_key(*sch);
+  if (tx == 0 && ty == 0 && tz == 0) {
+    ctx->tuned.erase(key);
+    return AMR_OK;
+  }
+  if (tx == -1) {
+    ctx->tuned[key] = FusedConfig{-1, 0, 0};
+    return AMR_OK;
+  }
+  if (tz <= 0) return fail(AMR_ERR_ARG, "tz must be positive");
+  if (!fused_supported(*sch, tx, ty))
+    return fail(AMR_ERR_ARG, "no fused kernel compiled for this scheme and tile shape");
+  ctx->tuned[key] = FusedConfig{tx, ty, tz};
+  return AMR_OK;
+}
+
+int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
+                 int32_t tiling[3]) {
+  if (!bt || !cls || !sch || !state || !tiling) return fail(AMR_ERR_ARG, "null argument");
+  int rc;
+  if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls))) return rc;
+  std::string key = scheme_key(*sch);
+  int maxz = 0;
+  for (auto& B : bt->bt.h_boxes) maxz = std::max(maxz, B.hi[2] - B.lo[2] + 1);
+  std::vector<FusedConfig> cand;
+  const int shapes[3][2] = {{16, 16}, {32, 8}, {8, 8}};
+  const int tzs[4] = {16, 32, 64, 128};
+  for (auto& sh : shapes)
+    for (int tz : tzs) {
+      FusedConfig c{sh[0], sh[1], tz};
+      if (tz > 2 * maxz && tz != 16) continue;
+      if (fused_supported(*sch, c.tx, c.ty) && fused_fits(bt, c)) cand.push_back(c);
+    }
+  if (cand.empty()) {
+    bt->ctx->tuned[key] = FusedConfig{-1, 0, 0};
+    tiling[0] = -1;
+    tiling[1] = tiling[2] = 0;
+    return AMR_OK;
+  }
+  if ((rc = ensure_ws(bt, state))) return rc;
+  DescSet *din = nullptr, *dout = nullptr;
+  if ((rc = get_desc(bt, state, NGHOST, &din))) return rc;
+  if ((rc = get_desc(bt, bt->ws_fabs[0].data(), 0, &dout))) return rc;
+  hipStream_t st = bt->ctx->stream;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 1e30f;
+  FusedConfig bestc = cand[0];
+  StageCoef sc{2, 0, 0, 0, 0};
+  for (auto& cfg : cand) {
+    bt->ctx->tuned[key] = cfg;
+    float tbest = 1e30f;
+    for (int rep = 0; rep < 4; ++rep) {
+      HIPCHK(hipEventRecord(e0, st));
+      if ((rc = do_stage(bt, *cls, *sch, sc, nullptr, din, dout))) return rc;
+      HIPCHK(hipEventRecord(e1, st));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (rep > 0) tbest = std::min(tbest, ms);
+    }
+    if (tbest < best) {
+      best = tbest;
+      bestc = cfg;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  bt->ctx->tuned[key] = bestc;
+  tiling[0] = bestc.tx;
+  tiling[1] = bestc.ty;
+  tiling[2] = bestc.tz;
   return AMR_OK;
 }
 

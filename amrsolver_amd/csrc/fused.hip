@@ -1,14 +1,439 @@
-// Fused KEEP + viscous stage kernel (placeholder until the tiled kernel lands).
+// Fused KEEP (order 2/4) + viscous RHS and SSPRK stage update.
+//
+// One workgroup owns a TX x TY column tile of one box and marches up a
+// run of planes (2.5D blocking).  Per plane it
+//   1. converts the next plane's conserved state (tile + halo) to primitives
+//      straight into LDS (cons2prim, CNS_hydro_K.h:15-34) -- primitives never
+//      touch HBM;
+//   2. computes cell-centred viscous fluxes (ViscousFluxes, CNS_hydro_K.h:108-172)
+//      for the tile plus a one-cell ring, x/y parts into LDS, z part in
+//      registers;
+//   3. computes every x and y face flux of the plane once (KEEP,
+//      CentralKEEP.h:40-145, plus the viscous face average,
+//      CNS_hydro_K.h:254-261) into LDS;
+//   4. computes the z face below the plane from a per-thread register queue
+//      of its own column and finalises the plane below: flux difference
+//      (CNS_advance.cpp:384-390) and stage combination (CNS::advance
+//      LinComb/Saxpy/Xpay) written straight to the output.
+// HBM traffic per cell and stage: the input state (plus tile halos, mostly
+// served from L2), U^n when the stage needs it, and the output.
 #include "kernels.h"
 
 namespace amr {
 
-bool fused_supported(const amr_scheme_t&, int, int) { return false; }
+namespace {
 
-hipError_t launch_fused_stage(const amr_scheme_t&, const FusedConfig&, const TileDev*, int, const BoxDev*,
-                              const Closures&, const double*, const StageCoef&, const Fab*, const Fab*,
-                              const Fab*, hipStream_t) {
-  return hipErrorInvalidValue;
+struct VDer {
+  double ux, uy, uz, T;
+  double dudx, dvdx, dwdx, dTdx, dudy, dvdy, dwdy, dTdy, dudz, dvdz, dwdz, dTdz;
+};
+
+// ViscousFluxes (CNS_hydro_K.h:133-171), components UMX..UET (URHO is 0)
+template <bool WX, bool WY, bool WZ>
+__device__ __forceinline__ void visc_flux(const VDer& d, const Closures& c, double fx[4], double fy[4],
+                                          double fz[4]) {
+  double div = d.dudx + d.dvdy + d.dwdz;
+  double mu = visc(c, d.T);
+  double lambda = cond(c, d.T, mu);
+  double r1_3 = 1.0 / 3.0;
+  double tauxx = 2.0 * mu * (d.dudx - r1_3 * div);
+  double tauxy = mu * (d.dudy + d.dvdx);
+  double tauxz = mu * (d.dudz + d.dwdx);
+  double tauyy = 2.0 * mu * (d.dvdy - r1_3 * div);
+  double tauyz = mu * (d.dvdz + d.dwdy);
+  double tauzz = 2.0 * mu * (d.dwdz - r1_3 * div);
+  if (WX) {
+    fx[0] = -tauxx;
+    fx[1] = -tauxy;
+    fx[2] = -tauxz;
+    fx[3] = -lambda * d.dTdx - tauxx * d.ux - tauxy * d.uy - tauxz * d.uz;
+  }
+  if (WY) {
+    fy[0] = -tauxy;
+    fy[1] = -tauyy;
+    fy[2] = -tauyz;
+    fy[3] = -lambda * d.dTdy - tauxy * d.ux - tauyy * d.uy - tauyz * d.uz;
+  }
+  if (WZ) {
+    fz[0] = -tauxz;
+    fz[1] = -tauyz;
+    fz[2] = -tauzz;
+    fz[3] = -lambda * d.dTdz - tauxz * d.ux - tauyz * d.uy - tauzz * d.uz;
+  }
+}
+
+template <int H, bool VISC, int TX, int TY>
+struct Cfg {
+  static constexpr int HX = (VISC && H < 2) ? 2 : H;  // xy halo
+  static constexpr int KB = (VISC && H < 2) ? 2 : H;  // planes below k0 loaded
+  static constexpr int QN = (H > 1 ? H : 1) + 2;      // own-column queue: c-QLO .. c+1
+  static constexpr int NT = TX * TY;
+  static constexpr int RX = TX + 2 * HX, RY = TY + 2 * HX;
+  static constexpr int NREG = RX * RY;
+  static constexpr int PL = NREG;  // LDS plane stride per primitive
+  static constexpr int NLOAD = (NREG + NT - 1) / NT;
+  static constexpr int NRING = 2 * TX + 2 * TY;
+  // first thread of the extra y faces: the second wave when there is one
+  static constexpr int XS = NT >= 128 ? 64 : TY;
+};
+
+// KEEP face flux in direction NRM at the face below LDS cell `idx`; cells
+// idx + m*st, m = -H..H-1 (CentralKEEP.h:58-142 accumulation order).
+template <int H, int NRM, int PL>
+__device__ __forceinline__ void keep_face(const double* sq, int idx, int st, const double co[3], double cv,
+                                          double f[5]) {
+#pragma unroll
+  for (int n = 0; n < 5; ++n) f[n] = 0.0;
+#pragma unroll
+  for (int l = 1; l <= H; ++l) {
+#pragma unroll
+    for (int m = 0; m <= l - 1; ++m) {
+      const int a = idx + m * st, b = idx + (m - l) * st;
+      keep_pair<NRM>(co[l - 1], sq[QRHO * PL + a], sq[QU * PL + a], sq[QV * PL + a], sq[QW * PL + a],
+                     cv * sq[QT * PL + a], sq[QPRES * PL + a], sq[QRHO * PL + b], sq[QU * PL + b],
+                     sq[QV * PL + b], sq[QW * PL + b], cv * sq[QT * PL + b], sq[QPRES * PL + b], f);
+    }
+  }
+}
+
+template <int H, bool VISC, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ tiles, Closures c, double co0,
+                                                  double co1, double co2, double dxi0, double dxi1, double dxi2,
+                                                  StageCoef sc, const Fab* __restrict__ s1f,
+                                                  const Fab* __restrict__ sinf, const Fab* __restrict__ outf) {
+  using C = Cfg<H, VISC, TX, TY>;
+  constexpr int RX = C::RX, HX = C::HX, PL = C::PL, QN = C::QN, NT = C::NT;
+  __shared__ double s_q[2][NPRIM * PL];
+  __shared__ double s_px[VISC ? 4 : 1][TY][TX + 2];
+  __shared__ double s_py[VISC ? 4 : 1][TY + 2][TX];
+  __shared__ double s_fx[5][TY][TX + 1];
+  __shared__ double s_fy[5][TY + 1][TX];
+
+  const TileDev T = tiles[blockIdx.x];
+  const Fab S = sinf[T.box];
+  const Fab O = outf[T.box];
+  const bool use_s1 = sc.mode != 2 && sc.a != 0.0;
+  Fab S1;
+  if (use_s1) S1 = s1f[T.box];
+  const int tid = threadIdx.x;
+  const int ti = tid % TX, tj = tid / TX;
+  const int gi = T.i0 + ti, gj = T.j0 + tj;
+  const int k0 = T.k0, nz = T.nz;
+  const int kfirst = k0 - C::KB;
+  const int klast = k0 + nz + 1;
+  const double co[3] = {co0, co1, co2};
+  const double cv = c.cv;
+  const int own = (tj + HX) * RX + (ti + HX);  // LDS index of the own cell
+
+  // ring cell of this thread (viscous x-flux at i=-1,TX or y-flux at j=-1,TY)
+  int ring_idx = 0, ring_kind = -1, ring_a = 0, ring_b = 0;
+  if (VISC && tid < C::NRING) {
+    int r = tid;
+    if (r < TY) { ring_kind = 0; ring_a = -1; ring_b = r; }
+    else if (r < 2 * TY) { ring_kind = 0; ring_a = TX; ring_b = r - TY; }
+    else if (r < 2 * TY + TX) { ring_kind = 1; ring_a = r - 2 * TY; ring_b = -1; }
+    else { ring_kind = 1; ring_a = r - 2 * TY - TX; ring_b = TY; }
+    ring_idx = (ring_b + HX) * RX + (ring_a + HX);
+  }
+
+  double pre[C::NLOAD][5];
+  auto load_plane = [&](int p) {
+#pragma unroll
+    for (int s = 0; s < C::NLOAD; ++s) {
+      int cell = tid + s * NT;
+      if (cell < C::NREG) {
+        int ii = cell % RX, jj = cell / RX;
+        int64_t ix = S.idx(T.i0 - HX + ii, T.j0 - HX + jj, p);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) pre[s][n] = S.p[ix + n * S.ns];
+      }
+    }
+  };
+  auto store_plane = [&](int p) {
+    double* sq = s_q[p & 1];
+#pragma unroll
+    for (int s = 0; s < C::NLOAD; ++s) {
+      int cell = tid + s * NT;
+      if (cell < C::NREG) {
+        double q[6];
+        cons2prim(pre[s][0], pre[s][1], pre[s][2], pre[s][3], pre[s][4], c, q);
+#pragma unroll
+        for (int n = 0; n < NPRIM; ++n) sq[n * PL + cell] = q[n];
+      }
+    }
+  };
+
+  // own-column primitive queue: q[QN-1] is the newest plane
+  double qq[QN][NPRIM];
+  // ring queue (u,v,w,T at c-1, c, c+1)
+  double rq[3][4];
+  double pfz_prev[4] = {0, 0, 0, 0}, pfz_cur[4] = {0, 0, 0, 0};
+  double fz_prev[5] = {0, 0, 0, 0, 0};
+  double xy_prev[5] = {0, 0, 0, 0, 0};
+
+  auto push = [&](int p) {
+    const double* sq = s_q[p & 1];
+#pragma unroll
+    for (int t = 0; t < QN - 1; ++t)
+#pragma unroll
+      for (int n = 0; n < NPRIM; ++n) qq[t][n] = qq[t + 1][n];
+#pragma unroll
+    for (int n = 0; n < NPRIM; ++n) qq[QN - 1][n] = sq[n * PL + own];
+    if (VISC) {
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) rq[t][n] = rq[t + 1][n];
+      if (ring_kind >= 0)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) rq[2][n] = sq[(QU + n) * PL + ring_idx];
+    }
+  };
+
+  // prologue: plane kfirst into LDS and the queues
+  load_plane(kfirst);
+  store_plane(kfirst);
+  load_plane(kfirst + 1);
+  __syncthreads();
+  push(kfirst);
+
+  for (int cpl = kfirst; cpl <= k0 + nz; ++cpl) {
+    const int pn = cpl + 1;
+    // (a) next plane into LDS, prefetch the one after
+    if (pn <= klast) {
+      store_plane(pn);
+      if (pn + 1 <= klast) load_plane(pn + 1);
+    }
+    // epilogue operands of the plane finalised in this iteration
+    const bool fin = cpl >= k0 + 1;
+    double sv[5], s1v[5];
+    if (fin) {
+      int64_t ix = S.idx(gi, gj, cpl - 1);
+#pragma unroll
+      for (int n = 0; n < 5; ++n) sv[n] = S.p[ix + n * S.ns];
+      if (use_s1) {
+        int64_t i1 = S1.idx(gi, gj, cpl - 1);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) s1v[n] = S1.p[i1 + n * S1.ns];
+      } else {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) s1v[n] = 0.0;
+      }
+    }
+    __syncthreads();  // (b)
+    if (pn <= klast) push(pn);
+    if (cpl < k0 - 1) continue;  // uniform: only loading planes below the tile
+    if (!VISC && cpl < k0) continue;
+
+    const bool inner = cpl >= k0 && cpl < k0 + nz;  // x/y work on this plane
+    const double* sq = s_q[cpl & 1];
+    // queue slots: plane cpl-1 -> QN-3, cpl -> QN-2, cpl+1 -> QN-1
+    if (VISC) {
+      // own cell
+      {
+        VDer d;
+        d.ux = qq[QN - 2][QU];
+        d.uy = qq[QN - 2][QV];
+        d.uz = qq[QN - 2][QW];
+        d.T = qq[QN - 2][QT];
+        const int e = own + 1, w = own - 1, nn = own + RX, s = own - RX;
+        d.dudx = (sq[QU * PL + e] - sq[QU * PL + w]) * 0.5 * dxi0;
+        d.dvdx = (sq[QV * PL + e] - sq[QV * PL + w]) * 0.5 * dxi0;
+        d.dwdx = (sq[QW * PL + e] - sq[QW * PL + w]) * 0.5 * dxi0;
+        d.dTdx = (sq[QT * PL + e] - sq[QT * PL + w]) * 0.5 * dxi0;
+        d.dudy = (sq[QU * PL + nn] - sq[QU * PL + s]) * 0.5 * dxi1;
+        d.dvdy = (sq[QV * PL + nn] - sq[QV * PL + s]) * 0.5 * dxi1;
+        d.dwdy = (sq[QW * PL + nn] - sq[QW * PL + s]) * 0.5 * dxi1;
+        d.dTdy = (sq[QT * PL + nn] - sq[QT * PL + s]) * 0.5 * dxi1;
+        d.dudz = (qq[QN - 1][QU] - qq[QN - 3][QU]) * 0.5 * dxi2;
+        d.dvdz = (qq[QN - 1][QV] - qq[QN - 3][QV]) * 0.5 * dxi2;
+        d.dwdz = (qq[QN - 1][QW] - qq[QN - 3][QW]) * 0.5 * dxi2;
+        d.dTdz = (qq[QN - 1][QT] - qq[QN - 3][QT]) * 0.5 * dxi2;
+        double fx[4], fy[4];
+        if (inner) {
+          visc_flux<true, true, true>(d, c, fx, fy, pfz_cur);
+#pragma unroll
+          for (int n = 0; n < 4; ++n) {
+            s_px[n][tj][ti + 1] = fx[n];
+            s_py[n][tj + 1][ti] = fy[n];
+          }
+        } else {
+          visc_flux<false, false, true>(d, c, fx, fy, pfz_cur);
+        }
+      }
+      // ring cell
+      if (inner && ring_kind >= 0) {
+        VDer d;
+        const int r = ring_idx;
+        d.ux = sq[QU * PL + r];
+        d.uy = sq[QV * PL + r];
+        d.uz = sq[QW * PL + r];
+        d.T = sq[QT * PL + r];
+        const int e = r + 1, w = r - 1, nn = r + RX, s = r - RX;
+        d.dudx = (sq[QU * PL + e] - sq[QU * PL + w]) * 0.5 * dxi0;
+        d.dvdx = (sq[QV * PL + e] - sq[QV * PL + w]) * 0.5 * dxi0;
+        d.dwdx = (sq[QW * PL + e] - sq[QW * PL + w]) * 0.5 * dxi0;
+        d.dTdx = (sq[QT * PL + e] - sq[QT * PL + w]) * 0.5 * dxi0;
+        d.dudy = (sq[QU * PL + nn] - sq[QU * PL + s]) * 0.5 * dxi1;
+        d.dvdy = (sq[QV * PL + nn] - sq[QV * PL + s]) * 0.5 * dxi1;
+        d.dwdy = (sq[QW * PL + nn] - sq[QW * PL + s]) * 0.5 * dxi1;
+        d.dTdy = (sq[QT * PL + nn] - sq[QT * PL + s]) * 0.5 * dxi1;
+        d.dudz = (rq[2][0] - rq[0][0]) * 0.5 * dxi2;
+        d.dvdz = (rq[2][1] - rq[0][1]) * 0.5 * dxi2;
+        d.dwdz = (rq[2][2] - rq[0][2]) * 0.5 * dxi2;
+        d.dTdz = (rq[2][3] - rq[0][3]) * 0.5 * dxi2;
+        double fx[4], fy[4], fz[4];
+        if (ring_kind == 0) {
+          visc_flux<true, false, false>(d, c, fx, fy, fz);
+#pragma unroll
+          for (int n = 0; n < 4; ++n) s_px[n][ring_b][ring_a + 1] = fx[n];
+        } else {
+          visc_flux<false, true, false>(d, c, fx, fy, fz);
+#pragma unroll
+          for (int n = 0; n < 4; ++n) s_py[n][ring_b + 1][ring_a] = fy[n];
+        }
+      }
+      // (c) viscous x/y fluxes visible; below the tile this barrier instead
+      // keeps the next store from overwriting the plane just read
+      __syncthreads();
+    }
+
+    if (inner) {
+      // x and y faces below the own cell
+      double f[5];
+      keep_face<H, 0, PL>(sq, own, 1, co, cv, f);
+      if (VISC) {
+        f[0] += 0.5 * (0.0 + 0.0);
+#pragma unroll
+        for (int n = 0; n < 4; ++n) f[n + 1] += 0.5 * (s_px[n][tj][ti] + s_px[n][tj][ti + 1]);
+      }
+#pragma unroll
+      for (int n = 0; n < 5; ++n) s_fx[n][tj][ti] = f[n];
+      keep_face<H, 1, PL>(sq, own, RX, co, cv, f);
+      if (VISC) {
+        f[0] += 0.5 * (0.0 + 0.0);
+#pragma unroll
+        for (int n = 0; n < 4; ++n) f[n + 1] += 0.5 * (s_py[n][tj][ti] + s_py[n][tj + 1][ti]);
+      }
+#pragma unroll
+      for (int n = 0; n < 5; ++n) s_fy[n][tj][ti] = f[n];
+      // the faces at i = TX (wave 0) and j = TY (wave 1)
+      if (tid < TY) {
+        const int jj = tid;
+        keep_face<H, 0, PL>(sq, (jj + HX) * RX + (TX + HX), 1, co, cv, f);
+        if (VISC) {
+          f[0] += 0.5 * (0.0 + 0.0);
+#pragma unroll
+          for (int n = 0; n < 4; ++n) f[n + 1] += 0.5 * (s_px[n][jj][TX] + s_px[n][jj][TX + 1]);
+        }
+#pragma unroll
+        for (int n = 0; n < 5; ++n) s_fx[n][jj][TX] = f[n];
+      } else if (tid >= C::XS && tid < C::XS + TX) {
+        const int ii = tid - C::XS;
+        keep_face<H, 1, PL>(sq, (TY + HX) * RX + (ii + HX), RX, co, cv, f);
+        if (VISC) {
+          f[0] += 0.5 * (0.0 + 0.0);
+#pragma unroll
+          for (int n = 0; n < 4; ++n) f[n + 1] += 0.5 * (s_py[n][TY][ii] + s_py[n][TY + 1][ii]);
+        }
+#pragma unroll
+        for (int n = 0; n < 5; ++n) s_fy[n][TY][ii] = f[n];
+      }
+      __syncthreads();  // (d) faces visible
+    }
+
+    double xy[5];
+    if (inner) {
+#pragma unroll
+      for (int n = 0; n < 5; ++n)
+        xy[n] = dxi0 * (s_fx[n][tj][ti] - s_fx[n][tj][ti + 1]) + dxi1 * (s_fy[n][tj][ti] - s_fy[n][tj + 1][ti]);
+    }
+    if (cpl >= k0) {
+      // z face at cpl - 1/2 from the own-column queue: plane cpl+o sits in
+      // slot QN-2+o
+      double fz[5];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) fz[n] = 0.0;
+#pragma unroll
+      for (int l = 1; l <= H; ++l) {
+#pragma unroll
+        for (int m = 0; m <= l - 1; ++m) {
+          const int a = QN - 2 + m, b = QN - 2 + m - l;
+          keep_pair<2>(co[l - 1], qq[a][QRHO], qq[a][QU], qq[a][QV], qq[a][QW], cv * qq[a][QT], qq[a][QPRES],
+                       qq[b][QRHO], qq[b][QU], qq[b][QV], qq[b][QW], cv * qq[b][QT], qq[b][QPRES], fz);
+        }
+      }
+      if (VISC) {
+        fz[0] += 0.5 * (0.0 + 0.0);
+#pragma unroll
+        for (int n = 0; n < 4; ++n) fz[n + 1] += 0.5 * (pfz_prev[n] + pfz_cur[n]);
+      }
+      if (fin) {
+        const int kk = cpl - 1;
+        int64_t ox = O.idx(gi, gj, kk);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+          double L = xy_prev[n] + dxi2 * (fz_prev[n] - fz[n]);
+          O.p[ox + n * O.ns] = stage_combine(sc, s1v[n], sv[n], L);
+        }
+      }
+#pragma unroll
+      for (int n = 0; n < 5; ++n) fz_prev[n] = fz[n];
+    }
+    if (inner) {
+#pragma unroll
+      for (int n = 0; n < 5; ++n) xy_prev[n] = xy[n];
+    }
+    if (VISC) {
+#pragma unroll
+      for (int n = 0; n < 4; ++n) pfz_prev[n] = pfz_cur[n];
+    }
+  }
+}
+
+template <int H, bool VISC, int TX, int TY>
+void launch_t(const TileDev* tiles, int ntiles, const Closures& c, const double co[3], const double dxi[3],
+              const StageCoef& sc, const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
+  hipLaunchKernelGGL((k_fused<H, VISC, TX, TY>), dim3(ntiles), dim3(TX * TY), 0, st, tiles, c, co[0], co[1], co[2],
+                     dxi[0], dxi[1], dxi[2], sc, s1, sin, out);
+}
+
+template <int TX, int TY>
+bool launch_shape(const amr_scheme_t& s, const TileDev* tiles, int ntiles, const Closures& c, const double co[3],
+                  const double dxi[3], const StageCoef& sc, const Fab* s1, const Fab* sin, const Fab* out,
+                  hipStream_t st) {
+  const bool v = s.rhs_visc != 0;
+  if (s.order_keep == 4) {
+    if (v) launch_t<2, true, TX, TY>(tiles, ntiles, c, co, dxi, sc, s1, sin, out, st);
+    else launch_t<2, false, TX, TY>(tiles, ntiles, c, co, dxi, sc, s1, sin, out, st);
+  } else if (s.order_keep == 2) {
+    if (v) launch_t<1, true, TX, TY>(tiles, ntiles, c, co, dxi, sc, s1, sin, out, st);
+    else launch_t<1, false, TX, TY>(tiles, ntiles, c, co, dxi, sc, s1, sin, out, st);
+  } else {
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+bool fused_supported(const amr_scheme_t& s, int tx, int ty) {
+  if (!s.rhs_euler || s.flux_euler != 1) return false;
+  if (!(s.order_keep == 2 || s.order_keep == 4)) return false;
+  return (tx == 16 && ty == 16) || (tx == 8 && ty == 8) || (tx == 32 && ty == 8);
+}
+
+hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const TileDev* tiles, int ntiles,
+                              const BoxDev*, const Closures& c, const double dxinv[3], const StageCoef& sc,
+                              const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
+  double co[3];
+  keep_coefficients(s.order_keep, co);
+  bool ok = false;
+  if (cfg.tx == 16 && cfg.ty == 16) ok = launch_shape<16, 16>(s, tiles, ntiles, c, co, dxinv, sc, s1, sin, out, st);
+  else if (cfg.tx == 8 && cfg.ty == 8) ok = launch_shape<8, 8>(s, tiles, ntiles, c, co, dxinv, sc, s1, sin, out, st);
+  else if (cfg.tx == 32 && cfg.ty == 8) ok = launch_shape<32, 8>(s, tiles, ntiles, c, co, dxinv, sc, s1, sin, out, st);
+  if (!ok) return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 }  // namespace amr

@@ -33,7 +33,13 @@ class Context:
             raise capi.AmrError("no GPU visible: the HIP path needs a device")
         torch.cuda.set_device(device)
         self.device = device
-        self.stream = stream if stream is not None else torch.cuda.current_stream(device)
+        if stream is None:
+            stream = torch.cuda.current_stream(device)
+            if stream.cuda_stream == 0:
+                # one explicit stream shared by torch's copies and the library
+                stream = torch.cuda.Stream(device=device)
+                torch.cuda.set_stream(stream)
+        self.stream = stream
         h = ctypes.c_void_p()
         call("amr_ctx_create", device, ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h))
         self.h = h
@@ -192,6 +198,16 @@ class Level:
     def advance_graph(self, S, dt, nsteps):
         call("amr_advance_graph", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, dt,
              nsteps)
+
+    def set_tiling(self, tx, ty=0, tz=0):
+        """Force the fused-kernel tiling for this level's scheme (0,0,0 = auto,
+        -1 = pass-structured kernels)."""
+        call("amr_set_tiling", self.ctx.h, ctypes.byref(self.scheme), tx, ty, tz)
+
+    def autotune(self, S):
+        out = (ctypes.c_int32 * 3)()
+        call("amr_autotune", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, out)
+        return tuple(out)
 
     def est_time_step(self, S, cfl):
         out = ctypes.c_double()
