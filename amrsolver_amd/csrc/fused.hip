@@ -23,6 +23,12 @@ namespace amr {
 
 namespace {
 
+// Global-address-space views: loads and stores then compile to global_*
+// instructions (flat ones would share the LDS wait counter and serialise
+// the plane prefetch against every LDS wait).
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ gdouble* gptr(double* p) { return (gdouble*)p; }
+
 struct VDer {
   double ux, uy, uz, T;
   double dudx, dvdx, dwdx, dTdx, dudy, dvdy, dwdy, dTdy, dudz, dvdz, dwdz, dTdz;
@@ -73,8 +79,10 @@ struct Cfg {
   static constexpr int PL = NREG;  // LDS plane stride per primitive
   static constexpr int NLOAD = (NREG + NT - 1) / NT;
   static constexpr int NRING = 2 * TX + 2 * TY;
-  // first thread of the extra y faces: the second wave when there is one
-  static constexpr int XS = NT >= 128 ? 64 : TY;
+  // threads of the extra x faces (i = TX) and y faces (j = TY): waves of
+  // their own (the ring fluxes occupy wave 0) where the workgroup has them
+  static constexpr int XB = NT >= 128 ? 64 * ((NRING + 63) / 64) : (VISC ? NRING : 0);
+  static constexpr int YB = NT >= XB + 128 ? XB + 64 : XB + TY;
 };
 
 // KEEP face flux in direction NRM at the face below LDS cell `idx`; cells
@@ -124,6 +132,9 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
   const double co[3] = {co0, co1, co2};
   const double cv = c.cv;
   const int own = (tj + HX) * RX + (ti + HX);  // LDS index of the own cell
+  // (a-b)*0.5*dxinv is evaluated as (a-b)*(0.5*dxinv): exact in IEEE
+  // arithmetic for normal numbers (scaling by 0.5 is exact)
+  const double hx = 0.5 * dxi0, hy = 0.5 * dxi1, hz = 0.5 * dxi2;
 
   // ring cell of this thread (viscous x-flux at i=-1,TX or y-flux at j=-1,TY)
   int ring_idx = 0, ring_kind = -1, ring_a = 0, ring_b = 0;
@@ -136,16 +147,26 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
     ring_idx = (ring_b + HX) * RX + (ring_a + HX);
   }
 
+  // per-lane element offsets of the loaded plane cells; plane bases are
+  // wave-uniform
+  const gdouble* sbase = gptr(S.p) + S.off;
+  int loff[C::NLOAD];
+#pragma unroll
+  for (int s = 0; s < C::NLOAD; ++s) {
+    int cell = tid + s * NT;
+    int ii = cell % RX, jj = cell / RX;
+    loff[s] = (T.i0 - HX + ii) + (T.j0 - HX + jj) * int(S.js);
+  }
+  const int soff = gi + gj * int(S.js);
   double pre[C::NLOAD][5];
   auto load_plane = [&](int p) {
+    const gdouble* pb = sbase + int64_t(p) * S.ks;
 #pragma unroll
     for (int s = 0; s < C::NLOAD; ++s) {
       int cell = tid + s * NT;
       if (cell < C::NREG) {
-        int ii = cell % RX, jj = cell / RX;
-        int64_t ix = S.idx(T.i0 - HX + ii, T.j0 - HX + jj, p);
 #pragma unroll
-        for (int n = 0; n < 5; ++n) pre[s][n] = S.p[ix + n * S.ns];
+        for (int n = 0; n < 5; ++n) pre[s][n] = pb[n * S.ns + loff[s]];
       }
     }
   };
@@ -208,13 +229,13 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
     const bool fin = cpl >= k0 + 1;
     double sv[5], s1v[5];
     if (fin) {
-      int64_t ix = S.idx(gi, gj, cpl - 1);
+      const gdouble* pb = sbase + int64_t(cpl - 1) * S.ks;
 #pragma unroll
-      for (int n = 0; n < 5; ++n) sv[n] = S.p[ix + n * S.ns];
+      for (int n = 0; n < 5; ++n) sv[n] = pb[n * S.ns + soff];
       if (use_s1) {
-        int64_t i1 = S1.idx(gi, gj, cpl - 1);
+        const gdouble* p1 = gptr(S1.p) + S1.idx(gi, gj, cpl - 1);
 #pragma unroll
-        for (int n = 0; n < 5; ++n) s1v[n] = S1.p[i1 + n * S1.ns];
+        for (int n = 0; n < 5; ++n) s1v[n] = p1[n * S1.ns];
       } else {
 #pragma unroll
         for (int n = 0; n < 5; ++n) s1v[n] = 0.0;
@@ -237,18 +258,18 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
         d.uz = qq[QN - 2][QW];
         d.T = qq[QN - 2][QT];
         const int e = own + 1, w = own - 1, nn = own + RX, s = own - RX;
-        d.dudx = (sq[QU * PL + e] - sq[QU * PL + w]) * 0.5 * dxi0;
-        d.dvdx = (sq[QV * PL + e] - sq[QV * PL + w]) * 0.5 * dxi0;
-        d.dwdx = (sq[QW * PL + e] - sq[QW * PL + w]) * 0.5 * dxi0;
-        d.dTdx = (sq[QT * PL + e] - sq[QT * PL + w]) * 0.5 * dxi0;
-        d.dudy = (sq[QU * PL + nn] - sq[QU * PL + s]) * 0.5 * dxi1;
-        d.dvdy = (sq[QV * PL + nn] - sq[QV * PL + s]) * 0.5 * dxi1;
-        d.dwdy = (sq[QW * PL + nn] - sq[QW * PL + s]) * 0.5 * dxi1;
-        d.dTdy = (sq[QT * PL + nn] - sq[QT * PL + s]) * 0.5 * dxi1;
-        d.dudz = (qq[QN - 1][QU] - qq[QN - 3][QU]) * 0.5 * dxi2;
-        d.dvdz = (qq[QN - 1][QV] - qq[QN - 3][QV]) * 0.5 * dxi2;
-        d.dwdz = (qq[QN - 1][QW] - qq[QN - 3][QW]) * 0.5 * dxi2;
-        d.dTdz = (qq[QN - 1][QT] - qq[QN - 3][QT]) * 0.5 * dxi2;
+        d.dudx = (sq[QU * PL + e] - sq[QU * PL + w]) * hx;
+        d.dvdx = (sq[QV * PL + e] - sq[QV * PL + w]) * hx;
+        d.dwdx = (sq[QW * PL + e] - sq[QW * PL + w]) * hx;
+        d.dTdx = (sq[QT * PL + e] - sq[QT * PL + w]) * hx;
+        d.dudy = (sq[QU * PL + nn] - sq[QU * PL + s]) * hy;
+        d.dvdy = (sq[QV * PL + nn] - sq[QV * PL + s]) * hy;
+        d.dwdy = (sq[QW * PL + nn] - sq[QW * PL + s]) * hy;
+        d.dTdy = (sq[QT * PL + nn] - sq[QT * PL + s]) * hy;
+        d.dudz = (qq[QN - 1][QU] - qq[QN - 3][QU]) * hz;
+        d.dvdz = (qq[QN - 1][QV] - qq[QN - 3][QV]) * hz;
+        d.dwdz = (qq[QN - 1][QW] - qq[QN - 3][QW]) * hz;
+        d.dTdz = (qq[QN - 1][QT] - qq[QN - 3][QT]) * hz;
         double fx[4], fy[4];
         if (inner) {
           visc_flux<true, true, true>(d, c, fx, fy, pfz_cur);
@@ -270,18 +291,18 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
         d.uz = sq[QW * PL + r];
         d.T = sq[QT * PL + r];
         const int e = r + 1, w = r - 1, nn = r + RX, s = r - RX;
-        d.dudx = (sq[QU * PL + e] - sq[QU * PL + w]) * 0.5 * dxi0;
-        d.dvdx = (sq[QV * PL + e] - sq[QV * PL + w]) * 0.5 * dxi0;
-        d.dwdx = (sq[QW * PL + e] - sq[QW * PL + w]) * 0.5 * dxi0;
-        d.dTdx = (sq[QT * PL + e] - sq[QT * PL + w]) * 0.5 * dxi0;
-        d.dudy = (sq[QU * PL + nn] - sq[QU * PL + s]) * 0.5 * dxi1;
-        d.dvdy = (sq[QV * PL + nn] - sq[QV * PL + s]) * 0.5 * dxi1;
-        d.dwdy = (sq[QW * PL + nn] - sq[QW * PL + s]) * 0.5 * dxi1;
-        d.dTdy = (sq[QT * PL + nn] - sq[QT * PL + s]) * 0.5 * dxi1;
-        d.dudz = (rq[2][0] - rq[0][0]) * 0.5 * dxi2;
-        d.dvdz = (rq[2][1] - rq[0][1]) * 0.5 * dxi2;
-        d.dwdz = (rq[2][2] - rq[0][2]) * 0.5 * dxi2;
-        d.dTdz = (rq[2][3] - rq[0][3]) * 0.5 * dxi2;
+        d.dudx = (sq[QU * PL + e] - sq[QU * PL + w]) * hx;
+        d.dvdx = (sq[QV * PL + e] - sq[QV * PL + w]) * hx;
+        d.dwdx = (sq[QW * PL + e] - sq[QW * PL + w]) * hx;
+        d.dTdx = (sq[QT * PL + e] - sq[QT * PL + w]) * hx;
+        d.dudy = (sq[QU * PL + nn] - sq[QU * PL + s]) * hy;
+        d.dvdy = (sq[QV * PL + nn] - sq[QV * PL + s]) * hy;
+        d.dwdy = (sq[QW * PL + nn] - sq[QW * PL + s]) * hy;
+        d.dTdy = (sq[QT * PL + nn] - sq[QT * PL + s]) * hy;
+        d.dudz = (rq[2][0] - rq[0][0]) * hz;
+        d.dvdz = (rq[2][1] - rq[0][1]) * hz;
+        d.dwdz = (rq[2][2] - rq[0][2]) * hz;
+        d.dTdz = (rq[2][3] - rq[0][3]) * hz;
         double fx[4], fy[4], fz[4];
         if (ring_kind == 0) {
           visc_flux<true, false, false>(d, c, fx, fy, fz);
@@ -318,8 +339,8 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
 #pragma unroll
       for (int n = 0; n < 5; ++n) s_fy[n][tj][ti] = f[n];
       // the faces at i = TX (wave 0) and j = TY (wave 1)
-      if (tid < TY) {
-        const int jj = tid;
+      if (tid >= C::XB && tid < C::XB + TY) {
+        const int jj = tid - C::XB;
         keep_face<H, 0, PL>(sq, (jj + HX) * RX + (TX + HX), 1, co, cv, f);
         if (VISC) {
           f[0] += 0.5 * (0.0 + 0.0);
@@ -328,8 +349,8 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
         }
 #pragma unroll
         for (int n = 0; n < 5; ++n) s_fx[n][jj][TX] = f[n];
-      } else if (tid >= C::XS && tid < C::XS + TX) {
-        const int ii = tid - C::XS;
+      } else if (tid >= C::YB && tid < C::YB + TX) {
+        const int ii = tid - C::YB;
         keep_face<H, 1, PL>(sq, (TY + HX) * RX + (ii + HX), RX, co, cv, f);
         if (VISC) {
           f[0] += 0.5 * (0.0 + 0.0);
@@ -369,12 +390,11 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
         for (int n = 0; n < 4; ++n) fz[n + 1] += 0.5 * (pfz_prev[n] + pfz_cur[n]);
       }
       if (fin) {
-        const int kk = cpl - 1;
-        int64_t ox = O.idx(gi, gj, kk);
+        gdouble* po = gptr(O.p) + O.idx(gi, gj, cpl - 1);
 #pragma unroll
         for (int n = 0; n < 5; ++n) {
           double L = xy_prev[n] + dxi2 * (fz_prev[n] - fz[n]);
-          O.p[ox + n * O.ns] = stage_combine(sc, s1v[n], sv[n], L);
+          po[n * O.ns] = stage_combine(sc, s1v[n], sv[n], L);
         }
       }
 #pragma unroll

@@ -39,6 +39,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=N)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run plain device copies of this many bytes (profiler calibration)")
+    ap.add_argument("--tiling", type=str, default="auto",
+                    help="fused-kernel tiling: auto (autotune), default, or TX,TY,TZ")
     ap.add_argument("--traffic", type=str, default=os.path.join(ROOT, "profiles", "traffic_tgv256_keep4v.json"))
     return ap.parse_args()
 
@@ -65,8 +69,24 @@ def cpu_baseline():
                       f"{el:.1f} s"}
 
 
+def copy_calibration(nbytes):
+    """Plain device-to-device copies of a known size, for scaling the HBM
+    byte counters (FETCH_SIZE / WRITE_SIZE) of a profiler run."""
+    import torch
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(4):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 4}))
+    return 0
+
+
 def main():
     a = parse()
+    if a.copy_calibration:
+        return copy_calibration(a.copy_calibration)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
@@ -102,6 +122,14 @@ def main():
             if events is not None:
                 events[s][1].record(stream)
 
+    lev.fill_patch(S)
+    if a.tiling == "auto":
+        tiling = lev.autotune(S)
+    elif a.tiling == "default":
+        tiling = None
+    else:
+        tiling = tuple(int(x) for x in a.tiling.split(","))
+        lev.set_tiling(*tiling)
     t0 = np.array(lev.totals(S))
     for _ in range(a.warmup):
         step()
@@ -153,6 +181,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "stage_kernel_ms": [float(x * 1e3) for x in dur],
+        "tiling": tiling,
         "totals_drift": {"mass": drift[0], "energy": drift[1]},
     }
     if not a.no_cpu:
