@@ -76,6 +76,11 @@ SIGNATURES = {
     "amr_est_dt": ([_P, ctypes.POINTER(Closures), _PF, ctypes.c_double,
                     ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "amr_totals": ([_P, _PF, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "amr_batch_create": ([_P, ctypes.c_int, ctypes.POINTER(Geom), ctypes.POINTER(_P)], ctypes.c_int),
+    "amr_batch_advance": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF,
+                           ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "amr_batch_est_dt": ([_P, ctypes.POINTER(Closures), _PF, ctypes.c_double,
+                          ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "amr_set_tiling": ([_P, ctypes.POINTER(Scheme), ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "amr_autotune": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF,
                       ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),

@@ -124,7 +124,12 @@ struct GraphEntry {
 
 struct amr_boxtab_s {
   amr_ctx_t ctx = nullptr;
-  amr_geom_t geom;
+  amr_geom_t geom;               // the level's geometry (first problem for a batch)
+  std::vector<amr_geom_t> geoms;  // per box: its level's or its own problem's geometry
+  bool batch = false;             // independent problems, one box each
+  double* d_box_dt = nullptr;     // per-box time steps of a batch step
+  double* h_box_dt = nullptr;     // pinned staging for d_box_dt
+  hipEvent_t dt_copied = nullptr;
   BoxTable bt;
   std::vector<std::unique_ptr<DescSet>> descs;
   int64_t use_clock = 0;
@@ -225,22 +230,23 @@ int bc_type(int code, int comp, int dir) {
 
 int build_fill(amr_boxtab_s* bt, DescSet* ds) {
   if (ds->fill_built) return AMR_OK;
-  const amr_geom_t& g = bt->geom;
   int nb = bt->bt.nbox;
-  int len[3];
-  for (int d = 0; d < 3; ++d) len[d] = g.dom_hi[d] - g.dom_lo[d] + 1;
   std::vector<Mapping> map(nb);
   for (int b = 0; b < nb; ++b) map[b] = mapping_of(ds->host[b]);
   std::vector<CopyTask> copies;
   int64_t ncell = 0;
   for (int b = 0; b < nb; ++b) {
     const amr_fab_t& fb = ds->host[b];
+    const amr_geom_t& g = bt->geoms[b];
+    int len[3];
+    for (int d = 0; d < 3; ++d) len[d] = g.dom_hi[d] - g.dom_lo[d] + 1;
     int G[2][3];
     for (int d = 0; d < 3; ++d) {
       G[0][d] = fb.lo[d] - fb.ng;
       G[1][d] = fb.hi[d] + fb.ng;
     }
     for (int s = 0; s < nb; ++s) {
+      if (bt->batch && s != b) continue;  // independent problems share no data
       const BoxDev& S = bt->bt.h_boxes[s];
       for (int sx = -1; sx <= 1; ++sx)
         for (int sy = -1; sy <= 1; ++sy)
@@ -279,11 +285,12 @@ int build_fill(amr_boxtab_s* bt, DescSet* ds) {
     HIPCHK(hipMemcpy(ds->d_copy, copies.data(), sizeof(CopyTask) * copies.size(), hipMemcpyHostToDevice));
   }
   for (int d = 0; d < 3; ++d) {
-    if (g.periodic[d]) continue;
     std::vector<BcTask> tasks;
     int64_t nc = 0;
     for (int b = 0; b < nb; ++b) {
       const amr_fab_t& fb = ds->host[b];
+      const amr_geom_t& g = bt->geoms[b];
+      if (g.periodic[d]) continue;
       for (int side = 0; side < 2; ++side) {
         BcTask t;
         for (int e = 0; e < 3; ++e) {
@@ -488,11 +495,9 @@ FusedConfig choose_config(const amr_boxtab_s* bt, const amr_scheme_t& sch) {
   return default_config(bt);
 }
 
-int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageCoef& sc,
+int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageArgs& sa,
              DescSet* ds1, DescSet* dsin, DescSet* dout) {
   Closures c = make_closures(cls);
-  double dxinv[3];
-  for (int d = 0; d < 3; ++d) dxinv[d] = 1.0 / bt->geom.dx[d];
   hipStream_t st = bt->ctx->stream;
   const Fab* s1 = ds1 ? ds1->d : nullptr;
   FusedConfig cfg = choose_config(bt, sch);
@@ -500,12 +505,12 @@ int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sc
     TileTable* tt = nullptr;
     int rc = get_tiles(bt, cfg, &tt);
     if (rc) return rc;
-    HIPCHK(launch_fused_stage(sch, cfg, tt->d, tt->n, bt->bt.d_boxes, c, dxinv, sc, s1, dsin->d, dout->d, st));
+    HIPCHK(launch_fused_stage(sch, cfg, tt->d, tt->n, bt->bt.d_boxes, c, sa, s1, dsin->d, dout->d, st));
     return AMR_OK;
   }
   int rc = ensure_unfused(bt, sch);
   if (rc) return rc;
-  HIPCHK(launch_unfused_stage(bt->bt, bt->uw, c, sch, dxinv, sc, s1, dsin->d, dout->d, st));
+  HIPCHK(launch_unfused_stage(bt->bt, bt->uw, c, sch, sa, s1, dsin->d, dout->d, st));
   return AMR_OK;
 }
 
@@ -558,43 +563,13 @@ int ensure_ws(amr_boxtab_s* bt, const amr_fab_t* state) {
 }
 
 int stage_coeffs(int order_rk, int stages_rk, double dt, int stage, int* nst, StageCoef* sc) {
-  // CNS::advance, src/tim/CNS_advance.cpp:51-181
-  StageCoef B0{1, 0.0, 0.0, 1.0, dt};  // Sin + d*L
-  int n = 0;
-  StageCoef r = B0;
-  if (order_rk == -2) {
-    n = 2;
-    if (stage == 1) r = StageCoef{1, 0.5, 0.0, 0.5, 0.5 * dt};
-  } else if (order_rk == 0) {
-    n = 1;
-    r = StageCoef{2, 0, 0, 0, 0};
-  } else if (order_rk == 1) {
-    n = 1;
-  } else if (order_rk == 2) {
-    int m = stages_rk;
-    if (m < 2) return fail(AMR_ERR_ARG, "SSPRK2 number of stages must be >= 2");
-    n = m;
-    if (stage < m - 1) r = StageCoef{1, 0.0, 0.0, 1.0, dt / double(m - 1)};
-    else r = StageCoef{0, 1.0 / m, 1.0 / m, double(m - 1), dt};
-  } else if (order_rk == 3 && stages_rk == 3) {
-    n = 3;
-    if (stage == 1) r = StageCoef{0, 3.0 / 4, 1.0 / 4, 1.0, dt};
-    if (stage == 2) r = StageCoef{0, 1.0 / 3, 2.0 / 3, 1.0, dt};
-  } else if (order_rk == 3 && stages_rk == 4) {
-    n = 4;
-    r = StageCoef{1, 0.0, 0.0, 1.0, dt / 2};
-    if (stage == 2) r = StageCoef{1, 2.0 / 3, 0.0, 1.0 / 3, dt / 6};
-  } else {
-    return fail(AMR_ERR_ARG, "unsupported order_rk/stages_rk combination");
-  }
-  if (stage < 0 || stage >= n) return fail(AMR_ERR_ARG, "stage index out of range");
-  *nst = n;
-  *sc = r;
+  if (!stage_coef_of(order_rk, stages_rk, dt, stage, sc, nst))
+    return fail(AMR_ERR_ARG, "unsupported order_rk/stages_rk combination or stage index");
   return AMR_OK;
 }
 
 int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const amr_fab_t* state,
-                 double dt) {
+                 double dt, const double* d_box_dt = nullptr) {
   int rc;
   int nst = 0;
   StageCoef sc;
@@ -612,14 +587,27 @@ int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t
     DescSet* sin = s == 0 ? dst : dw[(s - 1) % 2];
     DescSet* out = (s == nst - 1 && s > 0) ? dst : dw[s % 2];
     if ((rc = enqueue_fill(bt, sin))) return rc;
-    if ((rc = do_stage(bt, cls, sch, sc, dst, sin, out))) return rc;
+    StageArgs sa{sc, d_box_dt, sch.order_rk, sch.stages_rk, s};
+    if ((rc = do_stage(bt, cls, sch, sa, dst, sin, out))) return rc;
   }
   if (nst == 1) {
     // single-stage schemes wrote into workspace 0; copy its valid cells back
-    HIPCHK(launch_copy_valid(bt->bt, dw[0]->d, dst->d, bt->ctx->stream));
+    HIPCHK(launch_copy_valid(bt->bt, dw[0]->d, dst->d, d_box_dt, bt->ctx->stream));
   }
   return AMR_OK;
 }
+
+int check_geom(const amr_geom_t& g) {
+  for (int d = 0; d < 3; ++d) {
+    if (g.dom_hi[d] < g.dom_lo[d]) return fail(AMR_ERR_ARG, "empty domain");
+    if (!(g.dx[d] > 0.0)) return fail(AMR_ERR_ARG, "dx must be positive");
+    if (g.periodic[d] && g.dom_hi[d] - g.dom_lo[d] + 1 < NGHOST)
+      return fail(AMR_ERR_ARG, "periodic direction shorter than the ghost width");
+  }
+  return AMR_OK;
+}
+
+int finish_boxtab(amr_boxtab_s* bt);
 
 }  // namespace
 
@@ -666,11 +654,9 @@ int amr_ctx_synchronize(amr_ctx_t ctx) {
 
 int amr_boxtab_create(amr_ctx_t ctx, const amr_geom_t* geom, int nbox, const int32_t* boxes, amr_boxtab_t* out) {
   if (!ctx || !geom || !boxes || !out || nbox <= 0) return fail(AMR_ERR_ARG, "invalid box table arguments");
-  for (int d = 0; d < 3; ++d) {
-    if (geom->dom_hi[d] < geom->dom_lo[d]) return fail(AMR_ERR_ARG, "empty domain");
-    if (!(geom->dx[d] > 0.0)) return fail(AMR_ERR_ARG, "dx must be positive");
-    if (geom->periodic[d] && geom->dom_hi[d] - geom->dom_lo[d] + 1 < NGHOST)
-      return fail(AMR_ERR_ARG, "periodic direction shorter than the ghost width");
+  {
+    int rc = check_geom(*geom);
+    if (rc) return rc;
   }
   for (int b = 0; b < nbox; ++b)
     for (int d = 0; d < 3; ++d) {
@@ -685,19 +671,13 @@ int amr_boxtab_create(amr_ctx_t ctx, const amr_geom_t* geom, int nbox, const int
   bt->geom = *geom;
   bt->bt.nbox = nbox;
   bt->bt.h_boxes.resize(nbox);
+  bt->geoms.assign(nbox, *geom);
   for (int b = 0; b < nbox; ++b)
     for (int d = 0; d < 3; ++d) {
       bt->bt.h_boxes[b].lo[d] = boxes[6 * b + d];
       bt->bt.h_boxes[b].hi[d] = boxes[6 * b + 3 + d];
     }
-  int rc = AMR_OK;
-  hipError_t e = hipMalloc(&bt->bt.d_boxes, sizeof(BoxDev) * nbox);
-  if (e == hipSuccess)
-    e = hipMemcpy(bt->bt.d_boxes, bt->bt.h_boxes.data(), sizeof(BoxDev) * nbox, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMalloc(&bt->d_partial, sizeof(double) * 5 * 1024);
-  if (e == hipSuccess) e = hipMalloc(&bt->d_result, sizeof(double) * 8);
-  if (e == hipSuccess) e = hipHostMalloc(&bt->h_result, sizeof(double) * 8);
-  if (e != hipSuccess) rc = fail(AMR_ERR_HIP, std::string("box table allocation: ") + hipGetErrorString(e));
+  int rc = finish_boxtab(bt);
   if (rc) {
     amr_boxtab_destroy(bt);
     return rc;
@@ -705,6 +685,100 @@ int amr_boxtab_create(amr_ctx_t ctx, const amr_geom_t* geom, int nbox, const int
   *out = bt;
   return AMR_OK;
 }
+
+int amr_batch_create(amr_ctx_t ctx, int nprob, const amr_geom_t* geoms, amr_boxtab_t* out) {
+  if (!ctx || !geoms || !out || nprob <= 0) return fail(AMR_ERR_ARG, "invalid batch arguments");
+  for (int b = 0; b < nprob; ++b) {
+    int rc = check_geom(geoms[b]);
+    if (rc) return rc;
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  auto* bt = new amr_boxtab_s();
+  bt->ctx = ctx;
+  bt->geom = geoms[0];
+  bt->batch = true;
+  bt->bt.nbox = nprob;
+  bt->bt.h_boxes.resize(nprob);
+  bt->geoms.assign(geoms, geoms + nprob);
+  for (int b = 0; b < nprob; ++b)
+    for (int d = 0; d < 3; ++d) {
+      bt->bt.h_boxes[b].lo[d] = geoms[b].dom_lo[d];
+      bt->bt.h_boxes[b].hi[d] = geoms[b].dom_hi[d];
+    }
+  int rc = finish_boxtab(bt);
+  if (rc) {
+    amr_boxtab_destroy(bt);
+    return rc;
+  }
+  *out = bt;
+  return AMR_OK;
+}
+
+int amr_batch_advance(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
+                      const double* dt) {
+  if (!bt || !cls || !sch || !state || !dt) return fail(AMR_ERR_ARG, "null argument");
+  int rc;
+  if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls))) return rc;
+  int nb = bt->bt.nbox;
+  bool any = false;
+  for (int b = 0; b < nb; ++b) {
+    if (std::isnan(dt[b]) || dt[b] < 0.0) return fail(AMR_ERR_ARG, "time steps must be >= 0");
+    any = any || dt[b] > 0.0;
+  }
+  if (!any) return AMR_OK;
+  hipStream_t st = bt->ctx->stream;
+  HIPCHK(hipEventSynchronize(bt->dt_copied));  // staging buffer free again
+  std::memcpy(bt->h_box_dt, dt, sizeof(double) * nb);
+  HIPCHK(hipMemcpyAsync(bt->d_box_dt, bt->h_box_dt, sizeof(double) * nb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipEventRecord(bt->dt_copied, st));
+  return advance_impl(bt, *cls, *sch, state, 1.0, bt->d_box_dt);
+}
+
+int amr_batch_est_dt(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, double cfl,
+                     double* dt_out) {
+  if (!bt || !cls || !state || !dt_out) return fail(AMR_ERR_ARG, "null argument");
+  int rc = check_closures(*cls);
+  if (rc) return rc;
+  DescSet* ds = nullptr;
+  if ((rc = get_desc(bt, state, 0, &ds))) return rc;
+  Closures c = make_closures(*cls);
+  hipStream_t st = bt->ctx->stream;
+  int nb = bt->bt.nbox;
+  HIPCHK(launch_estdt_boxes(bt->bt, ds->d, c, bt->d_partial, bt->d_result, st));
+  HIPCHK(hipMemcpyAsync(dt_out, bt->d_result, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int b = 0; b < nb; ++b) {
+    if (!(dt_out[b] > 0.0))
+      return fail(AMR_ERR_STATE, "non-physical state in problem " + std::to_string(b) +
+                                     ": CFL time step is not positive");
+    dt_out[b] *= cfl;
+  }
+  return AMR_OK;
+}
+
+namespace {
+int finish_boxtab(amr_boxtab_s* bt) {
+  int nbox = bt->bt.nbox;
+  for (int b = 0; b < nbox; ++b)
+    for (int d = 0; d < 3; ++d) {
+      bt->bt.h_boxes[b].dx[d] = bt->geoms[b].dx[d];
+      bt->bt.h_boxes[b].dxi[d] = 1.0 / bt->geoms[b].dx[d];  // Geometry::InvCellSize
+    }
+  int rc = AMR_OK;
+  hipError_t e = hipMalloc(&bt->bt.d_boxes, sizeof(BoxDev) * nbox);
+  if (e == hipSuccess)
+    e = hipMemcpy(bt->bt.d_boxes, bt->bt.h_boxes.data(), sizeof(BoxDev) * nbox, hipMemcpyHostToDevice);
+  size_t npart = std::max<size_t>(5 * 1024, size_t(estdt_box_blocks(bt->bt)) * nbox);
+  if (e == hipSuccess) e = hipMalloc(&bt->d_partial, sizeof(double) * npart);
+  if (e == hipSuccess) e = hipMalloc(&bt->d_result, sizeof(double) * std::max(8, nbox));
+  if (e == hipSuccess) e = hipHostMalloc(&bt->h_result, sizeof(double) * 8);
+  if (e == hipSuccess) e = hipMalloc(&bt->d_box_dt, sizeof(double) * nbox);
+  if (e == hipSuccess) e = hipHostMalloc(&bt->h_box_dt, sizeof(double) * nbox);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->dt_copied, hipEventDisableTiming);
+  if (e != hipSuccess) rc = fail(AMR_ERR_HIP, std::string("box table allocation: ") + hipGetErrorString(e));
+  return rc;
+}
+}  // namespace
 
 int amr_boxtab_destroy(amr_boxtab_t bt) {
   if (!bt) return AMR_OK;
@@ -720,6 +794,9 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   if (bt->d_partial) (void)hipFree(bt->d_partial);
   if (bt->d_result) (void)hipFree(bt->d_result);
   if (bt->h_result) (void)hipHostFree(bt->h_result);
+  if (bt->d_box_dt) (void)hipFree(bt->d_box_dt);
+  if (bt->h_box_dt) (void)hipHostFree(bt->h_box_dt);
+  if (bt->dt_copied) (void)hipEventDestroy(bt->dt_copied);
   delete bt;
   return AMR_OK;
 }
@@ -765,7 +842,8 @@ int amr_rk_stage(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
   for (int i = 0; i < bt->bt.nbox; ++i)
     if (mapping_of(out[i]) == mapping_of(sin[i]))
       return fail(AMR_ERR_ARG, "out must not alias sin");
-  return do_stage(bt, *cls, *sch, sc, ds1, dsin, dout);
+  StageArgs sa{sc, nullptr, 0, 0, 0};
+  return do_stage(bt, *cls, *sch, sa, ds1, dsin, dout);
 }
 
 int amr_rhs(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
@@ -895,13 +973,13 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
   HIPCHK(hipEventCreate(&e1));
   float best = 1e30f;
   FusedConfig bestc = cand[0];
-  StageCoef sc{2, 0, 0, 0, 0};
+  StageArgs sa{StageCoef{2, 0, 0, 0, 0}, nullptr, 0, 0, 0};
   for (auto& cfg : cand) {
     bt->ctx->tuned[key] = cfg;
     float tbest = 1e30f;
     for (int rep = 0; rep < 4; ++rep) {
       HIPCHK(hipEventRecord(e0, st));
-      if ((rc = do_stage(bt, *cls, *sch, sc, nullptr, din, dout))) return rc;
+      if ((rc = do_stage(bt, *cls, *sch, sa, nullptr, din, dout))) return rc;
       HIPCHK(hipEventRecord(e1, st));
       HIPCHK(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -930,7 +1008,7 @@ int amr_est_dt(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* stat
   if ((rc = get_desc(bt, state, 0, &ds))) return rc;
   Closures c = make_closures(*cls);
   hipStream_t st = bt->ctx->stream;
-  HIPCHK(launch_estdt(bt->bt, ds->d, c, bt->geom.dx, bt->d_partial, bt->d_result, st));
+  HIPCHK(launch_estdt(bt->bt, ds->d, c, bt->d_partial, bt->d_result, st));
   HIPCHK(hipMemcpyAsync(bt->h_result, bt->d_result, sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   double dt = bt->h_result[0];

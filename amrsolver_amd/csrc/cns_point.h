@@ -271,6 +271,65 @@ struct StageCoef {
   double a, b, c, d;
 };
 
+// Coefficients of stage `stage` of scheme (order_rk, stages_rk) for step
+// size dt, computed with the reference's own expressions
+// (src/tim/CNS_advance.cpp:51-181); false for an unknown scheme or stage.
+__host__ __device__ inline bool stage_coef_of(int order_rk, int stages_rk, double dt, int stage, StageCoef* out,
+                                              int* nstages) {
+  StageCoef r{1, 0.0, 0.0, 1.0, dt};  // Sin + d*L
+  int n = 0;
+  if (order_rk == -2) {
+    n = 2;
+    if (stage == 1) r = StageCoef{1, 0.5, 0.0, 0.5, 0.5 * dt};
+  } else if (order_rk == 0) {
+    n = 1;
+    r = StageCoef{2, 0, 0, 0, 0};
+  } else if (order_rk == 1) {
+    n = 1;
+  } else if (order_rk == 2) {
+    int m = stages_rk;
+    if (m < 2) return false;
+    n = m;
+    if (stage < m - 1) r = StageCoef{1, 0.0, 0.0, 1.0, dt / double(m - 1)};
+    else r = StageCoef{0, 1.0 / m, 1.0 / m, double(m - 1), dt};
+  } else if (order_rk == 3 && stages_rk == 3) {
+    n = 3;
+    if (stage == 1) r = StageCoef{0, 3.0 / 4, 1.0 / 4, 1.0, dt};
+    if (stage == 2) r = StageCoef{0, 1.0 / 3, 2.0 / 3, 1.0, dt};
+  } else if (order_rk == 3 && stages_rk == 4) {
+    n = 4;
+    r = StageCoef{1, 0.0, 0.0, 1.0, dt / 2};
+    if (stage == 2) r = StageCoef{1, 2.0 / 3, 0.0, 1.0 / 3, dt / 6};
+  } else {
+    return false;
+  }
+  if (stage < 0 || stage >= n) return false;
+  *out = r;
+  *nstages = n;
+  return true;
+}
+
+// Per-launch stage arguments: one coefficient set for every box, or (for a
+// batch of independent problems) coefficients from each box's own time step
+// box_dt[b]; boxes with box_dt[b] <= 0 are inactive and left untouched.
+struct StageArgs {
+  StageCoef sc;
+  const double* box_dt;
+  int order_rk, stages_rk, stage;
+};
+
+__device__ __forceinline__ bool box_active(const StageArgs& sa, int b) {
+  return sa.box_dt == nullptr || sa.box_dt[b] > 0.0;
+}
+
+__device__ __forceinline__ StageCoef box_stage(const StageArgs& sa, int b) {
+  if (sa.box_dt == nullptr) return sa.sc;
+  StageCoef r;
+  int n;
+  stage_coef_of(sa.order_rk, sa.stages_rk, sa.box_dt[b], sa.stage, &r, &n);
+  return r;
+}
+
 // With a == 0 the s1 term is left out entirely (s1 is then not read), which
 // reproduces LinComb(dst, 1.0, x, dt, y) = x + dt*y bit for bit.
 __device__ __forceinline__ double stage_combine(const StageCoef& sc, double s1, double sin,

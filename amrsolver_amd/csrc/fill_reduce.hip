@@ -66,8 +66,10 @@ __global__ void __launch_bounds__(FBLK) k_fill_bc(const BcTask* tasks, int ntask
   }
 }
 
-__global__ void __launch_bounds__(FBLK) k_copy_valid(const BoxDev* boxes, const Fab* src, const Fab* dst) {
+__global__ void __launch_bounds__(FBLK) k_copy_valid(const BoxDev* boxes, const Fab* src, const Fab* dst,
+                                                     const double* box_dt) {
   const int b = blockIdx.y;
+  if (box_dt != nullptr && !(box_dt[b] > 0.0)) return;
   const BoxDev bx = boxes[b];
   int nx = bx.hi[0] - bx.lo[0] + 1, ny = bx.hi[1] - bx.lo[1] + 1, nz = bx.hi[2] - bx.lo[2] + 1;
   int64_t t = int64_t(blockIdx.x) * FBLK + threadIdx.x;
@@ -85,13 +87,70 @@ constexpr int RBLK = 256;
 
 // cns_estdt (src/cls/CNS_K.h:12-47; z momentum read from UMY, CNS_K.h:29):
 // per-block minimum of the cell time steps.
+__device__ __forceinline__ double cell_estdt(const Fab& u, int i, int j, int k, const Closures& c, double dx0,
+                                             double dx1, double dx2) {
+  double rho = u(i, j, k, URHO);
+  double mx = u(i, j, k, UMX);
+  double my = u(i, j, k, UMY);
+  double mz = u(i, j, k, UMY);
+  double rhoinv = 1.0 / dmax(rho, 1.0e-40);
+  double vx = mx * rhoinv, vy = my * rhoinv, vz = mz * rhoinv;
+  double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+  double rhoei = (u(i, j, k, UET) - ke);
+  double p = c.gm1 * rhoei;
+  double cs = sqrt(c.gamma * p * rhoinv);
+  double dtx = dx0 / (fabs(vx) + cs);
+  double dty = dx1 / (fabs(vy) + cs);
+  double dtz = dx2 / (fabs(vz) + cs);
+  return dmin(dtx, dmin(dty, dtz));
+}
+
+__device__ __forceinline__ double block_min(double v, double* sm) {
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = RBLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = dmin(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  return sm[0];
+}
+
+// per-box minimum: blockIdx.y = box, partial[box * gridDim.x + blockIdx.x]
+__global__ void __launch_bounds__(RBLK) k_estdt_boxes(const BoxDev* boxes, const Fab* st, Closures c,
+                                                      double* partial) {
+  __shared__ double sm[RBLK];
+  const int b = blockIdx.y;
+  const BoxDev bx = boxes[b];
+  const Fab u = st[b];
+  double dt = 1.79769313486231570815e+308;
+  int nx = bx.hi[0] - bx.lo[0] + 1, ny = bx.hi[1] - bx.lo[1] + 1, nz = bx.hi[2] - bx.lo[2] + 1;
+  int64_t n = int64_t(nx) * ny * nz;
+  for (int64_t t = int64_t(blockIdx.x) * RBLK + threadIdx.x; t < n; t += int64_t(gridDim.x) * RBLK) {
+    int i = bx.lo[0] + int(t % nx);
+    int64_t r = t / nx;
+    int j = bx.lo[1] + int(r % ny);
+    int k = bx.lo[2] + int(r / ny);
+    dt = dmin(dt, cell_estdt(u, i, j, k, c, bx.dx[0], bx.dx[1], bx.dx[2]));
+  }
+  double m = block_min(dt, sm);
+  if (threadIdx.x == 0) partial[size_t(b) * gridDim.x + blockIdx.x] = m;
+}
+
+__global__ void __launch_bounds__(RBLK) k_min_boxes(const double* partial, int nper, double* result) {
+  __shared__ double sm[RBLK];
+  double v = 1.79769313486231570815e+308;
+  for (int t = threadIdx.x; t < nper; t += RBLK) v = dmin(v, partial[size_t(blockIdx.x) * nper + t]);
+  double m = block_min(v, sm);
+  if (threadIdx.x == 0) result[blockIdx.x] = m;
+}
+
 __global__ void __launch_bounds__(RBLK) k_estdt(const BoxDev* boxes, int nbox, const Fab* st,
-                                                Closures c, double dx0, double dx1, double dx2,
-                                                double* partial) {
+                                                Closures c, double* partial) {
   __shared__ double sm[RBLK];
   double dt = 1.79769313486231570815e+308;
   for (int b = 0; b < nbox; ++b) {
     const BoxDev bx = boxes[b];
+    const double dx0 = bx.dx[0], dx1 = bx.dx[1], dx2 = bx.dx[2];
     const Fab u = st[b];
     int nx = bx.hi[0] - bx.lo[0] + 1, ny = bx.hi[1] - bx.lo[1] + 1, nz = bx.hi[2] - bx.lo[2] + 1;
     int64_t n = int64_t(nx) * ny * nz;
@@ -100,20 +159,7 @@ __global__ void __launch_bounds__(RBLK) k_estdt(const BoxDev* boxes, int nbox, c
       int64_t r = t / nx;
       int j = bx.lo[1] + int(r % ny);
       int k = bx.lo[2] + int(r / ny);
-      double rho = u(i, j, k, URHO);
-      double mx = u(i, j, k, UMX);
-      double my = u(i, j, k, UMY);
-      double mz = u(i, j, k, UMY);
-      double rhoinv = 1.0 / dmax(rho, 1.0e-40);
-      double vx = mx * rhoinv, vy = my * rhoinv, vz = mz * rhoinv;
-      double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
-      double rhoei = (u(i, j, k, UET) - ke);
-      double p = c.gm1 * rhoei;
-      double cs = sqrt(c.gamma * p * rhoinv);
-      double dtx = dx0 / (fabs(vx) + cs);
-      double dty = dx1 / (fabs(vy) + cs);
-      double dtz = dx2 / (fabs(vz) + cs);
-      dt = dmin(dt, dmin(dtx, dmin(dty, dtz)));
+      dt = dmin(dt, cell_estdt(u, i, j, k, c, dx0, dx1, dx2));
     }
   }
   sm[threadIdx.x] = dt;
@@ -209,12 +255,13 @@ hipError_t launch_fill_bc(const BcTask* tasks, int ntask, int64_t ncell, const F
   return hipGetLastError();
 }
 
-hipError_t launch_copy_valid(const BoxTable& bt, const Fab* src, const Fab* dst, hipStream_t st) {
+hipError_t launch_copy_valid(const BoxTable& bt, const Fab* src, const Fab* dst, const double* box_dt,
+                             hipStream_t st) {
   int64_t mx = 1;
   for (auto& b : bt.h_boxes)
     mx = std::max<int64_t>(mx, int64_t(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1));
   hipLaunchKernelGGL(k_copy_valid, dim3(unsigned((mx + FBLK - 1) / FBLK), unsigned(bt.nbox)), dim3(FBLK), 0, st,
-                     bt.d_boxes, src, dst);
+                     bt.d_boxes, src, dst, box_dt);
   return hipGetLastError();
 }
 
@@ -228,11 +275,27 @@ int reduce_blocks(const BoxTable& bt) {
   return int(blocks);
 }
 
-hipError_t launch_estdt(const BoxTable& bt, const Fab* st, const Closures& c, const double dx[3], double* partial,
-                        double* result, hipStream_t s) {
+hipError_t launch_estdt(const BoxTable& bt, const Fab* st, const Closures& c, double* partial, double* result,
+                        hipStream_t s) {
   int nb = reduce_blocks(bt);
-  hipLaunchKernelGGL(k_estdt, dim3(nb), dim3(RBLK), 0, s, bt.d_boxes, bt.nbox, st, c, dx[0], dx[1], dx[2], partial);
+  hipLaunchKernelGGL(k_estdt, dim3(nb), dim3(RBLK), 0, s, bt.d_boxes, bt.nbox, st, c, partial);
   hipLaunchKernelGGL(k_min_final, dim3(1), dim3(RBLK), 0, s, partial, nb, result);
+  return hipGetLastError();
+}
+
+int estdt_box_blocks(const BoxTable& bt) {
+  int64_t mx = 1;
+  for (auto& b : bt.h_boxes)
+    mx = std::max<int64_t>(mx, int64_t(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1));
+  int64_t nb = (mx + RBLK * 8 - 1) / (RBLK * 8);
+  return int(std::max<int64_t>(1, std::min<int64_t>(nb, 64)));
+}
+
+hipError_t launch_estdt_boxes(const BoxTable& bt, const Fab* st, const Closures& c, double* partial, double* result,
+                              hipStream_t s) {
+  int nper = estdt_box_blocks(bt);
+  hipLaunchKernelGGL(k_estdt_boxes, dim3(nper, bt.nbox), dim3(RBLK), 0, s, bt.d_boxes, st, c, partial);
+  hipLaunchKernelGGL(k_min_boxes, dim3(bt.nbox), dim3(RBLK), 0, s, partial, nper, result);
   return hipGetLastError();
 }
 

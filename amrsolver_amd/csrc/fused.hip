@@ -105,10 +105,10 @@ __device__ __forceinline__ void keep_face(const double* sq, int idx, int st, con
 }
 
 template <int H, bool VISC, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ tiles, Closures c, double co0,
-                                                  double co1, double co2, double dxi0, double dxi1, double dxi2,
-                                                  StageCoef sc, const Fab* __restrict__ s1f,
-                                                  const Fab* __restrict__ sinf, const Fab* __restrict__ outf) {
+__global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ tiles, const BoxDev* __restrict__ boxes,
+                                                     Closures c, double co0, double co1, double co2, StageArgs sa,
+                                                     const Fab* __restrict__ s1f, const Fab* __restrict__ sinf,
+                                                     const Fab* __restrict__ outf) {
   using C = Cfg<H, VISC, TX, TY>;
   constexpr int RX = C::RX, HX = C::HX, PL = C::PL, QN = C::QN, NT = C::NT;
   __shared__ double s_q[2][NPRIM * PL];
@@ -118,6 +118,9 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
   __shared__ double s_fy[5][TY + 1][TX];
 
   const TileDev T = tiles[blockIdx.x];
+  if (!box_active(sa, T.box)) return;  // uniform: the whole workgroup leaves
+  const StageCoef sc = box_stage(sa, T.box);
+  const double dxi0 = boxes[T.box].dxi[0], dxi1 = boxes[T.box].dxi[1], dxi2 = boxes[T.box].dxi[2];
   const Fab S = sinf[T.box];
   const Fab O = outf[T.box];
   const bool use_s1 = sc.mode != 2 && sc.a != 0.0;
@@ -412,23 +415,23 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
 }
 
 template <int H, bool VISC, int TX, int TY>
-void launch_t(const TileDev* tiles, int ntiles, const Closures& c, const double co[3], const double dxi[3],
-              const StageCoef& sc, const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
-  hipLaunchKernelGGL((k_fused<H, VISC, TX, TY>), dim3(ntiles), dim3(TX * TY), 0, st, tiles, c, co[0], co[1], co[2],
-                     dxi[0], dxi[1], dxi[2], sc, s1, sin, out);
+void launch_t(const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c, const double co[3],
+              const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
+  hipLaunchKernelGGL((k_fused<H, VISC, TX, TY>), dim3(ntiles), dim3(TX * TY), 0, st, tiles, boxes, c, co[0], co[1],
+                     co[2], sa, s1, sin, out);
 }
 
 template <int TX, int TY>
-bool launch_shape(const amr_scheme_t& s, const TileDev* tiles, int ntiles, const Closures& c, const double co[3],
-                  const double dxi[3], const StageCoef& sc, const Fab* s1, const Fab* sin, const Fab* out,
+bool launch_shape(const amr_scheme_t& s, const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c,
+                  const double co[3], const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* out,
                   hipStream_t st) {
   const bool v = s.rhs_visc != 0;
   if (s.order_keep == 4) {
-    if (v) launch_t<2, true, TX, TY>(tiles, ntiles, c, co, dxi, sc, s1, sin, out, st);
-    else launch_t<2, false, TX, TY>(tiles, ntiles, c, co, dxi, sc, s1, sin, out, st);
+    if (v) launch_t<2, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, s1, sin, out, st);
+    else launch_t<2, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, s1, sin, out, st);
   } else if (s.order_keep == 2) {
-    if (v) launch_t<1, true, TX, TY>(tiles, ntiles, c, co, dxi, sc, s1, sin, out, st);
-    else launch_t<1, false, TX, TY>(tiles, ntiles, c, co, dxi, sc, s1, sin, out, st);
+    if (v) launch_t<1, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, s1, sin, out, st);
+    else launch_t<1, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, s1, sin, out, st);
   } else {
     return false;
   }
@@ -444,14 +447,14 @@ bool fused_supported(const amr_scheme_t& s, int tx, int ty) {
 }
 
 hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const TileDev* tiles, int ntiles,
-                              const BoxDev*, const Closures& c, const double dxinv[3], const StageCoef& sc,
-                              const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
+                              const BoxDev* boxes, const Closures& c, const StageArgs& sa, const Fab* s1,
+                              const Fab* sin, const Fab* out, hipStream_t st) {
   double co[3];
   keep_coefficients(s.order_keep, co);
   bool ok = false;
-  if (cfg.tx == 16 && cfg.ty == 16) ok = launch_shape<16, 16>(s, tiles, ntiles, c, co, dxinv, sc, s1, sin, out, st);
-  else if (cfg.tx == 8 && cfg.ty == 8) ok = launch_shape<8, 8>(s, tiles, ntiles, c, co, dxinv, sc, s1, sin, out, st);
-  else if (cfg.tx == 32 && cfg.ty == 8) ok = launch_shape<32, 8>(s, tiles, ntiles, c, co, dxinv, sc, s1, sin, out, st);
+  if (cfg.tx == 16 && cfg.ty == 16) ok = launch_shape<16, 16>(s, tiles, ntiles, boxes, c, co, sa, s1, sin, out, st);
+  else if (cfg.tx == 8 && cfg.ty == 8) ok = launch_shape<8, 8>(s, tiles, ntiles, boxes, c, co, sa, s1, sin, out, st);
+  else if (cfg.tx == 32 && cfg.ty == 8) ok = launch_shape<32, 8>(s, tiles, ntiles, boxes, c, co, sa, s1, sin, out, st);
   if (!ok) return hipErrorInvalidValue;
   return hipGetLastError();
 }

@@ -14,6 +14,8 @@ namespace amr {
 
 struct BoxDev {
   int lo[3], hi[3];
+  double dxi[3];  // inverse cell size of the box's level or problem
+  double dx[3];   // cell size (as given; not recomputed from dxi)
 };
 
 // The level's box list on host and device.
@@ -63,8 +65,8 @@ inline void keep_coefficients(int order, double co[3]) {
 }
 
 hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const Closures& c,
-                                const amr_scheme_t& s, const double dxinv[3], const StageCoef& sc,
-                                const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st);
+                                const amr_scheme_t& s, const StageArgs& sa, const Fab* s1, const Fab* sin,
+                                const Fab* out, hipStream_t st);
 
 hipError_t launch_fill_copy(const CopyTask* tasks, int ntask, int64_t ncell, const Fab* fabs,
                             hipStream_t st);
@@ -72,13 +74,18 @@ hipError_t launch_fill_bc(const BcTask* tasks, int ntask, int64_t ncell, const F
                           hipStream_t st);
 
 // dst = src on the valid cells of every box.
-hipError_t launch_copy_valid(const BoxTable& bt, const Fab* src, const Fab* dst, hipStream_t st);
+hipError_t launch_copy_valid(const BoxTable& bt, const Fab* src, const Fab* dst, const double* box_dt,
+                             hipStream_t st);
 
 // Reductions over valid cells.  `partial` holds at least reduce_partials()
 // doubles (x5 for totals); the result lands in result[0] (dt) or result[0..4].
 int reduce_blocks(const BoxTable& bt);
-hipError_t launch_estdt(const BoxTable& bt, const Fab* st, const Closures& c, const double dx[3],
-                        double* partial, double* result, hipStream_t s);
+hipError_t launch_estdt(const BoxTable& bt, const Fab* st, const Closures& c, double* partial,
+                        double* result, hipStream_t s);
+// per-box minimum of cns_estdt: result[b] for every box
+int estdt_box_blocks(const BoxTable& bt);
+hipError_t launch_estdt_boxes(const BoxTable& bt, const Fab* st, const Closures& c, double* partial,
+                              double* result, hipStream_t s);
 hipError_t launch_totals(const BoxTable& bt, const Fab* st, double* partial, double* result,
                          hipStream_t s);
 
@@ -93,8 +100,7 @@ struct FusedConfig {
 };
 bool fused_supported(const amr_scheme_t& s, int tx, int ty);
 hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const TileDev* tiles,
-                              int ntiles, const BoxDev* boxes, const Closures& c,
-                              const double dxinv[3], const StageCoef& sc, const Fab* s1,
-                              const Fab* sin, const Fab* out, hipStream_t st);
+                              int ntiles, const BoxDev* boxes, const Closures& c, const StageArgs& sa,
+                              const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st);
 
 }  // namespace amr

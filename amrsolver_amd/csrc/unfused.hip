@@ -33,13 +33,14 @@ __device__ __forceinline__ bool region_cell(const BoxDev& bx, const Region& r, i
 
 #define REGION_LOOP_BEGIN                                                   \
   const int b = blockIdx.y;                                                 \
+  if (box_dt != nullptr && !(box_dt[b] > 0.0)) return;                      \
   const BoxDev bx = boxes[b];                                               \
   int i, j, k;                                                              \
   if (!region_cell(bx, reg, int64_t(blockIdx.x) * BLK + threadIdx.x, i, j, k)) return;
 
 // ---------------------------------------------------------------- prims
 // CNS_advance.cpp:200-208: cons2prim on growntilebox(NGHOST)
-__global__ void __launch_bounds__(BLK) k_prims(const BoxDev* boxes, Region reg, const Fab* st,
+__global__ void __launch_bounds__(BLK) k_prims(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* st,
                                                const Fab* pr, Closures c) {
   REGION_LOOP_BEGIN
   const Fab u = st[b], q = pr[b];
@@ -52,7 +53,7 @@ __global__ void __launch_bounds__(BLK) k_prims(const BoxDev* boxes, Region reg, 
 // ---------------------------------------------------------------- KEEP
 // CentralKEEP.h:40-145 on grownnodaltilebox(-1,0)
 template <int H>
-__global__ void __launch_bounds__(BLK) k_keep(const BoxDev* boxes, Region reg, const Fab* pr,
+__global__ void __launch_bounds__(BLK) k_keep(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* pr,
                                               const Fab* nfx, const Fab* nfy, const Fab* nfz,
                                               Closures c, double c0, double c1, double c2) {
   REGION_LOOP_BEGIN
@@ -85,7 +86,7 @@ __global__ void __launch_bounds__(BLK) k_keep(const BoxDev* boxes, Region reg, c
 
 // ---------------------------------------------------------------- WENO5
 // High_resolution.h:83-121 cons2eulerflux_lambda on growntilebox(NGHOST)
-__global__ void __launch_bounds__(BLK) k_eulerflux(const BoxDev* boxes, Region reg, const Fab* st,
+__global__ void __launch_bounds__(BLK) k_eulerflux(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* st,
                                                    const Fab* pfx, const Fab* pfy, const Fab* pfz,
                                                    const Fab* lam, Closures c) {
   REGION_LOOP_BEGIN
@@ -139,7 +140,7 @@ __device__ __forceinline__ double weno_face(int i, int j, int k, int n, const Fa
   return flx;
 }
 
-__global__ void __launch_bounds__(BLK) k_weno(const BoxDev* boxes, Region reg, const Fab* st,
+__global__ void __launch_bounds__(BLK) k_weno(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* st,
                                               const Fab* pfx, const Fab* pfy, const Fab* pfz,
                                               const Fab* lam, const Fab* nfx, const Fab* nfy,
                                               const Fab* nfz) {
@@ -156,7 +157,7 @@ __global__ void __launch_bounds__(BLK) k_weno(const BoxDev* boxes, Region reg, c
 // ---------------------------------------------------------------- Riemann
 // Riemann.h:19-120 cns_slope_{x,y,z} on grow(bx,dir,1)
 template <int D>
-__global__ void __launch_bounds__(BLK) k_slope(const BoxDev* boxes, Region reg, const Fab* pr,
+__global__ void __launch_bounds__(BLK) k_slope(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* pr,
                                                const Fab* slope, Closures c) {
   REGION_LOOP_BEGIN
   constexpr int DI = D == 0, DJ = D == 1, DK = D == 2;
@@ -189,7 +190,7 @@ __global__ void __launch_bounds__(BLK) k_slope(const BoxDev* boxes, Region reg, 
 // Riemann.h:251-337 cns_riemann_{x,y,z} on surroundingNodes(bx,dir).  The z
 // left state takes its sound speed from T at k+1 (Riemann.h:316).
 template <int D>
-__global__ void __launch_bounds__(BLK) k_riemann(const BoxDev* boxes, Region reg, const Fab* pr,
+__global__ void __launch_bounds__(BLK) k_riemann(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* pr,
                                                  const Fab* slope, const Fab* nf, Closures c) {
   REGION_LOOP_BEGIN
   constexpr int DI = D == 0, DJ = D == 1, DK = D == 2;
@@ -229,10 +230,11 @@ __global__ void __launch_bounds__(BLK) k_riemann(const BoxDev* boxes, Region reg
 
 // ---------------------------------------------------------------- viscous
 // CNS_hydro_K.h:108-172 ViscousFluxes on growntilebox(1)
-__global__ void __launch_bounds__(BLK) k_visc_pf(const BoxDev* boxes, Region reg, const Fab* pr,
+__global__ void __launch_bounds__(BLK) k_visc_pf(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* pr,
                                                  const Fab* pfx, const Fab* pfy, const Fab* pfz,
-                                                 Closures c, double dx0, double dx1, double dx2) {
+                                                 Closures c) {
   REGION_LOOP_BEGIN
+  const double dx0 = bx.dxi[0], dx1 = bx.dxi[1], dx2 = bx.dxi[2];
   const Fab q = pr[b], fx = pfx[b], fy = pfy[b], fz = pfz[b];
   double ux = q(i, j, k, QU);
   double dudx = (q(i + 1, j, k, QU) - q(i - 1, j, k, QU)) * 0.5 * dx0;
@@ -278,7 +280,7 @@ __global__ void __launch_bounds__(BLK) k_visc_pf(const BoxDev* boxes, Region reg
 }
 
 // CNS_hydro_K.h:254-261 ViscousNumericalFluxes on grownnodaltilebox(-1,0)
-__global__ void __launch_bounds__(BLK) k_visc_add(const BoxDev* boxes, Region reg, const Fab* pfx,
+__global__ void __launch_bounds__(BLK) k_visc_add(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* pfx,
                                                   const Fab* pfy, const Fab* pfz, const Fab* nfx,
                                                   const Fab* nfy, const Fab* nfz) {
   REGION_LOOP_BEGIN
@@ -293,11 +295,12 @@ __global__ void __launch_bounds__(BLK) k_visc_add(const BoxDev* boxes, Region re
 
 // ---------------------------------------------------------------- assembly
 // CNS_advance.cpp:376-391 plus the stage combination of CNS::advance
-__global__ void __launch_bounds__(BLK) k_assemble(const BoxDev* boxes, Region reg, const Fab* nfx,
-                                                  const Fab* nfy, const Fab* nfz, double dx0,
-                                                  double dx1, double dx2, StageCoef sc,
+__global__ void __launch_bounds__(BLK) k_assemble(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* nfx,
+                                                  const Fab* nfy, const Fab* nfz, StageArgs sa,
                                                   const Fab* s1, const Fab* sin, const Fab* out) {
   REGION_LOOP_BEGIN
+  const double dx0 = bx.dxi[0], dx1 = bx.dxi[1], dx2 = bx.dxi[2];
+  const StageCoef sc = box_stage(sa, b);
   const Fab fx = nfx[b], fy = nfy[b], fz = nfz[b], o = out[b];
 #pragma unroll
   for (int n = 0; n < NCONS; ++n) {
@@ -331,12 +334,12 @@ Region nodal() { return Region{{0, 0, 0}, {1, 1, 1}}; }
 
 // Flux phase + assembly + stage combination, reference pass structure.
 hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const Closures& c,
-                                const amr_scheme_t& s, const double dxinv[3], const StageCoef& sc,
-                                const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
+                                const amr_scheme_t& s, const StageArgs& sa, const Fab* s1, const Fab* sin,
+                                const Fab* out, hipStream_t st) {
   const BoxDev* B = bt.d_boxes;
   {
     Region r = grown(NGHOST);
-    hipLaunchKernelGGL(k_prims, grid_for(bt, r), dim3(BLK), 0, st, B, r, sin, w.prims, c);
+    hipLaunchKernelGGL(k_prims, grid_for(bt, r), dim3(BLK), 0, st, B, r, sa.box_dt, sin, w.prims, c);
   }
   if (s.rhs_euler) {
     if (s.flux_euler == 1) {
@@ -345,16 +348,16 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
       Region r = nodal();
       dim3 g = grid_for(bt, r);
       if (s.order_keep == 6)
-        hipLaunchKernelGGL(k_keep<3>, g, dim3(BLK), 0, st, B, r, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
+        hipLaunchKernelGGL(k_keep<3>, g, dim3(BLK), 0, st, B, r, sa.box_dt, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
       else if (s.order_keep == 4)
-        hipLaunchKernelGGL(k_keep<2>, g, dim3(BLK), 0, st, B, r, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
+        hipLaunchKernelGGL(k_keep<2>, g, dim3(BLK), 0, st, B, r, sa.box_dt, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
       else
-        hipLaunchKernelGGL(k_keep<1>, g, dim3(BLK), 0, st, B, r, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
+        hipLaunchKernelGGL(k_keep<1>, g, dim3(BLK), 0, st, B, r, sa.box_dt, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
     } else if (s.flux_euler == 2) {
       Region r = grown(NGHOST);
-      hipLaunchKernelGGL(k_eulerflux, grid_for(bt, r), dim3(BLK), 0, st, B, r, sin, w.pe[0], w.pe[1], w.pe[2], w.lam, c);
+      hipLaunchKernelGGL(k_eulerflux, grid_for(bt, r), dim3(BLK), 0, st, B, r, sa.box_dt, sin, w.pe[0], w.pe[1], w.pe[2], w.lam, c);
       Region rn = nodal();
-      hipLaunchKernelGGL(k_weno, grid_for(bt, rn), dim3(BLK), 0, st, B, rn, sin, w.pe[0], w.pe[1], w.pe[2], w.lam,
+      hipLaunchKernelGGL(k_weno, grid_for(bt, rn), dim3(BLK), 0, st, B, rn, sa.box_dt, sin, w.pe[0], w.pe[1], w.pe[2], w.lam,
                          w.nf[0], w.nf[1], w.nf[2]);
     } else {
       for (int d = 0; d < 3; ++d) {
@@ -365,29 +368,28 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
         rf.ghi[d] = 1;
         dim3 gs = grid_for(bt, rs), gf = grid_for(bt, rf);
         if (d == 0) {
-          hipLaunchKernelGGL(k_slope<0>, gs, dim3(BLK), 0, st, B, rs, w.prims, w.slope, c);
-          hipLaunchKernelGGL(k_riemann<0>, gf, dim3(BLK), 0, st, B, rf, w.prims, w.slope, w.nf[0], c);
+          hipLaunchKernelGGL(k_slope<0>, gs, dim3(BLK), 0, st, B, rs, sa.box_dt, w.prims, w.slope, c);
+          hipLaunchKernelGGL(k_riemann<0>, gf, dim3(BLK), 0, st, B, rf, sa.box_dt, w.prims, w.slope, w.nf[0], c);
         } else if (d == 1) {
-          hipLaunchKernelGGL(k_slope<1>, gs, dim3(BLK), 0, st, B, rs, w.prims, w.slope, c);
-          hipLaunchKernelGGL(k_riemann<1>, gf, dim3(BLK), 0, st, B, rf, w.prims, w.slope, w.nf[1], c);
+          hipLaunchKernelGGL(k_slope<1>, gs, dim3(BLK), 0, st, B, rs, sa.box_dt, w.prims, w.slope, c);
+          hipLaunchKernelGGL(k_riemann<1>, gf, dim3(BLK), 0, st, B, rf, sa.box_dt, w.prims, w.slope, w.nf[1], c);
         } else {
-          hipLaunchKernelGGL(k_slope<2>, gs, dim3(BLK), 0, st, B, rs, w.prims, w.slope, c);
-          hipLaunchKernelGGL(k_riemann<2>, gf, dim3(BLK), 0, st, B, rf, w.prims, w.slope, w.nf[2], c);
+          hipLaunchKernelGGL(k_slope<2>, gs, dim3(BLK), 0, st, B, rs, sa.box_dt, w.prims, w.slope, c);
+          hipLaunchKernelGGL(k_riemann<2>, gf, dim3(BLK), 0, st, B, rf, sa.box_dt, w.prims, w.slope, w.nf[2], c);
         }
       }
     }
   }
   if (s.rhs_visc) {
     Region r1 = grown(1);
-    hipLaunchKernelGGL(k_visc_pf, grid_for(bt, r1), dim3(BLK), 0, st, B, r1, w.prims, w.pv[0], w.pv[1], w.pv[2], c,
-                       dxinv[0], dxinv[1], dxinv[2]);
+    hipLaunchKernelGGL(k_visc_pf, grid_for(bt, r1), dim3(BLK), 0, st, B, r1, sa.box_dt, w.prims, w.pv[0], w.pv[1], w.pv[2], c);
     Region rn = nodal();
-    hipLaunchKernelGGL(k_visc_add, grid_for(bt, rn), dim3(BLK), 0, st, B, rn, w.pv[0], w.pv[1], w.pv[2], w.nf[0],
+    hipLaunchKernelGGL(k_visc_add, grid_for(bt, rn), dim3(BLK), 0, st, B, rn, sa.box_dt, w.pv[0], w.pv[1], w.pv[2], w.nf[0],
                        w.nf[1], w.nf[2]);
   }
   Region r0 = grown(0);
-  hipLaunchKernelGGL(k_assemble, grid_for(bt, r0), dim3(BLK), 0, st, B, r0, w.nf[0], w.nf[1], w.nf[2], dxinv[0],
-                     dxinv[1], dxinv[2], sc, s1, sin, out);
+  hipLaunchKernelGGL(k_assemble, grid_for(bt, r0), dim3(BLK), 0, st, B, r0, sa.box_dt, w.nf[0], w.nf[1], w.nf[2], sa, s1,
+                     sin, out);
   return hipGetLastError();
 }
 

@@ -234,3 +234,56 @@ def compute_new_dt(est, dt_prev, cur_time, stop_time, first):
     if stop_time >= 0.0 and (cur_time + dt0) > (stop_time - eps):
         dt0 = stop_time - cur_time
     return dt0
+
+
+class Batch(Level):
+    """Independent single-box problems advanced together (one launch per
+    kernel for all of them), each with its own geometry and time step."""
+
+    def __init__(self, geoms, closures, scheme, ctx=None, device=0):
+        self.ctx = ctx if ctx is not None else Context(device)
+        self.geoms = list(geoms)
+        self.geom = self.geoms[0]
+        self.boxes = [(tuple(g.dom_lo), tuple(g.dom_hi)) for g in self.geoms]
+        self.n = None
+        self.closures = closures
+        self.scheme = scheme
+        arr = (capi.Geom * len(self.geoms))(*self.geoms)
+        h = ctypes.c_void_p()
+        call("amr_batch_create", self.ctx.h, len(self.geoms), arr, ctypes.byref(h))
+        self.h = h
+
+    def new_state(self, layout="boxes", ng=3, ncomp=5):
+        if layout != "boxes":
+            raise ValueError("a batch holds one array per problem")
+        return BatchArrays(self, ncomp=ncomp, ng=ng)
+
+    def advance_dt(self, S, dts):
+        arr = (ctypes.c_double * len(dts))(*dts)
+        call("amr_batch_advance", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, arr)
+
+    def est_time_steps(self, S, cfl):
+        out = (ctypes.c_double * len(self.boxes))()
+        call("amr_batch_est_dt", self.h, ctypes.byref(self.closures), S.fabs, cfl, out)
+        return list(out)
+
+
+class BatchArrays(StateArrays):
+    """One ghosted array per problem; states are given per problem."""
+
+    def __init__(self, batch, ncomp=5, ng=3):
+        super().__init__(batch, ncomp=ncomp, ng=ng, layout="boxes")
+
+    def upload_problems(self, states):
+        torch = _torch_mod()
+        g = self.ng
+        for u, t in zip(states, self.tensors):
+            t[:, g:t.shape[1] - g, g:t.shape[2] - g, g:t.shape[3] - g] = torch.from_numpy(np.ascontiguousarray(u)).to(t.device)
+        torch.cuda.synchronize()
+
+    def download_problems(self):
+        torch = _torch_mod()
+        self.level.ctx.synchronize()
+        torch.cuda.synchronize()
+        g = self.ng
+        return [t[:, g:t.shape[1] - g, g:t.shape[2] - g, g:t.shape[3] - g].cpu().numpy().copy() for t in self.tensors]

@@ -218,3 +218,18 @@ def riemann_batch_params(n_inst=512, seed=20261202, lmin=64, lmax=2048):
         rr, ur, pr = g.uniform(0.1, 1.0), g.uniform(-1.0, 1.0), g.uniform(0.1, 1.0)
         inst.append(dict(L=int(L), rho_l=rl, u_l=ul, p_l=pl, rho_r=rr, u_r=ur, p_r=pr))
     return inst
+
+
+def riemann_batch_problems(cls, inst, ny=4, nz=4):
+    """Geometry and initial state of every instance of the batched 1D
+    Riemann workload: L x 4 x 4 cells on [0,1] x [0,4/L]^2, split at x = 0.5,
+    outflow in x, periodic in y and z."""
+    geoms, states = [], []
+    for p in inst:
+        L = p["L"]
+        n = (L, ny, nz)
+        g = geom(n, (0.0, 0.0, 0.0), (1.0, ny / L, nz / L), (0, 1, 1), lo_bc=(2, 0, 0), hi_bc=(2, 0, 0))
+        geoms.append(g)
+        states.append(sod_state(cls, g, n, rho_l=p["rho_l"], p_l=p["p_l"], u_l=p["u_l"],
+                                rho_r=p["rho_r"], p_r=p["p_r"], u_r=p["u_r"], prob_hi_x=1.0))
+    return geoms, states

@@ -147,6 +147,24 @@ int amr_advance_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_sche
 int amr_est_dt(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, double cfl,
                double* dt_out);
 
+/* Batch of `nprob` independent problems advanced together: problem b is
+ * the single box geoms[b].dom_lo..dom_hi with its own cell size, periodicity
+ * and boundary codes.  Every entry point that takes a box table accepts a
+ * batch (one amr_fab_t per problem).  Replaces running many CNS problems
+ * one after another (no reference counterpart for the batching itself). */
+int amr_batch_create(amr_ctx_t ctx, int nprob, const amr_geom_t* geoms, amr_boxtab_t* out);
+
+/* One CNS::advance step of every problem with its own time step dt[b];
+ * problems with dt[b] == 0 are left untouched.  All problems share one
+ * launch per kernel. */
+int amr_batch_advance(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch,
+                      const amr_fab_t* state, const double* dt);
+
+/* Per-problem CFL time step cfl * min cns_estdt (src/cls/CNS_K.h:12-47) into
+ * dt_out[nprob].  Synchronous. */
+int amr_batch_est_dt(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, double cfl,
+                     double* dt_out);
+
 /* Tiling of the fused stage kernel for a scheme on this context:
  * (tx,ty,tz) = (0,0,0) automatic (default or autotuned), tx = -1 forces the
  * pass-structured kernels, otherwise one of the compiled shapes
