@@ -775,6 +775,7 @@ int finish_boxtab(amr_boxtab_s* bt) {
   if (e == hipSuccess) e = hipMalloc(&bt->d_box_dt, sizeof(double) * nbox);
   if (e == hipSuccess) e = hipHostMalloc(&bt->h_box_dt, sizeof(double) * nbox);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->dt_copied, hipEventDisableTiming);
+  if (e == hipSuccess) e = prepare_block_maps(bt->bt);
   if (e != hipSuccess) rc = fail(AMR_ERR_HIP, std::string("box table allocation: ") + hipGetErrorString(e));
   return rc;
 }
@@ -790,6 +791,7 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   for (void* p : bt->uw_tables) (void)hipFree(p);
   for (int w = 0; w < 2; ++w)
     if (bt->ws_mem[w]) (void)hipFree(bt->ws_mem[w]);
+  for (auto& m : bt->bt.block_maps) (void)hipFree(m.second.first);
   if (bt->bt.d_boxes) (void)hipFree(bt->bt.d_boxes);
   if (bt->d_partial) (void)hipFree(bt->d_partial);
   if (bt->d_result) (void)hipFree(bt->d_result);

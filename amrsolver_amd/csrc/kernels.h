@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <array>
+#include <map>
 #include <vector>
 
 #include "amrsolver_hip.h"
@@ -18,11 +20,19 @@ struct BoxDev {
   double dx[3];   // cell size (as given; not recomputed from dxi)
 };
 
+// Block -> (box, block within the box) map of one launch region, so that
+// launches over uneven boxes carry no empty blocks.
+struct BlockRef {
+  int box, blk;
+};
+
 // The level's box list on host and device.
 struct BoxTable {
   int nbox = 0;
   std::vector<BoxDev> h_boxes;
   BoxDev* d_boxes = nullptr;
+  // region (grow lo/hi per direction) -> device block map and its length
+  mutable std::map<std::array<int, 6>, std::pair<BlockRef*, int>> block_maps;
 };
 
 // Per-box workspace views of the unfused pass structure (device arrays of
@@ -64,6 +74,9 @@ inline void keep_coefficients(int order, double co[3]) {
   }
 }
 
+// Builds the block maps of every region the pass-structured launches use
+// (called once when a box table is created; launches only look them up).
+hipError_t prepare_block_maps(const BoxTable& bt);
 hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const Closures& c,
                                 const amr_scheme_t& s, const StageArgs& sa, const Fab* s1, const Fab* sin,
                                 const Fab* out, hipStream_t st);

@@ -32,15 +32,16 @@ __device__ __forceinline__ bool region_cell(const BoxDev& bx, const Region& r, i
 }
 
 #define REGION_LOOP_BEGIN                                                   \
-  const int b = blockIdx.y;                                                 \
+  const BlockRef br = bmap[blockIdx.x];                                     \
+  const int b = br.box;                                                     \
   if (box_dt != nullptr && !(box_dt[b] > 0.0)) return;                      \
   const BoxDev bx = boxes[b];                                               \
   int i, j, k;                                                              \
-  if (!region_cell(bx, reg, int64_t(blockIdx.x) * BLK + threadIdx.x, i, j, k)) return;
+  if (!region_cell(bx, reg, int64_t(br.blk) * BLK + threadIdx.x, i, j, k)) return;
 
 // ---------------------------------------------------------------- prims
 // CNS_advance.cpp:200-208: cons2prim on growntilebox(NGHOST)
-__global__ void __launch_bounds__(BLK) k_prims(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* st,
+__global__ void __launch_bounds__(BLK) k_prims(const BoxDev* boxes, const BlockRef* bmap, Region reg, const double* box_dt, const Fab* st,
                                                const Fab* pr, Closures c) {
   REGION_LOOP_BEGIN
   const Fab u = st[b], q = pr[b];
@@ -53,7 +54,7 @@ __global__ void __launch_bounds__(BLK) k_prims(const BoxDev* boxes, Region reg, 
 // ---------------------------------------------------------------- KEEP
 // CentralKEEP.h:40-145 on grownnodaltilebox(-1,0)
 template <int H>
-__global__ void __launch_bounds__(BLK) k_keep(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* pr,
+__global__ void __launch_bounds__(BLK) k_keep(const BoxDev* boxes, const BlockRef* bmap, Region reg, const double* box_dt, const Fab* pr,
                                               const Fab* nfx, const Fab* nfy, const Fab* nfz,
                                               Closures c, double c0, double c1, double c2) {
   REGION_LOOP_BEGIN
@@ -86,7 +87,7 @@ __global__ void __launch_bounds__(BLK) k_keep(const BoxDev* boxes, Region reg, c
 
 // ---------------------------------------------------------------- WENO5
 // High_resolution.h:83-121 cons2eulerflux_lambda on growntilebox(NGHOST)
-__global__ void __launch_bounds__(BLK) k_eulerflux(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* st,
+__global__ void __launch_bounds__(BLK) k_eulerflux(const BoxDev* boxes, const BlockRef* bmap, Region reg, const double* box_dt, const Fab* st,
                                                    const Fab* pfx, const Fab* pfy, const Fab* pfz,
                                                    const Fab* lam, Closures c) {
   REGION_LOOP_BEGIN
@@ -140,7 +141,7 @@ __device__ __forceinline__ double weno_face(int i, int j, int k, int n, const Fa
   return flx;
 }
 
-__global__ void __launch_bounds__(BLK) k_weno(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* st,
+__global__ void __launch_bounds__(BLK) k_weno(const BoxDev* boxes, const BlockRef* bmap, Region reg, const double* box_dt, const Fab* st,
                                               const Fab* pfx, const Fab* pfy, const Fab* pfz,
                                               const Fab* lam, const Fab* nfx, const Fab* nfy,
                                               const Fab* nfz) {
@@ -157,7 +158,7 @@ __global__ void __launch_bounds__(BLK) k_weno(const BoxDev* boxes, Region reg, c
 // ---------------------------------------------------------------- Riemann
 // Riemann.h:19-120 cns_slope_{x,y,z} on grow(bx,dir,1)
 template <int D>
-__global__ void __launch_bounds__(BLK) k_slope(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* pr,
+__global__ void __launch_bounds__(BLK) k_slope(const BoxDev* boxes, const BlockRef* bmap, Region reg, const double* box_dt, const Fab* pr,
                                                const Fab* slope, Closures c) {
   REGION_LOOP_BEGIN
   constexpr int DI = D == 0, DJ = D == 1, DK = D == 2;
@@ -190,7 +191,7 @@ __global__ void __launch_bounds__(BLK) k_slope(const BoxDev* boxes, Region reg, 
 // Riemann.h:251-337 cns_riemann_{x,y,z} on surroundingNodes(bx,dir).  The z
 // left state takes its sound speed from T at k+1 (Riemann.h:316).
 template <int D>
-__global__ void __launch_bounds__(BLK) k_riemann(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* pr,
+__global__ void __launch_bounds__(BLK) k_riemann(const BoxDev* boxes, const BlockRef* bmap, Region reg, const double* box_dt, const Fab* pr,
                                                  const Fab* slope, const Fab* nf, Closures c) {
   REGION_LOOP_BEGIN
   constexpr int DI = D == 0, DJ = D == 1, DK = D == 2;
@@ -230,7 +231,7 @@ __global__ void __launch_bounds__(BLK) k_riemann(const BoxDev* boxes, Region reg
 
 // ---------------------------------------------------------------- viscous
 // CNS_hydro_K.h:108-172 ViscousFluxes on growntilebox(1)
-__global__ void __launch_bounds__(BLK) k_visc_pf(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* pr,
+__global__ void __launch_bounds__(BLK) k_visc_pf(const BoxDev* boxes, const BlockRef* bmap, Region reg, const double* box_dt, const Fab* pr,
                                                  const Fab* pfx, const Fab* pfy, const Fab* pfz,
                                                  Closures c) {
   REGION_LOOP_BEGIN
@@ -280,7 +281,7 @@ __global__ void __launch_bounds__(BLK) k_visc_pf(const BoxDev* boxes, Region reg
 }
 
 // CNS_hydro_K.h:254-261 ViscousNumericalFluxes on grownnodaltilebox(-1,0)
-__global__ void __launch_bounds__(BLK) k_visc_add(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* pfx,
+__global__ void __launch_bounds__(BLK) k_visc_add(const BoxDev* boxes, const BlockRef* bmap, Region reg, const double* box_dt, const Fab* pfx,
                                                   const Fab* pfy, const Fab* pfz, const Fab* nfx,
                                                   const Fab* nfy, const Fab* nfz) {
   REGION_LOOP_BEGIN
@@ -295,7 +296,7 @@ __global__ void __launch_bounds__(BLK) k_visc_add(const BoxDev* boxes, Region re
 
 // ---------------------------------------------------------------- assembly
 // CNS_advance.cpp:376-391 plus the stage combination of CNS::advance
-__global__ void __launch_bounds__(BLK) k_assemble(const BoxDev* boxes, Region reg, const double* box_dt, const Fab* nfx,
+__global__ void __launch_bounds__(BLK) k_assemble(const BoxDev* boxes, const BlockRef* bmap, Region reg, const double* box_dt, const Fab* nfx,
                                                   const Fab* nfy, const Fab* nfz, StageArgs sa,
                                                   const Fab* s1, const Fab* sin, const Fab* out) {
   REGION_LOOP_BEGIN
@@ -317,79 +318,126 @@ __global__ void __launch_bounds__(BLK) k_assemble(const BoxDev* boxes, Region re
   }
 }
 
-dim3 grid_for(const BoxTable& bt, const Region& r) {
-  int64_t mx = 1;
+// Block map of region r over all boxes: block -> (box, block within box), so a
+// launch over uneven boxes carries no empty blocks.
+struct Launch {
+  const BlockRef* map;
+  dim3 grid;
+};
+
+std::array<int, 6> region_key(const Region& r) {
+  return {r.glo[0], r.glo[1], r.glo[2], r.ghi[0], r.ghi[1], r.ghi[2]};
+}
+
+hipError_t build_map(const BoxTable& bt, const Region& r) {
+  const auto key = region_key(r);
+  if (bt.block_maps.count(key)) return hipSuccess;
+  std::vector<BlockRef> h;
   for (int b = 0; b < bt.nbox; ++b) {
     int64_t n = 1;
-    for (int d = 0; d < 3; ++d) n *= int64_t(bt.h_boxes[b].hi[d] + r.ghi[d] - bt.h_boxes[b].lo[d] - r.glo[d] + 1);
-    mx = n > mx ? n : mx;
+    for (int d = 0; d < 3; ++d)
+      n *= int64_t(bt.h_boxes[b].hi[d] + r.ghi[d] - bt.h_boxes[b].lo[d] - r.glo[d] + 1);
+    const int nblk = int((n + BLK - 1) / BLK);
+    for (int k = 0; k < nblk; ++k) h.push_back(BlockRef{b, k});
   }
-  return dim3(unsigned((mx + BLK - 1) / BLK), unsigned(bt.nbox), 1);
+  BlockRef* d = nullptr;
+  hipError_t e = hipMalloc(&d, sizeof(BlockRef) * h.size());
+  if (e != hipSuccess) return e;
+  e = hipMemcpy(d, h.data(), sizeof(BlockRef) * h.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return e;
+  }
+  bt.block_maps.emplace(key, std::make_pair(d, int(h.size())));
+  return hipSuccess;
+}
+
+// Lookup only (no allocation), so launches can be recorded into a graph.
+Launch grid_for(const BoxTable& bt, const Region& r) {
+  auto it = bt.block_maps.find(region_key(r));
+  if (it == bt.block_maps.end()) return Launch{nullptr, dim3(0)};
+  return Launch{it->second.first, dim3(unsigned(it->second.second))};
 }
 
 Region grown(int g) { return Region{{-g, -g, -g}, {g, g, g}}; }
 Region nodal() { return Region{{0, 0, 0}, {1, 1, 1}}; }
+Region slope_region(int d) {
+  Region r{{0, 0, 0}, {0, 0, 0}};
+  r.glo[d] = -1;
+  r.ghi[d] = 1;
+  return r;
+}
+Region face_region(int d) {
+  Region r{{0, 0, 0}, {0, 0, 0}};
+  r.ghi[d] = 1;
+  return r;
+}
 
 }  // namespace
+
+hipError_t prepare_block_maps(const BoxTable& bt) {
+  const Region fixed[4] = {grown(NGHOST), nodal(), grown(1), grown(0)};
+  for (const Region& r : fixed) {
+    hipError_t e = build_map(bt, r);
+    if (e != hipSuccess) return e;
+  }
+  for (int d = 0; d < 3; ++d) {
+    hipError_t e = build_map(bt, slope_region(d));
+    if (e == hipSuccess) e = build_map(bt, face_region(d));
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
 
 // Flux phase + assembly + stage combination, reference pass structure.
 hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const Closures& c,
                                 const amr_scheme_t& s, const StageArgs& sa, const Fab* s1, const Fab* sin,
                                 const Fab* out, hipStream_t st) {
   const BoxDev* B = bt.d_boxes;
-  {
-    Region r = grown(NGHOST);
-    hipLaunchKernelGGL(k_prims, grid_for(bt, r), dim3(BLK), 0, st, B, r, sa.box_dt, sin, w.prims, c);
-  }
+  const double* bdt = sa.box_dt;
+#define LAUNCH(KERNEL, REG, ...)                                                                    \
+  do {                                                                                             \
+    Launch L_ = grid_for(bt, REG);                                                                 \
+    if (!L_.map) return hipErrorNotInitialized;                                                      \
+    hipLaunchKernelGGL(KERNEL, L_.grid, dim3(BLK), 0, st, B, L_.map, REG, bdt, __VA_ARGS__);     \
+  } while (0)
+  const Region g3 = grown(NGHOST), nod = nodal(), g1 = grown(1), g0 = grown(0);
+  LAUNCH(k_prims, g3, sin, w.prims, c);
   if (s.rhs_euler) {
     if (s.flux_euler == 1) {
       double co[3];
       keep_coefficients(s.order_keep, co);
-      Region r = nodal();
-      dim3 g = grid_for(bt, r);
       if (s.order_keep == 6)
-        hipLaunchKernelGGL(k_keep<3>, g, dim3(BLK), 0, st, B, r, sa.box_dt, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
+        LAUNCH(k_keep<3>, nod, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
       else if (s.order_keep == 4)
-        hipLaunchKernelGGL(k_keep<2>, g, dim3(BLK), 0, st, B, r, sa.box_dt, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
+        LAUNCH(k_keep<2>, nod, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
       else
-        hipLaunchKernelGGL(k_keep<1>, g, dim3(BLK), 0, st, B, r, sa.box_dt, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
+        LAUNCH(k_keep<1>, nod, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
     } else if (s.flux_euler == 2) {
-      Region r = grown(NGHOST);
-      hipLaunchKernelGGL(k_eulerflux, grid_for(bt, r), dim3(BLK), 0, st, B, r, sa.box_dt, sin, w.pe[0], w.pe[1], w.pe[2], w.lam, c);
-      Region rn = nodal();
-      hipLaunchKernelGGL(k_weno, grid_for(bt, rn), dim3(BLK), 0, st, B, rn, sa.box_dt, sin, w.pe[0], w.pe[1], w.pe[2], w.lam,
-                         w.nf[0], w.nf[1], w.nf[2]);
+      LAUNCH(k_eulerflux, g3, sin, w.pe[0], w.pe[1], w.pe[2], w.lam, c);
+      LAUNCH(k_weno, nod, sin, w.pe[0], w.pe[1], w.pe[2], w.lam, w.nf[0], w.nf[1], w.nf[2]);
     } else {
       for (int d = 0; d < 3; ++d) {
-        Region rs{{0, 0, 0}, {0, 0, 0}};
-        rs.glo[d] = -1;
-        rs.ghi[d] = 1;
-        Region rf{{0, 0, 0}, {0, 0, 0}};
-        rf.ghi[d] = 1;
-        dim3 gs = grid_for(bt, rs), gf = grid_for(bt, rf);
+        const Region rs = slope_region(d), rf = face_region(d);
         if (d == 0) {
-          hipLaunchKernelGGL(k_slope<0>, gs, dim3(BLK), 0, st, B, rs, sa.box_dt, w.prims, w.slope, c);
-          hipLaunchKernelGGL(k_riemann<0>, gf, dim3(BLK), 0, st, B, rf, sa.box_dt, w.prims, w.slope, w.nf[0], c);
+          LAUNCH(k_slope<0>, rs, w.prims, w.slope, c);
+          LAUNCH(k_riemann<0>, rf, w.prims, w.slope, w.nf[0], c);
         } else if (d == 1) {
-          hipLaunchKernelGGL(k_slope<1>, gs, dim3(BLK), 0, st, B, rs, sa.box_dt, w.prims, w.slope, c);
-          hipLaunchKernelGGL(k_riemann<1>, gf, dim3(BLK), 0, st, B, rf, sa.box_dt, w.prims, w.slope, w.nf[1], c);
+          LAUNCH(k_slope<1>, rs, w.prims, w.slope, c);
+          LAUNCH(k_riemann<1>, rf, w.prims, w.slope, w.nf[1], c);
         } else {
-          hipLaunchKernelGGL(k_slope<2>, gs, dim3(BLK), 0, st, B, rs, sa.box_dt, w.prims, w.slope, c);
-          hipLaunchKernelGGL(k_riemann<2>, gf, dim3(BLK), 0, st, B, rf, sa.box_dt, w.prims, w.slope, w.nf[2], c);
+          LAUNCH(k_slope<2>, rs, w.prims, w.slope, c);
+          LAUNCH(k_riemann<2>, rf, w.prims, w.slope, w.nf[2], c);
         }
       }
     }
   }
   if (s.rhs_visc) {
-    Region r1 = grown(1);
-    hipLaunchKernelGGL(k_visc_pf, grid_for(bt, r1), dim3(BLK), 0, st, B, r1, sa.box_dt, w.prims, w.pv[0], w.pv[1], w.pv[2], c);
-    Region rn = nodal();
-    hipLaunchKernelGGL(k_visc_add, grid_for(bt, rn), dim3(BLK), 0, st, B, rn, sa.box_dt, w.pv[0], w.pv[1], w.pv[2], w.nf[0],
-                       w.nf[1], w.nf[2]);
+    LAUNCH(k_visc_pf, g1, w.prims, w.pv[0], w.pv[1], w.pv[2], c);
+    LAUNCH(k_visc_add, nod, w.pv[0], w.pv[1], w.pv[2], w.nf[0], w.nf[1], w.nf[2]);
   }
-  Region r0 = grown(0);
-  hipLaunchKernelGGL(k_assemble, grid_for(bt, r0), dim3(BLK), 0, st, B, r0, sa.box_dt, w.nf[0], w.nf[1], w.nf[2], sa, s1,
-                     sin, out);
+  LAUNCH(k_assemble, g0, w.nf[0], w.nf[1], w.nf[2], sa, s1, sin, out);
+#undef LAUNCH
   return hipGetLastError();
 }
 

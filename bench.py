@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run plain device copies of this many bytes (profiler calibration)")
+    ap.add_argument("--workload", type=str, default="tgv",
+                    choices=["tgv", "sod-w", "sod-r", "riemann-batch-w", "riemann-batch-r"])
     ap.add_argument("--tiling", type=str, default="auto",
                     help="fused-kernel tiling: auto (autotune), default, or TX,TY,TZ")
     ap.add_argument("--traffic", type=str, default=os.path.join(ROOT, "profiles", "traffic_tgv256_keep4v.json"))
@@ -83,6 +85,116 @@ def copy_calibration(nbytes):
     return 0
 
 
+def base_line(metric_value, elapsed_ms_per_step, steps, warmup, workload, extra):
+    out = {
+        "metric": "FP64 cell-updates/sec per RK stage",
+        "value": metric_value,
+        "unit": "cell-stage/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": elapsed_ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": workload},
+    }
+    out.update(extra)
+    return out
+
+
+def run_sod(a, flux):
+    """Sod tube 256x4x4 (4 boxes of 64x4x4), outflow in x, periodic y/z,
+    SSPRK(3,3), CFL 0.3 with the 1.1x growth cap, to t = 0.2."""
+    import torch
+    from amrsolver_amd import level as LV
+    from amrsolver_amd import problems as P
+    n = (256, 4, 4)
+    cls = P.closures_reference()
+    g = P.geom(n, (0, 0, 0), (1.0, 4 / 256, 4 / 256), (0, 1, 1), lo_bc=(2, 0, 0), hi_bc=(2, 0, 0))
+    boxes = P.decompose(n, (64, 4, 4))
+    sch = P.scheme(flux_euler=flux, visc=False)
+    lev = LV.Level(g, boxes, cls, sch)
+    S = lev.new_state(layout="boxes")
+    u0 = P.sod_state(cls, g, n)
+
+    def run(max_steps):
+        S.upload(u0)
+        t, dt, k = 0.0, None, 0
+        while t < 0.2 - 1e-14 and k < max_steps:
+            dt = LV.compute_new_dt(lev.est_time_step(S, 0.3), dt, t, 0.2, k == 0)
+            lev.advance(S, t, dt)
+            t += dt
+            k += 1
+        torch.cuda.synchronize()
+        return k
+
+    run(a.warmup)
+    t0 = time.perf_counter()
+    k = run(10 ** 9)
+    el = time.perf_counter() - t0
+    name = "Sod-W (WENO5-JS + LF)" if flux == 2 else "Sod-R (MUSCL + Riemann)"
+    return base_line(n[0] * n[1] * n[2] * 3 * k / el, el / k * 1e3, k, a.warmup,
+                     f"{name} 256x4x4, 4 boxes, SSPRK(3,3), CFL 0.3, t=0.2", {"data": "synthetic (Sod initial state)"})
+
+
+def run_riemann_batch(a, flux):
+    """512 independent 1D Riemann problems, L uniform in [64, 2048] cells (x4x4),
+    seed 20261202, SSPRK(3,3), CFL 0.3 per instance, to t = 0.15; one launch
+    per kernel for all instances."""
+    import torch
+    from amrsolver_amd import level as LV
+    from amrsolver_amd import problems as P
+    cls = P.closures_reference()
+    inst = P.riemann_batch_params()
+    geoms, states = P.riemann_batch_problems(cls, inst)
+    sch = P.scheme(flux_euler=flux, visc=False)
+    bat = LV.Batch(geoms, cls, sch)
+    S = bat.new_state()
+    cells = [p["L"] * 16 for p in inst]
+    stop = 0.15
+
+    def run(max_steps):
+        S.upload_problems(states)
+        nb = len(inst)
+        t = [0.0] * nb
+        dtp = [None] * nb
+        done = 0
+        k = 0
+        while k < max_steps:
+            est = bat.est_time_steps(S, 0.3)
+            dts = []
+            for b in range(nb):
+                if t[b] >= stop - 1e-14:
+                    dts.append(0.0)
+                    continue
+                d = LV.compute_new_dt(est[b], dtp[b], t[b], stop, dtp[b] is None)
+                dtp[b] = d
+                dts.append(d)
+            if not any(dts):
+                break
+            bat.advance_dt(S, dts)
+            for b in range(nb):
+                if dts[b] > 0.0:
+                    t[b] += dts[b]
+                    done += cells[b] * 3
+            k += 1
+        torch.cuda.synchronize()
+        return k, done
+
+    run(a.warmup)
+    t0 = time.perf_counter()
+    k, done = run(10 ** 9)
+    el = time.perf_counter() - t0
+    name = "WENO5-JS + LF" if flux == 2 else "MUSCL + Riemann"
+    return base_line(done / el, el / k * 1e3, k, a.warmup,
+                     f"batched 1D Riemann, 512 instances L in [64,2048] x4x4, {name}, SSPRK(3,3), CFL 0.3, t=0.15",
+                     {"data": "synthetic (seed 20261202, SplitMix64)",
+                      "cell_stages": done, "batch_steps": k})
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -91,6 +203,12 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
         return 2
+    if a.workload != "tgv":
+        fn = {"sod-w": lambda: run_sod(a, 2), "sod-r": lambda: run_sod(a, 0),
+              "riemann-batch-w": lambda: run_riemann_batch(a, 2),
+              "riemann-batch-r": lambda: run_riemann_batch(a, 0)}[a.workload]
+        print(json.dumps(fn()))
+        return 0
     import torch
     from amrsolver_amd import capi
     from amrsolver_amd import level as LV
