@@ -35,7 +35,7 @@ ALG_BYTES = [80.0, 120.0, 120.0]
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=N)
     ap.add_argument("--no-cpu", action="store_true")
@@ -52,7 +52,7 @@ def parse():
 def cpu_baseline():
     """The CPU oracle (a restatement of the reference's unfused passes) on a
     bounded sample of the same workload: TGV 128^3 in 8 boxes of 64^3, same
-    closure and scheme, 3 SSPRK(3,3) steps, one thread."""
+    closure and scheme, 12 SSPRK(3,3) steps (about 10 s), one thread."""
     from amrsolver_amd import problems as P
     from oracle import oracle as O
     n = (128, 128, 128)
@@ -61,7 +61,7 @@ def cpu_baseline():
     sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
     ol = O.OracleLevel(g, P.decompose(n, BOX))
     ol.set_state(P.tgv_state(cls, g, n))
-    steps = 3
+    steps = 12
     t0 = time.perf_counter()
     for _ in range(steps):
         ol.advance(cls, sch, DT, nthreads=1)
