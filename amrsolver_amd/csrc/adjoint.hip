@@ -1,0 +1,503 @@
+// Reverse mode of the KEEP + viscous right-hand side and of the SSPRK stage
+// update: hand-written vector-Jacobian products over the box table.
+//
+// For a box b with ghost-filled input S (ghost width 3), the forward is
+//   q = cons2prim(S)                       on grow(b,3)
+//   F_d(f) = sum_l sum_m co_l P_d(q(f+m), q(f+m-l))    KEEP pairs, faces f
+//          + 0.5 (pf_d(f-1) + pf_d(f))    viscous cell fluxes on grow(b,1)
+//   dSdt(c) = sum_d dxi_d (F_d(c) - F_d(c+e_d))        valid cells
+// (src/rhs/CNS_hydro_K.h:15-34,108-172,254-261; src/rhs/CentralKEEP.h:40-145;
+// src/tim/CNS_advance.cpp:375-391).  Given lam = adjoint of dSdt on the valid
+// cells, the kernels here compute G = (d dSdt / d S)^T lam on grow(b,3) by
+// gathering, per cell, every term that reads that cell:
+//   * the face adjoint is dxi_d (lam(f) - lam(f-1)), lam = 0 outside the box,
+//     so the weight of pair (a, a-l) telescopes to co_l dxi_d (lam(a) - lam(a-l));
+//   * viscous: a pre-pass stores per cell of grow(b,1) the adjoints of the
+//     velocity/temperature differences and of the cell's own values;
+//   * cons2prim is transposed per cell.
+// The ghost-fill is then transposed (BC tasks in reverse direction order,
+// then the copy tasks) with atomic accumulation into the source cells, and
+// the stage combination is transposed on the valid cells.
+#include "cns_point.h"
+#include "kernels.h"
+#include "region.h"
+
+namespace amr {
+
+namespace {
+
+constexpr int NE = 16;  // L_u, L_v, L_w, L_T, then (E_u, E_v, E_w, E_T) for x, y, z
+
+__device__ __forceinline__ bool in_box(const BoxDev& bx, int g, int i, int j, int k) {
+  return i >= bx.lo[0] - g && i <= bx.hi[0] + g && j >= bx.lo[1] - g && j <= bx.hi[1] + g &&
+         k >= bx.lo[2] - g && k <= bx.hi[2] + g;
+}
+
+// adjoint of dSdt at (i,j,k): zero outside the valid box
+__device__ __forceinline__ void lam_at(const Fab& L, const BoxDev& bx, int i, int j, int k, double v[5]) {
+  if (in_box(bx, 0, i, j, k)) {
+#pragma unroll
+    for (int n = 0; n < NCONS; ++n) v[n] = L(i, j, k, n);
+  } else {
+#pragma unroll
+    for (int n = 0; n < NCONS; ++n) v[n] = 0.0;
+  }
+}
+
+// d mu / dT and d k / dT of the closures (Sutherland: mu (1.5/T - 1/(S+T)))
+__device__ __forceinline__ void transport_derivs(const Closures& c, double T, double mu, double k, double& dmu,
+                                                 double& dk) {
+  dmu = c.visc_law == 0 ? 0.0 : mu * (1.5 / T - 1.0 / (c.Tvisc_ref + T));
+  if (c.cond_law == 0) dk = 0.0;
+  else if (c.cond_law == 1) dk = k * (1.5 / T - 1.0 / (c.Tcond_ref + T));
+  else dk = dmu * c.cp / c.Pr;
+}
+
+// Reverse of one KEEP pair in direction NRM (keep_pair): w[5] is the weight
+// of the pair's five flux components; adds (dP/dq1)^T w to g1 (G1) and
+// (dP/dq2)^T w to g2 (G2), q = (rho, u, v, w, T, p), ie = cv T.
+// With A = (rho1+rho2)(un1+un2) the pair is
+//   P.w = A Gs + 0.5 (p1+p2) w[1+NRM] + 0.5 w[4] (p1 un2 + p2 un1),
+//   Gs = w0/4 + sum_t w[1+t] (ut1+ut2)/8 + w[4] (ke/4 + (ie1+ie2)/8).
+template <int NRM, bool G1, bool G2>
+__device__ __forceinline__ void keep_pair_vjp(const double w[5], const double q1[6], const double q2[6], double cv,
+                                              double g1[6], double g2[6]) {
+  const double R = q1[QRHO] + q2[QRHO];
+  const double un1 = q1[QU + NRM], un2 = q2[QU + NRM];
+  const double U = un1 + un2;
+  const double A = R * U;
+  const double ke = 0.5 * (q1[QU] * q2[QU] + q1[QV] * q2[QV] + q1[QW] * q2[QW]);
+  const double we = w[UET];
+  double Gs = 0.25 * w[URHO] + we * (0.25 * ke + 0.125 * cv * (q1[QT] + q2[QT]));
+#pragma unroll
+  for (int t = 0; t < 3; ++t) Gs += 0.125 * w[UMX + t] * (q1[QU + t] + q2[QU + t]);
+  const double A8 = 0.125 * A;
+  if (G1) {
+    g1[QRHO] += U * Gs;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) g1[QU + t] += A8 * (w[UMX + t] + we * q2[QU + t]);
+    g1[QU + NRM] += R * Gs + 0.5 * we * q2[QPRES];
+    g1[QT] += A8 * we * cv;
+    g1[QPRES] += 0.5 * (w[UMX + NRM] + we * un2);
+  }
+  if (G2) {
+    g2[QRHO] += U * Gs;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) g2[QU + t] += A8 * (w[UMX + t] + we * q1[QU + t]);
+    g2[QU + NRM] += R * Gs + 0.5 * we * q1[QPRES];
+    g2[QT] += A8 * we * cv;
+    g2[QPRES] += 0.5 * (w[UMX + NRM] + we * un1);
+  }
+}
+
+__device__ __forceinline__ void load_q(const Fab& q, int i, int j, int k, double v[6]) {
+#pragma unroll
+  for (int n = 0; n < NPRIM; ++n) v[n] = q(i, j, k, n);
+}
+
+// KEEP gather of direction D into g for cell (i,j,k) with primitives qc.
+template <int H, int D>
+__device__ __forceinline__ void adj_keep_dir(const Fab& q, const Fab& L, const BoxDev& bx, int i, int j, int k,
+                                             const double qc[6], const double co[3], double dxi, double cv,
+                                             double g[6]) {
+  const int ci[3] = {i, j, k};
+#pragma unroll
+  for (int t = 0; t < 3; ++t)
+    if (t != D && (ci[t] < bx.lo[t] || ci[t] > bx.hi[t])) return;  // every weight is zero
+  const int x = ci[D], lo = bx.lo[D], hi = bx.hi[D];
+  const int di = D == 0, dj = D == 1, dk = D == 2;
+  double lc[5];
+  lam_at(L, bx, i, j, k, lc);
+#pragma unroll
+  for (int l = 1; l <= H; ++l) {
+    const double cw = co[l - 1] * dxi;
+    if (x >= lo && x <= hi + l) {  // cell 1 of the pair (c, c-l)
+      double lm[5], w[5], qm[6];
+      lam_at(L, bx, i - l * di, j - l * dj, k - l * dk, lm);
+#pragma unroll
+      for (int n = 0; n < NCONS; ++n) w[n] = cw * (lc[n] - lm[n]);
+      load_q(q, i - l * di, j - l * dj, k - l * dk, qm);
+      keep_pair_vjp<D, true, false>(w, qc, qm, cv, g, nullptr);
+    }
+    if (x >= lo - l && x <= hi) {  // cell 2 of the pair (c+l, c)
+      double lp[5], w[5], qp[6];
+      lam_at(L, bx, i + l * di, j + l * dj, k + l * dk, lp);
+#pragma unroll
+      for (int n = 0; n < NCONS; ++n) w[n] = cw * (lp[n] - lc[n]);
+      load_q(q, i + l * di, j + l * dj, k + l * dk, qp);
+      keep_pair_vjp<D, false, true>(w, qp, qc, cv, nullptr, g);
+    }
+  }
+}
+
+// Viscous pre-pass on grow(b,1): the adjoint of the cell fluxes pf_d(j) is
+// 0.5 dxi_d (lam(j+e_d) - lam(j-e_d)); store the adjoints of the central
+// differences (times their 0.5 dxi factor) and of the cell's own u, v, w, T.
+__global__ void __launch_bounds__(BLK) k_adj_visc(const BoxDev* boxes, const BlockRef* bmap, Region reg,
+                                                  const double* box_dt, const Fab* pr, const Fab* lam,
+                                                  const Fab* ew, Closures c) {
+  REGION_LOOP_BEGIN
+  const double dx0 = bx.dxi[0], dx1 = bx.dxi[1], dx2 = bx.dxi[2];
+  const Fab q = pr[b], L = lam[b], E = ew[b];
+  double a[5], bb[5], cc[5], lp[5], lm[5];
+  lam_at(L, bx, i + 1, j, k, lp);
+  lam_at(L, bx, i - 1, j, k, lm);
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) a[n] = 0.5 * dx0 * (lp[n] - lm[n]);
+  lam_at(L, bx, i, j + 1, k, lp);
+  lam_at(L, bx, i, j - 1, k, lm);
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) bb[n] = 0.5 * dx1 * (lp[n] - lm[n]);
+  lam_at(L, bx, i, j, k + 1, lp);
+  lam_at(L, bx, i, j, k - 1, lm);
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) cc[n] = 0.5 * dx2 * (lp[n] - lm[n]);
+  // forward values at the cell (as in the viscous flux kernel)
+  const double ux = q(i, j, k, QU), uy = q(i, j, k, QV), uz = q(i, j, k, QW), T = q(i, j, k, QT);
+  const double dudx = (q(i + 1, j, k, QU) - q(i - 1, j, k, QU)) * 0.5 * dx0;
+  const double dvdx = (q(i + 1, j, k, QV) - q(i - 1, j, k, QV)) * 0.5 * dx0;
+  const double dwdx = (q(i + 1, j, k, QW) - q(i - 1, j, k, QW)) * 0.5 * dx0;
+  const double dTdx = (q(i + 1, j, k, QT) - q(i - 1, j, k, QT)) * 0.5 * dx0;
+  const double dudy = (q(i, j + 1, k, QU) - q(i, j - 1, k, QU)) * 0.5 * dx1;
+  const double dvdy = (q(i, j + 1, k, QV) - q(i, j - 1, k, QV)) * 0.5 * dx1;
+  const double dwdy = (q(i, j + 1, k, QW) - q(i, j - 1, k, QW)) * 0.5 * dx1;
+  const double dTdy = (q(i, j + 1, k, QT) - q(i, j - 1, k, QT)) * 0.5 * dx1;
+  const double dudz = (q(i, j, k + 1, QU) - q(i, j, k - 1, QU)) * 0.5 * dx2;
+  const double dvdz = (q(i, j, k + 1, QV) - q(i, j, k - 1, QV)) * 0.5 * dx2;
+  const double dwdz = (q(i, j, k + 1, QW) - q(i, j, k - 1, QW)) * 0.5 * dx2;
+  const double dTdz = (q(i, j, k + 1, QT) - q(i, j, k - 1, QT)) * 0.5 * dx2;
+  const double div = dudx + dvdy + dwdz;
+  const double mu = visc(c, T);
+  const double kk = cond(c, T, mu);
+  const double r1_3 = 1.0 / 3.0;
+  const double sxx = 2.0 * (dudx - r1_3 * div), syy = 2.0 * (dvdy - r1_3 * div), szz = 2.0 * (dwdz - r1_3 * div);
+  const double sxy = dudy + dvdx, sxz = dudz + dwdx, syz = dvdz + dwdy;
+  const double txx = mu * sxx, tyy = mu * syy, tzz = mu * szz, txy = mu * sxy, txz = mu * sxz, tyz = mu * syz;
+  // adjoints of the stresses
+  const double gxx = -a[UMX] - a[UET] * ux;
+  const double gxy = -a[UMY] - a[UET] * uy - bb[UMX] - bb[UET] * ux;
+  const double gxz = -a[UMZ] - a[UET] * uz - cc[UMX] - cc[UET] * ux;
+  const double gyy = -bb[UMY] - bb[UET] * uy;
+  const double gyz = -bb[UMZ] - bb[UET] * uz - cc[UMY] - cc[UET] * uy;
+  const double gzz = -cc[UMZ] - cc[UET] * uz;
+  const double s3 = (gxx + gyy + gzz) * r1_3;
+  const double Dux = 2.0 * mu * (gxx - s3), Dvy = 2.0 * mu * (gyy - s3), Dwz = 2.0 * mu * (gzz - s3);
+  const double Dxy = mu * gxy, Dxz = mu * gxz, Dyz = mu * gyz;
+  // own-value adjoints: u, v, w through the work terms; T through mu and k
+  const double Lu = -a[UET] * txx - bb[UET] * txy - cc[UET] * txz;
+  const double Lv = -a[UET] * txy - bb[UET] * tyy - cc[UET] * tyz;
+  const double Lw = -a[UET] * txz - bb[UET] * tyz - cc[UET] * tzz;
+  const double gmu = gxx * sxx + gyy * syy + gzz * szz + gxy * sxy + gxz * sxz + gyz * syz;
+  const double gk = -(a[UET] * dTdx + bb[UET] * dTdy + cc[UET] * dTdz);
+  double dmu, dk;
+  transport_derivs(c, T, mu, kk, dmu, dk);
+  const double LT = gmu * dmu + gk * dk;
+  const double hx = 0.5 * dx0, hy = 0.5 * dx1, hz = 0.5 * dx2;
+  E(i, j, k, 0) = Lu;
+  E(i, j, k, 1) = Lv;
+  E(i, j, k, 2) = Lw;
+  E(i, j, k, 3) = LT;
+  E(i, j, k, 4) = hx * Dux;
+  E(i, j, k, 5) = hx * Dxy;
+  E(i, j, k, 6) = hx * Dxz;
+  E(i, j, k, 7) = hx * (-kk * a[UET]);
+  E(i, j, k, 8) = hy * Dxy;
+  E(i, j, k, 9) = hy * Dvy;
+  E(i, j, k, 10) = hy * Dyz;
+  E(i, j, k, 11) = hy * (-kk * bb[UET]);
+  E(i, j, k, 12) = hz * Dxz;
+  E(i, j, k, 13) = hz * Dyz;
+  E(i, j, k, 14) = hz * Dwz;
+  E(i, j, k, 15) = hz * (-kk * cc[UET]);
+}
+
+// Per cell of grow(b,3): adjoint of the primitives (KEEP pairs + viscous
+// terms), transposed through cons2prim into G.
+template <int H, bool EULER, bool VISC>
+__global__ void __launch_bounds__(BLK) k_adj_cell(const BoxDev* boxes, const BlockRef* bmap, Region reg,
+                                                  const double* box_dt, const Fab* pr, const Fab* lam,
+                                                  const Fab* ew, const Fab* gw, Closures c, double co0,
+                                                  double co1, double co2) {
+  REGION_LOOP_BEGIN
+  const Fab q = pr[b], G = gw[b];
+  double qc[6];
+  load_q(q, i, j, k, qc);
+  double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (EULER) {
+    const Fab L = lam[b];
+    const double co[3] = {co0, co1, co2};
+    adj_keep_dir<H, 0>(q, L, bx, i, j, k, qc, co, bx.dxi[0], c.cv, g);
+    adj_keep_dir<H, 1>(q, L, bx, i, j, k, qc, co, bx.dxi[1], c.cv, g);
+    adj_keep_dir<H, 2>(q, L, bx, i, j, k, qc, co, bx.dxi[2], c.cv, g);
+  }
+  if (VISC) {
+    const Fab E = ew[b];
+    if (in_box(bx, 1, i, j, k)) {
+      g[QU] += E(i, j, k, 0);
+      g[QV] += E(i, j, k, 1);
+      g[QW] += E(i, j, k, 2);
+      g[QT] += E(i, j, k, 3);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int di = d == 0, dj = d == 1, dk = d == 2;
+      if (in_box(bx, 1, i - di, j - dj, k - dk)) {
+        g[QU] += E(i - di, j - dj, k - dk, 4 + 4 * d);
+        g[QV] += E(i - di, j - dj, k - dk, 5 + 4 * d);
+        g[QW] += E(i - di, j - dj, k - dk, 6 + 4 * d);
+        g[QT] += E(i - di, j - dj, k - dk, 7 + 4 * d);
+      }
+      if (in_box(bx, 1, i + di, j + dj, k + dk)) {
+        g[QU] -= E(i + di, j + dj, k + dk, 4 + 4 * d);
+        g[QV] -= E(i + di, j + dj, k + dk, 5 + 4 * d);
+        g[QW] -= E(i + di, j + dj, k + dk, 6 + 4 * d);
+        g[QT] -= E(i + di, j + dj, k + dk, 7 + 4 * d);
+      }
+    }
+  }
+  // cons2prim transposed: T = p/(rho R), p = (gamma-1)(E - |m|^2/(2 rho)), v = m/rho
+  const double rinv = 1.0 / qc[QRHO];
+  const double ux = qc[QU], uy = qc[QV], uz = qc[QW];
+  const double gp = g[QPRES] + g[QT] * rinv / c.Rspec;
+  const double gr = g[QRHO] - g[QT] * qc[QT] * rinv;
+  G(i, j, k, URHO) = gr + c.gm1 * 0.5 * (ux * ux + uy * uy + uz * uz) * gp -
+                     (g[QU] * ux + g[QV] * uy + g[QW] * uz) * rinv;
+  G(i, j, k, UMX) = g[QU] * rinv - c.gm1 * ux * gp;
+  G(i, j, k, UMY) = g[QV] * rinv - c.gm1 * uy * gp;
+  G(i, j, k, UMZ) = g[QW] * rinv - c.gm1 * uz * gp;
+  G(i, j, k, UET) = c.gm1 * gp;
+}
+
+constexpr int FB = 256;
+
+template <class T>
+__device__ __forceinline__ int find_task(const T* tasks, int ntask, int64_t t) {
+  int lo = 0, hi = ntask - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].start <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Transposed physical-BC fill: the ghost's adjoint moves into its source
+// cell (foextrap: boundary cell; reflect: mirror, negated for odd).  The
+// ghost is cleared, because the forward overwrote it.
+__global__ void __launch_bounds__(FB) k_fold_bc(const BcTask* tasks, int ntask, int64_t ncell, const Fab* fabs) {
+  int64_t t = int64_t(blockIdx.x) * FB + threadIdx.x;
+  if (t >= ncell) return;
+  const BcTask& T = tasks[find_task(tasks, ntask, t)];
+  int64_t r = t - T.start;
+  int nx = T.hi[0] - T.lo[0] + 1, ny = T.hi[1] - T.lo[1] + 1;
+  int c[3];
+  c[0] = T.lo[0] + int(r % nx);
+  int64_t r2 = r / nx;
+  c[1] = T.lo[1] + int(r2 % ny);
+  c[2] = T.lo[2] + int(r2 / ny);
+  const Fab f = fabs[T.box];
+  const int cd = c[T.dir];
+  for (int n = 0; n < NCONS; ++n) {
+    int ty = T.type[n];
+    if (ty == 0) continue;
+    int s[3] = {c[0], c[1], c[2]};
+    if (ty == 2) s[T.dir] = T.bnd;
+    else s[T.dir] = T.side == 0 ? 2 * T.bnd - 1 - cd : 2 * T.bnd + 1 - cd;
+    double v = f(c[0], c[1], c[2], n);
+    f(c[0], c[1], c[2], n) = 0.0;
+    atomicAdd(&f(s[0], s[1], s[2], n), ty == 4 ? -v : v);
+  }
+}
+
+// Transposed same-level / periodic copies: ghost adjoint of `dst` added to
+// the valid source cell of `src`.
+__global__ void __launch_bounds__(FB) k_fold_copy(const CopyTask* tasks, int ntask, int64_t ncell,
+                                                  const Fab* fabs) {
+  int64_t t = int64_t(blockIdx.x) * FB + threadIdx.x;
+  if (t >= ncell) return;
+  const CopyTask& T = tasks[find_task(tasks, ntask, t)];
+  int64_t r = t - T.start;
+  int nx = T.hi[0] - T.lo[0] + 1, ny = T.hi[1] - T.lo[1] + 1;
+  int i = T.lo[0] + int(r % nx);
+  int64_t r2 = r / nx;
+  int j = T.lo[1] + int(r2 % ny);
+  int k = T.lo[2] + int(r2 / ny);
+  const Fab d = fabs[T.dst], s = fabs[T.src];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) atomicAdd(&s(i - T.sh[0], j - T.sh[1], k - T.sh[2], n), d(i, j, k, n));
+}
+
+// Stage combination transposed on the valid cells:
+//   mode 0  out = a s1 + b (c sin + d L):  lam_sin = b (c lam_out + d G)
+//   mode 1  out = a s1 + c sin + d L:      lam_sin = c lam_out + d G
+//   mode 2  out = L:                       lam_sin = G
+// lam_sin is assigned or accumulated; lam_s1 += a lam_out (modes 0, 1).
+__global__ void __launch_bounds__(BLK) k_adj_combine(const BoxDev* boxes, const BlockRef* bmap, Region reg,
+                                                     const double* box_dt, StageArgs sa, const Fab* gw,
+                                                     const Fab* lam_out, const Fab* lam_s1, const Fab* lam_sin,
+                                                     int accumulate) {
+  REGION_LOOP_BEGIN
+  const StageCoef sc = box_stage(sa, b);
+  const Fab G = gw[b], Lo = lam_out[b], Li = lam_sin[b];
+  const bool s1 = lam_s1 != nullptr && sc.mode != 2 && sc.a != 0.0;
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) {
+    const double lo = Lo(i, j, k, n), gv = G(i, j, k, n);
+    double v;
+    if (sc.mode == 0) v = sc.b * (sc.c * lo + sc.d * gv);
+    else if (sc.mode == 1) v = sc.c * lo + sc.d * gv;
+    else v = gv;
+    if (accumulate) v = Li(i, j, k, n) + v;
+    Li(i, j, k, n) = v;
+    if (s1) {
+      const Fab S = lam_s1[b];
+      S(i, j, k, n) = S(i, j, k, n) + sc.a * lo;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(BLK) k_zero_valid(const BoxDev* boxes, const BlockRef* bmap, Region reg,
+                                                    const double* box_dt, const Fab* f) {
+  REGION_LOOP_BEGIN
+  const Fab F = f[b];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) F(i, j, k, n) = 0.0;
+}
+
+constexpr int OB = 256;
+
+// Objective over the valid cells and its gradient:
+//   kind 0: J = sum |m|^2 / (2 rho) dV  (kinetic energy; dV per box)
+//   kind 1: J = sum w . U
+__global__ void __launch_bounds__(OB) k_objective(const BoxDev* boxes, int nbox, int kind, const Fab* st,
+                                                  const Fab* wts, const Fab* lam, double* partial) {
+  __shared__ double sm[OB];
+  double acc = 0.0;
+  for (int b = 0; b < nbox; ++b) {
+    const BoxDev bx = boxes[b];
+    const Fab u = st[b], L = lam[b];
+    const double dV = bx.dx[0] * bx.dx[1] * bx.dx[2];
+    int nx = bx.hi[0] - bx.lo[0] + 1, ny = bx.hi[1] - bx.lo[1] + 1, nz = bx.hi[2] - bx.lo[2] + 1;
+    int64_t n = int64_t(nx) * ny * nz;
+    for (int64_t t = int64_t(blockIdx.x) * OB + threadIdx.x; t < n; t += int64_t(gridDim.x) * OB) {
+      int i = bx.lo[0] + int(t % nx);
+      int64_t r = t / nx;
+      int j = bx.lo[1] + int(r % ny);
+      int k = bx.lo[2] + int(r / ny);
+      if (kind == 0) {
+        const double rho = u(i, j, k, URHO), mx = u(i, j, k, UMX), my = u(i, j, k, UMY), mz = u(i, j, k, UMZ);
+        const double rinv = 1.0 / rho;
+        const double m2 = mx * mx + my * my + mz * mz;
+        acc += 0.5 * m2 * rinv * dV;
+        L(i, j, k, URHO) = -0.5 * m2 * rinv * rinv * dV;
+        L(i, j, k, UMX) = mx * rinv * dV;
+        L(i, j, k, UMY) = my * rinv * dV;
+        L(i, j, k, UMZ) = mz * rinv * dV;
+        L(i, j, k, UET) = 0.0;
+      } else {
+        const Fab w = wts[b];
+#pragma unroll
+        for (int c = 0; c < NCONS; ++c) {
+          const double wc = w(i, j, k, c);
+          acc += wc * u(i, j, k, c);
+          L(i, j, k, c) = wc;
+        }
+      }
+    }
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = OB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+}
+
+__global__ void __launch_bounds__(OB) k_sum1(const double* partial, int n, double* result) {
+  __shared__ double sm[OB];
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < n; t += OB) acc += partial[t];
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = OB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) result[0] = sm[0];
+}
+
+}  // namespace
+
+bool adjoint_supported(const amr_scheme_t& s) {
+  if (s.rhs_euler && s.flux_euler != 1) return false;
+  return s.rhs_euler || s.rhs_visc;
+}
+
+hipError_t launch_adj_rhs(const BoxTable& bt, const UnfusedWork& w, const AdjWork& aw, const Closures& c,
+                          const amr_scheme_t& s, const double* box_dt, const Fab* sin, const Fab* lam,
+                          hipStream_t st) {
+  const BoxDev* B = bt.d_boxes;
+#define LAUNCH(KERNEL, REG, ...)                                                              \
+  do {                                                                                       \
+    Launch L_ = grid_for(bt, REG);                                                           \
+    if (!L_.map) return hipErrorNotInitialized;                                              \
+    hipLaunchKernelGGL(KERNEL, L_.grid, dim3(BLK), 0, st, B, L_.map, REG, box_dt, __VA_ARGS__); \
+  } while (0)
+  const Region g3 = grown(NGHOST), g1 = grown(1);
+  hipError_t e = launch_prims(bt, w, c, box_dt, sin, st);
+  if (e != hipSuccess) return e;
+  if (s.rhs_visc) LAUNCH(k_adj_visc, g1, w.prims, lam, aw.e, c);
+  double co[3] = {0.0, 0.0, 0.0};
+  keep_coefficients(s.order_keep, co);
+  const bool eu = s.rhs_euler != 0, vi = s.rhs_visc != 0;
+  const int h = s.order_keep / 2;
+#define CELL(HH, EE, VV) LAUNCH((k_adj_cell<HH, EE, VV>), g3, w.prims, lam, aw.e, aw.g, c, co[0], co[1], co[2])
+  if (!eu) CELL(1, false, true);
+  else if (h == 3) { if (vi) CELL(3, true, true); else CELL(3, true, false); }
+  else if (h == 2) { if (vi) CELL(2, true, true); else CELL(2, true, false); }
+  else { if (vi) CELL(1, true, true); else CELL(1, true, false); }
+#undef CELL
+#undef LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_fold(const CopyTask* copies, int ncopy, int64_t ncopy_cells, BcTask* const bc[3],
+                       const int nbc[3], const int64_t nbc_cells[3], const Fab* g, hipStream_t st) {
+  for (int d = 2; d >= 0; --d)
+    if (nbc_cells[d] > 0)
+      hipLaunchKernelGGL(k_fold_bc, dim3(unsigned((nbc_cells[d] + FB - 1) / FB)), dim3(FB), 0, st, bc[d], nbc[d],
+                         nbc_cells[d], g);
+  if (ncopy_cells > 0)
+    hipLaunchKernelGGL(k_fold_copy, dim3(unsigned((ncopy_cells + FB - 1) / FB)), dim3(FB), 0, st, copies, ncopy,
+                       ncopy_cells, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_combine(const BoxTable& bt, const AdjWork& aw, const StageArgs& sa, const Fab* lam_out,
+                              const Fab* lam_s1, const Fab* lam_sin, bool accumulate, hipStream_t st) {
+  const Region g0 = grown(0);
+  Launch L = grid_for(bt, g0);
+  if (!L.map) return hipErrorNotInitialized;
+  hipLaunchKernelGGL(k_adj_combine, L.grid, dim3(BLK), 0, st, bt.d_boxes, L.map, g0, sa.box_dt, sa, aw.g, lam_out,
+                     lam_s1, lam_sin, accumulate ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_valid(const BoxTable& bt, const Fab* f, hipStream_t st) {
+  const Region g0 = grown(0);
+  Launch L = grid_for(bt, g0);
+  if (!L.map) return hipErrorNotInitialized;
+  hipLaunchKernelGGL(k_zero_valid, L.grid, dim3(BLK), 0, st, bt.d_boxes, L.map, g0, nullptr, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_objective(const BoxTable& bt, int kind, const Fab* st, const Fab* wts, const Fab* lam,
+                            double* partial, double* result, hipStream_t s) {
+  const int nb = reduce_blocks(bt);
+  hipLaunchKernelGGL(k_objective, dim3(nb), dim3(OB), 0, s, bt.d_boxes, bt.nbox, kind, st, wts, lam, partial);
+  hipLaunchKernelGGL(k_sum1, dim3(1), dim3(OB), 0, s, partial, nb, result);
+  return hipGetLastError();
+}
+
+}  // namespace amr

@@ -120,6 +120,27 @@ struct GraphEntry {
   }
 };
 
+// Buffers of one checkpointed adjoint run: checkpoints ck[nseg] (state at
+// the start of every segment of `k` steps), the stage inputs sb of the steps
+// of one segment (sb(n, 0) = start of step n, sb(0, 0) = the checkpoint),
+// the final state, and three adjoint arrays on the valid cells.
+struct AdjRun {
+  std::vector<char> key;
+  int nsteps = 0, k = 0, nst = 0, nseg = 0, krows = 0;
+  std::vector<double*> mem;
+  std::vector<std::unique_ptr<DescSet>> states;
+  std::unique_ptr<DescSet> lam[3];
+  DescSet* ck(int seg) { return states[seg].get(); }
+  DescSet* sb(int seg, int n, int s) {
+    if (n == 0 && s == 0) return ck(seg);
+    return states[nseg + n * nst + s - 1].get();
+  }
+  DescSet* uf() { return states[nseg + krows * nst - 1].get(); }
+  ~AdjRun() {
+    for (double* p : mem) (void)hipFree(p);
+  }
+};
+
 }  // namespace
 
 struct amr_boxtab_s {
@@ -149,6 +170,13 @@ struct amr_boxtab_s {
   // fused tiles per config
   std::map<std::tuple<int, int, int>, std::unique_ptr<TileTable>> tiles;
   std::vector<std::unique_ptr<GraphEntry>> graphs;
+  // adjoint workspace: g (NCONS on grow 3) and e (16 on grow 1) per box
+  double* adj_mem = nullptr;
+  int adj_visc = -1;
+  AdjWork aw;
+  Fab* adj_e_table = nullptr;
+  std::unique_ptr<DescSet> adj_g;
+  std::unique_ptr<AdjRun> run;
 };
 
 namespace {
@@ -174,17 +202,9 @@ int check_closures(const amr_closures_t& c) {
   return AMR_OK;
 }
 
-// Descriptor set for `fabs` (nbox entries), uploaded once and cached.
-int get_desc(amr_boxtab_s* bt, const amr_fab_t* fabs, int min_ng, DescSet** out) {
+// Checks that fabs[nbox] describe the boxes of the table with >= min_ng ghosts.
+int check_fabs(amr_boxtab_s* bt, const amr_fab_t* fabs, int min_ng) {
   int nb = bt->bt.nbox;
-  size_t bytes = sizeof(amr_fab_t) * nb;
-  for (auto& d : bt->descs) {
-    if (std::memcmp(d->host.data(), fabs, bytes) == 0) {
-      d->last_use = ++bt->use_clock;
-      *out = d.get();
-      return AMR_OK;
-    }
-  }
   for (int b = 0; b < nb; ++b) {
     const amr_fab_t& f = fabs[b];
     const BoxDev& B = bt->bt.h_boxes[b];
@@ -200,17 +220,44 @@ int get_desc(amr_boxtab_s* bt, const amr_fab_t* fabs, int min_ng, DescSet** out)
     if (f.jstride < nx || f.kstride < f.jstride * ny || f.nstride < f.kstride * nz)
       return fail(AMR_ERR_ARG, "fab " + std::to_string(b) + " strides do not cover its box");
   }
-  if (bt->descs.size() >= 24) {
-    auto it = std::min_element(bt->descs.begin(), bt->descs.end(),
-                               [](const auto& a, const auto& b2) { return a->last_use < b2->last_use; });
-    bt->descs.erase(it);
-  }
+  return AMR_OK;
+}
+
+// A descriptor set (host copy + device Fab table) for fabs[nbox].
+int new_desc(amr_boxtab_s* bt, const amr_fab_t* fabs, std::unique_ptr<DescSet>* out) {
+  int nb = bt->bt.nbox;
   auto ds = std::make_unique<DescSet>();
   ds->host.assign(fabs, fabs + nb);
   std::vector<Fab> h(nb);
   for (int b = 0; b < nb; ++b) h[b] = make_fab(fabs[b]);
   HIPCHK(hipMalloc(&ds->d, sizeof(Fab) * nb));
   HIPCHK(hipMemcpy(ds->d, h.data(), sizeof(Fab) * nb, hipMemcpyHostToDevice));
+  *out = std::move(ds);
+  return AMR_OK;
+}
+
+// Descriptor set for caller-provided `fabs` (nbox entries), uploaded once and
+// cached (least recently used of 24 evicted).
+int get_desc(amr_boxtab_s* bt, const amr_fab_t* fabs, int min_ng, DescSet** out) {
+  int nb = bt->bt.nbox;
+  size_t bytes = sizeof(amr_fab_t) * nb;
+  for (auto& d : bt->descs) {
+    if (std::memcmp(d->host.data(), fabs, bytes) == 0) {
+      if (d->host[0].ng < min_ng) break;
+      d->last_use = ++bt->use_clock;
+      *out = d.get();
+      return AMR_OK;
+    }
+  }
+  int rc = check_fabs(bt, fabs, min_ng);
+  if (rc) return rc;
+  if (bt->descs.size() >= 24) {
+    auto it = std::min_element(bt->descs.begin(), bt->descs.end(),
+                               [](const auto& a, const auto& b2) { return a->last_use < b2->last_use; });
+    bt->descs.erase(it);
+  }
+  std::unique_ptr<DescSet> ds;
+  if ((rc = new_desc(bt, fabs, &ds))) return rc;
   ds->last_use = ++bt->use_clock;
   *out = ds.get();
   bt->descs.push_back(std::move(ds));
@@ -514,19 +561,12 @@ int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sc
   return AMR_OK;
 }
 
-// Workspace state arrays mirroring the layout of `state` (a level-wide
-// allocation stays level-wide, so its ghost fill stays cheap).
-int ensure_ws(amr_boxtab_s* bt, const amr_fab_t* state) {
+// A state array with the layout of `state`: a level-wide allocation stays
+// level-wide (so its ghost fill stays cheap), separate FABs become compact
+// FABs.  *mem receives the allocation.
+int alloc_mirror(amr_boxtab_s* bt, const amr_fab_t* state, double** mem, std::vector<amr_fab_t>& fabs) {
   int nb = bt->bt.nbox;
-  if (int(bt->ws_key.size()) == nb && std::memcmp(bt->ws_key.data(), state, sizeof(amr_fab_t) * nb) == 0)
-    return AMR_OK;
-  HIPCHK(hipStreamSynchronize(bt->ctx->stream));
-  for (int w = 0; w < 2; ++w) {
-    if (bt->ws_mem[w]) HIPCHK(hipFree(bt->ws_mem[w]));
-    bt->ws_mem[w] = nullptr;
-    bt->ws_fabs[w].assign(nb, amr_fab_t());
-  }
-  // span of the caller's storage
+  fabs.assign(nb, amr_fab_t());
   intptr_t lo = INTPTR_MAX, hi = 0;
   int64_t sum = 0;
   for (int b = 0; b < nb; ++b) {
@@ -541,22 +581,38 @@ int ensure_ws(amr_boxtab_s* bt, const amr_fab_t* state) {
   }
   int64_t span = int64_t(hi - lo) / 8;
   bool mirror = span <= sum + sum / 2;
-  for (int w = 0; w < 2; ++w) {
-    if (mirror) {
-      HIPCHK(hipMalloc(&bt->ws_mem[w], sizeof(double) * span));
-      for (int b = 0; b < nb; ++b) {
-        bt->ws_fabs[w][b] = state[b];
-        bt->ws_fabs[w][b].data = bt->ws_mem[w] + (intptr_t(state[b].data) - lo) / 8;
-      }
-    } else {
-      HIPCHK(hipMalloc(&bt->ws_mem[w], sizeof(double) * sum));
-      int64_t off = 0;
-      for (int b = 0; b < nb; ++b) {
-        const amr_fab_t& f = state[b];
-        bt->ws_fabs[w][b] = compact_fab(f.lo, f.hi, f.ng, bt->ws_mem[w] + off);
-        off += compact_len(f.lo, f.hi, f.ng, NCONS);
-      }
+  if (mirror) {
+    HIPCHK(hipMalloc(mem, sizeof(double) * span));
+    for (int b = 0; b < nb; ++b) {
+      fabs[b] = state[b];
+      fabs[b].data = *mem + (intptr_t(state[b].data) - lo) / 8;
     }
+  } else {
+    HIPCHK(hipMalloc(mem, sizeof(double) * sum));
+    int64_t off = 0;
+    for (int b = 0; b < nb; ++b) {
+      const amr_fab_t& f = state[b];
+      fabs[b] = compact_fab(f.lo, f.hi, f.ng, *mem + off);
+      off += compact_len(f.lo, f.hi, f.ng, NCONS);
+    }
+  }
+  return AMR_OK;
+}
+
+// Workspace state arrays (two) mirroring the layout of `state`.
+int ensure_ws(amr_boxtab_s* bt, const amr_fab_t* state) {
+  int nb = bt->bt.nbox;
+  if (int(bt->ws_key.size()) == nb && std::memcmp(bt->ws_key.data(), state, sizeof(amr_fab_t) * nb) == 0)
+    return AMR_OK;
+  HIPCHK(hipStreamSynchronize(bt->ctx->stream));
+  for (int w = 0; w < 2; ++w) {
+    if (bt->ws_mem[w]) HIPCHK(hipFree(bt->ws_mem[w]));
+    bt->ws_mem[w] = nullptr;
+  }
+  bt->ws_key.clear();
+  for (int w = 0; w < 2; ++w) {
+    int rc = alloc_mirror(bt, state, &bt->ws_mem[w], bt->ws_fabs[w]);
+    if (rc) return rc;
   }
   bt->ws_key.assign(state, state + nb);
   return AMR_OK;
@@ -594,6 +650,124 @@ int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t
     // single-stage schemes wrote into workspace 0; copy its valid cells back
     HIPCHK(launch_copy_valid(bt->bt, dw[0]->d, dst->d, d_box_dt, bt->ctx->stream));
   }
+  return AMR_OK;
+}
+
+// ------------------------------------------------------------ adjoint
+constexpr int ADJ_NE = 16;
+
+int ensure_adjoint(amr_boxtab_s* bt, const amr_scheme_t& s) {
+  const int visc = s.rhs_visc ? 1 : 0;
+  if (bt->adj_mem && bt->adj_visc >= visc) return AMR_OK;
+  HIPCHK(hipStreamSynchronize(bt->ctx->stream));
+  bt->adj_g.reset();
+  if (bt->adj_e_table) HIPCHK(hipFree(bt->adj_e_table));
+  if (bt->adj_mem) HIPCHK(hipFree(bt->adj_mem));
+  bt->adj_e_table = nullptr;
+  bt->adj_mem = nullptr;
+  bt->aw = AdjWork();
+  const int nb = bt->bt.nbox;
+  int64_t total = 0;
+  for (int b = 0; b < nb; ++b) {
+    const BoxDev& B = bt->bt.h_boxes[b];
+    total += compact_len(B.lo, B.hi, NGHOST, NCONS);
+    if (visc) total += compact_len(B.lo, B.hi, 1, ADJ_NE);
+  }
+  HIPCHK(hipMalloc(&bt->adj_mem, sizeof(double) * total));
+  std::vector<amr_fab_t> gf(nb);
+  std::vector<Fab> he(nb);
+  int64_t off = 0;
+  for (int b = 0; b < nb; ++b) {
+    const BoxDev& B = bt->bt.h_boxes[b];
+    gf[b] = compact_fab(B.lo, B.hi, NGHOST, bt->adj_mem + off);
+    off += compact_len(B.lo, B.hi, NGHOST, NCONS);
+  }
+  if (visc)
+    for (int b = 0; b < nb; ++b) {
+      const BoxDev& B = bt->bt.h_boxes[b];
+      he[b] = make_fab(compact_fab(B.lo, B.hi, 1, bt->adj_mem + off));
+      off += compact_len(B.lo, B.hi, 1, ADJ_NE);
+    }
+  int rc = new_desc(bt, gf.data(), &bt->adj_g);
+  if (rc) return rc;
+  if ((rc = build_fill(bt, bt->adj_g.get()))) return rc;
+  bt->aw.g = bt->adj_g->d;
+  if (visc) {
+    HIPCHK(hipMalloc(&bt->adj_e_table, sizeof(Fab) * nb));
+    HIPCHK(hipMemcpy(bt->adj_e_table, he.data(), sizeof(Fab) * nb, hipMemcpyHostToDevice));
+    bt->aw.e = bt->adj_e_table;
+  }
+  bt->adj_visc = visc;
+  return AMR_OK;
+}
+
+// Reverse of one stage: lam_sin (=|+=) stage terms of lam_out through the
+// RHS VJP at `sin` (ghost-filled); lam_s1 += a lam_out.
+int adj_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageArgs& sa,
+              const DescSet* sin, const DescSet* lam_out, const DescSet* lam_s1, const DescSet* lam_sin, bool acc) {
+  int rc;
+  if ((rc = ensure_unfused(bt, sch))) return rc;
+  if ((rc = ensure_adjoint(bt, sch))) return rc;
+  const Closures c = make_closures(cls);
+  hipStream_t st = bt->ctx->stream;
+  HIPCHK(launch_adj_rhs(bt->bt, bt->uw, bt->aw, c, sch, sa.box_dt, sin->d, lam_out->d, st));
+  const DescSet* g = bt->adj_g.get();
+  HIPCHK(launch_fold(g->d_copy, g->ncopy, g->ncopy_cells, g->d_bc, g->nbc, g->nbc_cells, g->d, st));
+  HIPCHK(launch_adj_combine(bt->bt, bt->aw, sa, lam_out->d, lam_s1 ? lam_s1->d : nullptr, lam_sin->d, acc, st));
+  return AMR_OK;
+}
+
+int check_adjoint(const amr_scheme_t& s) {
+  if (!adjoint_supported(s))
+    return fail(AMR_ERR_ARG, "the adjoint covers the KEEP (flux_euler=1) and viscous right-hand sides only");
+  return AMR_OK;
+}
+
+int ensure_run(amr_boxtab_s* bt, const amr_fab_t* state, int nsteps, int k, int nst) {
+  const int nb = bt->bt.nbox;
+  std::vector<char> key(sizeof(amr_fab_t) * nb + 3 * sizeof(int));
+  std::memcpy(key.data(), state, sizeof(amr_fab_t) * nb);
+  int tail[3] = {nsteps, k, nst};
+  std::memcpy(key.data() + sizeof(amr_fab_t) * nb, tail, sizeof tail);
+  if (bt->run && bt->run->key == key) return AMR_OK;
+  HIPCHK(hipStreamSynchronize(bt->ctx->stream));
+  bt->run.reset();
+  auto R = std::make_unique<AdjRun>();
+  R->key = key;
+  R->nsteps = nsteps;
+  R->k = k;
+  R->nst = nst;
+  R->nseg = (nsteps + k - 1) / k;
+  R->krows = std::min(k, nsteps);
+  const int nstates = R->nseg + R->krows * nst;
+  int rc;
+  for (int i = 0; i < nstates; ++i) {
+    double* m = nullptr;
+    std::vector<amr_fab_t> f;
+    rc = alloc_mirror(bt, state, &m, f);
+    if (m) R->mem.push_back(m);
+    if (rc) return rc;
+    std::unique_ptr<DescSet> ds;
+    if ((rc = new_desc(bt, f.data(), &ds))) return rc;
+    if ((rc = build_fill(bt, ds.get()))) return rc;
+    R->states.push_back(std::move(ds));
+  }
+  int64_t per = 0;
+  for (int b = 0; b < nb; ++b) per += compact_len(bt->bt.h_boxes[b].lo, bt->bt.h_boxes[b].hi, 0, NCONS);
+  double* lm = nullptr;
+  HIPCHK(hipMalloc(&lm, sizeof(double) * per * 3));
+  R->mem.push_back(lm);
+  for (int w = 0; w < 3; ++w) {
+    std::vector<amr_fab_t> f(nb);
+    int64_t off = 0;
+    for (int b = 0; b < nb; ++b) {
+      const BoxDev& B = bt->bt.h_boxes[b];
+      f[b] = compact_fab(B.lo, B.hi, 0, lm + w * per + off);
+      off += compact_len(B.lo, B.hi, 0, NCONS);
+    }
+    if ((rc = new_desc(bt, f.data(), &R->lam[w]))) return rc;
+  }
+  bt->run = std::move(R);
   return AMR_OK;
 }
 
@@ -791,6 +965,10 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   for (void* p : bt->uw_tables) (void)hipFree(p);
   for (int w = 0; w < 2; ++w)
     if (bt->ws_mem[w]) (void)hipFree(bt->ws_mem[w]);
+  bt->run.reset();
+  bt->adj_g.reset();
+  if (bt->adj_e_table) (void)hipFree(bt->adj_e_table);
+  if (bt->adj_mem) (void)hipFree(bt->adj_mem);
   for (auto& m : bt->bt.block_maps) (void)hipFree(m.second.first);
   if (bt->bt.d_boxes) (void)hipFree(bt->bt.d_boxes);
   if (bt->d_partial) (void)hipFree(bt->d_partial);
@@ -1032,6 +1210,115 @@ int amr_totals(amr_boxtab_t bt, const amr_fab_t* state, double out[5]) {
   double probsize = 1.0;
   for (int d = 0; d < 3; ++d) probsize *= (bt->geom.dom_hi[d] - bt->geom.dom_lo[d] + 1) * bt->geom.dx[d];
   for (int n = 0; n < 5; ++n) out[n] = bt->h_result[n] * probsize;  // src/CNS.cpp:541
+  return AMR_OK;
+}
+
+int amr_vjp_stage(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, int mode, double a, double b,
+                  double c, double d, const amr_fab_t* sin, const amr_fab_t* lam_out, const amr_fab_t* lam_s1,
+                  const amr_fab_t* lam_sin, int accumulate) {
+  if (!bt || !cls || !sch || !sin || !lam_out || !lam_sin) return fail(AMR_ERR_ARG, "null argument");
+  int rc;
+  if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls)) || (rc = check_adjoint(*sch))) return rc;
+  if (mode < 0 || mode > 2) return fail(AMR_ERR_ARG, "mode must be 0, 1 or 2");
+  DescSet *ds = nullptr, *dlo = nullptr, *dls1 = nullptr, *dls = nullptr;
+  if ((rc = get_desc(bt, sin, NGHOST, &ds))) return rc;
+  if ((rc = get_desc(bt, lam_out, 0, &dlo))) return rc;
+  if (lam_s1 && (rc = get_desc(bt, lam_s1, 0, &dls1))) return rc;
+  if ((rc = get_desc(bt, lam_sin, 0, &dls))) return rc;
+  StageArgs sa{StageCoef{mode, a, b, c, d}, nullptr, 0, 0, 0};
+  return adj_stage(bt, *cls, *sch, sa, ds, dlo, dls1, dls, accumulate != 0);
+}
+
+int amr_vjp_rhs(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
+                const amr_fab_t* lam, const amr_fab_t* grad) {
+  return amr_vjp_stage(bt, cls, sch, 2, 0.0, 0.0, 0.0, 0.0, state, lam, nullptr, grad, 0);
+}
+
+int amr_adjoint_run(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state0,
+                    double dt, int nsteps, int kcheck, int objective, const amr_fab_t* weights,
+                    const amr_fab_t* grad, double* J_out) {
+  if (!bt || !cls || !sch || !state0 || !grad) return fail(AMR_ERR_ARG, "null argument");
+  int rc;
+  if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls)) || (rc = check_adjoint(*sch))) return rc;
+  if (nsteps < 1 || kcheck < 1) return fail(AMR_ERR_ARG, "nsteps and kcheck must be >= 1");
+  if (objective != 0 && objective != 1) return fail(AMR_ERR_ARG, "objective must be 0 or 1");
+  if (objective == 1 && !weights) return fail(AMR_ERR_ARG, "objective 1 needs weights");
+  if (bt->batch) return fail(AMR_ERR_ARG, "adjoint runs take a level box table, not a batch");
+  int nst = 0;
+  StageCoef sc;
+  if ((rc = stage_coeffs(sch->order_rk, sch->stages_rk, dt, 0, &nst, &sc))) return rc;
+  DescSet *d0 = nullptr, *dg = nullptr, *dw = nullptr;
+  if ((rc = get_desc(bt, state0, NGHOST, &d0))) return rc;
+  if ((rc = get_desc(bt, grad, 0, &dg))) return rc;
+  if (objective == 1 && (rc = get_desc(bt, weights, 0, &dw))) return rc;
+  if ((rc = ensure_run(bt, state0, nsteps, kcheck, nst))) return rc;
+  AdjRun* R = bt->run.get();
+  hipStream_t st = bt->ctx->stream;
+  const int k = kcheck;
+  auto stage_args = [&](int s, StageArgs* sa) {
+    StageCoef c2;
+    int n2 = 0;
+    int r = stage_coeffs(sch->order_rk, sch->stages_rk, dt, s, &n2, &c2);
+    *sa = StageArgs{c2, nullptr, sch->order_rk, sch->stages_rk, s};
+    return r;
+  };
+  // forward step n (global index); the last stage is skipped on request
+  auto run_step = [&](int n, bool skip_last) -> int {
+    const int seg = n / k, nl = n % k;
+    for (int s = 0; s < nst; ++s) {
+      DescSet* sin = R->sb(seg, nl, s);
+      if (skip_last && s == nst - 1) {
+        // the stage itself is not needed, but the reverse sweep reads its
+        // input with ghost cells
+        int r = enqueue_fill(bt, sin);
+        if (r) return r;
+        break;
+      }
+      StageArgs sa;
+      int r = stage_args(s, &sa);
+      if (r) return r;
+      DescSet* s1 = R->sb(seg, nl, 0);
+      DescSet* out;
+      if (s < nst - 1) out = R->sb(seg, nl, s + 1);
+      else if (n + 1 == nsteps) out = R->uf();
+      else if ((n + 1) % k == 0) out = R->ck(seg + 1);
+      else out = R->sb(seg, nl + 1, 0);
+      if ((r = enqueue_fill(bt, sin))) return r;
+      if ((r = do_stage(bt, *cls, *sch, sa, s1, sin, out))) return r;
+    }
+    return AMR_OK;
+  };
+  HIPCHK(launch_copy_valid(bt->bt, d0->d, R->ck(0)->d, nullptr, st));
+  for (int n = 0; n < nsteps; ++n)
+    if ((rc = run_step(n, false))) return rc;
+  DescSet *lc = R->lam[0].get(), *lt = R->lam[1].get(), *la = R->lam[2].get();
+  HIPCHK(launch_objective(bt->bt, objective, R->uf()->d, dw ? dw->d : nullptr, lc->d, bt->d_partial,
+                          bt->d_result, st));
+  HIPCHK(hipMemcpyAsync(bt->h_result, bt->d_result, sizeof(double), hipMemcpyDeviceToHost, st));
+  for (int seg = R->nseg - 1; seg >= 0; --seg) {
+    const int n0 = seg * k, len = std::min(k, nsteps - n0);
+    if (seg != R->nseg - 1)
+      for (int n = n0; n < n0 + len; ++n)
+        if ((rc = run_step(n, n == n0 + len - 1))) return rc;
+    for (int nl = len - 1; nl >= 0; --nl) {
+      HIPCHK(launch_zero_valid(bt->bt, la->d, st));
+      for (int s = nst - 1; s >= 0; --s) {
+        StageArgs sa;
+        if ((rc = stage_args(s, &sa))) return rc;
+        DescSet* sin = R->sb(seg, nl, s);
+        if (s > 0) {
+          if ((rc = adj_stage(bt, *cls, *sch, sa, sin, lc, la, lt, false))) return rc;
+          std::swap(lc, lt);
+        } else {
+          if ((rc = adj_stage(bt, *cls, *sch, sa, sin, lc, la, la, true))) return rc;
+        }
+      }
+      std::swap(lc, la);
+    }
+  }
+  HIPCHK(launch_copy_valid(bt->bt, lc->d, dg->d, nullptr, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (J_out) *J_out = bt->h_result[0];
   return AMR_OK;
 }
 

@@ -81,6 +81,34 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
                                 const amr_scheme_t& s, const StageArgs& sa, const Fab* s1, const Fab* sin,
                                 const Fab* out, hipStream_t st);
 
+// cons2prim of `sin` into w.prims on grow(bx,3) for every box.
+hipError_t launch_prims(const BoxTable& bt, const UnfusedWork& w, const Closures& c, const double* box_dt,
+                        const Fab* sin, hipStream_t st);
+
+// Reverse mode of the KEEP + viscous RHS (adjoint.hip).  Workspace per box:
+// e = 16 comps on grow(bx,1) (viscous), g = NCONS comps on grow(bx,3).
+struct AdjWork {
+  Fab* e = nullptr;
+  Fab* g = nullptr;
+};
+bool adjoint_supported(const amr_scheme_t& s);
+// g = (d dSdt / d sin)^T lam on grow(bx,3) of every box (before the fold);
+// sin ghost-filled, lam read on valid cells only.
+hipError_t launch_adj_rhs(const BoxTable& bt, const UnfusedWork& w, const AdjWork& aw, const Closures& c,
+                          const amr_scheme_t& s, const double* box_dt, const Fab* sin, const Fab* lam,
+                          hipStream_t st);
+// Transposed ghost fill of g: BC tasks z, y, x, then copy tasks (atomic adds).
+hipError_t launch_fold(const CopyTask* copies, int ncopy, int64_t ncopy_cells, BcTask* const bc[3],
+                       const int nbc[3], const int64_t nbc_cells[3], const Fab* g, hipStream_t st);
+// Stage combination transposed on valid cells (lam_s1 may be null).
+hipError_t launch_adj_combine(const BoxTable& bt, const AdjWork& aw, const StageArgs& sa, const Fab* lam_out,
+                              const Fab* lam_s1, const Fab* lam_sin, bool accumulate, hipStream_t st);
+hipError_t launch_zero_valid(const BoxTable& bt, const Fab* f, hipStream_t st);
+// Objective of the final state and its gradient lam (kind 0 kinetic energy,
+// kind 1 linear with weights); result[0] = J.
+hipError_t launch_objective(const BoxTable& bt, int kind, const Fab* st, const Fab* wts, const Fab* lam,
+                            double* partial, double* result, hipStream_t s);
+
 hipError_t launch_fill_copy(const CopyTask* tasks, int ntask, int64_t ncell, const Fab* fabs,
                             hipStream_t st);
 hipError_t launch_fill_bc(const BcTask* tasks, int ntask, int64_t ncell, const Fab* fabs,

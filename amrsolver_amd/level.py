@@ -209,6 +209,26 @@ class Level:
         call("amr_autotune", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, out)
         return tuple(out)
 
+    # --- reverse mode ----------------------------------------------------
+    def vjp_rhs(self, Sborder, lam, grad):
+        """grad = (dL/dU)^T lam at the ghost-filled state Sborder."""
+        call("amr_vjp_rhs", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), Sborder.fabs,
+             lam.fabs, grad.fabs)
+
+    def vjp_stage(self, mode, a, b, c, d, Sin, lam_out, lam_s1, lam_sin, accumulate=False):
+        call("amr_vjp_stage", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), mode, a, b, c, d,
+             Sin.fabs, lam_out.fabs, lam_s1.fabs if lam_s1 is not None else None, lam_sin.fabs,
+             1 if accumulate else 0)
+
+    def adjoint_run(self, S0, dt, nsteps, kcheck, grad, objective=0, weights=None):
+        """Gradient of the objective after nsteps steps w.r.t. the valid
+        cells of S0 (checkpoints every kcheck steps); returns J."""
+        out = ctypes.c_double()
+        call("amr_adjoint_run", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S0.fabs, dt,
+             nsteps, kcheck, objective, weights.fabs if weights is not None else None, grad.fabs,
+             ctypes.byref(out))
+        return out.value
+
     def est_time_step(self, S, cfl):
         out = ctypes.c_double()
         call("amr_est_dt", self.h, ctypes.byref(self.closures), S.fabs, cfl, ctypes.byref(out))

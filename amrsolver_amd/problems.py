@@ -152,6 +152,49 @@ def tgv_vorticity_magnitude(g, n, mach=0.1, L=1.0, T0=110.4, cls=None):
     return np.sqrt(wx * wx + wy * wy + wz * wz)
 
 
+# ---------------------------------------------------------------- channel-like box
+
+
+def channel_closures(Re=500.0, mach=0.2, Ub=1.0, rho0=1.0, H=1.0, Pr=0.71):
+    """Scaled Sutherland closure of the periodic channel-like box: mu(T0) =
+    rho0 Ub H / Re at the reference temperature T0 = p0 / (rho0 R) with
+    p0 = rho0 c^2 / gamma, c = Ub / mach."""
+    g = perfect_gas()
+    c = Ub / mach
+    p0 = rho0 * c * c / g["gamma"]
+    T0 = p0 / (rho0 * g["Rspec"])
+    return closures_scaled_sutherland(rho0 * Ub * H / Re, T0=T0, Pr=Pr), p0, T0
+
+
+def channel_geom(n=(128, 64, 64)):
+    """[0, 2 pi] x [-1, 1] x [0, pi], periodic in every direction."""
+    return geom(n, (0.0, -1.0, 0.0), (2 * math.pi, 1.0, math.pi), (1, 1, 1))
+
+
+def channel_state(cls, g, n, p0, seed=20261202, ampl=0.05):
+    """rho = 1, u = 1.5 (1 - y^2) plus seeded U[-1,1] perturbations of
+    amplitude `ampl` on u, v, w (numpy PCG64), uniform pressure p0."""
+    rng = np.random.default_rng(seed)
+    X, Y, Z = cell_centres(g, n)
+    shape = X.shape
+    vel = [1.5 * (1.0 - Y * Y) + ampl * rng.uniform(-1, 1, size=shape),
+           ampl * rng.uniform(-1, 1, size=shape),
+           ampl * rng.uniform(-1, 1, size=shape)]
+    rho = np.ones(shape)
+    u = np.empty((5,) + shape)
+    u[0] = rho
+    for d in range(3):
+        u[1 + d] = rho * vel[d]
+    u[4] = p0 / (cls.gamma - 1.0) + 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
+    return to_fab_order(u)
+
+
+def kinetic_energy(u, g):
+    """sum |m|^2 / (2 rho) dx dy dz over a (5, nz, ny, nx) state."""
+    dV = g.dx[0] * g.dx[1] * g.dx[2]
+    return float(np.sum(0.5 * (u[1] ** 2 + u[2] ** 2 + u[3] ** 2) / u[0]) * dV)
+
+
 # ---------------------------------------------------------------- random smooth states
 
 

@@ -179,6 +179,45 @@ int amr_set_tiling(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, int t
 int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch,
                  const amr_fab_t* state, int32_t tiling[3]);
 
+/* ---- Reverse mode (no reference counterpart: the reference has no adjoint;
+ * its lib/autodiff submodule is empty).  Implemented for the KEEP
+ * (flux_euler = 1, orders 2/4/6) and viscous right-hand sides of
+ * CNS::compute_rhs (src/tim/CNS_advance.cpp:186-427) and every stage scheme
+ * of CNS::advance.  Adjoint arrays hold NCONS components on the valid cells
+ * (any ghost width).  Ghost-fill transposes accumulate with atomics, so
+ * results may differ in the last bits between runs. */
+
+/* grad = (dL/dU)^T lam for the map U (valid cells) -> L(fill(U)) of amr_rhs.
+ * `state` must be ghost-filled (ng >= 3). */
+int amr_vjp_rhs(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
+                const amr_fab_t* lam, const amr_fab_t* grad);
+
+/* Reverse of amr_rk_stage(mode, a, b, c, d, s1, sin, out) with respect to
+ * the valid cells of s1 and sin, given lam_out = adjoint of out:
+ *   lam_sin  = b (c lam_out + d J^T lam_out)   (mode 0)
+ *            = c lam_out + d J^T lam_out        (mode 1)
+ *            = J^T lam_out                      (mode 2)
+ *   assigned, or added to lam_sin when `accumulate` is nonzero;
+ *   lam_s1 += a lam_out (modes 0 and 1; lam_s1 may be NULL).
+ * J = dL/dU at `sin` (ghost-filled, ng >= 3).  lam_sin may alias lam_out or
+ * lam_s1; no adjoint array may alias `sin`. */
+int amr_vjp_stage(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, int mode, double a, double b,
+                  double c, double d, const amr_fab_t* sin, const amr_fab_t* lam_out, const amr_fab_t* lam_s1,
+                  const amr_fab_t* lam_sin, int accumulate);
+
+/* Gradient of an objective of the state after `nsteps` CNS::advance steps
+ * of fixed dt from state0 (ng >= 3, left unchanged), with respect to the
+ * valid cells of state0.  The forward stores a checkpoint every `kcheck`
+ * steps and the stage inputs of the last segment; the reverse sweep
+ * recomputes the stage inputs of every other segment from its checkpoint.
+ *   objective 0: J = sum over valid cells of |m|^2 / (2 rho) dx dy dz
+ *   objective 1: J = sum over valid cells and components of weights . U
+ * Writes grad (valid cells) and *J_out.  Synchronous.  Workspace: about
+ * (nsteps/kcheck + kcheck * stages) state arrays. */
+int amr_adjoint_run(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state0,
+                    double dt, int nsteps, int kcheck, int objective, const amr_fab_t* weights,
+                    const amr_fab_t* grad, double* J_out);
+
 /* Conserved totals sum(U_n) * ProbSize (CNS::printTotal, src/CNS.cpp:537-542).
  * Synchronous. */
 int amr_totals(amr_boxtab_t bt, const amr_fab_t* state, double out[5]);
