@@ -77,7 +77,8 @@ struct amr_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  // autotuned fused-kernel configuration per (scheme key, box-shape key)
+  // fused-kernel tiling set explicitly per scheme key (amr_set_tiling); it
+  // takes precedence over the per-box-table autotuned choice
   std::map<std::string, FusedConfig> tuned;
 };
 
@@ -177,6 +178,13 @@ struct amr_boxtab_s {
   Fab* adj_e_table = nullptr;
   std::unique_ptr<DescSet> adj_g;
   std::unique_ptr<AdjRun> run;
+  // autotuned tiling per scheme key for this box table
+  std::map<std::string, FusedConfig> tuned;
+  // ghost cells of each box covered by no box of the level (coarse-fine
+  // ghosts of a refined level): regions of box b as tasks b -> b
+  CopyTask* d_uncov = nullptr;
+  int nuncov = 0;
+  int64_t nuncov_cells = 0;
 };
 
 namespace {
@@ -390,6 +398,112 @@ int enqueue_fill(amr_boxtab_s* bt, DescSet* ds) {
   return AMR_OK;
 }
 
+struct IBox {
+  int lo[3], hi[3];
+};
+
+bool ibox_empty(const IBox& a) { return a.lo[0] > a.hi[0] || a.lo[1] > a.hi[1] || a.lo[2] > a.hi[2]; }
+
+// a minus b as up to six disjoint boxes
+void ibox_subtract(const IBox& a, const IBox& b, std::vector<IBox>& out) {
+  IBox x;
+  for (int d = 0; d < 3; ++d) {
+    x.lo[d] = std::max(a.lo[d], b.lo[d]);
+    x.hi[d] = std::min(a.hi[d], b.hi[d]);
+  }
+  if (ibox_empty(x)) {
+    out.push_back(a);
+    return;
+  }
+  IBox rest = a;
+  for (int d = 0; d < 3; ++d) {
+    if (rest.lo[d] < x.lo[d]) {
+      IBox p = rest;
+      p.hi[d] = x.lo[d] - 1;
+      out.push_back(p);
+      rest.lo[d] = x.lo[d];
+    }
+    if (rest.hi[d] > x.hi[d]) {
+      IBox p = rest;
+      p.lo[d] = x.hi[d] + 1;
+      out.push_back(p);
+      rest.hi[d] = x.hi[d];
+    }
+  }
+}
+
+// Ghost cells (width NGHOST) of every box that no box of the same level or
+// problem covers, periodic images included, and that lie inside the domain
+// in every non-periodic direction (outside it the physical BCs fill them).
+int build_uncovered(amr_boxtab_s* bt) {
+  const int nb = bt->bt.nbox;
+  std::vector<CopyTask> tasks;
+  int64_t ncell = 0;
+  for (int b = 0; b < nb; ++b) {
+    const BoxDev& B = bt->bt.h_boxes[b];
+    const amr_geom_t& g = bt->geoms[b];
+    int len[3];
+    for (int d = 0; d < 3; ++d) len[d] = g.dom_hi[d] - g.dom_lo[d] + 1;
+    IBox gb;
+    for (int d = 0; d < 3; ++d) {
+      gb.lo[d] = B.lo[d] - NGHOST;
+      gb.hi[d] = B.hi[d] + NGHOST;
+      if (!g.periodic[d]) {
+        gb.lo[d] = std::max(gb.lo[d], g.dom_lo[d]);
+        gb.hi[d] = std::min(gb.hi[d], g.dom_hi[d]);
+      }
+    }
+    std::vector<IBox> pieces{gb}, next;
+    for (int s = 0; s < nb && !pieces.empty(); ++s) {
+      if (bt->batch && s != b) continue;
+      const BoxDev& S = bt->bt.h_boxes[s];
+      for (int sx = -1; sx <= 1; ++sx)
+        for (int sy = -1; sy <= 1; ++sy)
+          for (int sz = -1; sz <= 1; ++sz) {
+            const int sv[3] = {sx, sy, sz};
+            bool ok = true;
+            IBox img;
+            for (int d = 0; d < 3; ++d) {
+              if (sv[d] != 0 && !g.periodic[d]) ok = false;
+              img.lo[d] = S.lo[d] + sv[d] * len[d];
+              img.hi[d] = S.hi[d] + sv[d] * len[d];
+            }
+            if (!ok) continue;
+            next.clear();
+            for (const IBox& p : pieces) ibox_subtract(p, img, next);
+            pieces.swap(next);
+          }
+    }
+    for (const IBox& p : pieces) {
+      CopyTask t;
+      t.dst = t.src = b;
+      for (int d = 0; d < 3; ++d) {
+        t.lo[d] = p.lo[d];
+        t.hi[d] = p.hi[d];
+        t.sh[d] = 0;
+      }
+      t.start = ncell;
+      ncell += cells(t.lo, t.hi);
+      tasks.push_back(t);
+    }
+  }
+  bt->nuncov = int(tasks.size());
+  bt->nuncov_cells = ncell;
+  if (!tasks.empty()) {
+    HIPCHK(hipMalloc(&bt->d_uncov, sizeof(CopyTask) * tasks.size()));
+    HIPCHK(hipMemcpy(bt->d_uncov, tasks.data(), sizeof(CopyTask) * tasks.size(), hipMemcpyHostToDevice));
+  }
+  return AMR_OK;
+}
+
+// Uncovered ghost cells of `from` into `to` (no-op on levels that cover
+// their own ghost cells).
+int copy_uncovered(amr_boxtab_s* bt, const DescSet* from, const DescSet* to) {
+  if (bt->nuncov_cells == 0) return AMR_OK;
+  HIPCHK(launch_copy_regions(bt->d_uncov, bt->nuncov, bt->nuncov_cells, from->d, to->d, bt->ctx->stream));
+  return AMR_OK;
+}
+
 enum { UW_PRIMS = 1, UW_VISC = 2, UW_WENO = 4, UW_SLOPE = 8 };
 
 int ensure_unfused(amr_boxtab_s* bt, const amr_scheme_t& s) {
@@ -537,8 +651,11 @@ bool fused_fits(const amr_boxtab_s* bt, const FusedConfig& cfg) {
 }
 
 FusedConfig choose_config(const amr_boxtab_s* bt, const amr_scheme_t& sch) {
-  auto tk = bt->ctx->tuned.find(scheme_key(sch));
+  const std::string key = scheme_key(sch);
+  auto tk = bt->ctx->tuned.find(key);
   if (tk != bt->ctx->tuned.end()) return tk->second;
+  auto tb = bt->tuned.find(key);
+  if (tb != bt->tuned.end()) return tb->second;
   return default_config(bt);
 }
 
@@ -638,6 +755,9 @@ int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t
   if ((rc = build_fill(bt, dst))) return rc;
   for (int w = 0; w < 2; ++w)
     if ((rc = build_fill(bt, dw[w]))) return rc;
+  // stage inputs keep the caller's values in ghost cells no box covers
+  for (int w = 0; w < 2; ++w)
+    if ((rc = copy_uncovered(bt, dst, dw[w]))) return rc;
   for (int s = 0; s < nst; ++s) {
     if ((rc = stage_coeffs(sch.order_rk, sch.stages_rk, dt, s, &nst, &sc))) return rc;
     DescSet* sin = s == 0 ? dst : dw[(s - 1) % 2];
@@ -950,7 +1070,8 @@ int finish_boxtab(amr_boxtab_s* bt) {
   if (e == hipSuccess) e = hipHostMalloc(&bt->h_box_dt, sizeof(double) * nbox);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->dt_copied, hipEventDisableTiming);
   if (e == hipSuccess) e = prepare_block_maps(bt->bt);
-  if (e != hipSuccess) rc = fail(AMR_ERR_HIP, std::string("box table allocation: ") + hipGetErrorString(e));
+  if (e != hipSuccess) return fail(AMR_ERR_HIP, std::string("box table allocation: ") + hipGetErrorString(e));
+  rc = build_uncovered(bt);
   return rc;
 }
 }  // namespace
@@ -965,6 +1086,7 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   for (void* p : bt->uw_tables) (void)hipFree(p);
   for (int w = 0; w < 2; ++w)
     if (bt->ws_mem[w]) (void)hipFree(bt->ws_mem[w]);
+  if (bt->d_uncov) (void)hipFree(bt->d_uncov);
   bt->run.reset();
   bt->adj_g.reset();
   if (bt->adj_e_table) (void)hipFree(bt->adj_e_table);
@@ -1126,6 +1248,7 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
   int rc;
   if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls))) return rc;
   std::string key = scheme_key(*sch);
+  bt->ctx->tuned.erase(key);  // tuning replaces an explicit setting
   int maxz = 0;
   for (auto& B : bt->bt.h_boxes) maxz = std::max(maxz, B.hi[2] - B.lo[2] + 1);
   std::vector<FusedConfig> cand;
@@ -1138,7 +1261,7 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
       if (fused_supported(*sch, c.tx, c.ty) && fused_fits(bt, c)) cand.push_back(c);
     }
   if (cand.empty()) {
-    bt->ctx->tuned[key] = FusedConfig{-1, 0, 0};
+    bt->tuned[key] = FusedConfig{-1, 0, 0};
     tiling[0] = -1;
     tiling[1] = tiling[2] = 0;
     return AMR_OK;
@@ -1155,7 +1278,7 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
   FusedConfig bestc = cand[0];
   StageArgs sa{StageCoef{2, 0, 0, 0, 0}, nullptr, 0, 0, 0};
   for (auto& cfg : cand) {
-    bt->ctx->tuned[key] = cfg;
+    bt->tuned[key] = cfg;
     float tbest = 1e30f;
     for (int rep = 0; rep < 4; ++rep) {
       HIPCHK(hipEventRecord(e0, st));
@@ -1173,7 +1296,7 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  bt->ctx->tuned[key] = bestc;
+  bt->tuned[key] = bestc;
   tiling[0] = bestc.tx;
   tiling[1] = bestc.ty;
   tiling[2] = bestc.tz;
@@ -1289,6 +1412,8 @@ int amr_adjoint_run(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme
     return AMR_OK;
   };
   HIPCHK(launch_copy_valid(bt->bt, d0->d, R->ck(0)->d, nullptr, st));
+  for (auto& ds : R->states)
+    if ((rc = copy_uncovered(bt, d0, ds.get()))) return rc;
   for (int n = 0; n < nsteps; ++n)
     if ((rc = run_step(n, false))) return rc;
   DescSet *lc = R->lam[0].get(), *lt = R->lam[1].get(), *la = R->lam[2].get();

@@ -66,6 +66,23 @@ __global__ void __launch_bounds__(FBLK) k_fill_bc(const BcTask* tasks, int ntask
   }
 }
 
+// Region copies between two FAB sets: dst[T.dst](c) = src[T.src](c - sh).
+__global__ void __launch_bounds__(FBLK) k_copy_regions(const CopyTask* tasks, int ntask, int64_t ncell,
+                                                       const Fab* src, const Fab* dst) {
+  int64_t t = int64_t(blockIdx.x) * FBLK + threadIdx.x;
+  if (t >= ncell) return;
+  const CopyTask& T = tasks[find_task(tasks, ntask, t)];
+  int64_t r = t - T.start;
+  int nx = T.hi[0] - T.lo[0] + 1, ny = T.hi[1] - T.lo[1] + 1;
+  int i = T.lo[0] + int(r % nx);
+  int64_t r2 = r / nx;
+  int j = T.lo[1] + int(r2 % ny);
+  int k = T.lo[2] + int(r2 / ny);
+  const Fab d = dst[T.dst], s = src[T.src];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) d(i, j, k, n) = s(i - T.sh[0], j - T.sh[1], k - T.sh[2], n);
+}
+
 __global__ void __launch_bounds__(FBLK) k_copy_valid(const BoxDev* boxes, const Fab* src, const Fab* dst,
                                                      const double* box_dt) {
   const int b = blockIdx.y;
@@ -296,6 +313,14 @@ hipError_t launch_estdt_boxes(const BoxTable& bt, const Fab* st, const Closures&
   int nper = estdt_box_blocks(bt);
   hipLaunchKernelGGL(k_estdt_boxes, dim3(nper, bt.nbox), dim3(RBLK), 0, s, bt.d_boxes, st, c, partial);
   hipLaunchKernelGGL(k_min_boxes, dim3(bt.nbox), dim3(RBLK), 0, s, partial, nper, result);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_regions(const CopyTask* tasks, int ntask, int64_t ncell, const Fab* src, const Fab* dst,
+                               hipStream_t st) {
+  if (ncell <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_regions, dim3(unsigned((ncell + FBLK - 1) / FBLK)), dim3(FBLK), 0, st, tasks, ntask,
+                     ncell, src, dst);
   return hipGetLastError();
 }
 

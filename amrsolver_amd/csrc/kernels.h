@@ -114,6 +114,10 @@ hipError_t launch_fill_copy(const CopyTask* tasks, int ntask, int64_t ncell, con
 hipError_t launch_fill_bc(const BcTask* tasks, int ntask, int64_t ncell, const Fab* fabs,
                           hipStream_t st);
 
+// dst[T.dst](c) = src[T.src](c - sh) over the task regions (two FAB sets).
+hipError_t launch_copy_regions(const CopyTask* tasks, int ntask, int64_t ncell, const Fab* src, const Fab* dst,
+                               hipStream_t st);
+
 // dst = src on the valid cells of every box.
 hipError_t launch_copy_valid(const BoxTable& bt, const Fab* src, const Fab* dst, const double* box_dt,
                              hipStream_t st);

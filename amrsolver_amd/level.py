@@ -119,6 +119,17 @@ class StateArrays:
                 t[:, g:t.shape[1] - g, g:t.shape[2] - g, g:t.shape[3] - g] = torch.from_numpy(np.ascontiguousarray(sub)).to(t.device)
         torch.cuda.synchronize()
 
+    def upload_box(self, b, arr):
+        """Whole ghosted array of box b, (ncomp, nz+2g, ny+2g, nx+2g)
+        (separate-box layout only)."""
+        torch = _torch_mod()
+        if self.layout != "boxes":
+            raise ValueError("upload_box needs the separate-box layout")
+        t = self.tensors[b]
+        if tuple(arr.shape) != tuple(t.shape):
+            raise ValueError(f"box {b}: shape {arr.shape} != {tuple(t.shape)}")
+        t.copy_(torch.from_numpy(np.ascontiguousarray(arr)))
+
     def download(self):
         """Valid cells of every box as one (ncomp, nz, ny, nx) numpy array."""
         torch = _torch_mod()

@@ -116,6 +116,23 @@ def tgv_closures(mach=0.1, Re=1600.0, L=1.0, T0=110.4, Pr=0.71, rho0=1.0):
 def tgv_state(cls, g, n, mach=0.1, Re=1600.0, L=1.0, T0=110.4):
     """exm/TGV/prob.h:92-134 (omega = 1, convecting = false)."""
     X, Y, Z = cell_centres(g, n)
+    return _tgv_fields(cls, X, Y, Z, mach, Re, L, T0)
+
+
+def box_centres(g, lo, hi, ng):
+    """Cell centres of the ghosted box [lo-ng, hi+ng] (index arrays, ij order)."""
+    ax = [g.prob_lo[d] + (np.arange(lo[d] - ng, hi[d] + ng + 1) + 0.5) * g.dx[d] for d in range(3)]
+    return np.meshgrid(*ax, indexing="ij")
+
+
+def tgv_state_box(cls, g, lo, hi, ng=3, mach=0.1, Re=1600.0, L=1.0, T0=110.4):
+    """The TGV initial state on the ghosted box [lo-ng, hi+ng] (FAB order);
+    ghost cells hold the analytic field (periodic in 2 pi)."""
+    X, Y, Z = box_centres(g, lo, hi, ng)
+    return _tgv_fields(cls, X, Y, Z, mach, Re, L, T0)
+
+
+def _tgv_fields(cls, X, Y, Z, mach, Re, L, T0):
     cs = math.sqrt(cls.gamma * cls.Rspec * T0)
     v0 = mach * cs
     if cls.visc_law == 0:
@@ -139,9 +156,11 @@ def tgv_state(cls, g, n, mach=0.1, Re=1600.0, L=1.0, T0=110.4):
     return to_fab_order(u)
 
 
-def tgv_vorticity_magnitude(g, n, mach=0.1, L=1.0, T0=110.4, cls=None):
-    """Analytic |omega| of the TGV velocity field at t = 0."""
-    X, Y, Z = cell_centres(g, n)
+def tgv_vorticity_magnitude(g, n, mach=0.1, L=1.0, T0=110.4, cls=None, X=None, Y=None, Z=None):
+    """Analytic |omega| of the TGV velocity field at t = 0 (at the cell
+    centres of the domain, or at the given coordinates)."""
+    if X is None:
+        X, Y, Z = cell_centres(g, n)
     gam = 1.4 if cls is None else cls.gamma
     Rs = (8.31451 / 28.96e-3) if cls is None else cls.Rspec
     v0 = mach * math.sqrt(gam * Rs * T0)
@@ -150,6 +169,127 @@ def tgv_vorticity_magnitude(g, n, mach=0.1, L=1.0, T0=110.4, cls=None):
     wy = -v0 * np.sin(X) * np.cos(Y) * np.sin(Z) / L  # du/dz - dw/dx
     wz = 2.0 * v0 * np.sin(X) * np.sin(Y) * np.cos(Z) / L  # dv/dx - du/dy
     return np.sqrt(wx * wx + wy * wy + wz * wz)
+
+
+# ---------------------------------------------------------------- AMR hierarchy
+
+
+def cluster_blocks(tags, max_blocks, eff=0.7):
+    """Berger-Rigoutsos style clustering of a boolean block mask tags[i,j,k]
+    into boxes (inclusive block bounds): shrink to the tags' bounding box;
+    accept it when the tagged fraction is >= eff and no side exceeds
+    max_blocks; otherwise cut at a hole of the signature nearest the centre,
+    else at the strongest inflection of the signature, else in the middle of
+    the longest side, and recurse."""
+    out = []
+    stack = [((0, 0, 0), tuple(n - 1 for n in tags.shape))]
+    while stack:
+        lo, hi = stack.pop()
+        sub = tags[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1]
+        if not sub.any():
+            continue
+        nlo, nhi = [], []
+        for d in range(3):
+            others = tuple(a for a in range(3) if a != d)
+            nz = np.nonzero(sub.any(axis=others))[0]
+            nlo.append(lo[d] + int(nz[0]))
+            nhi.append(lo[d] + int(nz[-1]))
+        lo, hi = tuple(nlo), tuple(nhi)
+        sub = tags[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1]
+        size = [hi[d] - lo[d] + 1 for d in range(3)]
+        if sub.sum() >= eff * sub.size and max(size) <= max_blocks:
+            out.append((lo, hi))
+            continue
+        cut = None
+        sigs = [sub.sum(axis=tuple(a for a in range(3) if a != d)) for d in range(3)]
+        best = None
+        for d in range(3):  # holes
+            if size[d] < 2:
+                continue
+            for c in range(1, size[d]):
+                if sigs[d][c - 1] == 0 or sigs[d][c] == 0:
+                    dist = abs(c - size[d] / 2)
+                    if best is None or dist < best[0]:
+                        best = (dist, d, c)
+        if best is None:  # inflection of the Laplacian of the signature
+            for d in range(3):
+                if size[d] < 4:
+                    continue
+                lap = sigs[d][:-2] - 2 * sigs[d][1:-1] + sigs[d][2:]
+                for c in range(1, len(lap)):
+                    if lap[c - 1] * lap[c] < 0:
+                        jump = abs(lap[c] - lap[c - 1])
+                        if best is None or jump > best[0]:
+                            best = (jump, d, c + 1)
+        if best is not None:
+            _, d, c = best
+        else:
+            d = int(np.argmax(size))
+            c = size[d] // 2
+        if max(size) > max_blocks and size[d] <= max_blocks:
+            d = int(np.argmax(size))
+            c = size[d] // 2
+        cut = lo[d] + c
+        h1 = list(hi)
+        h1[d] = cut - 1
+        l2 = list(lo)
+        l2[d] = cut
+        stack.append((lo, tuple(h1)))
+        stack.append((tuple(l2), hi))
+    return sorted(out, key=lambda b: (b[0][2], b[0][1], b[0][0]))
+
+
+def _intersect(a, b):
+    lo = tuple(max(a[0][d], b[0][d]) for d in range(3))
+    hi = tuple(min(a[1][d], b[1][d]) for d in range(3))
+    return (lo, hi) if all(lo[d] <= hi[d] for d in range(3)) else None
+
+
+def tgv_amr_hierarchy(n_base=128, nlev=2, ratio=2, pct=70.0, blocking=8, max_grid=64, base_grid=64, eff=0.7):
+    """Levels of the batched-AMR TGV workload on [0, 2 pi]^3: a uniform base
+    level, then `nlev` refined levels (ratio 2).  On each level the cells
+    whose analytic |omega| (t = 0) exceeds the level's `pct` percentile are
+    tagged; tags are clustered in blocks of blocking/ratio coarse cells into
+    boxes of at most max_grid fine cells, refined, and clipped to the coarse
+    level's boxes (proper nesting).  Returns [(geom, boxes)] per level."""
+    levels = []
+    n = (n_base,) * 3
+    g = geom(n, (0, 0, 0), (2 * math.pi,) * 3, (1, 1, 1))
+    boxes = decompose(n, base_grid)
+    levels.append((g, boxes))
+    cb = blocking // ratio          # block edge in coarse cells
+    mb = max_grid // blocking       # max box edge in blocks
+    for _ in range(nlev):
+        g, boxes = levels[-1]
+        n = [g.dom_hi[d] + 1 for d in range(3)]
+        # |omega| on the level's cells, its percentile, tags by block
+        vals = []
+        for lo, hi in boxes:
+            X, Y, Z = box_centres(g, lo, hi, 0)
+            vals.append(tgv_vorticity_magnitude(g, n, X=X, Y=Y, Z=Z))
+        thr = np.percentile(np.concatenate([v.ravel() for v in vals]), pct)
+        nb = [n[d] // cb for d in range(3)]
+        tags = np.zeros(nb, dtype=bool)
+        for (lo, hi), v in zip(boxes, vals):
+            t = v > thr
+            sh = [(hi[d] - lo[d] + 1) // cb for d in range(3)]
+            blk = t.reshape(sh[0], cb, sh[1], cb, sh[2], cb).any(axis=(1, 3, 5))
+            tags[lo[0] // cb:lo[0] // cb + sh[0], lo[1] // cb:lo[1] // cb + sh[1],
+                 lo[2] // cb:lo[2] // cb + sh[2]] |= blk
+        clusters = cluster_blocks(tags, mb, eff)
+        fine = []
+        for blo, bhi in clusters:
+            clo = tuple(blo[d] * cb for d in range(3))
+            chi = tuple((bhi[d] + 1) * cb - 1 for d in range(3))
+            for B in boxes:
+                x = _intersect((clo, chi), B)
+                if x is not None:
+                    fine.append((tuple(x[0][d] * ratio for d in range(3)),
+                                 tuple((x[1][d] + 1) * ratio - 1 for d in range(3))))
+        nf = [n[d] * ratio for d in range(3)]
+        gf = geom(nf, (0, 0, 0), (2 * math.pi,) * 3, (1, 1, 1))
+        levels.append((gf, sorted(fine, key=lambda b: (b[0][2], b[0][1], b[0][0]))))
+    return levels
 
 
 # ---------------------------------------------------------------- channel-like box

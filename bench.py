@@ -39,10 +39,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=N)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run plain device copies of this many bytes (profiler calibration)")
     ap.add_argument("--workload", type=str, default="tgv",
-                    choices=["tgv", "sod-w", "sod-r", "riemann-batch-w", "riemann-batch-r"])
+                    choices=["tgv", "sod-w", "sod-r", "riemann-batch-w", "riemann-batch-r", "adjoint", "amr"])
     ap.add_argument("--tiling", type=str, default="auto",
                     help="fused-kernel tiling: auto (autotune), default, or TX,TY,TZ")
     ap.add_argument("--traffic", type=str, default=os.path.join(ROOT, "profiles", "traffic_tgv256_keep4v.json"))
@@ -195,6 +196,144 @@ def run_riemann_batch(a, flux):
                       "cell_stages": done, "batch_steps": k})
 
 
+def adjoint_timing(a, fd_dirs=0):
+    """Channel-like periodic box 128x64x64 (2 boxes of 64^3), KEEP4 + viscous,
+    SSPRK(3,3), fixed dt from CFL 0.3, objective = kinetic energy after 20
+    steps; gradient by the checkpointed adjoint (every 5 steps).  Returns the
+    time ratio adjoint run / plain forward, and with fd_dirs > 0 the largest
+    relative difference to central differences on seeded unit directions
+    (step size per direction from a sweep)."""
+    import torch
+    from amrsolver_amd import level as LV
+    from amrsolver_amd import problems as P
+    n, nsteps, kcheck = (128, 64, 64), 20, 5
+    cls, p0, _ = P.channel_closures()
+    g = P.channel_geom(n)
+    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
+    lev = LV.Level(g, P.decompose(n, BOX), cls, sch)
+    u = P.channel_state(cls, g, n, p0)
+    S0, S = lev.new_state(), lev.new_state()
+    grad = lev.new_state(ng=0)
+    S0.upload(u)
+    lev.fill_patch(S0)
+    dt = lev.est_time_step(S0, 0.3)
+    tiling = lev.autotune(S0)
+
+    def forward():
+        for _ in range(nsteps):
+            lev.advance(S, 0.0, dt)
+
+    reps = 5
+    S.upload(u)
+    forward()
+    lev.adjoint_run(S0, dt, nsteps, kcheck, grad)
+    torch.cuda.synchronize()
+    tf = []
+    for _ in range(reps):
+        S.upload(u)
+        lev.synchronize()
+        t0 = time.perf_counter()
+        forward()
+        lev.synchronize()
+        tf.append(time.perf_counter() - t0)
+    ta = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        J = lev.adjoint_run(S0, dt, nsteps, kcheck, grad)
+        ta.append(time.perf_counter() - t0)
+    t_fwd, t_adj = min(tf), min(ta)
+    out = {"workload": f"channel {n[0]}x{n[1]}x{n[2]}, 2 boxes of 64^3, KEEP4+viscous, SSPRK(3,3), "
+                       f"{nsteps} steps, checkpoint every {kcheck}", "ratio": t_adj / t_fwd,
+           "forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3, "J": J, "dt": dt, "tiling": tiling}
+    if fd_dirs:
+        gr = grad.download()
+        rng = np.random.default_rng(20261202)
+        worst = 0.0
+        steps_used = []
+
+        def J_of(h, v):
+            S.upload(u + h * v)
+            forward()
+            return P.kinetic_energy(S.download(), g)
+
+        for _ in range(fd_dirs):
+            v = rng.standard_normal(u.shape)
+            v /= np.linalg.norm(v)
+            got = float(np.sum(gr * v))
+            # sweep: fourth-order central differences at decreasing steps; keep
+            # the step whose estimate changes least against the next one
+            hs = [0.08, 0.04, 0.02, 0.01, 0.005]
+            ests = [(8.0 * (J_of(h, v) - J_of(-h, v)) - (J_of(2 * h, v) - J_of(-2 * h, v))) / (12.0 * h) for h in hs]
+            k = int(np.argmin([abs(ests[i + 1] - ests[i]) for i in range(len(hs) - 1)]))
+            fd = ests[k + 1]
+            steps_used.append(hs[k + 1])
+            worst = max(worst, abs(got - fd) / abs(fd))
+        out["fd_directions"] = fd_dirs
+        out["fd_max_rel_err"] = worst
+        out["fd_steps"] = sorted(set(steps_used))
+    return out
+
+
+def amr_timing(a, n_base=128, steps=5, reps=3):
+    """Batched-AMR TGV levels (base n_base^3, 2 levels of ratio 2, tags where
+    the analytic |omega| exceeds the level's 70th percentile, blocking factor
+    8, boxes of 8..64 cells): every level is one box table (separate ghosted
+    FABs), autotuned, and advanced by hipGraph-captured SSPRK(3,3) steps.
+    Per-cell cost = step time / (3 * cells), against the uniform 256^3 level
+    with the same scheme (level-wide array, and separate 64^3 FABs)."""
+    import torch
+    from amrsolver_amd import level as LV
+    from amrsolver_amd import problems as P
+    cls = P.tgv_closures()
+    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
+    ctx = LV.Context(0)
+
+    def time_level(g, boxes, layout, dt):
+        lev = LV.Level(g, boxes, cls, sch, ctx=ctx)
+        S = lev.new_state(layout=layout)
+        if layout == "boxes":
+            for b, (lo, hi) in enumerate(lev.boxes):
+                S.upload_box(b, P.tgv_state_box(cls, g, lo, hi, 3))
+        else:
+            n = [g.dom_hi[d] + 1 for d in range(3)]
+            S.upload(P.tgv_state(cls, g, n))
+        lev.fill_patch(S)
+        tiling = lev.autotune(S)
+        lev.advance_graph(S, dt, steps)
+        lev.synchronize()
+        best = 1e30
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            lev.advance_graph(S, dt, steps)
+            lev.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        ncell = sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) for lo, hi in lev.boxes)
+        ms = best / steps * 1e3
+        out = {"boxes": len(boxes), "cells": ncell, "tiling": tiling, "ms_per_step": ms,
+               "ns_per_cell_stage": ms * 1e6 / (3 * ncell)}
+        lev.close()
+        del S
+        torch.cuda.empty_cache()
+        return out
+
+    n = (256, 256, 256)
+    gu = P.geom(n, (0, 0, 0), (2 * math.pi,) * 3, (1, 1, 1))
+    uni_level = time_level(gu, P.decompose(n, BOX), "level", DT)
+    uni_boxes = time_level(gu, P.decompose(n, BOX), "boxes", DT)
+    levels = []
+    for l, (g, boxes) in enumerate(P.tgv_amr_hierarchy(n_base=n_base, nlev=2)):
+        if l == 0:
+            continue
+        r = time_level(g, boxes, "boxes", DT / 2 ** (l - 1))
+        r["level"] = l
+        r["vs_uniform_level_layout"] = r["ns_per_cell_stage"] / uni_level["ns_per_cell_stage"]
+        r["vs_uniform_box_layout"] = r["ns_per_cell_stage"] / uni_boxes["ns_per_cell_stage"]
+        levels.append(r)
+    return {"workload": f"TGV AMR base {n_base}^3 + 2 levels (ratio 2, |omega| > 70th percentile, blocking 8, "
+                        f"boxes 8..64), KEEP4+viscous, SSPRK(3,3), hipGraph of {steps} steps",
+            "uniform_256_level_layout": uni_level, "uniform_256_box_layout": uni_boxes, "levels": levels}
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -203,6 +342,12 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
         return 2
+    if a.workload == "amr":
+        print(json.dumps(amr_timing(a)))
+        return 0
+    if a.workload == "adjoint":
+        print(json.dumps(adjoint_timing(a, fd_dirs=32)))
+        return 0
     if a.workload != "tgv":
         fn = {"sod-w": lambda: run_sod(a, 2), "sod-r": lambda: run_sod(a, 0),
               "riemann-batch-w": lambda: run_riemann_batch(a, 2),
@@ -302,6 +447,8 @@ def main():
         "tiling": tiling,
         "totals_drift": {"mass": drift[0], "energy": drift[1]},
     }
+    if not a.no_adjoint:
+        out["adjoint"] = adjoint_timing(a)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

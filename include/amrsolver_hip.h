@@ -165,15 +165,17 @@ int amr_batch_advance(amr_boxtab_t bt, const amr_closures_t* cls, const amr_sche
 int amr_batch_est_dt(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, double cfl,
                      double* dt_out);
 
-/* Tiling of the fused stage kernel for a scheme on this context:
- * (tx,ty,tz) = (0,0,0) automatic (default or autotuned), tx = -1 forces the
+/* Tiling of the fused stage kernel for a scheme on every box table of this
+ * context: (tx,ty,tz) = (0,0,0) automatic (each table's autotuned choice,
+ * else a default), tx = -1 forces the
  * pass-structured kernels, otherwise one of the compiled shapes
  * (16x16, 32x8, 8x8 columns; tz planes per workgroup).  No reference
  * counterpart: AMReX picks launch shapes itself (ParallelFor). */
 int amr_set_tiling(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, int tz);
 
 /* Time every compiled tiling that fits the level on `state` (ghost-filled,
- * ng >= 3) and keep the fastest for this scheme on the context.  Writes the
+ * ng >= 3) and keep the fastest for this scheme on this box table (it
+ * clears an explicit amr_set_tiling for the scheme).  Writes the
  * choice to tiling[3] (tx,ty,tz; tx = -1 when only the pass-structured
  * kernels apply).  Synchronous; leaves `state` unchanged. */
 int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch,
