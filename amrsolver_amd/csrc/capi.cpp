@@ -642,12 +642,16 @@ FusedConfig default_config(const amr_boxtab_s* bt) {
   return c;
 }
 
-// The fused kernel needs every box to be a whole number of tiles in x and y
-// and a state array with at least three ghost cells.
+// The fused kernel takes any box (tiles overhanging a box in x or y are
+// masked); it skips only shapes whose tiles would be mostly idle.
 bool fused_fits(const amr_boxtab_s* bt, const FusedConfig& cfg) {
-  for (auto& B : bt->bt.h_boxes)
-    if ((B.hi[0] - B.lo[0] + 1) % cfg.tx != 0 || (B.hi[1] - B.lo[1] + 1) % cfg.ty != 0) return false;
-  return true;
+  int64_t cells_ = 0, lanes = 0;
+  for (auto& B : bt->bt.h_boxes) {
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+    cells_ += int64_t(nx) * ny * nz;
+    lanes += int64_t((nx + cfg.tx - 1) / cfg.tx) * cfg.tx * ((ny + cfg.ty - 1) / cfg.ty) * cfg.ty * nz;
+  }
+  return 2 * cells_ >= lanes;
 }
 
 FusedConfig choose_config(const amr_boxtab_s* bt, const amr_scheme_t& sch) {
@@ -697,7 +701,12 @@ int alloc_mirror(amr_boxtab_s* bt, const amr_fab_t* state, double** mem, std::ve
     sum += compact_len(f.lo, f.hi, f.ng, NCONS);
   }
   int64_t span = int64_t(hi - lo) / 8;
-  bool mirror = span <= sum + sum / 2;
+  // windows of one array (every box maps cell (0,0,0) to the same address)
+  // are mirrored whatever the span: their ghost cells are neighbours' cells
+  bool shared = true;
+  const Mapping m0 = mapping_of(state[0]);
+  for (int b = 1; b < nb && shared; ++b) shared = mapping_of(state[b]) == m0;
+  bool mirror = shared || span <= sum + sum / 2;
   if (mirror) {
     HIPCHK(hipMalloc(mem, sizeof(double) * span));
     for (int b = 0; b < nb; ++b) {

@@ -129,6 +129,10 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
   const int tid = threadIdx.x;
   const int ti = tid % TX, tj = tid / TX;
   const int gi = T.i0 + ti, gj = T.j0 + tj;
+  // a tile may overhang the box in x and y: loads are clamped to the box's
+  // ghosted extent and only cells of the box are written
+  const int xcap = boxes[T.box].hi[0] + NGHOST, ycap = boxes[T.box].hi[1] + NGHOST;
+  const bool own_in = gi <= boxes[T.box].hi[0] && gj <= boxes[T.box].hi[1];
   const int k0 = T.k0, nz = T.nz;
   const int kfirst = k0 - C::KB;
   const int klast = k0 + nz + 1;
@@ -158,7 +162,7 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
   for (int s = 0; s < C::NLOAD; ++s) {
     int cell = tid + s * NT;
     int ii = cell % RX, jj = cell / RX;
-    loff[s] = (T.i0 - HX + ii) + (T.j0 - HX + jj) * int(S.js);
+    loff[s] = min(T.i0 - HX + ii, xcap) + min(T.j0 - HX + jj, ycap) * int(S.js);
   }
   const int soff = gi + gj * int(S.js);
   double pre[C::NLOAD][5];
@@ -229,7 +233,7 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
       if (pn + 1 <= klast) load_plane(pn + 1);
     }
     // epilogue operands of the plane finalised in this iteration
-    const bool fin = cpl >= k0 + 1;
+    const bool fin = cpl >= k0 + 1 && own_in;
     double sv[5], s1v[5];
     if (fin) {
       const gdouble* pb = sbase + int64_t(cpl - 1) * S.ks;
@@ -441,6 +445,7 @@ bool launch_shape(const amr_scheme_t& s, const TileDev* tiles, int ntiles, const
 }  // namespace
 
 bool fused_supported(const amr_scheme_t& s, int tx, int ty) {
+  // any box shape: tiles overhanging a box are masked
   if (!s.rhs_euler || s.flux_euler != 1) return false;
   if (!(s.order_keep == 2 || s.order_keep == 4)) return false;
   return (tx == 16 && ty == 16) || (tx == 8 && ty == 8) || (tx == 32 && ty == 8);

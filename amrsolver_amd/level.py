@@ -119,6 +119,20 @@ class StateArrays:
                 t[:, g:t.shape[1] - g, g:t.shape[2] - g, g:t.shape[3] - g] = torch.from_numpy(np.ascontiguousarray(sub)).to(t.device)
         torch.cuda.synchronize()
 
+    def upload_ghosted(self, arr):
+        """The whole level array including every ghost cell,
+        (ncomp, N2+2g, N1+2g, N0+2g) (level layout only); arr may be a numpy
+        array or a device tensor."""
+        torch = _torch_mod()
+        if self.layout != "level":
+            raise ValueError("upload_ghosted needs the level layout")
+        t = self.tensors[0]
+        if tuple(arr.shape) != tuple(t.shape):
+            raise ValueError(f"shape {tuple(arr.shape)} != {tuple(t.shape)}")
+        if isinstance(arr, np.ndarray):
+            arr = torch.from_numpy(np.ascontiguousarray(arr))
+        t.copy_(arr)
+
     def upload_box(self, b, arr):
         """Whole ghosted array of box b, (ncomp, nz+2g, ny+2g, nx+2g)
         (separate-box layout only)."""

@@ -96,3 +96,31 @@ def test_autotune_picks_a_fitting_tiling(ctx):
     assert t[0] in (8, 16, 32) and t[2] > 0
     assert np.array_equal(before, S.download())
     lev.set_tiling(0, 0, 0)
+
+
+@pytest.mark.parametrize("layout", ["level", "boxes"])
+@pytest.mark.parametrize("tiling", [(16, 16, 64), (32, 8, 8), (8, 8, 16)], ids=["16x16", "32x8", "8x8"])
+def test_fused_partial_tiles(ctx, tiling, layout):
+    """Boxes whose x/y extents are not multiples of the tile (tiles overhang
+    the box and are masked), including boxes narrower than one tile."""
+    from amrsolver_amd import level as L
+    from oracle import oracle as O
+    n = (40, 28, 12)
+    cls = P.closures_reference()
+    g = P.geom(n, (0.0, 0.0, 0.0), (1.0, 0.7, 0.3), (1, 1, 1))
+    boxes = [((0, 0, 0), (23, 11, 11)), ((24, 0, 0), (39, 11, 11)), ((0, 12, 0), (7, 27, 11)),
+             ((8, 12, 0), (39, 27, 5)), ((8, 12, 6), (39, 27, 11))]
+    u = P.smooth_random_state(cls, n, 17)
+    sch = P.scheme(flux_euler=1, order_keep=4, visc=True)
+    print(f"case fused partial {tiling} {layout}", flush=True)
+    lev = L.Level(g, boxes, cls, sch, ctx=ctx)
+    lev.set_tiling(*tiling)
+    S = lev.new_state(layout=layout)
+    S.upload(u)
+    ol = O.OracleLevel(g, boxes)
+    ol.set_state(u)
+    for _ in range(2):
+        lev.advance(S, 0.0, 1e-6)
+        ol.advance(cls, sch, 1e-6)
+    lev.set_tiling(0, 0, 0)
+    assert rel_err(S.download(), ol.get_state()) <= TOL
