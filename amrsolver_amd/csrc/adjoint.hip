@@ -15,9 +15,10 @@
 //   * viscous: a pre-pass stores per cell of grow(b,1) the adjoints of the
 //     velocity/temperature differences and of the cell's own values;
 //   * cons2prim is transposed per cell.
-// The ghost-fill is then transposed (BC tasks in reverse direction order,
-// then the copy tasks) with atomic accumulation into the source cells, and
-// the stage combination is transposed on the valid cells.
+// The stage combination is transposed on the valid cells in the same pass;
+// the ghost fill is then transposed (BC tasks in reverse direction order,
+// then the copy tasks) with weighted atomic accumulation into the stage
+// adjoint at the source cells.
 #include "cns_point.h"
 #include "kernels.h"
 #include "region.h"
@@ -211,20 +212,25 @@ __global__ void __launch_bounds__(BLK) k_adj_visc(const BoxDev* boxes, const Blo
   E(i, j, k, 15) = hz * (-kk * cc[UET]);
 }
 
-// Per cell of grow(b,3): adjoint of the primitives (KEEP pairs + viscous
-// terms), transposed through cons2prim into G.
+// Reverse of the RHS plus the stage combination, one cell of grow(b,3) per
+// thread (global-memory gathers; the neighbourhoods are cache resident).
+// On valid cells the stage transpose is applied at once
+//   mode 0: lam_sin (=|+=) b (c lam_out + d G),  mode 1: c lam_out + d G,
+//   mode 2: G;   lam_s1 (+=|=) a lam_out
+// where G = (d dSdt / d S)^T lam_out; on ghost cells G itself is stored for
+// the transposed ghost fill.
 template <int H, bool EULER, bool VISC>
 __global__ void __launch_bounds__(BLK) k_adj_cell(const BoxDev* boxes, const BlockRef* bmap, Region reg,
                                                   const double* box_dt, const Fab* pr, const Fab* lam,
                                                   const Fab* ew, const Fab* gw, Closures c, double co0,
-                                                  double co1, double co2) {
+                                                  double co1, double co2, StageArgs sa, const Fab* lam_s1,
+                                                  const Fab* lam_sin, int accumulate, int s1_mode) {
   REGION_LOOP_BEGIN
-  const Fab q = pr[b], G = gw[b];
+  const Fab q = pr[b], L = lam[b];
   double qc[6];
   load_q(q, i, j, k, qc);
   double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (EULER) {
-    const Fab L = lam[b];
     const double co[3] = {co0, co1, co2};
     adj_keep_dir<H, 0>(q, L, bx, i, j, k, qc, co, bx.dxi[0], c.cv, g);
     adj_keep_dir<H, 1>(q, L, bx, i, j, k, qc, co, bx.dxi[1], c.cv, g);
@@ -233,25 +239,19 @@ __global__ void __launch_bounds__(BLK) k_adj_cell(const BoxDev* boxes, const Blo
   if (VISC) {
     const Fab E = ew[b];
     if (in_box(bx, 1, i, j, k)) {
-      g[QU] += E(i, j, k, 0);
-      g[QV] += E(i, j, k, 1);
-      g[QW] += E(i, j, k, 2);
-      g[QT] += E(i, j, k, 3);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) g[QU + m] += E(i, j, k, m);
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const int di = d == 0, dj = d == 1, dk = d == 2;
       if (in_box(bx, 1, i - di, j - dj, k - dk)) {
-        g[QU] += E(i - di, j - dj, k - dk, 4 + 4 * d);
-        g[QV] += E(i - di, j - dj, k - dk, 5 + 4 * d);
-        g[QW] += E(i - di, j - dj, k - dk, 6 + 4 * d);
-        g[QT] += E(i - di, j - dj, k - dk, 7 + 4 * d);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) g[QU + m] += E(i - di, j - dj, k - dk, 4 + 4 * d + m);
       }
       if (in_box(bx, 1, i + di, j + dj, k + dk)) {
-        g[QU] -= E(i + di, j + dj, k + dk, 4 + 4 * d);
-        g[QV] -= E(i + di, j + dj, k + dk, 5 + 4 * d);
-        g[QW] -= E(i + di, j + dj, k + dk, 6 + 4 * d);
-        g[QT] -= E(i + di, j + dj, k + dk, 7 + 4 * d);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) g[QU + m] -= E(i + di, j + dj, k + dk, 4 + 4 * d + m);
       }
     }
   }
@@ -260,12 +260,41 @@ __global__ void __launch_bounds__(BLK) k_adj_cell(const BoxDev* boxes, const Blo
   const double ux = qc[QU], uy = qc[QV], uz = qc[QW];
   const double gp = g[QPRES] + g[QT] * rinv / c.Rspec;
   const double gr = g[QRHO] - g[QT] * qc[QT] * rinv;
-  G(i, j, k, URHO) = gr + c.gm1 * 0.5 * (ux * ux + uy * uy + uz * uz) * gp -
-                     (g[QU] * ux + g[QV] * uy + g[QW] * uz) * rinv;
-  G(i, j, k, UMX) = g[QU] * rinv - c.gm1 * ux * gp;
-  G(i, j, k, UMY) = g[QV] * rinv - c.gm1 * uy * gp;
-  G(i, j, k, UMZ) = g[QW] * rinv - c.gm1 * uz * gp;
-  G(i, j, k, UET) = c.gm1 * gp;
+  double G[5];
+  G[URHO] = gr + c.gm1 * 0.5 * (ux * ux + uy * uy + uz * uz) * gp - (g[QU] * ux + g[QV] * uy + g[QW] * uz) * rinv;
+  G[UMX] = g[QU] * rinv - c.gm1 * ux * gp;
+  G[UMY] = g[QV] * rinv - c.gm1 * uy * gp;
+  G[UMZ] = g[QW] * rinv - c.gm1 * uz * gp;
+  G[UET] = c.gm1 * gp;
+  if (!in_box(bx, 0, i, j, k)) {
+    const Fab Gh = gw[b];
+#pragma unroll
+    for (int n = 0; n < NCONS; ++n) Gh(i, j, k, n) = G[n];
+    return;
+  }
+  const StageCoef sc = box_stage(sa, b);
+  const Fab Li = lam_sin[b];
+  double lo[5];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) lo[n] = L(i, j, k, n);
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) {
+    double v;
+    if (sc.mode == 0) v = sc.b * (sc.c * lo[n] + sc.d * G[n]);
+    else if (sc.mode == 1) v = sc.c * lo[n] + sc.d * G[n];
+    else v = G[n];
+    if (accumulate) v = Li(i, j, k, n) + v;
+    Li(i, j, k, n) = v;
+  }
+  if (s1_mode != 0 && lam_s1 != nullptr) {
+    const Fab S = lam_s1[b];
+    const double a = sc.mode == 2 ? 0.0 : sc.a;
+#pragma unroll
+    for (int n = 0; n < NCONS; ++n) {
+      if (s1_mode == 2) S(i, j, k, n) = a * lo[n];
+      else if (a != 0.0) S(i, j, k, n) = S(i, j, k, n) + a * lo[n];
+    }
+  }
 }
 
 constexpr int FB = 256;
@@ -281,13 +310,22 @@ __device__ __forceinline__ int find_task(const T* tasks, int ntask, int64_t t) {
   return lo;
 }
 
+// Weight of G in lam_sin for a box's stage (b d, d or 1).
+__device__ __forceinline__ double g_weight(const StageArgs& sa, int box) {
+  const StageCoef sc = box_stage(sa, box);
+  return sc.mode == 0 ? sc.b * sc.d : (sc.mode == 1 ? sc.d : 1.0);
+}
+
 // Transposed physical-BC fill: the ghost's adjoint moves into its source
-// cell (foextrap: boundary cell; reflect: mirror, negated for odd).  The
-// ghost is cleared, because the forward overwrote it.
-__global__ void __launch_bounds__(FB) k_fold_bc(const BcTask* tasks, int ntask, int64_t ncell, const Fab* fabs) {
+// cell (foextrap: boundary cell; reflect: mirror, negated for odd) -- into
+// lam_sin (weighted) when the source is a valid cell, else into the source
+// ghost of G.  The ghost is cleared, because the forward overwrote it.
+__global__ void __launch_bounds__(FB) k_fold_bc(const BcTask* tasks, int ntask, int64_t ncell, const Fab* gfabs,
+                                                const BoxDev* boxes, StageArgs sa, const Fab* lam_sin) {
   int64_t t = int64_t(blockIdx.x) * FB + threadIdx.x;
   if (t >= ncell) return;
   const BcTask& T = tasks[find_task(tasks, ntask, t)];
+  if (!box_active(sa, T.box)) return;
   int64_t r = t - T.start;
   int nx = T.hi[0] - T.lo[0] + 1, ny = T.hi[1] - T.lo[1] + 1;
   int c[3];
@@ -295,7 +333,9 @@ __global__ void __launch_bounds__(FB) k_fold_bc(const BcTask* tasks, int ntask, 
   int64_t r2 = r / nx;
   c[1] = T.lo[1] + int(r2 % ny);
   c[2] = T.lo[2] + int(r2 / ny);
-  const Fab f = fabs[T.box];
+  const Fab f = gfabs[T.box], Li = lam_sin[T.box];
+  const BoxDev bx = boxes[T.box];
+  const double w = g_weight(sa, T.box);
   const int cd = c[T.dir];
   for (int n = 0; n < NCONS; ++n) {
     int ty = T.type[n];
@@ -305,63 +345,30 @@ __global__ void __launch_bounds__(FB) k_fold_bc(const BcTask* tasks, int ntask, 
     else s[T.dir] = T.side == 0 ? 2 * T.bnd - 1 - cd : 2 * T.bnd + 1 - cd;
     double v = f(c[0], c[1], c[2], n);
     f(c[0], c[1], c[2], n) = 0.0;
-    atomicAdd(&f(s[0], s[1], s[2], n), ty == 4 ? -v : v);
+    if (ty == 4) v = -v;
+    if (in_box(bx, 0, s[0], s[1], s[2])) atomicAdd(&Li(s[0], s[1], s[2], n), w * v);
+    else atomicAdd(&f(s[0], s[1], s[2], n), v);
   }
 }
 
-// Transposed same-level / periodic copies: ghost adjoint of `dst` added to
-// the valid source cell of `src`.
+// Transposed same-level / periodic copies: ghost adjoint of `dst` added,
+// weighted, to lam_sin at the valid source cell of `src`.
 __global__ void __launch_bounds__(FB) k_fold_copy(const CopyTask* tasks, int ntask, int64_t ncell,
-                                                  const Fab* fabs) {
+                                                  const Fab* gfabs, StageArgs sa, const Fab* lam_sin) {
   int64_t t = int64_t(blockIdx.x) * FB + threadIdx.x;
   if (t >= ncell) return;
   const CopyTask& T = tasks[find_task(tasks, ntask, t)];
+  if (!box_active(sa, T.dst)) return;
   int64_t r = t - T.start;
   int nx = T.hi[0] - T.lo[0] + 1, ny = T.hi[1] - T.lo[1] + 1;
   int i = T.lo[0] + int(r % nx);
   int64_t r2 = r / nx;
   int j = T.lo[1] + int(r2 % ny);
   int k = T.lo[2] + int(r2 / ny);
-  const Fab d = fabs[T.dst], s = fabs[T.src];
+  const Fab d = gfabs[T.dst], s = lam_sin[T.src];
+  const double w = g_weight(sa, T.src);
 #pragma unroll
-  for (int n = 0; n < NCONS; ++n) atomicAdd(&s(i - T.sh[0], j - T.sh[1], k - T.sh[2], n), d(i, j, k, n));
-}
-
-// Stage combination transposed on the valid cells:
-//   mode 0  out = a s1 + b (c sin + d L):  lam_sin = b (c lam_out + d G)
-//   mode 1  out = a s1 + c sin + d L:      lam_sin = c lam_out + d G
-//   mode 2  out = L:                       lam_sin = G
-// lam_sin is assigned or accumulated; lam_s1 += a lam_out (modes 0, 1).
-__global__ void __launch_bounds__(BLK) k_adj_combine(const BoxDev* boxes, const BlockRef* bmap, Region reg,
-                                                     const double* box_dt, StageArgs sa, const Fab* gw,
-                                                     const Fab* lam_out, const Fab* lam_s1, const Fab* lam_sin,
-                                                     int accumulate) {
-  REGION_LOOP_BEGIN
-  const StageCoef sc = box_stage(sa, b);
-  const Fab G = gw[b], Lo = lam_out[b], Li = lam_sin[b];
-  const bool s1 = lam_s1 != nullptr && sc.mode != 2 && sc.a != 0.0;
-#pragma unroll
-  for (int n = 0; n < NCONS; ++n) {
-    const double lo = Lo(i, j, k, n), gv = G(i, j, k, n);
-    double v;
-    if (sc.mode == 0) v = sc.b * (sc.c * lo + sc.d * gv);
-    else if (sc.mode == 1) v = sc.c * lo + sc.d * gv;
-    else v = gv;
-    if (accumulate) v = Li(i, j, k, n) + v;
-    Li(i, j, k, n) = v;
-    if (s1) {
-      const Fab S = lam_s1[b];
-      S(i, j, k, n) = S(i, j, k, n) + sc.a * lo;
-    }
-  }
-}
-
-__global__ void __launch_bounds__(BLK) k_zero_valid(const BoxDev* boxes, const BlockRef* bmap, Region reg,
-                                                    const double* box_dt, const Fab* f) {
-  REGION_LOOP_BEGIN
-  const Fab F = f[b];
-#pragma unroll
-  for (int n = 0; n < NCONS; ++n) F(i, j, k, n) = 0.0;
+  for (int n = 0; n < NCONS; ++n) atomicAdd(&s(i - T.sh[0], j - T.sh[1], k - T.sh[2], n), w * d(i, j, k, n));
 }
 
 constexpr int OB = 256;
@@ -434,61 +441,41 @@ bool adjoint_supported(const amr_scheme_t& s) {
   return s.rhs_euler || s.rhs_visc;
 }
 
-hipError_t launch_adj_rhs(const BoxTable& bt, const UnfusedWork& w, const AdjWork& aw, const Closures& c,
-                          const amr_scheme_t& s, const double* box_dt, const Fab* sin, const Fab* lam,
-                          hipStream_t st) {
+hipError_t launch_adj_stage(const BoxTable& bt, const UnfusedWork& w, const AdjWork& aw, const Closures& c, const amr_scheme_t& s, const StageArgs& sa,
+                            const Fab* sin, const Fab* lam_out, const Fab* lam_s1, const Fab* lam_sin,
+                            bool accumulate, int s1_mode, const FoldTasks& ft, hipStream_t st) {
   const BoxDev* B = bt.d_boxes;
-#define LAUNCH(KERNEL, REG, ...)                                                              \
-  do {                                                                                       \
-    Launch L_ = grid_for(bt, REG);                                                           \
-    if (!L_.map) return hipErrorNotInitialized;                                              \
-    hipLaunchKernelGGL(KERNEL, L_.grid, dim3(BLK), 0, st, B, L_.map, REG, box_dt, __VA_ARGS__); \
-  } while (0)
-  const Region g3 = grown(NGHOST), g1 = grown(1);
-  hipError_t e = launch_prims(bt, w, c, box_dt, sin, st);
+  hipError_t e = launch_prims(bt, w, c, sa.box_dt, sin, st);
   if (e != hipSuccess) return e;
-  if (s.rhs_visc) LAUNCH(k_adj_visc, g1, w.prims, lam, aw.e, c);
+  if (s.rhs_visc) {
+    const Region g1 = grown(1);
+    Launch L = grid_for(bt, g1);
+    if (!L.map) return hipErrorNotInitialized;
+    hipLaunchKernelGGL(k_adj_visc, L.grid, dim3(BLK), 0, st, B, L.map, g1, sa.box_dt, w.prims, lam_out, aw.e, c);
+  }
   double co[3] = {0.0, 0.0, 0.0};
   keep_coefficients(s.order_keep, co);
   const bool eu = s.rhs_euler != 0, vi = s.rhs_visc != 0;
   const int h = s.order_keep / 2;
-#define CELL(HH, EE, VV) LAUNCH((k_adj_cell<HH, EE, VV>), g3, w.prims, lam, aw.e, aw.g, c, co[0], co[1], co[2])
+  const Region g3 = grown(NGHOST);
+  Launch L3 = grid_for(bt, g3);
+  if (!L3.map) return hipErrorNotInitialized;
+#define CELL(HH, EE, VV)                                                                                     \
+  hipLaunchKernelGGL((k_adj_cell<HH, EE, VV>), L3.grid, dim3(BLK), 0, st, B, L3.map, g3, sa.box_dt, w.prims, \
+                     lam_out, aw.e, aw.g, c, co[0], co[1], co[2], sa, lam_s1, lam_sin, accumulate ? 1 : 0,   \
+                     s1_mode)
   if (!eu) CELL(1, false, true);
   else if (h == 3) { if (vi) CELL(3, true, true); else CELL(3, true, false); }
   else if (h == 2) { if (vi) CELL(2, true, true); else CELL(2, true, false); }
   else { if (vi) CELL(1, true, true); else CELL(1, true, false); }
 #undef CELL
-#undef LAUNCH
-  return hipGetLastError();
-}
-
-hipError_t launch_fold(const CopyTask* copies, int ncopy, int64_t ncopy_cells, BcTask* const bc[3],
-                       const int nbc[3], const int64_t nbc_cells[3], const Fab* g, hipStream_t st) {
   for (int d = 2; d >= 0; --d)
-    if (nbc_cells[d] > 0)
-      hipLaunchKernelGGL(k_fold_bc, dim3(unsigned((nbc_cells[d] + FB - 1) / FB)), dim3(FB), 0, st, bc[d], nbc[d],
-                         nbc_cells[d], g);
-  if (ncopy_cells > 0)
-    hipLaunchKernelGGL(k_fold_copy, dim3(unsigned((ncopy_cells + FB - 1) / FB)), dim3(FB), 0, st, copies, ncopy,
-                       ncopy_cells, g);
-  return hipGetLastError();
-}
-
-hipError_t launch_adj_combine(const BoxTable& bt, const AdjWork& aw, const StageArgs& sa, const Fab* lam_out,
-                              const Fab* lam_s1, const Fab* lam_sin, bool accumulate, hipStream_t st) {
-  const Region g0 = grown(0);
-  Launch L = grid_for(bt, g0);
-  if (!L.map) return hipErrorNotInitialized;
-  hipLaunchKernelGGL(k_adj_combine, L.grid, dim3(BLK), 0, st, bt.d_boxes, L.map, g0, sa.box_dt, sa, aw.g, lam_out,
-                     lam_s1, lam_sin, accumulate ? 1 : 0);
-  return hipGetLastError();
-}
-
-hipError_t launch_zero_valid(const BoxTable& bt, const Fab* f, hipStream_t st) {
-  const Region g0 = grown(0);
-  Launch L = grid_for(bt, g0);
-  if (!L.map) return hipErrorNotInitialized;
-  hipLaunchKernelGGL(k_zero_valid, L.grid, dim3(BLK), 0, st, bt.d_boxes, L.map, g0, nullptr, f);
+    if (ft.nbc_cells[d] > 0)
+      hipLaunchKernelGGL(k_fold_bc, dim3(unsigned((ft.nbc_cells[d] + FB - 1) / FB)), dim3(FB), 0, st, ft.bc[d],
+                         ft.nbc[d], ft.nbc_cells[d], aw.g, B, sa, lam_sin);
+  if (ft.ncopy_cells > 0)
+    hipLaunchKernelGGL(k_fold_copy, dim3(unsigned((ft.ncopy_cells + FB - 1) / FB)), dim3(FB), 0, st, ft.copies,
+                       ft.ncopy, ft.ncopy_cells, aw.g, sa, lam_sin);
   return hipGetLastError();
 }
 

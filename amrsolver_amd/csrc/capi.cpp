@@ -831,19 +831,40 @@ int ensure_adjoint(amr_boxtab_s* bt, const amr_scheme_t& s) {
 }
 
 // Reverse of one stage: lam_sin (=|+=) stage terms of lam_out through the
-// RHS VJP at `sin` (ghost-filled); lam_s1 += a lam_out.
+// RHS VJP at `sin` (ghost-filled); lam_s1 gets a lam_out (s1_mode 1: +=,
+// 2: =).  lam_sin and lam_s1 must not alias lam_out.
 int adj_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageArgs& sa,
-              const DescSet* sin, const DescSet* lam_out, const DescSet* lam_s1, const DescSet* lam_sin, bool acc) {
+              const DescSet* sin, const DescSet* lam_out, const DescSet* lam_s1, const DescSet* lam_sin, bool acc,
+              int s1_mode) {
   int rc;
   if ((rc = ensure_unfused(bt, sch))) return rc;
   if ((rc = ensure_adjoint(bt, sch))) return rc;
   const Closures c = make_closures(cls);
-  hipStream_t st = bt->ctx->stream;
-  HIPCHK(launch_adj_rhs(bt->bt, bt->uw, bt->aw, c, sch, sa.box_dt, sin->d, lam_out->d, st));
   const DescSet* g = bt->adj_g.get();
-  HIPCHK(launch_fold(g->d_copy, g->ncopy, g->ncopy_cells, g->d_bc, g->nbc, g->nbc_cells, g->d, st));
-  HIPCHK(launch_adj_combine(bt->bt, bt->aw, sa, lam_out->d, lam_s1 ? lam_s1->d : nullptr, lam_sin->d, acc, st));
+  const FoldTasks ft{g->d_copy, g->ncopy, g->ncopy_cells, g->d_bc, g->nbc, g->nbc_cells};
+  HIPCHK(launch_adj_stage(bt->bt, bt->uw, bt->aw, c, sch, sa, sin->d, lam_out->d,
+                          lam_s1 ? lam_s1->d : nullptr, lam_sin->d, acc, lam_s1 ? s1_mode : 0, ft,
+                          bt->ctx->stream));
   return AMR_OK;
+}
+
+// Whether the storage spans of two FAB sets overlap.
+bool fabs_overlap(const amr_fab_t* a, const amr_fab_t* b, int nb) {
+  auto span = [nb](const amr_fab_t* f, intptr_t& lo, intptr_t& hi) {
+    lo = INTPTR_MAX;
+    hi = 0;
+    for (int i = 0; i < nb; ++i) {
+      int64_t nx = f[i].hi[0] - f[i].lo[0] + 1 + 2 * f[i].ng, ny = f[i].hi[1] - f[i].lo[1] + 1 + 2 * f[i].ng;
+      int64_t nz = f[i].hi[2] - f[i].lo[2] + 1 + 2 * f[i].ng;
+      int64_t last = (nx - 1) + (ny - 1) * f[i].jstride + (nz - 1) * f[i].kstride + (NCONS - 1) * f[i].nstride;
+      lo = std::min(lo, intptr_t(f[i].data));
+      hi = std::max(hi, intptr_t(f[i].data + last + 1));
+    }
+  };
+  intptr_t alo, ahi, blo, bhi;
+  span(a, alo, ahi);
+  span(b, blo, bhi);
+  return alo < bhi && blo < ahi;
 }
 
 int check_adjoint(const amr_scheme_t& s) {
@@ -1357,8 +1378,11 @@ int amr_vjp_stage(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t
   if ((rc = get_desc(bt, lam_out, 0, &dlo))) return rc;
   if (lam_s1 && (rc = get_desc(bt, lam_s1, 0, &dls1))) return rc;
   if ((rc = get_desc(bt, lam_sin, 0, &dls))) return rc;
+  const int nb = bt->bt.nbox;
+  if (fabs_overlap(lam_sin, lam_out, nb) || (lam_s1 && fabs_overlap(lam_s1, lam_out, nb)))
+    return fail(AMR_ERR_ARG, "lam_sin and lam_s1 must not share storage with lam_out");
   StageArgs sa{StageCoef{mode, a, b, c, d}, nullptr, 0, 0, 0};
-  return adj_stage(bt, *cls, *sch, sa, ds, dlo, dls1, dls, accumulate != 0);
+  return adj_stage(bt, *cls, *sch, sa, ds, dlo, dls1, dls, accumulate != 0, 1);
 }
 
 int amr_vjp_rhs(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
@@ -1435,16 +1459,17 @@ int amr_adjoint_run(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme
       for (int n = n0; n < n0 + len; ++n)
         if ((rc = run_step(n, n == n0 + len - 1))) return rc;
     for (int nl = len - 1; nl >= 0; --nl) {
-      HIPCHK(launch_zero_valid(bt->bt, la->d, st));
+      // la accumulates the adjoint of U^n: assigned by the last stage,
+      // added to by the others (stage 0's input is U^n itself)
       for (int s = nst - 1; s >= 0; --s) {
         StageArgs sa;
         if ((rc = stage_args(s, &sa))) return rc;
         DescSet* sin = R->sb(seg, nl, s);
         if (s > 0) {
-          if ((rc = adj_stage(bt, *cls, *sch, sa, sin, lc, la, lt, false))) return rc;
+          if ((rc = adj_stage(bt, *cls, *sch, sa, sin, lc, la, lt, false, s == nst - 1 ? 2 : 1))) return rc;
           std::swap(lc, lt);
         } else {
-          if ((rc = adj_stage(bt, *cls, *sch, sa, sin, lc, la, la, true))) return rc;
+          if ((rc = adj_stage(bt, *cls, *sch, sa, sin, lc, la, la, nst > 1, 1))) return rc;
         }
       }
       std::swap(lc, la);

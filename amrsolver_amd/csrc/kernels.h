@@ -92,18 +92,22 @@ struct AdjWork {
   Fab* g = nullptr;
 };
 bool adjoint_supported(const amr_scheme_t& s);
-// g = (d dSdt / d sin)^T lam on grow(bx,3) of every box (before the fold);
-// sin ghost-filled, lam read on valid cells only.
-hipError_t launch_adj_rhs(const BoxTable& bt, const UnfusedWork& w, const AdjWork& aw, const Closures& c,
-                          const amr_scheme_t& s, const double* box_dt, const Fab* sin, const Fab* lam,
-                          hipStream_t st);
-// Transposed ghost fill of g: BC tasks z, y, x, then copy tasks (atomic adds).
-hipError_t launch_fold(const CopyTask* copies, int ncopy, int64_t ncopy_cells, BcTask* const bc[3],
-                       const int nbc[3], const int64_t nbc_cells[3], const Fab* g, hipStream_t st);
-// Stage combination transposed on valid cells (lam_s1 may be null).
-hipError_t launch_adj_combine(const BoxTable& bt, const AdjWork& aw, const StageArgs& sa, const Fab* lam_out,
-                              const Fab* lam_s1, const Fab* lam_sin, bool accumulate, hipStream_t st);
-hipError_t launch_zero_valid(const BoxTable& bt, const Fab* f, hipStream_t st);
+// Ghost-fill task lists of the adjoint workspace g.
+struct FoldTasks {
+  const CopyTask* copies;
+  int ncopy;
+  int64_t ncopy_cells;
+  BcTask* const* bc;
+  const int* nbc;
+  const int64_t* nbc_cells;
+};
+// Reverse of one stage (modes as amr_rk_stage) at the ghost-filled input
+// sin: lam_sin (=|+=) stage terms of lam_out through the RHS transpose, the
+// transposed ghost fill included; lam_s1 updated with a lam_out (s1_mode 0
+// none, 1 +=, 2 =).  lam_sin and lam_s1 must not alias lam_out.
+hipError_t launch_adj_stage(const BoxTable& bt, const UnfusedWork& w, const AdjWork& aw, const Closures& c, const amr_scheme_t& s, const StageArgs& sa,
+                            const Fab* sin, const Fab* lam_out, const Fab* lam_s1, const Fab* lam_sin,
+                            bool accumulate, int s1_mode, const FoldTasks& ft, hipStream_t st);
 // Objective of the final state and its gradient lam (kind 0 kinetic energy,
 // kind 1 linear with weights); result[0] = J.
 hipError_t launch_objective(const BoxTable& bt, int kind, const Fab* st, const Fab* wts, const Fab* lam,

@@ -201,8 +201,9 @@ int amr_vjp_rhs(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* 
  *            = J^T lam_out                      (mode 2)
  *   assigned, or added to lam_sin when `accumulate` is nonzero;
  *   lam_s1 += a lam_out (modes 0 and 1; lam_s1 may be NULL).
- * J = dL/dU at `sin` (ghost-filled, ng >= 3).  lam_sin may alias lam_out or
- * lam_s1; no adjoint array may alias `sin`. */
+ * J = dL/dU at `sin` (ghost-filled, ng >= 3).  lam_sin may alias lam_s1 (the
+ * lam_sin update comes first); neither may share storage with lam_out
+ * (AMR_ERR_ARG), and no adjoint array may alias `sin`. */
 int amr_vjp_stage(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, int mode, double a, double b,
                   double c, double d, const amr_fab_t* sin, const amr_fab_t* lam_out, const amr_fab_t* lam_s1,
                   const amr_fab_t* lam_sin, int accumulate);
