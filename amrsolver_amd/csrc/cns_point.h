@@ -114,32 +114,48 @@ __device__ __forceinline__ double fghCubic(double f, double fl, double g, double
   return 0.1250 * (f + fl) * (g + gl) * (h + hl);
 }
 
-// One (l,m) pair of the KEEP flux in direction `nrm` (0,1,2), accumulated
-// into f[5] with coefficient co (CentralKEEP.h:58-84 and the y/z copies).
+// The five flux terms of one (l,m) pair of the KEEP flux in direction `nrm`
+// (0,1,2) before the coefficient (CentralKEEP.h:58-84 and the y/z copies).
 // Cell 1 = (rho1,u1,v1,w1,ie1,p1), cell 2 likewise.
+template <int NRM>
+__device__ __forceinline__ void keep_terms(double rho1, double ux1, double uy1, double uz1, double ie1,
+                                           double p1, double rho2, double ux2, double uy2, double uz2,
+                                           double ie2, double p2, double t[5]) {
+  const double un1 = NRM == 0 ? ux1 : (NRM == 1 ? uy1 : uz1);
+  const double un2 = NRM == 0 ? ux2 : (NRM == 1 ? uy2 : uz2);
+  double massflx = fgQuad(rho1, rho2, un1, un2);
+  double ke = 0.5 * (ux1 * ux2 + uy1 * uy2 + uz1 * uz2);
+  t[URHO] = massflx;
+  if (NRM == 0)
+    t[UMX] = fghCubic(rho1, rho2, un1, un2, un1, un2) + fDiv(p1, p2);
+  else
+    t[UMX] = fghCubic(rho1, rho2, un1, un2, ux1, ux2);
+  if (NRM == 1)
+    t[UMY] = fghCubic(rho1, rho2, un1, un2, un1, un2) + fDiv(p1, p2);
+  else
+    t[UMY] = fghCubic(rho1, rho2, un1, un2, uy1, uy2);
+  if (NRM == 2)
+    t[UMZ] = fghCubic(rho1, rho2, un1, un2, un1, un2) + fDiv(p1, p2);
+  else
+    t[UMZ] = fghCubic(rho1, rho2, un1, un2, uz1, uz2);
+  t[UET] = massflx * ke + fghCubic(rho1, rho2, un1, un2, ie1, ie2) + fgDiv(p1, p2, un1, un2);
+}
+
+// One pair accumulated into f[5] with coefficient co: f += co * terms, in
+// the reference's component order.
+__device__ __forceinline__ void keep_accum(double co, const double t[5], double f[5]) {
+#pragma unroll
+  for (int n = 0; n < 5; ++n) f[n] += co * t[n];
+}
+
 template <int NRM>
 __device__ __forceinline__ void keep_pair(double co, double rho1, double ux1, double uy1,
                                           double uz1, double ie1, double p1, double rho2,
                                           double ux2, double uy2, double uz2, double ie2,
                                           double p2, double f[5]) {
-  const double un1 = NRM == 0 ? ux1 : (NRM == 1 ? uy1 : uz1);
-  const double un2 = NRM == 0 ? ux2 : (NRM == 1 ? uy2 : uz2);
-  double massflx = fgQuad(rho1, rho2, un1, un2);
-  double ke = 0.5 * (ux1 * ux2 + uy1 * uy2 + uz1 * uz2);
-  f[URHO] += co * massflx;
-  if (NRM == 0)
-    f[UMX] += co * (fghCubic(rho1, rho2, un1, un2, un1, un2) + fDiv(p1, p2));
-  else
-    f[UMX] += co * fghCubic(rho1, rho2, un1, un2, ux1, ux2);
-  if (NRM == 1)
-    f[UMY] += co * (fghCubic(rho1, rho2, un1, un2, un1, un2) + fDiv(p1, p2));
-  else
-    f[UMY] += co * fghCubic(rho1, rho2, un1, un2, uy1, uy2);
-  if (NRM == 2)
-    f[UMZ] += co * (fghCubic(rho1, rho2, un1, un2, un1, un2) + fDiv(p1, p2));
-  else
-    f[UMZ] += co * fghCubic(rho1, rho2, un1, un2, uz1, uz2);
-  f[UET] += co * (massflx * ke + fghCubic(rho1, rho2, un1, un2, ie1, ie2) + fgDiv(p1, p2, un1, un2));
+  double t[5];
+  keep_terms<NRM>(rho1, ux1, uy1, uz1, ie1, p1, rho2, ux2, uy2, uz2, ie2, p2, t);
+  keep_accum(co, t, f);
 }
 
 // src/rhs/High_resolution.h:124-159 (beta1 with the 4th power of (s1-s3),

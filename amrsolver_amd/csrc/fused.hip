@@ -72,7 +72,7 @@ template <int H, bool VISC, int TX, int TY>
 struct Cfg {
   static constexpr int HX = (VISC && H < 2) ? 2 : H;  // xy halo
   static constexpr int KB = (VISC && H < 2) ? 2 : H;  // planes below k0 loaded
-  static constexpr int QN = (H > 1 ? H : 1) + 2;      // own-column queue: c-QLO .. c+1
+  // own-column queue c-1, c, c+1 (KEEP4 keeps its c-2 pair in a carry)
   static constexpr int NT = TX * TY;
   static constexpr int RX = TX + 2 * HX, RY = TY + 2 * HX;
   static constexpr int NREG = RX * RY;
@@ -104,13 +104,19 @@ __device__ __forceinline__ void keep_face(const double* sq, int idx, int st, con
   }
 }
 
+// Terms of the z-direction KEEP pair (cell 1 = q1, cell 2 = q2).
+__device__ __forceinline__ void zterms(const double q1[NPRIM], const double q2[NPRIM], double cv, double t[5]) {
+  keep_terms<2>(q1[QRHO], q1[QU], q1[QV], q1[QW], cv * q1[QT], q1[QPRES], q2[QRHO], q2[QU], q2[QV], q2[QW],
+                cv * q2[QT], q2[QPRES], t);
+}
+
 template <int H, bool VISC, int TX, int TY>
 __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ tiles, const BoxDev* __restrict__ boxes,
                                                      Closures c, double co0, double co1, double co2, StageArgs sa,
                                                      const Fab* __restrict__ s1f, const Fab* __restrict__ sinf,
                                                      const Fab* __restrict__ outf) {
   using C = Cfg<H, VISC, TX, TY>;
-  constexpr int RX = C::RX, HX = C::HX, PL = C::PL, QN = C::QN, NT = C::NT;
+  constexpr int RX = C::RX, HX = C::HX, PL = C::PL, NT = C::NT;
   __shared__ double s_q[2][NPRIM * PL];
   __shared__ double s_px[VISC ? 4 : 1][TY][TX + 2];
   __shared__ double s_py[VISC ? 4 : 1][TY + 2][TX];
@@ -191,39 +197,26 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
     }
   };
 
-  // own-column primitive queue: q[QN-1] is the newest plane
-  double qq[QN][NPRIM];
-  // ring queue (u,v,w,T at c-1, c, c+1)
-  double rq[3][4];
+  // own column: plane c is snapshotted from LDS before the first barrier of
+  // its iteration, plane c+1 is read from its LDS slot (overwritten only two
+  // iterations later), plane c-1 is the previous snapshot; likewise u,v,w,T
+  // of the ring cell
+  double qm1[NPRIM], qs[NPRIM];
+  double rm1[4], rs[4];
   double pfz_prev[4] = {0, 0, 0, 0}, pfz_cur[4] = {0, 0, 0, 0};
   double fz_prev[5] = {0, 0, 0, 0, 0};
+  double zcarry[5] = {0, 0, 0, 0, 0};  // z pair (c+1, c-1) terms for the next plane
   double xy_prev[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int n = 0; n < NPRIM; ++n) qm1[n] = 0.0;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) rm1[n] = 0.0;
 
-  auto push = [&](int p) {
-    const double* sq = s_q[p & 1];
-#pragma unroll
-    for (int t = 0; t < QN - 1; ++t)
-#pragma unroll
-      for (int n = 0; n < NPRIM; ++n) qq[t][n] = qq[t + 1][n];
-#pragma unroll
-    for (int n = 0; n < NPRIM; ++n) qq[QN - 1][n] = sq[n * PL + own];
-    if (VISC) {
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int n = 0; n < 4; ++n) rq[t][n] = rq[t + 1][n];
-      if (ring_kind >= 0)
-#pragma unroll
-        for (int n = 0; n < 4; ++n) rq[2][n] = sq[(QU + n) * PL + ring_idx];
-    }
-  };
-
-  // prologue: plane kfirst into LDS and the queues
+  // prologue: plane kfirst into LDS
   load_plane(kfirst);
   store_plane(kfirst);
   load_plane(kfirst + 1);
   __syncthreads();
-  push(kfirst);
 
   for (int cpl = kfirst; cpl <= k0 + nz; ++cpl) {
     const int pn = cpl + 1;
@@ -248,22 +241,43 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
         for (int n = 0; n < 5; ++n) s1v[n] = 0.0;
       }
     }
+    const double* sq = s_q[cpl & 1];        // plane c
+    const double* sqn = s_q[(cpl + 1) & 1];  // plane c+1 (stored above)
+#pragma unroll
+    for (int n = 0; n < NPRIM; ++n) qs[n] = sq[n * PL + own];
+    if (VISC && ring_kind >= 0)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) rs[n] = sq[(QU + n) * PL + ring_idx];
     __syncthreads();  // (b)
-    if (pn <= klast) push(pn);
-    if (cpl < k0 - 1) continue;  // uniform: only loading planes below the tile
-    if (!VISC && cpl < k0) continue;
+    if (cpl < k0 - 1 || (!VISC && H == 1 && cpl < k0)) {  // uniform: only loading planes below the tile
+#pragma unroll
+      for (int n = 0; n < NPRIM; ++n) qm1[n] = qs[n];
+#pragma unroll
+      for (int n = 0; n < 4; ++n) rm1[n] = rs[n];
+      continue;
+    }
+    // z pair (k0, k0-2) for the first face
+    if (H == 2 && cpl == k0 - 1) {
+      double qn[NPRIM];
+#pragma unroll
+      for (int n = 0; n < NPRIM; ++n) qn[n] = sqn[n * PL + own];
+      zterms(qn, qm1, cv, zcarry);
+      if (!VISC) {
+#pragma unroll
+        for (int n = 0; n < NPRIM; ++n) qm1[n] = qs[n];
+        continue;
+      }
+    }
 
     const bool inner = cpl >= k0 && cpl < k0 + nz;  // x/y work on this plane
-    const double* sq = s_q[cpl & 1];
-    // queue slots: plane cpl-1 -> QN-3, cpl -> QN-2, cpl+1 -> QN-1
     if (VISC) {
       // own cell
       {
         VDer d;
-        d.ux = qq[QN - 2][QU];
-        d.uy = qq[QN - 2][QV];
-        d.uz = qq[QN - 2][QW];
-        d.T = qq[QN - 2][QT];
+        d.ux = qs[QU];
+        d.uy = qs[QV];
+        d.uz = qs[QW];
+        d.T = qs[QT];
         const int e = own + 1, w = own - 1, nn = own + RX, s = own - RX;
         d.dudx = (sq[QU * PL + e] - sq[QU * PL + w]) * hx;
         d.dvdx = (sq[QV * PL + e] - sq[QV * PL + w]) * hx;
@@ -273,10 +287,10 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
         d.dvdy = (sq[QV * PL + nn] - sq[QV * PL + s]) * hy;
         d.dwdy = (sq[QW * PL + nn] - sq[QW * PL + s]) * hy;
         d.dTdy = (sq[QT * PL + nn] - sq[QT * PL + s]) * hy;
-        d.dudz = (qq[QN - 1][QU] - qq[QN - 3][QU]) * hz;
-        d.dvdz = (qq[QN - 1][QV] - qq[QN - 3][QV]) * hz;
-        d.dwdz = (qq[QN - 1][QW] - qq[QN - 3][QW]) * hz;
-        d.dTdz = (qq[QN - 1][QT] - qq[QN - 3][QT]) * hz;
+        d.dudz = (sqn[QU * PL + own] - qm1[QU]) * hz;
+        d.dvdz = (sqn[QV * PL + own] - qm1[QV]) * hz;
+        d.dwdz = (sqn[QW * PL + own] - qm1[QW]) * hz;
+        d.dTdz = (sqn[QT * PL + own] - qm1[QT]) * hz;
         double fx[4], fy[4];
         if (inner) {
           visc_flux<true, true, true>(d, c, fx, fy, pfz_cur);
@@ -293,10 +307,10 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
       if (inner && ring_kind >= 0) {
         VDer d;
         const int r = ring_idx;
-        d.ux = sq[QU * PL + r];
-        d.uy = sq[QV * PL + r];
-        d.uz = sq[QW * PL + r];
-        d.T = sq[QT * PL + r];
+        d.ux = rs[0];
+        d.uy = rs[1];
+        d.uz = rs[2];
+        d.T = rs[3];
         const int e = r + 1, w = r - 1, nn = r + RX, s = r - RX;
         d.dudx = (sq[QU * PL + e] - sq[QU * PL + w]) * hx;
         d.dvdx = (sq[QV * PL + e] - sq[QV * PL + w]) * hx;
@@ -306,10 +320,10 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
         d.dvdy = (sq[QV * PL + nn] - sq[QV * PL + s]) * hy;
         d.dwdy = (sq[QW * PL + nn] - sq[QW * PL + s]) * hy;
         d.dTdy = (sq[QT * PL + nn] - sq[QT * PL + s]) * hy;
-        d.dudz = (rq[2][0] - rq[0][0]) * hz;
-        d.dvdz = (rq[2][1] - rq[0][1]) * hz;
-        d.dwdz = (rq[2][2] - rq[0][2]) * hz;
-        d.dTdz = (rq[2][3] - rq[0][3]) * hz;
+        d.dudz = (sqn[QU * PL + r] - rm1[0]) * hz;
+        d.dvdz = (sqn[QV * PL + r] - rm1[1]) * hz;
+        d.dwdz = (sqn[QW * PL + r] - rm1[2]) * hz;
+        d.dTdz = (sqn[QT * PL + r] - rm1[3]) * hz;
         double fx[4], fy[4], fz[4];
         if (ring_kind == 0) {
           visc_flux<true, false, false>(d, c, fx, fy, fz);
@@ -377,19 +391,21 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
         xy[n] = dxi0 * (s_fx[n][tj][ti] - s_fx[n][tj][ti + 1]) + dxi1 * (s_fy[n][tj][ti] - s_fy[n][tj + 1][ti]);
     }
     if (cpl >= k0) {
-      // z face at cpl - 1/2 from the own-column queue: plane cpl+o sits in
-      // slot QN-2+o
+      // z face at cpl - 1/2 from the own column (planes c-1, c, c+1)
       double fz[5];
 #pragma unroll
       for (int n = 0; n < 5; ++n) fz[n] = 0.0;
+      double t[5];
+      zterms(qs, qm1, cv, t);  // (c, c-1)
+      keep_accum(co[0], t, fz);
+      if (H == 2) {
+        // (c, c-2) is the previous plane's (c+1, c-1), carried in registers
+        keep_accum(co[1], zcarry, fz);
+        double qn[NPRIM];
 #pragma unroll
-      for (int l = 1; l <= H; ++l) {
-#pragma unroll
-        for (int m = 0; m <= l - 1; ++m) {
-          const int a = QN - 2 + m, b = QN - 2 + m - l;
-          keep_pair<2>(co[l - 1], qq[a][QRHO], qq[a][QU], qq[a][QV], qq[a][QW], cv * qq[a][QT], qq[a][QPRES],
-                       qq[b][QRHO], qq[b][QU], qq[b][QV], qq[b][QW], cv * qq[b][QT], qq[b][QPRES], fz);
-        }
+        for (int n = 0; n < NPRIM; ++n) qn[n] = sqn[n * PL + own];
+        zterms(qn, qm1, cv, zcarry);  // (c+1, c-1)
+        keep_accum(co[1], zcarry, fz);
       }
       if (VISC) {
         fz[0] += 0.5 * (0.0 + 0.0);
@@ -415,6 +431,10 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
 #pragma unroll
       for (int n = 0; n < 4; ++n) pfz_prev[n] = pfz_cur[n];
     }
+#pragma unroll
+    for (int n = 0; n < NPRIM; ++n) qm1[n] = qs[n];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) rm1[n] = rs[n];
   }
 }
 
