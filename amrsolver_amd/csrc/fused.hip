@@ -83,6 +83,12 @@ struct Cfg {
   // their own (the ring fluxes occupy wave 0) where the workgroup has them
   static constexpr int XB = NT >= 128 ? 64 * ((NRING + 63) / 64) : (VISC ? NRING : 0);
   static constexpr int YB = NT >= XB + 128 ? XB + 64 : XB + TY;
+  // first ring thread: the last wave where it holds the whole ring and the
+  // extra faces sit on waves 1 and 2, else thread 0
+  static constexpr int RB = (NT >= 256 && NRING <= 64 && XB == 64) ? NT - 64 : 0;
+  // the KEEP part of the extra faces is computed before the viscous barrier
+  // (balancing the phases) where the extra-face and ring waves differ
+  static constexpr bool EARLY = VISC && RB > 0;
 };
 
 // KEEP face flux in direction NRM at the face below LDS cell `idx`; cells
@@ -151,8 +157,8 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
 
   // ring cell of this thread (viscous x-flux at i=-1,TX or y-flux at j=-1,TY)
   int ring_idx = 0, ring_kind = -1, ring_a = 0, ring_b = 0;
-  if (VISC && tid < C::NRING) {
-    int r = tid;
+  if (VISC && tid >= C::RB && tid < C::RB + C::NRING) {
+    int r = tid - C::RB;
     if (r < TY) { ring_kind = 0; ring_a = -1; ring_b = r; }
     else if (r < 2 * TY) { ring_kind = 0; ring_a = TX; ring_b = r - TY; }
     else if (r < 2 * TY + TX) { ring_kind = 1; ring_a = r - 2 * TY; ring_b = -1; }
@@ -335,6 +341,22 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
           for (int n = 0; n < 4; ++n) s_py[n][ring_b + 1][ring_a] = fy[n];
         }
       }
+      // KEEP part of the faces at i = TX (wave 1) and j = TY (wave 2); their
+      // viscous part is added after the barrier
+      if (C::EARLY && inner) {
+        double f[5];
+        if (tid >= C::XB && tid < C::XB + TY) {
+          const int jj = tid - C::XB;
+          keep_face<H, 0, PL>(sq, (jj + HX) * RX + (TX + HX), 1, co, cv, f);
+#pragma unroll
+          for (int n = 0; n < 5; ++n) s_fx[n][jj][TX] = f[n];
+        } else if (tid >= C::YB && tid < C::YB + TX) {
+          const int ii = tid - C::YB;
+          keep_face<H, 1, PL>(sq, (TY + HX) * RX + (ii + HX), RX, co, cv, f);
+#pragma unroll
+          for (int n = 0; n < 5; ++n) s_fy[n][TY][ii] = f[n];
+        }
+      }
       // (c) viscous x/y fluxes visible; below the tile this barrier instead
       // keeps the next store from overwriting the plane just read
       __syncthreads();
@@ -359,10 +381,15 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
       }
 #pragma unroll
       for (int n = 0; n < 5; ++n) s_fy[n][tj][ti] = f[n];
-      // the faces at i = TX (wave 0) and j = TY (wave 1)
+      // the faces at i = TX and j = TY
       if (tid >= C::XB && tid < C::XB + TY) {
         const int jj = tid - C::XB;
-        keep_face<H, 0, PL>(sq, (jj + HX) * RX + (TX + HX), 1, co, cv, f);
+        if (C::EARLY) {
+#pragma unroll
+          for (int n = 0; n < 5; ++n) f[n] = s_fx[n][jj][TX];
+        } else {
+          keep_face<H, 0, PL>(sq, (jj + HX) * RX + (TX + HX), 1, co, cv, f);
+        }
         if (VISC) {
           f[0] += 0.5 * (0.0 + 0.0);
 #pragma unroll
@@ -372,7 +399,12 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
         for (int n = 0; n < 5; ++n) s_fx[n][jj][TX] = f[n];
       } else if (tid >= C::YB && tid < C::YB + TX) {
         const int ii = tid - C::YB;
-        keep_face<H, 1, PL>(sq, (TY + HX) * RX + (ii + HX), RX, co, cv, f);
+        if (C::EARLY) {
+#pragma unroll
+          for (int n = 0; n < 5; ++n) f[n] = s_fy[n][TY][ii];
+        } else {
+          keep_face<H, 1, PL>(sq, (TY + HX) * RX + (ii + HX), RX, co, cv, f);
+        }
         if (VISC) {
           f[0] += 0.5 * (0.0 + 0.0);
 #pragma unroll
