@@ -441,11 +441,11 @@ bool adjoint_supported(const amr_scheme_t& s) {
   return s.rhs_euler || s.rhs_visc;
 }
 
-hipError_t launch_adj_stage(const BoxTable& bt, const UnfusedWork& w, const AdjWork& aw, const Closures& c, const amr_scheme_t& s, const StageArgs& sa,
-                            const Fab* sin, const Fab* lam_out, const Fab* lam_s1, const Fab* lam_sin,
+hipError_t launch_adj_stage(const BoxTable& bt, const UnfusedWork& w, const AdjWork& aw, const Closures& c,
+                            const amr_scheme_t& s, const StageArgs& sa, const WrapArgs& wa, const Fab* sin, const Fab* lam_out, const Fab* lam_s1, const Fab* lam_sin,
                             bool accumulate, int s1_mode, const FoldTasks& ft, hipStream_t st) {
   const BoxDev* B = bt.d_boxes;
-  hipError_t e = launch_prims(bt, w, c, sa.box_dt, sin, st);
+  hipError_t e = launch_prims(bt, w, c, sa.box_dt, sin, wa, st);
   if (e != hipSuccess) return e;
   if (s.rhs_visc) {
     const Region g1 = grown(1);

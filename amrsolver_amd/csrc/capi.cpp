@@ -95,6 +95,7 @@ struct DescSet {
   int nbc[3] = {0, 0, 0};
   int64_t nbc_cells[3] = {0, 0, 0};
   int64_t last_use = 0;
+  int wrap = -1;  // one array over the whole periodic domain (-1 unknown)
   ~DescSet() {
     if (d) (void)hipFree(d);
     if (d_copy) (void)hipFree(d_copy);
@@ -663,17 +664,61 @@ FusedConfig choose_config(const amr_boxtab_s* bt, const amr_scheme_t& sch) {
   return default_config(bt);
 }
 
+// Whether the fused stage kernel handles this scheme on this table.
+bool fused_path(const amr_boxtab_s* bt, const amr_scheme_t& sch, FusedConfig* out) {
+  FusedConfig cfg = choose_config(bt, sch);
+  if (out) *out = cfg;
+  return cfg.tx > 0 && fused_supported(sch, cfg.tx, cfg.ty) && fused_fits(bt, cfg);
+}
+
+// Whether `ds` is one array over the whole periodic domain (every box a
+// window of it, the boxes tiling the domain): the fused kernel then reads
+// periodic images of halo cells itself and needs no ghost fill.
+bool wrap_ok(const amr_boxtab_s* bt, DescSet* ds) {
+  if (ds->wrap >= 0) return ds->wrap == 1;
+  ds->wrap = 0;
+  if (bt->batch) return false;
+  const amr_geom_t& g = bt->geom;
+  int64_t dom = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (!g.periodic[d]) return false;
+    dom *= g.dom_hi[d] - g.dom_lo[d] + 1;
+  }
+  const Mapping m0 = mapping_of(ds->host[0]);
+  int64_t sum = 0;
+  for (int b = 0; b < bt->bt.nbox; ++b) {
+    if (!(mapping_of(ds->host[b]) == m0)) return false;
+    sum += cells(bt->bt.h_boxes[b].lo, bt->bt.h_boxes[b].hi);
+  }
+  if (sum != dom) return false;
+  ds->wrap = 1;
+  return true;
+}
+
+WrapArgs wrap_args(const amr_boxtab_s* bt, DescSet* ds) {
+  WrapArgs wa{0, {0, 0, 0}, {1, 1, 1}};
+  if (wrap_ok(bt, ds)) {
+    wa.on = 1;
+    for (int d = 0; d < 3; ++d) {
+      wa.lo[d] = bt->geom.dom_lo[d];
+      wa.len[d] = bt->geom.dom_hi[d] - bt->geom.dom_lo[d] + 1;
+    }
+  }
+  return wa;
+}
+
 int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageArgs& sa,
              DescSet* ds1, DescSet* dsin, DescSet* dout) {
   Closures c = make_closures(cls);
   hipStream_t st = bt->ctx->stream;
   const Fab* s1 = ds1 ? ds1->d : nullptr;
-  FusedConfig cfg = choose_config(bt, sch);
-  if (cfg.tx > 0 && fused_supported(sch, cfg.tx, cfg.ty) && fused_fits(bt, cfg)) {
+  FusedConfig cfg;
+  if (fused_path(bt, sch, &cfg)) {
     TileTable* tt = nullptr;
     int rc = get_tiles(bt, cfg, &tt);
     if (rc) return rc;
-    HIPCHK(launch_fused_stage(sch, cfg, tt->d, tt->n, bt->bt.d_boxes, c, sa, s1, dsin->d, dout->d, st));
+    const WrapArgs wa = wrap_args(bt, dsin);
+    HIPCHK(launch_fused_stage(sch, cfg, wa, tt->d, tt->n, bt->bt.d_boxes, c, sa, s1, dsin->d, dout->d, st));
     return AMR_OK;
   }
   int rc = ensure_unfused(bt, sch);
@@ -771,7 +816,9 @@ int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t
     if ((rc = stage_coeffs(sch.order_rk, sch.stages_rk, dt, s, &nst, &sc))) return rc;
     DescSet* sin = s == 0 ? dst : dw[(s - 1) % 2];
     DescSet* out = (s == nst - 1 && s > 0) ? dst : dw[s % 2];
-    if ((rc = enqueue_fill(bt, sin))) return rc;
+    // a periodic domain-wide array needs no fill on the fused path
+    if (!(fused_path(bt, sch, nullptr) && wrap_ok(bt, sin)))
+      if ((rc = enqueue_fill(bt, sin))) return rc;
     StageArgs sa{sc, d_box_dt, sch.order_rk, sch.stages_rk, s};
     if ((rc = do_stage(bt, cls, sch, sa, dst, sin, out))) return rc;
   }
@@ -834,7 +881,7 @@ int ensure_adjoint(amr_boxtab_s* bt, const amr_scheme_t& s) {
 // RHS VJP at `sin` (ghost-filled); lam_s1 gets a lam_out (s1_mode 1: +=,
 // 2: =).  lam_sin and lam_s1 must not alias lam_out.
 int adj_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageArgs& sa,
-              const DescSet* sin, const DescSet* lam_out, const DescSet* lam_s1, const DescSet* lam_sin, bool acc,
+              DescSet* sin, const DescSet* lam_out, const DescSet* lam_s1, const DescSet* lam_sin, bool acc,
               int s1_mode) {
   int rc;
   if ((rc = ensure_unfused(bt, sch))) return rc;
@@ -842,7 +889,7 @@ int adj_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& s
   const Closures c = make_closures(cls);
   const DescSet* g = bt->adj_g.get();
   const FoldTasks ft{g->d_copy, g->ncopy, g->ncopy_cells, g->d_bc, g->nbc, g->nbc_cells};
-  HIPCHK(launch_adj_stage(bt->bt, bt->uw, bt->aw, c, sch, sa, sin->d, lam_out->d,
+  HIPCHK(launch_adj_stage(bt->bt, bt->uw, bt->aw, c, sch, sa, wrap_args(bt, sin), sin->d, lam_out->d,
                           lam_s1 ? lam_s1->d : nullptr, lam_sin->d, acc, lam_s1 ? s1_mode : 0, ft,
                           bt->ctx->stream));
   return AMR_OK;
@@ -1423,11 +1470,16 @@ int amr_adjoint_run(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme
     const int seg = n / k, nl = n % k;
     for (int s = 0; s < nst; ++s) {
       DescSet* sin = R->sb(seg, nl, s);
+      // stage inputs need their ghost cells (for the forward and the reverse
+      // sweep) unless the level is one periodic array read through images
+      const bool need_fill = !wrap_ok(bt, sin);
       if (skip_last && s == nst - 1) {
         // the stage itself is not needed, but the reverse sweep reads its
         // input with ghost cells
-        int r = enqueue_fill(bt, sin);
-        if (r) return r;
+        if (need_fill) {
+          int r = enqueue_fill(bt, sin);
+          if (r) return r;
+        }
         break;
       }
       StageArgs sa;
@@ -1439,7 +1491,7 @@ int amr_adjoint_run(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme
       else if (n + 1 == nsteps) out = R->uf();
       else if ((n + 1) % k == 0) out = R->ck(seg + 1);
       else out = R->sb(seg, nl + 1, 0);
-      if ((r = enqueue_fill(bt, sin))) return r;
+      if (need_fill && (r = enqueue_fill(bt, sin))) return r;
       if ((r = do_stage(bt, *cls, *sch, sa, s1, sin, out))) return r;
     }
     return AMR_OK;

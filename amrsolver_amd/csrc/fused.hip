@@ -119,7 +119,7 @@ __device__ __forceinline__ void zterms(const double q1[NPRIM], const double q2[N
 template <int H, bool VISC, int TX, int TY>
 __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ tiles, const BoxDev* __restrict__ boxes,
                                                      Closures c, double co0, double co1, double co2, StageArgs sa,
-                                                     const Fab* __restrict__ s1f, const Fab* __restrict__ sinf,
+                                                     WrapArgs wa, const Fab* __restrict__ s1f, const Fab* __restrict__ sinf,
                                                      const Fab* __restrict__ outf) {
   using C = Cfg<H, VISC, TX, TY>;
   constexpr int RX = C::RX, HX = C::HX, PL = C::PL, NT = C::NT;
@@ -174,11 +174,20 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
   for (int s = 0; s < C::NLOAD; ++s) {
     int cell = tid + s * NT;
     int ii = cell % RX, jj = cell / RX;
-    loff[s] = min(T.i0 - HX + ii, xcap) + min(T.j0 - HX + jj, ycap) * int(S.js);
+    int x = T.i0 - HX + ii, y = T.j0 - HX + jj;
+    if (wa.on) {
+      x = x < wa.lo[0] ? x + wa.len[0] : (x >= wa.lo[0] + wa.len[0] ? x - wa.len[0] : x);
+      y = y < wa.lo[1] ? y + wa.len[1] : (y >= wa.lo[1] + wa.len[1] ? y - wa.len[1] : y);
+    } else {
+      x = min(x, xcap);
+      y = min(y, ycap);
+    }
+    loff[s] = x + y * int(S.js);
   }
   const int soff = gi + gj * int(S.js);
   double pre[C::NLOAD][5];
   auto load_plane = [&](int p) {
+    if (wa.on) p = p < wa.lo[2] ? p + wa.len[2] : (p >= wa.lo[2] + wa.len[2] ? p - wa.len[2] : p);
     const gdouble* pb = sbase + int64_t(p) * S.ks;
 #pragma unroll
     for (int s = 0; s < C::NLOAD; ++s) {
@@ -472,22 +481,22 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
 
 template <int H, bool VISC, int TX, int TY>
 void launch_t(const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c, const double co[3],
-              const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
+              const StageArgs& sa, const WrapArgs& wa, const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
   hipLaunchKernelGGL((k_fused<H, VISC, TX, TY>), dim3(ntiles), dim3(TX * TY), 0, st, tiles, boxes, c, co[0], co[1],
-                     co[2], sa, s1, sin, out);
+                     co[2], sa, wa, s1, sin, out);
 }
 
 template <int TX, int TY>
 bool launch_shape(const amr_scheme_t& s, const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c,
-                  const double co[3], const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* out,
-                  hipStream_t st) {
+                  const double co[3], const StageArgs& sa, const WrapArgs& wa, const Fab* s1, const Fab* sin,
+                  const Fab* out, hipStream_t st) {
   const bool v = s.rhs_visc != 0;
   if (s.order_keep == 4) {
-    if (v) launch_t<2, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, s1, sin, out, st);
-    else launch_t<2, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, s1, sin, out, st);
+    if (v) launch_t<2, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, s1, sin, out, st);
+    else launch_t<2, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, s1, sin, out, st);
   } else if (s.order_keep == 2) {
-    if (v) launch_t<1, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, s1, sin, out, st);
-    else launch_t<1, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, s1, sin, out, st);
+    if (v) launch_t<1, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, s1, sin, out, st);
+    else launch_t<1, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, s1, sin, out, st);
   } else {
     return false;
   }
@@ -503,15 +512,16 @@ bool fused_supported(const amr_scheme_t& s, int tx, int ty) {
   return (tx == 16 && ty == 16) || (tx == 8 && ty == 8) || (tx == 32 && ty == 8);
 }
 
-hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const TileDev* tiles, int ntiles,
+hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const WrapArgs& wa, const TileDev* tiles,
+                              int ntiles,
                               const BoxDev* boxes, const Closures& c, const StageArgs& sa, const Fab* s1,
                               const Fab* sin, const Fab* out, hipStream_t st) {
   double co[3];
   keep_coefficients(s.order_keep, co);
   bool ok = false;
-  if (cfg.tx == 16 && cfg.ty == 16) ok = launch_shape<16, 16>(s, tiles, ntiles, boxes, c, co, sa, s1, sin, out, st);
-  else if (cfg.tx == 8 && cfg.ty == 8) ok = launch_shape<8, 8>(s, tiles, ntiles, boxes, c, co, sa, s1, sin, out, st);
-  else if (cfg.tx == 32 && cfg.ty == 8) ok = launch_shape<32, 8>(s, tiles, ntiles, boxes, c, co, sa, s1, sin, out, st);
+  if (cfg.tx == 16 && cfg.ty == 16) ok = launch_shape<16, 16>(s, tiles, ntiles, boxes, c, co, sa, wa, s1, sin, out, st);
+  else if (cfg.tx == 8 && cfg.ty == 8) ok = launch_shape<8, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, s1, sin, out, st);
+  else if (cfg.tx == 32 && cfg.ty == 8) ok = launch_shape<32, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, s1, sin, out, st);
   if (!ok) return hipErrorInvalidValue;
   return hipGetLastError();
 }

@@ -46,6 +46,14 @@ struct UnfusedWork {
   Fab* slope = nullptr;     // NCONS comps on grow(bx,1)   (MUSCL slopes)
 };
 
+// Periodic halo wrap for a state that is one array over the whole periodic
+// domain: indices outside [lo, lo+len) read their periodic image, so the
+// kernel needs no ghost fill.
+struct WrapArgs {
+  int on;
+  int lo[3], len[3];
+};
+
 // Ghost-fill tasks.  Copy: dst(i,j,k) = src(i - sh) for (i,j,k) in [lo,hi].
 struct CopyTask {
   int dst, src;
@@ -81,9 +89,10 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
                                 const amr_scheme_t& s, const StageArgs& sa, const Fab* s1, const Fab* sin,
                                 const Fab* out, hipStream_t st);
 
-// cons2prim of `sin` into w.prims on grow(bx,3) for every box.
+// cons2prim of `sin` into w.prims on grow(bx,3) for every box (halo cells
+// from their periodic images when wa.on).
 hipError_t launch_prims(const BoxTable& bt, const UnfusedWork& w, const Closures& c, const double* box_dt,
-                        const Fab* sin, hipStream_t st);
+                        const Fab* sin, const WrapArgs& wa, hipStream_t st);
 
 // Reverse mode of the KEEP + viscous RHS (adjoint.hip).  Workspace per box:
 // e = 16 comps on grow(bx,1) (viscous), g = NCONS comps on grow(bx,3).
@@ -105,8 +114,8 @@ struct FoldTasks {
 // sin: lam_sin (=|+=) stage terms of lam_out through the RHS transpose, the
 // transposed ghost fill included; lam_s1 updated with a lam_out (s1_mode 0
 // none, 1 +=, 2 =).  lam_sin and lam_s1 must not alias lam_out.
-hipError_t launch_adj_stage(const BoxTable& bt, const UnfusedWork& w, const AdjWork& aw, const Closures& c, const amr_scheme_t& s, const StageArgs& sa,
-                            const Fab* sin, const Fab* lam_out, const Fab* lam_s1, const Fab* lam_sin,
+hipError_t launch_adj_stage(const BoxTable& bt, const UnfusedWork& w, const AdjWork& aw, const Closures& c,
+                            const amr_scheme_t& s, const StageArgs& sa, const WrapArgs& wa, const Fab* sin, const Fab* lam_out, const Fab* lam_s1, const Fab* lam_sin,
                             bool accumulate, int s1_mode, const FoldTasks& ft, hipStream_t st);
 // Objective of the final state and its gradient lam (kind 0 kinetic energy,
 // kind 1 linear with weights); result[0] = J.
@@ -148,7 +157,7 @@ struct FusedConfig {
   int tx, ty, tz;  // tile extents
 };
 bool fused_supported(const amr_scheme_t& s, int tx, int ty);
-hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const TileDev* tiles,
+hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const WrapArgs& wa, const TileDev* tiles,
                               int ntiles, const BoxDev* boxes, const Closures& c, const StageArgs& sa,
                               const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st);
 

@@ -14,11 +14,17 @@ namespace {
 // ---------------------------------------------------------------- prims
 // CNS_advance.cpp:200-208: cons2prim on growntilebox(NGHOST)
 __global__ void __launch_bounds__(BLK) k_prims(const BoxDev* boxes, const BlockRef* bmap, Region reg, const double* box_dt, const Fab* st,
-                                               const Fab* pr, Closures c) {
+                                               const Fab* pr, Closures c, WrapArgs wa) {
   REGION_LOOP_BEGIN
   const Fab u = st[b], q = pr[b];
+  int ii = i, jj = j, kk = k;
+  if (wa.on) {
+    ii = ii < wa.lo[0] ? ii + wa.len[0] : (ii >= wa.lo[0] + wa.len[0] ? ii - wa.len[0] : ii);
+    jj = jj < wa.lo[1] ? jj + wa.len[1] : (jj >= wa.lo[1] + wa.len[1] ? jj - wa.len[1] : jj);
+    kk = kk < wa.lo[2] ? kk + wa.len[2] : (kk >= wa.lo[2] + wa.len[2] ? kk - wa.len[2] : kk);
+  }
   double qq[6];
-  cons2prim(u(i, j, k, 0), u(i, j, k, 1), u(i, j, k, 2), u(i, j, k, 3), u(i, j, k, 4), c, qq);
+  cons2prim(u(ii, jj, kk, 0), u(ii, jj, kk, 1), u(ii, jj, kk, 2), u(ii, jj, kk, 3), u(ii, jj, kk, 4), c, qq);
 #pragma unroll
   for (int n = 0; n < NPRIM; ++n) q(i, j, k, n) = qq[n];
 }
@@ -293,11 +299,11 @@ __global__ void __launch_bounds__(BLK) k_assemble(const BoxDev* boxes, const Blo
 }  // namespace
 
 hipError_t launch_prims(const BoxTable& bt, const UnfusedWork& w, const Closures& c, const double* box_dt,
-                        const Fab* sin, hipStream_t st) {
+                        const Fab* sin, const WrapArgs& wa, hipStream_t st) {
   const Region g3 = grown(NGHOST);
   Launch L = grid_for(bt, g3);
   if (!L.map) return hipErrorNotInitialized;
-  hipLaunchKernelGGL(k_prims, L.grid, dim3(BLK), 0, st, bt.d_boxes, L.map, g3, box_dt, sin, w.prims, c);
+  hipLaunchKernelGGL(k_prims, L.grid, dim3(BLK), 0, st, bt.d_boxes, L.map, g3, box_dt, sin, w.prims, c, wa);
   return hipGetLastError();
 }
 
@@ -328,7 +334,7 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
     hipLaunchKernelGGL(KERNEL, L_.grid, dim3(BLK), 0, st, B, L_.map, REG, bdt, __VA_ARGS__);     \
   } while (0)
   const Region g3 = grown(NGHOST), nod = nodal(), g1 = grown(1), g0 = grown(0);
-  LAUNCH(k_prims, g3, sin, w.prims, c);
+  LAUNCH(k_prims, g3, sin, w.prims, c, WrapArgs{0, {0, 0, 0}, {1, 1, 1}});
   if (s.rhs_euler) {
     if (s.flux_euler == 1) {
       double co[3];

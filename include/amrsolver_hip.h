@@ -130,9 +130,10 @@ int amr_stage_coeffs(int order_rk, int stages_rk, double dt, int stage, int* nst
 
 /* One time step of the level: CNS::advance (src/tim/CNS_advance.cpp:14-183)
  * = per stage FillPatch + compute_rhs + stage update.  `state` (ng >= 3)
- * holds U^n on entry and U^{n+1} on exit (valid cells; ghosts are left as
- * filled for the last stage).  Uses two ghosted workspace arrays owned by
- * the box table. */
+ * holds U^n on entry and U^{n+1} on exit (valid cells; ghost cells are not
+ * guaranteed to be filled on exit: a periodic level held as one array is
+ * advanced without ghost fills, its halos read from periodic images).  Uses
+ * two ghosted workspace arrays owned by the box table. */
 int amr_advance(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch,
                 const amr_fab_t* state, double time, double dt);
 

@@ -124,3 +124,25 @@ def test_fused_partial_tiles(ctx, tiling, layout):
         ol.advance(cls, sch, 1e-6)
     lev.set_tiling(0, 0, 0)
     assert rel_err(S.download(), ol.get_state()) <= TOL
+
+
+def test_fused_periodic_level_reads_halo_images(ctx):
+    """A periodic level held as one array: the fused kernel reads halo cells
+    from their periodic images, so the result does not depend on the ghost
+    cells (left unfilled here)."""
+    from amrsolver_amd import level as L
+    from oracle import oracle as O
+    cls, g, boxes, u = setup(23)
+    sch = P.scheme(flux_euler=1, order_keep=4, visc=True)
+    lev = L.Level(g, boxes, cls, sch, ctx=ctx)
+    lev.set_tiling(16, 16, 16)
+    S = lev.new_state()
+    S.upload(u)  # ghost cells stay zero
+    D = lev.new_state(ng=0)
+    lev.compute_rhs(S, D)
+    got = D.download()
+    ol = O.OracleLevel(g, boxes)
+    ol.set_state(u)
+    ref = ol.rhs(cls, sch)
+    lev.set_tiling(0, 0, 0)
+    assert rel_err(got, ref) <= TOL
