@@ -84,6 +84,7 @@ SIGNATURES = {
     "amr_set_tiling": ([_P, ctypes.POINTER(Scheme), ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "amr_autotune": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF,
                       ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+    "amr_set_source": ([_P, _PF], ctypes.c_int),
     "amr_vjp_rhs": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF, _PF, _PF], ctypes.c_int),
     "amr_vjp_stage": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), ctypes.c_int,
                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,

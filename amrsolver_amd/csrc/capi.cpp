@@ -181,6 +181,8 @@ struct amr_boxtab_s {
   std::unique_ptr<AdjRun> run;
   // autotuned tiling per scheme key for this box table
   std::map<std::string, FusedConfig> tuned;
+  // source added to dSdt for schemes with rhs_source = 1 (amr_set_source)
+  std::unique_ptr<DescSet> src;
   // ghost cells of each box covered by no box of the level (coarse-fine
   // ghosts of a refined level): regions of box b as tasks b -> b
   CopyTask* d_uncov = nullptr;
@@ -192,7 +194,7 @@ namespace {
 
 int check_scheme(const amr_scheme_t& s) {
   if (s.art_diss != 0) return fail(AMR_ERR_ARG, "art_diss=1 (JST dissipation) is not supported");
-  if (s.rhs_source != 0) return fail(AMR_ERR_ARG, "rhs_source=1: user_source is problem code and is not supported");
+  if (s.rhs_source != 0 && s.rhs_source != 1) return fail(AMR_ERR_ARG, "rhs_source must be 0 or 1");
   if (!s.rhs_euler && s.rhs_visc)
     return fail(AMR_ERR_ARG, "rhs_euler=0 with rhs_visc=1 is not supported (the reference accumulates into an "
                              "uninitialised flux array)");
@@ -707,8 +709,13 @@ WrapArgs wrap_args(const amr_boxtab_s* bt, DescSet* ds) {
   return wa;
 }
 
-int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageArgs& sa,
+int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageArgs& sa_in,
              DescSet* ds1, DescSet* dsin, DescSet* dout) {
+  StageArgs sa = sa_in;
+  if (sch.rhs_source) {
+    if (!bt->src) return fail(AMR_ERR_ARG, "rhs_source=1 needs a source registered with amr_set_source");
+    sa.src = bt->src->d;
+  }
   Closures c = make_closures(cls);
   hipStream_t st = bt->ctx->stream;
   const Fab* s1 = ds1 ? ds1->d : nullptr;
@@ -1165,6 +1172,7 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
     if (bt->ws_mem[w]) (void)hipFree(bt->ws_mem[w]);
   if (bt->d_uncov) (void)hipFree(bt->d_uncov);
   bt->run.reset();
+  bt->src.reset();
   bt->adj_g.reset();
   if (bt->adj_e_table) (void)hipFree(bt->adj_e_table);
   if (bt->adj_mem) (void)hipFree(bt->adj_mem);
@@ -1410,6 +1418,22 @@ int amr_totals(amr_boxtab_t bt, const amr_fab_t* state, double out[5]) {
   double probsize = 1.0;
   for (int d = 0; d < 3; ++d) probsize *= (bt->geom.dom_hi[d] - bt->geom.dom_lo[d] + 1) * bt->geom.dx[d];
   for (int n = 0; n < 5; ++n) out[n] = bt->h_result[n] * probsize;  // src/CNS.cpp:541
+  return AMR_OK;
+}
+
+int amr_set_source(amr_boxtab_t bt, const amr_fab_t* source) {
+  if (!bt) return fail(AMR_ERR_ARG, "null box table");
+  HIPCHK(hipStreamSynchronize(bt->ctx->stream));
+  if (!source) {
+    bt->src.reset();
+    return AMR_OK;
+  }
+  int rc = check_fabs(bt, source, 0);
+  if (rc) return rc;
+  std::unique_ptr<DescSet> ds;
+  if ((rc = new_desc(bt, source, &ds))) return rc;
+  bt->src = std::move(ds);
+  bt->graphs.clear();  // captured steps refer to the previous table
   return AMR_OK;
 }
 

@@ -332,6 +332,7 @@ struct StageArgs {
   StageCoef sc;
   const double* box_dt;
   int order_rk, stages_rk, stage;
+  const Fab* src = nullptr;  // per-box source added to dSdt (rhs_source = 1)
 };
 
 __device__ __forceinline__ bool box_active(const StageArgs& sa, int b) {

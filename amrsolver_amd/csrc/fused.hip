@@ -455,9 +455,11 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
       }
       if (fin) {
         gdouble* po = gptr(O.p) + O.idx(gi, gj, cpl - 1);
+        const Fab* srcf = sa.src;
 #pragma unroll
         for (int n = 0; n < 5; ++n) {
           double L = xy_prev[n] + dxi2 * (fz_prev[n] - fz[n]);
+          if (srcf) L = L + srcf[T.box](gi, gj, cpl - 1, n);  // user_source
           po[n * O.ns] = stage_combine(sc, s1v[n], sv[n], L);
         }
       }

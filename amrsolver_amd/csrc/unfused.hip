@@ -285,6 +285,7 @@ __global__ void __launch_bounds__(BLK) k_assemble(const BoxDev* boxes, const Blo
   for (int n = 0; n < NCONS; ++n) {
     double L = dx0 * (fx(i, j, k, n) - fx(i + 1, j, k, n)) + dx1 * (fy(i, j, k, n) - fy(i, j + 1, k, n)) +
                dx2 * (fz(i, j, k, n) - fz(i, j, k + 1, n));
+    if (sa.src) L = L + sa.src[b](i, j, k, n);  // user_source, CNS_advance.cpp:397-407
     double v;
     if (sc.mode == 2) {
       v = L;

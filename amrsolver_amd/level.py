@@ -234,6 +234,11 @@ class Level:
         call("amr_autotune", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, out)
         return tuple(out)
 
+    def set_source(self, src):
+        """Register the source arrays (a StateArrays, or None) added to dSdt
+        for schemes with rhs_source = 1."""
+        call("amr_set_source", self.h, src.fabs if src is not None else None)
+
     # --- reverse mode ----------------------------------------------------
     def vjp_rhs(self, Sborder, lam, grad):
         """grad = (dL/dU)^T lam at the ghost-filled state Sborder."""

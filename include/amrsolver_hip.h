@@ -182,6 +182,14 @@ int amr_set_tiling(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, int t
 int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch,
                  const amr_fab_t* state, int32_t tiling[3]);
 
+/* Source for schemes with rhs_source = 1: source[nbox] (NCONS components on
+ * the valid cells, any ghost width) is added to dSdt after the flux
+ * assembly, where CNS::compute_rhs calls the problem's user_source
+ * (src/tim/CNS_advance.cpp:397-407; e.g. exm/MMS_euler/prob.h:219-283, whose
+ * source depends only on position).  The arrays are read at every call, so
+ * the caller may update them in place; NULL unregisters.  Synchronous. */
+int amr_set_source(amr_boxtab_t bt, const amr_fab_t* source);
+
 /* ---- Reverse mode (no reference counterpart: the reference has no adjoint;
  * its lib/autodiff submodule is empty).  Implemented for the KEEP
  * (flux_euler = 1, orders 2/4/6) and viscous right-hand sides of

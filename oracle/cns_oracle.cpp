@@ -531,7 +531,6 @@ void viscous_point(int i, int j, int k, const A4& q, const A4& fx, const A4& fy,
 int check_args(const orc_scheme& s, int ng) {
   if (ng < NGHOST) return fail("state FAB needs at least 3 ghost cells");
   if (s.art_diss != 0) return fail("art_diss=1 (JST) is not part of this path");
-  if (s.rhs_source != 0) return fail("rhs_source=1: user_source is problem code; not supported");
   if (s.rhs_euler && !(s.flux_euler >= 0 && s.flux_euler <= 2)) return fail("flux_euler must be 0, 1 or 2");
   if (s.rhs_euler && s.flux_euler == 1 && !(s.order_keep == 2 || s.order_keep == 4 || s.order_keep == 6))
     return fail("order_keep must be 2, 4 or 6");
@@ -621,8 +620,10 @@ void box_assemble(const Box& bx, const A4 nf[3], const double dxinv[3], const A4
     });
 }
 
+// src: source FAB on the valid box (NCONS comps) added after the assembly
+// when rhs_source = 1 (the user_source hook, CNS_advance.cpp:397-407)
 int box_rhs(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, const A4& u,
-            const double dxinv[3], const A4& dsdt) {
+            const double dxinv[3], const A4& dsdt, const double* src) {
   Box bn = grow(nodal_all(bx), NGHOST);
   std::vector<double> fb[3];
   A4 nf[3];
@@ -633,6 +634,12 @@ int box_rhs(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, c
   int rc = box_flux(c, s, bx, ng, u, dxinv, nf);
   if (rc) return rc;
   box_assemble(bx, nf, dxinv, dsdt);
+  if (s.rhs_source) {
+    if (!src) return fail("rhs_source=1 needs a source array");
+    A4 sv = view(const_cast<double*>(src), bx.lo, bx.hi);
+    for (int n = 0; n < NCONS; ++n)
+      for_box(bx, [&](int i, int j, int k) { dsdt(i, j, k, n) = dsdt(i, j, k, n) + sv(i, j, k, n); });
+  }
   return 0;
 }
 
@@ -822,6 +829,7 @@ struct LevelOps {
   std::vector<std::vector<double>> vec[3];  // per vector, per box: valid data
   std::vector<Box> vbox;
   double* const* state;  // ghosted FABs (Sborder storage and final output)
+  const double* const* srcs = nullptr;  // per-box source arrays (rhs_source)
 
   long nval(int b) const { return npts(vbox[b].lo, vbox[b].hi) * NCONS; }
 
@@ -843,7 +851,7 @@ struct LevelOps {
       Box gb = grow(vbox[b], ng);
       A4 u = view(state[b], gb.lo, gb.hi);
       A4 d = view(vec[2][b].data(), vbox[b].lo, vbox[b].hi);
-      if (box_rhs(*c, *s, vbox[b], ng, u, dxinv, d)) err |= 1;
+      if (box_rhs(*c, *s, vbox[b], ng, u, dxinv, d, srcs ? srcs[b] : nullptr)) err |= 1;
     }
     return err ? -1 : 0;
   }
@@ -959,7 +967,7 @@ int orc_box_numflux(const orc_closures* c, const orc_scheme* s, const int lo[3],
 }
 
 int orc_box_rhs(const orc_closures* c, const orc_scheme* s, const int lo[3], const int hi[3], int ng,
-                const double* state, const double dxinv[3], double* dsdt) {
+                const double* state, const double dxinv[3], const double* src, double* dsdt) {
   Box bx;
   for (int d = 0; d < 3; ++d) {
     bx.lo[d] = lo[d];
@@ -968,7 +976,7 @@ int orc_box_rhs(const orc_closures* c, const orc_scheme* s, const int lo[3], con
   Box gb = grow(bx, ng);
   A4 u = view(const_cast<double*>(state), gb.lo, gb.hi);
   A4 d = view(dsdt, bx.lo, bx.hi);
-  return box_rhs(*c, *s, bx, ng, u, dxinv, d);
+  return box_rhs(*c, *s, bx, ng, u, dxinv, d, src);
 }
 
 int orc_fill_patch(const orc_geom* g, int nbox, const int* boxes, int ng, double* const* fabs) {
@@ -977,7 +985,7 @@ int orc_fill_patch(const orc_geom* g, int nbox, const int* boxes, int ng, double
 }
 
 int orc_advance(const orc_geom* g, const orc_closures* c, const orc_scheme* s, int nbox, const int* boxes,
-                int ng, double* const* state, double time, double dt, int nthreads) {
+                int ng, double* const* state, const double* const* src, double time, double dt, int nthreads) {
   (void)time;
   Level L = make_level(g, nbox, boxes, ng);
   LevelOps ops;
@@ -988,6 +996,7 @@ int orc_advance(const orc_geom* g, const orc_closures* c, const orc_scheme* s, i
   for (int d = 0; d < 3; ++d) ops.dxinv[d] = 1.0 / g->dx[d];
   ops.vbox = L.boxes;
   ops.state = state;
+  ops.srcs = src;
   for (int v = 0; v < 3; ++v) ops.vec[v].resize(nbox);
   for (int b = 0; b < nbox; ++b) {
     long n = npts(L.boxes[b].lo, L.boxes[b].hi) * NCONS;

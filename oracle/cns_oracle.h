@@ -63,16 +63,18 @@ int orc_box_prims(const orc_closures* c, const int lo[3], const int hi[3], int n
 int orc_box_numflux(const orc_closures* c, const orc_scheme* s, const int lo[3],
                     const int hi[3], int ng, const double* state, const double dxinv[3],
                     double* fx, double* fy, double* fz);
+/* src: NULL, or a source array on the valid box added to dSdt when
+   rhs_source = 1 (the user_source hook) */
 int orc_box_rhs(const orc_closures* c, const orc_scheme* s, const int lo[3],
                 const int hi[3], int ng, const double* state, const double dxinv[3],
-                double* dsdt);
+                const double* src, double* dsdt);
 
 /* ---- level (list of boxes, one ghosted FAB per box) ---- */
 int orc_fill_patch(const orc_geom* g, int nbox, const int* boxes, int ng,
                    double* const* fabs);
 int orc_advance(const orc_geom* g, const orc_closures* c, const orc_scheme* s,
                 int nbox, const int* boxes, int ng, double* const* state,
-                double time, double dt, int nthreads);
+                const double* const* src, double time, double dt, int nthreads);
 double orc_est_dt(const orc_geom* g, const orc_closures* c, int nbox,
                   const int* boxes, int ng, double* const* state, double cfl);
 void orc_totals(const orc_geom* g, int nbox, const int* boxes, int ng,

@@ -94,12 +94,12 @@ def load():
     lib.orc_box_numflux.argtypes = [ctypes.POINTER(OClosures), ctypes.POINTER(OScheme), I3, I3, ctypes.c_int,
                                     _DP, _DP, _DP, _DP, _DP]
     lib.orc_box_rhs.argtypes = [ctypes.POINTER(OClosures), ctypes.POINTER(OScheme), I3, I3, ctypes.c_int,
-                                _DP, _DP, _DP]
+                                _DP, _DP, _DP, _DP]
     PP = ctypes.POINTER(_DP)
     lib.orc_fill_patch.argtypes = [ctypes.POINTER(OGeom), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                    ctypes.c_int, PP]
     lib.orc_advance.argtypes = [ctypes.POINTER(OGeom), ctypes.POINTER(OClosures), ctypes.POINTER(OScheme),
-                                ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, PP,
+                                ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, PP, PP,
                                 ctypes.c_double, ctypes.c_double, ctypes.c_int]
     lib.orc_est_dt.restype = ctypes.c_double
     lib.orc_est_dt.argtypes = [ctypes.POINTER(OGeom), ctypes.POINTER(OClosures), ctypes.c_int,
@@ -159,13 +159,17 @@ def riemann_prob(gamma, rl, ul, pl, ut1l, ut2l, rr, ur, pr, ut1r, ut2r, smallp=1
 # A box FAB is a numpy array (ncomp, nz+2ng, ny+2ng, nx+2ng) (i fastest).
 
 
-def box_rhs(c, s, lo, hi, ng, state, dxinv):
+def box_rhs(c, s, lo, hi, ng, state, dxinv, src=None):
+    """dSdt of one box; src: (5, nz, ny, nx) source on the valid box, added
+    when the scheme has rhs_source = 1."""
     n = [hi[d] - lo[d] + 1 for d in range(3)]
     out = np.zeros((5, n[2], n[1], n[0]))
     I3 = ctypes.c_int * 3
     dx = np.ascontiguousarray(dxinv, dtype=np.float64)
+    sp = None if src is None else np.ascontiguousarray(src, dtype=np.float64)
     _check(load().orc_box_rhs(ctypes.byref(closures(c)), ctypes.byref(scheme(s)), I3(*lo), I3(*hi), ng,
-                              _ptr(np.ascontiguousarray(state)), _ptr(dx), _ptr(out)))
+                              _ptr(np.ascontiguousarray(state)), _ptr(dx), None if sp is None else _ptr(sp),
+                              _ptr(out)))
     return out
 
 
@@ -233,9 +237,22 @@ class OracleLevel:
     def fill_patch(self):
         _check(load().orc_fill_patch(ctypes.byref(self.g), len(self.boxes), self.barr, self.ng, self._pp()))
 
-    def advance(self, c, s, dt, time=0.0, nthreads=1):
+    def _src(self, source):
+        """Per-box source arrays from a (5, nz, ny, nx) domain array."""
+        if source is None:
+            return None, None
+        keep = [np.ascontiguousarray(source[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
+                for lo, hi in self.boxes]
+        arr = (_DP * len(keep))()
+        for b, a in enumerate(keep):
+            arr[b] = _ptr(a)
+        return arr, keep
+
+    def advance(self, c, s, dt, time=0.0, nthreads=1, source=None):
+        sp, keep = self._src(source)
         _check(load().orc_advance(ctypes.byref(self.g), ctypes.byref(closures(c)), ctypes.byref(scheme(s)),
-                                  len(self.boxes), self.barr, self.ng, self._pp(), time, dt, nthreads))
+                                  len(self.boxes), self.barr, self.ng, self._pp(), sp, time, dt, nthreads))
+        del keep
 
     def est_dt(self, c, cfl):
         return load().orc_est_dt(ctypes.byref(self.g), ctypes.byref(closures(c)), len(self.boxes), self.barr,
@@ -246,13 +263,14 @@ class OracleLevel:
         load().orc_totals(ctypes.byref(self.g), len(self.boxes), self.barr, self.ng, self._pp(), _ptr(out))
         return out
 
-    def rhs(self, c, s):
+    def rhs(self, c, s, source=None):
         """dSdt of every box after a FillPatch of the current state."""
         self.fill_patch()
         dx = [1.0 / self.g.dx[d] for d in range(3)]
         out = np.zeros((5, self.n[2], self.n[1], self.n[0]))
         for (lo, hi), f in zip(self.boxes, self.fabs):
-            out[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = box_rhs(c, s, lo, hi, self.ng, f, dx)
+            src = None if source is None else source[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+            out[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = box_rhs(c, s, lo, hi, self.ng, f, dx, src)
         return out
 
 
