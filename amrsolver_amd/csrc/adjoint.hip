@@ -34,6 +34,13 @@ __device__ __forceinline__ bool in_box(const BoxDev& bx, int g, int i, int j, in
          k >= bx.lo[2] - g && k <= bx.hi[2] + g;
 }
 
+// Periodic index of direction d for a domain-wide array (identity when off).
+template <bool WRAP>
+__device__ __forceinline__ int wr(const WrapArgs& wa, int d, int x) {
+  if (!WRAP) return x;
+  return x < wa.lo[d] ? x + wa.len[d] : (x >= wa.lo[d] + wa.len[d] ? x - wa.len[d] : x);
+}
+
 // adjoint of dSdt at (i,j,k): zero outside the valid box
 __device__ __forceinline__ void lam_at(const Fab& L, const BoxDev& bx, int i, int j, int k, double v[5]) {
   if (in_box(bx, 0, i, j, k)) {
@@ -97,35 +104,63 @@ __device__ __forceinline__ void load_q(const Fab& q, int i, int j, int k, double
 }
 
 // KEEP gather of direction D into g for cell (i,j,k) with primitives qc.
-template <int H, int D>
+// With WRAP (a periodic domain-wide level) every cell is valid: no range
+// conditions, neighbour indices wrap periodically.
+template <int H, int D, bool WRAP = false>
 __device__ __forceinline__ void adj_keep_dir(const Fab& q, const Fab& L, const BoxDev& bx, int i, int j, int k,
                                              const double qc[6], const double co[3], double dxi, double cv,
-                                             double g[6]) {
+                                             double g[6], const WrapArgs& wa = WrapArgs{}) {
   const int ci[3] = {i, j, k};
+  if (!WRAP) {
 #pragma unroll
-  for (int t = 0; t < 3; ++t)
-    if (t != D && (ci[t] < bx.lo[t] || ci[t] > bx.hi[t])) return;  // every weight is zero
+    for (int t = 0; t < 3; ++t)
+      if (t != D && (ci[t] < bx.lo[t] || ci[t] > bx.hi[t])) return;  // every weight is zero
+  }
   const int x = ci[D], lo = bx.lo[D], hi = bx.hi[D];
   const int di = D == 0, dj = D == 1, dk = D == 2;
+  auto nb = [&](int o, int& ii, int& jj, int& kk) {
+    ii = wr<WRAP>(wa, 0, i + o * di);
+    jj = wr<WRAP>(wa, 1, j + o * dj);
+    kk = wr<WRAP>(wa, 2, k + o * dk);
+  };
   double lc[5];
-  lam_at(L, bx, i, j, k, lc);
+  if (WRAP) {
+#pragma unroll
+    for (int n = 0; n < NCONS; ++n) lc[n] = L(i, j, k, n);
+  } else {
+    lam_at(L, bx, i, j, k, lc);
+  }
 #pragma unroll
   for (int l = 1; l <= H; ++l) {
     const double cw = co[l - 1] * dxi;
-    if (x >= lo && x <= hi + l) {  // cell 1 of the pair (c, c-l)
+    if (WRAP || (x >= lo && x <= hi + l)) {  // cell 1 of the pair (c, c-l)
       double lm[5], w[5], qm[6];
-      lam_at(L, bx, i - l * di, j - l * dj, k - l * dk, lm);
+      int ii, jj, kk;
+      nb(-l, ii, jj, kk);
+      if (WRAP) {
+#pragma unroll
+        for (int n = 0; n < NCONS; ++n) lm[n] = L(ii, jj, kk, n);
+      } else {
+        lam_at(L, bx, ii, jj, kk, lm);
+      }
 #pragma unroll
       for (int n = 0; n < NCONS; ++n) w[n] = cw * (lc[n] - lm[n]);
-      load_q(q, i - l * di, j - l * dj, k - l * dk, qm);
+      load_q(q, ii, jj, kk, qm);
       keep_pair_vjp<D, true, false>(w, qc, qm, cv, g, nullptr);
     }
-    if (x >= lo - l && x <= hi) {  // cell 2 of the pair (c+l, c)
+    if (WRAP || (x >= lo - l && x <= hi)) {  // cell 2 of the pair (c+l, c)
       double lp[5], w[5], qp[6];
-      lam_at(L, bx, i + l * di, j + l * dj, k + l * dk, lp);
+      int ii, jj, kk;
+      nb(l, ii, jj, kk);
+      if (WRAP) {
+#pragma unroll
+        for (int n = 0; n < NCONS; ++n) lp[n] = L(ii, jj, kk, n);
+      } else {
+        lam_at(L, bx, ii, jj, kk, lp);
+      }
 #pragma unroll
       for (int n = 0; n < NCONS; ++n) w[n] = cw * (lp[n] - lc[n]);
-      load_q(q, i + l * di, j + l * dj, k + l * dk, qp);
+      load_q(q, ii, jj, kk, qp);
       keep_pair_vjp<D, false, true>(w, qp, qc, cv, nullptr, g);
     }
   }
@@ -134,39 +169,51 @@ __device__ __forceinline__ void adj_keep_dir(const Fab& q, const Fab& L, const B
 // Viscous pre-pass on grow(b,1): the adjoint of the cell fluxes pf_d(j) is
 // 0.5 dxi_d (lam(j+e_d) - lam(j-e_d)); store the adjoints of the central
 // differences (times their 0.5 dxi factor) and of the cell's own u, v, w, T.
+template <bool WRAP>
 __global__ void __launch_bounds__(BLK) k_adj_visc(const BoxDev* boxes, const BlockRef* bmap, Region reg,
                                                   const double* box_dt, const Fab* pr, const Fab* lam,
-                                                  const Fab* ew, Closures c) {
+                                                  const Fab* ew, Closures c, WrapArgs wa) {
   REGION_LOOP_BEGIN
   const double dx0 = bx.dxi[0], dx1 = bx.dxi[1], dx2 = bx.dxi[2];
   const Fab q = pr[b], L = lam[b], E = ew[b];
+  const int ip = wr<WRAP>(wa, 0, i + 1), im = wr<WRAP>(wa, 0, i - 1);
+  const int jp = wr<WRAP>(wa, 1, j + 1), jm = wr<WRAP>(wa, 1, j - 1);
+  const int kp = wr<WRAP>(wa, 2, k + 1), km = wr<WRAP>(wa, 2, k - 1);
+  auto lget = [&](int ii, int jj, int kk, double v[5]) {
+    if (WRAP) {
+#pragma unroll
+      for (int n = 0; n < NCONS; ++n) v[n] = L(ii, jj, kk, n);
+    } else {
+      lam_at(L, bx, ii, jj, kk, v);
+    }
+  };
   double a[5], bb[5], cc[5], lp[5], lm[5];
-  lam_at(L, bx, i + 1, j, k, lp);
-  lam_at(L, bx, i - 1, j, k, lm);
+  lget(ip, j, k, lp);
+  lget(im, j, k, lm);
 #pragma unroll
   for (int n = 0; n < NCONS; ++n) a[n] = 0.5 * dx0 * (lp[n] - lm[n]);
-  lam_at(L, bx, i, j + 1, k, lp);
-  lam_at(L, bx, i, j - 1, k, lm);
+  lget(i, jp, k, lp);
+  lget(i, jm, k, lm);
 #pragma unroll
   for (int n = 0; n < NCONS; ++n) bb[n] = 0.5 * dx1 * (lp[n] - lm[n]);
-  lam_at(L, bx, i, j, k + 1, lp);
-  lam_at(L, bx, i, j, k - 1, lm);
+  lget(i, j, kp, lp);
+  lget(i, j, km, lm);
 #pragma unroll
   for (int n = 0; n < NCONS; ++n) cc[n] = 0.5 * dx2 * (lp[n] - lm[n]);
   // forward values at the cell (as in the viscous flux kernel)
   const double ux = q(i, j, k, QU), uy = q(i, j, k, QV), uz = q(i, j, k, QW), T = q(i, j, k, QT);
-  const double dudx = (q(i + 1, j, k, QU) - q(i - 1, j, k, QU)) * 0.5 * dx0;
-  const double dvdx = (q(i + 1, j, k, QV) - q(i - 1, j, k, QV)) * 0.5 * dx0;
-  const double dwdx = (q(i + 1, j, k, QW) - q(i - 1, j, k, QW)) * 0.5 * dx0;
-  const double dTdx = (q(i + 1, j, k, QT) - q(i - 1, j, k, QT)) * 0.5 * dx0;
-  const double dudy = (q(i, j + 1, k, QU) - q(i, j - 1, k, QU)) * 0.5 * dx1;
-  const double dvdy = (q(i, j + 1, k, QV) - q(i, j - 1, k, QV)) * 0.5 * dx1;
-  const double dwdy = (q(i, j + 1, k, QW) - q(i, j - 1, k, QW)) * 0.5 * dx1;
-  const double dTdy = (q(i, j + 1, k, QT) - q(i, j - 1, k, QT)) * 0.5 * dx1;
-  const double dudz = (q(i, j, k + 1, QU) - q(i, j, k - 1, QU)) * 0.5 * dx2;
-  const double dvdz = (q(i, j, k + 1, QV) - q(i, j, k - 1, QV)) * 0.5 * dx2;
-  const double dwdz = (q(i, j, k + 1, QW) - q(i, j, k - 1, QW)) * 0.5 * dx2;
-  const double dTdz = (q(i, j, k + 1, QT) - q(i, j, k - 1, QT)) * 0.5 * dx2;
+  const double dudx = (q(ip, j, k, QU) - q(im, j, k, QU)) * 0.5 * dx0;
+  const double dvdx = (q(ip, j, k, QV) - q(im, j, k, QV)) * 0.5 * dx0;
+  const double dwdx = (q(ip, j, k, QW) - q(im, j, k, QW)) * 0.5 * dx0;
+  const double dTdx = (q(ip, j, k, QT) - q(im, j, k, QT)) * 0.5 * dx0;
+  const double dudy = (q(i, jp, k, QU) - q(i, jm, k, QU)) * 0.5 * dx1;
+  const double dvdy = (q(i, jp, k, QV) - q(i, jm, k, QV)) * 0.5 * dx1;
+  const double dwdy = (q(i, jp, k, QW) - q(i, jm, k, QW)) * 0.5 * dx1;
+  const double dTdy = (q(i, jp, k, QT) - q(i, jm, k, QT)) * 0.5 * dx1;
+  const double dudz = (q(i, j, kp, QU) - q(i, j, km, QU)) * 0.5 * dx2;
+  const double dvdz = (q(i, j, kp, QV) - q(i, j, km, QV)) * 0.5 * dx2;
+  const double dwdz = (q(i, j, kp, QW) - q(i, j, km, QW)) * 0.5 * dx2;
+  const double dTdz = (q(i, j, kp, QT) - q(i, j, km, QT)) * 0.5 * dx2;
   const double div = dudx + dvdy + dwdz;
   const double mu = visc(c, T);
   const double kk = cond(c, T, mu);
@@ -219,12 +266,13 @@ __global__ void __launch_bounds__(BLK) k_adj_visc(const BoxDev* boxes, const Blo
 //   mode 2: G;   lam_s1 (+=|=) a lam_out
 // where G = (d dSdt / d S)^T lam_out; on ghost cells G itself is stored for
 // the transposed ghost fill.
-template <int H, bool EULER, bool VISC>
+// With WRAP the arrays are domain-wide and every cell of the region is valid.
+template <int H, bool EULER, bool VISC, bool WRAP>
 __global__ void __launch_bounds__(BLK) k_adj_cell(const BoxDev* boxes, const BlockRef* bmap, Region reg,
                                                   const double* box_dt, const Fab* pr, const Fab* lam,
                                                   const Fab* ew, const Fab* gw, Closures c, double co0,
                                                   double co1, double co2, StageArgs sa, const Fab* lam_s1,
-                                                  const Fab* lam_sin, int accumulate, int s1_mode) {
+                                                  const Fab* lam_sin, int accumulate, int s1_mode, WrapArgs wa) {
   REGION_LOOP_BEGIN
   const Fab q = pr[b], L = lam[b];
   double qc[6];
@@ -232,26 +280,28 @@ __global__ void __launch_bounds__(BLK) k_adj_cell(const BoxDev* boxes, const Blo
   double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (EULER) {
     const double co[3] = {co0, co1, co2};
-    adj_keep_dir<H, 0>(q, L, bx, i, j, k, qc, co, bx.dxi[0], c.cv, g);
-    adj_keep_dir<H, 1>(q, L, bx, i, j, k, qc, co, bx.dxi[1], c.cv, g);
-    adj_keep_dir<H, 2>(q, L, bx, i, j, k, qc, co, bx.dxi[2], c.cv, g);
+    adj_keep_dir<H, 0, WRAP>(q, L, bx, i, j, k, qc, co, bx.dxi[0], c.cv, g, wa);
+    adj_keep_dir<H, 1, WRAP>(q, L, bx, i, j, k, qc, co, bx.dxi[1], c.cv, g, wa);
+    adj_keep_dir<H, 2, WRAP>(q, L, bx, i, j, k, qc, co, bx.dxi[2], c.cv, g, wa);
   }
   if (VISC) {
     const Fab E = ew[b];
-    if (in_box(bx, 1, i, j, k)) {
+    if (WRAP || in_box(bx, 1, i, j, k)) {
 #pragma unroll
       for (int m = 0; m < 4; ++m) g[QU + m] += E(i, j, k, m);
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const int di = d == 0, dj = d == 1, dk = d == 2;
-      if (in_box(bx, 1, i - di, j - dj, k - dk)) {
+      const int im = wr<WRAP>(wa, 0, i - di), jm = wr<WRAP>(wa, 1, j - dj), km = wr<WRAP>(wa, 2, k - dk);
+      const int ip = wr<WRAP>(wa, 0, i + di), jp = wr<WRAP>(wa, 1, j + dj), kp = wr<WRAP>(wa, 2, k + dk);
+      if (WRAP || in_box(bx, 1, im, jm, km)) {
 #pragma unroll
-        for (int m = 0; m < 4; ++m) g[QU + m] += E(i - di, j - dj, k - dk, 4 + 4 * d + m);
+        for (int m = 0; m < 4; ++m) g[QU + m] += E(im, jm, km, 4 + 4 * d + m);
       }
-      if (in_box(bx, 1, i + di, j + dj, k + dk)) {
+      if (WRAP || in_box(bx, 1, ip, jp, kp)) {
 #pragma unroll
-        for (int m = 0; m < 4; ++m) g[QU + m] -= E(i + di, j + dj, k + dk, 4 + 4 * d + m);
+        for (int m = 0; m < 4; ++m) g[QU + m] -= E(ip, jp, kp, 4 + 4 * d + m);
       }
     }
   }
@@ -266,7 +316,7 @@ __global__ void __launch_bounds__(BLK) k_adj_cell(const BoxDev* boxes, const Blo
   G[UMY] = g[QV] * rinv - c.gm1 * uy * gp;
   G[UMZ] = g[QW] * rinv - c.gm1 * uz * gp;
   G[UET] = c.gm1 * gp;
-  if (!in_box(bx, 0, i, j, k)) {
+  if (!WRAP && !in_box(bx, 0, i, j, k)) {
     const Fab Gh = gw[b];
 #pragma unroll
     for (int n = 0; n < NCONS; ++n) Gh(i, j, k, n) = G[n];
@@ -446,25 +496,51 @@ hipError_t launch_adj_stage(const BoxTable& bt, const UnfusedWork& w, const AdjW
                             const amr_scheme_t& s, const StageArgs& sa, const WrapArgs& wa, const Fab* sin, const Fab* lam_out, const Fab* lam_s1, const Fab* lam_sin,
                             bool accumulate, int s1_mode, const FoldTasks& ft, hipStream_t st) {
   const BoxDev* B = bt.d_boxes;
+  double co[3] = {0.0, 0.0, 0.0};
+  keep_coefficients(s.order_keep, co);
+  const bool eu = s.rhs_euler != 0, vi = s.rhs_visc != 0;
+  const int h = s.order_keep / 2;
+  if (wa.on && aw.lq) {
+    // periodic domain-wide level: the transpose of the periodic stencil,
+    // gathered on the valid cells; no ghost outputs, no transposed fill
+    const Region g0 = grown(0);
+    Launch L0 = grid_for(bt, g0);
+    if (!L0.map) return hipErrorNotInitialized;
+    UnfusedWork lw = w;
+    lw.prims = aw.lq;
+    hipError_t e = launch_prims_region(bt, lw, c, sa.box_dt, sin, g0, st);
+    if (e != hipSuccess) return e;
+    if (vi)
+      hipLaunchKernelGGL(k_adj_visc<true>, L0.grid, dim3(BLK), 0, st, B, L0.map, g0, sa.box_dt, aw.lq, lam_out,
+                         aw.le, c, wa);
+#define CELLW(HH, EE, VV)                                                                                      \
+  hipLaunchKernelGGL((k_adj_cell<HH, EE, VV, true>), L0.grid, dim3(BLK), 0, st, B, L0.map, g0, sa.box_dt,     \
+                     aw.lq, lam_out, aw.le, aw.g, c, co[0], co[1], co[2], sa, lam_s1, lam_sin, accumulate ? 1 : 0, \
+                     s1_mode, wa)
+    if (!eu) CELLW(1, false, true);
+    else if (h == 3) { if (vi) CELLW(3, true, true); else CELLW(3, true, false); }
+    else if (h == 2) { if (vi) CELLW(2, true, true); else CELLW(2, true, false); }
+    else { if (vi) CELLW(1, true, true); else CELLW(1, true, false); }
+#undef CELLW
+    return hipGetLastError();
+  }
+  const WrapArgs nowrap{0, {0, 0, 0}, {1, 1, 1}};
   hipError_t e = launch_prims(bt, w, c, sa.box_dt, sin, wa, st);
   if (e != hipSuccess) return e;
   if (s.rhs_visc) {
     const Region g1 = grown(1);
     Launch L = grid_for(bt, g1);
     if (!L.map) return hipErrorNotInitialized;
-    hipLaunchKernelGGL(k_adj_visc, L.grid, dim3(BLK), 0, st, B, L.map, g1, sa.box_dt, w.prims, lam_out, aw.e, c);
+    hipLaunchKernelGGL(k_adj_visc<false>, L.grid, dim3(BLK), 0, st, B, L.map, g1, sa.box_dt, w.prims, lam_out,
+                       aw.e, c, nowrap);
   }
-  double co[3] = {0.0, 0.0, 0.0};
-  keep_coefficients(s.order_keep, co);
-  const bool eu = s.rhs_euler != 0, vi = s.rhs_visc != 0;
-  const int h = s.order_keep / 2;
   const Region g3 = grown(NGHOST);
   Launch L3 = grid_for(bt, g3);
   if (!L3.map) return hipErrorNotInitialized;
 #define CELL(HH, EE, VV)                                                                                     \
-  hipLaunchKernelGGL((k_adj_cell<HH, EE, VV>), L3.grid, dim3(BLK), 0, st, B, L3.map, g3, sa.box_dt, w.prims, \
-                     lam_out, aw.e, aw.g, c, co[0], co[1], co[2], sa, lam_s1, lam_sin, accumulate ? 1 : 0,   \
-                     s1_mode)
+  hipLaunchKernelGGL((k_adj_cell<HH, EE, VV, false>), L3.grid, dim3(BLK), 0, st, B, L3.map, g3, sa.box_dt,   \
+                     w.prims, lam_out, aw.e, aw.g, c, co[0], co[1], co[2], sa, lam_s1, lam_sin, accumulate ? 1 : 0, \
+                     s1_mode, nowrap)
   if (!eu) CELL(1, false, true);
   else if (h == 3) { if (vi) CELL(3, true, true); else CELL(3, true, false); }
   else if (h == 2) { if (vi) CELL(2, true, true); else CELL(2, true, false); }

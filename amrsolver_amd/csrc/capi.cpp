@@ -178,6 +178,10 @@ struct amr_boxtab_s {
   AdjWork aw;
   Fab* adj_e_table = nullptr;
   std::unique_ptr<DescSet> adj_g;
+  // domain-wide reverse-mode arrays of a periodic level held as one array
+  double* adj_lmem = nullptr;
+  Fab* adj_lq_table = nullptr;
+  Fab* adj_le_table = nullptr;
   std::unique_ptr<AdjRun> run;
   // autotuned tiling per scheme key for this box table
   std::map<std::string, FusedConfig> tuned;
@@ -884,11 +888,48 @@ int ensure_adjoint(amr_boxtab_s* bt, const amr_scheme_t& s) {
   return AMR_OK;
 }
 
+// Descriptor of one array over the level's domain (no ghost cells) with nc
+// components at `base`.
+amr_fab_t domain_fab(const amr_boxtab_s* bt, double* base) {
+  return compact_fab(bt->geom.dom_lo, bt->geom.dom_hi, 0, base);
+}
+
+// Domain-wide primitives and viscous pre-pass arrays for the reverse stage of
+// a periodic level held as one array.
+int ensure_adjoint_level(amr_boxtab_s* bt, const amr_scheme_t& s) {
+  const bool visc = s.rhs_visc != 0;
+  if (bt->adj_lmem && (bt->adj_le_table || !visc)) {
+    bt->aw.lq = bt->adj_lq_table;
+    bt->aw.le = bt->adj_le_table;
+    return AMR_OK;
+  }
+  HIPCHK(hipStreamSynchronize(bt->ctx->stream));
+  if (bt->adj_lq_table) HIPCHK(hipFree(bt->adj_lq_table));
+  if (bt->adj_le_table) HIPCHK(hipFree(bt->adj_le_table));
+  if (bt->adj_lmem) HIPCHK(hipFree(bt->adj_lmem));
+  bt->adj_lq_table = bt->adj_le_table = nullptr;
+  bt->adj_lmem = nullptr;
+  const int nb = bt->bt.nbox;
+  const int64_t n = cells(bt->geom.dom_lo, bt->geom.dom_hi);
+  HIPCHK(hipMalloc(&bt->adj_lmem, sizeof(double) * n * (NPRIM + (visc ? ADJ_NE : 0))));
+  std::vector<Fab> h(nb, make_fab(domain_fab(bt, bt->adj_lmem)));
+  HIPCHK(hipMalloc(&bt->adj_lq_table, sizeof(Fab) * nb));
+  HIPCHK(hipMemcpy(bt->adj_lq_table, h.data(), sizeof(Fab) * nb, hipMemcpyHostToDevice));
+  if (visc) {
+    std::vector<Fab> he(nb, make_fab(domain_fab(bt, bt->adj_lmem + n * NPRIM)));
+    HIPCHK(hipMalloc(&bt->adj_le_table, sizeof(Fab) * nb));
+    HIPCHK(hipMemcpy(bt->adj_le_table, he.data(), sizeof(Fab) * nb, hipMemcpyHostToDevice));
+  }
+  bt->aw.lq = bt->adj_lq_table;
+  bt->aw.le = bt->adj_le_table;
+  return AMR_OK;
+}
+
 // Reverse of one stage: lam_sin (=|+=) stage terms of lam_out through the
 // RHS VJP at `sin` (ghost-filled); lam_s1 gets a lam_out (s1_mode 1: +=,
 // 2: =).  lam_sin and lam_s1 must not alias lam_out.
 int adj_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageArgs& sa,
-              DescSet* sin, const DescSet* lam_out, const DescSet* lam_s1, const DescSet* lam_sin, bool acc,
+              DescSet* sin, DescSet* lam_out, const DescSet* lam_s1, const DescSet* lam_sin, bool acc,
               int s1_mode) {
   int rc;
   if ((rc = ensure_unfused(bt, sch))) return rc;
@@ -896,7 +937,13 @@ int adj_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& s
   const Closures c = make_closures(cls);
   const DescSet* g = bt->adj_g.get();
   const FoldTasks ft{g->d_copy, g->ncopy, g->ncopy_cells, g->d_bc, g->nbc, g->nbc_cells};
-  HIPCHK(launch_adj_stage(bt->bt, bt->uw, bt->aw, c, sch, sa, wrap_args(bt, sin), sin->d, lam_out->d,
+  const WrapArgs wa = wrap_args(bt, sin);
+  // the domain-wide path needs lam_out as one periodic array as well
+  const bool level = wa.on && wrap_ok(bt, lam_out);
+  if (level && (rc = ensure_adjoint_level(bt, sch))) return rc;
+  AdjWork aw = bt->aw;
+  if (!level) aw.lq = aw.le = nullptr;
+  HIPCHK(launch_adj_stage(bt->bt, bt->uw, aw, c, sch, sa, wa, sin->d, lam_out->d,
                           lam_s1 ? lam_s1->d : nullptr, lam_sin->d, acc, lam_s1 ? s1_mode : 0, ft,
                           bt->ctx->stream));
   return AMR_OK;
@@ -956,6 +1003,10 @@ int ensure_run(amr_boxtab_s* bt, const amr_fab_t* state, int nsteps, int k, int 
     if ((rc = build_fill(bt, ds.get()))) return rc;
     R->states.push_back(std::move(ds));
   }
+  // adjoint arrays: windows of one domain-wide array when the state is a
+  // periodic level held as one array (the reverse stage then gathers across
+  // boxes), else one compact array per box
+  const bool level = wrap_ok(bt, R->states[0].get());
   int64_t per = 0;
   for (int b = 0; b < nb; ++b) per += compact_len(bt->bt.h_boxes[b].lo, bt->bt.h_boxes[b].hi, 0, NCONS);
   double* lm = nullptr;
@@ -964,10 +1015,21 @@ int ensure_run(amr_boxtab_s* bt, const amr_fab_t* state, int nsteps, int k, int 
   for (int w = 0; w < 3; ++w) {
     std::vector<amr_fab_t> f(nb);
     int64_t off = 0;
+    const amr_fab_t whole = domain_fab(bt, lm + w * per);
     for (int b = 0; b < nb; ++b) {
       const BoxDev& B = bt->bt.h_boxes[b];
-      f[b] = compact_fab(B.lo, B.hi, 0, lm + w * per + off);
-      off += compact_len(B.lo, B.hi, 0, NCONS);
+      if (level) {
+        f[b] = whole;
+        for (int d = 0; d < 3; ++d) {
+          f[b].lo[d] = B.lo[d];
+          f[b].hi[d] = B.hi[d];
+        }
+        f[b].data = whole.data + (B.lo[0] - bt->geom.dom_lo[0]) + (B.lo[1] - bt->geom.dom_lo[1]) * whole.jstride +
+                    (B.lo[2] - bt->geom.dom_lo[2]) * whole.kstride;
+      } else {
+        f[b] = compact_fab(B.lo, B.hi, 0, lm + w * per + off);
+        off += compact_len(B.lo, B.hi, 0, NCONS);
+      }
     }
     if ((rc = new_desc(bt, f.data(), &R->lam[w]))) return rc;
   }
@@ -1176,6 +1238,9 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   bt->adj_g.reset();
   if (bt->adj_e_table) (void)hipFree(bt->adj_e_table);
   if (bt->adj_mem) (void)hipFree(bt->adj_mem);
+  if (bt->adj_lq_table) (void)hipFree(bt->adj_lq_table);
+  if (bt->adj_le_table) (void)hipFree(bt->adj_le_table);
+  if (bt->adj_lmem) (void)hipFree(bt->adj_lmem);
   for (auto& m : bt->bt.block_maps) (void)hipFree(m.second.first);
   if (bt->bt.d_boxes) (void)hipFree(bt->bt.d_boxes);
   if (bt->d_partial) (void)hipFree(bt->d_partial);

@@ -99,6 +99,10 @@ hipError_t launch_prims(const BoxTable& bt, const UnfusedWork& w, const Closures
 struct AdjWork {
   Fab* e = nullptr;
   Fab* g = nullptr;
+  // domain-wide primitives and viscous pre-pass arrays of a periodic level
+  // held as one array (every box's entry maps the same storage)
+  Fab* lq = nullptr;
+  Fab* le = nullptr;
 };
 bool adjoint_supported(const amr_scheme_t& s);
 // Ghost-fill task lists of the adjoint workspace g.

@@ -93,4 +93,8 @@ inline Region face_region(int d) {
 }
 
 
+// cons2prim of `sin` into w.prims on region r of every box.
+hipError_t launch_prims_region(const BoxTable& bt, const UnfusedWork& w, const Closures& c, const double* box_dt,
+                               const Fab* sin, const Region& r, hipStream_t st);
+
 }  // namespace amr

@@ -308,6 +308,15 @@ hipError_t launch_prims(const BoxTable& bt, const UnfusedWork& w, const Closures
   return hipGetLastError();
 }
 
+hipError_t launch_prims_region(const BoxTable& bt, const UnfusedWork& w, const Closures& c, const double* box_dt,
+                               const Fab* sin, const Region& r, hipStream_t st) {
+  Launch L = grid_for(bt, r);
+  if (!L.map) return hipErrorNotInitialized;
+  hipLaunchKernelGGL(k_prims, L.grid, dim3(BLK), 0, st, bt.d_boxes, L.map, r, box_dt, sin, w.prims, c,
+                     WrapArgs{0, {0, 0, 0}, {1, 1, 1}});
+  return hipGetLastError();
+}
+
 hipError_t prepare_block_maps(const BoxTable& bt) {
   const Region fixed[4] = {grown(NGHOST), nodal(), grown(1), grown(0)};
   for (const Region& r : fixed) {
