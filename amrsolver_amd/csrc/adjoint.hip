@@ -105,7 +105,11 @@ __device__ __forceinline__ void load_q(const Fab& q, int i, int j, int k, double
 
 // KEEP gather of direction D into g for cell (i,j,k) with primitives qc.
 // With WRAP (a periodic domain-wide level) every cell is valid: no range
-// conditions, neighbour indices wrap periodically.
+// conditions, neighbour indices wrap periodically.  Each pair still sits
+// behind a uniform branch on wa.on (always set when WRAP is launched): in one
+// straight-line block the compiler issues every neighbour load of the three
+// directions first and needs ~370 VGPRs (one wave per SIMD); with a block per
+// pair it keeps ~110 (four waves).
 template <int H, int D, bool WRAP = false>
 __device__ __forceinline__ void adj_keep_dir(const Fab& q, const Fab& L, const BoxDev& bx, int i, int j, int k,
                                              const double qc[6], const double co[3], double dxi, double cv,
@@ -133,7 +137,7 @@ __device__ __forceinline__ void adj_keep_dir(const Fab& q, const Fab& L, const B
 #pragma unroll
   for (int l = 1; l <= H; ++l) {
     const double cw = co[l - 1] * dxi;
-    if (WRAP || (x >= lo && x <= hi + l)) {  // cell 1 of the pair (c, c-l)
+    if (WRAP ? wa.on != 0 : (x >= lo && x <= hi + l)) {  // cell 1 of the pair (c, c-l)
       double lm[5], w[5], qm[6];
       int ii, jj, kk;
       nb(-l, ii, jj, kk);
@@ -148,7 +152,7 @@ __device__ __forceinline__ void adj_keep_dir(const Fab& q, const Fab& L, const B
       load_q(q, ii, jj, kk, qm);
       keep_pair_vjp<D, true, false>(w, qc, qm, cv, g, nullptr);
     }
-    if (WRAP || (x >= lo - l && x <= hi)) {  // cell 2 of the pair (c+l, c)
+    if (WRAP ? wa.on != 0 : (x >= lo - l && x <= hi)) {  // cell 2 of the pair (c+l, c)
       double lp[5], w[5], qp[6];
       int ii, jj, kk;
       nb(l, ii, jj, kk);
