@@ -36,6 +36,24 @@ int64_t cells(const int lo[3], const int hi[3]) {
   return int64_t(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
 }
 
+// Block map of a task list: TASK_BLK cells of one task per workgroup.
+template <class T>
+int upload_blocks(const std::vector<T>& tasks, TaskBlock** d, int* nblk) {
+  std::vector<TaskBlock> h;
+  for (int t = 0; t < int(tasks.size()); ++t) {
+    const int64_t n = cells(tasks[t].lo, tasks[t].hi);
+    if (n > int64_t(INT32_MAX) - TASK_BLK) return fail(AMR_ERR_ARG, "a ghost-fill region exceeds 2^31 cells");
+    for (int64_t f = 0; f < n; f += TASK_BLK) h.push_back(TaskBlock{t, int(f)});
+  }
+  if (h.size() > size_t(INT32_MAX)) return fail(AMR_ERR_ARG, "ghost-fill task list too long");
+  *nblk = int(h.size());
+  if (!h.empty()) {
+    HIPCHK(hipMalloc(d, sizeof(TaskBlock) * h.size()));
+    HIPCHK(hipMemcpy(*d, h.data(), sizeof(TaskBlock) * h.size(), hipMemcpyHostToDevice));
+  }
+  return AMR_OK;
+}
+
 // amr_fab_t for a compact array of `nc` components on [lo-ng, hi+ng]
 amr_fab_t compact_fab(const int lo[3], const int hi[3], int ng, double* base) {
   amr_fab_t f;
@@ -91,16 +109,24 @@ struct DescSet {
   CopyTask* d_copy = nullptr;
   int ncopy = 0;
   int64_t ncopy_cells = 0;
+  TaskBlock* d_copy_blk = nullptr;
+  int ncopy_blk = 0;
   BcTask* d_bc[3] = {nullptr, nullptr, nullptr};
   int nbc[3] = {0, 0, 0};
   int64_t nbc_cells[3] = {0, 0, 0};
+  TaskBlock* d_bc_blk[3] = {nullptr, nullptr, nullptr};
+  int nbc_blk[3] = {0, 0, 0};
   int64_t last_use = 0;
-  int wrap = -1;  // one array over the whole periodic domain (-1 unknown)
+  int wrap = -1;   // one array over the whole periodic domain (-1 unknown)
+  int level = -1;  // one array holding the boxes' bounding box (-1 unknown)
   ~DescSet() {
     if (d) (void)hipFree(d);
     if (d_copy) (void)hipFree(d_copy);
-    for (int i = 0; i < 3; ++i)
+    if (d_copy_blk) (void)hipFree(d_copy_blk);
+    for (int i = 0; i < 3; ++i) {
       if (d_bc[i]) (void)hipFree(d_bc[i]);
+      if (d_bc_blk[i]) (void)hipFree(d_bc_blk[i]);
+    }
   }
 };
 
@@ -170,7 +196,10 @@ struct amr_boxtab_s {
   double* d_result = nullptr;
   double* h_result = nullptr;
   // fused tiles per config
-  std::map<std::tuple<int, int, int>, std::unique_ptr<TileTable>> tiles;
+  std::map<std::tuple<int, int, int, int>, std::unique_ptr<TileTable>> tiles;  // (tx, ty, tz, level tiling)
+  Cover cover{};                      // level tiling (on = 0: not available)
+  std::vector<unsigned> cover_bits;   // host copy of cover.bits
+  unsigned* d_cover_bits = nullptr;
   std::vector<std::unique_ptr<GraphEntry>> graphs;
   // adjoint workspace: g (NCONS on grow 3) and e (16 on grow 1) per box
   double* adj_mem = nullptr;
@@ -192,6 +221,8 @@ struct amr_boxtab_s {
   CopyTask* d_uncov = nullptr;
   int nuncov = 0;
   int64_t nuncov_cells = 0;
+  TaskBlock* d_uncov_blk = nullptr;
+  int nuncov_blk = 0;
 };
 
 namespace {
@@ -346,6 +377,8 @@ int build_fill(amr_boxtab_s* bt, DescSet* ds) {
     HIPCHK(hipMalloc(&ds->d_copy, sizeof(CopyTask) * copies.size()));
     HIPCHK(hipMemcpy(ds->d_copy, copies.data(), sizeof(CopyTask) * copies.size(), hipMemcpyHostToDevice));
   }
+  int rc = upload_blocks(copies, &ds->d_copy_blk, &ds->ncopy_blk);
+  if (rc) return rc;
   for (int d = 0; d < 3; ++d) {
     std::vector<BcTask> tasks;
     int64_t nc = 0;
@@ -393,6 +426,7 @@ int build_fill(amr_boxtab_s* bt, DescSet* ds) {
       HIPCHK(hipMalloc(&ds->d_bc[d], sizeof(BcTask) * tasks.size()));
       HIPCHK(hipMemcpy(ds->d_bc[d], tasks.data(), sizeof(BcTask) * tasks.size(), hipMemcpyHostToDevice));
     }
+    if ((rc = upload_blocks(tasks, &ds->d_bc_blk[d], &ds->nbc_blk[d]))) return rc;
   }
   ds->fill_built = true;
   return AMR_OK;
@@ -400,8 +434,8 @@ int build_fill(amr_boxtab_s* bt, DescSet* ds) {
 
 int enqueue_fill(amr_boxtab_s* bt, DescSet* ds) {
   hipStream_t st = bt->ctx->stream;
-  HIPCHK(launch_fill_copy(ds->d_copy, ds->ncopy, ds->ncopy_cells, ds->d, st));
-  for (int d = 0; d < 3; ++d) HIPCHK(launch_fill_bc(ds->d_bc[d], ds->nbc[d], ds->nbc_cells[d], ds->d, st));
+  HIPCHK(launch_fill_copy(ds->d_copy, ds->d_copy_blk, ds->ncopy_blk, ds->d, st));
+  for (int d = 0; d < 3; ++d) HIPCHK(launch_fill_bc(ds->d_bc[d], ds->d_bc_blk[d], ds->nbc_blk[d], ds->d, st));
   return AMR_OK;
 }
 
@@ -500,14 +534,15 @@ int build_uncovered(amr_boxtab_s* bt) {
     HIPCHK(hipMalloc(&bt->d_uncov, sizeof(CopyTask) * tasks.size()));
     HIPCHK(hipMemcpy(bt->d_uncov, tasks.data(), sizeof(CopyTask) * tasks.size(), hipMemcpyHostToDevice));
   }
-  return AMR_OK;
+  return upload_blocks(tasks, &bt->d_uncov_blk, &bt->nuncov_blk);
 }
 
-// Uncovered ghost cells of `from` into `to` (no-op on levels that cover
-// their own ghost cells).
-int copy_uncovered(amr_boxtab_s* bt, const DescSet* from, const DescSet* to) {
+// Uncovered ghost cells of `from` into `to` and, if given, `to2` (no-op on
+// levels that cover their own ghost cells).
+int copy_uncovered(amr_boxtab_s* bt, const DescSet* from, const DescSet* to, const DescSet* to2 = nullptr) {
   if (bt->nuncov_cells == 0) return AMR_OK;
-  HIPCHK(launch_copy_regions(bt->d_uncov, bt->nuncov, bt->nuncov_cells, from->d, to->d, bt->ctx->stream));
+  HIPCHK(launch_copy_regions(bt->d_uncov, bt->d_uncov_blk, bt->nuncov_blk, from->d, to->d, to2 ? to2->d : nullptr,
+                             bt->ctx->stream));
   return AMR_OK;
 }
 
@@ -591,27 +626,74 @@ std::string scheme_key(const amr_scheme_t& s) {
 // Tile table for the fused kernel; blockIdx -> tile is permuted so that
 // consecutive tiles of a box land on one XCD (blocks are dealt round-robin
 // over the 8 XCDs, each with its own L2).
-int get_tiles(amr_boxtab_s* bt, const FusedConfig& cfg, TileTable** out) {
-  auto key = std::make_tuple(cfg.tx, cfg.ty, cfg.tz);
+bool cover_bit(const amr_boxtab_s* bt, int64_t bx, int64_t by, int64_t bz) {
+  const Cover& c = bt->cover;
+  if (bt->cover_bits.empty()) return true;
+  const int64_t bit = (bz * c.nb[1] + by) * c.nb[0] + bx;
+  return (bt->cover_bits[size_t(bit >> 5)] >> (bit & 31)) & 1u;
+}
+
+// Tiles of the fused kernel: box by box, or (level tiling) TX x TY columns on
+// a grid anchored at the bounding box's corner, split into runs of planes
+// that hold covered cells, each run cut into near-equal pieces of at most
+// tz planes.
+int get_tiles(amr_boxtab_s* bt, const FusedConfig& cfg, bool level, TileTable** out) {
+  auto key = std::make_tuple(cfg.tx, cfg.ty, cfg.tz, level ? 1 : 0);
   auto it = bt->tiles.find(key);
   if (it != bt->tiles.end()) {
     *out = it->second.get();
     return AMR_OK;
   }
   std::vector<TileDev> logical;
-  for (int b = 0; b < bt->bt.nbox; ++b) {
-    const BoxDev& B = bt->bt.h_boxes[b];
-    for (int k0 = B.lo[2]; k0 <= B.hi[2]; k0 += cfg.tz)
-      for (int j0 = B.lo[1]; j0 <= B.hi[1]; j0 += cfg.ty)
-        for (int i0 = B.lo[0]; i0 <= B.hi[0]; i0 += cfg.tx) {
-          TileDev t;
-          t.box = b;
-          t.i0 = i0;
-          t.j0 = j0;
-          t.k0 = k0;
-          t.nz = std::min(cfg.tz, B.hi[2] - k0 + 1);
-          logical.push_back(t);
+  if (level) {
+    const Cover& c = bt->cover;
+    const int sh = c.sh;
+    for (int j0 = c.lo[1]; j0 <= c.hi[1]; j0 += cfg.ty)
+      for (int i0 = c.lo[0]; i0 <= c.hi[0]; i0 += cfg.tx) {
+        const int bx0 = (i0 - c.lo[0]) >> sh, bx1 = std::min(i0 + cfg.tx - 1 - c.lo[0], c.hi[0] - c.lo[0]) >> sh;
+        const int by0 = (j0 - c.lo[1]) >> sh, by1 = std::min(j0 + cfg.ty - 1 - c.lo[1], c.hi[1] - c.lo[1]) >> sh;
+        auto plane_block_used = [&](int bz) {
+          for (int by = by0; by <= by1; ++by)
+            for (int bx = bx0; bx <= bx1; ++bx)
+              if (cover_bit(bt, bx, by, bz)) return true;
+          return false;
+        };
+        int bz = 0;
+        while (bz < c.nb[2]) {
+          if (!plane_block_used(bz)) {
+            ++bz;
+            continue;
+          }
+          int ez = bz;
+          while (ez + 1 < c.nb[2] && plane_block_used(ez + 1)) ++ez;
+          const int k0 = c.lo[2] + (bz << sh);
+          const int k1 = std::min(c.hi[2], c.lo[2] + ((ez + 1) << sh) - 1);
+          const int run = k1 - k0 + 1;
+          const int pieces = (run + cfg.tz - 1) / cfg.tz;
+          int k = k0;
+          for (int p = 0; p < pieces; ++p) {
+            const int nz = run / pieces + (p < run % pieces ? 1 : 0);
+            logical.push_back(TileDev{0, i0, j0, k, nz});
+            k += nz;
+          }
+          bz = ez + 1;
         }
+      }
+  } else {
+    for (int b = 0; b < bt->bt.nbox; ++b) {
+      const BoxDev& B = bt->bt.h_boxes[b];
+      for (int k0 = B.lo[2]; k0 <= B.hi[2]; k0 += cfg.tz)
+        for (int j0 = B.lo[1]; j0 <= B.hi[1]; j0 += cfg.ty)
+          for (int i0 = B.lo[0]; i0 <= B.hi[0]; i0 += cfg.tx) {
+            TileDev t;
+            t.box = b;
+            t.i0 = i0;
+            t.j0 = j0;
+            t.k0 = k0;
+            t.nz = std::min(cfg.tz, B.hi[2] - k0 + 1);
+            logical.push_back(t);
+          }
+    }
   }
   const int NXCD = 8;
   int n = int(logical.size());
@@ -651,7 +733,8 @@ FusedConfig default_config(const amr_boxtab_s* bt) {
 
 // The fused kernel takes any box (tiles overhanging a box in x or y are
 // masked); it skips only shapes whose tiles would be mostly idle.
-bool fused_fits(const amr_boxtab_s* bt, const FusedConfig& cfg) {
+bool fused_fits(const amr_boxtab_s* bt, const FusedConfig& cfg, bool level = false) {
+  if (level) return true;  // level tiles idle only along the union's boundary
   int64_t cells_ = 0, lanes = 0;
   for (auto& B : bt->bt.h_boxes) {
     const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
@@ -671,10 +754,88 @@ FusedConfig choose_config(const amr_boxtab_s* bt, const amr_scheme_t& sch) {
 }
 
 // Whether the fused stage kernel handles this scheme on this table.
-bool fused_path(const amr_boxtab_s* bt, const amr_scheme_t& sch, FusedConfig* out) {
+bool fused_path(const amr_boxtab_s* bt, const amr_scheme_t& sch, FusedConfig* out, bool level = false) {
   FusedConfig cfg = choose_config(bt, sch);
   if (out) *out = cfg;
-  return cfg.tx > 0 && fused_supported(sch, cfg.tx, cfg.ty) && fused_fits(bt, cfg);
+  return cfg.tx > 0 && fused_supported(sch, cfg.tx, cfg.ty) && fused_fits(bt, cfg, level);
+}
+
+// Level tiling needs the boxes' bounding box [lo, hi] and a bitmap of its
+// covered blocks; the block edge is the largest power of two (at most 2^6)
+// dividing every box boundary's offset from lo.  Built with the box table.
+int build_cover(amr_boxtab_s* bt) {
+  bt->cover = Cover{};
+  bt->cover_bits.clear();
+  if (bt->batch || bt->bt.nbox == 0) return AMR_OK;
+  const auto& B = bt->bt.h_boxes;
+  Cover c{};
+  for (int d = 0; d < 3; ++d) {
+    c.lo[d] = B[0].lo[d];
+    c.hi[d] = B[0].hi[d];
+  }
+  for (const BoxDev& b : B)
+    for (int d = 0; d < 3; ++d) {
+      c.lo[d] = std::min(c.lo[d], b.lo[d]);
+      c.hi[d] = std::max(c.hi[d], b.hi[d]);
+    }
+  unsigned g = 1u << 6;
+  for (const BoxDev& b : B)
+    for (int d = 0; d < 3; ++d) g |= unsigned(b.lo[d] - c.lo[d]) | unsigned(b.hi[d] + 1 - c.lo[d]);
+  c.sh = __builtin_ctz(g);
+  int64_t nbits = 1, bbox = 1, sum = 0;
+  for (int d = 0; d < 3; ++d) {
+    c.nb[d] = ((c.hi[d] - c.lo[d] + 1) + (1 << c.sh) - 1) >> c.sh;
+    nbits *= c.nb[d];
+    bbox *= c.hi[d] - c.lo[d] + 1;
+  }
+  for (const BoxDev& b : B) sum += cells(b.lo, b.hi);
+  if (nbits > (int64_t(1) << 28)) return AMR_OK;  // bitmap too large: box tiling only
+  c.on = 1;
+  c.bits = nullptr;
+  if (sum != bbox) {
+    bt->cover_bits.assign(size_t((nbits + 31) / 32), 0u);
+    for (const BoxDev& b : B)
+      for (int bz = (b.lo[2] - c.lo[2]) >> c.sh; bz <= (b.hi[2] - c.lo[2]) >> c.sh; ++bz)
+        for (int by = (b.lo[1] - c.lo[1]) >> c.sh; by <= (b.hi[1] - c.lo[1]) >> c.sh; ++by)
+          for (int bx = (b.lo[0] - c.lo[0]) >> c.sh; bx <= (b.hi[0] - c.lo[0]) >> c.sh; ++bx) {
+            const int64_t bit = (int64_t(bz) * c.nb[1] + by) * c.nb[0] + bx;
+            bt->cover_bits[size_t(bit >> 5)] |= 1u << (bit & 31);
+          }
+    HIPCHK(hipMalloc(&bt->d_cover_bits, sizeof(unsigned) * bt->cover_bits.size()));
+    HIPCHK(hipMemcpy(bt->d_cover_bits, bt->cover_bits.data(), sizeof(unsigned) * bt->cover_bits.size(),
+                     hipMemcpyHostToDevice));
+    c.bits = bt->d_cover_bits;
+  }
+  bt->cover = c;
+  return AMR_OK;
+}
+
+// Every box of `ds` is a window of one array (same mapping) whose address
+// range also holds the ghosted bounding box of the boxes.
+bool level_ok(const amr_boxtab_s* bt, DescSet* ds) {
+  if (ds->level >= 0) return ds->level == 1;
+  ds->level = 0;
+  if (!bt->cover.on) return false;
+  const int nb = bt->bt.nbox;
+  const Mapping m0 = mapping_of(ds->host[0]);
+  intptr_t lo = INTPTR_MAX, hi = 0;
+  for (int b = 0; b < nb; ++b) {
+    const amr_fab_t& f = ds->host[b];
+    if (!(mapping_of(f) == m0)) return false;
+    int64_t nx = f.hi[0] - f.lo[0] + 1 + 2 * f.ng, ny = f.hi[1] - f.lo[1] + 1 + 2 * f.ng;
+    int64_t nz = f.hi[2] - f.lo[2] + 1 + 2 * f.ng;
+    int64_t last = (nx - 1) + (ny - 1) * f.jstride + (nz - 1) * f.kstride + (NCONS - 1) * f.nstride;
+    lo = std::min(lo, intptr_t(f.data));
+    hi = std::max(hi, intptr_t(f.data + last + 1));
+  }
+  const Cover& c = bt->cover;
+  const intptr_t a0 = m0.origin + 8 * intptr_t((c.lo[0] - NGHOST) + (c.lo[1] - NGHOST) * m0.js +
+                                               (c.lo[2] - NGHOST) * m0.ks);
+  const intptr_t a1 = m0.origin + 8 * intptr_t((c.hi[0] + NGHOST) + (c.hi[1] + NGHOST) * m0.js +
+                                               (c.hi[2] + NGHOST) * m0.ks + (NCONS - 1) * m0.ns) + 8;
+  if (a0 < lo || a1 > hi) return false;
+  ds->level = 1;
+  return true;
 }
 
 // Whether `ds` is one array over the whole periodic domain (every box a
@@ -713,6 +874,16 @@ WrapArgs wrap_args(const amr_boxtab_s* bt, DescSet* ds) {
   return wa;
 }
 
+// Level tiling for a fused stage: every array of the stage is one array over
+// the level, and the level is not a periodic domain-wide array (that case
+// reads periodic images through box tiles).
+bool use_level_tiles(amr_boxtab_s* bt, const amr_scheme_t& sch, DescSet* ds1, DescSet* dsin, DescSet* dout) {
+  if (!bt->cover.on || wrap_ok(bt, dsin)) return false;
+  if (!level_ok(bt, dsin) || !level_ok(bt, dout) || (ds1 && !level_ok(bt, ds1))) return false;
+  if (sch.rhs_source && !(bt->src && level_ok(bt, bt->src.get()))) return false;
+  return true;
+}
+
 int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageArgs& sa_in,
              DescSet* ds1, DescSet* dsin, DescSet* dout) {
   StageArgs sa = sa_in;
@@ -724,12 +895,14 @@ int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sc
   hipStream_t st = bt->ctx->stream;
   const Fab* s1 = ds1 ? ds1->d : nullptr;
   FusedConfig cfg;
-  if (fused_path(bt, sch, &cfg)) {
+  const bool level = use_level_tiles(bt, sch, ds1, dsin, dout);
+  if (fused_path(bt, sch, &cfg, level)) {
     TileTable* tt = nullptr;
-    int rc = get_tiles(bt, cfg, &tt);
+    int rc = get_tiles(bt, cfg, level, &tt);
     if (rc) return rc;
     const WrapArgs wa = wrap_args(bt, dsin);
-    HIPCHK(launch_fused_stage(sch, cfg, wa, tt->d, tt->n, bt->bt.d_boxes, c, sa, s1, dsin->d, dout->d, st));
+    const Cover cov = level ? bt->cover : Cover{};
+    HIPCHK(launch_fused_stage(sch, cfg, wa, cov, tt->d, tt->n, bt->bt.d_boxes, c, sa, s1, dsin->d, dout->d, st));
     return AMR_OK;
   }
   int rc = ensure_unfused(bt, sch);
@@ -821,8 +994,7 @@ int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t
   for (int w = 0; w < 2; ++w)
     if ((rc = build_fill(bt, dw[w]))) return rc;
   // stage inputs keep the caller's values in ghost cells no box covers
-  for (int w = 0; w < 2; ++w)
-    if ((rc = copy_uncovered(bt, dst, dw[w]))) return rc;
+  if ((rc = copy_uncovered(bt, dst, dw[0], dw[1]))) return rc;
   for (int s = 0; s < nst; ++s) {
     if ((rc = stage_coeffs(sch.order_rk, sch.stages_rk, dt, s, &nst, &sc))) return rc;
     DescSet* sin = s == 0 ? dst : dw[(s - 1) % 2];
@@ -1218,6 +1390,7 @@ int finish_boxtab(amr_boxtab_s* bt) {
   if (e == hipSuccess) e = prepare_block_maps(bt->bt);
   if (e != hipSuccess) return fail(AMR_ERR_HIP, std::string("box table allocation: ") + hipGetErrorString(e));
   rc = build_uncovered(bt);
+  if (rc == AMR_OK) rc = build_cover(bt);
   return rc;
 }
 }  // namespace
@@ -1233,6 +1406,8 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   for (int w = 0; w < 2; ++w)
     if (bt->ws_mem[w]) (void)hipFree(bt->ws_mem[w]);
   if (bt->d_uncov) (void)hipFree(bt->d_uncov);
+  if (bt->d_uncov_blk) (void)hipFree(bt->d_uncov_blk);
+  if (bt->d_cover_bits) (void)hipFree(bt->d_cover_bits);
   bt->run.reset();
   bt->src.reset();
   bt->adj_g.reset();
@@ -1318,9 +1493,12 @@ int amr_advance_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_sche
   int rc;
   if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls))) return rc;
   int nb = bt->bt.nbox;
+  // the tiling in force is part of the key: a graph records its launches
+  const FusedConfig kcfg = choose_config(bt, *sch);
   std::vector<char> key(sizeof(amr_fab_t) * nb + sizeof(amr_closures_t) + sizeof(amr_scheme_t) + sizeof(double) +
-                        sizeof(int));
+                        sizeof(int) + sizeof(FusedConfig));
   char* p = key.data();
+  std::memcpy(p + key.size() - sizeof(FusedConfig), &kcfg, sizeof(FusedConfig));
   std::memcpy(p, state, sizeof(amr_fab_t) * nb);
   p += sizeof(amr_fab_t) * nb;
   std::memcpy(p, cls, sizeof(amr_closures_t));
@@ -1345,12 +1523,13 @@ int amr_advance_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_sche
       if ((rc = get_desc(bt, bt->ws_fabs[w].data(), NGHOST, &dw)) || (rc = build_fill(bt, dw))) return rc;
     }
     FusedConfig cfg = choose_config(bt, *sch);
-    if (cfg.tx > 0 && fused_supported(*sch, cfg.tx, cfg.ty) && fused_fits(bt, cfg)) {
-      TileTable* tt = nullptr;
-      if ((rc = get_tiles(bt, cfg, &tt))) return rc;
-    } else if ((rc = ensure_unfused(bt, *sch))) {
-      return rc;
-    }
+    // (tables for box tiles, level tiles and the pass-structured kernels, as
+    // the stage arrays may choose)
+    const bool fused = cfg.tx > 0 && fused_supported(*sch, cfg.tx, cfg.ty);
+    TileTable* tt = nullptr;
+    if (fused && fused_fits(bt, cfg) && (rc = get_tiles(bt, cfg, false, &tt))) return rc;
+    if (fused && bt->cover.on && (rc = get_tiles(bt, cfg, true, &tt))) return rc;
+    if (!(fused && fused_fits(bt, cfg)) && (rc = ensure_unfused(bt, *sch))) return rc;
     HIPCHK(hipStreamSynchronize(st));
     auto g = std::make_unique<GraphEntry>();
     g->key = key;
@@ -1399,8 +1578,14 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
   if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls))) return rc;
   std::string key = scheme_key(*sch);
   bt->ctx->tuned.erase(key);  // tuning replaces an explicit setting
+  if ((rc = ensure_ws(bt, state))) return rc;
+  DescSet *din = nullptr, *dout = nullptr;
+  if ((rc = get_desc(bt, state, NGHOST, &din))) return rc;
+  if ((rc = get_desc(bt, bt->ws_fabs[0].data(), 0, &dout))) return rc;
+  const bool level = use_level_tiles(bt, *sch, nullptr, din, dout);
   int maxz = 0;
   for (auto& B : bt->bt.h_boxes) maxz = std::max(maxz, B.hi[2] - B.lo[2] + 1);
+  if (level) maxz = bt->cover.hi[2] - bt->cover.lo[2] + 1;  // level tiles run across boxes
   std::vector<FusedConfig> cand;
   const int shapes[3][2] = {{16, 16}, {32, 8}, {8, 8}};
   const int tzs[4] = {16, 32, 64, 128};
@@ -1408,7 +1593,7 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
     for (int tz : tzs) {
       FusedConfig c{sh[0], sh[1], tz};
       if (tz > 2 * maxz && tz != 16) continue;
-      if (fused_supported(*sch, c.tx, c.ty) && fused_fits(bt, c)) cand.push_back(c);
+      if (fused_supported(*sch, c.tx, c.ty) && fused_fits(bt, c, level)) cand.push_back(c);
     }
   if (cand.empty()) {
     bt->tuned[key] = FusedConfig{-1, 0, 0};
@@ -1416,10 +1601,6 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
     tiling[1] = tiling[2] = 0;
     return AMR_OK;
   }
-  if ((rc = ensure_ws(bt, state))) return rc;
-  DescSet *din = nullptr, *dout = nullptr;
-  if ((rc = get_desc(bt, state, NGHOST, &din))) return rc;
-  if ((rc = get_desc(bt, bt->ws_fabs[0].data(), 0, &dout))) return rc;
   hipStream_t st = bt->ctx->stream;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));

@@ -8,51 +8,42 @@ namespace amr {
 
 namespace {
 
-constexpr int FBLK = 256;
+constexpr int FBLK = TASK_BLK;
 
+// Cell (i,j,k) of task region [lo,hi] for lane `tid` of block `tb`; false
+// past the region's last cell.
 template <class T>
-__device__ __forceinline__ int find_task(const T* tasks, int ntask, int64_t t) {
-  int lo = 0, hi = ntask - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (tasks[mid].start <= t) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
+__device__ __forceinline__ bool task_cell(const T& t, const TaskBlock& tb, int tid, int c[3]) {
+  const int nx = t.hi[0] - t.lo[0] + 1, ny = t.hi[1] - t.lo[1] + 1, nz = t.hi[2] - t.lo[2] + 1;
+  const int r = tb.first + tid;
+  if (r >= nx * ny * nz) return false;
+  const int r2 = r / nx;
+  c[0] = t.lo[0] + (r - r2 * nx);
+  c[1] = t.lo[1] + r2 % ny;
+  c[2] = t.lo[2] + r2 / ny;
+  return true;
 }
 
 // FillBoundary + periodic images: copy valid cells of `src` into ghost
 // cells of `dst` (AMReX FillBoundary semantics restated).
-__global__ void __launch_bounds__(FBLK) k_fill_copy(const CopyTask* tasks, int ntask, int64_t ncell,
+__global__ void __launch_bounds__(FBLK) k_fill_copy(const CopyTask* tasks, const TaskBlock* blocks,
                                                     const Fab* fabs) {
-  int64_t t = int64_t(blockIdx.x) * FBLK + threadIdx.x;
-  if (t >= ncell) return;
-  const CopyTask& T = tasks[find_task(tasks, ntask, t)];
-  int64_t r = t - T.start;
-  int nx = T.hi[0] - T.lo[0] + 1, ny = T.hi[1] - T.lo[1] + 1;
-  int i = T.lo[0] + int(r % nx);
-  int64_t r2 = r / nx;
-  int j = T.lo[1] + int(r2 % ny);
-  int k = T.lo[2] + int(r2 / ny);
+  const TaskBlock tb = blocks[blockIdx.x];
+  const CopyTask& T = tasks[tb.task];
+  int c[3];
+  if (!task_cell(T, tb, threadIdx.x, c)) return;
   const Fab d = fabs[T.dst], s = fabs[T.src];
 #pragma unroll
-  for (int n = 0; n < NCONS; ++n) d(i, j, k, n) = s(i - T.sh[0], j - T.sh[1], k - T.sh[2], n);
+  for (int n = 0; n < NCONS; ++n) d(c[0], c[1], c[2], n) = s(c[0] - T.sh[0], c[1] - T.sh[1], c[2] - T.sh[2], n);
 }
 
 // Physical boundaries (src/set/CNS_setup.cpp:18-31 BC map; AMReX foextrap
 // and reflect_even/odd): ghost value from the boundary cell or its mirror.
-__global__ void __launch_bounds__(FBLK) k_fill_bc(const BcTask* tasks, int ntask, int64_t ncell,
-                                                  const Fab* fabs) {
-  int64_t t = int64_t(blockIdx.x) * FBLK + threadIdx.x;
-  if (t >= ncell) return;
-  const BcTask& T = tasks[find_task(tasks, ntask, t)];
-  int64_t r = t - T.start;
-  int nx = T.hi[0] - T.lo[0] + 1, ny = T.hi[1] - T.lo[1] + 1;
+__global__ void __launch_bounds__(FBLK) k_fill_bc(const BcTask* tasks, const TaskBlock* blocks, const Fab* fabs) {
+  const TaskBlock tb = blocks[blockIdx.x];
+  const BcTask& T = tasks[tb.task];
   int c[3];
-  c[0] = T.lo[0] + int(r % nx);
-  int64_t r2 = r / nx;
-  c[1] = T.lo[1] + int(r2 % ny);
-  c[2] = T.lo[2] + int(r2 / ny);
+  if (!task_cell(T, tb, threadIdx.x, c)) return;
   const Fab f = fabs[T.box];
   const int cd = c[T.dir];
   for (int n = 0; n < NCONS; ++n) {
@@ -66,21 +57,24 @@ __global__ void __launch_bounds__(FBLK) k_fill_bc(const BcTask* tasks, int ntask
   }
 }
 
-// Region copies between two FAB sets: dst[T.dst](c) = src[T.src](c - sh).
-__global__ void __launch_bounds__(FBLK) k_copy_regions(const CopyTask* tasks, int ntask, int64_t ncell,
-                                                       const Fab* src, const Fab* dst) {
-  int64_t t = int64_t(blockIdx.x) * FBLK + threadIdx.x;
-  if (t >= ncell) return;
-  const CopyTask& T = tasks[find_task(tasks, ntask, t)];
-  int64_t r = t - T.start;
-  int nx = T.hi[0] - T.lo[0] + 1, ny = T.hi[1] - T.lo[1] + 1;
-  int i = T.lo[0] + int(r % nx);
-  int64_t r2 = r / nx;
-  int j = T.lo[1] + int(r2 % ny);
-  int k = T.lo[2] + int(r2 / ny);
-  const Fab d = dst[T.dst], s = src[T.src];
+// Region copies between FAB sets: dst0[T.dst](c) = dst1[T.dst](c) = src[T.src](c - sh).
+__global__ void __launch_bounds__(FBLK) k_copy_regions(const CopyTask* tasks, const TaskBlock* blocks,
+                                                       const Fab* src, const Fab* dst0, const Fab* dst1) {
+  const TaskBlock tb = blocks[blockIdx.x];
+  const CopyTask& T = tasks[tb.task];
+  int c[3];
+  if (!task_cell(T, tb, threadIdx.x, c)) return;
+  const Fab s = src[T.src], d0 = dst0[T.dst];
+  double v[NCONS];
 #pragma unroll
-  for (int n = 0; n < NCONS; ++n) d(i, j, k, n) = s(i - T.sh[0], j - T.sh[1], k - T.sh[2], n);
+  for (int n = 0; n < NCONS; ++n) v[n] = s(c[0] - T.sh[0], c[1] - T.sh[1], c[2] - T.sh[2], n);
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) d0(c[0], c[1], c[2], n) = v[n];
+  if (dst1 != nullptr) {
+    const Fab d1 = dst1[T.dst];
+#pragma unroll
+    for (int n = 0; n < NCONS; ++n) d1(c[0], c[1], c[2], n) = v[n];
+  }
 }
 
 __global__ void __launch_bounds__(FBLK) k_copy_valid(const BoxDev* boxes, const Fab* src, const Fab* dst,
@@ -257,18 +251,16 @@ __global__ void __launch_bounds__(RBLK) k_sum_final(const double* partial, int n
 
 }  // namespace
 
-hipError_t launch_fill_copy(const CopyTask* tasks, int ntask, int64_t ncell, const Fab* fabs,
+hipError_t launch_fill_copy(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* fabs,
                             hipStream_t st) {
-  if (ntask == 0 || ncell == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_copy, dim3(unsigned((ncell + FBLK - 1) / FBLK)), dim3(FBLK), 0, st, tasks, ntask,
-                     ncell, fabs);
+  if (nblk == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_copy, dim3(unsigned(nblk)), dim3(FBLK), 0, st, tasks, blocks, fabs);
   return hipGetLastError();
 }
 
-hipError_t launch_fill_bc(const BcTask* tasks, int ntask, int64_t ncell, const Fab* fabs, hipStream_t st) {
-  if (ntask == 0 || ncell == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_bc, dim3(unsigned((ncell + FBLK - 1) / FBLK)), dim3(FBLK), 0, st, tasks, ntask,
-                     ncell, fabs);
+hipError_t launch_fill_bc(const BcTask* tasks, const TaskBlock* blocks, int nblk, const Fab* fabs, hipStream_t st) {
+  if (nblk == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_bc, dim3(unsigned(nblk)), dim3(FBLK), 0, st, tasks, blocks, fabs);
   return hipGetLastError();
 }
 
@@ -316,11 +308,10 @@ hipError_t launch_estdt_boxes(const BoxTable& bt, const Fab* st, const Closures&
   return hipGetLastError();
 }
 
-hipError_t launch_copy_regions(const CopyTask* tasks, int ntask, int64_t ncell, const Fab* src, const Fab* dst,
-                               hipStream_t st) {
-  if (ncell <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_copy_regions, dim3(unsigned((ncell + FBLK - 1) / FBLK)), dim3(FBLK), 0, st, tasks, ntask,
-                     ncell, src, dst);
+hipError_t launch_copy_regions(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* src,
+                               const Fab* dst0, const Fab* dst1, hipStream_t st) {
+  if (nblk == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_regions, dim3(unsigned(nblk)), dim3(FBLK), 0, st, tasks, blocks, src, dst0, dst1);
   return hipGetLastError();
 }
 

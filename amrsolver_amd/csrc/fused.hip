@@ -27,6 +27,7 @@ namespace {
 // instructions (flat ones would share the LDS wait counter and serialise
 // the plane prefetch against every LDS wait).
 typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) const unsigned guint;
 __device__ __forceinline__ gdouble* gptr(double* p) { return (gdouble*)p; }
 
 struct VDer {
@@ -119,8 +120,8 @@ __device__ __forceinline__ void zterms(const double q1[NPRIM], const double q2[N
 template <int H, bool VISC, int TX, int TY>
 __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ tiles, const BoxDev* __restrict__ boxes,
                                                      Closures c, double co0, double co1, double co2, StageArgs sa,
-                                                     WrapArgs wa, const Fab* __restrict__ s1f, const Fab* __restrict__ sinf,
-                                                     const Fab* __restrict__ outf) {
+                                                     WrapArgs wa, Cover cov, const Fab* __restrict__ s1f,
+                                                     const Fab* __restrict__ sinf, const Fab* __restrict__ outf) {
   using C = Cfg<H, VISC, TX, TY>;
   constexpr int RX = C::RX, HX = C::HX, PL = C::PL, NT = C::NT;
   __shared__ double s_q[2][NPRIM * PL];
@@ -141,10 +142,15 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
   const int tid = threadIdx.x;
   const int ti = tid % TX, tj = tid / TX;
   const int gi = T.i0 + ti, gj = T.j0 + tj;
-  // a tile may overhang the box in x and y: loads are clamped to the box's
-  // ghosted extent and only cells of the box are written
-  const int xcap = boxes[T.box].hi[0] + NGHOST, ycap = boxes[T.box].hi[1] + NGHOST;
-  const bool own_in = gi <= boxes[T.box].hi[0] && gj <= boxes[T.box].hi[1];
+  // a tile may overhang its box (or, with level tiling, the boxes' bounding
+  // box) in x and y: loads are clamped to the ghosted extent and only cells
+  // inside are written; with level tiling only covered cells
+  const int xhi = cov.on ? cov.hi[0] : boxes[T.box].hi[0], yhi = cov.on ? cov.hi[1] : boxes[T.box].hi[1];
+  const int xcap = xhi + NGHOST, ycap = yhi + NGHOST;
+  const bool own_in = gi <= xhi && gj <= yhi;
+  guint* cbits = (guint*)cov.bits;
+  const int cplane = cov.nb[0] * cov.nb[1];
+  const int ccol = own_in && cbits ? ((gj - cov.lo[1]) >> cov.sh) * cov.nb[0] + ((gi - cov.lo[0]) >> cov.sh) : 0;
   const int k0 = T.k0, nz = T.nz;
   const int kfirst = k0 - C::KB;
   const int klast = k0 + nz + 1;
@@ -241,7 +247,11 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
       if (pn + 1 <= klast) load_plane(pn + 1);
     }
     // epilogue operands of the plane finalised in this iteration
-    const bool fin = cpl >= k0 + 1 && own_in;
+    bool fin = cpl >= k0 + 1 && own_in;
+    if (fin && cbits) {
+      const int bit = ((cpl - 1 - cov.lo[2]) >> cov.sh) * cplane + ccol;
+      fin = (cbits[bit >> 5] >> (bit & 31)) & 1u;
+    }
     double sv[5], s1v[5];
     if (fin) {
       const gdouble* pb = sbase + int64_t(cpl - 1) * S.ks;
@@ -483,22 +493,23 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
 
 template <int H, bool VISC, int TX, int TY>
 void launch_t(const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c, const double co[3],
-              const StageArgs& sa, const WrapArgs& wa, const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
+              const StageArgs& sa, const WrapArgs& wa, const Cover& cov, const Fab* s1, const Fab* sin,
+              const Fab* out, hipStream_t st) {
   hipLaunchKernelGGL((k_fused<H, VISC, TX, TY>), dim3(ntiles), dim3(TX * TY), 0, st, tiles, boxes, c, co[0], co[1],
-                     co[2], sa, wa, s1, sin, out);
+                     co[2], sa, wa, cov, s1, sin, out);
 }
 
 template <int TX, int TY>
 bool launch_shape(const amr_scheme_t& s, const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c,
-                  const double co[3], const StageArgs& sa, const WrapArgs& wa, const Fab* s1, const Fab* sin,
-                  const Fab* out, hipStream_t st) {
+                  const double co[3], const StageArgs& sa, const WrapArgs& wa, const Cover& cov, const Fab* s1,
+                  const Fab* sin, const Fab* out, hipStream_t st) {
   const bool v = s.rhs_visc != 0;
   if (s.order_keep == 4) {
-    if (v) launch_t<2, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, s1, sin, out, st);
-    else launch_t<2, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, s1, sin, out, st);
+    if (v) launch_t<2, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, s1, sin, out, st);
+    else launch_t<2, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, s1, sin, out, st);
   } else if (s.order_keep == 2) {
-    if (v) launch_t<1, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, s1, sin, out, st);
-    else launch_t<1, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, s1, sin, out, st);
+    if (v) launch_t<1, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, s1, sin, out, st);
+    else launch_t<1, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, s1, sin, out, st);
   } else {
     return false;
   }
@@ -514,16 +525,15 @@ bool fused_supported(const amr_scheme_t& s, int tx, int ty) {
   return (tx == 16 && ty == 16) || (tx == 8 && ty == 8) || (tx == 32 && ty == 8);
 }
 
-hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const WrapArgs& wa, const TileDev* tiles,
-                              int ntiles,
-                              const BoxDev* boxes, const Closures& c, const StageArgs& sa, const Fab* s1,
-                              const Fab* sin, const Fab* out, hipStream_t st) {
+hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const WrapArgs& wa, const Cover& cov,
+                              const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c,
+                              const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
   double co[3];
   keep_coefficients(s.order_keep, co);
   bool ok = false;
-  if (cfg.tx == 16 && cfg.ty == 16) ok = launch_shape<16, 16>(s, tiles, ntiles, boxes, c, co, sa, wa, s1, sin, out, st);
-  else if (cfg.tx == 8 && cfg.ty == 8) ok = launch_shape<8, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, s1, sin, out, st);
-  else if (cfg.tx == 32 && cfg.ty == 8) ok = launch_shape<32, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, s1, sin, out, st);
+  if (cfg.tx == 16 && cfg.ty == 16) ok = launch_shape<16, 16>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, s1, sin, out, st);
+  else if (cfg.tx == 8 && cfg.ty == 8) ok = launch_shape<8, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, s1, sin, out, st);
+  else if (cfg.tx == 32 && cfg.ty == 8) ok = launch_shape<32, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, s1, sin, out, st);
   if (!ok) return hipErrorInvalidValue;
   return hipGetLastError();
 }

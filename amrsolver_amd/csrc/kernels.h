@@ -54,12 +54,32 @@ struct WrapArgs {
   int lo[3], len[3];
 };
 
+// Level tiling: when every state array of a stage is one array over the
+// level (boxes are windows), tiles are laid over the union of the boxes
+// instead of box by box, so they cross box boundaries.  [lo, hi] is the
+// bounding box of the boxes; bits (null when the bounding box is fully
+// covered) marks covered blocks of 2^sh cells a side, x fastest, nb blocks
+// per direction.  Only covered cells are written.
+struct Cover {
+  int on;
+  int lo[3], hi[3];
+  int nb[3];
+  int sh;
+  const unsigned* bits;
+};
+
 // Ghost-fill tasks.  Copy: dst(i,j,k) = src(i - sh) for (i,j,k) in [lo,hi].
 struct CopyTask {
   int dst, src;
   int lo[3], hi[3];
   int sh[3];
   int64_t start;  // exclusive prefix of cell counts
+};
+// Block map of a task list: workgroup w copies cells [first, first + TASK_BLK)
+// of task `task` (cells numbered x fastest), so no lane searches the list.
+constexpr int TASK_BLK = 256;
+struct TaskBlock {
+  int task, first;
 };
 // Physical BC: for cells [lo,hi] of box `box` in direction `dir`, side
 // `side` (0 lo, 1 hi) with domain boundary index `bnd`; per-component type
@@ -126,14 +146,16 @@ hipError_t launch_adj_stage(const BoxTable& bt, const UnfusedWork& w, const AdjW
 hipError_t launch_objective(const BoxTable& bt, int kind, const Fab* st, const Fab* wts, const Fab* lam,
                             double* partial, double* result, hipStream_t s);
 
-hipError_t launch_fill_copy(const CopyTask* tasks, int ntask, int64_t ncell, const Fab* fabs,
+// Ghost fill over block-mapped task lists (nblk workgroups).
+hipError_t launch_fill_copy(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* fabs,
                             hipStream_t st);
-hipError_t launch_fill_bc(const BcTask* tasks, int ntask, int64_t ncell, const Fab* fabs,
+hipError_t launch_fill_bc(const BcTask* tasks, const TaskBlock* blocks, int nblk, const Fab* fabs,
                           hipStream_t st);
 
-// dst[T.dst](c) = src[T.src](c - sh) over the task regions (two FAB sets).
-hipError_t launch_copy_regions(const CopyTask* tasks, int ntask, int64_t ncell, const Fab* src, const Fab* dst,
-                               hipStream_t st);
+// dst[T.dst](c) = src[T.src](c - sh) over the task regions, into one or two
+// FAB sets (dst1 may be null): the source is read once for both.
+hipError_t launch_copy_regions(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* src,
+                               const Fab* dst0, const Fab* dst1, hipStream_t st);
 
 // dst = src on the valid cells of every box.
 hipError_t launch_copy_valid(const BoxTable& bt, const Fab* src, const Fab* dst, const double* box_dt,
@@ -161,8 +183,8 @@ struct FusedConfig {
   int tx, ty, tz;  // tile extents
 };
 bool fused_supported(const amr_scheme_t& s, int tx, int ty);
-hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const WrapArgs& wa, const TileDev* tiles,
-                              int ntiles, const BoxDev* boxes, const Closures& c, const StageArgs& sa,
-                              const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st);
+hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const WrapArgs& wa, const Cover& cov,
+                              const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c,
+                              const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st);
 
 }  // namespace amr
