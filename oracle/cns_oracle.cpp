@@ -32,6 +32,11 @@
  *   SSPRK variants      linear ODE -> stability polynomials; conservation
  *   ghost fill          closed forms (periodic image, foextrap, reflections)
  *   estdt               closed form on uniform states (incl. UMY quirk)
+ *   tangent (Dual)      the KEEP/viscous RHS, ghost fill and SSPRK sequence
+ *                       are templates over the scalar type; the dual-number
+ *                       instantiation's value part equals the double path
+ *                       bit for bit and its derivative part matches central
+ *                       differences of the double path
  */
 #include "cns_oracle.h"
 
@@ -41,6 +46,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -59,18 +65,51 @@ int fail(const std::string& m) {
   return -1;
 }
 
+// Forward-mode dual number (value, derivative along one direction).  Every
+// value operation is the double operation of the plain path, in the same
+// order, so the value part of a Dual evaluation equals the double result.
+struct Dual {
+  double v = 0.0, d = 0.0;
+  Dual() = default;
+  Dual(double x) : v(x), d(0.0) {}  // NOLINT: constants enter as (x, 0)
+  Dual(double x, double y) : v(x), d(y) {}
+  Dual& operator+=(const Dual& o) {
+    v += o.v;
+    d += o.d;
+    return *this;
+  }
+};
+inline Dual operator+(const Dual& a, const Dual& b) { return Dual(a.v + b.v, a.d + b.d); }
+inline Dual operator-(const Dual& a, const Dual& b) { return Dual(a.v - b.v, a.d - b.d); }
+inline Dual operator-(const Dual& a) { return Dual(-a.v, -a.d); }
+inline Dual operator*(const Dual& a, const Dual& b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+inline Dual operator/(const Dual& a, const Dual& b) {
+  const double q = a.v / b.v;
+  return Dual(q, (a.d - q * b.d) / b.v);
+}
+inline Dual sqrt(const Dual& a) {
+  const double r = std::sqrt(a.v);
+  return Dual(r, a.d / (2.0 * r));
+}
+// std::max(lo, x) as the plain path evaluates it: x if lo < x, else lo
+inline double clamp_low(double lo, double x) { return std::max(lo, x); }
+inline Dual clamp_low(double lo, const Dual& x) { return lo < x.v ? x : Dual(lo); }
+
 // Fortran-order view, component outermost (AMReX Array4 convention).
-struct A4 {
-  double* p = nullptr;
+template <class R>
+struct A4T {
+  R* p = nullptr;
   int b[3] = {0, 0, 0};  // lower corner of the allocated box
   long js = 0, ks = 0, ns = 0;
-  inline double& operator()(int i, int j, int k, int n) const {
+  inline R& operator()(int i, int j, int k, int n) const {
     return p[(i - b[0]) + (j - b[1]) * js + (k - b[2]) * ks + n * ns];
   }
 };
+using A4 = A4T<double>;
 
-A4 view(double* p, const int lo[3], const int hi[3]) {
-  A4 a;
+template <class R>
+A4T<R> view(R* p, const int lo[3], const int hi[3]) {
+  A4T<R> a;
   a.p = p;
   for (int d = 0; d < 3; ++d) a.b[d] = lo[d];
   long nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
@@ -124,28 +163,33 @@ void for_box(const Box& b, F&& f) {
 // src/cls/Closures.h:103-120 (viscosity), :130-166 (conductivity).  Law 2 is
 // the constant-Prandtl closure k = mu*cp/Pr that Closures.h:145-153 sketches;
 // it is the closure used for the TGV and channel workloads.
-double visc(const orc_closures& c, double T) {
+template <class R>
+R visc(const orc_closures& c, R T) {
+  using std::sqrt;
   if (c.visc_law == 0) return c.visc_ref;                        // :108-109
-  return c.visc_ref * T * std::sqrt(T) / (c.Tvisc_ref + T);      // :118-119
+  return c.visc_ref * T * sqrt(T) / (c.Tvisc_ref + T);           // :118-119
 }
-double cond(const orc_closures& c, double T) {
+template <class R>
+R cond(const orc_closures& c, R T) {
+  using std::sqrt;
   if (c.cond_law == 0) return c.cond_ref;                        // :135-136
-  if (c.cond_law == 1) return c.cond_ref * T * std::sqrt(T) / (c.Tcond_ref + T);  // :161
+  if (c.cond_law == 1) return c.cond_ref * T * sqrt(T) / (c.Tcond_ref + T);  // :161
   return visc(c, T) * c.cp / c.Pr;
 }
 
 // ---------------------------------------------------------------- cons2prim
 // src/rhs/CNS_hydro_K.h:15-34
-void cons2prim(int i, int j, int k, const A4& u, const A4& q, const orc_closures& c) {
-  double rho = u(i, j, k, URHO);
-  rho = std::max(1e-40, rho);
-  double rhoinv = 1.0 / rho;
-  double ux = u(i, j, k, UMX) * rhoinv;
-  double uy = u(i, j, k, UMY) * rhoinv;
-  double uz = u(i, j, k, UMZ) * rhoinv;
-  double rhoke = 0.5 * rho * (ux * ux + uy * uy + uz * uz);
-  double rhoei = (u(i, j, k, UET) - rhoke);
-  double p = (c.gamma - 1.0) * rhoei;
+template <class R>
+void cons2prim(int i, int j, int k, const A4T<R>& u, const A4T<R>& q, const orc_closures& c) {
+  R rho = u(i, j, k, URHO);
+  rho = clamp_low(1e-40, rho);
+  R rhoinv = 1.0 / rho;
+  R ux = u(i, j, k, UMX) * rhoinv;
+  R uy = u(i, j, k, UMY) * rhoinv;
+  R uz = u(i, j, k, UMZ) * rhoinv;
+  R rhoke = 0.5 * rho * (ux * ux + uy * uy + uz * uz);
+  R rhoei = (u(i, j, k, UET) - rhoke);
+  R p = (c.gamma - 1.0) * rhoei;
   q(i, j, k, QRHO) = rho;
   q(i, j, k, QU) = ux;
   q(i, j, k, QV) = uy;
@@ -156,10 +200,14 @@ void cons2prim(int i, int j, int k, const A4& u, const A4& q, const orc_closures
 
 // ---------------------------------------------------------------- KEEP
 // src/rhs/CentralKEEP.h:19-36 split forms
-inline double fDiv(double f, double fl) { return 0.5 * (f + fl); }
-inline double fgDiv(double f, double fl, double g, double gl) { return 0.5 * (f * gl + fl * g); }
-inline double fgQuad(double f, double fl, double g, double gl) { return 0.25 * (f + fl) * (g + gl); }
-inline double fghCubic(double f, double fl, double g, double gl, double h, double hl) {
+template <class R>
+inline R fDiv(R f, R fl) { return 0.5 * (f + fl); }
+template <class R>
+inline R fgDiv(R f, R fl, R g, R gl) { return 0.5 * (f * gl + fl * g); }
+template <class R>
+inline R fgQuad(R f, R fl, R g, R gl) { return 0.25 * (f + fl) * (g + gl); }
+template <class R>
+inline R fghCubic(R f, R fl, R g, R gl, R h, R hl) {
   return 0.1250 * (f + fl) * (g + gl) * (h + hl);
 }
 
@@ -175,8 +223,9 @@ void keep_coeffs(int order, double c[3]) {
 }
 
 // src/rhs/CentralKEEP.h:40-145: fluxes at i-1/2, j-1/2, k-1/2 of node (i,j,k)
-void keep_point(int i, int j, int k, int halfsten, const double co[3], const A4& q,
-                const A4& fx, const A4& fy, const A4& fz, const orc_closures& c) {
+template <class R>
+void keep_point(int i, int j, int k, int halfsten, const double co[3], const A4T<R>& q,
+                const A4T<R>& fx, const A4T<R>& fy, const A4T<R>& fz, const orc_closures& c) {
   for (int n = 0; n < NCONS; ++n) {
     fx(i, j, k, n) = 0.0;
     fy(i, j, k, n) = 0.0;
@@ -184,19 +233,19 @@ void keep_point(int i, int j, int k, int halfsten, const double co[3], const A4&
   }
   // one direction at a time; (di,dj,dk) is the unit vector, `nrm` the normal
   // velocity component; the accumulation order is l = 1..h, m = 0..l-1
-  auto dir = [&](const A4& f, int di, int dj, int dk, int nrm) {
+  auto dir = [&](const A4T<R>& f, int di, int dj, int dk, int nrm) {
     for (int l = 1; l <= halfsten; ++l) {
       for (int m = 0; m <= l - 1; ++m) {
         int i1 = i + m * di, j1 = j + m * dj, k1 = k + m * dk;
         int i2 = i + (m - l) * di, j2 = j + (m - l) * dj, k2 = k + (m - l) * dk;
-        double rho1 = q(i1, j1, k1, QRHO), ux1 = q(i1, j1, k1, QU), uy1 = q(i1, j1, k1, QV),
-               uz1 = q(i1, j1, k1, QW), ie1 = c.cv * q(i1, j1, k1, QT), p1 = q(i1, j1, k1, QPRES);
-        double rho2 = q(i2, j2, k2, QRHO), ux2 = q(i2, j2, k2, QU), uy2 = q(i2, j2, k2, QV),
-               uz2 = q(i2, j2, k2, QW), ie2 = c.cv * q(i2, j2, k2, QT), p2 = q(i2, j2, k2, QPRES);
-        double u1[3] = {ux1, uy1, uz1}, u2[3] = {ux2, uy2, uz2};
-        double un1 = u1[nrm], un2 = u2[nrm];
-        double massflx = fgQuad(rho1, rho2, un1, un2);
-        double ke = 0.5 * (ux1 * ux2 + uy1 * uy2 + uz1 * uz2);
+        R rho1 = q(i1, j1, k1, QRHO), ux1 = q(i1, j1, k1, QU), uy1 = q(i1, j1, k1, QV),
+          uz1 = q(i1, j1, k1, QW), ie1 = c.cv * q(i1, j1, k1, QT), p1 = q(i1, j1, k1, QPRES);
+        R rho2 = q(i2, j2, k2, QRHO), ux2 = q(i2, j2, k2, QU), uy2 = q(i2, j2, k2, QV),
+          uz2 = q(i2, j2, k2, QW), ie2 = c.cv * q(i2, j2, k2, QT), p2 = q(i2, j2, k2, QPRES);
+        R u1[3] = {ux1, uy1, uz1}, u2[3] = {ux2, uy2, uz2};
+        R un1 = u1[nrm], un2 = u2[nrm];
+        R massflx = fgQuad(rho1, rho2, un1, un2);
+        R ke = 0.5 * (ux1 * ux2 + uy1 * uy2 + uz1 * uz2);
         f(i, j, k, URHO) += co[l - 1] * massflx;
         for (int a = 0; a < 3; ++a) {
           if (a == nrm)
@@ -483,33 +532,35 @@ void riemann_point(int i, int j, int k, int dir, const A4& f, const A4& dq, cons
 
 // ---------------------------------------------------------------- viscous
 // src/rhs/CNS_hydro_K.h:108-172 (cell-centred viscous fluxes)
-void viscous_point(int i, int j, int k, const A4& q, const A4& fx, const A4& fy, const A4& fz,
+template <class R>
+void viscous_point(int i, int j, int k, const A4T<R>& q, const A4T<R>& fx, const A4T<R>& fy,
+                   const A4T<R>& fz,
                    const double dxinv[3], const orc_closures& c) {
-  double ux = q(i, j, k, QU);
-  double dudx = (q(i + 1, j, k, QU) - q(i - 1, j, k, QU)) * 0.5 * dxinv[0];
-  double dvdx = (q(i + 1, j, k, QV) - q(i - 1, j, k, QV)) * 0.5 * dxinv[0];
-  double dwdx = (q(i + 1, j, k, QW) - q(i - 1, j, k, QW)) * 0.5 * dxinv[0];
-  double dTdx = (q(i + 1, j, k, QT) - q(i - 1, j, k, QT)) * 0.5 * dxinv[0];
-  double uy = q(i, j, k, QV);
-  double dudy = (q(i, j + 1, k, QU) - q(i, j - 1, k, QU)) * 0.5 * dxinv[1];
-  double dvdy = (q(i, j + 1, k, QV) - q(i, j - 1, k, QV)) * 0.5 * dxinv[1];
-  double dwdy = (q(i, j + 1, k, QW) - q(i, j - 1, k, QW)) * 0.5 * dxinv[1];
-  double dTdy = (q(i, j + 1, k, QT) - q(i, j - 1, k, QT)) * 0.5 * dxinv[1];
-  double uz = q(i, j, k, QW);
-  double dudz = (q(i, j, k + 1, QU) - q(i, j, k - 1, QU)) * 0.5 * dxinv[2];
-  double dvdz = (q(i, j, k + 1, QV) - q(i, j, k - 1, QV)) * 0.5 * dxinv[2];
-  double dwdz = (q(i, j, k + 1, QW) - q(i, j, k - 1, QW)) * 0.5 * dxinv[2];
-  double dTdz = (q(i, j, k + 1, QT) - q(i, j, k - 1, QT)) * 0.5 * dxinv[2];
-  double div = dudx + dvdy + dwdz;
-  double mu = visc(c, q(i, j, k, QT));
-  double lambda = cond(c, q(i, j, k, QT));
+  R ux = q(i, j, k, QU);
+  R dudx = (q(i + 1, j, k, QU) - q(i - 1, j, k, QU)) * 0.5 * dxinv[0];
+  R dvdx = (q(i + 1, j, k, QV) - q(i - 1, j, k, QV)) * 0.5 * dxinv[0];
+  R dwdx = (q(i + 1, j, k, QW) - q(i - 1, j, k, QW)) * 0.5 * dxinv[0];
+  R dTdx = (q(i + 1, j, k, QT) - q(i - 1, j, k, QT)) * 0.5 * dxinv[0];
+  R uy = q(i, j, k, QV);
+  R dudy = (q(i, j + 1, k, QU) - q(i, j - 1, k, QU)) * 0.5 * dxinv[1];
+  R dvdy = (q(i, j + 1, k, QV) - q(i, j - 1, k, QV)) * 0.5 * dxinv[1];
+  R dwdy = (q(i, j + 1, k, QW) - q(i, j - 1, k, QW)) * 0.5 * dxinv[1];
+  R dTdy = (q(i, j + 1, k, QT) - q(i, j - 1, k, QT)) * 0.5 * dxinv[1];
+  R uz = q(i, j, k, QW);
+  R dudz = (q(i, j, k + 1, QU) - q(i, j, k - 1, QU)) * 0.5 * dxinv[2];
+  R dvdz = (q(i, j, k + 1, QV) - q(i, j, k - 1, QV)) * 0.5 * dxinv[2];
+  R dwdz = (q(i, j, k + 1, QW) - q(i, j, k - 1, QW)) * 0.5 * dxinv[2];
+  R dTdz = (q(i, j, k + 1, QT) - q(i, j, k - 1, QT)) * 0.5 * dxinv[2];
+  R div = dudx + dvdy + dwdz;
+  R mu = visc(c, q(i, j, k, QT));
+  R lambda = cond(c, q(i, j, k, QT));
   double r1_3 = 1.0 / 3.0;
-  double tauxx = 2.0 * mu * (dudx - r1_3 * div);
-  double tauxy = mu * (dudy + dvdx);
-  double tauxz = mu * (dudz + dwdx);
-  double tauyy = 2.0 * mu * (dvdy - r1_3 * div);
-  double tauyz = mu * (dvdz + dwdy);
-  double tauzz = 2.0 * mu * (dwdz - r1_3 * div);
+  R tauxx = 2.0 * mu * (dudx - r1_3 * div);
+  R tauxy = mu * (dudy + dvdx);
+  R tauxz = mu * (dudz + dwdx);
+  R tauyy = 2.0 * mu * (dvdy - r1_3 * div);
+  R tauyz = mu * (dvdz + dwdy);
+  R tauzz = 2.0 * mu * (dwdz - r1_3 * div);
   fx(i, j, k, URHO) = 0.0;
   fx(i, j, k, UMX) = -tauxx;
   fx(i, j, k, UMY) = -tauxy;
@@ -542,18 +593,27 @@ int check_args(const orc_scheme& s, int ng) {
 // Flux phase of CNS::compute_rhs (src/tim/CNS_advance.cpp:200-353) for one
 // box: fills nf[3] (nodal FABs on [lo-ng, hi+1+ng]) in the same regions the
 // reference writes.
-int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, const A4& u,
-             const double dxinv[3], const A4 nf[3]) {
+template <class R>
+int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, const A4T<R>& u,
+             const double dxinv[3], const A4T<R> nf[3]) {
   int rc = check_args(s, ng);
   if (rc) return rc;
+  constexpr bool plain = std::is_same<R, double>::value;
+  if (!plain && s.rhs_euler && s.flux_euler != 1)
+    return fail("the tangent (dual-number) path covers the KEEP and viscous right-hand sides");
   Box bg = grow(bx, NGHOST);
-  std::vector<double> qbuf(npts(bg.lo, bg.hi) * NPRIM);
-  A4 q = view(qbuf.data(), bg.lo, bg.hi);
+  std::vector<R> qbuf(npts(bg.lo, bg.hi) * NPRIM);
+  A4T<R> q = view(qbuf.data(), bg.lo, bg.hi);
   // CNS_advance.cpp:200-208 cons2prim on growntilebox(NGHOST)
   for_box(bg, [&](int i, int j, int k) { cons2prim(i, j, k, u, q, c); });
 
   if (s.rhs_euler) {
-    if (s.flux_euler == 2) {
+    if constexpr (!plain) {
+      double co[3];
+      keep_coeffs(s.order_keep, co);
+      int h = s.order_keep / 2;
+      for_box(nodal_all(bx), [&](int i, int j, int k) { keep_point(i, j, k, h, co, q, nf[0], nf[1], nf[2], c); });
+    } else if (s.flux_euler == 2) {
       // High_resolution.h:279-319
       std::vector<double> pfb[3], lb(npts(bg.lo, bg.hi) * 3);
       A4 pf[3];
@@ -592,10 +652,10 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
     // CNS_advance.cpp:276-353 (no walls: Flux_2nd_Order_KEEP and
     // ViscousWallFluxes apply only to bc=6 y-walls)
     Box b1 = grow(bx, 1);
-    std::vector<double> pb[3];
-    A4 pv[3];
+    std::vector<R> pb[3];
+    A4T<R> pv[3];
     for (int d = 0; d < 3; ++d) {
-      pb[d].assign(npts(b1.lo, b1.hi) * NCONS, 0.0);
+      pb[d].assign(npts(b1.lo, b1.hi) * NCONS, R(0.0));
       pv[d] = view(pb[d].data(), b1.lo, b1.hi);
     }
     for_box(b1, [&](int i, int j, int k) { viscous_point(i, j, k, q, pv[0], pv[1], pv[2], dxinv, c); });
@@ -611,7 +671,8 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
 }
 
 // CNS_advance.cpp:376-391 assembly
-void box_assemble(const Box& bx, const A4 nf[3], const double dxinv[3], const A4& dsdt) {
+template <class R>
+void box_assemble(const Box& bx, const A4T<R> nf[3], const double dxinv[3], const A4T<R>& dsdt) {
   for (int n = 0; n < NCONS; ++n)
     for_box(bx, [&](int i, int j, int k) {
       dsdt(i, j, k, n) = dxinv[0] * (nf[0](i, j, k, n) - nf[0](i + 1, j, k, n)) +
@@ -622,13 +683,14 @@ void box_assemble(const Box& bx, const A4 nf[3], const double dxinv[3], const A4
 
 // src: source FAB on the valid box (NCONS comps) added after the assembly
 // when rhs_source = 1 (the user_source hook, CNS_advance.cpp:397-407)
-int box_rhs(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, const A4& u,
-            const double dxinv[3], const A4& dsdt, const double* src) {
+template <class R>
+int box_rhs(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, const A4T<R>& u,
+            const double dxinv[3], const A4T<R>& dsdt, const double* src) {
   Box bn = grow(nodal_all(bx), NGHOST);
-  std::vector<double> fb[3];
-  A4 nf[3];
+  std::vector<R> fb[3];
+  A4T<R> nf[3];
   for (int d = 0; d < 3; ++d) {
-    fb[d].assign(npts(bn.lo, bn.hi) * NCONS, 0.0);
+    fb[d].assign(npts(bn.lo, bn.hi) * NCONS, R(0.0));
     nf[d] = view(fb[d].data(), bn.lo, bn.hi);
   }
   int rc = box_flux(c, s, bx, ng, u, dxinv, nf);
@@ -667,10 +729,11 @@ int bc_type(int code, int comp, int dir) {
 // cells get (1) the valid data of the box covering their periodic image
 // (AMReX FillBoundary), then (2) physical BCs per non-periodic direction in
 // the order x, y, z (src/set/bcs.cpp + AMReX foextrap/reflect fills).
-int fill_patch(const Level& L, double* const* fabs) {
+template <class R>
+int fill_patch(const Level& L, R* const* fabs) {
   const orc_geom& g = *L.g;
   int ng = L.ng;
-  std::vector<A4> v(L.nbox);
+  std::vector<A4T<R>> v(L.nbox);
   for (int b = 0; b < L.nbox; ++b) {
     Box gb = grow(L.boxes[b], ng);
     v[b] = view(fabs[b], gb.lo, gb.hi);
@@ -731,7 +794,7 @@ int fill_patch(const Level& L, double* const* fabs) {
           if (t == INT_DIR) return;
           int s[3] = {i, j, k};
           s[d] = src;
-          double val = v[b](s[0], s[1], s[2], n);
+          R val = v[b](s[0], s[1], s[2], n);
           v[b](i, j, k, n) = (t == REFLECT_ODD) ? -val : val;
         });
       }
@@ -820,15 +883,16 @@ int advance_sequence(Ops& ops, int order_rk, int stages_rk, double dt) {
 
 // MultiFab::LinComb / Saxpy / Xpay / Copy on valid cells (AMReX semantics:
 // LinComb dst = a*x + b*y; Saxpy dst += a*src; Xpay dst = src + a*dst)
+template <class R>
 struct LevelOps {
   const Level* L;
   const orc_closures* c;
   const orc_scheme* s;
   double dxinv[3];
   int nthreads;
-  std::vector<std::vector<double>> vec[3];  // per vector, per box: valid data
+  std::vector<std::vector<R>> vec[3];  // per vector, per box: valid data
   std::vector<Box> vbox;
-  double* const* state;  // ghosted FABs (Sborder storage and final output)
+  R* const* state;  // ghosted FABs (Sborder storage and final output)
   const double* const* srcs = nullptr;  // per-box source arrays (rhs_source)
 
   long nval(int b) const { return npts(vbox[b].lo, vbox[b].hi) * NCONS; }
@@ -838,8 +902,8 @@ struct LevelOps {
     // FillPatch(Sborder <- src): copy valid cells into the ghosted FABs
     for (int b = 0; b < nb; ++b) {
       Box gb = grow(vbox[b], ng);
-      A4 dst = view(state[b], gb.lo, gb.hi);
-      A4 sv = view(vec[src][b].data(), vbox[b].lo, vbox[b].hi);
+      A4T<R> dst = view(state[b], gb.lo, gb.hi);
+      A4T<R> sv = view(vec[src][b].data(), vbox[b].lo, vbox[b].hi);
       for (int n = 0; n < NCONS; ++n)
         for_box(vbox[b], [&](int i, int j, int k) { dst(i, j, k, n) = sv(i, j, k, n); });
     }
@@ -849,8 +913,8 @@ struct LevelOps {
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads) reduction(| : err)
     for (int b = 0; b < nb; ++b) {
       Box gb = grow(vbox[b], ng);
-      A4 u = view(state[b], gb.lo, gb.hi);
-      A4 d = view(vec[2][b].data(), vbox[b].lo, vbox[b].hi);
+      A4T<R> u = view(state[b], gb.lo, gb.hi);
+      A4T<R> d = view(vec[2][b].data(), vbox[b].lo, vbox[b].hi);
       if (box_rhs(*c, *s, vbox[b], ng, u, dxinv, d, srcs ? srcs[b] : nullptr)) err |= 1;
     }
     return err ? -1 : 0;
@@ -907,6 +971,43 @@ Level make_level(const orc_geom* g, int nbox, const int* boxes, int ng) {
       L.boxes[b].hi[d] = boxes[6 * b + 3 + d];
     }
   return L;
+}
+
+// One CNS::advance step of a level (valid data in, ghosted FABs as Sborder
+// storage; the new state is written back into the valid region).
+template <class R>
+int advance_level(const Level& L, const orc_geom* g, const orc_closures* c, const orc_scheme* s, R* const* state,
+                  const double* const* src, double dt, int nthreads) {
+  const int nbox = L.nbox, ng = L.ng;
+  LevelOps<R> ops;
+  ops.L = &L;
+  ops.c = c;
+  ops.s = s;
+  ops.nthreads = nthreads > 0 ? nthreads : 1;
+  for (int d = 0; d < 3; ++d) ops.dxinv[d] = 1.0 / g->dx[d];
+  ops.vbox = L.boxes;
+  ops.state = state;
+  ops.srcs = src;
+  for (int v = 0; v < 3; ++v) ops.vec[v].resize(nbox);
+  for (int b = 0; b < nbox; ++b) {
+    long n = npts(L.boxes[b].lo, L.boxes[b].hi) * NCONS;
+    for (int v = 0; v < 3; ++v) ops.vec[v][b].assign(n, R(0.0));
+    Box gb = grow(L.boxes[b], ng);
+    A4T<R> st = view(state[b], gb.lo, gb.hi);
+    A4T<R> s1 = view(ops.vec[0][b].data(), L.boxes[b].lo, L.boxes[b].hi);
+    for (int nn = 0; nn < NCONS; ++nn)
+      for_box(L.boxes[b], [&](int i, int j, int k) { s1(i, j, k, nn) = st(i, j, k, nn); });
+  }
+  int rc = advance_sequence(ops, s->order_rk, s->stages_rk, dt);
+  if (rc) return rc;
+  for (int b = 0; b < nbox; ++b) {
+    Box gb = grow(L.boxes[b], ng);
+    A4T<R> st = view(state[b], gb.lo, gb.hi);
+    A4T<R> s2 = view(ops.vec[1][b].data(), L.boxes[b].lo, L.boxes[b].hi);
+    for (int nn = 0; nn < NCONS; ++nn)
+      for_box(L.boxes[b], [&](int i, int j, int k) { st(i, j, k, nn) = s2(i, j, k, nn); });
+  }
+  return 0;
 }
 
 }  // namespace
@@ -988,35 +1089,75 @@ int orc_advance(const orc_geom* g, const orc_closures* c, const orc_scheme* s, i
                 int ng, double* const* state, const double* const* src, double time, double dt, int nthreads) {
   (void)time;
   Level L = make_level(g, nbox, boxes, ng);
-  LevelOps ops;
-  ops.L = &L;
-  ops.c = c;
-  ops.s = s;
-  ops.nthreads = nthreads > 0 ? nthreads : 1;
-  for (int d = 0; d < 3; ++d) ops.dxinv[d] = 1.0 / g->dx[d];
-  ops.vbox = L.boxes;
-  ops.state = state;
-  ops.srcs = src;
-  for (int v = 0; v < 3; ++v) ops.vec[v].resize(nbox);
-  for (int b = 0; b < nbox; ++b) {
-    long n = npts(L.boxes[b].lo, L.boxes[b].hi) * NCONS;
-    for (int v = 0; v < 3; ++v) ops.vec[v][b].assign(n, 0.0);
-    Box gb = grow(L.boxes[b], ng);
-    A4 st = view(state[b], gb.lo, gb.hi);
-    A4 s1 = view(ops.vec[0][b].data(), L.boxes[b].lo, L.boxes[b].hi);
-    for (int nn = 0; nn < NCONS; ++nn)
-      for_box(L.boxes[b], [&](int i, int j, int k) { s1(i, j, k, nn) = st(i, j, k, nn); });
+  return advance_level(L, g, c, s, state, src, dt, nthreads);
+}
+
+int orc_box_rhs_tangent(const orc_closures* c, const orc_scheme* s, const int lo[3], const int hi[3], int ng,
+                        const double* state, const double* dstate, const double dxinv[3], const double* src,
+                        double* dsdt, double* ddsdt) {
+  Box bx;
+  for (int d = 0; d < 3; ++d) {
+    bx.lo[d] = lo[d];
+    bx.hi[d] = hi[d];
   }
-  int rc = advance_sequence(ops, s->order_rk, s->stages_rk, dt);
+  Box gb = grow(bx, ng);
+  const long ng_pts = npts(gb.lo, gb.hi) * NCONS, nv = npts(bx.lo, bx.hi) * NCONS;
+  std::vector<Dual> ub(ng_pts), db(nv);
+  for (long t = 0; t < ng_pts; ++t) ub[t] = Dual(state[t], dstate[t]);
+  A4T<Dual> u = view(ub.data(), gb.lo, gb.hi);
+  A4T<Dual> d = view(db.data(), bx.lo, bx.hi);
+  int rc = box_rhs(*c, *s, bx, ng, u, dxinv, d, src);
   if (rc) return rc;
-  // write the new state (S2) back into the valid region of the FABs
+  for (long t = 0; t < nv; ++t) {
+    dsdt[t] = db[t].v;
+    ddsdt[t] = db[t].d;
+  }
+  return 0;
+}
+
+int orc_fill_patch_tangent(const orc_geom* g, int nbox, const int* boxes, int ng, double* const* fabs,
+                           double* const* dfabs) {
+  Level L = make_level(g, nbox, boxes, ng);
+  std::vector<std::vector<Dual>> buf(nbox);
+  std::vector<Dual*> ptr(nbox);
   for (int b = 0; b < nbox; ++b) {
     Box gb = grow(L.boxes[b], ng);
-    A4 st = view(state[b], gb.lo, gb.hi);
-    A4 s2 = view(ops.vec[1][b].data(), L.boxes[b].lo, L.boxes[b].hi);
-    for (int nn = 0; nn < NCONS; ++nn)
-      for_box(L.boxes[b], [&](int i, int j, int k) { st(i, j, k, nn) = s2(i, j, k, nn); });
+    long n = npts(gb.lo, gb.hi) * NCONS;
+    buf[b].resize(n);
+    for (long t = 0; t < n; ++t) buf[b][t] = Dual(fabs[b][t], dfabs[b][t]);
+    ptr[b] = buf[b].data();
   }
+  int rc = fill_patch(L, ptr.data());
+  if (rc) return rc;
+  for (int b = 0; b < nbox; ++b)
+    for (size_t t = 0; t < buf[b].size(); ++t) {
+      fabs[b][t] = buf[b][t].v;
+      dfabs[b][t] = buf[b][t].d;
+    }
+  return 0;
+}
+
+int orc_advance_tangent(const orc_geom* g, const orc_closures* c, const orc_scheme* s, int nbox, const int* boxes,
+                        int ng, double* const* state, double* const* dstate, const double* const* src, double time,
+                        double dt, int nthreads) {
+  (void)time;
+  Level L = make_level(g, nbox, boxes, ng);
+  std::vector<std::vector<Dual>> buf(nbox);
+  std::vector<Dual*> ptr(nbox);
+  for (int b = 0; b < nbox; ++b) {
+    Box gb = grow(L.boxes[b], ng);
+    long n = npts(gb.lo, gb.hi) * NCONS;
+    buf[b].resize(n);
+    for (long t = 0; t < n; ++t) buf[b][t] = Dual(state[b][t], dstate[b][t]);
+    ptr[b] = buf[b].data();
+  }
+  int rc = advance_level(L, g, c, s, ptr.data(), src, dt, nthreads);
+  if (rc) return rc;
+  for (int b = 0; b < nbox; ++b)
+    for (size_t t = 0; t < buf[b].size(); ++t) {
+      state[b][t] = buf[b][t].v;
+      dstate[b][t] = buf[b][t].d;
+    }
   return 0;
 }
 

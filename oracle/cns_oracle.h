@@ -84,6 +84,20 @@ void orc_totals(const orc_geom* g, int nbox, const int* boxes, int ng,
 int orc_rk_linear(int order_rk, int stages_rk, double lam, double dt, int n,
                   double* u);
 
+/* Tangent (forward-mode, dual numbers) of the KEEP/viscous path: the same
+ * operations as the plain functions on (value, derivative) pairs.  dstate is
+ * the direction (ghosted like state); dsdt/ddsdt receive L(state) and
+ * dL(state)[dstate].  The fill and advance variants update state and
+ * dstate in place. */
+int orc_box_rhs_tangent(const orc_closures* c, const orc_scheme* s, const int lo[3],
+                        const int hi[3], int ng, const double* state, const double* dstate,
+                        const double dxinv[3], const double* src, double* dsdt, double* ddsdt);
+int orc_fill_patch_tangent(const orc_geom* g, int nbox, const int* boxes, int ng,
+                           double* const* fabs, double* const* dfabs);
+int orc_advance_tangent(const orc_geom* g, const orc_closures* c, const orc_scheme* s,
+                        int nbox, const int* boxes, int ng, double* const* state,
+                        double* const* dstate, const double* const* src, double time,
+                        double dt, int nthreads);
 const char* orc_last_error(void);
 
 #ifdef __cplusplus

@@ -108,6 +108,13 @@ def load():
                                ctypes.c_int, PP, _DP]
     lib.orc_rk_linear.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_int, _DP]
+    lib.orc_box_rhs_tangent.argtypes = [ctypes.POINTER(OClosures), ctypes.POINTER(OScheme), I3, I3, ctypes.c_int,
+                                        _DP, _DP, _DP, _DP, _DP, _DP]
+    lib.orc_fill_patch_tangent.argtypes = [ctypes.POINTER(OGeom), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                           ctypes.c_int, PP, PP]
+    lib.orc_advance_tangent.argtypes = [ctypes.POINTER(OGeom), ctypes.POINTER(OClosures),
+                                        ctypes.POINTER(OScheme), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.c_int, PP, PP, PP, ctypes.c_double, ctypes.c_double, ctypes.c_int]
     _lib = lib
     return lib
 
@@ -173,6 +180,21 @@ def box_rhs(c, s, lo, hi, ng, state, dxinv, src=None):
     return out
 
 
+def box_rhs_tangent(c, s, lo, hi, ng, state, dstate, dxinv, src=None):
+    """(dSdt, its derivative along dstate) of one box by dual numbers
+    (KEEP and viscous right-hand sides)."""
+    n = [hi[d] - lo[d] + 1 for d in range(3)]
+    out = np.zeros((5, n[2], n[1], n[0]))
+    dout = np.zeros_like(out)
+    I3 = ctypes.c_int * 3
+    dx = np.ascontiguousarray(dxinv, dtype=np.float64)
+    sp = None if src is None else np.ascontiguousarray(src, dtype=np.float64)
+    _check(load().orc_box_rhs_tangent(ctypes.byref(closures(c)), ctypes.byref(scheme(s)), I3(*lo), I3(*hi), ng,
+                                      _ptr(np.ascontiguousarray(state)), _ptr(np.ascontiguousarray(dstate)),
+                                      _ptr(dx), None if sp is None else _ptr(sp), _ptr(out), _ptr(dout)))
+    return out, dout
+
+
 def box_numflux(c, s, lo, hi, ng, state, dxinv):
     n = [hi[d] - lo[d] + 2 + 2 * ng for d in range(3)]
     f = [np.zeros((5, n[2], n[1], n[0])) for _ in range(3)]
@@ -212,24 +234,27 @@ class OracleLevel:
         for lo, hi in self.boxes:
             m = [hi[d] - lo[d] + 1 + 2 * ng for d in range(3)]
             self.fabs.append(np.zeros((5, m[2], m[1], m[0])))
+        # tangent FABs (direction of a forward-mode derivative), same layout
+        self.tfabs = [np.zeros_like(f) for f in self.fabs]
 
-    def _pp(self):
-        arr = (_DP * len(self.fabs))()
-        for b, f in enumerate(self.fabs):
+    def _pp(self, fabs=None):
+        fabs = self.fabs if fabs is None else fabs
+        arr = (_DP * len(fabs))()
+        for b, f in enumerate(fabs):
             arr[b] = _ptr(f)
         return arr
 
-    def set_state(self, u):
+    def set_state(self, u, fabs=None):
         """u: (5, nz, ny, nx) domain array of valid cells."""
         g = self.ng
-        for (lo, hi), f in zip(self.boxes, self.fabs):
+        for (lo, hi), f in zip(self.boxes, self.fabs if fabs is None else fabs):
             f[:, g:f.shape[1] - g, g:f.shape[2] - g, g:f.shape[3] - g] = \
                 u[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
 
-    def get_state(self):
+    def get_state(self, fabs=None):
         g = self.ng
         out = np.zeros((5, self.n[2], self.n[1], self.n[0]))
-        for (lo, hi), f in zip(self.boxes, self.fabs):
+        for (lo, hi), f in zip(self.boxes, self.fabs if fabs is None else fabs):
             out[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = \
                 f[:, g:f.shape[1] - g, g:f.shape[2] - g, g:f.shape[3] - g]
         return out
@@ -253,6 +278,43 @@ class OracleLevel:
         _check(load().orc_advance(ctypes.byref(self.g), ctypes.byref(closures(c)), ctypes.byref(scheme(s)),
                                   len(self.boxes), self.barr, self.ng, self._pp(), sp, time, dt, nthreads))
         del keep
+
+    # --- tangent (forward mode, dual numbers; KEEP and viscous schemes) ---
+    def set_tangent(self, v):
+        """Direction v on the valid cells; ghost cells of the tangent are
+        zero (coarse-fine ghosts are constants) until a fill sets them."""
+        for t in self.tfabs:
+            t[...] = 0.0
+        self.set_state(v, self.tfabs)
+
+    def get_tangent(self):
+        return self.get_state(self.tfabs)
+
+    def fill_patch_tangent(self):
+        _check(load().orc_fill_patch_tangent(ctypes.byref(self.g), len(self.boxes), self.barr, self.ng,
+                                             self._pp(), self._pp(self.tfabs)))
+
+    def advance_tangent(self, c, s, dt, time=0.0, nthreads=1, source=None):
+        """One step of the state and of its tangent (the derivative of the
+        step along the current tangent)."""
+        sp, keep = self._src(source)
+        _check(load().orc_advance_tangent(ctypes.byref(self.g), ctypes.byref(closures(c)),
+                                          ctypes.byref(scheme(s)), len(self.boxes), self.barr, self.ng,
+                                          self._pp(), self._pp(self.tfabs), sp, time, dt, nthreads))
+        del keep
+
+    def rhs_tangent(self, c, s, source=None):
+        """(dSdt, its derivative along the tangent) after a FillPatch of
+        both."""
+        self.fill_patch_tangent()
+        dx = [1.0 / self.g.dx[d] for d in range(3)]
+        out = np.zeros((5, self.n[2], self.n[1], self.n[0]))
+        dout = np.zeros_like(out)
+        for (lo, hi), f, t in zip(self.boxes, self.fabs, self.tfabs):
+            src = None if source is None else source[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+            sl = (slice(None), slice(lo[2], hi[2] + 1), slice(lo[1], hi[1] + 1), slice(lo[0], hi[0] + 1))
+            out[sl], dout[sl] = box_rhs_tangent(c, s, lo, hi, self.ng, f, t, dx, src)
+        return out, dout
 
     def est_dt(self, c, cfl):
         return load().orc_est_dt(ctypes.byref(self.g), ctypes.byref(closures(c)), len(self.boxes), self.barr,
