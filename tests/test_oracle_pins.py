@@ -232,14 +232,35 @@ def test_riemann_weak_waves_match_exact_solver(seed):
 
 
 def test_exact_solver_reproduces_reference_sod_values():
-    # KEEP_AD_validation.ipynb cell 5 output: region 3 = (p, rho, u)
-    p, u = ER.star_state(1.0, 0.0, 1.0, 0.125, 0.0, 0.1)
-    assert p == pytest.approx(0.303130178050647, rel=1e-12)
-    assert u == pytest.approx(0.9274526200489505, rel=1e-12)
-    r3 = ER.sample(0.6 / 0.2 - 2.5, 1.0, 0.0, 1.0, 0.125, 0.0, 0.1)  # left of the contact
-    assert r3[0] == pytest.approx(0.4263194281784954, rel=1e-12)
-    r4 = ER.sample((0.8 - 0.5) / 0.2, 1.0, 0.0, 1.0, 0.125, 0.0, 0.1)
-    assert r4[0] == pytest.approx(0.2655737117053072, rel=1e-12)
+    """Known answers the reference holds: the exact Sod solution printed by
+    its notebook (tests/golden/sod_exact_notebook.json, extracted by
+    tests/golden/extract_sod_notebook.py): star state, region densities and
+    the four wave positions at t = 0.2."""
+    import json
+    import os
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "sod_exact_notebook.json")))
+    t, g, x0 = ref["case"]["t"], ref["case"]["gamma"], ref["case"]["x0"]
+    pl, rl, ul = ref["case"]["left_p_rho_u"]
+    pr, rr, ur = ref["case"]["right_p_rho_u"]
+    p3, r3, u3 = ref["regions_p_rho_u"]["3"]
+    p4, r4, u4 = ref["regions_p_rho_u"]["4"]
+    p, u = ER.star_state(rl, ul, pl, rr, ur, pr, g)
+    assert p == pytest.approx(p3, rel=1e-12) and p == pytest.approx(p4, rel=1e-12)
+    assert u == pytest.approx(u3, rel=1e-12) and u == pytest.approx(u4, rel=1e-12)
+    # region 3 (left of the contact) and region 4 (between contact and shock)
+    xc = ref["positions"]["Contact Discontinuity"]
+    assert ER.sample((xc - 0.01 - x0) / t, rl, ul, pl, rr, ur, pr, g)[0] == pytest.approx(r3, rel=1e-12)
+    assert ER.sample((xc + 0.01 - x0) / t, rl, ul, pl, rr, ur, pr, g)[0] == pytest.approx(r4, rel=1e-12)
+    # wave positions x0 + speed t (Toro 4.55-4.56, 4.59-4.60)
+    cl, cr = math.sqrt(g * pl / rl), math.sqrt(g * pr / rr)
+    c3 = cl * (p / pl) ** ((g - 1.0) / (2.0 * g))
+    ss = ur + cr * math.sqrt((g + 1.0) / (2.0 * g) * p / pr + (g - 1.0) / (2.0 * g))
+    pos = ref["positions"]
+    assert x0 + (ul - cl) * t == pytest.approx(pos["Head of Rarefaction"], rel=1e-12)
+    assert x0 + (u - c3) * t == pytest.approx(pos["Foot of Rarefaction"], rel=1e-12)
+    assert x0 + u * t == pytest.approx(xc, rel=1e-12)
+    assert x0 + ss * t == pytest.approx(pos["Shock"], rel=1e-12)
+    assert ss == pytest.approx(ref["us"], rel=1e-12)
 
 
 def test_riemann_sod_interface_flux_near_exact():
