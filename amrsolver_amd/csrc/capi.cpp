@@ -1608,6 +1608,11 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
   float best = 1e30f;
   FusedConfig bestc = cand[0];
   StageArgs sa{StageCoef{2, 0, 0, 0, 0}, nullptr, 0, 0, 0};
+  // untimed stages first, so that the first candidates are not timed while
+  // the clocks ramp up
+  bt->tuned[key] = cand[0];
+  for (int rep = 0; rep < 6; ++rep)
+    if ((rc = do_stage(bt, *cls, *sch, sa, nullptr, din, dout))) return rc;
   for (auto& cfg : cand) {
     bt->tuned[key] = cfg;
     float tbest = 1e30f;
