@@ -200,6 +200,7 @@ struct amr_boxtab_s {
   Cover cover{};                      // level tiling (on = 0: not available)
   std::vector<unsigned> cover_bits;   // host copy of cover.bits
   unsigned* d_cover_bits = nullptr;
+  unsigned* d_all_covered = nullptr;  // one all-ones word: the bitmap of a fully covered tile set
   std::vector<std::unique_ptr<GraphEntry>> graphs;
   // adjoint workspace: g (NCONS on grow 3) and e (16 on grow 1) per box
   double* adj_mem = nullptr;
@@ -766,6 +767,11 @@ bool fused_path(const amr_boxtab_s* bt, const amr_scheme_t& sch, FusedConfig* ou
 int build_cover(amr_boxtab_s* bt) {
   bt->cover = Cover{};
   bt->cover_bits.clear();
+  if (!bt->d_all_covered) {
+    const unsigned ones = ~0u;
+    HIPCHK(hipMalloc(&bt->d_all_covered, sizeof(unsigned)));
+    HIPCHK(hipMemcpy(bt->d_all_covered, &ones, sizeof(unsigned), hipMemcpyHostToDevice));
+  }
   if (bt->batch || bt->bt.nbox == 0) return AMR_OK;
   const auto& B = bt->bt.h_boxes;
   Cover c{};
@@ -874,6 +880,21 @@ WrapArgs wrap_args(const amr_boxtab_s* bt, DescSet* ds) {
   return wa;
 }
 
+// The fused kernel's view of the coverage: the level's bitmap under level
+// tiling, else (and for a fully covered level) one all-ones word that every
+// cell maps to (block edge 2^30).
+Cover kernel_cover(const amr_boxtab_s* bt, bool level) {
+  Cover c{};
+  if (level) c = bt->cover;
+  c.on = level ? 1 : 0;
+  if (!c.bits) {
+    c.bits = bt->d_all_covered;
+    c.sh = 30;
+    for (int d = 0; d < 3; ++d) c.nb[d] = 1;
+  }
+  return c;
+}
+
 // Level tiling for a fused stage: every array of the stage is one array over
 // the level, and the level is not a periodic domain-wide array (that case
 // reads periodic images through box tiles).
@@ -901,7 +922,7 @@ int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sc
     int rc = get_tiles(bt, cfg, level, &tt);
     if (rc) return rc;
     const WrapArgs wa = wrap_args(bt, dsin);
-    const Cover cov = level ? bt->cover : Cover{};
+    const Cover cov = kernel_cover(bt, level);
     HIPCHK(launch_fused_stage(sch, cfg, wa, cov, tt->d, tt->n, bt->bt.d_boxes, c, sa, s1, dsin->d, dout->d, st));
     return AMR_OK;
   }
@@ -1408,6 +1429,7 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   if (bt->d_uncov) (void)hipFree(bt->d_uncov);
   if (bt->d_uncov_blk) (void)hipFree(bt->d_uncov_blk);
   if (bt->d_cover_bits) (void)hipFree(bt->d_cover_bits);
+  if (bt->d_all_covered) (void)hipFree(bt->d_all_covered);
   bt->run.reset();
   bt->src.reset();
   bt->adj_g.reset();

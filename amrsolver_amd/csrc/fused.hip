@@ -137,8 +137,12 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
   const Fab S = sinf[T.box];
   const Fab O = outf[T.box];
   const bool use_s1 = sc.mode != 2 && sc.a != 0.0;
-  Fab S1;
-  if (use_s1) S1 = s1f[T.box];
+  // without s1 the epilogue still loads from sin (cache hits of the sv
+  // loads): every iteration then issues the same loads in the same order,
+  // and the compiler's load counters never wait on the plane prefetch
+  const Fab S1 = use_s1 ? s1f[T.box] : S;
+  const bool has_src = sa.src != nullptr;
+  const Fab SRC = has_src ? sa.src[T.box] : S;  // user_source FAB (valid cells)
   const int tid = threadIdx.x;
   const int ti = tid % TX, tj = tid / TX;
   const int gi = T.i0 + ti, gj = T.j0 + tj;
@@ -148,12 +152,12 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
   const int xhi = cov.on ? cov.hi[0] : boxes[T.box].hi[0], yhi = cov.on ? cov.hi[1] : boxes[T.box].hi[1];
   const int xcap = xhi + NGHOST, ycap = yhi + NGHOST;
   const bool own_in = gi <= xhi && gj <= yhi;
+  // coverage bitmap (one all-ones word when every cell is covered)
   guint* cbits = (guint*)cov.bits;
   const int cplane = cov.nb[0] * cov.nb[1];
-  const int ccol = own_in && cbits ? ((gj - cov.lo[1]) >> cov.sh) * cov.nb[0] + ((gi - cov.lo[0]) >> cov.sh) : 0;
+  const int ccol = own_in ? max((gj - cov.lo[1]) >> cov.sh, 0) * cov.nb[0] + max((gi - cov.lo[0]) >> cov.sh, 0) : 0;
   const int k0 = T.k0, nz = T.nz;
   const int kfirst = k0 - C::KB;
-  const int klast = k0 + nz + 1;
   const double co[3] = {co0, co1, co2};
   const double cv = c.cv;
   const int own = (tj + HX) * RX + (ti + HX);  // LDS index of the own cell
@@ -178,7 +182,7 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
   int loff[C::NLOAD];
 #pragma unroll
   for (int s = 0; s < C::NLOAD; ++s) {
-    int cell = tid + s * NT;
+    const int cell = min(tid + s * NT, C::NREG - 1);  // lanes past the region load a region cell
     int ii = cell % RX, jj = cell / RX;
     int x = T.i0 - HX + ii, y = T.j0 - HX + jj;
     if (wa.on) {
@@ -190,19 +194,19 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
     }
     loff[s] = x + y * int(S.js);
   }
-  const int soff = gi + gj * int(S.js);
+  // own column for the epilogue loads, clamped into the valid region (s1
+  // may have no ghost cells): overhanging lanes load there and never store
+  const int soff = min(gi, xhi) + min(gj, yhi) * int(S.js);
+  const gdouble* s1base = gptr(S1.p) + S1.off;
+  const int s1off = min(gi, xhi) + min(gj, yhi) * int(S1.js);
   double pre[C::NLOAD][5];
   auto load_plane = [&](int p) {
     if (wa.on) p = p < wa.lo[2] ? p + wa.len[2] : (p >= wa.lo[2] + wa.len[2] ? p - wa.len[2] : p);
     const gdouble* pb = sbase + int64_t(p) * S.ks;
 #pragma unroll
-    for (int s = 0; s < C::NLOAD; ++s) {
-      int cell = tid + s * NT;
-      if (cell < C::NREG) {
+    for (int s = 0; s < C::NLOAD; ++s)
 #pragma unroll
-        for (int n = 0; n < 5; ++n) pre[s][n] = pb[n * S.ns + loff[s]];
-      }
-    }
+      for (int n = 0; n < 5; ++n) pre[s][n] = pb[n * S.ns + loff[s]];
   };
   auto store_plane = [&](int p) {
     double* sq = s_q[p & 1];
@@ -241,31 +245,28 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
 
   for (int cpl = kfirst; cpl <= k0 + nz; ++cpl) {
     const int pn = cpl + 1;
-    // (a) next plane into LDS, prefetch the one after
-    if (pn <= klast) {
-      store_plane(pn);
-      if (pn + 1 <= klast) load_plane(pn + 1);
-    }
-    // epilogue operands of the plane finalised in this iteration
-    bool fin = cpl >= k0 + 1 && own_in;
-    if (fin && cbits) {
-      const int bit = ((cpl - 1 - cov.lo[2]) >> cov.sh) * cplane + ccol;
-      fin = (cbits[bit >> 5] >> (bit & 31)) & 1u;
-    }
+    // epilogue operands of the plane finalised in this iteration and its
+    // coverage word, loaded by every lane on every iteration (the plane is
+    // clamped to the tile's first one before the tile starts finalising)
+    // and before the prefetch below: loads complete in issue order, so
+    // waiting for these later never waits for the prefetch, which keeps a
+    // whole iteration to arrive
+    const bool fin = cpl >= k0 + 1 && own_in;
+    const int ep = max(cpl - 1, k0);
     double sv[5], s1v[5];
-    if (fin) {
-      const gdouble* pb = sbase + int64_t(cpl - 1) * S.ks;
+    {
+      const gdouble* pb = sbase + int64_t(ep) * S.ks;
 #pragma unroll
       for (int n = 0; n < 5; ++n) sv[n] = pb[n * S.ns + soff];
-      if (use_s1) {
-        const gdouble* p1 = gptr(S1.p) + S1.idx(gi, gj, cpl - 1);
+      const gdouble* p1 = s1base + int64_t(ep) * S1.ks;
 #pragma unroll
-        for (int n = 0; n < 5; ++n) s1v[n] = p1[n * S1.ns];
-      } else {
-#pragma unroll
-        for (int n = 0; n < 5; ++n) s1v[n] = 0.0;
-      }
+      for (int n = 0; n < 5; ++n) s1v[n] = p1[n * S1.ns + s1off];
     }
+    const unsigned cword = cbits[(max((ep - cov.lo[2]) >> cov.sh, 0) * cplane + ccol) >> 5];
+    // (a) next plane into LDS, prefetch the one after (the last iteration
+    // prefetches plane k0+nz+2, still inside the ghost layers)
+    store_plane(pn);
+    load_plane(pn + 1);
     const double* sq = s_q[cpl & 1];        // plane c
     const double* sqn = s_q[(cpl + 1) & 1];  // plane c+1 (stored above)
 #pragma unroll
@@ -463,14 +464,21 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
 #pragma unroll
         for (int n = 0; n < 4; ++n) fz[n + 1] += 0.5 * (pfz_prev[n] + pfz_cur[n]);
       }
-      if (fin) {
+      if (fin && ((cword >> ((((cpl - 1 - cov.lo[2]) >> cov.sh) * cplane + ccol) & 31)) & 1u)) {
         gdouble* po = gptr(O.p) + O.idx(gi, gj, cpl - 1);
-        const Fab* srcf = sa.src;
+        double L[5];
 #pragma unroll
         for (int n = 0; n < 5; ++n) {
-          double L = xy_prev[n] + dxi2 * (fz_prev[n] - fz[n]);
-          if (srcf) L = L + srcf[T.box](gi, gj, cpl - 1, n);  // user_source
-          po[n * O.ns] = stage_combine(sc, s1v[n], sv[n], L);
+          L[n] = xy_prev[n] + dxi2 * (fz_prev[n] - fz[n]);
+          if (has_src) L[n] = L[n] + SRC(gi, gj, cpl - 1, n);  // user_source
+        }
+        // s1 is read only where the stage uses it (it is not loaded otherwise)
+        if (use_s1) {
+#pragma unroll
+          for (int n = 0; n < 5; ++n) po[n * O.ns] = stage_combine(sc, s1v[n], sv[n], L[n]);
+        } else {
+#pragma unroll
+          for (int n = 0; n < 5; ++n) po[n * O.ns] = stage_combine(sc, 0.0, sv[n], L[n]);
         }
       }
 #pragma unroll
