@@ -701,8 +701,13 @@ int get_tiles(amr_boxtab_s* bt, const FusedConfig& cfg, bool level, TileTable** 
   std::vector<TileDev> table(n);
   int per = (n + NXCD - 1) / NXCD;
   // block t runs on XCD t % 8; give XCD x the contiguous logical range
-  // [x*per, (x+1)*per)
-  std::vector<int> used(n, 0);
+  // [x*per, (x+1)*per), its longest tiles first (the workgroups that start
+  // last then are the short ones, which trims the tail of the launch)
+  for (int x = 0; x < NXCD; ++x) {
+    const int b = std::min(n, x * per), e = std::min(n, (x + 1) * per);
+    std::stable_sort(logical.begin() + b, logical.begin() + e,
+                     [](const TileDev& p, const TileDev& q) { return p.nz > q.nz; });
+  }
   std::vector<int> order;
   order.reserve(n);
   for (int t = 0; t < per * NXCD; ++t) {
