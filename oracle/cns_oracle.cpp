@@ -95,6 +95,34 @@ inline Dual sqrt(const Dual& a) {
 inline double clamp_low(double lo, double x) { return std::max(lo, x); }
 inline Dual clamp_low(double lo, const Dual& x) { return lo < x.v ? x : Dual(lo); }
 
+// Operation-counting scalar: the plain double value plus a count of every
+// FP64 operation applied to it (the reference arithmetic's flop count).
+// Counting runs single-threaded.
+struct FlopCounts {
+  long add = 0, mul = 0, div = 0, sqrt = 0, cmp = 0;
+};
+FlopCounts g_flops;
+struct Counted {
+  double v = 0.0;
+  Counted() = default;
+  Counted(double x) : v(x) {}  // NOLINT: constants enter uncounted
+  Counted& operator+=(const Counted& o) {
+    ++g_flops.add;
+    v += o.v;
+    return *this;
+  }
+};
+inline Counted operator+(const Counted& a, const Counted& b) { ++g_flops.add; return Counted(a.v + b.v); }
+inline Counted operator-(const Counted& a, const Counted& b) { ++g_flops.add; return Counted(a.v - b.v); }
+inline Counted operator-(const Counted& a) { return Counted(-a.v); }
+inline Counted operator*(const Counted& a, const Counted& b) { ++g_flops.mul; return Counted(a.v * b.v); }
+inline Counted operator/(const Counted& a, const Counted& b) { ++g_flops.div; return Counted(a.v / b.v); }
+inline Counted sqrt(const Counted& a) { ++g_flops.sqrt; return Counted(std::sqrt(a.v)); }
+inline Counted clamp_low(double lo, const Counted& x) {
+  ++g_flops.cmp;
+  return lo < x.v ? x : Counted(lo);
+}
+
 // Fortran-order view, component outermost (AMReX Array4 convention).
 template <class R>
 struct A4T {
@@ -1158,6 +1186,31 @@ int orc_advance_tangent(const orc_geom* g, const orc_closures* c, const orc_sche
       state[b][t] = buf[b][t].v;
       dstate[b][t] = buf[b][t].d;
     }
+  return 0;
+}
+
+int orc_advance_flops(const orc_geom* g, const orc_closures* c, const orc_scheme* s, int nbox, const int* boxes,
+                      int ng, double* const* state, double dt, long counts[5]) {
+  Level L = make_level(g, nbox, boxes, ng);
+  std::vector<std::vector<Counted>> buf(nbox);
+  std::vector<Counted*> ptr(nbox);
+  for (int b = 0; b < nbox; ++b) {
+    Box gb = grow(L.boxes[b], ng);
+    long n = npts(gb.lo, gb.hi) * NCONS;
+    buf[b].resize(n);
+    for (long t = 0; t < n; ++t) buf[b][t] = Counted(state[b][t]);
+    ptr[b] = buf[b].data();
+  }
+  g_flops = FlopCounts();
+  int rc = advance_level(L, g, c, s, ptr.data(), nullptr, dt, 1);
+  if (rc) return rc;
+  counts[0] = g_flops.add;
+  counts[1] = g_flops.mul;
+  counts[2] = g_flops.div;
+  counts[3] = g_flops.sqrt;
+  counts[4] = g_flops.cmp;
+  for (int b = 0; b < nbox; ++b)
+    for (size_t t = 0; t < buf[b].size(); ++t) state[b][t] = buf[b][t].v;
   return 0;
 }
 

@@ -98,6 +98,12 @@ int orc_advance_tangent(const orc_geom* g, const orc_closures* c, const orc_sche
                         int nbox, const int* boxes, int ng, double* const* state,
                         double* const* dstate, const double* const* src, double time,
                         double dt, int nthreads);
+/* One step of the level (as orc_advance, one thread) on an operation-counting
+ * scalar: counts[] = FP64 adds/subtracts, multiplies, divides, square roots
+ * and comparisons of the reference arithmetic. */
+int orc_advance_flops(const orc_geom* g, const orc_closures* c, const orc_scheme* s,
+                      int nbox, const int* boxes, int ng, double* const* state, double dt,
+                      long counts[5]);
 const char* orc_last_error(void);
 
 #ifdef __cplusplus

@@ -112,6 +112,9 @@ def load():
                                         _DP, _DP, _DP, _DP, _DP, _DP]
     lib.orc_fill_patch_tangent.argtypes = [ctypes.POINTER(OGeom), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                            ctypes.c_int, PP, PP]
+    lib.orc_advance_flops.argtypes = [ctypes.POINTER(OGeom), ctypes.POINTER(OClosures), ctypes.POINTER(OScheme),
+                                      ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, PP,
+                                      ctypes.c_double, ctypes.POINTER(ctypes.c_long)]
     lib.orc_advance_tangent.argtypes = [ctypes.POINTER(OGeom), ctypes.POINTER(OClosures),
                                         ctypes.POINTER(OScheme), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                         ctypes.c_int, PP, PP, PP, ctypes.c_double, ctypes.c_double, ctypes.c_int]
@@ -315,6 +318,14 @@ class OracleLevel:
             sl = (slice(None), slice(lo[2], hi[2] + 1), slice(lo[1], hi[1] + 1), slice(lo[0], hi[0] + 1))
             out[sl], dout[sl] = box_rhs_tangent(c, s, lo, hi, self.ng, f, t, dx, src)
         return out, dout
+
+    def advance_flops(self, c, s, dt):
+        """One step on an operation-counting scalar (one thread); returns the
+        counts {add, mul, div, sqrt, cmp} of FP64 operations."""
+        cnt = (ctypes.c_long * 5)()
+        _check(load().orc_advance_flops(ctypes.byref(self.g), ctypes.byref(closures(c)), ctypes.byref(scheme(s)),
+                                        len(self.boxes), self.barr, self.ng, self._pp(), dt, cnt))
+        return dict(zip(("add", "mul", "div", "sqrt", "cmp"), list(cnt)))
 
     def est_dt(self, c, cfl):
         return load().orc_est_dt(ctypes.byref(self.g), ctypes.byref(closures(c)), len(self.boxes), self.barr,

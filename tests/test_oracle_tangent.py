@@ -7,6 +7,9 @@ reverse-mode GPU tests use as their exact directional derivative.  CPU only.
   plain path (bar 1e-7 relative, the differences' own truncation/rounding);
 - the derivative is linear in the direction (to round-off);
 - the ghost fill of a tangent is the fill itself (it is linear).
+
+The same templates instantiated on an operation-counting scalar give the
+reference arithmetic's FP64 operation count; its values equal the plain path.
 """
 import numpy as np
 import pytest
@@ -133,3 +136,26 @@ def test_advance_tangent(bc, rk):
 
     best = min(abs(_fd4(f, h) - got) for h in (1e-4, 3e-4, 1e-3, 3e-3))
     assert best / abs(got) <= 1e-7, f"{bc} {rk}: {best / abs(got):.2e}"
+
+
+def test_operation_counting_scalar():
+    """The counting instantiation (the reference arithmetic's flop count)
+    returns the plain path's values bit for bit, and its counts depend on the
+    scheme and the grid only, not on the data."""
+    import math
+    cls = P.tgv_closures()
+    n = (16, 16, 16)
+    g = P.geom(n, (0, 0, 0), (2 * math.pi,) * 3, (1, 1, 1))
+    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
+    counts = []
+    for seed in (1, 2):
+        u = P.smooth_random_state(cls, n, seed)
+        a = O.OracleLevel(g, P.decompose(n, 8))
+        a.set_state(u)
+        counts.append(a.advance_flops(cls, sch, 1e-5))
+        b = O.OracleLevel(g, P.decompose(n, 8))
+        b.set_state(u)
+        b.advance(cls, sch, 1e-5)
+        assert np.array_equal(a.get_state(), b.get_state())
+    assert counts[0] == counts[1]
+    assert counts[0]["div"] > 0 and counts[0]["sqrt"] > 0
