@@ -74,6 +74,49 @@ def cpu_baseline():
                       f"{el:.1f} s"}
 
 
+def cpu_extra():
+    """The same oracle on all host cores the process may use (OpenMP over
+    boxes, TGV 256^3 in 64 boxes of 64^3, 2 steps) and, one thread, built for
+    x86-64-v3 (TGV 128^3, 6 steps); plus the host's CPU model."""
+    import os
+    from amrsolver_amd import problems as P
+    from oracle import oracle as O
+    cls = P.tgv_closures()
+    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
+    threads = len(os.sched_getaffinity(0))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    out = {"cpu_model": model, "threads_available": threads}
+    n = (256, 256, 256)
+    g = P.geom(n, (0, 0, 0), (2 * math.pi,) * 3, (1, 1, 1))
+    ol = O.OracleLevel(g, P.decompose(n, BOX))
+    ol.set_state(P.tgv_state(cls, g, n))
+    t0 = time.perf_counter()
+    for _ in range(2):
+        ol.advance(cls, sch, DT, nthreads=threads)
+    el = time.perf_counter() - t0
+    out["all_cores"] = {"value": n[0] * n[1] * n[2] * 3 * 2 / el, "unit": "cell-stage/s", "cores": threads,
+                        "sample": f"TGV 256^3 (64 boxes of 64^3), 2 SSPRK(3,3) steps, {el:.1f} s"}
+    del ol
+    n = (128, 128, 128)
+    g = P.geom(n, (0, 0, 0), (2 * math.pi,) * 3, (1, 1, 1))
+    ol = O.OracleLevel(g, P.decompose(n, BOX), lib=O.load("x86v3"))
+    ol.set_state(P.tgv_state(cls, g, n))
+    t0 = time.perf_counter()
+    for _ in range(6):
+        ol.advance(cls, sch, DT, nthreads=1)
+    el = time.perf_counter() - t0
+    out["x86_64_v3_one_thread"] = {"value": n[0] * n[1] * n[2] * 3 * 6 / el, "unit": "cell-stage/s", "cores": 1,
+                                   "sample": f"TGV 128^3 (8 boxes of 64^3), 6 SSPRK(3,3) steps, {el:.1f} s"}
+    return out
+
+
 def copy_calibration(nbytes):
     """Plain device-to-device copies of a known size, for scaling the HBM
     byte counters (FETCH_SIZE / WRITE_SIZE) of a profiler run."""
@@ -481,6 +524,7 @@ def main():
         out["adjoint"] = adjoint_timing(a)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+        out["cpu_other"] = cpu_extra()
     print(json.dumps(out))
     return 0
 

@@ -71,13 +71,16 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def load():
+def load(variant=None):
+    """The oracle library; variant "x86v3" is the same code built for
+    x86-64-v3 hosts (CPU-baseline timing only)."""
     global _lib
-    if _lib is not None:
+    if variant is None and _lib is not None:
         return _lib
-    if not os.path.exists(LIBPATH):
+    path = LIBPATH if variant is None else os.path.join(HERE, "_build", f"libcns_oracle_{variant}.so")
+    if not os.path.exists(path):
         build()
-    lib = ctypes.CDLL(LIBPATH)
+    lib = ctypes.CDLL(path)
     I3 = ctypes.c_int * 3
     lib.orc_last_error.restype = ctypes.c_char_p
     lib.orc_visc.restype = ctypes.c_double
@@ -118,7 +121,8 @@ def load():
     lib.orc_advance_tangent.argtypes = [ctypes.POINTER(OGeom), ctypes.POINTER(OClosures),
                                         ctypes.POINTER(OScheme), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                         ctypes.c_int, PP, PP, PP, ctypes.c_double, ctypes.c_double, ctypes.c_int]
-    _lib = lib
+    if variant is None:
+        _lib = lib
     return lib
 
 
@@ -223,7 +227,8 @@ def box_prims(c, lo, hi, ng, state):
 class OracleLevel:
     """Boxes with one ghosted FAB each (AMReX MultiFab layout) on the CPU."""
 
-    def __init__(self, g, boxes, ng=3):
+    def __init__(self, g, boxes, ng=3, lib=None):
+        self.lib = lib  # None: the default build
         self.g = geom(g)
         self.boxes = [(tuple(lo), tuple(hi)) for lo, hi in boxes]
         self.ng = ng
@@ -278,8 +283,9 @@ class OracleLevel:
 
     def advance(self, c, s, dt, time=0.0, nthreads=1, source=None):
         sp, keep = self._src(source)
-        _check(load().orc_advance(ctypes.byref(self.g), ctypes.byref(closures(c)), ctypes.byref(scheme(s)),
-                                  len(self.boxes), self.barr, self.ng, self._pp(), sp, time, dt, nthreads))
+        _check((self.lib or load()).orc_advance(ctypes.byref(self.g), ctypes.byref(closures(c)),
+                                                ctypes.byref(scheme(s)), len(self.boxes), self.barr, self.ng,
+                                                self._pp(), sp, time, dt, nthreads))
         del keep
 
     # --- tangent (forward mode, dual numbers; KEEP and viscous schemes) ---
