@@ -36,8 +36,12 @@ class Geom(ctypes.Structure):
     _fields_ = [
         ("dom_lo", ctypes.c_int32 * 3), ("dom_hi", ctypes.c_int32 * 3),
         ("periodic", ctypes.c_int32 * 3), ("lo_bc", ctypes.c_int32 * 3), ("hi_bc", ctypes.c_int32 * 3),
-        ("dx", ctypes.c_double * 3), ("prob_lo", ctypes.c_double * 3),
+        ("dx", ctypes.c_double * 3), ("prob_lo", ctypes.c_double * 3), ("prob_hi", ctypes.c_double * 3),
     ]
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("wall_T", ctypes.c_double), ("body_force_x", ctypes.c_double)]
 
 
 class Fab(ctypes.Structure):
@@ -85,6 +89,7 @@ SIGNATURES = {
     "amr_autotune": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF,
                       ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     "amr_set_source": ([_P, _PF], ctypes.c_int),
+    "amr_set_problem": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Problem)], ctypes.c_int),
     "amr_vjp_rhs": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF, _PF, _PF], ctypes.c_int),
     "amr_vjp_stage": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), ctypes.c_int,
                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,

@@ -200,7 +200,15 @@ struct amr_boxtab_s {
   Cover cover{};                      // level tiling (on = 0: not available)
   std::vector<unsigned> cover_bits;   // host copy of cover.bits
   unsigned* d_cover_bits = nullptr;
-  unsigned* d_all_covered = nullptr;  // one all-ones word: the bitmap of a fully covered tile set
+  unsigned* d_all_covered = nullptr;
+  // problem parameters (bc 6 walls, body force) and the wall flux tasks
+  bool has_problem = false;
+  amr_problem_t problem{};
+  WallFill wall_fill{};
+  bool walls = false;  // a y boundary has bc code 6
+  WallTask* d_wall_tasks = nullptr;
+  TaskBlock* d_wall_blk = nullptr;
+  WallArgs wall_args{};  // one all-ones word: the bitmap of a fully covered tile set
   std::vector<std::unique_ptr<GraphEntry>> graphs;
   // adjoint workspace: g (NCONS on grow 3) and e (16 on grow 1) per box
   double* adj_mem = nullptr;
@@ -403,6 +411,19 @@ int build_fill(amr_boxtab_s* bt, DescSet* ds) {
           t.bnd = g.dom_hi[d];
         }
         int code = side == 0 ? g.lo_bc[d] : g.hi_bc[d];
+        t.plo = g.prob_lo[d];
+        t.phi = g.prob_hi[d];
+        if (code == 6) {  // the channel's isothermal no-slip wall (y only)
+          if (d != 1) return fail(AMR_ERR_ARG, "bc code 6 walls are supported in y only, as in the reference");
+          for (int n = 0; n < 5; ++n) t.type[n] = BC_WALL;
+          t.box = b;
+          t.dir = d;
+          t.side = side;
+          t.start = nc;
+          nc += cells(t.lo, t.hi);
+          tasks.push_back(t);
+          continue;
+        }
         bool any = false;
         for (int n = 0; n < 5; ++n) {
           int ty = bc_type(code, n, d);
@@ -435,8 +456,77 @@ int build_fill(amr_boxtab_s* bt, DescSet* ds) {
 
 int enqueue_fill(amr_boxtab_s* bt, DescSet* ds) {
   hipStream_t st = bt->ctx->stream;
+  if (bt->walls && !bt->has_problem)
+    return fail(AMR_ERR_ARG, "bc code 6 walls need amr_set_problem (wall temperature) first");
   HIPCHK(launch_fill_copy(ds->d_copy, ds->d_copy_blk, ds->ncopy_blk, ds->d, st));
-  for (int d = 0; d < 3; ++d) HIPCHK(launch_fill_bc(ds->d_bc[d], ds->d_bc_blk[d], ds->nbc_blk[d], ds->d, st));
+  for (int d = 0; d < 3; ++d)
+    HIPCHK(launch_fill_bc(ds->d_bc[d], ds->d_bc_blk[d], ds->nbc_blk[d], ds->d, bt->bt.d_boxes, bt->wall_fill, st));
+  return AMR_OK;
+}
+
+// Flux corrections next to bc 6 walls in y (src/rhs/CentralKEEP.h:233-286
+// order reduction; src/tim/CNS_advance.cpp:303-327 viscous wall fluxes): per
+// box touching a wall, the three y-face planes next to it and the ghost row of
+// cell fluxes outside it, over the box's nodal (faces) or grown-by-1 (cell
+// fluxes) extent in x and z.  Built with the box table.
+int build_walls(amr_boxtab_s* bt) {
+  const amr_geom_t& g = bt->geom;
+  bt->walls = !bt->batch && !g.periodic[1] && (g.lo_bc[1] == 6 || g.hi_bc[1] == 6);
+  for (int d = 0; d < 3; d += 2)
+    if (!bt->batch && !g.periodic[d] && (g.lo_bc[d] == 6 || g.hi_bc[d] == 6))
+      return fail(AMR_ERR_ARG, "bc code 6 walls are supported in y only, as in the reference");
+  if (bt->batch) {
+    for (const amr_geom_t& gg : bt->geoms)
+      for (int d = 0; d < 3; ++d)
+        if (!gg.periodic[d] && (gg.lo_bc[d] == 6 || gg.hi_bc[d] == 6))
+          return fail(AMR_ERR_ARG, "bc code 6 walls are not supported in batches of independent problems");
+    return AMR_OK;
+  }
+  if (!bt->walls) return AMR_OK;
+  if (!(g.prob_hi[1] > g.prob_lo[1])) return fail(AMR_ERR_ARG, "bc code 6 walls need prob_hi in the geometry");
+  std::vector<WallTask> keep, visc;
+  for (int b = 0; b < bt->bt.nbox; ++b) {
+    const BoxDev& B = bt->bt.h_boxes[b];
+    for (int side = 0; side < 2; ++side) {
+      const bool wall = side == 0 ? (g.lo_bc[1] == 6 && B.lo[1] == g.dom_lo[1]) : (g.hi_bc[1] == 6 && B.hi[1] == g.dom_hi[1]);
+      if (!wall) continue;
+      WallTask k{b, side, {B.lo[0], 0, B.lo[2]}, {B.hi[0] + 1, 0, B.hi[2] + 1}};
+      // (clipped to the box's own faces lo..hi+1: the flux arrays hold no
+      // more, and faces outside them do not enter the box's assembly)
+      if (side == 0) {
+        k.lo[1] = B.lo[1];
+        k.hi[1] = std::min(B.lo[1] + 2, B.hi[1] + 1);
+      } else {
+        k.lo[1] = std::max(B.hi[1] - 2, B.lo[1]);
+        k.hi[1] = B.hi[1];
+      }
+      keep.push_back(k);
+      const int jj = side == 0 ? B.lo[1] - 1 : B.hi[1] + 1;
+      visc.push_back(WallTask{b, side, {B.lo[0] - 1, jj, B.lo[2] - 1}, {B.hi[0] + 1, jj, B.hi[2] + 1}});
+    }
+  }
+  std::vector<WallTask> all(keep);
+  all.insert(all.end(), visc.begin(), visc.end());
+  std::vector<TaskBlock> blk;
+  int nkeep_blk = 0;
+  for (int t = 0; t < int(all.size()); ++t) {
+    const int64_t n = cells(all[t].lo, all[t].hi);
+    for (int64_t f = 0; f < n; f += TASK_BLK) blk.push_back(TaskBlock{t, int(f)});
+    if (t + 1 == int(keep.size())) nkeep_blk = int(blk.size());
+  }
+  HIPCHK(hipMalloc(&bt->d_wall_tasks, sizeof(WallTask) * std::max<size_t>(all.size(), 1)));
+  HIPCHK(hipMemcpy(bt->d_wall_tasks, all.data(), sizeof(WallTask) * all.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&bt->d_wall_blk, sizeof(TaskBlock) * std::max<size_t>(blk.size(), 1)));
+  HIPCHK(hipMemcpy(bt->d_wall_blk, blk.data(), sizeof(TaskBlock) * blk.size(), hipMemcpyHostToDevice));
+  WallArgs& w = bt->wall_args;
+  w.keep = bt->d_wall_tasks;
+  w.keep_blk = bt->d_wall_blk;
+  w.nkeep_blk = nkeep_blk;
+  w.visc = bt->d_wall_tasks;  // block entries index the combined task list
+  w.visc_blk = bt->d_wall_blk + nkeep_blk;
+  w.nvisc_blk = int(blk.size()) - nkeep_blk;
+  w.dlo = g.dom_lo[1];
+  w.dhi = g.dom_hi[1];
   return AMR_OK;
 }
 
@@ -763,6 +853,7 @@ FusedConfig choose_config(const amr_boxtab_s* bt, const amr_scheme_t& sch) {
 bool fused_path(const amr_boxtab_s* bt, const amr_scheme_t& sch, FusedConfig* out, bool level = false) {
   FusedConfig cfg = choose_config(bt, sch);
   if (out) *out = cfg;
+  if (bt->walls) return false;  // the wall flux corrections run in the pass-structured kernels
   return cfg.tx > 0 && fused_supported(sch, cfg.tx, cfg.ty) && fused_fits(bt, cfg, level);
 }
 
@@ -914,8 +1005,11 @@ int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sc
              DescSet* ds1, DescSet* dsin, DescSet* dout) {
   StageArgs sa = sa_in;
   if (sch.rhs_source) {
-    if (!bt->src) return fail(AMR_ERR_ARG, "rhs_source=1 needs a source registered with amr_set_source");
-    sa.src = bt->src->d;
+    const bool body = bt->has_problem && bt->problem.body_force_x != 0.0;
+    if (!bt->src && !body)
+      return fail(AMR_ERR_ARG, "rhs_source=1 needs a source (amr_set_source) or a body force (amr_set_problem)");
+    sa.src = bt->src ? bt->src->d : nullptr;
+    sa.bfx = body ? bt->problem.body_force_x : 0.0;
   }
   Closures c = make_closures(cls);
   hipStream_t st = bt->ctx->stream;
@@ -933,7 +1027,7 @@ int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sc
   }
   int rc = ensure_unfused(bt, sch);
   if (rc) return rc;
-  HIPCHK(launch_unfused_stage(bt->bt, bt->uw, c, sch, sa, s1, dsin->d, dout->d, st));
+  HIPCHK(launch_unfused_stage(bt->bt, bt->uw, c, sch, sa, bt->wall_args, s1, dsin->d, dout->d, st));
   return AMR_OK;
 }
 
@@ -1169,6 +1263,15 @@ bool fabs_overlap(const amr_fab_t* a, const amr_fab_t* b, int nb) {
 int check_adjoint(const amr_scheme_t& s) {
   if (!adjoint_supported(s))
     return fail(AMR_ERR_ARG, "the adjoint covers the KEEP (flux_euler=1) and viscous right-hand sides only");
+  return AMR_OK;
+}
+
+// Box-table conditions of the reverse mode: no bc 6 walls and no
+// state-dependent (body force) source.
+int check_adjoint_table(const amr_boxtab_s* bt, const amr_scheme_t& s) {
+  if (bt->walls) return fail(AMR_ERR_ARG, "the adjoint does not cover bc code 6 walls");
+  if (s.rhs_source && bt->has_problem && bt->problem.body_force_x != 0.0)
+    return fail(AMR_ERR_ARG, "the adjoint does not cover the body-force source");
   return AMR_OK;
 }
 
@@ -1417,6 +1520,7 @@ int finish_boxtab(amr_boxtab_s* bt) {
   if (e != hipSuccess) return fail(AMR_ERR_HIP, std::string("box table allocation: ") + hipGetErrorString(e));
   rc = build_uncovered(bt);
   if (rc == AMR_OK) rc = build_cover(bt);
+  if (rc == AMR_OK) rc = build_walls(bt);
   return rc;
 }
 }  // namespace
@@ -1435,6 +1539,8 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   if (bt->d_uncov_blk) (void)hipFree(bt->d_uncov_blk);
   if (bt->d_cover_bits) (void)hipFree(bt->d_cover_bits);
   if (bt->d_all_covered) (void)hipFree(bt->d_all_covered);
+  if (bt->d_wall_tasks) (void)hipFree(bt->d_wall_tasks);
+  if (bt->d_wall_blk) (void)hipFree(bt->d_wall_blk);
   bt->run.reset();
   bt->src.reset();
   bt->adj_g.reset();
@@ -1552,7 +1658,7 @@ int amr_advance_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_sche
     FusedConfig cfg = choose_config(bt, *sch);
     // (tables for box tiles, level tiles and the pass-structured kernels, as
     // the stage arrays may choose)
-    const bool fused = cfg.tx > 0 && fused_supported(*sch, cfg.tx, cfg.ty);
+    const bool fused = cfg.tx > 0 && fused_supported(*sch, cfg.tx, cfg.ty) && !bt->walls;
     TileTable* tt = nullptr;
     if (fused && fused_fits(bt, cfg) && (rc = get_tiles(bt, cfg, false, &tt))) return rc;
     if (fused && bt->cover.on && (rc = get_tiles(bt, cfg, true, &tt))) return rc;
@@ -1715,12 +1821,33 @@ int amr_set_source(amr_boxtab_t bt, const amr_fab_t* source) {
   return AMR_OK;
 }
 
+int amr_set_problem(amr_boxtab_t bt, const amr_closures_t* cls, const amr_problem_t* problem) {
+  if (!bt) return fail(AMR_ERR_ARG, "null box table");
+  HIPCHK(hipStreamSynchronize(bt->ctx->stream));
+  bt->graphs.clear();  // captured steps hold the previous parameters
+  if (!problem) {
+    bt->has_problem = false;
+    bt->problem = amr_problem_t{};
+    return AMR_OK;
+  }
+  if (!cls) return fail(AMR_ERR_ARG, "amr_set_problem needs the closures of the wall fill");
+  int rc = check_closures(*cls);
+  if (rc) return rc;
+  if (bt->walls && !(problem->wall_T > 0.0)) return fail(AMR_ERR_ARG, "bc code 6 walls need a wall temperature > 0");
+  bt->problem = *problem;
+  bt->has_problem = true;
+  bt->wall_fill = WallFill{problem->wall_T, cls->gamma, cls->Rspec};
+  return AMR_OK;
+}
+
 int amr_vjp_stage(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, int mode, double a, double b,
                   double c, double d, const amr_fab_t* sin, const amr_fab_t* lam_out, const amr_fab_t* lam_s1,
                   const amr_fab_t* lam_sin, int accumulate) {
   if (!bt || !cls || !sch || !sin || !lam_out || !lam_sin) return fail(AMR_ERR_ARG, "null argument");
   int rc;
-  if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls)) || (rc = check_adjoint(*sch))) return rc;
+  if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls)) || (rc = check_adjoint(*sch)) ||
+      (rc = check_adjoint_table(bt, *sch)))
+    return rc;
   if (mode < 0 || mode > 2) return fail(AMR_ERR_ARG, "mode must be 0, 1 or 2");
   DescSet *ds = nullptr, *dlo = nullptr, *dls1 = nullptr, *dls = nullptr;
   if ((rc = get_desc(bt, sin, NGHOST, &ds))) return rc;
@@ -1744,7 +1871,9 @@ int amr_adjoint_run(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme
                     const amr_fab_t* grad, double* J_out) {
   if (!bt || !cls || !sch || !state0 || !grad) return fail(AMR_ERR_ARG, "null argument");
   int rc;
-  if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls)) || (rc = check_adjoint(*sch))) return rc;
+  if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls)) || (rc = check_adjoint(*sch)) ||
+      (rc = check_adjoint_table(bt, *sch)))
+    return rc;
   if (nsteps < 1 || kcheck < 1) return fail(AMR_ERR_ARG, "nsteps and kcheck must be >= 1");
   if (objective != 0 && objective != 1) return fail(AMR_ERR_ARG, "objective must be 0 or 1");
   if (objective == 1 && !weights) return fail(AMR_ERR_ARG, "objective 1 needs weights");

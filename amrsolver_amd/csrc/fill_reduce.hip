@@ -10,20 +10,6 @@ namespace {
 
 constexpr int FBLK = TASK_BLK;
 
-// Cell (i,j,k) of task region [lo,hi] for lane `tid` of block `tb`; false
-// past the region's last cell.
-template <class T>
-__device__ __forceinline__ bool task_cell(const T& t, const TaskBlock& tb, int tid, int c[3]) {
-  const int nx = t.hi[0] - t.lo[0] + 1, ny = t.hi[1] - t.lo[1] + 1, nz = t.hi[2] - t.lo[2] + 1;
-  const int r = tb.first + tid;
-  if (r >= nx * ny * nz) return false;
-  const int r2 = r / nx;
-  c[0] = t.lo[0] + (r - r2 * nx);
-  c[1] = t.lo[1] + r2 % ny;
-  c[2] = t.lo[2] + r2 / ny;
-  return true;
-}
-
 // FillBoundary + periodic images: copy valid cells of `src` into ghost
 // cells of `dst` (AMReX FillBoundary semantics restated).
 __global__ void __launch_bounds__(FBLK) k_fill_copy(const CopyTask* tasks, const TaskBlock* blocks,
@@ -37,14 +23,60 @@ __global__ void __launch_bounds__(FBLK) k_fill_copy(const CopyTask* tasks, const
   for (int n = 0; n < NCONS; ++n) d(c[0], c[1], c[2], n) = s(c[0] - T.sh[0], c[1] - T.sh[1], c[2] - T.sh[2], n);
 }
 
+// bc code 6 ghost cell: CnsFillExtDir (src/set/bcs.cpp:17-87) calling the
+// channel's bcnormal (exm/Channel/prob.h:157-187), an isothermal no-slip
+// wall with dp/dn = 0.  s_int is the first interior cell of the column, and
+// every ghost layer uses it; dratio = ghost distance / first-cell distance
+// from the wall, formed as the reference forms it.
+__device__ __forceinline__ void wall_ghost(const BcTask& T, const Fab& f, const int c[3], const BoxDev& bx,
+                                           const WallFill& wf) {
+  int s[3] = {c[0], c[1], c[2]};
+  s[T.dir] = T.bnd;
+  double si[5];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) si[n] = f(s[0], s[1], s[2], n);
+  const double dxd = bx.dx[T.dir];
+  const double x = T.plo + static_cast<double>(c[T.dir] + 0.5) * dxd;
+  double di, de;
+  if (T.side == 0) {
+    di = static_cast<double>(T.bnd + 0.5) * dxd - T.plo;
+    de = T.plo - x;
+  } else {
+    di = T.phi - static_cast<double>(T.bnd + 0.5) * dxd;
+    de = x - T.phi;
+  }
+  const double dratio = de / di;
+  const double qu = -si[UMX] / si[URHO];
+  const double qv = -si[UMY] / si[URHO];
+  const double qw = -si[UMZ] / si[URHO];
+  const double eint_int = (si[UET] - 0.5 * (si[UMX] * si[UMX] + si[UMY] * si[UMY] + si[UMZ] * si[UMZ]) / si[URHO]) / si[URHO];
+  const double p_int = (wf.gamma - 1.0) * si[URHO] * eint_int;
+  const double T_int = p_int / (wf.Rspec * si[URHO]);
+  const double Tx = wf.Tw + dratio * (wf.Tw - T_int);
+  const double qT = Tx < 50.0 ? 50.0 : Tx;
+  const double qr = p_int / (wf.Rspec * qT);
+  const double ekin = 0.5 * (qu * qu + qv * qv + qw * qw);
+  const double eint = p_int / (qr * (wf.gamma - 1.0));
+  f(c[0], c[1], c[2], URHO) = qr;
+  f(c[0], c[1], c[2], UMX) = qr * qu;
+  f(c[0], c[1], c[2], UMY) = qr * qv;
+  f(c[0], c[1], c[2], UMZ) = qr * qw;
+  f(c[0], c[1], c[2], UET) = qr * (eint + ekin);
+}
+
 // Physical boundaries (src/set/CNS_setup.cpp:18-31 BC map; AMReX foextrap
 // and reflect_even/odd): ghost value from the boundary cell or its mirror.
-__global__ void __launch_bounds__(FBLK) k_fill_bc(const BcTask* tasks, const TaskBlock* blocks, const Fab* fabs) {
+__global__ void __launch_bounds__(FBLK) k_fill_bc(const BcTask* tasks, const TaskBlock* blocks, const Fab* fabs,
+                                                  const BoxDev* boxes, WallFill wf) {
   const TaskBlock tb = blocks[blockIdx.x];
   const BcTask& T = tasks[tb.task];
   int c[3];
   if (!task_cell(T, tb, threadIdx.x, c)) return;
   const Fab f = fabs[T.box];
+  if (T.type[0] == BC_WALL) {
+    wall_ghost(T, f, c, boxes[T.box], wf);
+    return;
+  }
   const int cd = c[T.dir];
   for (int n = 0; n < NCONS; ++n) {
     int ty = T.type[n];
@@ -258,9 +290,10 @@ hipError_t launch_fill_copy(const CopyTask* tasks, const TaskBlock* blocks, int 
   return hipGetLastError();
 }
 
-hipError_t launch_fill_bc(const BcTask* tasks, const TaskBlock* blocks, int nblk, const Fab* fabs, hipStream_t st) {
+hipError_t launch_fill_bc(const BcTask* tasks, const TaskBlock* blocks, int nblk, const Fab* fabs,
+                          const BoxDev* boxes, const WallFill& wf, hipStream_t st) {
   if (nblk == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_bc, dim3(unsigned(nblk)), dim3(FBLK), 0, st, tasks, blocks, fabs);
+  hipLaunchKernelGGL(k_fill_bc, dim3(unsigned(nblk)), dim3(FBLK), 0, st, tasks, blocks, fabs, boxes, wf);
   return hipGetLastError();
 }
 

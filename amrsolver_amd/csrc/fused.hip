@@ -472,6 +472,10 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
           L[n] = xy_prev[n] + dxi2 * (fz_prev[n] - fz[n]);
           if (has_src) L[n] = L[n] + SRC(gi, gj, cpl - 1, n);  // user_source
         }
+        if (sa.bfx != 0.0) {  // body-force user_source (exm/Channel/prob.h:189-193)
+          L[UMX] = L[UMX] + sv[URHO] * sa.bfx;
+          L[UET] = L[UET] + sv[UMX] * sa.bfx;
+        }
         // s1 is read only where the stage uses it (it is not loaded otherwise)
         if (use_s1) {
 #pragma unroll

@@ -81,14 +81,54 @@ constexpr int TASK_BLK = 256;
 struct TaskBlock {
   int task, first;
 };
+// Cell (i,j,k) of task region [lo,hi] for lane `tid` of block `tb` (cells
+// numbered x fastest); false past the region's last cell.
+template <class T>
+__device__ __forceinline__ bool task_cell(const T& t, const TaskBlock& tb, int tid, int c[3]) {
+  const int nx = t.hi[0] - t.lo[0] + 1, ny = t.hi[1] - t.lo[1] + 1, nz = t.hi[2] - t.lo[2] + 1;
+  const int r = tb.first + tid;
+  if (r >= nx * ny * nz) return false;
+  const int r2 = r / nx;
+  c[0] = t.lo[0] + (r - r2 * nx);
+  c[1] = t.lo[1] + r2 % ny;
+  c[2] = t.lo[2] + r2 / ny;
+  return true;
+}
+
 // Physical BC: for cells [lo,hi] of box `box` in direction `dir`, side
 // `side` (0 lo, 1 hi) with domain boundary index `bnd`; per-component type
-// 0 nothing, 2 foextrap, 3 reflect_even, 4 reflect_odd.
+// 0 nothing, 2 foextrap, 3 reflect_even, 4 reflect_odd, 5 the isothermal
+// no-slip wall of bc code 6 (all components at once; plo/phi are the
+// problem's lower and upper bounds in `dir`).
 struct BcTask {
   int box, dir, side, bnd;
   int lo[3], hi[3];
   int type[5];
   int64_t start;
+  double plo, phi;
+};
+constexpr int BC_WALL = 5;
+// Wall parameters of the fill: wall temperature and the gas constants.
+struct WallFill {
+  double Tw, gamma, Rspec;
+};
+
+// Flux corrections next to bc 6 walls (y): task regions of boxes touching a
+// wall, side 0 (lower) or 1 (upper).  KEEP tasks cover the three y-face
+// planes next to the wall (the kernel keeps order_keep/2 of them), viscous
+// tasks the ghost row of cell fluxes outside the wall.
+struct WallTask {
+  int box, side;
+  int lo[3], hi[3];
+};
+struct WallArgs {
+  const WallTask* keep = nullptr;
+  const TaskBlock* keep_blk = nullptr;
+  int nkeep_blk = 0;
+  const WallTask* visc = nullptr;
+  const TaskBlock* visc_blk = nullptr;
+  int nvisc_blk = 0;
+  int dlo = 0, dhi = 0;  // domain bounds in y
 };
 
 inline void keep_coefficients(int order, double co[3]) {
@@ -106,8 +146,8 @@ inline void keep_coefficients(int order, double co[3]) {
 // (called once when a box table is created; launches only look them up).
 hipError_t prepare_block_maps(const BoxTable& bt);
 hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const Closures& c,
-                                const amr_scheme_t& s, const StageArgs& sa, const Fab* s1, const Fab* sin,
-                                const Fab* out, hipStream_t st);
+                                const amr_scheme_t& s, const StageArgs& sa, const WallArgs& wa, const Fab* s1,
+                                const Fab* sin, const Fab* out, hipStream_t st);
 
 // cons2prim of `sin` into w.prims on grow(bx,3) for every box (halo cells
 // from their periodic images when wa.on).
@@ -150,7 +190,7 @@ hipError_t launch_objective(const BoxTable& bt, int kind, const Fab* st, const F
 hipError_t launch_fill_copy(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* fabs,
                             hipStream_t st);
 hipError_t launch_fill_bc(const BcTask* tasks, const TaskBlock* blocks, int nblk, const Fab* fabs,
-                          hipStream_t st);
+                          const BoxDev* boxes, const WallFill& wf, hipStream_t st);
 
 // dst[T.dst](c) = src[T.src](c - sh) over the task regions, into one or two
 // FAB sets (dst1 may be null): the source is read once for both.

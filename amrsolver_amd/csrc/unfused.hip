@@ -272,6 +272,100 @@ __global__ void __launch_bounds__(BLK) k_visc_add(const BoxDev* boxes, const Blo
   }
 }
 
+// ---------------------------------------------------------------- walls
+// Flux_2nd_Order_KEEP (src/rhs/CentralKEEP.h:233-286): next to a bc 6 wall
+// the y fluxes of faces j = lo .. lo+h-1 (lower wall) or hi-h+1 .. hi (upper
+// wall; the wall face hi+1 itself keeps its full-order flux, as in the
+// reference) are recomputed with second-order KEEP (KEEPy, :147-186, one
+// pair with coefficient 1).
+__global__ void __launch_bounds__(BLK) k_keep_wall(const WallTask* tasks, const TaskBlock* blocks,
+                                                   const BoxDev* boxes, const Fab* pr, const Fab* nfy, Closures c,
+                                                   int h) {
+  const TaskBlock tb = blocks[blockIdx.x];
+  const WallTask& T = tasks[tb.task];
+  int cc[3];
+  if (!task_cell(T, tb, threadIdx.x, cc)) return;
+  const int i = cc[0], j = cc[1], k = cc[2];
+  const int wall_j = T.side == 0 ? boxes[T.box].lo[1] : boxes[T.box].hi[1];
+  // lower wall: faces lo .. lo+h-1; upper wall: faces hi-h+1 .. hi (the task
+  // region starts at the box's lowest face / ends at its highest cell)
+  if (T.side == 0 ? j > wall_j + h - 1 : j < wall_j - h + 1) return;
+  const Fab q = pr[T.box];
+  double f[5] = {0, 0, 0, 0, 0};
+  keep_pair<1>(1.0, q(i, j, k, QRHO), q(i, j, k, QU), q(i, j, k, QV), q(i, j, k, QW), c.cv * q(i, j, k, QT),
+               q(i, j, k, QPRES), q(i, j - 1, k, QRHO), q(i, j - 1, k, QU), q(i, j - 1, k, QV), q(i, j - 1, k, QW),
+               c.cv * q(i, j - 1, k, QT), q(i, j - 1, k, QPRES), f);
+  const Fab oy = nfy[T.box];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) oy(i, j, k, n) = f[n];
+}
+
+// ViscousWallFluxes (src/rhs/CNS_hydro_K.h:177-251) on the ghost row outside
+// a bc 6 wall (src/tim/CNS_advance.cpp:303-327): the cell fluxes there are
+// recomputed with one-sided second-order y derivatives, from the row and
+// the two rows inward.
+__global__ void __launch_bounds__(BLK) k_visc_wall(const WallTask* tasks, const TaskBlock* blocks,
+                                                   const BoxDev* boxes, const Fab* pr, const Fab* pfx,
+                                                   const Fab* pfy, const Fab* pfz, Closures c) {
+  const TaskBlock tb = blocks[blockIdx.x];
+  const WallTask& T = tasks[tb.task];
+  int cc[3];
+  if (!task_cell(T, tb, threadIdx.x, cc)) return;
+  const int i = cc[0], j = cc[1], k = cc[2];
+  const BoxDev bx = boxes[T.box];
+  const double dx0 = bx.dxi[0], dx1 = bx.dxi[1], dx2 = bx.dxi[2];
+  const Fab q = pr[T.box], fx = pfx[T.box], fy = pfy[T.box], fz = pfz[T.box];
+  double ux = q(i, j, k, QU);
+  double dudx = (q(i + 1, j, k, QU) - q(i - 1, j, k, QU)) * 0.5 * dx0;
+  double dvdx = (q(i + 1, j, k, QV) - q(i - 1, j, k, QV)) * 0.5 * dx0;
+  double dwdx = (q(i + 1, j, k, QW) - q(i - 1, j, k, QW)) * 0.5 * dx0;
+  double dTdx = (q(i + 1, j, k, QT) - q(i - 1, j, k, QT)) * 0.5 * dx0;
+  double uy = q(i, j, k, QV);
+  double dudy, dvdy, dwdy, dTdy;
+  if (T.side == 0) {
+    dudy = (-q(i, j + 2, k, QU) + 4 * q(i, j + 1, k, QU) - 3 * q(i, j, k, QU)) * 0.5 * dx1;
+    dvdy = (-q(i, j + 2, k, QV) + 4 * q(i, j + 1, k, QV) - 3 * q(i, j, k, QV)) * 0.5 * dx1;
+    dwdy = (-q(i, j + 2, k, QW) + 4 * q(i, j + 1, k, QW) - 3 * q(i, j, k, QW)) * 0.5 * dx1;
+    dTdy = (-q(i, j + 2, k, QT) + 4 * q(i, j + 1, k, QT) - 3 * q(i, j, k, QT)) * 0.5 * dx1;
+  } else {
+    dudy = (3 * q(i, j, k, QU) - 4 * q(i, j - 1, k, QU) + q(i, j - 2, k, QU)) * 0.5 * dx1;
+    dvdy = (3 * q(i, j, k, QV) - 4 * q(i, j - 1, k, QV) + q(i, j - 2, k, QV)) * 0.5 * dx1;
+    dwdy = (3 * q(i, j, k, QW) - 4 * q(i, j - 1, k, QW) + q(i, j - 2, k, QW)) * 0.5 * dx1;
+    dTdy = (3 * q(i, j, k, QT) - 4 * q(i, j - 1, k, QT) + q(i, j - 2, k, QT)) * 0.5 * dx1;
+  }
+  double uz = q(i, j, k, QW);
+  double dudz = (q(i, j, k + 1, QU) - q(i, j, k - 1, QU)) * 0.5 * dx2;
+  double dvdz = (q(i, j, k + 1, QV) - q(i, j, k - 1, QV)) * 0.5 * dx2;
+  double dwdz = (q(i, j, k + 1, QW) - q(i, j, k - 1, QW)) * 0.5 * dx2;
+  double dTdz = (q(i, j, k + 1, QT) - q(i, j, k - 1, QT)) * 0.5 * dx2;
+  double div = dudx + dvdy + dwdz;
+  double T_ = q(i, j, k, QT);
+  double mu = visc(c, T_);
+  double lambda = cond(c, T_, mu);
+  double r1_3 = 1.0 / 3.0;
+  double tauxx = 2.0 * mu * (dudx - r1_3 * div);
+  double tauxy = mu * (dudy + dvdx);
+  double tauxz = mu * (dudz + dwdx);
+  double tauyy = 2.0 * mu * (dvdy - r1_3 * div);
+  double tauyz = mu * (dvdz + dwdy);
+  double tauzz = 2.0 * mu * (dwdz - r1_3 * div);
+  fx(i, j, k, URHO) = 0.0;
+  fx(i, j, k, UMX) = -tauxx;
+  fx(i, j, k, UMY) = -tauxy;
+  fx(i, j, k, UMZ) = -tauxz;
+  fx(i, j, k, UET) = -lambda * dTdx - tauxx * ux - tauxy * uy - tauxz * uz;
+  fy(i, j, k, URHO) = 0.0;
+  fy(i, j, k, UMX) = -tauxy;
+  fy(i, j, k, UMY) = -tauyy;
+  fy(i, j, k, UMZ) = -tauyz;
+  fy(i, j, k, UET) = -lambda * dTdy - tauxy * ux - tauyy * uy - tauyz * uz;
+  fz(i, j, k, URHO) = 0.0;
+  fz(i, j, k, UMX) = -tauxz;
+  fz(i, j, k, UMY) = -tauyz;
+  fz(i, j, k, UMZ) = -tauzz;
+  fz(i, j, k, UET) = -lambda * dTdz - tauxz * ux - tauyz * uy - tauzz * uz;
+}
+
 // ---------------------------------------------------------------- assembly
 // CNS_advance.cpp:376-391 plus the stage combination of CNS::advance
 __global__ void __launch_bounds__(BLK) k_assemble(const BoxDev* boxes, const BlockRef* bmap, Region reg, const double* box_dt, const Fab* nfx,
@@ -286,6 +380,9 @@ __global__ void __launch_bounds__(BLK) k_assemble(const BoxDev* boxes, const Blo
     double L = dx0 * (fx(i, j, k, n) - fx(i + 1, j, k, n)) + dx1 * (fy(i, j, k, n) - fy(i, j + 1, k, n)) +
                dx2 * (fz(i, j, k, n) - fz(i, j, k + 1, n));
     if (sa.src) L = L + sa.src[b](i, j, k, n);  // user_source, CNS_advance.cpp:397-407
+    // body-force user_source (exm/Channel/prob.h:189-193)
+    if (sa.bfx != 0.0 && n == UMX) L = L + sin[b](i, j, k, URHO) * sa.bfx;
+    if (sa.bfx != 0.0 && n == UET) L = L + sin[b](i, j, k, UMX) * sa.bfx;
     double v;
     if (sc.mode == 2) {
       v = L;
@@ -333,8 +430,8 @@ hipError_t prepare_block_maps(const BoxTable& bt) {
 
 // Flux phase + assembly + stage combination, reference pass structure.
 hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const Closures& c,
-                                const amr_scheme_t& s, const StageArgs& sa, const Fab* s1, const Fab* sin,
-                                const Fab* out, hipStream_t st) {
+                                const amr_scheme_t& s, const StageArgs& sa, const WallArgs& wa, const Fab* s1,
+                                const Fab* sin, const Fab* out, hipStream_t st) {
   const BoxDev* B = bt.d_boxes;
   const double* bdt = sa.box_dt;
 #define LAUNCH(KERNEL, REG, ...)                                                                    \
@@ -355,6 +452,10 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
         LAUNCH(k_keep<2>, nod, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
       else
         LAUNCH(k_keep<1>, nod, w.prims, w.nf[0], w.nf[1], w.nf[2], c, co[0], co[1], co[2]);
+      // CNS_advance.cpp:226-229: order reduction next to bc 6 walls
+      if (s.order_keep != 2 && wa.nkeep_blk > 0)
+        hipLaunchKernelGGL(k_keep_wall, dim3(unsigned(wa.nkeep_blk)), dim3(BLK), 0, st, wa.keep, wa.keep_blk, B,
+                           w.prims, w.nf[1], c, s.order_keep / 2);
     } else if (s.flux_euler == 2) {
       LAUNCH(k_eulerflux, g3, sin, w.pe[0], w.pe[1], w.pe[2], w.lam, c);
       LAUNCH(k_weno, nod, sin, w.pe[0], w.pe[1], w.pe[2], w.lam, w.nf[0], w.nf[1], w.nf[2]);
@@ -376,6 +477,9 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
   }
   if (s.rhs_visc) {
     LAUNCH(k_visc_pf, g1, w.prims, w.pv[0], w.pv[1], w.pv[2], c);
+    if (wa.nvisc_blk > 0)
+      hipLaunchKernelGGL(k_visc_wall, dim3(unsigned(wa.nvisc_blk)), dim3(BLK), 0, st, wa.visc, wa.visc_blk, B,
+                         w.prims, w.pv[0], w.pv[1], w.pv[2], c);
     LAUNCH(k_visc_add, nod, w.pv[0], w.pv[1], w.pv[2], w.nf[0], w.nf[1], w.nf[2]);
   }
   LAUNCH(k_assemble, g0, w.nf[0], w.nf[1], w.nf[2], sa, s1, sin, out);

@@ -239,6 +239,15 @@ class Level:
         for schemes with rhs_source = 1."""
         call("amr_set_source", self.h, src.fabs if src is not None else None)
 
+    def set_problem(self, wall_T=0.0, body_force_x=0.0, clear=False):
+        """Problem parameters: the bc 6 wall temperature and the x body force
+        of the channel's user_source (added when rhs_source = 1)."""
+        if clear:
+            call("amr_set_problem", self.h, ctypes.byref(self.closures), None)
+            return
+        p = capi.Problem(wall_T=wall_T, body_force_x=body_force_x)
+        call("amr_set_problem", self.h, ctypes.byref(self.closures), ctypes.byref(p))
+
     # --- reverse mode ----------------------------------------------------
     def vjp_rhs(self, Sborder, lam, grad):
         """grad = (dL/dU)^T lam at the ghost-filled state Sborder."""

@@ -52,6 +52,7 @@ def geom(n, prob_lo, prob_hi, periodic, lo_bc=(0, 0, 0), hi_bc=(0, 0, 0)):
         g.hi_bc[d] = hi_bc[d]
         g.dx[d] = (prob_hi[d] - prob_lo[d]) / n[d]
         g.prob_lo[d] = prob_lo[d]
+        g.prob_hi[d] = prob_hi[d]
     return g
 
 
@@ -416,3 +417,56 @@ def riemann_batch_problems(cls, inst, ny=4, nz=4):
         states.append(sod_state(cls, g, n, rho_l=p["rho_l"], p_l=p["p_l"], u_l=p["u_l"],
                                 rho_r=p["rho_r"], p_r=p["p_r"], u_r=p["u_r"], prob_hi_x=1.0))
     return geoms, states
+
+
+# --------------------------------------------------------------- wall channel
+def wall_channel_parm():
+    """ProbParm of exm/Channel/prob.h:71-98 (same formulas, same order)."""
+    h, Re_tau, Re_b, Ma_b, u_tau, Tw, Rgas = 0.006845, 220.0, 3.0e3, 1.5, 35.0, 500.0, 287.0
+    muw = 2.670533348e-5
+    rho_w = Re_tau * muw / (u_tau * h)
+    Pw = rho_w * Rgas * Tw
+    aw = math.pow(1.4 * Rgas * Tw, 0.5)
+    ub = Ma_b * aw
+    rho_b = Re_b * muw / (ub * h)
+    fx = rho_w * u_tau * u_tau / (h * rho_b)
+    return {"h": h, "Tw": Tw, "Pw": Pw, "fx": fx, "ub": ub, "rho_b": rho_b}
+
+
+def wall_channel(n=(96, 128, 64)):
+    """The compressible channel of exm/Channel: [0, 0.08214] x [0, 0.0136871]
+    x [0, 0.02738] (inputs:12-15), periodic in x and z, bc 6 isothermal
+    no-slip walls in y (prob.h:47-48), reference Sutherland closures, KEEP4 +
+    viscous + body-force source, SSPRK(3,3) (prob.h:49-56).  Returns
+    (closures, geometry, scheme, ProbParm dict)."""
+    cls = closures_reference()
+    g = geom(n, (0.0, 0.0, 0.0), (0.08214, 0.0136871, 0.02738), (1, 0, 1), lo_bc=(0, 6, 0), hi_bc=(0, 6, 0))
+    sch = scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3, source=True)
+    return cls, g, sch, wall_channel_parm()
+
+
+def wall_channel_state(cls, g, n, parm):
+    """prob_initdata of exm/Channel/prob.h:100-155 with the GPU build's
+    rand_num = 0.5 (turbulent-profile mean flow plus one Fourier mode)."""
+    lo = [g.prob_lo[d] for d in range(3)]
+    ln = [g.prob_hi[d] - g.prob_lo[d] for d in range(3)]
+    ax = [lo[d] + (np.arange(n[d]) + 0.5) * g.dx[d] for d in range(3)]
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    h = parm["h"]
+    Y = np.where(Y > h, 2 * h - Y, Y)
+    T = 677.04110513 + 255.46817 * np.power(Y, 0.5) - 176.38967315 * np.exp(-3081.95957181 * Y) * 1.025
+    ux = 581.54524091 + 2687.84321585 * np.power(Y, 0.5) - 593.73493364 * np.exp(-2008.87100276 * Y)
+    pi = 3.14159265358979323846
+    rand_num, A = 0.5, 100 * 2.5
+    nx, ny, nz = 1, 2, 3
+    up = rand_num * A * np.cos(2 * pi * X * nx / ln[0]) * np.sin(pi * Y * ny / ln[1]) * np.cos(2 * pi * Z * nz / ln[2])
+    vp = rand_num * -A * np.sin(2 * pi * X * nx / ln[0]) * np.cos(pi * Y * ny / ln[1]) * np.sin(2 * pi * Z * nz / ln[2])
+    wp = rand_num * -A * np.sin(2 * pi * X * nx / ln[0]) * np.cos(pi * Y * ny / ln[1]) * np.sin(2 * pi * Z * nz / ln[2])
+    rho = parm["Pw"] / (cls.Rspec * T)
+    u = np.empty((5,) + X.shape)
+    u[0] = rho
+    u[1] = rho * (ux + up)
+    u[2] = rho * vp
+    u[3] = rho * wp
+    u[4] = rho * cls.cv * T + 0.5 * rho * (ux * ux + up * up + vp * vp + wp * wp)
+    return u

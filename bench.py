@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run plain device copies of this many bytes (profiler calibration)")
     ap.add_argument("--workload", type=str, default="tgv",
-                    choices=["tgv", "sod-w", "sod-r", "riemann-batch-w", "riemann-batch-r", "adjoint", "amr"])
+                    choices=["tgv", "sod-w", "sod-r", "riemann-batch-w", "riemann-batch-r", "adjoint", "amr",
+                             "channel"])
     ap.add_argument("--tiling", type=str, default="auto",
                     help="fused-kernel tiling: auto (autotune), default, or TX,TY,TZ")
     ap.add_argument("--traffic", type=str, default=os.path.join(ROOT, "profiles", "traffic_tgv256_keep4v.json"))
@@ -319,6 +320,38 @@ def adjoint_timing(a, fd_dirs=0):
     return out
 
 
+def channel_timing(a, steps=20, reps=3):
+    """The reference's wall-bounded channel (exm/Channel: 96x128x64 in 4 boxes
+    of at most 64^3, bc 6 isothermal no-slip walls in y, KEEP4 + viscous +
+    body force, SSPRK(3,3)), dt from CFL 0.9 at the initial state; hipGraph
+    of `steps` steps.  The walls route it through the pass-structured
+    kernels."""
+    from amrsolver_amd import level as LV
+    from amrsolver_amd import problems as P
+    n = (96, 128, 64)
+    cls, g, sch, parm = P.wall_channel(n)
+    lev = LV.Level(g, P.decompose(n, BOX), cls, sch)
+    lev.set_problem(wall_T=parm["Tw"], body_force_x=parm["fx"])
+    S = lev.new_state()
+    S.upload(P.wall_channel_state(cls, g, n, parm))
+    lev.fill_patch(S)
+    dt = lev.est_time_step(S, 0.9)
+    lev.advance_graph(S, dt, steps)
+    lev.synchronize()
+    best = 1e30
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        lev.advance_graph(S, dt, steps)
+        lev.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    ncell = n[0] * n[1] * n[2]
+    u = S.download()
+    return {"workload": "exm/Channel 96x128x64, 4 boxes, bc 6 walls in y, KEEP4+viscous+body force, SSPRK(3,3), "
+                        f"dt from CFL 0.9 ({dt:.3e} s), hipGraph of {steps} steps",
+            "value": ncell * 3 * steps / best, "unit": "cell-stage/s", "ms_per_step": best / steps * 1e3,
+            "finite": bool(np.isfinite(u).all())}
+
+
 def amr_timing(a, n_base=128, steps=5, reps=3):
     """Batched-AMR TGV levels (base n_base^3, 2 levels of ratio 2, tags where
     the analytic |omega| exceeds the level's 70th percentile, blocking factor
@@ -417,6 +450,9 @@ def main():
         return 2
     if a.workload == "amr":
         print(json.dumps(amr_timing(a)))
+        return 0
+    if a.workload == "channel":
+        print(json.dumps(channel_timing(a)))
         return 0
     if a.workload == "adjoint":
         print(json.dumps(adjoint_timing(a, fd_dirs=32)))

@@ -57,14 +57,28 @@ typedef struct {
   int32_t order_rk, stages_rk;
 } amr_scheme_t;
 
-/* Level geometry (AMReX Geometry + the cns.lo_bc/hi_bc codes 0..6). */
+/* Level geometry (AMReX Geometry + the cns.lo_bc/hi_bc codes 0..6;
+ * src/set/CNS_setup.cpp:18-31).  Code 6 is the y wall of exm/Channel: its
+ * ghost cells come from the problem's bcnormal (exm/Channel/prob.h:157-187,
+ * an isothermal no-slip wall; see amr_problem_t) and the KEEP and viscous
+ * fluxes next to it take the reference's wall forms. */
 typedef struct {
   int32_t dom_lo[3], dom_hi[3];
   int32_t periodic[3];
   int32_t lo_bc[3], hi_bc[3];
   double dx[3];
   double prob_lo[3];
+  double prob_hi[3]; /* Geometry::ProbHi (read by the wall boundary) */
 } amr_geom_t;
+
+/* Problem parameters the path reads (the ProbParm fields of
+ * exm/Channel/prob.h:71-98): the wall temperature of bc 6 walls and the x
+ * body force that the channel's user_source adds (rho*fx to rho*u, (rho*u)*fx
+ * to rho*E; exm/Channel/prob.h:189-193). */
+typedef struct {
+  double wall_T;       /* ProbParm::Tw */
+  double body_force_x; /* ProbParm::fx; 0: no body force */
+} amr_problem_t;
 
 /* One box's array (FArrayBox / Array4). */
 typedef struct {
@@ -189,6 +203,13 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
  * source depends only on position).  The arrays are read at every call, so
  * the caller may update them in place; NULL unregisters.  Synchronous. */
 int amr_set_source(amr_boxtab_t bt, const amr_fab_t* source);
+
+/* Problem parameters of a box table (walls, body force) and the closures
+ * the wall fill uses (gamma, Rspec: bcnormal's ProbClosures); problem NULL
+ * clears them.  With rhs_source = 1 the body force is added after the
+ * registered source array (if any).  Tables with bc 6 walls need this call
+ * before any fill or stage.  Synchronous. */
+int amr_set_problem(amr_boxtab_t bt, const amr_closures_t* cls, const amr_problem_t* problem);
 
 /* ---- Reverse mode (no reference counterpart: the reference has no adjoint;
  * its lib/autodiff submodule is empty).  Implemented for the KEEP

@@ -60,6 +60,14 @@ constexpr int NGHOST = 3;  // exm/*/prob.h: #define NGHOST 3
 
 thread_local std::string g_err;
 
+// problem parameters (orc_set_problem): exm/Channel/prob.h ProbParm::Tw and
+// ::fx, and the gas constants its bcnormal reads from the closures
+struct Problem {
+  double Tw = 0.0, fx = 0.0, gamma = 1.4, Rspec = 287.0;
+  bool set = false;
+};
+Problem g_prob;
+
 int fail(const std::string& m) {
   g_err = m;
   return -1;
@@ -94,6 +102,10 @@ inline Dual sqrt(const Dual& a) {
 // std::max(lo, x) as the plain path evaluates it: x if lo < x, else lo
 inline double clamp_low(double lo, double x) { return std::max(lo, x); }
 inline Dual clamp_low(double lo, const Dual& x) { return lo < x.v ? x : Dual(lo); }
+// max(a, b) as the reference's device code evaluates it for the wall
+// temperature floor: b when a < b, else a
+inline double max_of(double a, double b) { return a < b ? b : a; }
+inline Dual max_of(const Dual& a, double b) { return a.v < b ? Dual(b) : a; }
 
 // Operation-counting scalar: the plain double value plus a count of every
 // FP64 operation applied to it (the reference arithmetic's flop count).
@@ -121,6 +133,10 @@ inline Counted sqrt(const Counted& a) { ++g_flops.sqrt; return Counted(std::sqrt
 inline Counted clamp_low(double lo, const Counted& x) {
   ++g_flops.cmp;
   return lo < x.v ? x : Counted(lo);
+}
+inline Counted max_of(const Counted& a, double b) {
+  ++g_flops.cmp;
+  return a.v < b ? Counted(b) : a;
 }
 
 // Fortran-order view, component outermost (AMReX Array4 convention).
@@ -289,6 +305,32 @@ void keep_point(int i, int j, int k, int halfsten, const double co[3], const A4T
   dir(fx, 1, 0, 0, 0);
   dir(fy, 0, 1, 0, 1);
   dir(fz, 0, 0, 1, 2);
+}
+
+// src/rhs/CentralKEEP.h:147-186 KEEPy: the y flux at j-1/2 of node (i,j,k)
+template <class R>
+void keepy_point(int i, int j, int k, int halfsten, const double co[3], const A4T<R>& q, const A4T<R>& fy,
+                 const orc_closures& c) {
+  R flx[NCONS];
+  for (int n = 0; n < NCONS; ++n) flx[n] = 0.0;
+  for (int l = 1; l <= halfsten; ++l) {
+    for (int m = 0; m <= l - 1; ++m) {
+      const int j1 = j + m, j2 = j + m - l;
+      R rho1 = q(i, j1, k, QRHO), ux1 = q(i, j1, k, QU), uy1 = q(i, j1, k, QV), uz1 = q(i, j1, k, QW),
+        ie1 = c.cv * q(i, j1, k, QT), p1 = q(i, j1, k, QPRES);
+      R rho2 = q(i, j2, k, QRHO), ux2 = q(i, j2, k, QU), uy2 = q(i, j2, k, QV), uz2 = q(i, j2, k, QW),
+        ie2 = c.cv * q(i, j2, k, QT), p2 = q(i, j2, k, QPRES);
+      R massflx = fgQuad(rho1, rho2, uy1, uy2);
+      R ke = 0.5 * (ux1 * ux2 + uy1 * uy2 + uz1 * uz2);
+      flx[URHO] = flx[URHO] + co[l - 1] * massflx;
+      flx[UMX] = flx[UMX] + co[l - 1] * fghCubic(rho1, rho2, uy1, uy2, ux1, ux2);
+      flx[UMY] = flx[UMY] + co[l - 1] * (fghCubic(rho1, rho2, uy1, uy2, uy1, uy2) + fDiv(p1, p2));
+      flx[UMZ] = flx[UMZ] + co[l - 1] * fghCubic(rho1, rho2, uy1, uy2, uz1, uz2);
+      flx[UET] = flx[UET] + co[l - 1] * (massflx * ke + fghCubic(rho1, rho2, uy1, uy2, ie1, ie2) +
+                                         fgDiv(p1, p2, uy1, uy2));
+    }
+  }
+  for (int n = 0; n < NCONS; ++n) fy(i, j, k, n) = flx[n];
 }
 
 // ---------------------------------------------------------------- WENO5
@@ -606,6 +648,67 @@ void viscous_point(int i, int j, int k, const A4T<R>& q, const A4T<R>& fx, const
   fz(i, j, k, UET) = -lambda * dTdz - tauxz * ux - tauyz * uy - tauzz * uz;
 }
 
+// src/rhs/CNS_hydro_K.h:177-251 ViscousWallFluxes: loc 0 (lower wall) and
+// 1 (upper wall) take one-sided second-order y derivatives
+template <class R>
+void viscous_wall_point(int i, int j, int k, int loc, const A4T<R>& q, const A4T<R>& fx, const A4T<R>& fy,
+                        const A4T<R>& fz, const double dxinv[3], const orc_closures& c) {
+  R ux = q(i, j, k, QU);
+  R dudx = (q(i + 1, j, k, QU) - q(i - 1, j, k, QU)) * 0.5 * dxinv[0];
+  R dvdx = (q(i + 1, j, k, QV) - q(i - 1, j, k, QV)) * 0.5 * dxinv[0];
+  R dwdx = (q(i + 1, j, k, QW) - q(i - 1, j, k, QW)) * 0.5 * dxinv[0];
+  R dTdx = (q(i + 1, j, k, QT) - q(i - 1, j, k, QT)) * 0.5 * dxinv[0];
+  R uy = q(i, j, k, QV);
+  R dudy, dvdy, dwdy, dTdy;
+  if (loc == 0) {
+    dudy = (-q(i, j + 2, k, QU) + 4 * q(i, j + 1, k, QU) - 3 * q(i, j, k, QU)) * 0.5 * dxinv[1];
+    dvdy = (-q(i, j + 2, k, QV) + 4 * q(i, j + 1, k, QV) - 3 * q(i, j, k, QV)) * 0.5 * dxinv[1];
+    dwdy = (-q(i, j + 2, k, QW) + 4 * q(i, j + 1, k, QW) - 3 * q(i, j, k, QW)) * 0.5 * dxinv[1];
+    dTdy = (-q(i, j + 2, k, QT) + 4 * q(i, j + 1, k, QT) - 3 * q(i, j, k, QT)) * 0.5 * dxinv[1];
+  } else {
+    dudy = (3 * q(i, j, k, QU) - 4 * q(i, j - 1, k, QU) + q(i, j - 2, k, QU)) * 0.5 * dxinv[1];
+    dvdy = (3 * q(i, j, k, QV) - 4 * q(i, j - 1, k, QV) + q(i, j - 2, k, QV)) * 0.5 * dxinv[1];
+    dwdy = (3 * q(i, j, k, QW) - 4 * q(i, j - 1, k, QW) + q(i, j - 2, k, QW)) * 0.5 * dxinv[1];
+    dTdy = (3 * q(i, j, k, QT) - 4 * q(i, j - 1, k, QT) + q(i, j - 2, k, QT)) * 0.5 * dxinv[1];
+  }
+  R uz = q(i, j, k, QW);
+  R dudz = (q(i, j, k + 1, QU) - q(i, j, k - 1, QU)) * 0.5 * dxinv[2];
+  R dvdz = (q(i, j, k + 1, QV) - q(i, j, k - 1, QV)) * 0.5 * dxinv[2];
+  R dwdz = (q(i, j, k + 1, QW) - q(i, j, k - 1, QW)) * 0.5 * dxinv[2];
+  R dTdz = (q(i, j, k + 1, QT) - q(i, j, k - 1, QT)) * 0.5 * dxinv[2];
+  R div = dudx + dvdy + dwdz;
+  R mu = visc(c, q(i, j, k, QT));
+  R lambda = cond(c, q(i, j, k, QT));
+  double r1_3 = 1.0 / 3.0;
+  R tauxx = 2.0 * mu * (dudx - r1_3 * div);
+  R tauxy = mu * (dudy + dvdx);
+  R tauxz = mu * (dudz + dwdx);
+  R tauyy = 2.0 * mu * (dvdy - r1_3 * div);
+  R tauyz = mu * (dvdz + dwdy);
+  R tauzz = 2.0 * mu * (dwdz - r1_3 * div);
+  fx(i, j, k, URHO) = 0.0;
+  fx(i, j, k, UMX) = -tauxx;
+  fx(i, j, k, UMY) = -tauxy;
+  fx(i, j, k, UMZ) = -tauxz;
+  fx(i, j, k, UET) = -lambda * dTdx - tauxx * ux - tauxy * uy - tauxz * uz;
+  fy(i, j, k, URHO) = 0.0;
+  fy(i, j, k, UMX) = -tauxy;
+  fy(i, j, k, UMY) = -tauyy;
+  fy(i, j, k, UMZ) = -tauyz;
+  fy(i, j, k, UET) = -lambda * dTdy - tauxy * ux - tauyy * uy - tauyz * uz;
+  fz(i, j, k, URHO) = 0.0;
+  fz(i, j, k, UMX) = -tauxz;
+  fz(i, j, k, UMY) = -tauyz;
+  fz(i, j, k, UMZ) = -tauzz;
+  fz(i, j, k, UET) = -lambda * dTdz - tauxz * ux - tauyz * uy - tauzz * uz;
+}
+
+// Which y faces of a box sit on bc 6 walls (the box's own lower/upper end
+// is the domain's and the code there is 6).
+struct Walls {
+  bool lo = false, hi = false;
+};
+
 // ---------------------------------------------------------------- box RHS
 int check_args(const orc_scheme& s, int ng) {
   if (ng < NGHOST) return fail("state FAB needs at least 3 ghost cells");
@@ -623,7 +726,7 @@ int check_args(const orc_scheme& s, int ng) {
 // reference writes.
 template <class R>
 int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, const A4T<R>& u,
-             const double dxinv[3], const A4T<R> nf[3]) {
+             const double dxinv[3], const A4T<R> nf[3], const Walls& walls = Walls()) {
   int rc = check_args(s, ng);
   if (rc) return rc;
   constexpr bool plain = std::is_same<R, double>::value;
@@ -635,12 +738,34 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
   // CNS_advance.cpp:200-208 cons2prim on growntilebox(NGHOST)
   for_box(bg, [&](int i, int j, int k) { cons2prim(i, j, k, u, q, c); });
 
+  // CentralKEEP.h:233-286 Flux_2nd_Order_KEEP (called at CNS_advance.cpp:226-229):
+  // y faces lo..lo+h-1 next to a lower wall and hi-h+1..hi next to an upper
+  // wall redone with second-order KEEP (KEEPy with one pair, coefficient 1)
+  auto wall_order_reduction = [&]() {
+    if (!(s.flux_euler == 1 && s.order_keep != 2)) return;
+    const int h = s.order_keep / 2;
+    const double co2[3] = {1.0, 0.0, 0.0};
+    Box bn = nodal_all(bx);
+    for (int side = 0; side < 2; ++side) {
+      if (side == 0 ? !walls.lo : !walls.hi) continue;
+      Box bw = bn;
+      if (side == 0) {
+        bw.lo[1] = bx.lo[1];
+        bw.hi[1] = bx.lo[1] + h - 1;
+      } else {
+        bw.lo[1] = bx.hi[1] - h + 1;
+        bw.hi[1] = bx.hi[1];
+      }
+      for_box(bw, [&](int i, int j, int k) { keepy_point(i, j, k, 1, co2, q, nf[1], c); });
+    }
+  };
   if (s.rhs_euler) {
     if constexpr (!plain) {
       double co[3];
       keep_coeffs(s.order_keep, co);
       int h = s.order_keep / 2;
       for_box(nodal_all(bx), [&](int i, int j, int k) { keep_point(i, j, k, h, co, q, nf[0], nf[1], nf[2], c); });
+      wall_order_reduction();
     } else if (s.flux_euler == 2) {
       // High_resolution.h:279-319
       std::vector<double> pfb[3], lb(npts(bg.lo, bg.hi) * 3);
@@ -664,6 +789,7 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
       keep_coeffs(s.order_keep, co);
       int h = s.order_keep / 2;
       for_box(nodal_all(bx), [&](int i, int j, int k) { keep_point(i, j, k, h, co, q, nf[0], nf[1], nf[2], c); });
+      wall_order_reduction();
     } else {
       // Riemann.h:339-408
       Box bg1 = grow(bx, 1);
@@ -687,6 +813,14 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
       pv[d] = view(pb[d].data(), b1.lo, b1.hi);
     }
     for_box(b1, [&](int i, int j, int k) { viscous_point(i, j, k, q, pv[0], pv[1], pv[2], dxinv, c); });
+    // CNS_advance.cpp:303-327: the ghost row outside a bc 6 wall takes
+    // ViscousWallFluxes over growntilebox(1)'s x and z extent
+    for (int side = 0; side < 2; ++side) {
+      if (side == 0 ? !walls.lo : !walls.hi) continue;
+      Box bw = b1;
+      bw.lo[1] = bw.hi[1] = side == 0 ? bx.lo[1] - 1 : bx.hi[1] + 1;
+      for_box(bw, [&](int i, int j, int k) { viscous_wall_point(i, j, k, side, q, pv[0], pv[1], pv[2], dxinv, c); });
+    }
     // CNS_hydro_K.h:254-261 on grownnodaltilebox(-1,0)
     for (int n = 0; n < NCONS; ++n)
       for_box(nodal_all(bx), [&](int i, int j, int k) {
@@ -713,7 +847,7 @@ void box_assemble(const Box& bx, const A4T<R> nf[3], const double dxinv[3], cons
 // when rhs_source = 1 (the user_source hook, CNS_advance.cpp:397-407)
 template <class R>
 int box_rhs(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, const A4T<R>& u,
-            const double dxinv[3], const A4T<R>& dsdt, const double* src) {
+            const double dxinv[3], const A4T<R>& dsdt, const double* src, const Walls& walls = Walls()) {
   Box bn = grow(nodal_all(bx), NGHOST);
   std::vector<R> fb[3];
   A4T<R> nf[3];
@@ -721,14 +855,24 @@ int box_rhs(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, c
     fb[d].assign(npts(bn.lo, bn.hi) * NCONS, R(0.0));
     nf[d] = view(fb[d].data(), bn.lo, bn.hi);
   }
-  int rc = box_flux(c, s, bx, ng, u, dxinv, nf);
+  int rc = box_flux(c, s, bx, ng, u, dxinv, nf, walls);
   if (rc) return rc;
   box_assemble(bx, nf, dxinv, dsdt);
   if (s.rhs_source) {
-    if (!src) return fail("rhs_source=1 needs a source array");
-    A4 sv = view(const_cast<double*>(src), bx.lo, bx.hi);
-    for (int n = 0; n < NCONS; ++n)
-      for_box(bx, [&](int i, int j, int k) { dsdt(i, j, k, n) = dsdt(i, j, k, n) + sv(i, j, k, n); });
+    const bool body = g_prob.set && g_prob.fx != 0.0;
+    if (!src && !body) return fail("rhs_source=1 needs a source array or a body force");
+    if (src) {
+      A4 sv = view(const_cast<double*>(src), bx.lo, bx.hi);
+      for (int n = 0; n < NCONS; ++n)
+        for_box(bx, [&](int i, int j, int k) { dsdt(i, j, k, n) = dsdt(i, j, k, n) + sv(i, j, k, n); });
+    }
+    if (body) {  // exm/Channel/prob.h:189-193 user_source
+      const double fx = g_prob.fx;
+      for_box(bx, [&](int i, int j, int k) {
+        dsdt(i, j, k, UMX) = dsdt(i, j, k, UMX) + u(i, j, k, URHO) * fx;
+        dsdt(i, j, k, UET) = dsdt(i, j, k, UET) + u(i, j, k, UMX) * fx;
+      });
+    }
   }
   return 0;
 }
@@ -804,8 +948,50 @@ int fill_patch(const Level& L, R* const* fabs) {
     if (g.periodic[d]) continue;
     for (int b = 0; b < L.nbox; ++b) {
       Box gb = grow(L.boxes[b], ng);
+      // bc code 6: src/set/bcs.cpp:17-87 CnsFillExtDir with the channel's
+      // bcnormal (exm/Channel/prob.h:157-187), all components at once
+      for (int side = 0; side < 2; ++side) {
+        if ((side == 0 ? g.lo_bc[d] : g.hi_bc[d]) != 6) continue;
+        if (d != 1) return fail("bc code 6 walls are supported in y only");
+        if (!g_prob.set) return fail("bc code 6 walls need orc_set_problem (wall temperature)");
+        for_box(gb, [&](int i, int j, int k) {
+          int iv[3] = {i, j, k};
+          if (side == 0 ? !(iv[d] < g.dom_lo[d]) : !(iv[d] > g.dom_hi[d])) return;
+          int firs[3] = {i, j, k};
+          firs[d] = side == 0 ? g.dom_lo[d] : g.dom_hi[d];
+          R si[NCONS];
+          for (int n = 0; n < NCONS; ++n) si[n] = v[b](firs[0], firs[1], firs[2], n);
+          const double x = g.prob_lo[d] + static_cast<double>(iv[d] + 0.5) * g.dx[d];
+          double di, de;
+          if (side == 0) {
+            di = static_cast<double>(firs[d] + 0.5) * g.dx[d] - g.prob_lo[d];
+            de = g.prob_lo[d] - x;
+          } else {
+            di = g.prob_hi[d] - static_cast<double>(firs[d] + 0.5) * g.dx[d];
+            de = x - g.prob_hi[d];
+          }
+          const double dratio = de / di;
+          R qu = -si[UMX] / si[URHO];
+          R qv = -si[UMY] / si[URHO];
+          R qw = -si[UMZ] / si[URHO];
+          R eint_int = (si[UET] - 0.5 * (si[UMX] * si[UMX] + si[UMY] * si[UMY] + si[UMZ] * si[UMZ]) / si[URHO]) /
+                       si[URHO];
+          R p_int = (g_prob.gamma - 1.0) * si[URHO] * eint_int;
+          R T_int = p_int / (g_prob.Rspec * si[URHO]);
+          R qT = max_of(g_prob.Tw + dratio * (g_prob.Tw - T_int), 50.0);
+          R qr = p_int / (g_prob.Rspec * qT);
+          R ekin = 0.5 * (qu * qu + qv * qv + qw * qw);
+          R eint = p_int / (qr * (g_prob.gamma - 1.0));
+          v[b](i, j, k, URHO) = qr;
+          v[b](i, j, k, UMX) = qr * qu;
+          v[b](i, j, k, UMY) = qr * qv;
+          v[b](i, j, k, UMZ) = qr * qw;
+          v[b](i, j, k, UET) = qr * (eint + ekin);
+        });
+      }
       for (int n = 0; n < NCONS; ++n) {
-        int tlo = bc_type(g.lo_bc[d], n, d), thi = bc_type(g.hi_bc[d], n, d);
+        int tlo = g.lo_bc[d] == 6 ? INT_DIR : bc_type(g.lo_bc[d], n, d);
+        int thi = g.hi_bc[d] == 6 ? INT_DIR : bc_type(g.hi_bc[d], n, d);
         if (tlo == EXT_DIR || thi == EXT_DIR) return fail("ext_dir boundaries need a problem bcnormal");
         if (tlo < 0 || thi < 0) return fail("invalid physical BC code");
         for_box(gb, [&](int i, int j, int k) {
@@ -943,7 +1129,11 @@ struct LevelOps {
       Box gb = grow(vbox[b], ng);
       A4T<R> u = view(state[b], gb.lo, gb.hi);
       A4T<R> d = view(vec[2][b].data(), vbox[b].lo, vbox[b].hi);
-      if (box_rhs(*c, *s, vbox[b], ng, u, dxinv, d, srcs ? srcs[b] : nullptr)) err |= 1;
+      Walls w;
+      const orc_geom& g = *L->g;
+      w.lo = !g.periodic[1] && g.lo_bc[1] == 6 && vbox[b].lo[1] == g.dom_lo[1];
+      w.hi = !g.periodic[1] && g.hi_bc[1] == 6 && vbox[b].hi[1] == g.dom_hi[1];
+      if (box_rhs(*c, *s, vbox[b], ng, u, dxinv, d, srcs ? srcs[b] : nullptr, w)) err |= 1;
     }
     return err ? -1 : 0;
   }
@@ -1044,6 +1234,14 @@ int advance_level(const Level& L, const orc_geom* g, const orc_closures* c, cons
 extern "C" {
 
 const char* orc_last_error(void) { return g_err.c_str(); }
+
+void orc_set_problem(const orc_closures* c, double wall_T, double body_force_x) {
+  g_prob.Tw = wall_T;
+  g_prob.fx = body_force_x;
+  g_prob.gamma = c ? c->gamma : 1.4;
+  g_prob.Rspec = c ? c->Rspec : 287.0;
+  g_prob.set = c != nullptr;
+}
 
 double orc_visc(const orc_closures* c, double T) { return visc(*c, T); }
 double orc_cond(const orc_closures* c, double T) { return cond(*c, T); }

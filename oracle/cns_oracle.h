@@ -40,7 +40,13 @@ typedef struct {
   int lo_bc[3], hi_bc[3]; /* reference physical BC codes 0..6 (CNS_setup.cpp:18-31) */
   double dx[3];
   double prob_lo[3];
+  double prob_hi[3];
 } orc_geom;
+
+/* Problem parameters (exm/Channel/prob.h ProbParm): bc 6 wall temperature
+ * and x body force of the channel's user_source, with the closures its
+ * bcnormal uses.  Process-wide; set before fills and steps that need them. */
+void orc_set_problem(const orc_closures* c, double wall_T, double body_force_x);
 
 /* ---- point functions ---- */
 double orc_visc(const orc_closures* c, double T);

@@ -34,7 +34,7 @@ class OGeom(ctypes.Structure):
     _fields_ = [
         ("dom_lo", ctypes.c_int * 3), ("dom_hi", ctypes.c_int * 3), ("periodic", ctypes.c_int * 3),
         ("lo_bc", ctypes.c_int * 3), ("hi_bc", ctypes.c_int * 3),
-        ("dx", ctypes.c_double * 3), ("prob_lo", ctypes.c_double * 3),
+        ("dx", ctypes.c_double * 3), ("prob_lo", ctypes.c_double * 3), ("prob_hi", ctypes.c_double * 3),
     ]
 
 
@@ -109,6 +109,8 @@ def load(variant=None):
                                ctypes.POINTER(ctypes.c_int), ctypes.c_int, PP, ctypes.c_double]
     lib.orc_totals.argtypes = [ctypes.POINTER(OGeom), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                ctypes.c_int, PP, _DP]
+    lib.orc_set_problem.restype = None
+    lib.orc_set_problem.argtypes = [ctypes.POINTER(OClosures), ctypes.c_double, ctypes.c_double]
     lib.orc_rk_linear.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_int, _DP]
     lib.orc_box_rhs_tangent.argtypes = [ctypes.POINTER(OClosures), ctypes.POINTER(OScheme), I3, I3, ctypes.c_int,
@@ -351,6 +353,12 @@ class OracleLevel:
             src = None if source is None else source[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
             out[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = box_rhs(c, s, lo, hi, self.ng, f, dx, src)
         return out
+
+
+def set_problem(c, wall_T=0.0, body_force_x=0.0):
+    """Problem parameters (bc 6 wall temperature, x body force) and the
+    closures of the wall fill, for every later oracle call; c None clears."""
+    load().orc_set_problem(None if c is None else ctypes.byref(closures(c)), wall_T, body_force_x)
 
 
 def rk_linear(order_rk, stages_rk, lam, dt, u):

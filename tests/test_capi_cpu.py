@@ -35,7 +35,8 @@ def test_struct_layouts_match_header():
     # sizes of the POD structs as laid out by the C compiler (x86-64 SysV)
     assert ctypes.sizeof(capi.Fab) == 64
     assert ctypes.sizeof(capi.Scheme) == 32
-    assert ctypes.sizeof(capi.Geom) == 112
+    assert ctypes.sizeof(capi.Geom) == 136
+    assert ctypes.sizeof(capi.Problem) == 16
 
 
 def test_stage_coefficients_cpu():
