@@ -119,6 +119,35 @@ class StateArrays:
                 t[:, g:t.shape[1] - g, g:t.shape[2] - g, g:t.shape[3] - g] = torch.from_numpy(np.ascontiguousarray(sub)).to(t.device)
         torch.cuda.synchronize()
 
+    def upload_tensor(self, u):
+        """u: (ncomp, nz, ny, nx) device tensor of the whole domain's valid
+        cells (no host round trip)."""
+        g = self.ng
+        if self.layout == "level":
+            t = self.tensors[0]
+            t[:, g:t.shape[1] - g, g:t.shape[2] - g, g:t.shape[3] - g] = u
+        else:
+            for (lo, hi), t in zip(self.level.boxes, self.tensors):
+                t[:, g:t.shape[1] - g, g:t.shape[2] - g, g:t.shape[3] - g] = \
+                    u[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+
+    def download_tensor(self):
+        """Valid cells of every box as one (ncomp, nz, ny, nx) device tensor
+        (zero where no box covers), after the context's work."""
+        torch = _torch_mod()
+        self.level.ctx.synchronize()
+        g = self.ng
+        N = self.level.n
+        if self.layout == "level":
+            t = self.tensors[0]
+            return t[:, g:t.shape[1] - g, g:t.shape[2] - g, g:t.shape[3] - g].clone()
+        out = torch.zeros((self.tensors[0].shape[0], N[2], N[1], N[0]), dtype=torch.float64,
+                          device=self.tensors[0].device)
+        for (lo, hi), t in zip(self.level.boxes, self.tensors):
+            out[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = \
+                t[:, g:t.shape[1] - g, g:t.shape[2] - g, g:t.shape[3] - g]
+        return out
+
     def upload_ghosted(self, arr):
         """The whole level array including every ghost cell,
         (ncomp, N2+2g, N1+2g, N0+2g) (level layout only); arr may be a numpy

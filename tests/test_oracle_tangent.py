@@ -159,3 +159,19 @@ def test_operation_counting_scalar():
         assert np.array_equal(a.get_state(), b.get_state())
     assert counts[0] == counts[1]
     assert counts[0]["div"] > 0 and counts[0]["sqrt"] > 0
+
+
+def test_bench_operation_count_constant():
+    """bench.py's FP64 operations per cell-stage is the counted value for its
+    workload (TGV, one 64^3 box, KEEP4 + viscous, SSPRK(3,3))."""
+    import math
+    import bench
+    cls = P.tgv_closures()
+    n = (64, 64, 64)
+    g = P.geom(n, (0, 0, 0), (2 * math.pi,) * 3, (1, 1, 1))
+    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
+    ol = O.OracleLevel(g, P.decompose(n, 64))
+    ol.set_state(P.tgv_state(cls, g, n))
+    c = ol.advance_flops(cls, sch, 5e-5)
+    ops = (c["add"] + c["mul"] + c["div"] + c["sqrt"]) / (n[0] * n[1] * n[2] * 3)
+    assert abs(ops - bench.FP64_OPS_PER_CELL_STAGE) <= 0.5, ops
