@@ -786,18 +786,37 @@ int get_tiles(amr_boxtab_s* bt, const FusedConfig& cfg, bool level, TileTable** 
           }
     }
   }
+  // Block t runs on XCD t % 8, and each XCD works through its share on its
+  // own 32 CUs, so the launch ends when the most loaded XCD does.  XCD x gets
+  // a contiguous range of the spatially ordered tiles (halo re-reads stay in
+  // its L2) holding 1/8 of the work, a tile costing one plane iteration per
+  // plane plus three for the pipeline; ranges are padded with empty tiles
+  // (nz = 0, which the kernel skips) to a common length, and each range runs
+  // its longest tiles first, so the workgroups that start last are short.
   const int NXCD = 8;
+  {
+    double tot = 0.0;
+    for (auto& t : logical) tot += t.nz + 3;
+    std::vector<std::vector<TileDev>> parts(NXCD);
+    double acc = 0.0;
+    int x = 0;
+    for (auto& t : logical) {
+      if (x < NXCD - 1 && acc + 0.5 * (t.nz + 3) > tot * (x + 1) / NXCD) ++x;
+      parts[x].push_back(t);
+      acc += t.nz + 3;
+    }
+    size_t mx = 0;
+    for (auto& p : parts) mx = std::max(mx, p.size());
+    logical.clear();
+    for (auto& p : parts) {
+      std::stable_sort(p.begin(), p.end(), [](const TileDev& a, const TileDev& b) { return a.nz > b.nz; });
+      p.resize(mx, TileDev{0, 0, 0, 0, 0});
+      logical.insert(logical.end(), p.begin(), p.end());
+    }
+  }
   int n = int(logical.size());
   std::vector<TileDev> table(n);
-  int per = (n + NXCD - 1) / NXCD;
-  // block t runs on XCD t % 8; give XCD x the contiguous logical range
-  // [x*per, (x+1)*per), its longest tiles first (the workgroups that start
-  // last then are the short ones, which trims the tail of the launch)
-  for (int x = 0; x < NXCD; ++x) {
-    const int b = std::min(n, x * per), e = std::min(n, (x + 1) * per);
-    std::stable_sort(logical.begin() + b, logical.begin() + e,
-                     [](const TileDev& p, const TileDev& q) { return p.nz > q.nz; });
-  }
+  const int per = n / NXCD;
   std::vector<int> order;
   order.reserve(n);
   for (int t = 0; t < per * NXCD; ++t) {
@@ -1721,7 +1740,7 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
   if (level) maxz = bt->cover.hi[2] - bt->cover.lo[2] + 1;  // level tiles run across boxes
   std::vector<FusedConfig> cand;
   const int shapes[3][2] = {{16, 16}, {32, 8}, {8, 8}};
-  const int tzs[4] = {16, 32, 64, 128};
+  const int tzs[6] = {16, 24, 32, 48, 64, 128};
   for (auto& sh : shapes)
     for (int tz : tzs) {
       FusedConfig c{sh[0], sh[1], tz};

@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
   __shared__ double s_fy[5][TY + 1][TX];
 
   const TileDev T = tiles[blockIdx.x];
-  if (!box_active(sa, T.box)) return;  // uniform: the whole workgroup leaves
+  if (T.nz <= 0 || !box_active(sa, T.box)) return;  // uniform: the whole workgroup leaves
   const StageCoef sc = box_stage(sa, T.box);
   const double dxi0 = boxes[T.box].dxi[0], dxi1 = boxes[T.box].dxi[1], dxi2 = boxes[T.box].dxi[2];
   const Fab S = sinf[T.box];
