@@ -86,6 +86,8 @@ SIGNATURES = {
     "amr_batch_est_dt": ([_P, ctypes.POINTER(Closures), _PF, ctypes.c_double,
                           ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "amr_set_tiling": ([_P, ctypes.POINTER(Scheme), ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "amr_set_box_tiling": ([_P, ctypes.POINTER(Scheme), ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "amr_step_plan": ([_P, ctypes.POINTER(Scheme), _PF, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     "amr_autotune": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF,
                       ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     "amr_set_source": ([_P, _PF], ctypes.c_int),

@@ -133,6 +133,7 @@ struct DescSet {
 struct TileTable {
   TileDev* d = nullptr;
   int n = 0;
+  int max_zblk = 0;  // most blocks of the uncovered-cell map a tile's loaded planes span
   ~TileTable() {
     if (d) (void)hipFree(d);
   }
@@ -232,6 +233,10 @@ struct amr_boxtab_s {
   int64_t nuncov_cells = 0;
   TaskBlock* d_uncov_blk = nullptr;
   int nuncov_blk = 0;
+  // blocks of those ghost cells for stages that read them from U^n
+  // (srcmap.on = 0: unavailable; u0 is set per launch)
+  UncovSrc srcmap{};
+  unsigned* d_srcmap_bits = nullptr;
 };
 
 namespace {
@@ -817,6 +822,13 @@ int get_tiles(amr_boxtab_s* bt, const FusedConfig& cfg, bool level, TileTable** 
   int n = int(logical.size());
   std::vector<TileDev> table(n);
   const int per = n / NXCD;
+  int max_zblk = 0;
+  if (bt->srcmap.on)
+    for (const TileDev& t : logical)
+      if (t.nz > 0) {
+        const int lo2 = bt->srcmap.lo[2], sh = bt->srcmap.sh;
+        max_zblk = std::max(max_zblk, ((t.k0 + t.nz + 1 - lo2) >> sh) - ((t.k0 - 2 - lo2) >> sh) + 1);
+      }
   std::vector<int> order;
   order.reserve(n);
   for (int t = 0; t < per * NXCD; ++t) {
@@ -827,6 +839,7 @@ int get_tiles(amr_boxtab_s* bt, const FusedConfig& cfg, bool level, TileTable** 
   for (int t = 0; t < n; ++t) table[t] = logical[order[t]];
   auto tt = std::make_unique<TileTable>();
   tt->n = n;
+  tt->max_zblk = max_zblk;
   HIPCHK(hipMalloc(&tt->d, sizeof(TileDev) * std::max(n, 1)));
   HIPCHK(hipMemcpy(tt->d, table.data(), sizeof(TileDev) * n, hipMemcpyHostToDevice));
   *out = tt.get();
@@ -931,8 +944,74 @@ int build_cover(amr_boxtab_s* bt) {
   return AMR_OK;
 }
 
+// Block map of the ghost cells no box covers (UncovSrc): blocks of 2^sh
+// cells, sh the cover's block exponent lowered until the domain boundaries
+// fall on block boundaries too (then every cell of a block shares the block's
+// class: inside a box or a periodic image of one, outside the domain in a
+// direction with a physical boundary, or such a ghost cell); the map spans
+// the boxes' bounding box grown by at least NGHOST cells.
+int build_srcmap(amr_boxtab_s* bt) {
+  bt->srcmap = UncovSrc{};
+  if (!bt->cover.on || bt->nuncov_cells == 0) return AMR_OK;
+  const Cover& c = bt->cover;
+  const amr_geom_t& g = bt->geom;
+  int sh = c.sh;
+  auto aligned = [&](int sh_) {
+    for (int d = 0; d < 3; ++d)
+      if (((g.dom_lo[d] - c.lo[d]) & ((1 << sh_) - 1)) || ((g.dom_hi[d] + 1 - c.lo[d]) & ((1 << sh_) - 1)))
+        return false;
+    return true;
+  };
+  while (sh > 0 && !aligned(sh)) --sh;
+  const int B = 1 << sh, G = (NGHOST + B - 1) / B;
+  UncovSrc m{};
+  int64_t nbits = 1;
+  for (int d = 0; d < 3; ++d) {
+    m.lo[d] = c.lo[d] - G * B;
+    m.nb[d] = ((c.hi[d] - c.lo[d] + 1) + B - 1) / B + 2 * G;
+    nbits *= m.nb[d];
+  }
+  if (nbits > (int64_t(1) << 28)) return AMR_OK;
+  m.sh = sh;
+  std::vector<unsigned> bits(size_t((nbits + 31) / 32), 0u);
+  for (int bz = 0; bz < m.nb[2]; ++bz)
+    for (int by = 0; by < m.nb[1]; ++by)
+      for (int bx = 0; bx < m.nb[0]; ++bx) {
+        const int b3[3] = {bx, by, bz};
+        int cell[3];
+        bool physical = false;
+        for (int d = 0; d < 3; ++d) {
+          int x = m.lo[d] + b3[d] * B;
+          const int len = g.dom_hi[d] - g.dom_lo[d] + 1;
+          if (x < g.dom_lo[d] || x > g.dom_hi[d]) {
+            if (!g.periodic[d]) physical = true;
+            x = g.dom_lo[d] + (((x - g.dom_lo[d]) % len) + len) % len;
+          }
+          cell[d] = x;
+        }
+        if (physical) continue;
+        bool inside = true;
+        for (int d = 0; d < 3; ++d) inside = inside && cell[d] >= c.lo[d] && cell[d] <= c.hi[d];
+        const bool covered = inside && cover_bit(bt, (cell[0] - c.lo[0]) >> c.sh, (cell[1] - c.lo[1]) >> c.sh,
+                                                 (cell[2] - c.lo[2]) >> c.sh);
+        if (!covered) {
+          const int64_t bit = (int64_t(bz) * m.nb[1] + by) * m.nb[0] + bx;
+          bits[size_t(bit >> 5)] |= 1u << (bit & 31);
+        }
+      }
+  HIPCHK(hipMalloc(&bt->d_srcmap_bits, sizeof(unsigned) * bits.size()));
+  HIPCHK(hipMemcpy(bt->d_srcmap_bits, bits.data(), sizeof(unsigned) * bits.size(), hipMemcpyHostToDevice));
+  m.bits = bt->d_srcmap_bits;
+  m.on = 1;
+  bt->srcmap = m;
+  return AMR_OK;
+}
+
 // Every box of `ds` is a window of one array (same mapping) whose address
-// range also holds the ghosted bounding box of the boxes.
+// range also holds what level tiles touch: the boxes' bounding box grown by
+// NGHOST in x and y and by two planes in z.  The address range of an array
+// is taken as the span of its boxes' windows (first to last element), all
+// of which lie in the caller's allocation.
 bool level_ok(const amr_boxtab_s* bt, DescSet* ds) {
   if (ds->level >= 0) return ds->level == 1;
   ds->level = 0;
@@ -951,9 +1030,9 @@ bool level_ok(const amr_boxtab_s* bt, DescSet* ds) {
   }
   const Cover& c = bt->cover;
   const intptr_t a0 = m0.origin + 8 * intptr_t((c.lo[0] - NGHOST) + (c.lo[1] - NGHOST) * m0.js +
-                                               (c.lo[2] - NGHOST) * m0.ks);
+                                               (c.lo[2] - 2) * m0.ks);
   const intptr_t a1 = m0.origin + 8 * intptr_t((c.hi[0] + NGHOST) + (c.hi[1] + NGHOST) * m0.js +
-                                               (c.hi[2] + NGHOST) * m0.ks + (NCONS - 1) * m0.ns) + 8;
+                                               (c.hi[2] + 2) * m0.ks + (NCONS - 1) * m0.ns) + 8;
   if (a0 < lo || a1 > hi) return false;
   ds->level = 1;
   return true;
@@ -1013,15 +1092,22 @@ Cover kernel_cover(const amr_boxtab_s* bt, bool level) {
 // Level tiling for a fused stage: every array of the stage is one array over
 // the level, and the level is not a periodic domain-wide array (that case
 // reads periodic images through box tiles).
-bool use_level_tiles(amr_boxtab_s* bt, const amr_scheme_t& sch, DescSet* ds1, DescSet* dsin, DescSet* dout) {
+bool level_possible(amr_boxtab_s* bt, const amr_scheme_t& sch, DescSet* ds1, DescSet* dsin, DescSet* dout) {
   if (!bt->cover.on || wrap_ok(bt, dsin)) return false;
   if (!level_ok(bt, dsin) || !level_ok(bt, dout) || (ds1 && !level_ok(bt, ds1))) return false;
   if (sch.rhs_source && !(bt->src && level_ok(bt, bt->src.get()))) return false;
   return true;
 }
 
+// Level tiles for a fused stage where they apply and the tiling in force does
+// not ask for box tiles (the autotuner times both: level tiles compute whole
+// tile columns where the union covers only part of them).
+bool use_level_tiles(amr_boxtab_s* bt, const amr_scheme_t& sch, DescSet* ds1, DescSet* dsin, DescSet* dout) {
+  return !choose_config(bt, sch).box_tiles && level_possible(bt, sch, ds1, dsin, dout);
+}
+
 int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageArgs& sa_in,
-             DescSet* ds1, DescSet* dsin, DescSet* dout) {
+             DescSet* ds1, DescSet* dsin, DescSet* dout, const Fab* uncov_u0 = nullptr) {
   StageArgs sa = sa_in;
   if (sch.rhs_source) {
     const bool body = bt->has_problem && bt->problem.body_force_x != 0.0;
@@ -1040,10 +1126,19 @@ int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sc
     int rc = get_tiles(bt, cfg, level, &tt);
     if (rc) return rc;
     const WrapArgs wa = wrap_args(bt, dsin);
+    UncovSrc us{};
+    if (uncov_u0) {
+      if (!(bt->srcmap.on && !wa.on && tt->max_zblk <= UNCOV_MAX_ZBLK))
+        return fail(AMR_ERR_ARG, "internal: uncovered ghost cells from U^n requested where the kernel cannot");
+      us = bt->srcmap;
+      us.u0 = uncov_u0;
+    }
     const Cover cov = kernel_cover(bt, level);
-    HIPCHK(launch_fused_stage(sch, cfg, wa, cov, tt->d, tt->n, bt->bt.d_boxes, c, sa, s1, dsin->d, dout->d, st));
+    HIPCHK(launch_fused_stage(sch, cfg, wa, cov, us, tt->d, tt->n, bt->bt.d_boxes, c, sa, s1, dsin->d, dout->d,
+                              st));
     return AMR_OK;
   }
+  if (uncov_u0) return fail(AMR_ERR_ARG, "internal: uncovered ghost cells from U^n requested off the fused path");
   int rc = ensure_unfused(bt, sch);
   if (rc) return rc;
   HIPCHK(launch_unfused_stage(bt->bt, bt->uw, c, sch, sa, bt->wall_args, s1, dsin->d, dout->d, st));
@@ -1118,8 +1213,39 @@ int stage_coeffs(int order_rk, int stages_rk, double dt, int stage, int* nst, St
   return AMR_OK;
 }
 
+// Whether the stages of a step from `dst` (U^n) through the workspaces
+// dw[0..1] after the first can read the ghost cells no box covers straight
+// from U^n (UncovSrc) instead of a copy in their inputs: every stage runs the
+// fused kernel with one kind of tiles, whose loaded planes span at most
+// UNCOV_MAX_ZBLK blocks of the map, and each box's arrays in dst and the
+// workspaces have equal strides.
+int uncovered_from_un(amr_boxtab_s* bt, const amr_scheme_t& sch, DescSet* dst, DescSet* const dw[2], bool* ok) {
+  *ok = false;
+  if (!bt->srcmap.on || wrap_ok(bt, dst)) return AMR_OK;
+  const bool level = use_level_tiles(bt, sch, dst, dst, dw[0]);
+  if (level && !level_ok(bt, dw[1])) return AMR_OK;
+  FusedConfig cfg;
+  if (!fused_path(bt, sch, &cfg, level)) return AMR_OK;
+  TileTable* tt = nullptr;
+  int rc = get_tiles(bt, cfg, level, &tt);
+  if (rc) return rc;
+  if (tt->max_zblk > UNCOV_MAX_ZBLK) return AMR_OK;
+  for (int b = 0; b < bt->bt.nbox; ++b) {
+    const amr_fab_t& f = dst->host[b];
+    for (int w = 0; w < 2; ++w) {
+      const amr_fab_t& g = dw[w]->host[b];
+      if (g.jstride != f.jstride || g.kstride != f.kstride || g.nstride != f.nstride) return AMR_OK;
+    }
+  }
+  *ok = true;
+  return AMR_OK;
+}
+
+// One step (CNS::advance).  scratch_out: the autotuner's timed step, which
+// leaves `state` untouched: no ghost fill of `state`, and the last stage
+// writes a workspace instead of `state`.
 int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const amr_fab_t* state,
-                 double dt, const double* d_box_dt = nullptr) {
+                 double dt, const double* d_box_dt = nullptr, bool scratch_out = false) {
   int rc;
   int nst = 0;
   StageCoef sc;
@@ -1132,19 +1258,22 @@ int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t
   if ((rc = build_fill(bt, dst))) return rc;
   for (int w = 0; w < 2; ++w)
     if ((rc = build_fill(bt, dw[w]))) return rc;
-  // stage inputs keep the caller's values in ghost cells no box covers
-  if ((rc = copy_uncovered(bt, dst, dw[0], dw[1]))) return rc;
+  // stage inputs keep the caller's values in ghost cells no box covers:
+  // read from U^n by the fused stages, else copied into the workspaces
+  bool direct = false;
+  if (nst > 1 && bt->nuncov_cells > 0 && (rc = uncovered_from_un(bt, sch, dst, dw, &direct))) return rc;
+  if (!direct && (rc = copy_uncovered(bt, dst, dw[0], dw[1]))) return rc;
   for (int s = 0; s < nst; ++s) {
     if ((rc = stage_coeffs(sch.order_rk, sch.stages_rk, dt, s, &nst, &sc))) return rc;
     DescSet* sin = s == 0 ? dst : dw[(s - 1) % 2];
-    DescSet* out = (s == nst - 1 && s > 0) ? dst : dw[s % 2];
+    DescSet* out = (s == nst - 1 && s > 0 && !scratch_out) ? dst : dw[s % 2];
     // a periodic domain-wide array needs no fill on the fused path
-    if (!(fused_path(bt, sch, nullptr) && wrap_ok(bt, sin)))
+    if (!(fused_path(bt, sch, nullptr) && wrap_ok(bt, sin)) && !(scratch_out && s == 0))
       if ((rc = enqueue_fill(bt, sin))) return rc;
     StageArgs sa{sc, d_box_dt, sch.order_rk, sch.stages_rk, s};
-    if ((rc = do_stage(bt, cls, sch, sa, dst, sin, out))) return rc;
+    if ((rc = do_stage(bt, cls, sch, sa, dst, sin, out, direct && s > 0 ? dst->d : nullptr))) return rc;
   }
-  if (nst == 1) {
+  if (nst == 1 && !scratch_out) {
     // single-stage schemes wrote into workspace 0; copy its valid cells back
     HIPCHK(launch_copy_valid(bt->bt, dw[0]->d, dst->d, d_box_dt, bt->ctx->stream));
   }
@@ -1539,6 +1668,7 @@ int finish_boxtab(amr_boxtab_s* bt) {
   if (e != hipSuccess) return fail(AMR_ERR_HIP, std::string("box table allocation: ") + hipGetErrorString(e));
   rc = build_uncovered(bt);
   if (rc == AMR_OK) rc = build_cover(bt);
+  if (rc == AMR_OK) rc = build_srcmap(bt);
   if (rc == AMR_OK) rc = build_walls(bt);
   return rc;
 }
@@ -1557,6 +1687,7 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   if (bt->d_uncov) (void)hipFree(bt->d_uncov);
   if (bt->d_uncov_blk) (void)hipFree(bt->d_uncov_blk);
   if (bt->d_cover_bits) (void)hipFree(bt->d_cover_bits);
+  if (bt->d_srcmap_bits) (void)hipFree(bt->d_srcmap_bits);
   if (bt->d_all_covered) (void)hipFree(bt->d_all_covered);
   if (bt->d_wall_tasks) (void)hipFree(bt->d_wall_tasks);
   if (bt->d_wall_blk) (void)hipFree(bt->d_wall_blk);
@@ -1723,6 +1854,43 @@ int amr_set_tiling(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, int t
   return AMR_OK;
 }
 
+int amr_set_box_tiling(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, int tz) {
+  if (!ctx || !sch) return fail(AMR_ERR_ARG, "null argument");
+  if (tz <= 0) return fail(AMR_ERR_ARG, "tz must be positive");
+  if (!fused_supported(*sch, tx, ty))
+    return fail(AMR_ERR_ARG, "no fused kernel compiled for this scheme and tile shape");
+  ctx->tuned[scheme_key(*sch)] = FusedConfig{tx, ty, tz, 1};
+  return AMR_OK;
+}
+
+int amr_step_plan(amr_boxtab_t bt, const amr_scheme_t* sch, const amr_fab_t* state, int32_t info[5]) {
+  if (!bt || !sch || !state || !info) return fail(AMR_ERR_ARG, "null argument");
+  int rc;
+  if ((rc = check_scheme(*sch))) return rc;
+  if ((rc = ensure_ws(bt, state))) return rc;
+  DescSet *dst = nullptr, *dw[2] = {nullptr, nullptr};
+  if ((rc = get_desc(bt, state, NGHOST, &dst))) return rc;
+  for (int w = 0; w < 2; ++w)
+    if ((rc = get_desc(bt, bt->ws_fabs[w].data(), NGHOST, &dw[w]))) return rc;
+  const bool level = use_level_tiles(bt, *sch, dst, dst, dw[0]);
+  FusedConfig cfg;
+  const bool fused = fused_path(bt, *sch, &cfg, level);
+  info[0] = fused ? cfg.tx : -1;
+  info[1] = fused ? cfg.ty : 0;
+  info[2] = fused ? cfg.tz : 0;
+  info[3] = fused ? (level ? 2 : 1) : 0;
+  info[4] = 0;
+  int nst = 0;
+  StageCoef sc;
+  if ((rc = stage_coeffs(sch->order_rk, sch->stages_rk, 1.0, 0, &nst, &sc))) return rc;
+  if (fused && nst > 1 && bt->nuncov_cells > 0) {
+    bool ok = false;
+    if ((rc = uncovered_from_un(bt, *sch, dst, dw, &ok))) return rc;
+    info[4] = ok ? 1 : 0;
+  }
+  return AMR_OK;
+}
+
 int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
                  int32_t tiling[3]) {
   if (!bt || !cls || !sch || !state || !tiling) return fail(AMR_ERR_ARG, "null argument");
@@ -1733,20 +1901,22 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
   if ((rc = ensure_ws(bt, state))) return rc;
   DescSet *din = nullptr, *dout = nullptr;
   if ((rc = get_desc(bt, state, NGHOST, &din))) return rc;
-  if ((rc = get_desc(bt, bt->ws_fabs[0].data(), 0, &dout))) return rc;
-  const bool level = use_level_tiles(bt, *sch, nullptr, din, dout);
+  if ((rc = get_desc(bt, bt->ws_fabs[0].data(), NGHOST, &dout))) return rc;
+  // level tiles and box tiles both where level tiles apply
+  const bool level = level_possible(bt, *sch, din, din, dout);
   int maxz = 0;
   for (auto& B : bt->bt.h_boxes) maxz = std::max(maxz, B.hi[2] - B.lo[2] + 1);
-  if (level) maxz = bt->cover.hi[2] - bt->cover.lo[2] + 1;  // level tiles run across boxes
+  const int maxz_level = level ? bt->cover.hi[2] - bt->cover.lo[2] + 1 : 0;  // level tiles run across boxes
   std::vector<FusedConfig> cand;
   const int shapes[3][2] = {{16, 16}, {32, 8}, {8, 8}};
   const int tzs[6] = {16, 24, 32, 48, 64, 128};
-  for (auto& sh : shapes)
-    for (int tz : tzs) {
-      FusedConfig c{sh[0], sh[1], tz};
-      if (tz > 2 * maxz && tz != 16) continue;
-      if (fused_supported(*sch, c.tx, c.ty) && fused_fits(bt, c, level)) cand.push_back(c);
-    }
+  for (int box_tiles = level ? 0 : 1; box_tiles <= 1; ++box_tiles)
+    for (auto& sh : shapes)
+      for (int tz : tzs) {
+        FusedConfig c{sh[0], sh[1], tz, box_tiles};
+        if (tz > 2 * (box_tiles ? maxz : maxz_level) && tz != 16) continue;
+        if (fused_supported(*sch, c.tx, c.ty) && fused_fits(bt, c, !box_tiles)) cand.push_back(c);
+      }
   if (cand.empty()) {
     bt->tuned[key] = FusedConfig{-1, 0, 0};
     tiling[0] = -1;
@@ -1759,29 +1929,46 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
   HIPCHK(hipEventCreate(&e1));
   float best = 1e30f;
   FusedConfig bestc = cand[0];
-  StageArgs sa{StageCoef{2, 0, 0, 0, 0}, nullptr, 0, 0, 0};
-  // untimed stages first, so that the first candidates are not timed while
-  // the clocks ramp up
+  // whole steps (fills and the uncovered-cell copy included) that write only
+  // the workspaces; untimed steps first, so that the first candidates are
+  // not timed while the clocks ramp up
+  const double dt_probe = 1e-12;
   bt->tuned[key] = cand[0];
-  for (int rep = 0; rep < 6; ++rep)
-    if ((rc = do_stage(bt, *cls, *sch, sa, nullptr, din, dout))) return rc;
-  for (auto& cfg : cand) {
+  for (int rep = 0; rep < 2; ++rep)
+    if ((rc = advance_impl(bt, *cls, *sch, state, dt_probe, nullptr, true))) return rc;
+  // best of three timed steps per candidate, then the three fastest again,
+  // interleaved, so that a clock or thermal drift does not pick the winner
+  auto time_cfg = [&](const FusedConfig& cfg, int reps, float* out) -> int {
     bt->tuned[key] = cfg;
     float tbest = 1e30f;
-    for (int rep = 0; rep < 4; ++rep) {
+    for (int rep = 0; rep < reps + 1; ++rep) {
       HIPCHK(hipEventRecord(e0, st));
-      if ((rc = do_stage(bt, *cls, *sch, sa, nullptr, din, dout))) return rc;
+      int r = advance_impl(bt, *cls, *sch, state, dt_probe, nullptr, true);
+      if (r) return r;
       HIPCHK(hipEventRecord(e1, st));
       HIPCHK(hipEventSynchronize(e1));
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, e0, e1));
       if (rep > 0) tbest = std::min(tbest, ms);
     }
-    if (tbest < best) {
-      best = tbest;
-      bestc = cfg;
+    *out = std::min(*out, tbest);
+    return AMR_OK;
+  };
+  std::vector<float> t(cand.size(), 1e30f);
+  for (size_t i = 0; i < cand.size(); ++i)
+    if ((rc = time_cfg(cand[i], 3, &t[i]))) return rc;
+  std::vector<size_t> order(cand.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return t[a] < t[b]; });
+  order.resize(std::min<size_t>(3, order.size()));
+  for (int round = 0; round < 2; ++round)
+    for (size_t i : order)
+      if ((rc = time_cfg(cand[i], 3, &t[i]))) return rc;
+  for (size_t i : order)
+    if (t[i] < best) {
+      best = t[i];
+      bestc = cand[i];
     }
-  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   bt->tuned[key] = bestc;

@@ -117,10 +117,10 @@ __device__ __forceinline__ void zterms(const double q1[NPRIM], const double q2[N
                 cv * q2[QT], q2[QPRES], t);
 }
 
-template <int H, bool VISC, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ tiles, const BoxDev* __restrict__ boxes,
+template <int H, bool VISC, int TX, int TY, bool UCS>
+__global__ void __launch_bounds__(TX* TY, 2) k_fused(const TileDev* __restrict__ tiles, const BoxDev* __restrict__ boxes,
                                                      Closures c, double co0, double co1, double co2, StageArgs sa,
-                                                     WrapArgs wa, Cover cov, const Fab* __restrict__ s1f,
+                                                     WrapArgs wa, Cover cov, UncovSrc us, const Fab* __restrict__ s1f,
                                                      const Fab* __restrict__ sinf, const Fab* __restrict__ outf) {
   using C = Cfg<H, VISC, TX, TY>;
   constexpr int RX = C::RX, HX = C::HX, PL = C::PL, NT = C::NT;
@@ -129,6 +129,7 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
   __shared__ double s_py[VISC ? 4 : 1][TY + 2][TX];
   __shared__ double s_fx[5][TY][TX + 1];
   __shared__ double s_fy[5][TY + 1][TX];
+  __shared__ unsigned s_um[UCS ? C::NLOAD * NT : 1];
 
   const TileDev T = tiles[blockIdx.x];
   if (T.nz <= 0 || !box_active(sa, T.box)) return;  // uniform: the whole workgroup leaves
@@ -194,6 +195,33 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
     }
     loff[s] = x + y * int(S.js);
   }
+  // ghost cells no box covers come from U^n (UCS): bit b of the lane's mask
+  // for load cell s (in LDS, which has room where registers have not) is
+  // set when the cell is such a ghost cell in plane block zb0 + b; at most
+  // UNCOV_MAX_ZBLK blocks per tile
+  const int zb0 = (kfirst - us.lo[2]) >> us.sh;
+  const gdouble* u0base = sbase;
+  if (UCS) {
+    const Fab U0 = us.u0[T.box];
+    u0base = gptr(U0.p) + U0.off;
+    const int zb1 = (k0 + nz + 1 - us.lo[2]) >> us.sh;
+    guint* ubits = (guint*)us.bits;
+#pragma unroll
+    for (int s = 0; s < C::NLOAD; ++s) {
+      const int cell = min(tid + s * NT, C::NREG - 1);
+      const int bx = (min(T.i0 - HX + cell % RX, xcap) - us.lo[0]) >> us.sh;
+      const int by = (min(T.j0 - HX + cell / RX, ycap) - us.lo[1]) >> us.sh;
+      const bool inxy = bx >= 0 && bx < us.nb[0] && by >= 0 && by < us.nb[1];
+      unsigned m = 0u;
+      for (int zb = zb0; zb <= zb1; ++zb) {
+        if (inxy && zb >= 0 && zb < us.nb[2]) {
+          const int bit = (zb * us.nb[1] + by) * us.nb[0] + bx;
+          if ((ubits[bit >> 5] >> (bit & 31)) & 1u) m |= 1u << (zb - zb0);
+        }
+      }
+      s_um[s * NT + tid] = m;
+    }
+  }
   // own column for the epilogue loads, clamped into the valid region (s1
   // may have no ghost cells): overhanging lanes load there and never store
   const int soff = min(gi, xhi) + min(gj, yhi) * int(S.js);
@@ -202,11 +230,22 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
   double pre[C::NLOAD][5];
   auto load_plane = [&](int p) {
     if (wa.on) p = p < wa.lo[2] ? p + wa.len[2] : (p >= wa.lo[2] + wa.len[2] ? p - wa.len[2] : p);
-    const gdouble* pb = sbase + int64_t(p) * S.ks;
+    if (UCS) {
+      const int b = ((p - us.lo[2]) >> us.sh) - zb0;
 #pragma unroll
-    for (int s = 0; s < C::NLOAD; ++s)
+      for (int s = 0; s < C::NLOAD; ++s) {
+        const bool fromu0 = (s_um[s * NT + tid] >> b) & 1u;
+        const gdouble* pb = (fromu0 ? u0base : sbase) + int64_t(p) * S.ks;
 #pragma unroll
-      for (int n = 0; n < 5; ++n) pre[s][n] = pb[n * S.ns + loff[s]];
+        for (int n = 0; n < 5; ++n) pre[s][n] = pb[n * S.ns + loff[s]];
+      }
+    } else {
+      const gdouble* pb = sbase + int64_t(p) * S.ks;
+#pragma unroll
+      for (int s = 0; s < C::NLOAD; ++s)
+#pragma unroll
+        for (int n = 0; n < 5; ++n) pre[s][n] = pb[n * S.ns + loff[s]];
+    }
   };
   auto store_plane = [&](int p) {
     double* sq = s_q[p & 1];
@@ -264,9 +303,10 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
     }
     const unsigned cword = cbits[(max((ep - cov.lo[2]) >> cov.sh, 0) * cplane + ccol) >> 5];
     // (a) next plane into LDS, prefetch the one after (the last iteration
-    // prefetches plane k0+nz+2, still inside the ghost layers)
+    // has nothing left to prefetch and reloads plane k0+nz+1: every plane a
+    // tile reads lies within two planes of its own)
     store_plane(pn);
-    load_plane(pn + 1);
+    load_plane(min(pn + 1, k0 + nz + 1));
     const double* sq = s_q[cpl & 1];        // plane c
     const double* sqn = s_q[(cpl + 1) & 1];  // plane c+1 (stored above)
 #pragma unroll
@@ -505,23 +545,27 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
 
 template <int H, bool VISC, int TX, int TY>
 void launch_t(const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c, const double co[3],
-              const StageArgs& sa, const WrapArgs& wa, const Cover& cov, const Fab* s1, const Fab* sin,
-              const Fab* out, hipStream_t st) {
-  hipLaunchKernelGGL((k_fused<H, VISC, TX, TY>), dim3(ntiles), dim3(TX * TY), 0, st, tiles, boxes, c, co[0], co[1],
-                     co[2], sa, wa, cov, s1, sin, out);
+              const StageArgs& sa, const WrapArgs& wa, const Cover& cov, const UncovSrc& us, const Fab* s1,
+              const Fab* sin, const Fab* out, hipStream_t st) {
+  if (us.on)
+    hipLaunchKernelGGL((k_fused<H, VISC, TX, TY, true>), dim3(ntiles), dim3(TX * TY), 0, st, tiles, boxes, c, co[0],
+                       co[1], co[2], sa, wa, cov, us, s1, sin, out);
+  else
+    hipLaunchKernelGGL((k_fused<H, VISC, TX, TY, false>), dim3(ntiles), dim3(TX * TY), 0, st, tiles, boxes, c, co[0],
+                       co[1], co[2], sa, wa, cov, us, s1, sin, out);
 }
 
 template <int TX, int TY>
 bool launch_shape(const amr_scheme_t& s, const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c,
-                  const double co[3], const StageArgs& sa, const WrapArgs& wa, const Cover& cov, const Fab* s1,
-                  const Fab* sin, const Fab* out, hipStream_t st) {
+                  const double co[3], const StageArgs& sa, const WrapArgs& wa, const Cover& cov,
+                  const UncovSrc& us, const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
   const bool v = s.rhs_visc != 0;
   if (s.order_keep == 4) {
-    if (v) launch_t<2, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, s1, sin, out, st);
-    else launch_t<2, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, s1, sin, out, st);
+    if (v) launch_t<2, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    else launch_t<2, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
   } else if (s.order_keep == 2) {
-    if (v) launch_t<1, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, s1, sin, out, st);
-    else launch_t<1, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, s1, sin, out, st);
+    if (v) launch_t<1, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    else launch_t<1, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
   } else {
     return false;
   }
@@ -538,14 +582,18 @@ bool fused_supported(const amr_scheme_t& s, int tx, int ty) {
 }
 
 hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const WrapArgs& wa, const Cover& cov,
-                              const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c,
-                              const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
+                              const UncovSrc& us, const TileDev* tiles, int ntiles, const BoxDev* boxes,
+                              const Closures& c, const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* out,
+                              hipStream_t st) {
   double co[3];
   keep_coefficients(s.order_keep, co);
   bool ok = false;
-  if (cfg.tx == 16 && cfg.ty == 16) ok = launch_shape<16, 16>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, s1, sin, out, st);
-  else if (cfg.tx == 8 && cfg.ty == 8) ok = launch_shape<8, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, s1, sin, out, st);
-  else if (cfg.tx == 32 && cfg.ty == 8) ok = launch_shape<32, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, s1, sin, out, st);
+  if (cfg.tx == 16 && cfg.ty == 16)
+    ok = launch_shape<16, 16>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+  else if (cfg.tx == 8 && cfg.ty == 8)
+    ok = launch_shape<8, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+  else if (cfg.tx == 32 && cfg.ty == 8)
+    ok = launch_shape<32, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
   if (!ok) return hipErrorInvalidValue;
   return hipGetLastError();
 }

@@ -220,11 +220,30 @@ struct TileDev {
   int nz;          // planes in this tile
 };
 struct FusedConfig {
-  int tx, ty, tz;  // tile extents
+  int tx, ty, tz;      // tile extents
+  int box_tiles = 0;   // 1: tiles per box even where level tiles would apply
 };
+// Ghost cells of a level that no box covers (coarse-fine ghost cells) keep
+// the caller's values of U^n through a step.  Where every stage array is one
+// array over the level, stages after the first read those cells straight
+// from U^n instead of a copy in their input: bit set in `bits` (blocks of
+// 2^sh cells from `lo`, nb blocks per direction, x fastest) = the block's
+// cells are such ghost cells (periodic images resolved; physical-boundary
+// and covered cells are clear).  u0: U^n's FABs, strides equal to the stage
+// input's.  on = 0: every cell comes from the stage input.
+struct UncovSrc {
+  int on;
+  int lo[3], nb[3], sh;
+  const unsigned* bits;
+  const Fab* u0;
+};
+// Plane blocks a tile's loaded planes may span (one bit each of a per-lane
+// mask).
+constexpr int UNCOV_MAX_ZBLK = 32;
 bool fused_supported(const amr_scheme_t& s, int tx, int ty);
 hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const WrapArgs& wa, const Cover& cov,
-                              const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c,
-                              const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st);
+                              const UncovSrc& us, const TileDev* tiles, int ntiles, const BoxDev* boxes,
+                              const Closures& c, const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* out,
+                              hipStream_t st);
 
 }  // namespace amr

@@ -258,6 +258,20 @@ class Level:
         -1 = pass-structured kernels)."""
         call("amr_set_tiling", self.ctx.h, ctypes.byref(self.scheme), tx, ty, tz)
 
+    def set_box_tiling(self, tx, ty, tz):
+        """Force the fused-kernel tiling with tiles per box, also on a
+        level-wide array."""
+        call("amr_set_box_tiling", self.ctx.h, ctypes.byref(self.scheme), tx, ty, tz)
+
+    def step_plan(self, S):
+        """How a step on S runs: tile, tile kind ("passes", "boxes" or
+        "level") and whether the stages after the first read the ghost
+        cells no box covers straight from U^n (no copy pass)."""
+        out = (ctypes.c_int32 * 5)()
+        call("amr_step_plan", self.h, ctypes.byref(self.scheme), S.fabs, out)
+        return {"tiling": (out[0], out[1], out[2]), "tiles": ("passes", "boxes", "level")[out[3]],
+                "uncovered_from_un": bool(out[4])}
+
     def autotune(self, S):
         out = (ctypes.c_int32 * 3)()
         call("amr_autotune", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, out)

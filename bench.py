@@ -418,7 +418,7 @@ def amr_timing(a, n_base=128, steps=5, reps=3):
             best = min(best, time.perf_counter() - t0)
         ncell = sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) for lo, hi in lev.boxes)
         ms = best / steps * 1e3
-        out = {"boxes": len(boxes), "cells": ncell, "tiling": tiling, "ms_per_step": ms,
+        out = {"boxes": len(boxes), "cells": ncell, "tiling": tiling, "plan": lev.step_plan(S), "ms_per_step": ms,
                "ns_per_cell_stage": ms * 1e6 / (3 * ncell)}
         lev.close()
         del S

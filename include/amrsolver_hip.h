@@ -188,11 +188,26 @@ int amr_batch_est_dt(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t
  * counterpart: AMReX picks launch shapes itself (ParallelFor). */
 int amr_set_tiling(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, int tz);
 
-/* Time every compiled tiling that fits the level on `state` (ghost-filled,
- * ng >= 3) and keep the fastest for this scheme on this box table (it
- * clears an explicit amr_set_tiling for the scheme).  Writes the
- * choice to tiling[3] (tx,ty,tz; tx = -1 when only the pass-structured
- * kernels apply).  Synchronous; leaves `state` unchanged. */
+/* As amr_set_tiling with tx > 0, but the fused kernel keeps to tiles per box
+ * also where the state is one array over the level (no reference
+ * counterpart). */
+int amr_set_box_tiling(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, int tz);
+
+/* How a step of `sch` on `state` runs under the tiling in force: info[0..2]
+ * the fused kernel's tile (tx,ty,tz; tx = -1: pass-structured kernels),
+ * info[3] 0 pass-structured, 1 tiles per box, 2 tiles over the union of the
+ * boxes, info[4] 1 when the stages after the first read the ghost cells no
+ * box covers straight from U^n (else a copy pass per step into the stage
+ * workspaces, where there are such cells).  No reference counterpart. */
+int amr_step_plan(amr_boxtab_t bt, const amr_scheme_t* sch, const amr_fab_t* state, int32_t info[5]);
+
+/* Time a whole step (fills and stages, the last one written to a workspace)
+ * on `state` (ghost-filled, ng >= 3) under every compiled tiling that fits
+ * the level -- tiles per box and, where the state is one array over the
+ * level, tiles over the union of the boxes -- and keep the fastest for this
+ * scheme on this box table (it clears an explicit amr_set_tiling for the
+ * scheme).  Writes the choice to tiling[3] (tx,ty,tz; tx = -1 when only the
+ * pass-structured kernels apply).  Synchronous; leaves `state` unchanged. */
 int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch,
                  const amr_fab_t* state, int32_t tiling[3]);
 

@@ -133,8 +133,10 @@ def _irregular_boxes(n, seed=7, keep=0.7):
 @pytest.mark.parametrize("case", ["irregular", "hierarchy"])
 def test_level_tiling_matches_oracle(ctx, hierarchy, case):
     """Level layout, fused kernel with tiles laid over the union of the boxes
-    (crossing box boundaries; only covered cells written) for several tile
-    shapes: every tiling agrees with the oracle to 1e-12 and bit for bit with
+    (crossing box boundaries; only covered cells written) and with tiles per
+    box, for several tile shapes, the stages after the first reading the
+    ghost cells no box covers from U^n where their tiles allow (else a copy
+    pass): every tiling agrees with the oracle to 1e-12 and bit for bit with
     every other tiling."""
     from amrsolver_amd import level as L
     from oracle import oracle as O
@@ -157,18 +159,29 @@ def test_level_tiling_matches_oracle(ctx, hierarchy, case):
     for _ in range(2):
         ol.advance(cls, sch, dt, nthreads=8)
     want = ol.get_state()
-    results = []
-    for tiling in [(16, 16, 16), (16, 16, 128), (32, 8, 32), (8, 8, 64), (8, 8, 16)]:
-        print(f"case level tiling {case} boxes {len(boxes)} tiling {tiling}", flush=True)
-        _boxes_state(lev, S, cls, g)
-        lev.fill_patch(S)
-        lev.set_tiling(*tiling)
-        for _ in range(2):
-            lev.advance(S, 0.0, dt)
-        got = _covered(S.download(), boxes)
-        err = rel_err(got, want)
-        assert err <= 1e-12, f"{case} {tiling}: {err:.3e}"
-        results.append(got)
+    results, in_kernel = [], set()
+    for kind in ("level", "boxes"):
+        for tiling in [(16, 16, 16), (16, 16, 128), (32, 8, 32), (8, 8, 64), (8, 8, 16)]:
+            print(f"case level tiling {case} boxes {len(boxes)} tiles {kind} tiling {tiling}", flush=True)
+            _boxes_state(lev, S, cls, g)
+            lev.fill_patch(S)
+            if kind == "level":
+                lev.set_tiling(*tiling)
+            else:
+                lev.set_box_tiling(*tiling)
+            plan = lev.step_plan(S)
+            if plan["tiles"] == "passes":  # tiles per box too idle for the fused kernel
+                assert kind == "boxes"
+                continue
+            assert plan["tiles"] == kind and plan["tiling"] == tiling, plan
+            in_kernel.add(plan["uncovered_from_un"])
+            for _ in range(2):
+                lev.advance(S, 0.0, dt)
+            got = _covered(S.download(), boxes)
+            err = rel_err(got, want)
+            assert err <= 1e-12, f"{case} {kind} {tiling}: {err:.3e}"
+            results.append(got)
     lev.set_tiling(0)
     for r in results[1:]:
         assert np.array_equal(r, results[0])
+    assert True in in_kernel
