@@ -33,7 +33,7 @@ DEVICE = [
     "-munsafe-fp-atomics",
 ]
 
-SOURCES = ["capi.cpp", "unfused.hip", "fused.hip", "fill_reduce.hip", "adjoint.hip"]
+SOURCES = ["capi.cpp", "unfused.hip", "fused.hip", "fill_reduce.hip", "adjoint.hip", "dir_flux.hip"]
 
 
 def _compile(src):

@@ -459,11 +459,13 @@ int build_fill(amr_boxtab_s* bt, DescSet* ds) {
   return AMR_OK;
 }
 
-int enqueue_fill(amr_boxtab_s* bt, DescSet* ds) {
+// bc_only: physical boundaries only (the periodic copies of a batch, whose
+// problems are single boxes, when the stage reads periodic images itself).
+int enqueue_fill(amr_boxtab_s* bt, DescSet* ds, bool bc_only = false) {
   hipStream_t st = bt->ctx->stream;
   if (bt->walls && !bt->has_problem)
     return fail(AMR_ERR_ARG, "bc code 6 walls need amr_set_problem (wall temperature) first");
-  HIPCHK(launch_fill_copy(ds->d_copy, ds->d_copy_blk, ds->ncopy_blk, ds->d, st));
+  if (!bc_only) HIPCHK(launch_fill_copy(ds->d_copy, ds->d_copy_blk, ds->ncopy_blk, ds->d, st));
   for (int d = 0; d < 3; ++d)
     HIPCHK(launch_fill_bc(ds->d_bc[d], ds->d_bc_blk[d], ds->nbc_blk[d], ds->d, bt->bt.d_boxes, bt->wall_fill, st));
   return AMR_OK;
@@ -709,8 +711,33 @@ int ensure_unfused(amr_boxtab_s* bt, const amr_scheme_t& s) {
     bt->uw.lam = tables[t++];
   }
   if (need & UW_SLOPE) bt->uw.slope = tables[t++];
+  if (need & (UW_WENO | UW_SLOPE)) {
+    for (int d = 0; d < 3; ++d) {
+      // a box spanning a periodic direction is its own periodic image there
+      std::vector<int> wrap(size_t(nb), 0);
+      for (int b = 0; b < nb; ++b) {
+        const amr_geom_t& g = bt->geoms[b];
+        const BoxDev& B = bt->bt.h_boxes[b];
+        if (g.periodic[d] && B.lo[d] == g.dom_lo[d] && B.hi[d] == g.dom_hi[d]) wrap[b] = B.hi[d] - B.lo[d] + 1;
+      }
+      const int n = weno_brick_count(bt->bt, d);
+      std::vector<WenoBrick> h(size_t(std::max(n, 1)));
+      weno_bricks(bt->bt, d, wrap.data(), h.data());
+      WenoBrick* dv = nullptr;
+      HIPCHK(hipMalloc(&dv, sizeof(WenoBrick) * h.size()));
+      HIPCHK(hipMemcpy(dv, h.data(), sizeof(WenoBrick) * h.size(), hipMemcpyHostToDevice));
+      bt->uw_tables.push_back(dv);
+      bt->uw.weno.d[d] = dv;
+      bt->uw.weno.n[d] = n;
+    }
+  }
   bt->uw_kinds = need;
   return AMR_OK;
+}
+
+// Schemes the directional flux passes run (unfused.hip).
+bool dir_flux_scheme(const amr_scheme_t& s) {
+  return s.rhs_euler && (s.flux_euler == 2 || s.flux_euler == 0) && !s.rhs_visc;
 }
 
 std::string scheme_key(const amr_scheme_t& s) {
@@ -1267,9 +1294,12 @@ int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t
     if ((rc = stage_coeffs(sch.order_rk, sch.stages_rk, dt, s, &nst, &sc))) return rc;
     DescSet* sin = s == 0 ? dst : dw[(s - 1) % 2];
     DescSet* out = (s == nst - 1 && s > 0 && !scratch_out) ? dst : dw[s % 2];
-    // a periodic domain-wide array needs no fill on the fused path
-    if (!(fused_path(bt, sch, nullptr) && wrap_ok(bt, sin)) && !(scratch_out && s == 0))
-      if ((rc = enqueue_fill(bt, sin))) return rc;
+    // a periodic domain-wide array needs no fill on the fused path; the
+    // directional flux passes read the periodic images of a batch's problems
+    // themselves
+    const bool fused = fused_path(bt, sch, nullptr);
+    if (!(fused && wrap_ok(bt, sin)) && !(scratch_out && s == 0))
+      if ((rc = enqueue_fill(bt, sin, bt->batch && !fused && dir_flux_scheme(sch)))) return rc;
     StageArgs sa{sc, d_box_dt, sch.order_rk, sch.stages_rk, s};
     if ((rc = do_stage(bt, cls, sch, sa, dst, sin, out, direct && s > 0 ? dst->d : nullptr))) return rc;
   }

@@ -281,6 +281,71 @@ __device__ __forceinline__ void riemann_prob(double gamma, double smallp, double
                     pgdnv / (gamma - 1.) + pgdnv);
 }
 
+// Characteristic slopes of a cell in direction D (src/rhs/Riemann.h:19-120
+// cns_slope_{x,y,z}): Q(o, comp) is primitive `comp` of the cell o steps
+// along D (o = -1, 0, 1); the x slopes add 2e-40 to the sound speed, y and z
+// 1e-40, as the reference does.
+template <int D, class QF>
+__device__ __forceinline__ void muscl_slopes(const QF& Q, const Closures& c, double dq[5]) {
+  constexpr int qn = QU + D;
+  constexpr int qt1 = D == 0 ? QV : QU;
+  constexpr int qt2 = D == 2 ? QV : QW;
+  const double add = D == 0 ? 2.e-40 : 1.e-40;
+  double cspeed = sqrt(c.gamma * c.Rspec * Q(0, QT)) + add;
+  double dlft = 0.5 * (Q(0, QPRES) - Q(-1, QPRES)) / cspeed - 0.5 * Q(0, QRHO) * (Q(0, qn) - Q(-1, qn));
+  double drgt = 0.5 * (Q(1, QPRES) - Q(0, QPRES)) / cspeed - 0.5 * Q(0, QRHO) * (Q(1, qn) - Q(0, qn));
+  dq[0] = limiter(dlft, drgt);
+  double cs2 = cspeed * cspeed;
+  dlft = (Q(0, QRHO) - Q(-1, QRHO)) - (Q(0, QPRES) - Q(-1, QPRES)) / cs2;
+  drgt = (Q(1, QRHO) - Q(0, QRHO)) - (Q(1, QPRES) - Q(0, QPRES)) / cs2;
+  dq[1] = limiter(dlft, drgt);
+  dlft = 0.5 * (Q(0, QPRES) - Q(-1, QPRES)) / cspeed + 0.5 * Q(0, QRHO) * (Q(0, qn) - Q(-1, qn));
+  drgt = 0.5 * (Q(1, QPRES) - Q(0, QPRES)) / cspeed + 0.5 * Q(0, QRHO) * (Q(1, qn) - Q(0, qn));
+  dq[2] = limiter(dlft, drgt);
+  dq[3] = limiter(Q(0, qt1) - Q(-1, qt1), Q(1, qt1) - Q(0, qt1));
+  dq[4] = limiter(Q(0, qt2) - Q(-1, qt2), Q(1, qt2) - Q(0, qt2));
+}
+
+// Face flux in direction D from the left cell's primitives ql and slopes dql,
+// the right cell's qr and dqr, and Tl, the temperature the left state's
+// sound speed is taken from (the left cell's, except in z, where the
+// reference reads T at k+1: Riemann.h:316); src/rhs/Riemann.h:251-337
+// cns_riemann_{x,y,z}.  f[] is in conserved order (the transverse momentum
+// components permuted back per direction).
+template <int D, class QL, class DL, class QR, class DR>
+__device__ __forceinline__ void muscl_face(const QL& ql, const DL& dql, const QR& qr, const DR& dqr, double Tl,
+                                           const Closures& c, double f[5]) {
+  constexpr int qn = QU + D;
+  constexpr int qt1 = D == 0 ? QV : QU;
+  constexpr int qt2 = D == 2 ? QV : QW;
+  double cspeed = sqrt(c.gamma * c.Rspec * Tl) + 1.e-40;
+  double rl = ql(QRHO) + 0.5 * ((dql(0) + dql(2)) / cspeed + dql(1));
+  rl = dmax(rl, 1.0e-40);
+  double ul = ql(qn) + 0.5 * ((dql(2) - dql(0)) / ql(QRHO));
+  double pl = ql(QPRES) + 0.5 * (dql(0) + dql(2)) * cspeed;
+  pl = dmax(pl, 1.0e-40);
+  double ut1l = ql(qt1) + 0.5 * dql(3);
+  double ut2l = ql(qt2) + 0.5 * dql(4);
+  cspeed = sqrt(c.gamma * c.Rspec * qr(QT)) + 1.e-40;
+  double rr = qr(QRHO) - 0.5 * ((dqr(0) + dqr(2)) / cspeed + dqr(1));
+  rr = dmax(rr, 1.0e-40);
+  double ur = qr(qn) - 0.5 * ((dqr(2) - dqr(0)) / qr(QRHO));
+  double pr_ = qr(QPRES) - 0.5 * (dqr(0) + dqr(2)) * cspeed;
+  pr_ = dmax(pr_, 1.0e-40);
+  double ut1r = qr(qt1) - 0.5 * dqr(3);
+  double ut2r = qr(qt2) - 0.5 * dqr(4);
+  double o[5];
+  riemann_prob(c.gamma, 1.0e-40, rl, ul, pl, ut1l, ut2l, rr, ur, pr_, ut1r, ut2r, o);
+  constexpr int cn = UMX + D;
+  constexpr int ct1 = D == 0 ? UMY : UMX;
+  constexpr int ct2 = D == 2 ? UMY : UMZ;
+  f[URHO] = o[0];
+  f[cn] = o[1];
+  f[ct1] = o[2];
+  f[ct2] = o[3];
+  f[UET] = o[4];
+}
+
 // Stage-combination epilogue (CNS::advance MultiFab ops, src/tim/CNS_advance.cpp:51-181).
 struct StageCoef {
   int mode;  // 0: a*s1 + b*(c*sin + d*L); 1: (a*s1 + c*sin) + d*L; 2: L

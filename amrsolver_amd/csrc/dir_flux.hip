@@ -1,0 +1,347 @@
+// Euler fluxes without viscous terms, one pass per direction: WENO5-JS with
+// global Lax-Friedrichs splitting (flux_euler = 2) and MUSCL slopes with the
+// two-shock Riemann solver (flux_euler = 0).
+//
+// The reference runs each scheme as full-box passes with every intermediate
+// in HBM: cons2eulerflux_lambda over the grown box and one
+// numericalflux_globallaxsplit pass per direction (High_resolution.h:83-319),
+// or primitives over the grown box and, per direction, cns_slope and
+// cns_riemann passes (Riemann.h:19-408), then the flux assembly
+// (CNS_advance.cpp:375-391).  Here a workgroup owns a brick of 64 cells along
+// x by 4 along a second axis and stages the pencil of cells its faces need
+// (ghost cells along the pass direction only) in LDS:
+// - WENO: conserved state, normal velocity, pressure and the direction's wave
+//   speed per cell; each thread reconstructs one or two faces (the 6-cell
+//   maximum of the wave speed, the split fluxes of the 6 cells, two WENO5-JS
+//   evaluations per component) from LDS;
+// - MUSCL: primitives per cell, then the characteristic slopes of the cells
+//   next to a face, then one Riemann problem per face, all from LDS.
+// After a barrier each thread forms its cell's flux difference.  Pass x
+// writes dxinv_x (F(i) - F(i+1)) to an accumulator, pass y adds its term,
+// pass z adds its term, the source and the stage combination and writes the
+// stage output, so the sum keeps the reference's order (x + y) + z.
+#include "cns_point.h"
+#include "kernels.h"
+
+namespace amr {
+
+namespace {
+
+constexpr int WX = 64, WB = 4, WNT = WX * WB;
+
+// Brick geometry of pass D: loaded cells LX x LB, faces FX x FB (x fastest);
+// the second axis is y for D = 0, 1 and z for D = 2.
+template <int D>
+struct WGeo {
+  static constexpr int LX = D == 0 ? WX + 6 : WX;
+  static constexpr int LB = D == 0 ? WB : WB + 6;
+  static constexpr int NL = LX * LB;
+  static constexpr int FX = D == 0 ? WX + 1 : WX;
+  static constexpr int FB = D == 0 ? WB : WB + 1;
+  static constexpr int NF = FX * FB;
+  static constexpr int NV = 8;  // per cell: 5 conserved, normal velocity, pressure, wave speed
+};
+
+// (x, b, c) brick coordinates -> (i, j, k)
+template <int D>
+__device__ __forceinline__ void brick_ijk(int x, int b, int c, int& i, int& j, int& k) {
+  i = x;
+  if (D == 2) {
+    j = c;
+    k = b;
+  } else {
+    j = b;
+    k = c;
+  }
+}
+
+// Flux difference of the thread's cell from the faces in sF ([5][NF], FX
+// faces per row) and, in the last pass, source and stage combination.
+template <int D, int LAST, int FX, int NF>
+__device__ __forceinline__ void dir_epilogue(const WenoBrick& W, const BoxDev& bx, const StageArgs& sa,
+                                             const double* sF, const Fab& S, const Fab* __restrict__ accf,
+                                             const Fab* __restrict__ s1f, const Fab* __restrict__ outf) {
+  constexpr int BA = D == 2 ? 2 : 1;
+  const int tid = threadIdx.x;
+  // flux difference of the own cell (CNS_advance.cpp:384-390)
+  const int cx = tid % WX, cb = tid / WX;
+  const int x = W.x0 + cx, b = W.b0 + cb;
+  if (x > bx.hi[0] || b > bx.hi[BA]) return;
+  int i, j, k;
+  brick_ijk<D>(x, b, W.c, i, j, k);
+  const int f0 = cb * FX + cx;
+  const int f1 = D == 0 ? f0 + 1 : f0 + FX;
+  const double dxi = bx.dxi[D];
+  const Fab A = accf[W.box];
+  if (!LAST) {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      const double t = dxi * (sF[n * NF + f0] - sF[n * NF + f1]);
+      A(i, j, k, n) = D == 0 ? t : A(i, j, k, n) + t;
+    }
+    return;
+  }
+  const StageCoef sc = box_stage(sa, W.box);
+  const Fab O = outf[W.box];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    double L = A(i, j, k, n) + dxi * (sF[n * NF + f0] - sF[n * NF + f1]);
+    if (sa.src) L = L + sa.src[W.box](i, j, k, n);  // user_source, CNS_advance.cpp:397-407
+    if (sa.bfx != 0.0 && n == UMX) L = L + S(i, j, k, URHO) * sa.bfx;
+    if (sa.bfx != 0.0 && n == UET) L = L + S(i, j, k, UMX) * sa.bfx;
+    double v;
+    if (sc.mode == 2) {
+      v = L;
+    } else {
+      const double a = (sc.a != 0.0) ? s1f[W.box](i, j, k, n) : 0.0;
+      v = stage_combine(sc, a, S(i, j, k, n), L);
+    }
+    O(i, j, k, n) = v;
+  }
+}
+
+template <int D, int LAST>
+__global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ bricks, const BoxDev* __restrict__ boxes,
+                                                  Closures c, StageArgs sa, const Fab* __restrict__ sinf,
+                                                  const Fab* __restrict__ accf, const Fab* __restrict__ s1f,
+                                                  const Fab* __restrict__ outf) {
+  using G = WGeo<D>;
+  __shared__ double sv[G::NV][G::NL];
+  const WenoBrick W = bricks[blockIdx.x];
+  if (!box_active(sa, W.box)) return;  // uniform: the whole workgroup leaves
+  const BoxDev bx = boxes[W.box];
+  const Fab S = sinf[W.box];
+  const int tid = threadIdx.x;
+  constexpr int BA = D == 2 ? 2 : 1;  // axis of the brick's second coordinate
+  const int xmax = bx.hi[0] + (D == 0 ? NGHOST : 0);
+  const int bmax = bx.hi[BA] + (D == 0 ? 0 : NGHOST);
+
+  // 1. stage the cells (High_resolution.h:83-121, the pass direction's part)
+  const int dlo = bx.lo[D];
+  for (int l = tid; l < G::NL; l += WNT) {
+    const int lx = l % G::LX, lb = l / G::LX;
+    int x = W.x0 + lx - (D == 0 ? 3 : 0);
+    int b = W.b0 + lb - (D == 0 ? 0 : 3);
+    if (D == 0 && W.wrap > 0) x = dlo + ((x - dlo) % W.wrap + W.wrap) % W.wrap;
+    else x = min(x, xmax);
+    if (D != 0 && W.wrap > 0) b = dlo + ((b - dlo) % W.wrap + W.wrap) % W.wrap;
+    else b = min(b, bmax);
+    int i, j, k;
+    brick_ijk<D>(x, b, W.c, i, j, k);
+    const double rho = S(i, j, k, URHO), momx = S(i, j, k, UMX), momy = S(i, j, k, UMY);
+    const double momz = S(i, j, k, UMZ), rhoet = S(i, j, k, UET);
+    const double rhoinv = 1.0 / rho;
+    const double ux = momx * rhoinv, uy = momy * rhoinv, uz = momz * rhoinv;
+    const double rhoekin = 0.5 * rhoinv * (momx * momx + momy * momy + momz * momz);
+    const double rhoeint = rhoet - rhoekin;
+    const double P = c.gm1 * rhoeint;
+    const double cs = sqrt(c.gamma * P * rhoinv);
+    const double ud = D == 0 ? ux : (D == 1 ? uy : uz);
+    sv[0][l] = rho;
+    sv[1][l] = momx;
+    sv[2][l] = momy;
+    sv[3][l] = momz;
+    sv[4][l] = rhoet;
+    sv[5][l] = ud;
+    sv[6][l] = P;
+    sv[7][l] = fabs(dmax(dmax(ud + cs, ud - cs), ud));
+  }
+  __syncthreads();
+
+  // 2. faces (numericalflux_globallaxsplit, High_resolution.h:162-246): face
+  // f sits below stencil cell 3 of cells l0 .. l0+5 (stride st along D)
+  constexpr int st = D == 0 ? 1 : G::LX;
+  constexpr int NFT = (G::NF + WNT - 1) / WNT;
+  double F[NFT][5];
+#pragma unroll
+  for (int r = 0; r < NFT; ++r) {
+    const int f = tid + r * WNT;
+    if (f < G::NF) {
+      const int fx = f % G::FX, fb = f / G::FX;
+      const int l0 = fb * G::LX + fx;
+      double me = sv[7][l0];
+#pragma unroll
+      for (int o = 1; o < 6; ++o) me = dmax(me, sv[7][l0 + o * st]);
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        double df[6], Pf[6];
+#pragma unroll
+        for (int o = 0; o < 6; ++o) {
+          const int l = l0 + o * st;
+          const double ud = sv[5][l];
+          double flux;
+          if (n == URHO) flux = sv[1 + D][l];
+          else if (n == UET) flux = (sv[4][l] + sv[6][l]) * ud;
+          else if (n == 1 + D) flux = sv[n][l] * ud + sv[6][l];
+          else flux = sv[n][l] * ud;
+          df[o] = me * sv[n][l];
+          Pf[o] = flux;
+        }
+        double flx = weno5js(0.5 * (Pf[0] + df[0]), 0.5 * (Pf[1] + df[1]), 0.5 * (Pf[2] + df[2]),
+                             0.5 * (Pf[3] + df[3]), 0.5 * (Pf[4] + df[4]));
+        flx += weno5js(0.5 * (Pf[5] - df[5]), 0.5 * (Pf[4] - df[4]), 0.5 * (Pf[3] - df[3]),
+                       0.5 * (Pf[2] - df[2]), 0.5 * (Pf[1] - df[1]));
+        F[r][n] = flx;
+      }
+    }
+  }
+  __syncthreads();  // every stencil read done: the faces reuse the cell slots
+  double* sF = &sv[0][0];  // [5][NF]
+#pragma unroll
+  for (int r = 0; r < NFT; ++r) {
+    const int f = tid + r * WNT;
+    if (f < G::NF)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) sF[n * G::NF + f] = F[r][n];
+  }
+  __syncthreads();
+
+  dir_epilogue<D, LAST, G::FX, G::NF>(W, bx, sa, sF, S, accf, s1f, outf);
+}
+
+// Brick geometry of the MUSCL pass D: faces f need cells f-2 .. f+1 (the
+// slopes of cells f-1 and f; the z pass also reads T at f+1).
+template <int D>
+struct MGeo {
+  static constexpr int LX = D == 0 ? WX + 4 : WX;
+  static constexpr int LB = D == 0 ? WB : WB + 4;
+  static constexpr int NL = LX * LB;
+  static constexpr int FX = D == 0 ? WX + 1 : WX;
+  static constexpr int FB = D == 0 ? WB : WB + 1;
+  static constexpr int NF = FX * FB;
+};
+
+template <int D, int LAST>
+__global__ void __launch_bounds__(WNT) k_muscl_dir(const WenoBrick* __restrict__ bricks,
+                                                   const BoxDev* __restrict__ boxes, Closures c, StageArgs sa,
+                                                   const Fab* __restrict__ sinf, const Fab* __restrict__ accf,
+                                                   const Fab* __restrict__ s1f, const Fab* __restrict__ outf) {
+  using G = MGeo<D>;
+  __shared__ double sq[NPRIM][G::NL];
+  __shared__ double sd[5][G::NL];
+  const WenoBrick W = bricks[blockIdx.x];
+  if (!box_active(sa, W.box)) return;  // uniform: the whole workgroup leaves
+  const BoxDev bx = boxes[W.box];
+  const Fab S = sinf[W.box];
+  const int tid = threadIdx.x;
+  constexpr int BA = D == 2 ? 2 : 1;
+  const int xmax = bx.hi[0] + (D == 0 ? NGHOST : 0);
+  const int bmax = bx.hi[BA] + (D == 0 ? 0 : NGHOST);
+  const int dlo = bx.lo[D];
+
+  // 1. primitives of the pencil (cons2prim, CNS_hydro_K.h:15-34)
+  for (int l = tid; l < G::NL; l += WNT) {
+    const int lx = l % G::LX, lb = l / G::LX;
+    int x = W.x0 + lx - (D == 0 ? 2 : 0);
+    int b = W.b0 + lb - (D == 0 ? 0 : 2);
+    if (D == 0 && W.wrap > 0) x = dlo + ((x - dlo) % W.wrap + W.wrap) % W.wrap;
+    else x = min(x, xmax);
+    if (D != 0 && W.wrap > 0) b = dlo + ((b - dlo) % W.wrap + W.wrap) % W.wrap;
+    else b = min(b, bmax);
+    int i, j, k;
+    brick_ijk<D>(x, b, W.c, i, j, k);
+    double q[NPRIM];
+    cons2prim(S(i, j, k, URHO), S(i, j, k, UMX), S(i, j, k, UMY), S(i, j, k, UMZ), S(i, j, k, UET), c, q);
+#pragma unroll
+    for (int n = 0; n < NPRIM; ++n) sq[n][l] = q[n];
+  }
+  __syncthreads();
+
+  // 2. slopes of the cells with both neighbours in the pencil (Riemann.h:19-120)
+  constexpr int st = D == 0 ? 1 : G::LX;
+  for (int l = tid; l < G::NL; l += WNT) {
+    const int ld = D == 0 ? l % G::LX : l / G::LX;
+    if (ld < 1 || ld > (D == 0 ? G::LX : G::LB) - 2) continue;
+    auto Q = [&](int o, int comp) { return sq[comp][l + o * st]; };
+    double d[5];
+    muscl_slopes<D>(Q, c, d);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) sd[n][l] = d[n];
+  }
+  __syncthreads();
+
+  // 3. faces (Riemann.h:251-337): face f between pencil cells lL and lL + st
+  constexpr int NFT = (G::NF + WNT - 1) / WNT;
+  double F[NFT][5];
+#pragma unroll
+  for (int r = 0; r < NFT; ++r) {
+    const int f = tid + r * WNT;
+    if (f < G::NF) {
+      const int fx = f % G::FX, fb = f / G::FX;
+      const int lL = D == 0 ? fb * G::LX + fx + 1 : (fb + 1) * G::LX + fx;
+      const int lR = lL + st;
+      const double Tl = D == 2 ? sq[QT][lR + st] : sq[QT][lL];
+      auto ql = [&](int n) { return sq[n][lL]; };
+      auto dql = [&](int n) { return sd[n][lL]; };
+      auto qr = [&](int n) { return sq[n][lR]; };
+      auto dqr = [&](int n) { return sd[n][lR]; };
+      muscl_face<D>(ql, dql, qr, dqr, Tl, c, F[r]);
+    }
+  }
+  __syncthreads();  // every pencil read done: the faces reuse the primitive slots
+  double* sF = &sq[0][0];  // [5][NF]
+#pragma unroll
+  for (int r = 0; r < NFT; ++r) {
+    const int f = tid + r * WNT;
+    if (f < G::NF)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) sF[n * G::NF + f] = F[r][n];
+  }
+  __syncthreads();
+  dir_epilogue<D, LAST, G::FX, G::NF>(W, bx, sa, sF, S, accf, s1f, outf);
+}
+
+}  // namespace
+
+int weno_brick_count(const BoxTable& bt, int d) {
+  int64_t n = 0;
+  for (int b = 0; b < bt.nbox; ++b) {
+    const BoxDev& B = bt.h_boxes[b];
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+    const int nb = d == 2 ? nz : ny, nc = d == 2 ? ny : nz;
+    n += int64_t((nx + WX - 1) / WX) * ((nb + WB - 1) / WB) * nc;
+  }
+  return int(n);
+}
+
+void weno_bricks(const BoxTable& bt, int d, const int* wrap, WenoBrick* out) {
+  int64_t t = 0;
+  for (int b = 0; b < bt.nbox; ++b) {
+    const BoxDev& B = bt.h_boxes[b];
+    const int ba = d == 2 ? 2 : 1, ca = d == 2 ? 1 : 2;
+    for (int cc = B.lo[ca]; cc <= B.hi[ca]; ++cc)
+      for (int b0 = B.lo[ba]; b0 <= B.hi[ba]; b0 += WB)
+        for (int x0 = B.lo[0]; x0 <= B.hi[0]; x0 += WX) out[t++] = WenoBrick{b, x0, b0, cc, wrap[b]};
+  }
+}
+
+hipError_t launch_dir_flux(int flux_euler, const WenoWork& ww, const BoxDev* boxes, const Closures& c,
+                           const StageArgs& sa, const Fab* sin, const Fab* acc, const Fab* s1, const Fab* out,
+                           hipStream_t st) {
+  if (flux_euler == 2) {
+    if (ww.n[0] > 0)
+      hipLaunchKernelGGL((k_weno_dir<0, 0>), dim3(unsigned(ww.n[0])), dim3(WNT), 0, st, ww.d[0], boxes, c, sa, sin,
+                         acc, s1, out);
+    if (ww.n[1] > 0)
+      hipLaunchKernelGGL((k_weno_dir<1, 0>), dim3(unsigned(ww.n[1])), dim3(WNT), 0, st, ww.d[1], boxes, c, sa, sin,
+                         acc, s1, out);
+    if (ww.n[2] > 0)
+      hipLaunchKernelGGL((k_weno_dir<2, 1>), dim3(unsigned(ww.n[2])), dim3(WNT), 0, st, ww.d[2], boxes, c, sa, sin,
+                         acc, s1, out);
+  } else if (flux_euler == 0) {
+    if (ww.n[0] > 0)
+      hipLaunchKernelGGL((k_muscl_dir<0, 0>), dim3(unsigned(ww.n[0])), dim3(WNT), 0, st, ww.d[0], boxes, c, sa, sin,
+                         acc, s1, out);
+    if (ww.n[1] > 0)
+      hipLaunchKernelGGL((k_muscl_dir<1, 0>), dim3(unsigned(ww.n[1])), dim3(WNT), 0, st, ww.d[1], boxes, c, sa, sin,
+                         acc, s1, out);
+    if (ww.n[2] > 0)
+      hipLaunchKernelGGL((k_muscl_dir<2, 1>), dim3(unsigned(ww.n[2])), dim3(WNT), 0, st, ww.d[2], boxes, c, sa, sin,
+                         acc, s1, out);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace amr

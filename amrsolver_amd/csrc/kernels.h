@@ -35,9 +35,25 @@ struct BoxTable {
   mutable std::map<std::array<int, 6>, std::pair<BlockRef*, int>> block_maps;
 };
 
+// Directional flux passes (WENO or MUSCL without viscous fluxes): brick =
+// 64 cells along x from x0 by 4 along y (passes x, y) or z (pass z) from b0,
+// at third coordinate c (z, or y for pass z) of box `box`.  wrap > 0: the box
+// spans a periodic domain of that many cells along the pass direction, and
+// the pass reads periodic images there instead of ghost cells.
+struct WenoBrick {
+  int box, x0, b0, c, wrap;
+};
+struct WenoWork {
+  WenoBrick* d[3] = {nullptr, nullptr, nullptr};
+  int n[3] = {0, 0, 0};
+};
+int weno_brick_count(const BoxTable& bt, int d);
+void weno_bricks(const BoxTable& bt, int d, const int* wrap, WenoBrick* out);
+
 // Per-box workspace views of the unfused pass structure (device arrays of
 // nbox Fab each).
 struct UnfusedWork {
+  WenoWork weno;            // brick tables of the directional flux passes
   Fab* prims = nullptr;     // NPRIM comps on grow(bx,3)
   Fab* nf[3] = {};          // NCONS comps on [lo, hi+1]^3
   Fab* pv[3] = {};          // NCONS comps on grow(bx,1)   (viscous)
@@ -151,6 +167,12 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
 
 // cons2prim of `sin` into w.prims on grow(bx,3) for every box (halo cells
 // from their periodic images when wa.on).
+// The three directional passes of a stage (flux_euler 2: WENO, 0: MUSCL):
+// pass x and y accumulate the flux differences in `acc`, pass z adds its
+// own, the source and the stage combination and writes `out`.
+hipError_t launch_dir_flux(int flux_euler, const WenoWork& ww, const BoxDev* boxes, const Closures& c,
+                           const StageArgs& sa, const Fab* sin, const Fab* acc, const Fab* s1, const Fab* out,
+                           hipStream_t st);
 hipError_t launch_prims(const BoxTable& bt, const UnfusedWork& w, const Closures& c, const double* box_dt,
                         const Fab* sin, const WrapArgs& wa, hipStream_t st);
 

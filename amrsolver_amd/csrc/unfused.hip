@@ -140,30 +140,12 @@ __global__ void __launch_bounds__(BLK) k_slope(const BoxDev* boxes, const BlockR
                                                const Fab* slope, Closures c) {
   REGION_LOOP_BEGIN
   constexpr int DI = D == 0, DJ = D == 1, DK = D == 2;
-  constexpr int qn = QU + D;
-  constexpr int qt1 = D == 0 ? QV : QU;
-  constexpr int qt2 = D == 2 ? QV : QW;
   const Fab q = pr[b], dq = slope[b];
   auto Q = [&](int o, int comp) { return q(i + o * DI, j + o * DJ, k + o * DK, comp); };
-  const double add = D == 0 ? 2.e-40 : 1.e-40;
-  double cspeed = sqrt(c.gamma * c.Rspec * Q(0, QT)) + add;
-  double dlft = 0.5 * (Q(0, QPRES) - Q(-1, QPRES)) / cspeed - 0.5 * Q(0, QRHO) * (Q(0, qn) - Q(-1, qn));
-  double drgt = 0.5 * (Q(1, QPRES) - Q(0, QPRES)) / cspeed - 0.5 * Q(0, QRHO) * (Q(1, qn) - Q(0, qn));
-  double d0 = limiter(dlft, drgt);
-  double cs2 = cspeed * cspeed;
-  dlft = (Q(0, QRHO) - Q(-1, QRHO)) - (Q(0, QPRES) - Q(-1, QPRES)) / cs2;
-  drgt = (Q(1, QRHO) - Q(0, QRHO)) - (Q(1, QPRES) - Q(0, QPRES)) / cs2;
-  double d1 = limiter(dlft, drgt);
-  dlft = 0.5 * (Q(0, QPRES) - Q(-1, QPRES)) / cspeed + 0.5 * Q(0, QRHO) * (Q(0, qn) - Q(-1, qn));
-  drgt = 0.5 * (Q(1, QPRES) - Q(0, QPRES)) / cspeed + 0.5 * Q(0, QRHO) * (Q(1, qn) - Q(0, qn));
-  double d2 = limiter(dlft, drgt);
-  double d3 = limiter(Q(0, qt1) - Q(-1, qt1), Q(1, qt1) - Q(0, qt1));
-  double d4 = limiter(Q(0, qt2) - Q(-1, qt2), Q(1, qt2) - Q(0, qt2));
-  dq(i, j, k, 0) = d0;
-  dq(i, j, k, 1) = d1;
-  dq(i, j, k, 2) = d2;
-  dq(i, j, k, 3) = d3;
-  dq(i, j, k, 4) = d4;
+  double d[5];
+  muscl_slopes<D>(Q, c, d);
+#pragma unroll
+  for (int n = 0; n < 5; ++n) dq(i, j, k, n) = d[n];
 }
 
 // Riemann.h:251-337 cns_riemann_{x,y,z} on surroundingNodes(bx,dir).  The z
@@ -173,38 +155,17 @@ __global__ void __launch_bounds__(BLK) k_riemann(const BoxDev* boxes, const Bloc
                                                  const Fab* slope, const Fab* nf, Closures c) {
   REGION_LOOP_BEGIN
   constexpr int DI = D == 0, DJ = D == 1, DK = D == 2;
-  constexpr int qn = QU + D;
-  constexpr int qt1 = D == 0 ? QV : QU;
-  constexpr int qt2 = D == 2 ? QV : QW;
   const Fab q = pr[b], dq = slope[b], f = nf[b];
   const int il = i - DI, jl = j - DJ, kl = k - DK;
-  double Tl = D == 2 ? q(i, j, k + 1, QT) : q(il, jl, kl, QT);
-  double cspeed = sqrt(c.gamma * c.Rspec * Tl) + 1.e-40;
-  double rl = q(il, jl, kl, QRHO) + 0.5 * ((dq(il, jl, kl, 0) + dq(il, jl, kl, 2)) / cspeed + dq(il, jl, kl, 1));
-  rl = dmax(rl, 1.0e-40);
-  double ul = q(il, jl, kl, qn) + 0.5 * ((dq(il, jl, kl, 2) - dq(il, jl, kl, 0)) / q(il, jl, kl, QRHO));
-  double pl = q(il, jl, kl, QPRES) + 0.5 * (dq(il, jl, kl, 0) + dq(il, jl, kl, 2)) * cspeed;
-  pl = dmax(pl, 1.0e-40);
-  double ut1l = q(il, jl, kl, qt1) + 0.5 * dq(il, jl, kl, 3);
-  double ut2l = q(il, jl, kl, qt2) + 0.5 * dq(il, jl, kl, 4);
-  cspeed = sqrt(c.gamma * c.Rspec * q(i, j, k, QT)) + 1.e-40;
-  double rr = q(i, j, k, QRHO) - 0.5 * ((dq(i, j, k, 0) + dq(i, j, k, 2)) / cspeed + dq(i, j, k, 1));
-  rr = dmax(rr, 1.0e-40);
-  double ur = q(i, j, k, qn) - 0.5 * ((dq(i, j, k, 2) - dq(i, j, k, 0)) / q(i, j, k, QRHO));
-  double pr_ = q(i, j, k, QPRES) - 0.5 * (dq(i, j, k, 0) + dq(i, j, k, 2)) * cspeed;
-  pr_ = dmax(pr_, 1.0e-40);
-  double ut1r = q(i, j, k, qt1) - 0.5 * dq(i, j, k, 3);
-  double ut2r = q(i, j, k, qt2) - 0.5 * dq(i, j, k, 4);
+  const double Tl = D == 2 ? q(i, j, k + 1, QT) : q(il, jl, kl, QT);
+  auto ql = [&](int n) { return q(il, jl, kl, n); };
+  auto dql = [&](int n) { return dq(il, jl, kl, n); };
+  auto qr = [&](int n) { return q(i, j, k, n); };
+  auto dqr = [&](int n) { return dq(i, j, k, n); };
   double o[5];
-  riemann_prob(c.gamma, 1.0e-40, rl, ul, pl, ut1l, ut2l, rr, ur, pr_, ut1r, ut2r, o);
-  constexpr int cn = UMX + D;
-  constexpr int ct1 = D == 0 ? UMY : UMX;
-  constexpr int ct2 = D == 2 ? UMY : UMZ;
-  f(i, j, k, URHO) = o[0];
-  f(i, j, k, cn) = o[1];
-  f(i, j, k, ct1) = o[2];
-  f(i, j, k, ct2) = o[3];
-  f(i, j, k, UET) = o[4];
+  muscl_face<D>(ql, dql, qr, dqr, Tl, c, o);
+#pragma unroll
+  for (int n = 0; n < 5; ++n) f(i, j, k, n) = o[n];
 }
 
 // ---------------------------------------------------------------- viscous
@@ -441,6 +402,10 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
     hipLaunchKernelGGL(KERNEL, L_.grid, dim3(BLK), 0, st, B, L_.map, REG, bdt, __VA_ARGS__);     \
   } while (0)
   const Region g3 = grown(NGHOST), nod = nodal(), g1 = grown(1), g0 = grown(0);
+  // WENO or MUSCL without viscous fluxes: directional passes from the
+  // conserved state, each forming what it needs of its pencil in LDS
+  if (s.rhs_euler && (s.flux_euler == 2 || s.flux_euler == 0) && !s.rhs_visc && w.weno.d[0] != nullptr)
+    return launch_dir_flux(s.flux_euler, w.weno, B, c, sa, sin, w.nf[0], s1, out, st);
   LAUNCH(k_prims, g3, sin, w.prims, c, WrapArgs{0, {0, 0, 0}, {1, 1, 1}});
   if (s.rhs_euler) {
     if (s.flux_euler == 1) {
