@@ -492,7 +492,7 @@ __global__ void __launch_bounds__(OB) k_sum1(const double* partial, int n, doubl
 
 // (a source added to dSdt does not depend on the state and drops out)
 bool adjoint_supported(const amr_scheme_t& s) {
-  if (s.rhs_euler && s.flux_euler != 1) return false;
+  if (s.rhs_euler && (s.flux_euler != 1 || s.art_diss)) return false;
   return s.rhs_euler || s.rhs_visc;
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -95,6 +96,10 @@ struct amr_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // the caller's handle plus one per live box table: the context (and its
+  // stream) is freed when the last of them goes, in whatever order the
+  // caller destroys them
+  std::atomic<int> refs{1};
   // fused-kernel tiling set explicitly per scheme key (amr_set_tiling); it
   // takes precedence over the per-box-table autotuned choice
   std::map<std::string, FusedConfig> tuned;
@@ -242,7 +247,7 @@ struct amr_boxtab_s {
 namespace {
 
 int check_scheme(const amr_scheme_t& s) {
-  if (s.art_diss != 0) return fail(AMR_ERR_ARG, "art_diss=1 (JST dissipation) is not supported");
+  if (s.art_diss != 0 && s.art_diss != 1) return fail(AMR_ERR_ARG, "art_diss must be 0 or 1");
   if (s.rhs_source != 0 && s.rhs_source != 1) return fail(AMR_ERR_ARG, "rhs_source must be 0 or 1");
   if (!s.rhs_euler && s.rhs_visc)
     return fail(AMR_ERR_ARG, "rhs_euler=0 with rhs_visc=1 is not supported (the reference accumulates into an "
@@ -644,13 +649,14 @@ int copy_uncovered(amr_boxtab_s* bt, const DescSet* from, const DescSet* to, con
   return AMR_OK;
 }
 
-enum { UW_PRIMS = 1, UW_VISC = 2, UW_WENO = 4, UW_SLOPE = 8 };
+enum { UW_PRIMS = 1, UW_VISC = 2, UW_WENO = 4, UW_SLOPE = 8, UW_JST = 16 };
 
 int ensure_unfused(amr_boxtab_s* bt, const amr_scheme_t& s) {
   int need = UW_PRIMS;
   if (s.rhs_visc) need |= UW_VISC;
   if (s.rhs_euler && s.flux_euler == 2) need |= UW_WENO;
   if (s.rhs_euler && s.flux_euler == 0) need |= UW_SLOPE;
+  if (s.rhs_euler && s.art_diss) need |= UW_JST;
   need |= bt->uw_kinds;
   if ((need & ~bt->uw_kinds) == 0) return AMR_OK;
   // (re)allocate everything needed in one block
@@ -663,7 +669,7 @@ int ensure_unfused(amr_boxtab_s* bt, const amr_scheme_t& s) {
   struct Arr {
     int ng, nc, nodal;
   };
-  std::vector<Arr> kinds;  // order: prims, nf0..2, [pv0..2], [pe0..2, lam], [slope]
+  std::vector<Arr> kinds;  // order: prims, nf0..2, [pv0..2], [pe0..2, lam], [slope], [jst]
   kinds.push_back({3, NPRIM, 0});
   for (int d = 0; d < 3; ++d) kinds.push_back({0, NCONS, 1});
   if (need & UW_VISC)
@@ -673,6 +679,7 @@ int ensure_unfused(amr_boxtab_s* bt, const amr_scheme_t& s) {
     kinds.push_back({3, 3, 0});
   }
   if (need & UW_SLOPE) kinds.push_back({1, NCONS, 0});
+  if (need & UW_JST) kinds.push_back({1, 6, 0});
   int64_t total = 0;
   for (auto& k : kinds)
     for (int b = 0; b < nb; ++b) {
@@ -711,6 +718,7 @@ int ensure_unfused(amr_boxtab_s* bt, const amr_scheme_t& s) {
     bt->uw.lam = tables[t++];
   }
   if (need & UW_SLOPE) bt->uw.slope = tables[t++];
+  if (need & UW_JST) bt->uw.jst = tables[t++];
   if (need & (UW_WENO | UW_SLOPE)) {
     for (int d = 0; d < 3; ++d) {
       // a box spanning a periodic direction is its own periodic image there
@@ -737,7 +745,7 @@ int ensure_unfused(amr_boxtab_s* bt, const amr_scheme_t& s) {
 
 // Schemes the directional flux passes run (unfused.hip).
 bool dir_flux_scheme(const amr_scheme_t& s) {
-  return s.rhs_euler && (s.flux_euler == 2 || s.flux_euler == 0) && !s.rhs_visc;
+  return s.rhs_euler && (s.flux_euler == 2 || s.flux_euler == 0) && !s.rhs_visc && !s.art_diss;
 }
 
 std::string scheme_key(const amr_scheme_t& s) {
@@ -1440,7 +1448,8 @@ bool fabs_overlap(const amr_fab_t* a, const amr_fab_t* b, int nb) {
 
 int check_adjoint(const amr_scheme_t& s) {
   if (!adjoint_supported(s))
-    return fail(AMR_ERR_ARG, "the adjoint covers the KEEP (flux_euler=1) and viscous right-hand sides only");
+    return fail(AMR_ERR_ARG,
+                "the adjoint covers the KEEP (flux_euler=1) and viscous right-hand sides without JST only");
   return AMR_OK;
 }
 
@@ -1557,11 +1566,18 @@ int amr_ctx_create(int device, void* stream, amr_ctx_t* out) {
   return AMR_OK;
 }
 
-int amr_ctx_destroy(amr_ctx_t ctx) {
-  if (!ctx) return AMR_OK;
+namespace {
+void release_ctx(amr_ctx_t ctx) {
+  if (ctx->refs.fetch_sub(1) != 1) return;
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+}
+}  // namespace
+
+int amr_ctx_destroy(amr_ctx_t ctx) {
+  if (!ctx) return AMR_OK;
+  release_ctx(ctx);
   return AMR_OK;
 }
 
@@ -1587,6 +1603,7 @@ int amr_boxtab_create(amr_ctx_t ctx, const amr_geom_t* geom, int nbox, const int
   HIPCHK(hipSetDevice(ctx->device));
   auto* bt = new amr_boxtab_s();
   bt->ctx = ctx;
+  ctx->refs.fetch_add(1);
   bt->geom = *geom;
   bt->bt.nbox = nbox;
   bt->bt.h_boxes.resize(nbox);
@@ -1614,6 +1631,7 @@ int amr_batch_create(amr_ctx_t ctx, int nprob, const amr_geom_t* geoms, amr_boxt
   HIPCHK(hipSetDevice(ctx->device));
   auto* bt = new amr_boxtab_s();
   bt->ctx = ctx;
+  ctx->refs.fetch_add(1);
   bt->geom = geoms[0];
   bt->batch = true;
   bt->bt.nbox = nprob;
@@ -1737,7 +1755,9 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   if (bt->d_box_dt) (void)hipFree(bt->d_box_dt);
   if (bt->h_box_dt) (void)hipHostFree(bt->h_box_dt);
   if (bt->dt_copied) (void)hipEventDestroy(bt->dt_copied);
+  amr_ctx_t ctx = bt->ctx;
   delete bt;
+  release_ctx(ctx);
   return AMR_OK;
 }
 

@@ -50,6 +50,7 @@ struct Closures {
   double gamma, gm1, cv, cp, Rspec;
   int visc_law, cond_law;
   double visc_ref, Tvisc_ref, cond_ref, Tcond_ref, Pr;
+  double Cshock, Cdamp;  // JST coefficients
 };
 
 inline Closures make_closures(const amr_closures_t& c) {
@@ -66,6 +67,8 @@ inline Closures make_closures(const amr_closures_t& c) {
   r.cond_ref = c.cond_ref;
   r.Tcond_ref = c.Tcond_ref;
   r.Pr = c.Pr;
+  r.Cshock = c.Cshock;
+  r.Cdamp = c.Cdamp;
   return r;
 }
 

@@ -576,7 +576,7 @@ bool launch_shape(const amr_scheme_t& s, const TileDev* tiles, int ntiles, const
 
 bool fused_supported(const amr_scheme_t& s, int tx, int ty) {
   // any box shape: tiles overhanging a box are masked
-  if (!s.rhs_euler || s.flux_euler != 1) return false;
+  if (!s.rhs_euler || s.flux_euler != 1 || s.art_diss) return false;
   if (!(s.order_keep == 2 || s.order_keep == 4)) return false;
   return (tx == 16 && ty == 16) || (tx == 8 && ty == 8) || (tx == 32 && ty == 8);
 }

@@ -60,6 +60,7 @@ struct UnfusedWork {
   Fab* pe[3] = {};          // NCONS comps on grow(bx,3)   (WENO point fluxes)
   Fab* lam = nullptr;       // 3 comps on grow(bx,3)       (WENO eigenvalues)
   Fab* slope = nullptr;     // NCONS comps on grow(bx,1)   (MUSCL slopes)
+  Fab* jst = nullptr;       // 6 comps on grow(bx,1): JST sensor (0-2), spectral radius (3-5)
 };
 
 // Periodic halo wrap for a state that is one array over the whole periodic

@@ -168,6 +168,67 @@ __global__ void __launch_bounds__(BLK) k_riemann(const BoxDev* boxes, const Bloc
   for (int n = 0; n < 5; ++n) f(i, j, k, n) = o[n];
 }
 
+// ---------------------------------------------------------------- JST
+// CNS_hydro_K.h:38-44 disconSensor
+__device__ __forceinline__ double discon_sensor(double pp, double pl, double pr) {
+  double pjst = pr + 2.0 * pp + pl;
+  double ptvd = fabs(pr - pp) + fabs(pp - pl);
+  return fabs(2.0 * (pr - 2.0 * pp + pl) / (pjst + ptvd + 1.0e-40));
+}
+
+// ComputeSensorLambda (CNS_hydro_K.h:46-81) on the valid box; the rest of
+// grow(bx,1) is set to 0, the value the reference's zeroed, never filled
+// sensor and spectral-radius MultiFabs hold there (CNS_advance.cpp:237-257)
+__global__ void __launch_bounds__(BLK) k_jst_sl(const BoxDev* boxes, const BlockRef* bmap, Region reg,
+                                                const double* box_dt, const Fab* pr, const Fab* jst, Closures c) {
+  REGION_LOOP_BEGIN
+  const Fab q = pr[b], J = jst[b];
+  const bool valid = i >= bx.lo[0] && i <= bx.hi[0] && j >= bx.lo[1] && j <= bx.hi[1] && k >= bx.lo[2] &&
+                     k <= bx.hi[2];
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (valid) {
+    const double pp = q(i, j, k, QPRES);
+    v[0] = discon_sensor(pp, q(i - 1, j, k, QPRES), q(i + 1, j, k, QPRES));
+    v[1] = discon_sensor(pp, q(i, j - 1, k, QPRES), q(i, j + 1, k, QPRES));
+    v[2] = discon_sensor(pp, q(i, j, k - 1, QPRES), q(i, j, k + 1, QPRES));
+    const double cs = sqrt(c.gamma * q(i, j, k, QPRES) / q(i, j, k, QRHO));
+    v[3] = fabs(q(i, j, k, QU)) + cs;
+    v[4] = fabs(q(i, j, k, QV)) + cs;
+    v[5] = fabs(q(i, j, k, QW)) + cs;
+  }
+#pragma unroll
+  for (int n = 0; n < 6; ++n) J(i, j, k, n) = v[n];
+}
+
+// JSTflux (CNS_hydro_K.h:84-104) on grownnodaltilebox(-1,0): every direction
+// reads component 0 of sensor and spectral radius, and the z damping stencil
+// reads cons(k-1) three times, as the reference does
+__global__ void __launch_bounds__(BLK) k_jst_flux(const BoxDev* boxes, const BlockRef* bmap, Region reg,
+                                                  const double* box_dt, const Fab* st, const Fab* jst,
+                                                  const Fab* nfx, const Fab* nfy, const Fab* nfz, Closures c) {
+  REGION_LOOP_BEGIN
+  const Fab u = st[b], J = jst[b], fx = nfx[b], fy = nfy[b], fz = nfz[b];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) {
+    const double uijk = u(i, j, k, n);
+    double dw = uijk - u(i - 1, j, k, n);
+    double sn = c.Cshock * dmax(J(i - 1, j, k, 0), J(i, j, k, 0));
+    double rr = dmax(J(i - 1, j, k, 3), J(i, j, k, 3));
+    double fd = u(i + 1, j, k, n) - 3.0 * u(i, j, k, n) + 3.0 * u(i - 1, j, k, n) - u(i - 2, j, k, n);
+    fx(i, j, k, n) = fx(i, j, k, n) - (sn * dw - dmax(0.0, c.Cdamp - sn) * fd) * rr;
+    dw = uijk - u(i, j - 1, k, n);
+    sn = c.Cshock * dmax(J(i, j - 1, k, 0), J(i, j, k, 0));
+    rr = dmax(J(i, j - 1, k, 3), J(i, j, k, 3));
+    fd = u(i, j + 1, k, n) - 3.0 * u(i, j, k, n) + 3.0 * u(i, j - 1, k, n) - u(i, j - 2, k, n);
+    fy(i, j, k, n) = fy(i, j, k, n) - (sn * dw - dmax(0.0, c.Cdamp - sn) * fd) * rr;
+    dw = uijk - u(i, j, k - 1, n);
+    sn = c.Cshock * dmax(J(i, j, k - 1, 0), J(i, j, k, 0));
+    rr = dmax(J(i, j, k - 1, 3), J(i, j, k, 3));
+    fd = u(i, j, k - 1, n) - 3.0 * u(i, j, k, n) + 3.0 * u(i, j, k - 1, n) - u(i, j, k - 1, n);
+    fz(i, j, k, n) = fz(i, j, k, n) - (sn * dw - dmax(0.0, c.Cdamp - sn) * fd) * rr;
+  }
+}
+
 // ---------------------------------------------------------------- viscous
 // CNS_hydro_K.h:108-172 ViscousFluxes on growntilebox(1)
 __global__ void __launch_bounds__(BLK) k_visc_pf(const BoxDev* boxes, const BlockRef* bmap, Region reg, const double* box_dt, const Fab* pr,
@@ -404,7 +465,8 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
   const Region g3 = grown(NGHOST), nod = nodal(), g1 = grown(1), g0 = grown(0);
   // WENO or MUSCL without viscous fluxes: directional passes from the
   // conserved state, each forming what it needs of its pencil in LDS
-  if (s.rhs_euler && (s.flux_euler == 2 || s.flux_euler == 0) && !s.rhs_visc && w.weno.d[0] != nullptr)
+  if (s.rhs_euler && (s.flux_euler == 2 || s.flux_euler == 0) && !s.rhs_visc && !s.art_diss &&
+      w.weno.d[0] != nullptr)
     return launch_dir_flux(s.flux_euler, w.weno, B, c, sa, sin, w.nf[0], s1, out, st);
   LAUNCH(k_prims, g3, sin, w.prims, c, WrapArgs{0, {0, 0, 0}, {1, 1, 1}});
   if (s.rhs_euler) {
@@ -439,6 +501,11 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
         }
       }
     }
+  }
+  // CNS_advance.cpp:235-265 JST dissipation subtracted from the Euler fluxes
+  if (s.rhs_euler && s.art_diss) {
+    LAUNCH(k_jst_sl, g1, w.prims, w.jst, c);
+    LAUNCH(k_jst_flux, nod, sin, w.jst, w.nf[0], w.nf[1], w.nf[2], c);
   }
   if (s.rhs_visc) {
     LAUNCH(k_visc_pf, g1, w.prims, w.pv[0], w.pv[1], w.pv[2], c);

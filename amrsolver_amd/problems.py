@@ -37,9 +37,9 @@ def closures_scaled_sutherland(mu_at_T0, T0=110.4, Pr=0.71, Tref=110.4):
                     cond_law=2, cond_ref=0.0, Tcond_ref=194.0, Pr=Pr, **g)
 
 
-def scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3, euler=True, source=False):
+def scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3, euler=True, source=False, jst=False):
     return Scheme(rhs_euler=int(euler), rhs_visc=int(visc), rhs_source=int(source), flux_euler=flux_euler,
-                  order_keep=order_keep, art_diss=0, order_rk=order_rk, stages_rk=stages_rk)
+                  order_keep=order_keep, art_diss=int(jst), order_rk=order_rk, stages_rk=stages_rk)
 
 
 def geom(n, prob_lo, prob_hi, periodic, lo_bc=(0, 0, 0), hi_bc=(0, 0, 0)):

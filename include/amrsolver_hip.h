@@ -53,7 +53,7 @@ typedef struct {
   int32_t rhs_euler, rhs_visc, rhs_source;
   int32_t flux_euler; /* 0 MUSCL+Riemann, 1 KEEP, 2 WENO5-JS with LF split */
   int32_t order_keep; /* 2, 4, 6 */
-  int32_t art_diss;   /* must be 0 */
+  int32_t art_diss;   /* 1: JST dissipation (CNS_advance.cpp:235-265) */
   int32_t order_rk, stages_rk;
 } amr_scheme_t;
 
@@ -99,6 +99,8 @@ const char* amr_last_error(void);
  * context creates its own).  Replaces the AMReX runtime state the CNS level
  * relies on (amrex::Initialize, The_Arena, Gpu::gpuStream). */
 int amr_ctx_create(int device, void* stream, amr_ctx_t* out);
+/* Releases the caller's handle; the context lives on until its last box
+ * table is destroyed (either order is safe). */
 int amr_ctx_destroy(amr_ctx_t ctx);
 int amr_ctx_synchronize(amr_ctx_t ctx);
 

@@ -712,13 +712,65 @@ struct Walls {
 // ---------------------------------------------------------------- box RHS
 int check_args(const orc_scheme& s, int ng) {
   if (ng < NGHOST) return fail("state FAB needs at least 3 ghost cells");
-  if (s.art_diss != 0) return fail("art_diss=1 (JST) is not part of this path");
+  if (!(s.art_diss == 0 || s.art_diss == 1)) return fail("art_diss must be 0 or 1");
   if (s.rhs_euler && !(s.flux_euler >= 0 && s.flux_euler <= 2)) return fail("flux_euler must be 0, 1 or 2");
   if (s.rhs_euler && s.flux_euler == 1 && !(s.order_keep == 2 || s.order_keep == 4 || s.order_keep == 6))
     return fail("order_keep must be 2, 4 or 6");
   if (!s.rhs_euler && s.rhs_visc)
     return fail("rhs_euler=0 with rhs_visc=1 accumulates into an uninitialised flux array in the reference");
   return 0;
+}
+
+// ---------------------------------------------------------------- JST
+// CNS_hydro_K.h:38-44 disconSensor
+double discon_sensor(double pp, double pl, double pr) {
+  double pjst = pr + 2.0 * pp + pl;
+  double ptvd = std::abs(pr - pp) + std::abs(pp - pl);
+  return std::abs(2.0 * (pr - 2.0 * pp + pl) / (pjst + ptvd + 1.0e-40));
+}
+
+// JST artificial dissipation of one box (art_diss = 1; CNS_advance.cpp:235-265):
+// sensor and spectral radius (3 components each, NGHOST ghosts set to 0)
+// computed on the valid box only (ComputeSensorLambda, CNS_hydro_K.h:46-81),
+// so at a box face the cells outside read 0; then JSTflux
+// (CNS_hydro_K.h:84-104) subtracts from every face flux of the all-nodal box,
+// reading component 0 of sensor and spectral radius in every direction and,
+// in z, cons(k-1) in place of cons(k+1) and cons(k-2).
+void jst_box(const orc_closures& c, const Box& bx, const A4& u, const A4& q, const A4 nf[3]) {
+  Box bg = grow(bx, NGHOST);
+  std::vector<double> senb(npts(bg.lo, bg.hi) * 3, 0.0), lamb(npts(bg.lo, bg.hi) * 3, 0.0);
+  A4 sen = view(senb.data(), bg.lo, bg.hi), lam = view(lamb.data(), bg.lo, bg.hi);
+  for_box(bx, [&](int i, int j, int k) {
+    double pp = q(i, j, k, QPRES);
+    sen(i, j, k, 0) = discon_sensor(pp, q(i - 1, j, k, QPRES), q(i + 1, j, k, QPRES));
+    sen(i, j, k, 1) = discon_sensor(pp, q(i, j - 1, k, QPRES), q(i, j + 1, k, QPRES));
+    sen(i, j, k, 2) = discon_sensor(pp, q(i, j, k - 1, QPRES), q(i, j, k + 1, QPRES));
+    double ux = q(i, j, k, QU), uy = q(i, j, k, QV), uz = q(i, j, k, QW);
+    double cs = std::sqrt(c.gamma * q(i, j, k, QPRES) / q(i, j, k, QRHO));
+    lam(i, j, k, 0) = std::abs(ux) + cs;
+    lam(i, j, k, 1) = std::abs(uy) + cs;
+    lam(i, j, k, 2) = std::abs(uz) + cs;
+  });
+  auto mx = [](double a, double b) { return a < b ? b : a; };  // amrex::max
+  for (int n = 0; n < NCONS; ++n)
+    for_box(nodal_all(bx), [&](int i, int j, int k) {
+      double uijk = u(i, j, k, n);
+      double dw = uijk - u(i - 1, j, k, n);
+      double sn = c.Cshock * mx(sen(i - 1, j, k, 0), sen(i, j, k, 0));
+      double rr = mx(lam(i - 1, j, k, 0), lam(i, j, k, 0));
+      double fd = u(i + 1, j, k, n) - 3.0 * u(i, j, k, n) + 3.0 * u(i - 1, j, k, n) - u(i - 2, j, k, n);
+      nf[0](i, j, k, n) -= (sn * dw - mx(0.0, c.Cdamp - sn) * fd) * rr;
+      dw = uijk - u(i, j - 1, k, n);
+      sn = c.Cshock * mx(sen(i, j - 1, k, 0), sen(i, j, k, 0));
+      rr = mx(lam(i, j - 1, k, 0), lam(i, j, k, 0));
+      fd = u(i, j + 1, k, n) - 3.0 * u(i, j, k, n) + 3.0 * u(i, j - 1, k, n) - u(i, j - 2, k, n);
+      nf[1](i, j, k, n) -= (sn * dw - mx(0.0, c.Cdamp - sn) * fd) * rr;
+      dw = uijk - u(i, j, k - 1, n);
+      sn = c.Cshock * mx(sen(i, j, k - 1, 0), sen(i, j, k, 0));
+      rr = mx(lam(i, j, k - 1, 0), lam(i, j, k, 0));
+      fd = u(i, j, k - 1, n) - 3.0 * u(i, j, k, n) + 3.0 * u(i, j, k - 1, n) - u(i, j, k - 1, n);
+      nf[2](i, j, k, n) -= (sn * dw - mx(0.0, c.Cdamp - sn) * fd) * rr;
+    });
 }
 
 // Flux phase of CNS::compute_rhs (src/tim/CNS_advance.cpp:200-353) for one
@@ -732,6 +784,8 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
   constexpr bool plain = std::is_same<R, double>::value;
   if (!plain && s.rhs_euler && s.flux_euler != 1)
     return fail("the tangent (dual-number) path covers the KEEP and viscous right-hand sides");
+  if (!plain && s.rhs_euler && s.art_diss)
+    return fail("the tangent (dual-number) path does not cover JST dissipation");
   Box bg = grow(bx, NGHOST);
   std::vector<R> qbuf(npts(bg.lo, bg.hi) * NPRIM);
   A4T<R> q = view(qbuf.data(), bg.lo, bg.hi);
@@ -800,6 +854,10 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
         for_box(surrounding_nodes(bx, d), [&](int i, int j, int k) { riemann_point(i, j, k, d, nf[d], sl, q, c); });
       }
     }
+  }
+
+  if constexpr (plain) {
+    if (s.rhs_euler && s.art_diss) jst_box(c, bx, u, q, nf);
   }
 
   if (s.rhs_visc) {

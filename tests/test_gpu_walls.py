@@ -20,6 +20,15 @@ def ctx():
     return level.Context(0)
 
 
+@pytest.fixture(scope="module", autouse=True)
+def _clear_oracle_problem():
+    """The oracle's problem parameters are process-wide: clear them after
+    this module so that later oracle calls see no walls or body force."""
+    yield
+    from oracle import oracle as O
+    O.set_problem(None)
+
+
 def _setup(ctx, sch=None):
     from amrsolver_amd import level as L
     from oracle import oracle as O
