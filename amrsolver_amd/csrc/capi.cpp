@@ -158,19 +158,27 @@ struct GraphEntry {
 // the start of every segment of `k` steps), the stage inputs sb of the steps
 // of one segment (sb(n, 0) = start of step n, sb(0, 0) = the checkpoint),
 // the final state, and three adjoint arrays on the valid cells.
+// Arrays of a checkpointed adjoint run: nseg checkpoints (U at the start of
+// every k-step segment), the final state, and two sets of stage inputs of one
+// segment (krows * nst - 1 arrays each; segment seg uses set seg % 2), so the
+// forward sweep leaves the last two segments in place and one segment can be
+// recomputed while the previous one is reversed.
 struct AdjRun {
   std::vector<char> key;
   int nsteps = 0, k = 0, nst = 0, nseg = 0, krows = 0;
   std::vector<double*> mem;
   std::vector<std::unique_ptr<DescSet>> states;
   std::unique_ptr<DescSet> lam[3];
+  std::vector<hipEvent_t> ev;  // reverse done (nseg), recompute done (nseg)
+  int slots() const { return krows * nst - 1; }
   DescSet* ck(int seg) { return states[seg].get(); }
+  DescSet* uf() { return states[nseg].get(); }
   DescSet* sb(int seg, int n, int s) {
     if (n == 0 && s == 0) return ck(seg);
-    return states[nseg + n * nst + s - 1].get();
+    return states[nseg + 1 + (seg % 2) * slots() + n * nst + s - 1].get();
   }
-  DescSet* uf() { return states[nseg + krows * nst - 1].get(); }
   ~AdjRun() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     for (double* p : mem) (void)hipFree(p);
   }
 };
@@ -227,6 +235,7 @@ struct amr_boxtab_s {
   Fab* adj_lq_table = nullptr;
   Fab* adj_le_table = nullptr;
   std::unique_ptr<AdjRun> run;
+  hipStream_t aux = nullptr;  // second stream (adjoint runs: recompute beside the reverse sweep)
   // autotuned tiling per scheme key for this box table
   std::map<std::string, FusedConfig> tuned;
   // source added to dSdt for schemes with rhs_source = 1 (amr_set_source)
@@ -466,8 +475,8 @@ int build_fill(amr_boxtab_s* bt, DescSet* ds) {
 
 // bc_only: physical boundaries only (the periodic copies of a batch, whose
 // problems are single boxes, when the stage reads periodic images itself).
-int enqueue_fill(amr_boxtab_s* bt, DescSet* ds, bool bc_only = false) {
-  hipStream_t st = bt->ctx->stream;
+int enqueue_fill(amr_boxtab_s* bt, DescSet* ds, bool bc_only = false, hipStream_t stream = nullptr) {
+  hipStream_t st = stream ? stream : bt->ctx->stream;
   if (bt->walls && !bt->has_problem)
     return fail(AMR_ERR_ARG, "bc code 6 walls need amr_set_problem (wall temperature) first");
   if (!bc_only) HIPCHK(launch_fill_copy(ds->d_copy, ds->d_copy_blk, ds->ncopy_blk, ds->d, st));
@@ -1142,7 +1151,8 @@ bool use_level_tiles(amr_boxtab_s* bt, const amr_scheme_t& sch, DescSet* ds1, De
 }
 
 int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageArgs& sa_in,
-             DescSet* ds1, DescSet* dsin, DescSet* dout, const Fab* uncov_u0 = nullptr) {
+             DescSet* ds1, DescSet* dsin, DescSet* dout, const Fab* uncov_u0 = nullptr,
+             hipStream_t stream = nullptr) {
   StageArgs sa = sa_in;
   if (sch.rhs_source) {
     const bool body = bt->has_problem && bt->problem.body_force_x != 0.0;
@@ -1152,7 +1162,7 @@ int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sc
     sa.bfx = body ? bt->problem.body_force_x : 0.0;
   }
   Closures c = make_closures(cls);
-  hipStream_t st = bt->ctx->stream;
+  hipStream_t st = stream ? stream : bt->ctx->stream;
   const Fab* s1 = ds1 ? ds1->d : nullptr;
   FusedConfig cfg;
   const bool level = use_level_tiles(bt, sch, ds1, dsin, dout);
@@ -1478,7 +1488,7 @@ int ensure_run(amr_boxtab_s* bt, const amr_fab_t* state, int nsteps, int k, int 
   R->nst = nst;
   R->nseg = (nsteps + k - 1) / k;
   R->krows = std::min(k, nsteps);
-  const int nstates = R->nseg + R->krows * nst;
+  const int nstates = R->nseg + 1 + (R->nseg > 1 ? 2 : 1) * (R->krows * nst - 1);
   int rc;
   for (int i = 0; i < nstates; ++i) {
     double* m = nullptr;
@@ -1755,6 +1765,7 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   if (bt->d_box_dt) (void)hipFree(bt->d_box_dt);
   if (bt->h_box_dt) (void)hipHostFree(bt->h_box_dt);
   if (bt->dt_copied) (void)hipEventDestroy(bt->dt_copied);
+  if (bt->aux) (void)hipStreamDestroy(bt->aux);
   amr_ctx_t ctx = bt->ctx;
   delete bt;
   release_ctx(ctx);
@@ -2152,8 +2163,9 @@ int amr_adjoint_run(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme
     *sa = StageArgs{c2, nullptr, sch->order_rk, sch->stages_rk, s};
     return r;
   };
-  // forward step n (global index); the last stage is skipped on request
-  auto run_step = [&](int n, bool skip_last) -> int {
+  // forward step n (global index) on stream sx; the last stage is skipped on
+  // request
+  auto run_step = [&](int n, bool skip_last, hipStream_t sx) -> int {
     const int seg = n / k, nl = n % k;
     for (int s = 0; s < nst; ++s) {
       DescSet* sin = R->sb(seg, nl, s);
@@ -2164,7 +2176,7 @@ int amr_adjoint_run(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme
         // the stage itself is not needed, but the reverse sweep reads its
         // input with ghost cells
         if (need_fill) {
-          int r = enqueue_fill(bt, sin);
+          int r = enqueue_fill(bt, sin, false, sx);
           if (r) return r;
         }
         break;
@@ -2178,8 +2190,8 @@ int amr_adjoint_run(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme
       else if (n + 1 == nsteps) out = R->uf();
       else if ((n + 1) % k == 0) out = R->ck(seg + 1);
       else out = R->sb(seg, nl + 1, 0);
-      if (need_fill && (r = enqueue_fill(bt, sin))) return r;
-      if ((r = do_stage(bt, *cls, *sch, sa, s1, sin, out))) return r;
+      if (need_fill && (r = enqueue_fill(bt, sin, false, sx))) return r;
+      if ((r = do_stage(bt, *cls, *sch, sa, s1, sin, out, nullptr, sx))) return r;
     }
     return AMR_OK;
   };
@@ -2187,16 +2199,42 @@ int amr_adjoint_run(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme
   for (auto& ds : R->states)
     if ((rc = copy_uncovered(bt, d0, ds.get()))) return rc;
   for (int n = 0; n < nsteps; ++n)
-    if ((rc = run_step(n, false))) return rc;
+    if ((rc = run_step(n, false, st))) return rc;
   DescSet *lc = R->lam[0].get(), *lt = R->lam[1].get(), *la = R->lam[2].get();
   HIPCHK(launch_objective(bt->bt, objective, R->uf()->d, dw ? dw->d : nullptr, lc->d, bt->d_partial,
                           bt->d_result, st));
   HIPCHK(hipMemcpyAsync(bt->h_result, bt->d_result, sizeof(double), hipMemcpyDeviceToHost, st));
+  // Reverse sweep, last segment first.  The forward sweep leaves the stage
+  // inputs of the last two segments in place; an earlier segment is
+  // recomputed from its checkpoint into its slot set, which the segment two
+  // later has finished with.  Where the forward stages run the fused kernel
+  // (no box-table workspace) the recompute of segment seg-1 runs on a second
+  // stream while segment seg is reversed: the forward kernel of a small level
+  // leaves most CUs to the reverse kernels.
+  const bool overlap = R->nseg > 2 && fused_path(bt, *sch, nullptr);
+  hipStream_t sr = st;
+  if (overlap) {
+    if (!bt->aux) HIPCHK(hipStreamCreateWithFlags(&bt->aux, hipStreamNonBlocking));
+    sr = bt->aux;
+    while (int(R->ev.size()) < 2 * R->nseg) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      R->ev.push_back(e);
+    }
+  }
+  auto ev_rev = [&](int seg) { return R->ev[seg]; };
+  auto ev_rec = [&](int seg) { return R->ev[R->nseg + seg]; };
   for (int seg = R->nseg - 1; seg >= 0; --seg) {
-    const int n0 = seg * k, len = std::min(k, nsteps - n0);
-    if (seg != R->nseg - 1)
+    const int r = seg - 1;  // recomputed next, beside this segment's reverse sweep
+    if (r >= 0 && r <= R->nseg - 3) {
+      if (overlap) HIPCHK(hipStreamWaitEvent(sr, ev_rev(seg + 1), 0));
+      const int n0 = r * k, len = std::min(k, nsteps - n0);
       for (int n = n0; n < n0 + len; ++n)
-        if ((rc = run_step(n, n == n0 + len - 1))) return rc;
+        if ((rc = run_step(n, n == n0 + len - 1, sr))) return rc;
+      if (overlap) HIPCHK(hipEventRecord(ev_rec(r), sr));
+    }
+    if (overlap && seg <= R->nseg - 3) HIPCHK(hipStreamWaitEvent(st, ev_rec(seg), 0));
+    const int len = std::min(k, nsteps - seg * k);
     for (int nl = len - 1; nl >= 0; --nl) {
       // la accumulates the adjoint of U^n: assigned by the last stage,
       // added to by the others (stage 0's input is U^n itself)
@@ -2213,6 +2251,7 @@ int amr_adjoint_run(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme
       }
       std::swap(lc, la);
     }
+    if (overlap) HIPCHK(hipEventRecord(ev_rev(seg), st));
   }
   HIPCHK(launch_copy_valid(bt->bt, lc->d, dg->d, nullptr, st));
   HIPCHK(hipStreamSynchronize(st));

@@ -118,7 +118,7 @@ __device__ __forceinline__ void zterms(const double q1[NPRIM], const double q2[N
 }
 
 template <int H, bool VISC, int TX, int TY, bool UCS>
-__global__ void __launch_bounds__(TX* TY, 2) k_fused(const TileDev* __restrict__ tiles, const BoxDev* __restrict__ boxes,
+__global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ tiles, const BoxDev* __restrict__ boxes,
                                                      Closures c, double co0, double co1, double co2, StageArgs sa,
                                                      WrapArgs wa, Cover cov, UncovSrc us, const Fab* __restrict__ s1f,
                                                      const Fab* __restrict__ sinf, const Fab* __restrict__ outf) {
@@ -232,12 +232,17 @@ __global__ void __launch_bounds__(TX* TY, 2) k_fused(const TileDev* __restrict__
     if (wa.on) p = p < wa.lo[2] ? p + wa.len[2] : (p >= wa.lo[2] + wa.len[2] ? p - wa.len[2] : p);
     if (UCS) {
       const int b = ((p - us.lo[2]) >> us.sh) - zb0;
+      const gdouble* pb = sbase + int64_t(p) * S.ks;
+      const gdouble* pu = u0base + int64_t(p) * S.ks;
 #pragma unroll
       for (int s = 0; s < C::NLOAD; ++s) {
-        const bool fromu0 = (s_um[s * NT + tid] >> b) & 1u;
-        const gdouble* pb = (fromu0 ? u0base : sbase) + int64_t(p) * S.ks;
+        if ((s_um[s * NT + tid] >> b) & 1u) {
 #pragma unroll
-        for (int n = 0; n < 5; ++n) pre[s][n] = pb[n * S.ns + loff[s]];
+          for (int n = 0; n < 5; ++n) pre[s][n] = pu[n * S.ns + loff[s]];
+        } else {
+#pragma unroll
+          for (int n = 0; n < 5; ++n) pre[s][n] = pb[n * S.ns + loff[s]];
+        }
       }
     } else {
       const gdouble* pb = sbase + int64_t(p) * S.ks;
