@@ -536,11 +536,14 @@ def main():
     mean_dur = float(dur.mean())
     achieved = alg_bytes / mean_dur / 1e9
     fp64_tflops = FP64_OPS_PER_CELL_STAGE * ncell / mean_dur / 1e12
+    # HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE for this
+    # tiling (null where no measurement of the tiling in force exists)
     traffic = None
-    if os.path.exists(a.traffic):
+    if os.path.exists(a.traffic) and tiling:
         try:
-            traffic = json.load(open(a.traffic)).get("bytes_per_launch")
-        except Exception:
+            entry = json.load(open(a.traffic)).get("by_tiling", {}).get(",".join(str(x) for x in tiling))
+            traffic = entry["bytes_per_launch"] if entry else None
+        except (OSError, ValueError, KeyError):
             traffic = None
     out = {
         "metric": "FP64 cell-updates/sec per RK stage",
