@@ -64,6 +64,9 @@ SIGNATURES = {
                            ctypes.POINTER(_P)], ctypes.c_int),
     "amr_boxtab_destroy": ([_P], ctypes.c_int),
     "amr_fill_boundary": ([_P, _PF], ctypes.c_int),
+    "amr_fill_two_levels": ([_P, _P, ctypes.c_int, ctypes.c_double, _PF, ctypes.c_double, _PF, _PF],
+                            ctypes.c_int),
+    "amr_average_down": ([_P, _P, ctypes.c_int, _PF, _PF], ctypes.c_int),
     "amr_rhs": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF, _PF], ctypes.c_int),
     "amr_rk_stage": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), ctypes.c_int,
                       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,

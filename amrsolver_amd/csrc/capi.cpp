@@ -185,6 +185,28 @@ struct AdjRun {
 
 }  // namespace
 
+// Coarse-fine tables of a fine level against one coarse level (its serial
+// number and the ratio): interpolation tasks (fine ghost regions per coarse
+// box) and average-down tasks (coarse regions per fine box).
+struct CfTable {
+  uint64_t crse_serial = 0;
+  int ratio = 0;
+  bool interp_built = false, avg_built = false;
+  CopyTask* d_interp = nullptr;
+  TaskBlock* d_interp_blk = nullptr;
+  int ninterp_blk = 0;
+  int64_t ninterp_cells = 0;
+  CopyTask* d_avg = nullptr;
+  TaskBlock* d_avg_blk = nullptr;
+  int navg_blk = 0;
+  ~CfTable() {
+    if (d_interp) (void)hipFree(d_interp);
+    if (d_interp_blk) (void)hipFree(d_interp_blk);
+    if (d_avg) (void)hipFree(d_avg);
+    if (d_avg_blk) (void)hipFree(d_avg_blk);
+  }
+};
+
 struct amr_boxtab_s {
   amr_ctx_t ctx = nullptr;
   amr_geom_t geom;               // the level's geometry (first problem for a batch)
@@ -251,6 +273,9 @@ struct amr_boxtab_s {
   // (srcmap.on = 0: unavailable; u0 is set per launch)
   UncovSrc srcmap{};
   unsigned* d_srcmap_bits = nullptr;
+  std::vector<CopyTask> h_uncov;  // host copy of the uncovered-ghost regions
+  uint64_t serial = 0;            // distinct per box table of the process
+  std::unique_ptr<CfTable> cf;    // coarse-fine tables (this table as the fine level)
 };
 
 namespace {
@@ -642,6 +667,7 @@ int build_uncovered(amr_boxtab_s* bt) {
   }
   bt->nuncov = int(tasks.size());
   bt->nuncov_cells = ncell;
+  bt->h_uncov = tasks;
   if (!tasks.empty()) {
     HIPCHK(hipMalloc(&bt->d_uncov, sizeof(CopyTask) * tasks.size()));
     HIPCHK(hipMemcpy(bt->d_uncov, tasks.data(), sizeof(CopyTask) * tasks.size(), hipMemcpyHostToDevice));
@@ -1612,6 +1638,8 @@ int amr_boxtab_create(amr_ctx_t ctx, const amr_geom_t* geom, int nbox, const int
     }
   HIPCHK(hipSetDevice(ctx->device));
   auto* bt = new amr_boxtab_s();
+  static std::atomic<uint64_t> next_serial{0};
+  bt->serial = ++next_serial;
   bt->ctx = ctx;
   ctx->refs.fetch_add(1);
   bt->geom = *geom;
@@ -1779,6 +1807,246 @@ int amr_fill_boundary(amr_boxtab_t bt, const amr_fab_t* fabs) {
   if (rc) return rc;
   if ((rc = build_fill(bt, ds))) return rc;
   return enqueue_fill(bt, ds);
+}
+
+namespace {
+
+int floor_div(int a, int r) { return a >= 0 ? a / r : -((-a + r - 1) / r); }
+
+int check_two_levels(amr_boxtab_s* f, amr_boxtab_s* c, int r) {
+  if (r != 2 && r != 4) return fail(AMR_ERR_ARG, "refinement ratio must be 2 or 4");
+  if (f->batch || c->batch) return fail(AMR_ERR_ARG, "coarse-fine coupling takes level box tables, not batches");
+  if (f->ctx != c->ctx) return fail(AMR_ERR_ARG, "fine and coarse box tables belong to different contexts");
+  const amr_geom_t& gf = f->geoms[0];
+  const amr_geom_t& gc = c->geoms[0];
+  for (int d = 0; d < 3; ++d) {
+    if (gf.dom_lo[d] != gc.dom_lo[d] * r || gf.dom_hi[d] != (gc.dom_hi[d] + 1) * r - 1)
+      return fail(AMR_ERR_ARG, "fine domain is not the coarse domain refined by the ratio");
+    if (gf.periodic[d] != gc.periodic[d] || gf.lo_bc[d] != gc.lo_bc[d] || gf.hi_bc[d] != gc.hi_bc[d])
+      return fail(AMR_ERR_ARG, "fine and coarse levels differ in periodicity or boundary codes");
+  }
+  return AMR_OK;
+}
+
+CfTable* cf_table(amr_boxtab_s* f, amr_boxtab_s* c, int r) {
+  if (!f->cf || f->cf->crse_serial != c->serial || f->cf->ratio != r) {
+    f->cf = std::make_unique<CfTable>();
+    f->cf->crse_serial = c->serial;
+    f->cf->ratio = r;
+  }
+  return f->cf.get();
+}
+
+// Fine ghost regions no fine box covers (inside the domain, periodic images
+// unwrapped), split by periodic shift and by the coarse box holding their
+// coarse cells.  The seven-point coarse stencil of every such cell must lie
+// in the coarse level (valid cells, periodic images) or outside the domain
+// across a physical boundary that is not ext_dir.
+int build_cf_interp(amr_boxtab_s* F, amr_boxtab_s* C, int r, CfTable* t) {
+  if (t->interp_built) return AMR_OK;
+  const amr_geom_t& g = F->geoms[0];
+  const amr_geom_t& gc = C->geoms[0];
+  int len[3], lenc[3];
+  for (int d = 0; d < 3; ++d) {
+    len[d] = g.dom_hi[d] - g.dom_lo[d] + 1;
+    lenc[d] = gc.dom_hi[d] - gc.dom_lo[d] + 1;
+  }
+  std::vector<IBox> cimg;  // coarse boxes and their periodic images
+  for (int cb = 0; cb < C->bt.nbox; ++cb) {
+    const BoxDev& B = C->bt.h_boxes[cb];
+    for (int sx = -1; sx <= 1; ++sx)
+      for (int sy = -1; sy <= 1; ++sy)
+        for (int sz = -1; sz <= 1; ++sz) {
+          const int sv[3] = {sx, sy, sz};
+          bool ok = true;
+          IBox img;
+          for (int d = 0; d < 3; ++d) {
+            if (sv[d] != 0 && !gc.periodic[d]) ok = false;
+            img.lo[d] = B.lo[d] + sv[d] * lenc[d];
+            img.hi[d] = B.hi[d] + sv[d] * lenc[d];
+          }
+          if (ok) cimg.push_back(img);
+        }
+  }
+  std::vector<CopyTask> tasks;
+  int64_t ncell = 0;
+  for (const CopyTask& p : F->h_uncov) {
+    struct Part {
+      IBox x;
+      int sh[3];
+    };
+    std::vector<Part> parts{Part{IBox{{p.lo[0], p.lo[1], p.lo[2]}, {p.hi[0], p.hi[1], p.hi[2]}}, {0, 0, 0}}}, nx;
+    for (int d = 0; d < 3; ++d) {
+      if (!g.periodic[d]) continue;
+      nx.clear();
+      for (const Part& q : parts) {
+        const int lo3[3] = {q.x.lo[d], std::max(q.x.lo[d], g.dom_lo[d]), std::max(q.x.lo[d], g.dom_hi[d] + 1)};
+        const int hi3[3] = {std::min(q.x.hi[d], g.dom_lo[d] - 1), std::min(q.x.hi[d], g.dom_hi[d]), q.x.hi[d]};
+        const int sh3[3] = {-len[d], 0, len[d]};
+        for (int s = 0; s < 3; ++s) {
+          if (lo3[s] > hi3[s]) continue;
+          Part w = q;
+          w.x.lo[d] = lo3[s];
+          w.x.hi[d] = hi3[s];
+          w.sh[d] = sh3[s];
+          nx.push_back(w);
+        }
+      }
+      parts.swap(nx);
+    }
+    for (const Part& q : parts) {
+      IBox W;
+      for (int d = 0; d < 3; ++d) {
+        W.lo[d] = q.x.lo[d] - q.sh[d];
+        W.hi[d] = q.x.hi[d] - q.sh[d];
+      }
+      const int64_t need = cells(W.lo, W.hi);
+      int64_t got = 0;
+      for (int cb = 0; cb < C->bt.nbox; ++cb) {
+        const BoxDev& B = C->bt.h_boxes[cb];
+        IBox Y;
+        for (int d = 0; d < 3; ++d) {
+          Y.lo[d] = std::max(W.lo[d], B.lo[d] * r);
+          Y.hi[d] = std::min(W.hi[d], (B.hi[d] + 1) * r - 1);
+        }
+        if (ibox_empty(Y)) continue;
+        // the coarse stencil: coarse cells of Y grown by one
+        IBox S;
+        for (int d = 0; d < 3; ++d) {
+          const int clo = floor_div(Y.lo[d], r), chi = floor_div(Y.hi[d], r);
+          if (!gc.periodic[d]) {
+            const bool elo = gc.lo_bc[d] == 1 || gc.lo_bc[d] == 6, ehi = gc.hi_bc[d] == 1 || gc.hi_bc[d] == 6;
+            if ((elo && clo == gc.dom_lo[d]) || (ehi && chi == gc.dom_hi[d]))
+              return fail(AMR_ERR_ARG, "coarse-fine interpolation next to ext_dir (bc code 1 or 6) boundaries is not "
+                                       "supported");
+          }
+          S.lo[d] = clo - 1;
+          S.hi[d] = chi + 1;
+          if (!gc.periodic[d]) {
+            S.lo[d] = std::max(S.lo[d], gc.dom_lo[d]);
+            S.hi[d] = std::min(S.hi[d], gc.dom_hi[d]);
+          }
+        }
+        std::vector<IBox> rest{S}, nr;
+        for (const IBox& img : cimg) {
+          nr.clear();
+          for (const IBox& a : rest) ibox_subtract(a, img, nr);
+          rest.swap(nr);
+          if (rest.empty()) break;
+        }
+        if (!rest.empty())
+          return fail(AMR_ERR_ARG, "coarse-fine interpolation: the coarse stencil leaves the coarse level (fine "
+                                   "level not properly nested)");
+        CopyTask tk;
+        tk.dst = p.dst;
+        tk.src = cb;
+        for (int d = 0; d < 3; ++d) {
+          tk.lo[d] = Y.lo[d] + q.sh[d];
+          tk.hi[d] = Y.hi[d] + q.sh[d];
+          tk.sh[d] = q.sh[d];
+        }
+        tk.start = ncell;
+        const int64_t n = cells(Y.lo, Y.hi);
+        ncell += n;
+        got += n;
+        tasks.push_back(tk);
+      }
+      if (got != need)
+        return fail(AMR_ERR_ARG, "coarse-fine interpolation: fine level not properly nested (ghost cells over no "
+                                 "coarse box)");
+    }
+  }
+  if (!tasks.empty()) {
+    HIPCHK(hipMalloc(&t->d_interp, sizeof(CopyTask) * tasks.size()));
+    HIPCHK(hipMemcpy(t->d_interp, tasks.data(), sizeof(CopyTask) * tasks.size(), hipMemcpyHostToDevice));
+  }
+  t->ninterp_cells = ncell;
+  int rc = upload_blocks(tasks, &t->d_interp_blk, &t->ninterp_blk);
+  if (rc) return rc;
+  t->interp_built = true;
+  return AMR_OK;
+}
+
+int build_cf_avg(amr_boxtab_s* F, amr_boxtab_s* C, int r, CfTable* t) {
+  if (t->avg_built) return AMR_OK;
+  std::vector<CopyTask> tasks;
+  int64_t ncell = 0;
+  for (int fb = 0; fb < F->bt.nbox; ++fb) {
+    const BoxDev& B = F->bt.h_boxes[fb];
+    IBox X;
+    for (int d = 0; d < 3; ++d) {
+      if (((B.lo[d] % r) + r) % r != 0 || (((B.hi[d] + 1) % r) + r) % r != 0)
+        return fail(AMR_ERR_ARG, "fine box " + std::to_string(fb) + " is not coarsenable by the ratio");
+      X.lo[d] = floor_div(B.lo[d], r);
+      X.hi[d] = floor_div(B.hi[d] + 1, r) - 1;
+    }
+    for (int cb = 0; cb < C->bt.nbox; ++cb) {
+      const BoxDev& D = C->bt.h_boxes[cb];
+      CopyTask tk;
+      tk.dst = cb;
+      tk.src = fb;
+      for (int d = 0; d < 3; ++d) {
+        tk.lo[d] = std::max(X.lo[d], D.lo[d]);
+        tk.hi[d] = std::min(X.hi[d], D.hi[d]);
+        tk.sh[d] = 0;
+      }
+      if (tk.lo[0] > tk.hi[0] || tk.lo[1] > tk.hi[1] || tk.lo[2] > tk.hi[2]) continue;
+      tk.start = ncell;
+      ncell += cells(tk.lo, tk.hi);
+      tasks.push_back(tk);
+    }
+  }
+  if (!tasks.empty()) {
+    HIPCHK(hipMalloc(&t->d_avg, sizeof(CopyTask) * tasks.size()));
+    HIPCHK(hipMemcpy(t->d_avg, tasks.data(), sizeof(CopyTask) * tasks.size(), hipMemcpyHostToDevice));
+  }
+  int rc = upload_blocks(tasks, &t->d_avg_blk, &t->navg_blk);
+  if (rc) return rc;
+  t->avg_built = true;
+  return AMR_OK;
+}
+
+}  // namespace
+
+int amr_fill_two_levels(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, double a_old, const amr_fab_t* crse_old,
+                        double b_new, const amr_fab_t* crse_new, const amr_fab_t* fine_state) {
+  if (!fine || !crse || !fine_state) return fail(AMR_ERR_ARG, "null argument");
+  if (!std::isfinite(a_old) || !std::isfinite(b_new)) return fail(AMR_ERR_ARG, "time weights must be finite");
+  const bool use_old = !(a_old == 0.0 && b_new != 0.0), use_new = b_new != 0.0;
+  if (use_old && !crse_old) return fail(AMR_ERR_ARG, "crse_old is required unless a_old == 0 and b_new != 0");
+  if (use_new && !crse_new) return fail(AMR_ERR_ARG, "crse_new is required when b_new != 0");
+  int rc;
+  if ((rc = check_two_levels(fine, crse, ratio))) return rc;
+  CfTable* t = cf_table(fine, crse, ratio);
+  if ((rc = build_cf_interp(fine, crse, ratio, t))) return rc;
+  DescSet *dsf = nullptr, *d0 = nullptr, *d1 = nullptr;
+  if ((rc = get_desc(fine, fine_state, NGHOST, &dsf))) return rc;
+  if ((rc = build_fill(fine, dsf))) return rc;
+  if (use_old && (rc = get_desc(crse, crse_old, 1, &d0))) return rc;
+  if (use_new && (rc = get_desc(crse, crse_new, 1, &d1))) return rc;
+  hipStream_t st = fine->ctx->stream;
+  if (fine->walls && !fine->has_problem)
+    return fail(AMR_ERR_ARG, "bc code 6 walls need amr_set_problem (wall temperature) first");
+  HIPCHK(launch_fill_copy(dsf->d_copy, dsf->d_copy_blk, dsf->ncopy_blk, dsf->d, st));
+  const bool blend = use_old && use_new;
+  const Fab* c0 = use_old ? d0->d : d1->d;
+  HIPCHK(launch_interp_cf(t->d_interp, t->d_interp_blk, t->ninterp_blk, c0, blend ? d1->d : c0, blend, a_old,
+                          b_new, ratio, dsf->d, st));
+  return enqueue_fill(fine, dsf, true);
+}
+
+int amr_average_down(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, const amr_fab_t* fine_state,
+                     const amr_fab_t* crse_state) {
+  if (!fine || !crse || !fine_state || !crse_state) return fail(AMR_ERR_ARG, "null argument");
+  int rc;
+  if ((rc = check_two_levels(fine, crse, ratio))) return rc;
+  CfTable* t = cf_table(fine, crse, ratio);
+  if ((rc = build_cf_avg(fine, crse, ratio, t))) return rc;
+  DescSet *df = nullptr, *dc = nullptr;
+  if ((rc = get_desc(fine, fine_state, 0, &df))) return rc;
+  if ((rc = get_desc(crse, crse_state, 0, &dc))) return rc;
+  HIPCHK(launch_avg_down(t->d_avg, t->d_avg_blk, t->navg_blk, df->d, dc->d, ratio, fine->ctx->stream));
+  return AMR_OK;
 }
 
 int amr_stage_coeffs(int order_rk, int stages_rk, double dt, int stage, int* nstages, int* mode, double* a,

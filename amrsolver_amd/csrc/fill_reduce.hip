@@ -341,6 +341,98 @@ hipError_t launch_estdt_boxes(const BoxTable& bt, const Fab* st, const Closures&
   return hipGetLastError();
 }
 
+// Coarse-fine interpolation (conservative linear with one limiter factor per
+// direction shared by the components): one lane per fine ghost cell.  The
+// seven-point coarse stencil is read around the coarse cell holding the
+// cell's in-domain image, from the FAB of the coarse box that owns it (its
+// ghost cells filled by the caller).
+template <bool BLEND>
+__global__ void __launch_bounds__(FBLK) k_interp_cf(const CopyTask* tasks, const TaskBlock* blocks, const Fab* c0,
+                                                    const Fab* c1, double a, double b, int r, const Fab* fine) {
+  const TaskBlock tb = blocks[blockIdx.x];
+  const CopyTask& T = tasks[tb.task];
+  int c[3];
+  if (!task_cell(T, tb, threadIdx.x, c)) return;
+  int ic[3], rem[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int w = c[d] - T.sh[d];
+    ic[d] = w >= 0 ? w / r : -((-w + r - 1) / r);
+    rem[d] = w - ic[d] * r;
+  }
+  const Fab U0 = c0[T.src];
+  const Fab U1 = BLEND ? c1[T.src] : U0;
+  auto U = [&](int di, int dj, int dk, int n) -> double {
+    const int i = ic[0] + di, j = ic[1] + dj, k = ic[2] + dk;
+    if (BLEND) return a * U0(i, j, k, n) + b * U1(i, j, k, n);
+    return U0(i, j, k, n);
+  };
+  double u0[NCONS], sl[3][NCONS];
+  double sf[3] = {1.0, 1.0, 1.0};
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) {
+    u0[n] = U(0, 0, 0, n);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int ex = d == 0, ey = d == 1, ez = d == 2;
+      const double um = U(-ex, -ey, -ez, n), up = U(ex, ey, ez, n);
+      const double dc = 0.5 * (up - um);
+      const double df = 2.0 * (up - u0[n]);
+      const double db = 2.0 * (u0[n] - um);
+      double s = (df * db >= 0.0) ? dmin(fabs(df), fabs(db)) : 0.0;
+      s = copysign(1.0, dc) * dmin(s, fabs(dc));
+      if (dc != 0.0) sf[d] = dmin(sf[d], s / dc);
+      sl[d][n] = dc;
+    }
+  }
+  double off[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) off[d] = (double(rem[d]) + 0.5) / double(r) - 0.5;
+  const Fab F = fine[T.dst];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) {
+    const double sx = sl[0][n] * sf[0], sy = sl[1][n] * sf[1], sz = sl[2][n] * sf[2];
+    F(c[0], c[1], c[2], n) = u0[n] + off[0] * sx + off[1] * sy + off[2] * sz;
+  }
+}
+
+// Average-down: one lane per coarse cell, children summed x fastest.
+__global__ void __launch_bounds__(FBLK) k_avg_down(const CopyTask* tasks, const TaskBlock* blocks, const Fab* fine,
+                                                   const Fab* crse, int r, double volfrac) {
+  const TaskBlock tb = blocks[blockIdx.x];
+  const CopyTask& T = tasks[tb.task];
+  int c[3];
+  if (!task_cell(T, tb, threadIdx.x, c)) return;
+  const Fab F = fine[T.src], C = crse[T.dst];
+  for (int n = 0; n < NCONS; ++n) {
+    double s = 0.0;
+    for (int kr = 0; kr < r; ++kr)
+      for (int jr = 0; jr < r; ++jr)
+        for (int ir = 0; ir < r; ++ir) s += F(c[0] * r + ir, c[1] * r + jr, c[2] * r + kr, n);
+    C(c[0], c[1], c[2], n) = volfrac * s;
+  }
+}
+
+hipError_t launch_interp_cf(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* c0, const Fab* c1,
+                            bool blend, double a, double b, int r, const Fab* fine, hipStream_t st) {
+  if (nblk == 0) return hipSuccess;
+  if (blend)
+    hipLaunchKernelGGL(k_interp_cf<true>, dim3(unsigned(nblk)), dim3(FBLK), 0, st, tasks, blocks, c0, c1, a, b, r,
+                       fine);
+  else
+    hipLaunchKernelGGL(k_interp_cf<false>, dim3(unsigned(nblk)), dim3(FBLK), 0, st, tasks, blocks, c0, c0, a, b, r,
+                       fine);
+  return hipGetLastError();
+}
+
+hipError_t launch_avg_down(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* fine,
+                           const Fab* crse, int r, hipStream_t st) {
+  if (nblk == 0) return hipSuccess;
+  const double volfrac = 1.0 / double(r * r * r);
+  hipLaunchKernelGGL(k_avg_down, dim3(unsigned(nblk)), dim3(FBLK), 0, st, tasks, blocks, fine, crse, r, volfrac);
+  return hipGetLastError();
+}
+
 hipError_t launch_copy_regions(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* src,
                                const Fab* dst0, const Fab* dst1, hipStream_t st) {
   if (nblk == 0) return hipSuccess;

@@ -220,6 +220,17 @@ hipError_t launch_fill_bc(const BcTask* tasks, const TaskBlock* blocks, int nblk
 hipError_t launch_copy_regions(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* src,
                                const Fab* dst0, const Fab* dst1, hipStream_t st);
 
+// Coarse-fine ghost cells of a fine level from the coarse level: per task,
+// dst = fine box, src = coarse box holding the coarse cells, lo/hi = fine
+// cells, sh = periodic shift (the cell's image inside the domain is c - sh).
+// Coarse data a*c0 + b*c1 (c0 alone when blend is false), ratio r.
+hipError_t launch_interp_cf(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* c0, const Fab* c1,
+                            bool blend, double a, double b, int r, const Fab* fine, hipStream_t st);
+// Coarse cells under fine boxes := mean of their r^3 children: per task,
+// dst = coarse box, src = fine box, lo/hi = coarse cells.
+hipError_t launch_avg_down(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* fine,
+                           const Fab* crse, int r, hipStream_t st);
+
 // dst = src on the valid cells of every box.
 hipError_t launch_copy_valid(const BoxTable& bt, const Fab* src, const Fab* dst, const double* box_dt,
                              hipStream_t st);

@@ -249,6 +249,18 @@ class Level:
     def advance(self, S, time, dt):
         call("amr_advance", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, time, dt)
 
+    # --- coarse-fine coupling (this level is the fine one) ---------------
+    def fill_two_levels(self, S, coarse, crse_old, crse_new, a, b, ratio=2):
+        """FillPatch of a fine level: same-level copies, coarse-fine ghost
+        cells interpolated from a*crse_old + b*crse_new (states of `coarse`
+        with filled ghost cells), physical BCs."""
+        call("amr_fill_two_levels", self.h, coarse.h, ratio, a, None if crse_old is None else crse_old.fabs, b,
+             None if crse_new is None else crse_new.fabs, S.fabs)
+
+    def average_down(self, S, coarse, Sc, ratio=2):
+        """CNS::avgDown: coarse cells of Sc under this level := mean of S."""
+        call("amr_average_down", self.h, coarse.h, ratio, S.fabs, Sc.fabs)
+
     def advance_graph(self, S, dt, nsteps):
         call("amr_advance_graph", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, dt,
              nsteps)
@@ -336,6 +348,99 @@ def compute_new_dt(est, dt_prev, cur_time, stop_time, first):
     if stop_time >= 0.0 and (cur_time + dt0) > (stop_time - eps):
         dt0 = stop_time - cur_time
     return dt0
+
+
+def fill_times(order_rk, stages_rk, time, dt):
+    """FillPatch times of the stages of CNS::advance, as its calls pass them
+    (src/tim/CNS_advance.cpp:54-161)."""
+    if order_rk == -2:
+        return [time, time + dt]
+    if order_rk in (0, 1):
+        return [time]
+    if order_rk == 2:
+        m = stages_rk
+        return [time + dt * float(i - 1) / float(m - 1) for i in range(1, m)] + [time + dt]
+    if order_rk == 3 and stages_rk == 3:
+        return [time, time + dt, time + dt / 2]
+    if order_rk == 3 and stages_rk == 4:
+        return [time, time + dt / 2, time + dt, time + dt / 2]
+    raise ValueError("unsupported order_rk/stages_rk combination")
+
+
+def cf_weights(t0, t1, t):
+    """Weights (a, b) of a coarse level's states at t0 and t1 at fill time t:
+    AMReX StateData::getData takes the new state within teps = 1e-3 (t1 - t0)
+    of t1, else the old one within teps of t0, else FillPatchSingleLevel
+    blends them with a = (t1 - t)/(t1 - t0), b = (t - t0)/(t1 - t0)."""
+    teps = (t1 - t0) * 1.0e-3
+    if t1 == t0 or (t1 - teps < t < t1 + teps):
+        return 0.0, 1.0
+    if t0 - teps < t < t0 + teps:
+        return 1.0, 0.0
+    return (t1 - t) / (t1 - t0), (t - t0) / (t1 - t0)
+
+
+class Hierarchy:
+    """Levels 0..L of an AMR hierarchy (refinement ratio r), advanced as
+    AMReX's Amr::timeStep does with subcycling: level l steps by dt_l, then
+    level l+1 takes r steps of dt_l / r at time + (i - 1) dt_{l+1}, then
+    CNS::post_timestep averages level l+1 down onto level l (src/CNS.cpp:398-412,
+    513-530; its Reflux adds nothing, the flux registers are never filled,
+    src/tim/CNS_advance.cpp:356-371).  Each level runs the stage loop of
+    CNS::advance with a FillPatch before every stage: amr_fill_boundary on
+    level 0, amr_fill_two_levels above it, with the coarse level's states at
+    the start and end of its step.  Per level three ghosted states: the step's
+    U^n (kept as the coarse old state of the finer level), and two that take
+    the stage outputs in turn."""
+
+    def __init__(self, levels, ratio=2, layout="level"):
+        self.levels = list(levels)
+        self.ratio = ratio
+        self.cur = [lev.new_state(layout=layout) for lev in self.levels]
+        self.old = [lev.new_state(layout=layout) for lev in self.levels]
+        self.tmp = [lev.new_state(layout=layout) for lev in self.levels]
+        self.tint = [(0.0, 0.0)] * len(self.levels)
+
+    def upload(self, us):
+        for S, u in zip(self.cur, us):
+            S.upload(u)
+
+    def download(self, lev):
+        return self.cur[lev].download()
+
+    def _fill(self, lev, S, t):
+        L = self.levels[lev]
+        if lev == 0:
+            L.fill_patch(S)
+            return
+        t0, t1 = self.tint[lev - 1]
+        a, b = cf_weights(t0, t1, t)
+        L.fill_two_levels(S, self.levels[lev - 1], self.old[lev - 1], self.cur[lev - 1], a, b, self.ratio)
+
+    def _step(self, lev, time, dt):
+        L = self.levels[lev]
+        sch = L.scheme
+        times = fill_times(sch.order_rk, sch.stages_rk, time, dt)
+        A, B, C = self.cur[lev], self.old[lev], self.tmp[lev]
+        sin = A
+        for s, t in enumerate(times):
+            self._fill(lev, sin, t)
+            _, mode, a, b, c, d = capi.stage_coeffs(sch.order_rk, sch.stages_rk, dt, s)
+            out = B if s % 2 == 0 else C
+            L.rk_stage(mode, a, b, c, d, A, sin, out)
+            sin = out
+        self.old[lev], self.cur[lev], self.tmp[lev] = A, sin, (C if sin is B else B)
+        self.tint[lev] = (time, time + dt)
+        if lev + 1 < len(self.levels):
+            self._fill(lev, self.cur[lev], time + dt)
+            dtf = dt / self.ratio
+            for i in range(1, self.ratio + 1):
+                self._step(lev + 1, time + (i - 1) * dtf, dtf)
+            self.levels[lev + 1].average_down(self.cur[lev + 1], L, self.cur[lev], self.ratio)
+
+    def advance(self, time, dt):
+        """One coarse step of dt (level l steps by dt / r^l)."""
+        self._step(0, time, dt)
 
 
 class Batch(Level):

@@ -117,6 +117,34 @@ int amr_boxtab_destroy(amr_boxtab_t bt);
  * ng >= 3. */
 int amr_fill_boundary(amr_boxtab_t bt, const amr_fab_t* fabs);
 
+/* Ghost fill of a fine AMR level: FillPatch of a level above the coarsest,
+ * which AMReX runs as FillPatchTwoLevels with the interpolater CNS
+ * registers for State_Type, lincc_interp = CellConservativeLinear with
+ * linear limiting (src/set/CNS_setup.cpp:112-114; FillPatch calls
+ * src/tim/CNS_advance.cpp:54-161).  In order: same-level copies and periodic
+ * images; every ghost cell inside the domain that no fine box covers from
+ * the coarse level's data at the fill time, a_old*crse_old + b_new*crse_new
+ * (crse_old alone when b_new == 0, crse_new alone when a_old == 0; AMReX's
+ * weights are a = (t1 - t)/(t1 - t0), b = (t - t0)/(t1 - t0), or one state
+ * alone within 1e-3 (t1 - t0) of its time), by conservative linear
+ * interpolation; physical BCs.  `crse` is the coarse level's box table,
+ * `fine` this level's; fine_state needs ng >= 3, the coarse states ng >= 1
+ * with their ghost cells filled (amr_fill_boundary or amr_fill_two_levels of
+ * the coarse level).  ratio 2 or 4.  The coarse stencil (coarse cell and its
+ * six neighbours) of every interpolated cell must lie in the coarse level or
+ * across a foextrap/reflect boundary; otherwise, or next to ext_dir (bc code
+ * 1 or 6) boundaries, AMR_ERR_ARG. */
+int amr_fill_two_levels(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, double a_old,
+                        const amr_fab_t* crse_old, double b_new, const amr_fab_t* crse_new,
+                        const amr_fab_t* fine_state);
+
+/* CNS::avgDown (src/CNS.cpp:513-530), i.e. amrex::average_down: every
+ * coarse cell under a fine box becomes the mean of its ratio^3 children
+ * (summed x fastest, times 1/ratio^3); other coarse cells are untouched.
+ * Fine boxes must be coarsenable by the ratio (2 or 4). */
+int amr_average_down(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, const amr_fab_t* fine_state,
+                     const amr_fab_t* crse_state);
+
 /* dSdt = L(state) for every box: CNS::compute_rhs (src/tim/CNS_advance.cpp:186-427)
  * with the per-scheme Flux functions (Riemann::Flux, Riemann.h:339;
  * HiRes::FluxWENO, High_resolution.h:279; CentralKEEP::FluxKEEP,

@@ -955,12 +955,191 @@ int bc_type(int code, int comp, int dir) {
   return (comp - UMX == dir) ? norm_vel_bc[code] : tang_vel_bc[code];
 }
 
+// ------------------------------------------------ coarse-fine coupling
+// The reference takes these from AMReX (submodule lib/amrex,
+// https://github.com/AMReX-Codes/amrex; the snapshot records no commit and the
+// submodule is empty).  Restated from AMReX's published algorithms:
+//  - FillPatch of a fine level (AmrLevel FillPatch -> FillPatchTwoLevels):
+//    same-level copies and periodic images first; every ghost cell inside the
+//    domain (periodic directions unbounded) that no fine box covers then takes
+//    the coarse level's data at the fill time, interpolated; physical BCs last;
+//  - the coarse data at time t (StateData::getData + FillPatchSingleLevel):
+//    the new data if |t - t1| < teps, else the old data if |t - t0| < teps
+//    (teps = 1e-3 (t1 - t0)), else a*old + b*new, a = (t1 - t)/(t1 - t0),
+//    b = (t - t0)/(t1 - t0);
+//  - CellConservativeLinear with linear limiting (lincc_interp, the
+//    interpolater of State_Type, src/set/CNS_setup.cpp:112-114): per coarse
+//    cell and direction d the central slope dc = (u+ - u-)/2 of every
+//    component, the MC-limited slope s = sign(dc) min(|dc|, min(|2(u+ - u0)|,
+//    |2(u0 - u-)|)) (0 where 2(u+ - u0) and 2(u0 - u-) differ in sign), one
+//    factor f_d = min over components of s/dc (components with dc = 0 skip
+//    it), slope f_d*dc; the fine cell gets u0 + x*sx + y*sy + z*sz with
+//    offsets (i - r*ic + 1/2)/r - 1/2;
+//  - average_down (CNS::avgDown, src/CNS.cpp:513-530): each coarse cell under
+//    a fine box becomes (1/r^3) * the sum of its r^3 children, summed with i
+//    fastest, then j, then k.
+// Slopes next to ext_dir boundaries (one-sided in AMReX) are not restated:
+// such stencils are rejected.
+struct CoarseFine {
+  const Level* C = nullptr;        // coarse level
+  const double* const* u0 = nullptr;  // coarse FABs at t0, ghost cells filled
+  const double* const* u1 = nullptr;  // coarse FABs at t1, ghost cells filled
+  double a = 1.0, b = 0.0;          // weights of u0 and u1 at the fill time
+  int ratio = 2;
+};
+
+void cf_weights(double t0, double t1, double t, double* a, double* b) {
+  const double teps = (t1 - t0) * 1.0e-3;
+  if (t1 == t0 || (t > t1 - teps && t < t1 + teps)) {
+    *a = 0.0;
+    *b = 1.0;
+  } else if (t > t0 - teps && t < t0 + teps) {
+    *a = 1.0;
+    *b = 0.0;
+  } else {
+    *a = (t1 - t) / (t1 - t0);
+    *b = (t - t0) / (t1 - t0);
+  }
+}
+
+int floor_div(int a, int r) { return a >= 0 ? a / r : -((-a + r - 1) / r); }
+
+bool in_box(const Box& s, const int c[3]) {
+  return c[0] >= s.lo[0] && c[0] <= s.hi[0] && c[1] >= s.lo[1] && c[1] <= s.hi[1] && c[2] >= s.lo[2] &&
+         c[2] <= s.hi[2];
+}
+
+// Box of `L` whose valid region holds cell c (periodic image taken), or -1;
+// -2 if c lies outside the domain in a non-periodic direction.
+int owner_box(const Level& L, const int c0[3]) {
+  const orc_geom& g = *L.g;
+  int c[3] = {c0[0], c0[1], c0[2]};
+  for (int d = 0; d < 3; ++d) {
+    const int len = g.dom_hi[d] - g.dom_lo[d] + 1;
+    if (c[d] < g.dom_lo[d] || c[d] > g.dom_hi[d]) {
+      if (!g.periodic[d]) return -2;
+      c[d] = g.dom_lo[d] + ((c[d] - g.dom_lo[d]) % len + len) % len;
+    }
+  }
+  for (int s = 0; s < L.nbox; ++s)
+    if (in_box(L.boxes[s], c)) return s;
+  return -1;
+}
+
+int check_two_levels(const orc_geom& gf, const orc_geom& gc, int r) {
+  if (r != 2 && r != 4) return fail("refinement ratio must be 2 or 4");
+  for (int d = 0; d < 3; ++d) {
+    if (gf.dom_lo[d] != gc.dom_lo[d] * r || gf.dom_hi[d] != (gc.dom_hi[d] + 1) * r - 1)
+      return fail("fine domain is not the coarse domain refined");
+    if (gf.periodic[d] != gc.periodic[d] || gf.lo_bc[d] != gc.lo_bc[d] || gf.hi_bc[d] != gc.hi_bc[d])
+      return fail("fine and coarse levels differ in periodicity or boundary codes");
+  }
+  return 0;
+}
+
+// Interpolation of fine cell c (inside the domain, periodic image taken) from
+// the coarse level.
+int interp_cell(const CoarseFine& cf, const int c[3], double out[NCONS]) {
+  const Level& C = *cf.C;
+  const orc_geom& gc = *C.g;
+  const int r = cf.ratio;
+  int ic[3];
+  for (int d = 0; d < 3; ++d) ic[d] = floor_div(c[d], r);
+  const int cb = owner_box(C, ic);
+  if (cb < 0) return fail("coarse-fine interpolation: fine level not properly nested (no coarse box holds a coarse cell)");
+  // periodic image of ic inside box cb: the stencil is read from that FAB
+  int icb[3];
+  for (int d = 0; d < 3; ++d) {
+    const int len = gc.dom_hi[d] - gc.dom_lo[d] + 1;
+    icb[d] = ic[d];
+    if (gc.periodic[d]) icb[d] = gc.dom_lo[d] + ((ic[d] - gc.dom_lo[d]) % len + len) % len;
+  }
+  for (int d = 0; d < 3; ++d) {
+    for (int e = -1; e <= 1; e += 2) {
+      int p[3] = {icb[0], icb[1], icb[2]};
+      p[d] += e;
+      if (owner_box(C, p) == -1)
+        return fail("coarse-fine interpolation: the coarse stencil leaves the coarse level (fine level not properly nested)");
+    }
+    if (!gc.periodic[d]) {
+      const bool elo = gc.lo_bc[d] == 1 || gc.lo_bc[d] == 6, ehi = gc.hi_bc[d] == 1 || gc.hi_bc[d] == 6;
+      if ((elo && icb[d] == gc.dom_lo[d]) || (ehi && icb[d] == gc.dom_hi[d]))
+        return fail("coarse-fine interpolation next to ext_dir boundaries is not supported");
+    }
+  }
+  if (C.ng < 1) return fail("coarse FABs need at least one ghost cell");
+  Box gb = grow(C.boxes[cb], C.ng);
+  A4 v0, v1;
+  if (cf.u0) v0 = view(const_cast<double*>(cf.u0[cb]), gb.lo, gb.hi);
+  if (cf.u1) v1 = view(const_cast<double*>(cf.u1[cb]), gb.lo, gb.hi);
+  auto U = [&](int di, int dj, int dk, int n) -> double {
+    const int i = icb[0] + di, j = icb[1] + dj, k = icb[2] + dk;
+    if (cf.b == 0.0) return v0(i, j, k, n);
+    if (cf.a == 0.0) return v1(i, j, k, n);
+    return cf.a * v0(i, j, k, n) + cf.b * v1(i, j, k, n);
+  };
+  static const int E[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  double u0[NCONS], sl[3][NCONS], sf[3] = {1.0, 1.0, 1.0};
+  for (int n = 0; n < NCONS; ++n) {
+    u0[n] = U(0, 0, 0, n);
+    for (int d = 0; d < 3; ++d) {
+      const double um = U(-E[d][0], -E[d][1], -E[d][2], n), up = U(E[d][0], E[d][1], E[d][2], n);
+      const double dc = 0.5 * (up - um);
+      const double df = 2.0 * (up - u0[n]);
+      const double db = 2.0 * (u0[n] - um);
+      double s = (df * db >= 0.0) ? std::min(std::abs(df), std::abs(db)) : 0.0;
+      s = std::copysign(1.0, dc) * std::min(s, std::abs(dc));
+      if (dc != 0.0) sf[d] = std::min(sf[d], s / dc);
+      sl[d][n] = dc;
+    }
+  }
+  double off[3];
+  for (int d = 0; d < 3; ++d) off[d] = (double(c[d] - ic[d] * r) + 0.5) / double(r) - 0.5;
+  for (int n = 0; n < NCONS; ++n) {
+    const double sx = sl[0][n] * sf[0], sy = sl[1][n] * sf[1], sz = sl[2][n] * sf[2];
+    out[n] = u0[n] + off[0] * sx + off[1] * sy + off[2] * sz;
+  }
+  return 0;
+}
+
+// Coarse-fine ghost cells of every fine box (after the same-level copies).
+int interp_coarse_fine(const Level& L, std::vector<A4>& v, const CoarseFine& cf) {
+  const orc_geom& g = *L.g;
+  int len[3];
+  for (int d = 0; d < 3; ++d) len[d] = g.dom_hi[d] - g.dom_lo[d] + 1;
+  int err = 0;
+  for (int b = 0; b < L.nbox && !err; ++b) {
+    const Box& bx = L.boxes[b];
+    Box gb = grow(bx, L.ng);
+    for_box(gb, [&](int i, int j, int k) {
+      if (err) return;
+      const int iv[3] = {i, j, k};
+      if (in_box(bx, iv)) return;
+      int c[3] = {i, j, k};
+      for (int d = 0; d < 3; ++d)
+        if (c[d] < g.dom_lo[d] || c[d] > g.dom_hi[d]) {
+          if (!g.periodic[d]) return;  // physical boundary
+          c[d] += (c[d] < g.dom_lo[d]) ? len[d] : -len[d];
+        }
+      for (int s = 0; s < L.nbox; ++s)
+        if (in_box(L.boxes[s], c)) return;  // covered: copied
+      double val[NCONS];
+      if (interp_cell(cf, c, val)) {
+        err = 1;
+        return;
+      }
+      for (int n = 0; n < NCONS; ++n) v[b](i, j, k, n) = val[n];
+    });
+  }
+  return err ? -1 : 0;
+}
+
 // Single-level FillPatch: valid data of every box is taken as given; ghost
 // cells get (1) the valid data of the box covering their periodic image
 // (AMReX FillBoundary), then (2) physical BCs per non-periodic direction in
 // the order x, y, z (src/set/bcs.cpp + AMReX foextrap/reflect fills).
 template <class R>
-int fill_patch(const Level& L, R* const* fabs) {
+int fill_patch(const Level& L, R* const* fabs, const CoarseFine* cf = nullptr) {
   const orc_geom& g = *L.g;
   int ng = L.ng;
   std::vector<A4T<R>> v(L.nbox);
@@ -1001,6 +1180,13 @@ int fill_patch(const Level& L, R* const* fabs) {
       hint = src;
       for (int n = 0; n < NCONS; ++n) v[b](i, j, k, n) = v[src](c[0], c[1], c[2], n);
     });
+  }
+  if (cf) {
+    if constexpr (std::is_same<R, double>::value) {
+      if (interp_coarse_fine(L, v, *cf)) return -1;
+    } else {
+      return fail("coarse-fine interpolation is restated for double data only");
+    }
   }
   for (int d = 0; d < 3; ++d) {
     if (g.periodic[d]) continue;
@@ -1166,6 +1352,12 @@ struct LevelOps {
   std::vector<Box> vbox;
   R* const* state;  // ghosted FABs (Sborder storage and final output)
   const double* const* srcs = nullptr;  // per-box source arrays (rhs_source)
+  // coarse level of a fine level: coarse-fine ghost cells are interpolated
+  // at each stage's FillPatch time tfill[stage] between its states at t0, t1
+  const CoarseFine* cf = nullptr;
+  std::vector<double> tfill;
+  double t0 = 0.0, t1 = 0.0;
+  int stage = 0;
 
   long nval(int b) const { return npts(vbox[b].lo, vbox[b].hi) * NCONS; }
 
@@ -1179,7 +1371,16 @@ struct LevelOps {
       for (int n = 0; n < NCONS; ++n)
         for_box(vbox[b], [&](int i, int j, int k) { dst(i, j, k, n) = sv(i, j, k, n); });
     }
-    int rc = fill_patch(*L, state);
+    int rc;
+    if (cf) {
+      CoarseFine w = *cf;
+      if (stage >= int(tfill.size())) return fail("internal: more stages than FillPatch times");
+      cf_weights(t0, t1, tfill[stage], &w.a, &w.b);
+      rc = fill_patch(*L, state, &w);
+    } else {
+      rc = fill_patch(*L, state);
+    }
+    ++stage;
     if (rc) return rc;
     int err = 0;
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads) reduction(| : err)
@@ -1249,11 +1450,36 @@ Level make_level(const orc_geom* g, int nbox, const int* boxes, int ng) {
   return L;
 }
 
+// FillPatch times of the stages of CNS::advance, as its calls pass them
+// (src/tim/CNS_advance.cpp:54,62,72,77,95,101,122,127,133,145,150,155,161).
+int fill_times(int order_rk, int stages_rk, double time, double dt, std::vector<double>& t) {
+  t.clear();
+  if (order_rk == -2) {
+    t = {time, time + dt};
+  } else if (order_rk == 0 || order_rk == 1) {
+    t = {time};
+  } else if (order_rk == 2) {
+    const int m = stages_rk;
+    if (m < 2) return fail("SSPRK2 number of stages must be >= 2");
+    for (int i = 1; i <= m - 1; ++i) t.push_back(time + dt * double(i - 1) / double(m - 1));
+    t.push_back(time + dt);
+  } else if (order_rk == 3 && stages_rk == 3) {
+    t = {time, time + dt, time + dt / 2};
+  } else if (order_rk == 3 && stages_rk == 4) {
+    t = {time, time + dt / 2, time + dt, time + dt / 2};
+  } else {
+    return fail("unsupported order_rk/stages_rk combination");
+  }
+  return 0;
+}
+
 // One CNS::advance step of a level (valid data in, ghosted FABs as Sborder
-// storage; the new state is written back into the valid region).
+// storage; the new state is written back into the valid region).  With `cf`
+// the level is a fine level whose coarse level holds states at t0 and t1.
 template <class R>
 int advance_level(const Level& L, const orc_geom* g, const orc_closures* c, const orc_scheme* s, R* const* state,
-                  const double* const* src, double dt, int nthreads) {
+                  const double* const* src, double dt, int nthreads, const CoarseFine* cf = nullptr,
+                  double time = 0.0, double t0 = 0.0, double t1 = 0.0) {
   const int nbox = L.nbox, ng = L.ng;
   LevelOps<R> ops;
   ops.L = &L;
@@ -1264,6 +1490,12 @@ int advance_level(const Level& L, const orc_geom* g, const orc_closures* c, cons
   ops.vbox = L.boxes;
   ops.state = state;
   ops.srcs = src;
+  if (cf) {
+    ops.cf = cf;
+    ops.t0 = t0;
+    ops.t1 = t1;
+    if (int rc = fill_times(s->order_rk, s->stages_rk, time, dt, ops.tfill)) return rc;
+  }
   for (int v = 0; v < 3; ++v) ops.vec[v].resize(nbox);
   for (int b = 0; b < nbox; ++b) {
     long n = npts(L.boxes[b].lo, L.boxes[b].hi) * NCONS;
@@ -1374,6 +1606,80 @@ int orc_advance(const orc_geom* g, const orc_closures* c, const orc_scheme* s, i
   (void)time;
   Level L = make_level(g, nbox, boxes, ng);
   return advance_level(L, g, c, s, state, src, dt, nthreads);
+}
+
+void orc_cf_weights(double t0, double t1, double t, double* a, double* b) { cf_weights(t0, t1, t, a, b); }
+
+int orc_fill_two_levels(const orc_geom* gf, int nbf, const int* boxes_f, int ngf, double* const* fabs_f,
+                        const orc_geom* gc, int nbc, const int* boxes_c, int ngc, const double* const* crse0,
+                        const double* const* crse1, double a, double b, int ratio) {
+  if (int rc = check_two_levels(*gf, *gc, ratio)) return rc;
+  if ((b != 0.0 && !crse1) || (a != 0.0 && !crse0)) return fail("missing coarse data for nonzero weight");
+  Level F = make_level(gf, nbf, boxes_f, ngf);
+  Level C = make_level(gc, nbc, boxes_c, ngc);
+  CoarseFine cf;
+  cf.C = &C;
+  cf.u0 = crse0;
+  cf.u1 = crse1;
+  cf.a = a;
+  cf.b = b;
+  cf.ratio = ratio;
+  return fill_patch(F, fabs_f, &cf);
+}
+
+int orc_average_down(int nbf, const int* boxes_f, int ngf, const double* const* fabs_f, int nbc, const int* boxes_c,
+                     int ngc, double* const* fabs_c, int ratio) {
+  if (ratio != 2 && ratio != 4) return fail("refinement ratio must be 2 or 4");
+  Level F = make_level(nullptr, nbf, boxes_f, ngf);
+  Level C = make_level(nullptr, nbc, boxes_c, ngc);
+  const int r = ratio;
+  const double volfrac = 1.0 / double(r * r * r);
+  for (const Box& bf : F.boxes)
+    for (int d = 0; d < 3; ++d)
+      if (bf.lo[d] % r != 0 || (bf.hi[d] + 1) % r != 0) return fail("fine boxes are not coarsenable by the ratio");
+  for (int fb = 0; fb < F.nbox; ++fb) {
+    const Box& bf = F.boxes[fb];
+    Box gf = grow(bf, ngf);
+    A4 fv = view(const_cast<double*>(fabs_f[fb]), gf.lo, gf.hi);
+    for (int cb = 0; cb < C.nbox; ++cb) {
+      const Box& bc = C.boxes[cb];
+      Box x;
+      bool empty = false;
+      for (int d = 0; d < 3; ++d) {
+        x.lo[d] = std::max(bf.lo[d] / r, bc.lo[d]);
+        x.hi[d] = std::min((bf.hi[d] + 1) / r - 1, bc.hi[d]);
+        empty = empty || x.lo[d] > x.hi[d];
+      }
+      if (empty) continue;
+      Box gcb = grow(bc, ngc);
+      A4 cv = view(fabs_c[cb], gcb.lo, gcb.hi);
+      for (int n = 0; n < NCONS; ++n)
+        for_box(x, [&](int i, int j, int k) {
+          double c = 0.0;
+          for (int kr = 0; kr < r; ++kr)
+            for (int jr = 0; jr < r; ++jr)
+              for (int ir = 0; ir < r; ++ir) c += fv(i * r + ir, j * r + jr, k * r + kr, n);
+          cv(i, j, k, n) = volfrac * c;
+        });
+    }
+  }
+  return 0;
+}
+
+int orc_advance_two_levels(const orc_geom* g, const orc_closures* c, const orc_scheme* s, int nbox, const int* boxes,
+                           int ng, double* const* state, const double* const* src, double time, double dt,
+                           int nthreads, const orc_geom* gc, int nbc, const int* boxes_c, int ngc,
+                           const double* const* crse0, const double* const* crse1, double t0, double t1, int ratio) {
+  if (int rc = check_two_levels(*g, *gc, ratio)) return rc;
+  if (!crse0 || !crse1) return fail("coarse states at t0 and t1 are required");
+  Level L = make_level(g, nbox, boxes, ng);
+  Level C = make_level(gc, nbc, boxes_c, ngc);
+  CoarseFine cf;
+  cf.C = &C;
+  cf.u0 = crse0;
+  cf.u1 = crse1;
+  cf.ratio = ratio;
+  return advance_level(L, g, c, s, state, src, dt, nthreads, &cf, time, t0, t1);
 }
 
 int orc_box_rhs_tangent(const orc_closures* c, const orc_scheme* s, const int lo[3], const int hi[3], int ng,

@@ -85,6 +85,22 @@ double orc_est_dt(const orc_geom* g, const orc_closures* c, int nbox,
                   const int* boxes, int ng, double* const* state, double cfl);
 void orc_totals(const orc_geom* g, int nbox, const int* boxes, int ng,
                 double* const* state, double out[5]);
+/* ---- coarse-fine coupling (AMReX FillPatchTwoLevels with lincc_interp,
+   average_down) ---- */
+void orc_cf_weights(double t0, double t1, double t, double* a, double* b);
+int orc_fill_two_levels(const orc_geom* gf, int nbf, const int* boxes_f, int ngf,
+                        double* const* fabs_f, const orc_geom* gc, int nbc,
+                        const int* boxes_c, int ngc, const double* const* crse0,
+                        const double* const* crse1, double a, double b, int ratio);
+int orc_average_down(int nbf, const int* boxes_f, int ngf, const double* const* fabs_f,
+                     int nbc, const int* boxes_c, int ngc, double* const* fabs_c,
+                     int ratio);
+int orc_advance_two_levels(const orc_geom* g, const orc_closures* c, const orc_scheme* s,
+                           int nbox, const int* boxes, int ng, double* const* state,
+                           const double* const* src, double time, double dt, int nthreads,
+                           const orc_geom* gc, int nbc, const int* boxes_c, int ngc,
+                           const double* const* crse0, const double* const* crse1,
+                           double t0, double t1, int ratio);
 /* generic SSPRK driver on a linear test ODE dU/dt = lam*U (one scalar per
    entry), used to pin the stage coefficients against stability polynomials */
 int orc_rk_linear(int order_rk, int stages_rk, double lam, double dt, int n,

@@ -123,6 +123,19 @@ def load(variant=None):
     lib.orc_advance_tangent.argtypes = [ctypes.POINTER(OGeom), ctypes.POINTER(OClosures),
                                         ctypes.POINTER(OScheme), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                         ctypes.c_int, PP, PP, PP, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    IP = ctypes.POINTER(ctypes.c_int)
+    lib.orc_cf_weights.restype = None
+    lib.orc_cf_weights.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _DP, _DP]
+    lib.orc_fill_two_levels.argtypes = [ctypes.POINTER(OGeom), ctypes.c_int, IP, ctypes.c_int, PP,
+                                        ctypes.POINTER(OGeom), ctypes.c_int, IP, ctypes.c_int, PP, PP,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    lib.orc_average_down.argtypes = [ctypes.c_int, IP, ctypes.c_int, PP, ctypes.c_int, IP, ctypes.c_int, PP,
+                                     ctypes.c_int]
+    lib.orc_advance_two_levels.argtypes = [ctypes.POINTER(OGeom), ctypes.POINTER(OClosures),
+                                           ctypes.POINTER(OScheme), ctypes.c_int, IP, ctypes.c_int, PP, PP,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           ctypes.POINTER(OGeom), ctypes.c_int, IP, ctypes.c_int, PP, PP,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_int]
     if variant is None:
         _lib = lib
     return lib
@@ -353,6 +366,95 @@ class OracleLevel:
             src = None if source is None else source[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
             out[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = box_rhs(c, s, lo, hi, self.ng, f, dx, src)
         return out
+
+
+    # --- coarse-fine coupling (this level is the fine one) ---
+    def fill_two_levels(self, coarse, crse0, crse1, a, b, ratio=2):
+        """FillPatch of a fine level: same-level copies, coarse-fine ghost
+        cells interpolated from a*crse0 + b*crse1 (lists of ghosted coarse
+        FABs, ghost cells filled), physical BCs."""
+        _check(load().orc_fill_two_levels(ctypes.byref(self.g), len(self.boxes), self.barr, self.ng, self._pp(),
+                                          ctypes.byref(coarse.g), len(coarse.boxes), coarse.barr, coarse.ng,
+                                          None if crse0 is None else coarse._pp(crse0),
+                                          None if crse1 is None else coarse._pp(crse1), a, b, ratio))
+
+    def average_down(self, coarse, ratio=2):
+        """Coarse cells under this level's boxes := mean of their children."""
+        _check(load().orc_average_down(len(self.boxes), self.barr, self.ng, self._pp(), len(coarse.boxes),
+                                       coarse.barr, coarse.ng, coarse._pp(), ratio))
+
+    def advance_two_levels(self, c, s, dt, time, coarse, crse0, crse1, t0, t1, ratio=2, nthreads=1, source=None):
+        """One CNS::advance step of a fine level whose every FillPatch takes
+        the coarse-fine ghost cells from the coarse states at t0 and t1."""
+        sp, keep = self._src(source)
+        _check(load().orc_advance_two_levels(ctypes.byref(self.g), ctypes.byref(closures(c)),
+                                             ctypes.byref(scheme(s)), len(self.boxes), self.barr, self.ng,
+                                             self._pp(), sp, time, dt, nthreads, ctypes.byref(coarse.g),
+                                             len(coarse.boxes), coarse.barr, coarse.ng, coarse._pp(crse0),
+                                             coarse._pp(crse1), t0, t1, ratio))
+        del keep
+
+
+def cf_weights(t0, t1, t):
+    """Weights (a, b) of the coarse states at t0 and t1 at fill time t."""
+    a, b = ctypes.c_double(), ctypes.c_double()
+    load().orc_cf_weights(t0, t1, t, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+class OracleHierarchy:
+    """Levels 0..L of one AMR hierarchy with refinement ratio r, advanced as
+    AMReX's Amr::timeStep does with subcycling: level l steps by dt_l, then
+    level l+1 takes r steps of dt_l / r at time + (i - 1) dt_{l+1}, then
+    CNS::post_timestep averages level l+1 down onto level l (src/CNS.cpp:398-412,
+    513-530; refluxing adds nothing there, its registers are never filled).
+    Every FillPatch of a fine level interpolates its coarse-fine ghost cells
+    from the coarse level's states at the start and end of the coarse step."""
+
+    def __init__(self, levels, ratio=2):
+        self.levels = [OracleLevel(g, boxes) for g, boxes in levels]
+        self.ratio = ratio
+        self.old = [None] * len(self.levels)    # ghosted FABs at the step's start
+        self.tint = [(0.0, 0.0)] * len(self.levels)
+
+    def set_state(self, us):
+        for L, u in zip(self.levels, us):
+            L.set_state(u)
+
+    def get_state(self, lev):
+        return self.levels[lev].get_state()
+
+    def _fill(self, lev, t):
+        L = self.levels[lev]
+        if lev == 0:
+            L.fill_patch()
+            return
+        C = self.levels[lev - 1]
+        t0, t1 = self.tint[lev - 1]
+        a, b = cf_weights(t0, t1, t)
+        L.fill_two_levels(C, self.old[lev - 1], C.fabs, a, b, self.ratio)
+
+    def _step(self, lev, c, s, time, dt, nthreads):
+        L = self.levels[lev]
+        self._fill(lev, time)
+        self.old[lev] = [f.copy() for f in L.fabs]
+        if lev == 0:
+            L.advance(c, s, dt, time, nthreads=nthreads)
+        else:
+            C = self.levels[lev - 1]
+            t0, t1 = self.tint[lev - 1]
+            L.advance_two_levels(c, s, dt, time, C, self.old[lev - 1], C.fabs, t0, t1, self.ratio, nthreads)
+        self.tint[lev] = (time, time + dt)
+        if lev + 1 < len(self.levels):
+            self._fill(lev, time + dt)
+            dtf = dt / self.ratio
+            for i in range(1, self.ratio + 1):
+                self._step(lev + 1, c, s, time + (i - 1) * dtf, dtf, nthreads)
+            self.levels[lev + 1].average_down(L, self.ratio)
+
+    def advance(self, c, s, time, dt, nthreads=1):
+        """One coarse step of dt (level l steps by dt / r^l)."""
+        self._step(0, c, s, time, dt, nthreads)
 
 
 def set_problem(c, wall_T=0.0, body_force_x=0.0):
