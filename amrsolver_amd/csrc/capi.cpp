@@ -192,10 +192,10 @@ struct CfTable {
   uint64_t crse_serial = 0;
   int ratio = 0;
   bool interp_built = false, avg_built = false;
-  CopyTask* d_interp = nullptr;
+  InterpTask* d_interp = nullptr;
   TaskBlock* d_interp_blk = nullptr;
   int ninterp_blk = 0;
-  int64_t ninterp_cells = 0;
+  int64_t ninterp_cells = 0;  // coarse cells (lanes) of the interpolation tasks
   CopyTask* d_avg = nullptr;
   TaskBlock* d_avg_blk = nullptr;
   int navg_blk = 0;
@@ -1868,7 +1868,7 @@ int build_cf_interp(amr_boxtab_s* F, amr_boxtab_s* C, int r, CfTable* t) {
           if (ok) cimg.push_back(img);
         }
   }
-  std::vector<CopyTask> tasks;
+  std::vector<InterpTask> tasks;
   int64_t ncell = 0;
   for (const CopyTask& p : F->h_uncov) {
     struct Part {
@@ -1937,18 +1937,19 @@ int build_cf_interp(amr_boxtab_s* F, amr_boxtab_s* C, int r, CfTable* t) {
         if (!rest.empty())
           return fail(AMR_ERR_ARG, "coarse-fine interpolation: the coarse stencil leaves the coarse level (fine "
                                    "level not properly nested)");
-        CopyTask tk;
+        InterpTask tk;
         tk.dst = p.dst;
         tk.src = cb;
         for (int d = 0; d < 3; ++d) {
-          tk.lo[d] = Y.lo[d] + q.sh[d];
-          tk.hi[d] = Y.hi[d] + q.sh[d];
+          tk.lo[d] = floor_div(Y.lo[d], r);
+          tk.hi[d] = floor_div(Y.hi[d], r);
+          tk.flo[d] = Y.lo[d];
+          tk.fhi[d] = Y.hi[d];
           tk.sh[d] = q.sh[d];
         }
         tk.start = ncell;
-        const int64_t n = cells(Y.lo, Y.hi);
-        ncell += n;
-        got += n;
+        ncell += cells(tk.lo, tk.hi);
+        got += cells(Y.lo, Y.hi);
         tasks.push_back(tk);
       }
       if (got != need)
@@ -1957,8 +1958,8 @@ int build_cf_interp(amr_boxtab_s* F, amr_boxtab_s* C, int r, CfTable* t) {
     }
   }
   if (!tasks.empty()) {
-    HIPCHK(hipMalloc(&t->d_interp, sizeof(CopyTask) * tasks.size()));
-    HIPCHK(hipMemcpy(t->d_interp, tasks.data(), sizeof(CopyTask) * tasks.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&t->d_interp, sizeof(InterpTask) * tasks.size()));
+    HIPCHK(hipMemcpy(t->d_interp, tasks.data(), sizeof(InterpTask) * tasks.size(), hipMemcpyHostToDevice));
   }
   t->ninterp_cells = ncell;
   int rc = upload_blocks(tasks, &t->d_interp_blk, &t->ninterp_blk);

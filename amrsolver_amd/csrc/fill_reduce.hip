@@ -342,24 +342,17 @@ hipError_t launch_estdt_boxes(const BoxTable& bt, const Fab* st, const Closures&
 }
 
 // Coarse-fine interpolation (conservative linear with one limiter factor per
-// direction shared by the components): one lane per fine ghost cell.  The
-// seven-point coarse stencil is read around the coarse cell holding the
-// cell's in-domain image, from the FAB of the coarse box that owns it (its
-// ghost cells filled by the caller).
+// direction shared by the components): one lane per coarse cell, which reads
+// its seven-point stencil once (from the FAB of the coarse box that owns it,
+// ghost cells filled by the caller) and writes its children inside the
+// task's fine region.
 template <bool BLEND>
-__global__ void __launch_bounds__(FBLK) k_interp_cf(const CopyTask* tasks, const TaskBlock* blocks, const Fab* c0,
+__global__ void __launch_bounds__(FBLK) k_interp_cf(const InterpTask* tasks, const TaskBlock* blocks, const Fab* c0,
                                                     const Fab* c1, double a, double b, int r, const Fab* fine) {
   const TaskBlock tb = blocks[blockIdx.x];
-  const CopyTask& T = tasks[tb.task];
-  int c[3];
-  if (!task_cell(T, tb, threadIdx.x, c)) return;
-  int ic[3], rem[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const int w = c[d] - T.sh[d];
-    ic[d] = w >= 0 ? w / r : -((-w + r - 1) / r);
-    rem[d] = w - ic[d] * r;
-  }
+  const InterpTask& T = tasks[tb.task];
+  int ic[3];
+  if (!task_cell(T, tb, threadIdx.x, ic)) return;
   const Fab U0 = c0[T.src];
   const Fab U1 = BLEND ? c1[T.src] : U0;
   auto U = [&](int di, int dj, int dk, int n) -> double {
@@ -385,14 +378,29 @@ __global__ void __launch_bounds__(FBLK) k_interp_cf(const CopyTask* tasks, const
       sl[d][n] = dc;
     }
   }
-  double off[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) off[d] = (double(rem[d]) + 0.5) / double(r) - 0.5;
+  for (int n = 0; n < NCONS; ++n)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) sl[d][n] = sl[d][n] * sf[d];
   const Fab F = fine[T.dst];
+  int wlo[3], whi[3];
 #pragma unroll
-  for (int n = 0; n < NCONS; ++n) {
-    const double sx = sl[0][n] * sf[0], sy = sl[1][n] * sf[1], sz = sl[2][n] * sf[2];
-    F(c[0], c[1], c[2], n) = u0[n] + off[0] * sx + off[1] * sy + off[2] * sz;
+  for (int d = 0; d < 3; ++d) {
+    wlo[d] = max(ic[d] * r, T.flo[d]);
+    whi[d] = min(ic[d] * r + r - 1, T.fhi[d]);
+  }
+  const double rr = double(r);
+  for (int wk = wlo[2]; wk <= whi[2]; ++wk) {
+    const double oz = (double(wk - ic[2] * r) + 0.5) / rr - 0.5;
+    for (int wj = wlo[1]; wj <= whi[1]; ++wj) {
+      const double oy = (double(wj - ic[1] * r) + 0.5) / rr - 0.5;
+      for (int wi = wlo[0]; wi <= whi[0]; ++wi) {
+        const double ox = (double(wi - ic[0] * r) + 0.5) / rr - 0.5;
+#pragma unroll
+        for (int n = 0; n < NCONS; ++n)
+          F(wi + T.sh[0], wj + T.sh[1], wk + T.sh[2], n) = u0[n] + ox * sl[0][n] + oy * sl[1][n] + oz * sl[2][n];
+      }
+    }
   }
 }
 
@@ -413,7 +421,7 @@ __global__ void __launch_bounds__(FBLK) k_avg_down(const CopyTask* tasks, const 
   }
 }
 
-hipError_t launch_interp_cf(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* c0, const Fab* c1,
+hipError_t launch_interp_cf(const InterpTask* tasks, const TaskBlock* blocks, int nblk, const Fab* c0, const Fab* c1,
                             bool blend, double a, double b, int r, const Fab* fine, hipStream_t st) {
   if (nblk == 0) return hipSuccess;
   if (blend)

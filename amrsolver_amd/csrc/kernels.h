@@ -221,10 +221,18 @@ hipError_t launch_copy_regions(const CopyTask* tasks, const TaskBlock* blocks, i
                                const Fab* dst0, const Fab* dst1, hipStream_t st);
 
 // Coarse-fine ghost cells of a fine level from the coarse level: per task,
-// dst = fine box, src = coarse box holding the coarse cells, lo/hi = fine
-// cells, sh = periodic shift (the cell's image inside the domain is c - sh).
-// Coarse data a*c0 + b*c1 (c0 alone when blend is false), ratio r.
-hipError_t launch_interp_cf(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* c0, const Fab* c1,
+// dst = fine box, src = coarse box, lo/hi = coarse cells (inside box src),
+// flo/fhi = the fine cells to fill (images inside the domain), sh = periodic
+// shift (fine cell w is stored at w + sh).  Coarse data a*c0 + b*c1 (c0 alone
+// when blend is false), ratio r.
+struct InterpTask {
+  int dst, src;
+  int lo[3], hi[3];
+  int flo[3], fhi[3];
+  int sh[3];
+  int64_t start;
+};
+hipError_t launch_interp_cf(const InterpTask* tasks, const TaskBlock* blocks, int nblk, const Fab* c0, const Fab* c1,
                             bool blend, double a, double b, int r, const Fab* fine, hipStream_t st);
 // Coarse cells under fine boxes := mean of their r^3 children: per task,
 // dst = coarse box, src = fine box, lo/hi = coarse cells.

@@ -175,13 +175,14 @@ def tgv_vorticity_magnitude(g, n, mach=0.1, L=1.0, T0=110.4, cls=None, X=None, Y
 # ---------------------------------------------------------------- AMR hierarchy
 
 
-def cluster_blocks(tags, max_blocks, eff=0.7):
+def cluster_blocks(tags, max_blocks, eff=0.7, allowed=None):
     """Berger-Rigoutsos style clustering of a boolean block mask tags[i,j,k]
     into boxes (inclusive block bounds): shrink to the tags' bounding box;
-    accept it when the tagged fraction is >= eff and no side exceeds
-    max_blocks; otherwise cut at a hole of the signature nearest the centre,
-    else at the strongest inflection of the signature, else in the middle of
-    the longest side, and recurse."""
+    accept it when the tagged fraction is >= eff, no side exceeds max_blocks
+    and (if given) every block of it is `allowed`; otherwise cut at a hole of
+    the signature nearest the centre, else at the strongest inflection of the
+    signature, else in the middle of the longest side, and recurse.  Tagged
+    blocks must be allowed."""
     out = []
     stack = [((0, 0, 0), tuple(n - 1 for n in tags.shape))]
     while stack:
@@ -198,7 +199,8 @@ def cluster_blocks(tags, max_blocks, eff=0.7):
         lo, hi = tuple(nlo), tuple(nhi)
         sub = tags[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1]
         size = [hi[d] - lo[d] + 1 for d in range(3)]
-        if sub.sum() >= eff * sub.size and max(size) <= max_blocks:
+        inside = allowed is None or bool(allowed[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1].all())
+        if sub.sum() >= eff * sub.size and max(size) <= max_blocks and inside:
             out.append((lo, hi))
             continue
         cut = None
@@ -246,13 +248,28 @@ def _intersect(a, b):
     return (lo, hi) if all(lo[d] <= hi[d] for d in range(3)) else None
 
 
-def tgv_amr_hierarchy(n_base=128, nlev=2, ratio=2, pct=70.0, blocking=8, max_grid=64, base_grid=64, eff=0.7):
+def _erode_periodic(mask, w):
+    """Cells of `mask` whose (2w+1)^3 neighbourhood (periodic) lies in it."""
+    out = mask.copy()
+    for d in range(3):
+        m = out.copy()
+        for s in range(1, w + 1):
+            m &= np.roll(out, s, axis=d) & np.roll(out, -s, axis=d)
+        out = m
+    return out
+
+
+def tgv_amr_hierarchy(n_base=128, nlev=2, ratio=2, pct=70.0, blocking=8, max_grid=64, base_grid=64, eff=0.7,
+                      nest=0):
     """Levels of the batched-AMR TGV workload on [0, 2 pi]^3: a uniform base
     level, then `nlev` refined levels (ratio 2).  On each level the cells
     whose analytic |omega| (t = 0) exceeds the level's `pct` percentile are
     tagged; tags are clustered in blocks of blocking/ratio coarse cells into
     boxes of at most max_grid fine cells, refined, and clipped to the coarse
-    level's boxes (proper nesting).  Returns [(geom, boxes)] per level."""
+    level's boxes.  With nest > 0 every fine box also stays `nest` coarse
+    cells inside the coarse level's union (tags outside are dropped, clusters
+    cut until they fit), as coarse-fine interpolation needs for ghost cells
+    of width 3 (nest = 3).  Returns [(geom, boxes)] per level."""
     levels = []
     n = (n_base,) * 3
     g = geom(n, (0, 0, 0), (2 * math.pi,) * 3, (1, 1, 1))
@@ -277,7 +294,15 @@ def tgv_amr_hierarchy(n_base=128, nlev=2, ratio=2, pct=70.0, blocking=8, max_gri
             blk = t.reshape(sh[0], cb, sh[1], cb, sh[2], cb).any(axis=(1, 3, 5))
             tags[lo[0] // cb:lo[0] // cb + sh[0], lo[1] // cb:lo[1] // cb + sh[1],
                  lo[2] // cb:lo[2] // cb + sh[2]] |= blk
-        clusters = cluster_blocks(tags, mb, eff)
+        allowed = None
+        if nest > 0:
+            cover = np.zeros(n, dtype=bool)
+            for lo, hi in boxes:
+                cover[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1] = True
+            inner = _erode_periodic(cover, nest)
+            allowed = inner.reshape(nb[0], cb, nb[1], cb, nb[2], cb).all(axis=(1, 3, 5))
+            tags &= allowed
+        clusters = cluster_blocks(tags, mb, eff, allowed)
         fine = []
         for blo, bhi in clusters:
             clo = tuple(blo[d] * cb for d in range(3))

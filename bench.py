@@ -52,7 +52,7 @@ def parse():
                     help="only run plain device copies of this many bytes (profiler calibration)")
     ap.add_argument("--workload", type=str, default="tgv",
                     choices=["tgv", "sod-w", "sod-r", "riemann-batch-w", "riemann-batch-r", "adjoint", "amr",
-                             "channel"])
+                             "amr-hierarchy", "channel"])
     ap.add_argument("--tiling", type=str, default="auto",
                     help="fused-kernel tiling: auto (autotune), default, or TX,TY,TZ")
     ap.add_argument("--traffic", type=str, default=os.path.join(ROOT, "profiles", "traffic_tgv256_keep4v.json"))
@@ -358,6 +358,34 @@ def channel_timing(a, steps=20, reps=3):
             "finite": bool(np.isfinite(u).all())}
 
 
+def tgv_device(cls, g, ng=3):
+    """The analytic TGV field on the whole ghosted level array, on the GPU
+    (same formulas as problems.tgv_state; float64)."""
+    import torch
+    n = [g.dom_hi[d] + 1 for d in range(3)]
+    dev = torch.device("cuda", 0)
+    ax = [g.prob_lo[d] + (torch.arange(-ng, n[d] + ng, dtype=torch.float64, device=dev) + 0.5) * g.dx[d]
+          for d in range(3)]
+    Z, Y, X = torch.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    T0, mach, Re = 110.4, 0.1, 1600.0
+    cs = math.sqrt(cls.gamma * cls.Rspec * T0)
+    v0 = mach * cs
+    mu = cls.visc_ref * T0 * math.sqrt(T0) / (cls.Tvisc_ref + T0)
+    rho0 = Re * mu / v0
+    p0 = rho0 * cls.Rspec * T0
+    u0 = v0 * torch.sin(X) * torch.cos(Y) * torch.cos(Z)
+    u1 = -v0 * torch.cos(X) * torch.sin(Y) * torch.cos(Z)
+    p = p0 + rho0 * v0 * v0 / 16.0 * (torch.cos(2.0 * X) + torch.cos(2.0 * Y)) * (torch.cos(2.0 * Z) + 2.0)
+    rho = p / (cls.Rspec * T0)
+    out = torch.empty((5,) + tuple(X.shape), dtype=torch.float64, device=dev)
+    out[0] = rho
+    out[1] = rho * u0
+    out[2] = rho * u1
+    out[3] = 0.0
+    out[4] = rho * (cls.cv * T0 + 0.5 * (u0 * u0 + u1 * u1))
+    return out
+
+
 def amr_timing(a, n_base=128, steps=5, reps=3):
     """Batched-AMR TGV levels (base n_base^3, 2 levels of ratio 2, tags where
     the analytic |omega| exceeds the level's 70th percentile, blocking factor
@@ -372,32 +400,6 @@ def amr_timing(a, n_base=128, steps=5, reps=3):
     sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
     ctx = LV.Context(0)
 
-    def tgv_device(g, ng=3):
-        """The analytic TGV field on the whole ghosted level array, on the GPU
-        (same formulas as problems.tgv_state; float64)."""
-        n = [g.dom_hi[d] + 1 for d in range(3)]
-        dev = torch.device("cuda", 0)
-        ax = [g.prob_lo[d] + (torch.arange(-ng, n[d] + ng, dtype=torch.float64, device=dev) + 0.5) * g.dx[d]
-              for d in range(3)]
-        Z, Y, X = torch.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
-        T0, mach, Re = 110.4, 0.1, 1600.0
-        cs = math.sqrt(cls.gamma * cls.Rspec * T0)
-        v0 = mach * cs
-        mu = cls.visc_ref * T0 * math.sqrt(T0) / (cls.Tvisc_ref + T0)
-        rho0 = Re * mu / v0
-        p0 = rho0 * cls.Rspec * T0
-        u0 = v0 * torch.sin(X) * torch.cos(Y) * torch.cos(Z)
-        u1 = -v0 * torch.cos(X) * torch.sin(Y) * torch.cos(Z)
-        p = p0 + rho0 * v0 * v0 / 16.0 * (torch.cos(2.0 * X) + torch.cos(2.0 * Y)) * (torch.cos(2.0 * Z) + 2.0)
-        rho = p / (cls.Rspec * T0)
-        out = torch.empty((5,) + tuple(X.shape), dtype=torch.float64, device=dev)
-        out[0] = rho
-        out[1] = rho * u0
-        out[2] = rho * u1
-        out[3] = 0.0
-        out[4] = rho * (cls.cv * T0 + 0.5 * (u0 * u0 + u1 * u1))
-        return out
-
     def time_level(g, boxes, layout, dt):
         lev = LV.Level(g, boxes, cls, sch, ctx=ctx)
         S = lev.new_state(layout=layout)
@@ -405,7 +407,7 @@ def amr_timing(a, n_base=128, steps=5, reps=3):
             for b, (lo, hi) in enumerate(lev.boxes):
                 S.upload_box(b, P.tgv_state_box(cls, g, lo, hi, 3))
         else:
-            S.upload_ghosted(tgv_device(g))
+            S.upload_ghosted(tgv_device(cls, g))
         lev.fill_patch(S)
         tiling = lev.autotune(S)
         lev.advance_graph(S, dt, steps)
@@ -446,6 +448,103 @@ def amr_timing(a, n_base=128, steps=5, reps=3):
             "uniform_256_level_layout": uni_level, "uniform_256_box_layout": uni_boxes, "levels": levels}
 
 
+def hierarchy_timing(a, layout, n_base=128, steps=3, reps=2, fill_reps=20):
+    """The TGV AMR hierarchy with levels nested 3 coarse cells inside each
+    other (base n_base^3, 2 levels of ratio 2, |omega| > 70th percentile,
+    blocking 8), advanced by whole coarse steps with subcycling: every stage
+    of every level fills its ghost cells (coarse-fine ones interpolated in
+    time and space from the level below), levels 1 and 2 take 2 and 4 steps
+    per coarse step, and each finer level is averaged down after its steps.
+    Throughput = cell-stages of all levels per second.  Also reported: the
+    same substeps of each level without coupling (amr_advance holding the
+    coarse-fine ghost cells), and single calls of the coupling kernels."""
+    import torch
+    from amrsolver_amd import level as LV
+    from amrsolver_amd import problems as P
+    cls = P.tgv_closures()
+    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
+    ctx = LV.Context(0)
+    levels = P.tgv_amr_hierarchy(n_base=n_base, nlev=2, nest=3)
+    print("hierarchy: levels built", file=sys.stderr, flush=True)
+    Ls = [LV.Level(g, boxes, cls, sch, ctx=ctx) for g, boxes in levels]
+    H = LV.Hierarchy(Ls, layout=layout)
+
+    def upload(S, l):
+        g, boxes = levels[l]
+        if layout == "level":
+            S.upload_ghosted(tgv_device(cls, g))
+        else:
+            for b, (lo, hi) in enumerate(boxes):
+                S.upload_box(b, P.tgv_state_box(cls, g, lo, hi, 3))
+
+    for l in range(len(levels)):
+        upload(H.cur[l], l)
+    print("hierarchy: states uploaded", file=sys.stderr, flush=True)
+    tilings = [L.autotune(H.cur[l]) for l, L in enumerate(Ls)]
+    print("hierarchy: autotuned", file=sys.stderr, flush=True)
+    dt0 = DT * 256 / n_base
+    t = 0.0
+    H.advance(t, dt0)
+    t += dt0
+    ctx.synchronize()
+    best = 1e30
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            H.advance(t, dt0)
+            t += dt0
+        ctx.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    ms = best / steps * 1e3
+    print(f"hierarchy: {ms:.2f} ms per coarse step", file=sys.stderr, flush=True)
+    cells = [sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) for lo, hi in boxes)
+             for _, boxes in levels]
+    cell_stages = sum(c * 3 * 2 ** l for l, c in enumerate(cells))
+    # the same substeps without coupling: amr_advance per level
+    plain = []
+    for l, L in enumerate(Ls):
+        S = H.tmp[l]
+        upload(S, l)
+        L.fill_patch(S)
+        L.advance(S, 0.0, dt0 / 2 ** l)
+        ctx.synchronize()
+        bl = 1e30
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            for _ in range(steps * 2 ** l):
+                L.advance(S, 0.0, dt0 / 2 ** l)
+            ctx.synchronize()
+            bl = min(bl, time.perf_counter() - t0)
+        plain.append(bl / steps * 1e3)
+    # single coupling calls on the finest level (time-blended fill; average-down)
+    top, below = len(Ls) - 1, len(Ls) - 2
+    fill_ms = avg_ms = 0.0
+    for kind in ("fill", "avg"):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(fill_reps):
+            if kind == "fill":
+                Ls[top].fill_two_levels(H.tmp[top], Ls[below], H.old[below], H.cur[below], 0.5, 0.5, 2)
+            else:
+                Ls[top].average_down(H.cur[top], Ls[below], H.tmp[below], 2)
+        ctx.synchronize()
+        v = (time.perf_counter() - t0) / fill_reps * 1e3
+        if kind == "fill":
+            fill_ms = v
+        else:
+            avg_ms = v
+    return {"workload": f"TGV AMR hierarchy base {n_base}^3 + 2 levels (ratio 2, |omega| > 70th percentile, "
+                        f"blocking 8, boxes 8..64, nested 3 coarse cells), KEEP4+viscous, SSPRK(3,3), subcycled "
+                        f"(1/2/4 steps per coarse step), "
+                        + ("one array per level (boxes are windows)" if layout == "level" else
+                           "one ghosted array per box"),
+            "metric": "FP64 cell-updates/sec per RK stage, all levels", "unit": "cell-stage/s",
+            "value": cell_stages / (ms * 1e-3), "ms_per_coarse_step": ms, "steps": steps,
+            "boxes": [len(b) for _, b in levels], "cells": cells, "tilings": tilings,
+            "levels_without_coupling_ms": plain, "coupling_overhead": ms / sum(plain) - 1.0,
+            "finest_fill_two_levels_ms": fill_ms, "finest_average_down_ms": avg_ms}
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -456,6 +555,13 @@ def main():
         return 2
     if a.workload == "amr":
         print(json.dumps(amr_timing(a)))
+        return 0
+    if a.workload == "amr-hierarchy":
+        for layout in ("level", "boxes"):
+            print(json.dumps(hierarchy_timing(a, layout)), flush=True)
+            torch_mod = sys.modules.get("torch")
+            if torch_mod is not None:
+                torch_mod.cuda.empty_cache()
         return 0
     if a.workload == "channel":
         print(json.dumps(channel_timing(a)))

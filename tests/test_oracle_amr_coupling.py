@@ -359,3 +359,36 @@ def test_hierarchy_three_levels_runs_and_stays_finite():
     H.advance(cls, P.scheme(**SCH), 0.0, 2e-3, nthreads=8)
     for lev in range(3):
         assert np.all(np.isfinite(H.get_state(lev)))
+
+
+def test_hierarchy_generator_nesting():
+    """With nest=3 every coarse stencil of every fine ghost cell lies in the
+    coarse level (what the interpolation needs); without it level 2 of the
+    TGV hierarchy is not nested that deeply."""
+    def dilate(m, w):
+        out = m.copy()
+        for d in range(3):
+            acc = out.copy()
+            for s in range(1, w + 1):
+                acc |= np.roll(out, s, axis=d) | np.roll(out, -s, axis=d)
+            out = acc
+        return out
+
+    def cover(g, boxes):
+        m = np.zeros([g.dom_hi[d] + 1 for d in range(3)], dtype=bool)
+        for lo, hi in boxes:
+            m[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1] = True
+        return m
+
+    def nested(levels):
+        out = []
+        for lev in range(1, len(levels)):
+            C, F = cover(*levels[lev - 1]), cover(*levels[lev])
+            ghost = dilate(F, 3) & ~F
+            n = ghost.shape
+            need = dilate(ghost.reshape(n[0] // 2, 2, n[1] // 2, 2, n[2] // 2, 2).any(axis=(1, 3, 5)), 1)
+            out.append(not (need & ~C).any())
+        return out
+
+    assert nested(P.tgv_amr_hierarchy(64, 2, nest=3)) == [True, True]
+    assert nested(P.tgv_amr_hierarchy(64, 2)) != [True, True]
