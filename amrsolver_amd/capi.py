@@ -67,6 +67,10 @@ SIGNATURES = {
     "amr_fill_two_levels": ([_P, _P, ctypes.c_int, ctypes.c_double, _PF, ctypes.c_double, _PF, _PF],
                             ctypes.c_int),
     "amr_average_down": ([_P, _P, ctypes.c_int, _PF, _PF], ctypes.c_int),
+    "amr_plot_vars": ([_P, ctypes.POINTER(Closures), _PF, ctypes.c_int, ctypes.c_int, _PF], ctypes.c_int),
+    "amr_write_plotfile": ([ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_P), ctypes.POINTER(_PF), ctypes.c_int,
+                            ctypes.POINTER(ctypes.c_char_p), ctypes.c_double, ctypes.POINTER(ctypes.c_int32),
+                            ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     "amr_rhs": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF, _PF], ctypes.c_int),
     "amr_rk_stage": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), ctypes.c_int,
                       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,

@@ -8,10 +8,15 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cerrno>
+#include <fstream>
+#include <sstream>
 #include <map>
 #include <memory>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include "amrsolver_hip.h"
 #include "kernels.h"
@@ -2525,6 +2530,169 @@ int amr_adjoint_run(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme
   HIPCHK(launch_copy_valid(bt->bt, lc->d, dg->d, nullptr, st));
   HIPCHK(hipStreamSynchronize(st));
   if (J_out) *J_out = bt->h_result[0];
+  return AMR_OK;
+}
+
+
+int amr_plot_vars(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, int state_vars,
+                  int derive_mask, const amr_fab_t* out) {
+  if (!bt || !cls || !state || !out) return fail(AMR_ERR_ARG, "null argument");
+  if (derive_mask < 0 || derive_mask > 31) return fail(AMR_ERR_ARG, "derive_mask uses bits 0..4 only");
+  if (!state_vars && !derive_mask) return fail(AMR_ERR_ARG, "no plot variable selected");
+  int rc;
+  if ((rc = check_closures(*cls))) return rc;
+  DescSet *ds = nullptr, *dout = nullptr;
+  if ((rc = get_desc(bt, state, 0, &ds))) return rc;
+  if ((rc = get_desc(bt, out, 0, &dout))) return rc;
+  for (int b = 0; b < bt->bt.nbox; ++b)
+    if (mapping_of(out[b]) == mapping_of(state[b])) return fail(AMR_ERR_ARG, "out must not alias state");
+  HIPCHK(launch_plot_vars(bt->bt, ds->d, dout->d, state_vars ? 1 : 0, derive_mask, make_closures(*cls),
+                          bt->ctx->stream));
+  return AMR_OK;
+}
+
+int amr_write_plotfile(const char* dir, int nlev, const amr_boxtab_t* levels, const amr_fab_t* const* plot,
+                       int ncomp, const char* const* names, double time, const int32_t* level_steps,
+                       const int32_t* ref_ratio) {
+  if (!dir || !levels || !plot || !names || !level_steps || nlev < 1 || ncomp < 1)
+    return fail(AMR_ERR_ARG, "invalid argument");
+  if (nlev > 1 && !ref_ratio) return fail(AMR_ERR_ARG, "ref_ratio is required with more than one level");
+  for (int n = 0; n < ncomp; ++n)
+    if (!names[n]) return fail(AMR_ERR_ARG, "null variable name");
+  for (int l = 0; l < nlev; ++l) {
+    if (!levels[l] || !plot[l]) return fail(AMR_ERR_ARG, "null level");
+    if (levels[l]->batch) return fail(AMR_ERR_ARG, "plotfiles are written for level box tables, not batches");
+    if (l > 0) {
+      const amr_geom_t& gf = levels[l]->geoms[0];
+      const amr_geom_t& gc = levels[l - 1]->geoms[0];
+      const int r = ref_ratio[l - 1];
+      for (int d = 0; d < 3; ++d)
+        if (r < 1 || gf.dom_lo[d] != gc.dom_lo[d] * r || gf.dom_hi[d] != (gc.dom_hi[d] + 1) * r - 1)
+          return fail(AMR_ERR_ARG, "level " + std::to_string(l) + " is not level " + std::to_string(l - 1) +
+                                       " refined by ref_ratio");
+    }
+  }
+  auto box_text = [](const int32_t* lo, const int32_t* hi) {
+    std::ostringstream o;
+    o << "((" << lo[0] << ',' << lo[1] << ',' << lo[2] << ") (" << hi[0] << ',' << hi[1] << ',' << hi[2]
+      << ") (0,0,0))";
+    return o.str();
+  };
+  auto make_dir = [](const std::string& d) {
+    if (mkdir(d.c_str(), 0755) != 0 && errno != EEXIST) return fail(AMR_ERR_ARG, "cannot create directory " + d);
+    return int(AMR_OK);
+  };
+  const std::string root(dir);
+  int rc;
+  if ((rc = make_dir(root))) return rc;
+  // per level: pack the valid cells on the device, copy them to the host,
+  // write Level_l/Cell_D_00000 and Level_l/Cell_H (one data file per level)
+  std::vector<std::vector<std::string>> level_boxes(nlev);
+  for (int l = 0; l < nlev; ++l) {
+    amr_boxtab_s* bt = levels[l];
+    const int nb = bt->bt.nbox;
+    DescSet* ds = nullptr;
+    if ((rc = get_desc(bt, plot[l], 0, &ds))) return rc;
+    std::vector<int64_t> off(nb + 1, 0);
+    for (int b = 0; b < nb; ++b) {
+      const BoxDev& B = bt->bt.h_boxes[b];
+      off[b + 1] = off[b] + int64_t(B.hi[0] - B.lo[0] + 1) * (B.hi[1] - B.lo[1] + 1) * (B.hi[2] - B.lo[2] + 1) * ncomp;
+    }
+    int64_t* d_off = nullptr;
+    double* d_buf = nullptr;
+    std::vector<double> h(static_cast<size_t>(off[nb]));
+    HIPCHK(hipMalloc(&d_off, sizeof(int64_t) * (nb + 1)));
+    HIPCHK(hipMemcpy(d_off, off.data(), sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice));
+    hipError_t e = hipMalloc(&d_buf, sizeof(double) * std::max<int64_t>(off[nb], 1));
+    if (e == hipSuccess) e = launch_pack_valid(bt->bt, ds->d, ncomp, d_off, d_buf, bt->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(bt->ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), d_buf, sizeof(double) * off[nb], hipMemcpyDeviceToHost);
+    if (d_buf) (void)hipFree(d_buf);
+    (void)hipFree(d_off);
+    if (e != hipSuccess) return fail(AMR_ERR_HIP, std::string("plotfile copy: ") + hipGetErrorString(e));
+    const std::string ldir = root + "/Level_" + std::to_string(l);
+    if ((rc = make_dir(ldir))) return rc;
+    std::ofstream fd(ldir + "/Cell_D_00000", std::ios::out | std::ios::trunc | std::ios::binary);
+    if (!fd) return fail(AMR_ERR_ARG, "cannot write " + ldir + "/Cell_D_00000");
+    std::vector<int64_t> fpos(nb);
+    std::vector<double> mn(size_t(nb) * ncomp), mx(size_t(nb) * ncomp);
+    for (int b = 0; b < nb; ++b) {
+      const BoxDev& B = bt->bt.h_boxes[b];
+      const std::string bs = box_text(B.lo, B.hi);
+      level_boxes[l].push_back(bs);
+      fpos[b] = int64_t(fd.tellp());
+      fd << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" << bs << ' ' << ncomp << '\n';
+      const int64_t nc = (off[b + 1] - off[b]) / ncomp;
+      const double* p = h.data() + off[b];
+      fd.write(reinterpret_cast<const char*>(p), std::streamsize(sizeof(double) * (off[b + 1] - off[b])));
+      for (int n = 0; n < ncomp; ++n) {
+        double lo = p[n * nc], hi = p[n * nc];
+        for (int64_t t = 1; t < nc; ++t) {
+          lo = std::min(lo, p[n * nc + t]);
+          hi = std::max(hi, p[n * nc + t]);
+        }
+        mn[size_t(b) * ncomp + n] = lo;
+        mx[size_t(b) * ncomp + n] = hi;
+      }
+    }
+    fd.close();
+    if (!fd) return fail(AMR_ERR_ARG, "write failed: " + ldir + "/Cell_D_00000");
+    std::ofstream fh(ldir + "/Cell_H", std::ios::out | std::ios::trunc);
+    fh.precision(17);
+    fh << 1 << '\n' << 1 << '\n' << ncomp << '\n' << 0 << '\n';
+    fh << '(' << nb << ' ' << 0 << '\n';
+    for (const std::string& bs : level_boxes[l]) fh << bs << '\n';
+    fh << ")\n";
+    fh << nb << '\n';
+    for (int b = 0; b < nb; ++b) fh << "FabOnDisk: Cell_D_00000 " << fpos[b] << '\n';
+    fh << '\n' << nb << ',' << ncomp << '\n';
+    for (int b = 0; b < nb; ++b) {
+      for (int n = 0; n < ncomp; ++n) fh << mn[size_t(b) * ncomp + n] << ',';
+      fh << '\n';
+    }
+    fh << '\n' << nb << ',' << ncomp << '\n';
+    for (int b = 0; b < nb; ++b) {
+      for (int n = 0; n < ncomp; ++n) fh << mx[size_t(b) * ncomp + n] << ',';
+      fh << '\n';
+    }
+    fh.close();
+    if (!fh) return fail(AMR_ERR_ARG, "write failed: " + ldir + "/Cell_H");
+  }
+  // the plotfile Header (CNS::writePlotFile, src/CNS.cpp:688-807)
+  std::ofstream os(root + "/Header", std::ios::out | std::ios::trunc);
+  os.precision(15);
+  const amr_geom_t& g0 = levels[0]->geoms[0];
+  os << "HyperCLaw-V1.1\n" << ncomp << '\n';
+  for (int n = 0; n < ncomp; ++n) os << names[n] << '\n';
+  os << 3 << '\n' << time << '\n' << nlev - 1 << '\n';
+  for (int d = 0; d < 3; ++d) os << g0.prob_lo[d] << ' ';
+  os << '\n';
+  for (int d = 0; d < 3; ++d) os << g0.prob_hi[d] << ' ';
+  os << '\n';
+  for (int l = 0; l + 1 < nlev; ++l) os << ref_ratio[l] << ' ';
+  os << '\n';
+  for (int l = 0; l < nlev; ++l) os << box_text(levels[l]->geoms[0].dom_lo, levels[l]->geoms[0].dom_hi) << ' ';
+  os << '\n';
+  for (int l = 0; l < nlev; ++l) os << level_steps[l] << ' ';
+  os << '\n';
+  for (int l = 0; l < nlev; ++l) {
+    for (int d = 0; d < 3; ++d) os << levels[l]->geoms[0].dx[d] << ' ';
+    os << '\n';
+  }
+  os << 0 << '\n' << 0 << '\n';
+  for (int l = 0; l < nlev; ++l) {
+    amr_boxtab_s* bt = levels[l];
+    const amr_geom_t& g = bt->geoms[0];
+    os << l << ' ' << bt->bt.nbox << ' ' << time << '\n' << level_steps[l] << '\n';
+    for (int b = 0; b < bt->bt.nbox; ++b) {
+      const BoxDev& B = bt->bt.h_boxes[b];
+      for (int d = 0; d < 3; ++d)
+        os << g.prob_lo[d] + B.lo[d] * g.dx[d] << ' ' << g.prob_lo[d] + (B.hi[d] + 1) * g.dx[d] << '\n';
+    }
+    os << "Level_" << l << "/Cell\n";
+  }
+  os.close();
+  if (!os) return fail(AMR_ERR_ARG, "write failed: " + root + "/Header");
   return AMR_OK;
 }
 

@@ -126,6 +126,59 @@ __global__ void __launch_bounds__(FBLK) k_copy_valid(const BoxDev* boxes, const 
   for (int n = 0; n < NCONS; ++n) d(i, j, k, n) = s(i, j, k, n);
 }
 
+// Plot variables of CNS::writePlotFile's plotMF: the five state components
+// when `state_vars`, then the derived variables selected by `mask` in the
+// order src/set/CNS_setup.cpp:156-175 registers them (bit 0 pressure, 1
+// temperature, 2..4 x/y/z velocity), each as derpres, dertemp and dervel
+// compute it (src/cls/CNS_K.h:76-129).
+__global__ void __launch_bounds__(FBLK) k_plot_vars(const BoxDev* boxes, const Fab* src, const Fab* dst,
+                                                    int state_vars, int mask, double gm1, double cv) {
+  const int b = blockIdx.y;
+  const BoxDev bx = boxes[b];
+  int nx = bx.hi[0] - bx.lo[0] + 1, ny = bx.hi[1] - bx.lo[1] + 1, nz = bx.hi[2] - bx.lo[2] + 1;
+  int64_t t = int64_t(blockIdx.x) * FBLK + threadIdx.x;
+  if (t >= int64_t(nx) * ny * nz) return;
+  int i = bx.lo[0] + int(t % nx);
+  int64_t r = t / nx;
+  int j = bx.lo[1] + int(r % ny);
+  int k = bx.lo[2] + int(r / ny);
+  const Fab s = src[b], d = dst[b];
+  double u[NCONS];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) u[n] = s(i, j, k, n);
+  int c = 0;
+  if (state_vars) {
+#pragma unroll
+    for (int n = 0; n < NCONS; ++n) d(i, j, k, c++) = u[n];
+  }
+  const double rhoinv = 1.0 / u[URHO];
+  const double rhoeint = u[UET] - 0.5 * rhoinv * (u[UMX] * u[UMX] + u[UMY] * u[UMY] + u[UMZ] * u[UMZ]);
+  if (mask & 1) d(i, j, k, c++) = gm1 * rhoeint;
+  if (mask & 2) d(i, j, k, c++) = (rhoeint * rhoinv) / cv;
+  if (mask & 4) d(i, j, k, c++) = u[UMX] / u[URHO];
+  if (mask & 8) d(i, j, k, c++) = u[UMY] / u[URHO];
+  if (mask & 16) d(i, j, k, c++) = u[UMZ] / u[URHO];
+}
+
+// Valid cells of every box packed into one buffer: box b's cells start at
+// off[b], component-major, x fastest within a component (the order of a FAB
+// on disk).
+__global__ void __launch_bounds__(FBLK) k_pack_valid(const BoxDev* boxes, const Fab* src, int ncomp,
+                                                     const int64_t* off, double* out) {
+  const int b = blockIdx.y;
+  const BoxDev bx = boxes[b];
+  int nx = bx.hi[0] - bx.lo[0] + 1, ny = bx.hi[1] - bx.lo[1] + 1, nz = bx.hi[2] - bx.lo[2] + 1;
+  const int64_t ncell = int64_t(nx) * ny * nz;
+  int64_t t = int64_t(blockIdx.x) * FBLK + threadIdx.x;
+  if (t >= ncell) return;
+  int i = bx.lo[0] + int(t % nx);
+  int64_t r = t / nx;
+  int j = bx.lo[1] + int(r % ny);
+  int k = bx.lo[2] + int(r / ny);
+  const Fab s = src[b];
+  for (int n = 0; n < ncomp; ++n) out[off[b] + n * ncell + t] = s(i, j, k, n);
+}
+
 constexpr int RBLK = 256;
 
 // cns_estdt (src/cls/CNS_K.h:12-47; z momentum read from UMY, CNS_K.h:29):
@@ -419,6 +472,26 @@ __global__ void __launch_bounds__(FBLK) k_avg_down(const CopyTask* tasks, const 
         for (int ir = 0; ir < r; ++ir) s += F(c[0] * r + ir, c[1] * r + jr, c[2] * r + kr, n);
     C(c[0], c[1], c[2], n) = volfrac * s;
   }
+}
+
+hipError_t launch_plot_vars(const BoxTable& bt, const Fab* src, const Fab* dst, int state_vars, int mask,
+                            const Closures& c, hipStream_t st) {
+  int64_t mx = 0;
+  for (auto& b : bt.h_boxes)
+    mx = std::max<int64_t>(mx, int64_t(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1));
+  hipLaunchKernelGGL(k_plot_vars, dim3(unsigned((mx + FBLK - 1) / FBLK), unsigned(bt.nbox)), dim3(FBLK), 0, st,
+                     bt.d_boxes, src, dst, state_vars, mask, c.gm1, c.cv);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_valid(const BoxTable& bt, const Fab* src, int ncomp, const int64_t* off, double* out,
+                             hipStream_t st) {
+  int64_t mx = 0;
+  for (auto& b : bt.h_boxes)
+    mx = std::max<int64_t>(mx, int64_t(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1));
+  hipLaunchKernelGGL(k_pack_valid, dim3(unsigned((mx + FBLK - 1) / FBLK), unsigned(bt.nbox)), dim3(FBLK), 0, st,
+                     bt.d_boxes, src, ncomp, off, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_interp_cf(const InterpTask* tasks, const TaskBlock* blocks, int nblk, const Fab* c0, const Fab* c1,

@@ -242,6 +242,13 @@ hipError_t launch_avg_down(const CopyTask* tasks, const TaskBlock* blocks, int n
 // dst = src on the valid cells of every box.
 hipError_t launch_copy_valid(const BoxTable& bt, const Fab* src, const Fab* dst, const double* box_dt,
                              hipStream_t st);
+// Plot variables (state components, then derived pressure, temperature and
+// velocities selected by `mask` bits 0..4) of every box's valid cells.
+hipError_t launch_plot_vars(const BoxTable& bt, const Fab* src, const Fab* dst, int state_vars, int mask,
+                            const Closures& c, hipStream_t st);
+// Valid cells of every box into one buffer (box b at off[b], component-major).
+hipError_t launch_pack_valid(const BoxTable& bt, const Fab* src, int ncomp, const int64_t* off, double* out,
+                             hipStream_t st);
 
 // Reductions over valid cells.  `partial` holds at least reduce_partials()
 // doubles (x5 for totals); the result lands in result[0] (dt) or result[0..4].

@@ -257,6 +257,13 @@ class Level:
         call("amr_fill_two_levels", self.h, coarse.h, ratio, a, None if crse_old is None else crse_old.fabs, b,
              None if crse_new is None else crse_new.fabs, S.fabs)
 
+    def plot_vars(self, S, out, state_vars=True, derive=()):
+        """Plot variables of S into out (valid cells): the state components
+        if state_vars, then the derived variables named in `derive`, in the
+        order CNS registers them (DERIVED_NAMES)."""
+        call("amr_plot_vars", self.h, ctypes.byref(self.closures), S.fabs, int(bool(state_vars)),
+             derive_mask(derive), out.fabs)
+
     def average_down(self, S, coarse, Sc, ratio=2):
         """CNS::avgDown: coarse cells of Sc under this level := mean of S."""
         call("amr_average_down", self.h, coarse.h, ratio, S.fabs, Sc.fabs)
@@ -348,6 +355,46 @@ def compute_new_dt(est, dt_prev, cur_time, stop_time, first):
     if stop_time >= 0.0 and (cur_time + dt0) > (stop_time - eps):
         dt0 = stop_time - cur_time
     return dt0
+
+
+STATE_NAMES = ("density", "xmom", "ymom", "zmom", "energy")
+DERIVED_NAMES = ("pressure", "temperature", "x_velocity", "y_velocity", "z_velocity")
+
+
+def derive_mask(derive):
+    m = 0
+    for name in derive:
+        if name not in DERIVED_NAMES:
+            raise ValueError(f"unknown derived variable {name!r}")
+        m |= 1 << DERIVED_NAMES.index(name)
+    return m
+
+
+def plot_names(state_vars=True, derive=()):
+    """Variable names of a plotfile, in the order CNS::writePlotFile writes
+    them: state components, then derived variables in registration order."""
+    m = derive_mask(derive)
+    return (list(STATE_NAMES) if state_vars else []) + [n for b, n in enumerate(DERIVED_NAMES) if m >> b & 1]
+
+
+def write_plotfile(path, levels, states, time, level_steps, ref_ratio=2, state_vars=True, derive=()):
+    """CNS::writePlotFile for levels 0..L (src/CNS.cpp:644-874): plot
+    variables per level on the device, then the AMReX plotfile (Header,
+    Level_l/Cell_H, Level_l/Cell_D_00000) under `path`."""
+    names = plot_names(state_vars, derive)
+    outs = []
+    for lev, S in zip(levels, states):
+        out = lev.new_state(layout=S.layout, ng=0, ncomp=len(names))
+        lev.plot_vars(S, out, state_vars, derive)
+        outs.append(out)
+    n = len(levels)
+    hs = (ctypes.c_void_p * n)(*[lev.h.value for lev in levels])
+    fabs = (ctypes.POINTER(Fab) * n)(*[ctypes.cast(o.fabs, ctypes.POINTER(Fab)) for o in outs])
+    cn = (ctypes.c_char_p * len(names))(*[x.encode() for x in names])
+    steps = (ctypes.c_int32 * n)(*level_steps)
+    rr = (ctypes.c_int32 * max(n - 1, 1))(*([ref_ratio] * max(n - 1, 1)))
+    call("amr_write_plotfile", str(path).encode(), n, hs, fabs, len(names), cn, time, steps, rr)
+    return names
 
 
 def fill_times(order_rk, stages_rk, time, dt):

@@ -533,6 +533,21 @@ def hierarchy_timing(a, layout, n_base=128, steps=3, reps=2, fill_reps=20):
             fill_ms = v
         else:
             avg_ms = v
+    # a plotfile of the hierarchy (state + the five derived variables), timed
+    # whole: plot variables on the device, packing, copy to the host, files
+    import shutil
+    import tempfile
+    tmpd = tempfile.mkdtemp(prefix="amr_plt_")
+    try:
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        names = LV.write_plotfile(os.path.join(tmpd, "plt"), Ls, H.cur, t, [steps + 1, 2 * (steps + 1),
+                                                                            4 * (steps + 1)],
+                                  derive=LV.DERIVED_NAMES)
+        plot_s = time.perf_counter() - t0
+        plot_bytes = sum(os.path.getsize(os.path.join(dp, f)) for dp, _, fs in os.walk(tmpd) for f in fs)
+    finally:
+        shutil.rmtree(tmpd, ignore_errors=True)
     return {"workload": f"TGV AMR hierarchy base {n_base}^3 + 2 levels (ratio 2, |omega| > 70th percentile, "
                         f"blocking 8, boxes 8..64, nested 3 coarse cells), KEEP4+viscous, SSPRK(3,3), subcycled "
                         f"(1/2/4 steps per coarse step), "
@@ -542,7 +557,9 @@ def hierarchy_timing(a, layout, n_base=128, steps=3, reps=2, fill_reps=20):
             "value": cell_stages / (ms * 1e-3), "ms_per_coarse_step": ms, "steps": steps,
             "boxes": [len(b) for _, b in levels], "cells": cells, "tilings": tilings,
             "levels_without_coupling_ms": plain, "coupling_overhead": ms / sum(plain) - 1.0,
-            "finest_fill_two_levels_ms": fill_ms, "finest_average_down_ms": avg_ms}
+            "finest_fill_two_levels_ms": fill_ms, "finest_average_down_ms": avg_ms,
+            "plotfile": {"variables": len(names), "bytes": plot_bytes, "seconds": plot_s,
+                         "GB_per_s": plot_bytes / plot_s / 1e9}}
 
 
 def main():

@@ -145,6 +145,32 @@ int amr_fill_two_levels(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, double 
 int amr_average_down(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, const amr_fab_t* fine_state,
                      const amr_fab_t* crse_state);
 
+/* Plot variables of one level (the plotMF of CNS::writePlotFile,
+ * src/CNS.cpp:644-874): into `out` (valid cells), the five state components
+ * when state_vars != 0, then the derived variables whose bits are set in
+ * derive_mask, in the order src/set/CNS_setup.cpp:156-175 registers them:
+ * bit 0 pressure (derpres), 1 temperature (dertemp), 2..4 x/y/z velocity
+ * (dervel), computed as src/cls/CNS_K.h:76-129 computes them.  `out` holds
+ * 5*(state_vars != 0) + popcount(derive_mask) components and must not alias
+ * `state`. */
+int amr_plot_vars(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, int state_vars,
+                  int derive_mask, const amr_fab_t* out);
+
+/* AMReX plotfile of levels 0..nlev-1 (CNS::writePlotFile with
+ * Amr::writePlotFile's layout, src/CNS.cpp:644-874): `dir`/Header
+ * (HyperCLaw-V1.1: variable names, time, finest level, problem extent,
+ * ratios, domains, level steps, cell sizes, then per level its boxes in
+ * physical coordinates), and per level `dir`/Level_l/Cell_H (VisMF header:
+ * boxes, FAB offsets, per-FAB minima and maxima) and one data file
+ * Level_l/Cell_D_00000 (per box a FAB header line and the valid cells as
+ * little-endian doubles, component-major, x fastest).  plot[l] are level
+ * l's plot variables (amr_plot_vars), ncomp of them named `names`;
+ * level_steps[l] = steps taken by level l; ref_ratio[l] between levels l and
+ * l+1.  Synchronous; allocates a device staging buffer per level. */
+int amr_write_plotfile(const char* dir, int nlev, const amr_boxtab_t* levels, const amr_fab_t* const* plot,
+                       int ncomp, const char* const* names, double time, const int32_t* level_steps,
+                       const int32_t* ref_ratio);
+
 /* dSdt = L(state) for every box: CNS::compute_rhs (src/tim/CNS_advance.cpp:186-427)
  * with the per-scheme Flux functions (Riemann::Flux, Riemann.h:339;
  * HiRes::FluxWENO, High_resolution.h:279; CentralKEEP::FluxKEEP,
