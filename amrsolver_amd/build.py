@@ -1,7 +1,7 @@
 """Build recipe for the HIP library (gfx950) and the C++ host driver.
 
-    python -m amrsolver_amd.build            # library -> amrsolver_amd/lib/
-    python -m amrsolver_amd.build --driver   # also the cns_driver executable
+    python -m amrsolver_amd.build              # library and the cns_driver executable -> amrsolver_amd/lib/
+    python -m amrsolver_amd.build --no-driver  # the library only
 
 Device code is compiled with -ffp-contract=off (every operation rounds as
 the reference's CPU build does) and with packed-FP32 instructions disabled.
@@ -80,11 +80,11 @@ def build_driver(verbose=True):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--driver", action="store_true")
+    ap.add_argument("--no-driver", action="store_true", help="build the library only")
     ap.add_argument("-j", type=int, default=4)
     a = ap.parse_args(argv)
     build_library(a.j)
-    if a.driver:
+    if not a.no_driver:
         build_driver()
 
 
