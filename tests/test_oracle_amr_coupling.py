@@ -392,3 +392,29 @@ def test_hierarchy_generator_nesting():
 
     assert nested(P.tgv_amr_hierarchy(64, 2, nest=3)) == [True, True]
     assert nested(P.tgv_amr_hierarchy(64, 2)) != [True, True]
+
+
+@pytest.mark.parametrize("t", [0.0, 0.5, 1.0, 0.9999, 1e-4, 0.25, 0.75, 2.0 / 3.0])
+def test_host_weights_match_the_oracle(t):
+    """The host's coarse-level weights (level.cf_weights, used by the
+    hierarchy step) are the oracle's, bit for bit."""
+    from amrsolver_amd import level as LV
+    for t0, t1 in [(0.0, 1.0), (0.1, 0.3), (2.0, 2.0)]:
+        tt = t0 + t * (t1 - t0)
+        assert LV.cf_weights(t0, t1, tt) == O.cf_weights(t0, t1, tt)
+
+
+def test_host_fill_times_follow_the_stage_sequence():
+    """level.fill_times: one FillPatch time per RHS evaluation of each
+    scheme, the times CNS::advance passes (first stage at t, last at the
+    scheme's final abscissa)."""
+    from amrsolver_amd import capi
+    from amrsolver_amd import level as LV
+    t, dt = 0.5, 0.125
+    cases = {(-2, 0): [t, t + dt], (0, 0): [t], (1, 0): [t], (2, 2): [t, t + dt], (2, 3): [t, t + dt / 2, t + dt],
+             (2, 5): [t, t + dt / 4, t + dt / 2, t + 3 * dt / 4, t + dt], (3, 3): [t, t + dt, t + dt / 2],
+             (3, 4): [t, t + dt / 2, t + dt, t + dt / 2]}
+    for (o, s), want in cases.items():
+        got = LV.fill_times(o, s, t, dt)
+        assert got == want, (o, s, got)
+        assert capi.stage_coeffs(o, s, dt, 0)[0] == len(got)  # host-only entry point
