@@ -448,6 +448,33 @@ def amr_timing(a, n_base=128, steps=5, reps=3):
             "uniform_256_level_layout": uni_level, "uniform_256_box_layout": uni_boxes, "levels": levels}
 
 
+def cpu_hierarchy_baseline(n_base=64):
+    """The CPU oracle's subcycled hierarchy step (two-level fills with
+    conservative linear interpolation, average-down, the levels' steps with
+    OpenMP over boxes on every host core the process may use) on a bounded
+    sample of the hierarchy workload: the same generator at base n_base^3,
+    one coarse step."""
+    import os
+    from amrsolver_amd import problems as P
+    from oracle import oracle as O
+    cls = P.tgv_closures()
+    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
+    levels = P.tgv_amr_hierarchy(n_base=n_base, nlev=2, nest=3, base_grid=32)  # 8 base boxes: the
+    threads = len(os.sched_getaffinity(0))                                        # oracle threads over boxes
+    H = O.OracleHierarchy(levels)
+    H.set_state([P.tgv_state(cls, g, tuple(g.dom_hi[d] + 1 for d in range(3))) for g, _ in levels])
+    dt0 = DT * 256 / n_base
+    t0 = time.perf_counter()
+    H.advance(cls, sch, 0.0, dt0, nthreads=threads)
+    el = time.perf_counter() - t0
+    cells = [sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) for lo, hi in b) for _, b in levels]
+    cs = sum(c * 3 * 2 ** l for l, c in enumerate(cells))
+    return {"value": cs / el, "unit": "cell-stage/s", "cores": threads, "kind": "port",
+            "sample": f"TGV hierarchy base {n_base}^3 (boxes of 32^3) + 2 levels (nested 3), cells {cells}, one "
+                      f"coarse step "
+                      f"(1/2/4 level steps), {threads} threads, {el:.1f} s"}
+
+
 def hierarchy_timing(a, layout, n_base=128, steps=3, reps=2, fill_reps=20):
     """The TGV AMR hierarchy with levels nested 3 coarse cells inside each
     other (base n_base^3, 2 levels of ratio 2, |omega| > 70th percentile,
@@ -575,7 +602,10 @@ def main():
         return 0
     if a.workload == "amr-hierarchy":
         for layout in ("level", "boxes"):
-            print(json.dumps(hierarchy_timing(a, layout)), flush=True)
+            r = hierarchy_timing(a, layout)
+            if layout == "level" and not a.no_cpu:
+                r["cpu_baseline"] = cpu_hierarchy_baseline()
+            print(json.dumps(r), flush=True)
             torch_mod = sys.modules.get("torch")
             if torch_mod is not None:
                 torch_mod.cuda.empty_cache()

@@ -980,12 +980,39 @@ int bc_type(int code, int comp, int dir) {
 //    fastest, then j, then k.
 // Slopes next to ext_dir boundaries (one-sided in AMReX) are not restated:
 // such stencils are rejected.
+// Owner box of every cell of a level's domain (-1: no box), for lookups in
+// constant time.
+struct OwnerMap {
+  int n[3] = {0, 0, 0}, lo[3] = {0, 0, 0};
+  std::vector<int> own;
+  void build(const Level& L) {
+    const orc_geom& g = *L.g;
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = g.dom_lo[d];
+      n[d] = g.dom_hi[d] - g.dom_lo[d] + 1;
+    }
+    own.assign(size_t(n[0]) * n[1] * n[2], -1);
+    for (int b = 0; b < L.nbox; ++b) {
+      const Box& x = L.boxes[b];
+      for (int k = x.lo[2]; k <= x.hi[2]; ++k)
+        for (int j = x.lo[1]; j <= x.hi[1]; ++j)
+          for (int i = x.lo[0]; i <= x.hi[0]; ++i)
+            own[(size_t(k - lo[2]) * n[1] + (j - lo[1])) * n[0] + (i - lo[0])] = b;
+    }
+  }
+  // c inside the domain
+  int at(const int c[3]) const {
+    return own[(size_t(c[2] - lo[2]) * n[1] + (c[1] - lo[1])) * n[0] + (c[0] - lo[0])];
+  }
+};
+
 struct CoarseFine {
   const Level* C = nullptr;        // coarse level
   const double* const* u0 = nullptr;  // coarse FABs at t0, ghost cells filled
   const double* const* u1 = nullptr;  // coarse FABs at t1, ghost cells filled
   double a = 1.0, b = 0.0;          // weights of u0 and u1 at the fill time
   int ratio = 2;
+  const OwnerMap* cmap = nullptr;   // owner boxes of the coarse level (set per fill)
 };
 
 void cf_weights(double t0, double t1, double t, double* a, double* b) {
@@ -1011,7 +1038,7 @@ bool in_box(const Box& s, const int c[3]) {
 
 // Box of `L` whose valid region holds cell c (periodic image taken), or -1;
 // -2 if c lies outside the domain in a non-periodic direction.
-int owner_box(const Level& L, const int c0[3]) {
+int owner_box(const Level& L, const OwnerMap& m, const int c0[3]) {
   const orc_geom& g = *L.g;
   int c[3] = {c0[0], c0[1], c0[2]};
   for (int d = 0; d < 3; ++d) {
@@ -1021,9 +1048,7 @@ int owner_box(const Level& L, const int c0[3]) {
       c[d] = g.dom_lo[d] + ((c[d] - g.dom_lo[d]) % len + len) % len;
     }
   }
-  for (int s = 0; s < L.nbox; ++s)
-    if (in_box(L.boxes[s], c)) return s;
-  return -1;
+  return m.at(c);
 }
 
 int check_two_levels(const orc_geom& gf, const orc_geom& gc, int r) {
@@ -1045,7 +1070,7 @@ int interp_cell(const CoarseFine& cf, const int c[3], double out[NCONS]) {
   const int r = cf.ratio;
   int ic[3];
   for (int d = 0; d < 3; ++d) ic[d] = floor_div(c[d], r);
-  const int cb = owner_box(C, ic);
+  const int cb = owner_box(C, *cf.cmap, ic);
   if (cb < 0) return fail("coarse-fine interpolation: fine level not properly nested (no coarse box holds a coarse cell)");
   // periodic image of ic inside box cb: the stencil is read from that FAB
   int icb[3];
@@ -1058,7 +1083,7 @@ int interp_cell(const CoarseFine& cf, const int c[3], double out[NCONS]) {
     for (int e = -1; e <= 1; e += 2) {
       int p[3] = {icb[0], icb[1], icb[2]};
       p[d] += e;
-      if (owner_box(C, p) == -1)
+      if (owner_box(C, *cf.cmap, p) == -1)
         return fail("coarse-fine interpolation: the coarse stencil leaves the coarse level (fine level not properly nested)");
     }
     if (!gc.periodic[d]) {
@@ -1103,12 +1128,19 @@ int interp_cell(const CoarseFine& cf, const int c[3], double out[NCONS]) {
 }
 
 // Coarse-fine ghost cells of every fine box (after the same-level copies).
-int interp_coarse_fine(const Level& L, std::vector<A4>& v, const CoarseFine& cf) {
+int interp_coarse_fine(const Level& L, std::vector<A4>& v, const CoarseFine& cf0) {
   const orc_geom& g = *L.g;
+  OwnerMap fmap, cmap;
+  fmap.build(L);
+  cmap.build(*cf0.C);
+  CoarseFine cf = cf0;
+  cf.cmap = &cmap;
   int len[3];
   for (int d = 0; d < 3; ++d) len[d] = g.dom_hi[d] - g.dom_lo[d] + 1;
   int err = 0;
-  for (int b = 0; b < L.nbox && !err; ++b) {
+  std::string msg;
+#pragma omp parallel for schedule(dynamic) reduction(| : err)
+  for (int b = 0; b < L.nbox; ++b) {
     const Box& bx = L.boxes[b];
     Box gb = grow(bx, L.ng);
     for_box(gb, [&](int i, int j, int k) {
@@ -1121,17 +1153,18 @@ int interp_coarse_fine(const Level& L, std::vector<A4>& v, const CoarseFine& cf)
           if (!g.periodic[d]) return;  // physical boundary
           c[d] += (c[d] < g.dom_lo[d]) ? len[d] : -len[d];
         }
-      for (int s = 0; s < L.nbox; ++s)
-        if (in_box(L.boxes[s], c)) return;  // covered: copied
+      if (fmap.at(c) >= 0) return;  // covered: copied
       double val[NCONS];
       if (interp_cell(cf, c, val)) {
         err = 1;
+#pragma omp critical(cf_error)
+        if (msg.empty()) msg = g_err;  // the error text is per thread
         return;
       }
       for (int n = 0; n < NCONS; ++n) v[b](i, j, k, n) = val[n];
     });
   }
-  return err ? -1 : 0;
+  return err ? fail(msg) : 0;
 }
 
 // Single-level FillPatch: valid data of every box is taken as given; ghost
@@ -1149,6 +1182,10 @@ int fill_patch(const Level& L, R* const* fabs, const CoarseFine* cf = nullptr) {
   }
   int len[3];
   for (int d = 0; d < 3; ++d) len[d] = g.dom_hi[d] - g.dom_lo[d] + 1;
+  OwnerMap owners;
+  owners.build(L);
+  // boxes in parallel: each writes only its own ghost cells, reading valid cells
+#pragma omp parallel for schedule(dynamic)
   for (int b = 0; b < L.nbox; ++b) {
     const Box& bx = L.boxes[b];
     Box gb = grow(bx, ng);
@@ -1171,11 +1208,7 @@ int fill_patch(const Level& L, R* const* fabs, const CoarseFine* cf = nullptr) {
       int src = -1;
       if (inside(L.boxes[hint])) src = hint;
       else
-        for (int s = 0; s < L.nbox; ++s)
-          if (inside(L.boxes[s])) {
-            src = s;
-            break;
-          }
+        src = owners.at(c);
       if (src < 0) return;  // uncovered (coarse-fine): left untouched
       hint = src;
       for (int n = 0; n < NCONS; ++n) v[b](i, j, k, n) = v[src](c[0], c[1], c[2], n);
