@@ -113,6 +113,38 @@ def test_fill_two_levels(ctx, kind, weights, layout):
         assert np.array_equal(Sf.download_box(b), of.fabs[b]), f"box {b}"
 
 
+@pytest.mark.parametrize("layout", ["level", "boxes"])
+def test_fill_two_levels_ratio4(ctx, layout):
+    """Ratio 4 (offsets -3/8 .. 3/8), periodic in x only, blended in time."""
+    from amrsolver_amd import level as L
+    from oracle import oracle as O
+    print(f"case fill2 ratio4 {layout}", flush=True)
+    gc = P.geom((12, 8, 8), (0, 0, 0), (1.2, 0.8, 0.8), (1, 0, 0), lo_bc=(0, 2, 3), hi_bc=(0, 2, 3))
+    gf = P.geom((48, 32, 32), (0, 0, 0), (1.2, 0.8, 0.8), (1, 0, 0), lo_bc=(0, 2, 3), hi_bc=(0, 2, 3))
+    cboxes = P.decompose((12, 8, 8), (6, 4, 8))
+    fboxes = [((0, 4, 8), (15, 23, 27)), ((36, 0, 0), (47, 15, 15))]
+    cls, sch = P.tgv_closures(), P.scheme()
+    C = L.Level(gc, cboxes, cls, sch, ctx=ctx)
+    F = L.Level(gf, fboxes, cls, sch, ctx=ctx)
+    u0, u1, uf = _rand(gc, 21), _rand(gc, 22), _rand(gf, 23)
+    S0, S1, Sf = C.new_state(layout=layout), C.new_state(layout=layout), F.new_state(layout=layout)
+    S0.upload(u0)
+    S1.upload(u1)
+    C.fill_patch(S0)
+    C.fill_patch(S1)
+    Sf.upload(uf)
+    F.fill_two_levels(Sf, C, S0, S1, 0.625, 0.375, 4)
+    o0, o1, of = O.OracleLevel(gc, cboxes), O.OracleLevel(gc, cboxes), O.OracleLevel(gf, fboxes)
+    o0.set_state(u0)
+    o1.set_state(u1)
+    o0.fill_patch()
+    o1.fill_patch()
+    of.set_state(uf)
+    of.fill_two_levels(o0, o0.fabs, o1.fabs, 0.625, 0.375, 4)
+    for b in range(len(fboxes)):
+        assert np.array_equal(Sf.download_box(b), of.fabs[b]), f"box {b}"
+
+
 def test_fill_two_levels_rejects_bad_nesting(ctx):
     from amrsolver_amd import level as L
     (gc, cboxes), (gf, fboxes) = _pair("periodic")
