@@ -129,6 +129,9 @@ struct DescSet {
   int64_t last_use = 0;
   int wrap = -1;   // one array over the whole periodic domain (-1 unknown)
   int level = -1;  // one array holding the boxes' bounding box (-1 unknown)
+  // fill tasks of the two inner ghost layers only (the fused stage kernel
+  // reads no further); their Fab table is this set's `d`
+  std::unique_ptr<DescSet> narrow;
   ~DescSet() {
     if (d) (void)hipFree(d);
     if (d_copy) (void)hipFree(d_copy);
@@ -505,13 +508,35 @@ int build_fill(amr_boxtab_s* bt, DescSet* ds) {
 
 // bc_only: physical boundaries only (the periodic copies of a batch, whose
 // problems are single boxes, when the stage reads periodic images itself).
-int enqueue_fill(amr_boxtab_s* bt, DescSet* ds, bool bc_only = false, hipStream_t stream = nullptr) {
+// Fill tasks limited to the two inner ghost layers: built from a copy of
+// the descriptors with ng = 2 (same element mapping), launched on the real
+// descriptors.
+int build_fill_narrow(amr_boxtab_s* bt, DescSet* ds) {
+  if (ds->narrow) return AMR_OK;
+  auto t = std::make_unique<DescSet>();
+  t->host = ds->host;
+  for (amr_fab_t& f : t->host) {
+    const int k = f.ng - std::min(f.ng, 2);
+    f.data += k * (1 + f.jstride + f.kstride);
+    f.ng -= k;
+  }
+  int rc = build_fill(bt, t.get());
+  if (rc) return rc;
+  ds->narrow = std::move(t);
+  return AMR_OK;
+}
+
+// narrow: fill only the two inner ghost layers (build_fill_narrow first)
+int enqueue_fill(amr_boxtab_s* bt, DescSet* ds, bool bc_only = false, hipStream_t stream = nullptr,
+                 bool narrow = false) {
   hipStream_t st = stream ? stream : bt->ctx->stream;
   if (bt->walls && !bt->has_problem)
     return fail(AMR_ERR_ARG, "bc code 6 walls need amr_set_problem (wall temperature) first");
-  if (!bc_only) HIPCHK(launch_fill_copy(ds->d_copy, ds->d_copy_blk, ds->ncopy_blk, ds->d, st));
+  const DescSet* tk = narrow ? ds->narrow.get() : ds;
+  if (!tk) return fail(AMR_ERR_ARG, "internal: narrow fill tasks not built");
+  if (!bc_only) HIPCHK(launch_fill_copy(tk->d_copy, tk->d_copy_blk, tk->ncopy_blk, ds->d, st));
   for (int d = 0; d < 3; ++d)
-    HIPCHK(launch_fill_bc(ds->d_bc[d], ds->d_bc_blk[d], ds->nbc_blk[d], ds->d, bt->bt.d_boxes, bt->wall_fill, st));
+    HIPCHK(launch_fill_bc(tk->d_bc[d], tk->d_bc_blk[d], tk->nbc_blk[d], ds->d, bt->bt.d_boxes, bt->wall_fill, st));
   return AMR_OK;
 }
 
@@ -1334,6 +1359,14 @@ int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t
   if ((rc = build_fill(bt, dst))) return rc;
   for (int w = 0; w < 2; ++w)
     if ((rc = build_fill(bt, dw[w]))) return rc;
+  // the fused stage kernel (KEEP 2/4, halo 2) reads two ghost layers: its
+  // stage inputs are filled that far only
+  const bool narrow = fused_path(bt, sch, nullptr) && sch.order_keep <= 4;
+  if (narrow) {
+    if ((rc = build_fill_narrow(bt, dst))) return rc;
+    for (int w = 0; w < 2; ++w)
+      if ((rc = build_fill_narrow(bt, dw[w]))) return rc;
+  }
   // stage inputs keep the caller's values in ghost cells no box covers:
   // read from U^n by the fused stages, else copied into the workspaces
   bool direct = false;
@@ -1348,7 +1381,8 @@ int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t
     // themselves
     const bool fused = fused_path(bt, sch, nullptr);
     if (!(fused && wrap_ok(bt, sin)) && !(scratch_out && s == 0))
-      if ((rc = enqueue_fill(bt, sin, bt->batch && !fused && dir_flux_scheme(sch)))) return rc;
+      if ((rc = enqueue_fill(bt, sin, bt->batch && !fused && dir_flux_scheme(sch), nullptr, narrow && fused)))
+        return rc;
     StageArgs sa{sc, d_box_dt, sch.order_rk, sch.stages_rk, s};
     if ((rc = do_stage(bt, cls, sch, sa, dst, sin, out, direct && s > 0 ? dst->d : nullptr))) return rc;
   }
