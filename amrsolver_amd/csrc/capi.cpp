@@ -42,6 +42,26 @@ int64_t cells(const int lo[3], const int hi[3]) {
   return int64_t(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
 }
 
+// Packed block map of a task list: workgroup w takes cells [w*TASK_BLK,
+// (w+1)*TASK_BLK) of the concatenated list (tasks numbered by `start`), so
+// small tasks share workgroups; `task` is the task holding the first cell.
+template <class T>
+int upload_packed_blocks(const std::vector<T>& tasks, int64_t ncell, TaskBlock** d, int* nblk) {
+  if (ncell > int64_t(INT32_MAX) - TASK_BLK) return fail(AMR_ERR_ARG, "ghost-fill task list exceeds 2^31 cells");
+  std::vector<TaskBlock> h;
+  int t = 0;
+  for (int64_t g = 0; g < ncell; g += TASK_BLK) {
+    while (t + 1 < int(tasks.size()) && tasks[t + 1].start <= g) ++t;
+    h.push_back(TaskBlock{t, int(g)});
+  }
+  *nblk = int(h.size());
+  if (!h.empty()) {
+    HIPCHK(hipMalloc(d, sizeof(TaskBlock) * h.size()));
+    HIPCHK(hipMemcpy(*d, h.data(), sizeof(TaskBlock) * h.size(), hipMemcpyHostToDevice));
+  }
+  return AMR_OK;
+}
+
 // Block map of a task list: TASK_BLK cells of one task per workgroup.
 template <class T>
 int upload_blocks(const std::vector<T>& tasks, TaskBlock** d, int* nblk) {
@@ -438,7 +458,7 @@ int build_fill(amr_boxtab_s* bt, DescSet* ds) {
     HIPCHK(hipMalloc(&ds->d_copy, sizeof(CopyTask) * copies.size()));
     HIPCHK(hipMemcpy(ds->d_copy, copies.data(), sizeof(CopyTask) * copies.size(), hipMemcpyHostToDevice));
   }
-  int rc = upload_blocks(copies, &ds->d_copy_blk, &ds->ncopy_blk);
+  int rc = upload_packed_blocks(copies, ncell, &ds->d_copy_blk, &ds->ncopy_blk);
   if (rc) return rc;
   for (int d = 0; d < 3; ++d) {
     std::vector<BcTask> tasks;
@@ -534,7 +554,7 @@ int enqueue_fill(amr_boxtab_s* bt, DescSet* ds, bool bc_only = false, hipStream_
     return fail(AMR_ERR_ARG, "bc code 6 walls need amr_set_problem (wall temperature) first");
   const DescSet* tk = narrow ? ds->narrow.get() : ds;
   if (!tk) return fail(AMR_ERR_ARG, "internal: narrow fill tasks not built");
-  if (!bc_only) HIPCHK(launch_fill_copy(tk->d_copy, tk->d_copy_blk, tk->ncopy_blk, ds->d, st));
+  if (!bc_only) HIPCHK(launch_fill_copy(tk->d_copy, tk->ncopy, tk->d_copy_blk, tk->ncopy_blk, ds->d, st));
   for (int d = 0; d < 3; ++d)
     HIPCHK(launch_fill_bc(tk->d_bc[d], tk->d_bc_blk[d], tk->nbc_blk[d], ds->d, bt->bt.d_boxes, bt->wall_fill, st));
   return AMR_OK;
@@ -2067,7 +2087,7 @@ int amr_fill_two_levels(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, double 
   hipStream_t st = fine->ctx->stream;
   if (fine->walls && !fine->has_problem)
     return fail(AMR_ERR_ARG, "bc code 6 walls need amr_set_problem (wall temperature) first");
-  HIPCHK(launch_fill_copy(dsf->d_copy, dsf->d_copy_blk, dsf->ncopy_blk, dsf->d, st));
+  HIPCHK(launch_fill_copy(dsf->d_copy, dsf->ncopy, dsf->d_copy_blk, dsf->ncopy_blk, dsf->d, st));
   const bool blend = use_old && use_new;
   const Fab* c0 = use_old ? d0->d : d1->d;
   HIPCHK(launch_interp_cf(t->d_interp, t->d_interp_blk, t->ninterp_blk, c0, blend ? d1->d : c0, blend, a_old,

@@ -12,12 +12,21 @@ constexpr int FBLK = TASK_BLK;
 
 // FillBoundary + periodic images: copy valid cells of `src` into ghost
 // cells of `dst` (AMReX FillBoundary semantics restated).
-__global__ void __launch_bounds__(FBLK) k_fill_copy(const CopyTask* tasks, const TaskBlock* blocks,
+// One lane per ghost cell of the concatenated task list (packed block map:
+// a workgroup's 256 cells may span several small tasks; each lane steps
+// forward from the task of the workgroup's first cell).
+__global__ void __launch_bounds__(FBLK) k_fill_copy(const CopyTask* tasks, int ntask, const TaskBlock* blocks,
                                                     const Fab* fabs) {
   const TaskBlock tb = blocks[blockIdx.x];
-  const CopyTask& T = tasks[tb.task];
-  int c[3];
-  if (!task_cell(T, tb, threadIdx.x, c)) return;
+  const int64_t g = int64_t(tb.first) + threadIdx.x;
+  int t = tb.task;
+  while (t + 1 < ntask && tasks[t + 1].start <= g) ++t;
+  const CopyTask& T = tasks[t];
+  const int nx = T.hi[0] - T.lo[0] + 1, ny = T.hi[1] - T.lo[1] + 1, nz = T.hi[2] - T.lo[2] + 1;
+  const int64_t r = g - T.start;
+  if (r >= int64_t(nx) * ny * nz) return;  // past the last task
+  const int r2 = int(r / nx);
+  int c[3] = {T.lo[0] + int(r - int64_t(r2) * nx), T.lo[1] + r2 % ny, T.lo[2] + r2 / ny};
   const Fab d = fabs[T.dst], s = fabs[T.src];
 #pragma unroll
   for (int n = 0; n < NCONS; ++n) d(c[0], c[1], c[2], n) = s(c[0] - T.sh[0], c[1] - T.sh[1], c[2] - T.sh[2], n);
@@ -336,10 +345,10 @@ __global__ void __launch_bounds__(RBLK) k_sum_final(const double* partial, int n
 
 }  // namespace
 
-hipError_t launch_fill_copy(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* fabs,
+hipError_t launch_fill_copy(const CopyTask* tasks, int ntask, const TaskBlock* blocks, int nblk, const Fab* fabs,
                             hipStream_t st) {
   if (nblk == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_copy, dim3(unsigned(nblk)), dim3(FBLK), 0, st, tasks, blocks, fabs);
+  hipLaunchKernelGGL(k_fill_copy, dim3(unsigned(nblk)), dim3(FBLK), 0, st, tasks, ntask, blocks, fabs);
   return hipGetLastError();
 }
 

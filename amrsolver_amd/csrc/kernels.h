@@ -210,7 +210,8 @@ hipError_t launch_objective(const BoxTable& bt, int kind, const Fab* st, const F
                             double* partial, double* result, hipStream_t s);
 
 // Ghost fill over block-mapped task lists (nblk workgroups).
-hipError_t launch_fill_copy(const CopyTask* tasks, const TaskBlock* blocks, int nblk, const Fab* fabs,
+// packed block map (workgroups span consecutive cells of the task list)
+hipError_t launch_fill_copy(const CopyTask* tasks, int ntask, const TaskBlock* blocks, int nblk, const Fab* fabs,
                             hipStream_t st);
 hipError_t launch_fill_bc(const BcTask* tasks, const TaskBlock* blocks, int nblk, const Fab* fabs,
                           const BoxDev* boxes, const WallFill& wf, hipStream_t st);
