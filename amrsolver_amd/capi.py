@@ -64,8 +64,8 @@ SIGNATURES = {
                            ctypes.POINTER(_P)], ctypes.c_int),
     "amr_boxtab_destroy": ([_P], ctypes.c_int),
     "amr_fill_boundary": ([_P, _PF], ctypes.c_int),
-    "amr_fill_two_levels": ([_P, _P, ctypes.c_int, ctypes.c_double, _PF, ctypes.c_double, _PF, _PF],
-                            ctypes.c_int),
+    "amr_fill_two_levels": ([_P, _P, ctypes.c_int, ctypes.c_double, _PF, ctypes.c_double, _PF, ctypes.c_int,
+                             _PF], ctypes.c_int),
     "amr_average_down": ([_P, _P, ctypes.c_int, _PF, _PF], ctypes.c_int),
     "amr_plot_vars": ([_P, ctypes.POINTER(Closures), _PF, ctypes.c_int, ctypes.c_int, _PF], ctypes.c_int),
     "amr_write_plotfile": ([ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_P), ctypes.POINTER(_PF), ctypes.c_int,

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -149,9 +150,9 @@ struct DescSet {
   int64_t last_use = 0;
   int wrap = -1;   // one array over the whole periodic domain (-1 unknown)
   int level = -1;  // one array holding the boxes' bounding box (-1 unknown)
-  // fill tasks of the two inner ghost layers only (the fused stage kernel
-  // reads no further); their Fab table is this set's `d`
-  std::unique_ptr<DescSet> narrow;
+  // fill tasks of the inner 1 or 2 ghost layers only (the fused stage
+  // kernel reads two); their Fab table is this set's `d`
+  std::unique_ptr<DescSet> narrow[2];
   ~DescSet() {
     if (d) (void)hipFree(d);
     if (d_copy) (void)hipFree(d_copy);
@@ -219,6 +220,7 @@ struct AdjRun {
 struct CfTable {
   uint64_t crse_serial = 0;
   int ratio = 0;
+  int ngrow = 0;  // ghost width the interpolation tasks cover
   bool interp_built = false, avg_built = false;
   InterpTask* d_interp = nullptr;
   TaskBlock* d_interp_blk = nullptr;
@@ -301,9 +303,10 @@ struct amr_boxtab_s {
   // (srcmap.on = 0: unavailable; u0 is set per launch)
   UncovSrc srcmap{};
   unsigned* d_srcmap_bits = nullptr;
-  std::vector<CopyTask> h_uncov;  // host copy of the uncovered-ghost regions
   uint64_t serial = 0;            // distinct per box table of the process
-  std::unique_ptr<CfTable> cf;    // coarse-fine tables (this table as the fine level)
+  // coarse-fine tables (this table as the fine level), per coarse table
+  // serial, ratio and ghost width (0 for the average-down tasks)
+  std::map<std::array<uint64_t, 3>, std::unique_ptr<CfTable>> cf;
 };
 
 namespace {
@@ -531,28 +534,30 @@ int build_fill(amr_boxtab_s* bt, DescSet* ds) {
 // Fill tasks limited to the two inner ghost layers: built from a copy of
 // the descriptors with ng = 2 (same element mapping), launched on the real
 // descriptors.
-int build_fill_narrow(amr_boxtab_s* bt, DescSet* ds) {
-  if (ds->narrow) return AMR_OK;
+int build_fill_narrow(amr_boxtab_s* bt, DescSet* ds, int width = 2) {
+  if (width < 1 || width > 2) return fail(AMR_ERR_ARG, "internal: narrow fills are 1 or 2 layers wide");
+  if (ds->narrow[width - 1]) return AMR_OK;
   auto t = std::make_unique<DescSet>();
   t->host = ds->host;
   for (amr_fab_t& f : t->host) {
-    const int k = f.ng - std::min(f.ng, 2);
+    const int k = f.ng - std::min(f.ng, width);
     f.data += k * (1 + f.jstride + f.kstride);
     f.ng -= k;
   }
   int rc = build_fill(bt, t.get());
   if (rc) return rc;
-  ds->narrow = std::move(t);
+  ds->narrow[width - 1] = std::move(t);
   return AMR_OK;
 }
 
-// narrow: fill only the two inner ghost layers (build_fill_narrow first)
+// width 1 or 2: fill only that many inner ghost layers (build_fill_narrow
+// first); 0: every ghost layer of the descriptors
 int enqueue_fill(amr_boxtab_s* bt, DescSet* ds, bool bc_only = false, hipStream_t stream = nullptr,
-                 bool narrow = false) {
+                 int width = 0) {
   hipStream_t st = stream ? stream : bt->ctx->stream;
   if (bt->walls && !bt->has_problem)
     return fail(AMR_ERR_ARG, "bc code 6 walls need amr_set_problem (wall temperature) first");
-  const DescSet* tk = narrow ? ds->narrow.get() : ds;
+  const DescSet* tk = (width == 1 || width == 2) ? ds->narrow[width - 1].get() : ds;
   if (!tk) return fail(AMR_ERR_ARG, "internal: narrow fill tasks not built");
   if (!bc_only) HIPCHK(launch_fill_copy(tk->d_copy, tk->ncopy, tk->d_copy_blk, tk->ncopy_blk, ds->d, st));
   for (int d = 0; d < 3; ++d)
@@ -663,7 +668,24 @@ void ibox_subtract(const IBox& a, const IBox& b, std::vector<IBox>& out) {
 // Ghost cells (width NGHOST) of every box that no box of the same level or
 // problem covers, periodic images included, and that lie inside the domain
 // in every non-periodic direction (outside it the physical BCs fill them).
+std::vector<CopyTask> uncovered_pieces(amr_boxtab_s* bt, int width, int64_t* ncell_out);
+
 int build_uncovered(amr_boxtab_s* bt) {
+  int64_t ncell = 0;
+  std::vector<CopyTask> tasks = uncovered_pieces(bt, NGHOST, &ncell);
+  bt->nuncov = int(tasks.size());
+  bt->nuncov_cells = ncell;
+  if (!tasks.empty()) {
+    HIPCHK(hipMalloc(&bt->d_uncov, sizeof(CopyTask) * tasks.size()));
+    HIPCHK(hipMemcpy(bt->d_uncov, tasks.data(), sizeof(CopyTask) * tasks.size(), hipMemcpyHostToDevice));
+  }
+  return upload_blocks(tasks, &bt->d_uncov_blk, &bt->nuncov_blk);
+}
+
+// Ghost cells (width `width`) of every box that no box of the same level or
+// problem covers, periodic images included, inside the domain in every
+// non-periodic direction: regions of box b as tasks b -> b.
+std::vector<CopyTask> uncovered_pieces(amr_boxtab_s* bt, int width, int64_t* ncell_out) {
   const int nb = bt->bt.nbox;
   std::vector<CopyTask> tasks;
   int64_t ncell = 0;
@@ -674,8 +696,8 @@ int build_uncovered(amr_boxtab_s* bt) {
     for (int d = 0; d < 3; ++d) len[d] = g.dom_hi[d] - g.dom_lo[d] + 1;
     IBox gb;
     for (int d = 0; d < 3; ++d) {
-      gb.lo[d] = B.lo[d] - NGHOST;
-      gb.hi[d] = B.hi[d] + NGHOST;
+      gb.lo[d] = B.lo[d] - width;
+      gb.hi[d] = B.hi[d] + width;
       if (!g.periodic[d]) {
         gb.lo[d] = std::max(gb.lo[d], g.dom_lo[d]);
         gb.hi[d] = std::min(gb.hi[d], g.dom_hi[d]);
@@ -715,14 +737,8 @@ int build_uncovered(amr_boxtab_s* bt) {
       tasks.push_back(t);
     }
   }
-  bt->nuncov = int(tasks.size());
-  bt->nuncov_cells = ncell;
-  bt->h_uncov = tasks;
-  if (!tasks.empty()) {
-    HIPCHK(hipMalloc(&bt->d_uncov, sizeof(CopyTask) * tasks.size()));
-    HIPCHK(hipMemcpy(bt->d_uncov, tasks.data(), sizeof(CopyTask) * tasks.size(), hipMemcpyHostToDevice));
-  }
-  return upload_blocks(tasks, &bt->d_uncov_blk, &bt->nuncov_blk);
+  *ncell_out = ncell;
+  return tasks;
 }
 
 // Uncovered ghost cells of `from` into `to` and, if given, `to2` (no-op on
@@ -1401,7 +1417,7 @@ int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t
     // themselves
     const bool fused = fused_path(bt, sch, nullptr);
     if (!(fused && wrap_ok(bt, sin)) && !(scratch_out && s == 0))
-      if ((rc = enqueue_fill(bt, sin, bt->batch && !fused && dir_flux_scheme(sch), nullptr, narrow && fused)))
+      if ((rc = enqueue_fill(bt, sin, bt->batch && !fused && dir_flux_scheme(sch), nullptr, narrow && fused ? 2 : 0)))
         return rc;
     StageArgs sa{sc, d_box_dt, sch.order_rk, sch.stages_rk, s};
     if ((rc = do_stage(bt, cls, sch, sa, dst, sin, out, direct && s > 0 ? dst->d : nullptr))) return rc;
@@ -1887,13 +1903,20 @@ int check_two_levels(amr_boxtab_s* f, amr_boxtab_s* c, int r) {
   return AMR_OK;
 }
 
-CfTable* cf_table(amr_boxtab_s* f, amr_boxtab_s* c, int r) {
-  if (!f->cf || f->cf->crse_serial != c->serial || f->cf->ratio != r) {
-    f->cf = std::make_unique<CfTable>();
-    f->cf->crse_serial = c->serial;
-    f->cf->ratio = r;
+CfTable* cf_table(amr_boxtab_s* f, amr_boxtab_s* c, int r, int ngrow) {
+  const std::array<uint64_t, 3> key{c->serial, uint64_t(r), uint64_t(ngrow)};
+  auto& t = f->cf[key];
+  if (!t) {
+    if (f->cf.size() > 16) {  // tables of coarse levels long gone: start over
+      f->cf.clear();
+      return cf_table(f, c, r, ngrow);
+    }
+    t = std::make_unique<CfTable>();
+    t->crse_serial = c->serial;
+    t->ratio = r;
+    t->ngrow = ngrow;
   }
-  return f->cf.get();
+  return t.get();
 }
 
 // Fine ghost regions no fine box covers (inside the domain, periodic images
@@ -1928,8 +1951,9 @@ int build_cf_interp(amr_boxtab_s* F, amr_boxtab_s* C, int r, CfTable* t) {
         }
   }
   std::vector<InterpTask> tasks;
-  int64_t ncell = 0;
-  for (const CopyTask& p : F->h_uncov) {
+  int64_t ncell = 0, nun = 0;
+  const std::vector<CopyTask> pieces = uncovered_pieces(F, t->ngrow, &nun);
+  for (const CopyTask& p : pieces) {
     struct Part {
       IBox x;
       int sh[3];
@@ -2069,30 +2093,34 @@ int build_cf_avg(amr_boxtab_s* F, amr_boxtab_s* C, int r, CfTable* t) {
 }  // namespace
 
 int amr_fill_two_levels(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, double a_old, const amr_fab_t* crse_old,
-                        double b_new, const amr_fab_t* crse_new, const amr_fab_t* fine_state) {
+                        double b_new, const amr_fab_t* crse_new, int ngrow, const amr_fab_t* fine_state) {
   if (!fine || !crse || !fine_state) return fail(AMR_ERR_ARG, "null argument");
+  if (ngrow < 1 || ngrow > NGHOST) return fail(AMR_ERR_ARG, "ngrow must be 1, 2 or 3");
   if (!std::isfinite(a_old) || !std::isfinite(b_new)) return fail(AMR_ERR_ARG, "time weights must be finite");
   const bool use_old = !(a_old == 0.0 && b_new != 0.0), use_new = b_new != 0.0;
   if (use_old && !crse_old) return fail(AMR_ERR_ARG, "crse_old is required unless a_old == 0 and b_new != 0");
   if (use_new && !crse_new) return fail(AMR_ERR_ARG, "crse_new is required when b_new != 0");
   int rc;
   if ((rc = check_two_levels(fine, crse, ratio))) return rc;
-  CfTable* t = cf_table(fine, crse, ratio);
+  CfTable* t = cf_table(fine, crse, ratio, ngrow);
   if ((rc = build_cf_interp(fine, crse, ratio, t))) return rc;
   DescSet *dsf = nullptr, *d0 = nullptr, *d1 = nullptr;
   if ((rc = get_desc(fine, fine_state, NGHOST, &dsf))) return rc;
   if ((rc = build_fill(fine, dsf))) return rc;
+  const int width = ngrow < NGHOST ? ngrow : 0;
+  if (width && (rc = build_fill_narrow(fine, dsf, width))) return rc;
+  const DescSet* tk = width ? dsf->narrow[width - 1].get() : dsf;
   if (use_old && (rc = get_desc(crse, crse_old, 1, &d0))) return rc;
   if (use_new && (rc = get_desc(crse, crse_new, 1, &d1))) return rc;
   hipStream_t st = fine->ctx->stream;
   if (fine->walls && !fine->has_problem)
     return fail(AMR_ERR_ARG, "bc code 6 walls need amr_set_problem (wall temperature) first");
-  HIPCHK(launch_fill_copy(dsf->d_copy, dsf->ncopy, dsf->d_copy_blk, dsf->ncopy_blk, dsf->d, st));
+  HIPCHK(launch_fill_copy(tk->d_copy, tk->ncopy, tk->d_copy_blk, tk->ncopy_blk, dsf->d, st));
   const bool blend = use_old && use_new;
   const Fab* c0 = use_old ? d0->d : d1->d;
   HIPCHK(launch_interp_cf(t->d_interp, t->d_interp_blk, t->ninterp_blk, c0, blend ? d1->d : c0, blend, a_old,
                           b_new, ratio, dsf->d, st));
-  return enqueue_fill(fine, dsf, true);
+  return enqueue_fill(fine, dsf, true, nullptr, width);
 }
 
 int amr_average_down(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, const amr_fab_t* fine_state,
@@ -2100,7 +2128,7 @@ int amr_average_down(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, const amr_
   if (!fine || !crse || !fine_state || !crse_state) return fail(AMR_ERR_ARG, "null argument");
   int rc;
   if ((rc = check_two_levels(fine, crse, ratio))) return rc;
-  CfTable* t = cf_table(fine, crse, ratio);
+  CfTable* t = cf_table(fine, crse, ratio, 0);
   if ((rc = build_cf_avg(fine, crse, ratio, t))) return rc;
   DescSet *df = nullptr, *dc = nullptr;
   if ((rc = get_desc(fine, fine_state, 0, &df))) return rc;

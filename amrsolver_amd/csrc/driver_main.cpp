@@ -185,6 +185,7 @@ struct Hierarchy {
   std::vector<Level> lev;
   std::vector<State> cur, old, tmp;
   std::vector<double> t0, t1;
+  std::vector<int> ngrow;  // ghost layers each level's stages read (0: not yet known)
   amr_closures_t cls;
   amr_scheme_t sch;
 
@@ -193,10 +194,15 @@ struct Hierarchy {
       check(amr_fill_boundary(lev[0].bt, S.fabs.data()), "amr_fill_boundary");
       return;
     }
+    if (ngrow[l] == 0) {  // the fused stage kernel reads two ghost layers, the others three
+      int32_t info[5];
+      check(amr_step_plan(lev[l].bt, &sch, S.fabs.data(), info), "amr_step_plan");
+      ngrow[l] = info[3] != 0 ? 2 : 3;
+    }
     double a, b;
     cf_weights(t0[l - 1], t1[l - 1], t, &a, &b);
     check(amr_fill_two_levels(lev[l].bt, lev[l - 1].bt, 2, a, old[l - 1].fabs.data(), b, cur[l - 1].fabs.data(),
-                              S.fabs.data()),
+                              ngrow[l], S.fabs.data()),
           "amr_fill_two_levels");
   }
 
@@ -348,6 +354,7 @@ int main(int argc, char** argv) {
   }
   H.t0.assign(nlev, 0.0);
   H.t1.assign(nlev, 0.0);
+  H.ngrow.assign(nlev, 0);
 
   double time = 0.0;
   hip_check(hipStreamSynchronize(stream), "sync");

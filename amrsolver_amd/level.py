@@ -250,12 +250,13 @@ class Level:
         call("amr_advance", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, time, dt)
 
     # --- coarse-fine coupling (this level is the fine one) ---------------
-    def fill_two_levels(self, S, coarse, crse_old, crse_new, a, b, ratio=2):
-        """FillPatch of a fine level: same-level copies, coarse-fine ghost
-        cells interpolated from a*crse_old + b*crse_new (states of `coarse`
-        with filled ghost cells), physical BCs."""
+    def fill_two_levels(self, S, coarse, crse_old, crse_new, a, b, ratio=2, ngrow=3):
+        """FillPatch of a fine level (its `ngrow` inner ghost layers):
+        same-level copies, coarse-fine ghost cells interpolated from
+        a*crse_old + b*crse_new (states of `coarse` with filled ghost cells),
+        physical BCs."""
         call("amr_fill_two_levels", self.h, coarse.h, ratio, a, None if crse_old is None else crse_old.fabs, b,
-             None if crse_new is None else crse_new.fabs, S.fabs)
+             None if crse_new is None else crse_new.fabs, ngrow, S.fabs)
 
     def plot_vars(self, S, out, state_vars=True, derive=()):
         """Plot variables of S into out (valid cells): the state components
@@ -447,6 +448,7 @@ class Hierarchy:
         self.old = [lev.new_state(layout=layout) for lev in self.levels]
         self.tmp = [lev.new_state(layout=layout) for lev in self.levels]
         self.tint = [(0.0, 0.0)] * len(self.levels)
+        self.ngrow = [None] * len(self.levels)
 
     def upload(self, us):
         for S, u in zip(self.cur, us):
@@ -460,9 +462,13 @@ class Hierarchy:
         if lev == 0:
             L.fill_patch(S)
             return
+        if self.ngrow[lev] is None:
+            # the fused stage kernel reads two ghost layers, the others three
+            self.ngrow[lev] = 2 if L.step_plan(S)["tiles"] != "passes" else 3
         t0, t1 = self.tint[lev - 1]
         a, b = cf_weights(t0, t1, t)
-        L.fill_two_levels(S, self.levels[lev - 1], self.old[lev - 1], self.cur[lev - 1], a, b, self.ratio)
+        L.fill_two_levels(S, self.levels[lev - 1], self.old[lev - 1], self.cur[lev - 1], a, b, self.ratio,
+                          self.ngrow[lev])
 
     def _step(self, lev, time, dt):
         L = self.levels[lev]

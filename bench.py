@@ -551,7 +551,8 @@ def hierarchy_timing(a, layout, n_base=128, steps=3, reps=2, fill_reps=20):
         t0 = time.perf_counter()
         for _ in range(fill_reps):
             if kind == "fill":
-                Ls[top].fill_two_levels(H.tmp[top], Ls[below], H.old[below], H.cur[below], 0.5, 0.5, 2)
+                Ls[top].fill_two_levels(H.tmp[top], Ls[below], H.old[below], H.cur[below], 0.5, 0.5, 2,
+                                        H.ngrow[top])
             else:
                 Ls[top].average_down(H.cur[top], Ls[below], H.tmp[below], 2)
         ctx.synchronize()

@@ -127,7 +127,10 @@ int amr_fill_boundary(amr_boxtab_t bt, const amr_fab_t* fabs);
  * (crse_old alone when b_new == 0, crse_new alone when a_old == 0; AMReX's
  * weights are a = (t1 - t)/(t1 - t0), b = (t - t0)/(t1 - t0), or one state
  * alone within 1e-3 (t1 - t0) of its time), by conservative linear
- * interpolation; physical BCs.  `crse` is the coarse level's box table,
+ * interpolation; physical BCs.  Only the `ngrow` (1..3) ghost layers next to
+ * each box are filled, as FillPatch's nghost argument does (the fused stage
+ * kernel of amr_rk_stage reads 2; the other schemes need 3).  `crse` is the
+ * coarse level's box table,
  * `fine` this level's; fine_state needs ng >= 3, the coarse states ng >= 1
  * with their ghost cells filled (amr_fill_boundary or amr_fill_two_levels of
  * the coarse level).  ratio 2 or 4.  The coarse stencil (coarse cell and its
@@ -135,7 +138,7 @@ int amr_fill_boundary(amr_boxtab_t bt, const amr_fab_t* fabs);
  * across a foextrap/reflect boundary; otherwise, or next to ext_dir (bc code
  * 1 or 6) boundaries, AMR_ERR_ARG. */
 int amr_fill_two_levels(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, double a_old,
-                        const amr_fab_t* crse_old, double b_new, const amr_fab_t* crse_new,
+                        const amr_fab_t* crse_old, double b_new, const amr_fab_t* crse_new, int ngrow,
                         const amr_fab_t* fine_state);
 
 /* CNS::avgDown (src/CNS.cpp:513-530), i.e. amrex::average_down: every

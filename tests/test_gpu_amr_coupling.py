@@ -74,7 +74,7 @@ def test_average_down(ctx, ratio, layout):
     assert np.array_equal(Sc.download(), oc.get_state())
 
 
-def _fill_case(ctx, kind, weights, layout, seed=5):
+def _fill_case(ctx, kind, weights, layout, seed=5, ngrow=3):
     from amrsolver_amd import level as L
     from oracle import oracle as O
     (gc, cboxes), (gf, fboxes) = _pair(kind)
@@ -90,7 +90,7 @@ def _fill_case(ctx, kind, weights, layout, seed=5):
     C.fill_patch(S1)
     Sf.upload(uf)
     a, b = weights
-    F.fill_two_levels(Sf, C, S0, S1, a, b, 2)
+    F.fill_two_levels(Sf, C, S0, S1, a, b, 2, ngrow)
     o0, o1, of = O.OracleLevel(gc, cboxes), O.OracleLevel(gc, cboxes), O.OracleLevel(gf, fboxes)
     o0.set_state(u0)
     o1.set_state(u1)
@@ -111,6 +111,23 @@ def test_fill_two_levels(ctx, kind, weights, layout):
     # the one array, whose ghost cells inside other boxes are their data)
     for b in range(len(F.boxes)):
         assert np.array_equal(Sf.download_box(b), of.fabs[b]), f"box {b}"
+
+
+@pytest.mark.parametrize("ngrow", [1, 2])
+@pytest.mark.parametrize("kind", ["periodic", "bcs"])
+def test_fill_two_levels_inner_layers(ctx, kind, ngrow):
+    """ngrow < 3 (separate arrays per box): the inner layers equal the full
+    fill's, the outer ones are left as they were (zero)."""
+    print(f"case fill2 {kind} ngrow {ngrow}", flush=True)
+    F, Sf, of = _fill_case(ctx, kind, (0.25, 0.75), "boxes", ngrow=ngrow)
+    k = 3 - ngrow
+    for b in range(len(F.boxes)):
+        got, want = Sf.download_box(b), of.fabs[b]
+        inner = (slice(None), slice(k, got.shape[1] - k), slice(k, got.shape[2] - k), slice(k, got.shape[3] - k))
+        assert np.array_equal(got[inner], want[inner]), f"box {b}"
+        outer = np.ones(got.shape[1:], dtype=bool)
+        outer[inner[1:]] = False
+        assert np.all(got[:, outer] == 0.0), f"box {b}"
 
 
 @pytest.mark.parametrize("layout", ["level", "boxes"])
@@ -157,6 +174,8 @@ def test_fill_two_levels_rejects_bad_nesting(ctx):
         F.fill_two_levels(Sf, C, Sc, None, 1.0, 0.0, 2)
     with pytest.raises(capi.AmrError, match="ratio"):
         F.fill_two_levels(Sf, C, Sc, None, 1.0, 0.0, 3)
+    with pytest.raises(capi.AmrError, match="ngrow"):
+        F.fill_two_levels(Sf, C, Sc, None, 1.0, 0.0, 2, 4)
 
 
 def _nested_levels():
