@@ -1933,23 +1933,27 @@ int build_cf_interp(amr_boxtab_s* F, amr_boxtab_s* C, int r, CfTable* t) {
     len[d] = g.dom_hi[d] - g.dom_lo[d] + 1;
     lenc[d] = gc.dom_hi[d] - gc.dom_lo[d] + 1;
   }
-  std::vector<IBox> cimg;  // coarse boxes and their periodic images
+  // coverage of the coarse domain by the coarse boxes, one byte per cell
+  std::vector<unsigned char> ccov(size_t(lenc[0]) * lenc[1] * lenc[2], 0);
   for (int cb = 0; cb < C->bt.nbox; ++cb) {
     const BoxDev& B = C->bt.h_boxes[cb];
-    for (int sx = -1; sx <= 1; ++sx)
-      for (int sy = -1; sy <= 1; ++sy)
-        for (int sz = -1; sz <= 1; ++sz) {
-          const int sv[3] = {sx, sy, sz};
-          bool ok = true;
-          IBox img;
-          for (int d = 0; d < 3; ++d) {
-            if (sv[d] != 0 && !gc.periodic[d]) ok = false;
-            img.lo[d] = B.lo[d] + sv[d] * lenc[d];
-            img.hi[d] = B.hi[d] + sv[d] * lenc[d];
-          }
-          if (ok) cimg.push_back(img);
-        }
+    for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+      for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+        for (int i = B.lo[0]; i <= B.hi[0]; ++i)
+          ccov[(size_t(k - gc.dom_lo[2]) * lenc[1] + (j - gc.dom_lo[1])) * lenc[0] + (i - gc.dom_lo[0])] = 1;
   }
+  // every cell of S (clipped to the domain in non-periodic directions,
+  // periodic images taken) is a coarse valid cell
+  auto covered = [&](const IBox& S) {
+    for (int k = S.lo[2]; k <= S.hi[2]; ++k)
+      for (int j = S.lo[1]; j <= S.hi[1]; ++j)
+        for (int i = S.lo[0]; i <= S.hi[0]; ++i) {
+          int c[3] = {i, j, k};
+          for (int d = 0; d < 3; ++d) c[d] = ((c[d] - gc.dom_lo[d]) % lenc[d] + lenc[d]) % lenc[d];
+          if (!ccov[(size_t(c[2]) * lenc[1] + c[1]) * lenc[0] + c[0]]) return false;
+        }
+    return true;
+  };
   std::vector<InterpTask> tasks;
   int64_t ncell = 0, nun = 0;
   const std::vector<CopyTask> pieces = uncovered_pieces(F, t->ngrow, &nun);
@@ -2010,14 +2014,7 @@ int build_cf_interp(amr_boxtab_s* F, amr_boxtab_s* C, int r, CfTable* t) {
             S.hi[d] = std::min(S.hi[d], gc.dom_hi[d]);
           }
         }
-        std::vector<IBox> rest{S}, nr;
-        for (const IBox& img : cimg) {
-          nr.clear();
-          for (const IBox& a : rest) ibox_subtract(a, img, nr);
-          rest.swap(nr);
-          if (rest.empty()) break;
-        }
-        if (!rest.empty())
+        if (!covered(S))
           return fail(AMR_ERR_ARG, "coarse-fine interpolation: the coarse stencil leaves the coarse level (fine "
                                    "level not properly nested)");
         InterpTask tk;
