@@ -304,6 +304,9 @@ struct amr_boxtab_s {
   UncovSrc srcmap{};
   unsigned* d_srcmap_bits = nullptr;
   uint64_t serial = 0;            // distinct per box table of the process
+  // boxes by 32-cell bins of the domain (level tables): neighbour queries
+  std::vector<std::vector<int>> bins;
+  int nbin[3] = {0, 0, 0};
   // coarse-fine tables (this table as the fine level), per coarse table
   // serial, ratio and ghost width (0 for the average-down tasks)
   std::map<std::array<uint64_t, 3>, std::unique_ptr<CfTable>> cf;
@@ -405,6 +408,61 @@ int bc_type(int code, int comp, int dir) {
   return (comp - UMX == dir) ? norm_vel_bc[code] : tang_vel_bc[code];
 }
 
+// Boxes of the table that may intersect grow(box b, w) or a periodic image
+// of it, in increasing order (every box of a batch's problem: b itself).
+constexpr int BIN = 32;
+std::vector<int> neighbour_boxes(amr_boxtab_s* bt, int b, int w) {
+  if (bt->batch) return {b};
+  const amr_geom_t& g = bt->geoms[0];
+  int len[3];
+  for (int d = 0; d < 3; ++d) len[d] = g.dom_hi[d] - g.dom_lo[d] + 1;
+  if (bt->bins.empty()) {
+    for (int d = 0; d < 3; ++d) bt->nbin[d] = (len[d] + BIN - 1) / BIN;
+    bt->bins.resize(size_t(bt->nbin[0]) * bt->nbin[1] * bt->nbin[2]);
+    for (int s = 0; s < bt->bt.nbox; ++s) {
+      const BoxDev& S = bt->bt.h_boxes[s];
+      int lo[3], hi[3];
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::max(0, (std::max(S.lo[d], g.dom_lo[d]) - g.dom_lo[d]) / BIN);
+        hi[d] = std::min(bt->nbin[d] - 1, (std::min(S.hi[d], g.dom_hi[d]) - g.dom_lo[d]) / BIN);
+      }
+      for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+          for (int i = lo[0]; i <= hi[0]; ++i)
+            bt->bins[(size_t(k) * bt->nbin[1] + j) * bt->nbin[0] + i].push_back(s);
+    }
+  }
+  // bins touched per direction by the grown box's cells (periodic images taken)
+  std::vector<int> bl[3];
+  const BoxDev& B = bt->bt.h_boxes[b];
+  for (int d = 0; d < 3; ++d) {
+    std::vector<char> on(bt->nbin[d], 0);
+    const int lo = B.lo[d] - w, hi = B.hi[d] + w;
+    if (g.periodic[d] && hi - lo + 1 >= len[d]) {
+      std::fill(on.begin(), on.end(), 1);
+    } else {
+      for (int c = lo; c <= hi; c += 1) {
+        int x = c - g.dom_lo[d];
+        if (g.periodic[d]) x = (x % len[d] + len[d]) % len[d];
+        else if (x < 0 || x >= len[d]) continue;
+        on[x / BIN] = 1;
+      }
+    }
+    for (int i = 0; i < bt->nbin[d]; ++i)
+      if (on[i]) bl[d].push_back(i);
+  }
+  std::vector<int> out;
+  for (int k : bl[2])
+    for (int j : bl[1])
+      for (int i : bl[0]) {
+        const auto& v = bt->bins[(size_t(k) * bt->nbin[1] + j) * bt->nbin[0] + i];
+        out.insert(out.end(), v.begin(), v.end());
+      }
+  std::sort(out.begin(), out.end());
+  out.erase(std::unique(out.begin(), out.end()), out.end());
+  return out;
+}
+
 int build_fill(amr_boxtab_s* bt, DescSet* ds) {
   if (ds->fill_built) return AMR_OK;
   int nb = bt->bt.nbox;
@@ -422,8 +480,7 @@ int build_fill(amr_boxtab_s* bt, DescSet* ds) {
       G[0][d] = fb.lo[d] - fb.ng;
       G[1][d] = fb.hi[d] + fb.ng;
     }
-    for (int s = 0; s < nb; ++s) {
-      if (bt->batch && s != b) continue;  // independent problems share no data
+    for (int s : neighbour_boxes(bt, b, fb.ng)) {  // a batch's problems share no data
       const BoxDev& S = bt->bt.h_boxes[s];
       for (int sx = -1; sx <= 1; ++sx)
         for (int sy = -1; sy <= 1; ++sy)
@@ -704,8 +761,8 @@ std::vector<CopyTask> uncovered_pieces(amr_boxtab_s* bt, int width, int64_t* nce
       }
     }
     std::vector<IBox> pieces{gb}, next;
-    for (int s = 0; s < nb && !pieces.empty(); ++s) {
-      if (bt->batch && s != b) continue;
+    for (int s : neighbour_boxes(bt, b, width)) {
+      if (pieces.empty()) break;
       const BoxDev& S = bt->bt.h_boxes[s];
       for (int sx = -1; sx <= 1; ++sx)
         for (int sy = -1; sy <= 1; ++sy)
