@@ -418,3 +418,38 @@ def test_host_fill_times_follow_the_stage_sequence():
         got = LV.fill_times(o, s, t, dt)
         assert got == want, (o, s, got)
         assert capi.stage_coeffs(o, s, dt, 0)[0] == len(got)  # host-only entry point
+
+
+def test_hierarchy_generator_boxes_are_disjoint_and_coarsenable():
+    """Every generated level's boxes are disjoint, inside the domain and
+    coarsenable by the ratio (average-down needs it), and each fine box lies
+    over the level below."""
+    levels = P.tgv_amr_hierarchy(64, 2, nest=3)
+    for lev, (g, boxes) in enumerate(levels):
+        n = [g.dom_hi[d] + 1 for d in range(3)]
+        m = np.zeros(n, dtype=np.int32)
+        for lo, hi in boxes:
+            assert all(0 <= lo[d] <= hi[d] < n[d] for d in range(3))
+            assert all(lo[d] % 2 == 0 and (hi[d] + 1) % 2 == 0 for d in range(3))
+            m[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1] += 1
+        assert m.max() == 1
+        if lev:
+            gc, cb = levels[lev - 1]
+            c = np.zeros([gc.dom_hi[d] + 1 for d in range(3)], dtype=bool)
+            for lo, hi in cb:
+                c[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1] = True
+            for lo, hi in boxes:
+                assert c[lo[0] // 2:(hi[0] + 1) // 2, lo[1] // 2:(hi[1] + 1) // 2, lo[2] // 2:(hi[2] + 1) // 2].all()
+
+
+def test_driver_rejects_bad_arguments():
+    """The C++ driver checks its arguments before touching the GPU."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "amrsolver_amd", "lib", "cns_driver")
+    if not os.path.exists(exe):
+        pytest.skip("driver not built")
+    for args in (["--stages", "5"], ["--n", "4"], ["--bogus"], ["--steps"]):
+        r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2, (args, r.returncode, r.stderr)
