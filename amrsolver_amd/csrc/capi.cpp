@@ -336,6 +336,16 @@ int check_closures(const amr_closures_t& c) {
 }
 
 // Checks that fabs[nbox] describe the boxes of the table with >= min_ng ghosts.
+// Captured steps hold the device addresses of descriptor tables, workspaces
+// and source arrays: every path that frees or reallocates one of them drops
+// the graphs first (after the stream has drained), so no replay can touch
+// freed memory; the next amr_advance_graph call captures again.
+void drop_graphs(amr_boxtab_s* bt) {
+  if (bt->graphs.empty()) return;
+  (void)hipStreamSynchronize(bt->ctx->stream);
+  bt->graphs.clear();
+}
+
 int check_fabs(amr_boxtab_s* bt, const amr_fab_t* fabs, int min_ng) {
   int nb = bt->bt.nbox;
   for (int b = 0; b < nb; ++b) {
@@ -387,6 +397,8 @@ int get_desc(amr_boxtab_s* bt, const amr_fab_t* fabs, int min_ng, DescSet** out)
   if (bt->descs.size() >= 24) {
     auto it = std::min_element(bt->descs.begin(), bt->descs.end(),
                                [](const auto& a, const auto& b2) { return a->last_use < b2->last_use; });
+    drop_graphs(bt);  // a captured step may use the evicted table
+    HIPCHK(hipStreamSynchronize(bt->ctx->stream));
     bt->descs.erase(it);
   }
   std::unique_ptr<DescSet> ds;
@@ -818,6 +830,7 @@ int ensure_unfused(amr_boxtab_s* bt, const amr_scheme_t& s) {
   need |= bt->uw_kinds;
   if ((need & ~bt->uw_kinds) == 0) return AMR_OK;
   // (re)allocate everything needed in one block
+  drop_graphs(bt);
   HIPCHK(hipStreamSynchronize(bt->ctx->stream));
   if (bt->uw_mem) HIPCHK(hipFree(bt->uw_mem));
   for (void* p : bt->uw_tables) HIPCHK(hipFree(p));
@@ -1382,11 +1395,19 @@ int alloc_mirror(amr_boxtab_s* bt, const amr_fab_t* state, double** mem, std::ve
   return AMR_OK;
 }
 
-// Workspace state arrays (two) mirroring the layout of `state`.
+// Workspace state arrays (two) mirroring the layout of `state`.  The key is
+// the layout relative to the lowest address of the state's arrays, so states
+// of one layout (several level-wide arrays of a level, say) share the
+// workspaces; another layout reallocates them.
 int ensure_ws(amr_boxtab_s* bt, const amr_fab_t* state) {
   int nb = bt->bt.nbox;
-  if (int(bt->ws_key.size()) == nb && std::memcmp(bt->ws_key.data(), state, sizeof(amr_fab_t) * nb) == 0)
+  intptr_t lo = INTPTR_MAX;
+  for (int b = 0; b < nb; ++b) lo = std::min(lo, intptr_t(state[b].data));
+  std::vector<amr_fab_t> key(state, state + nb);
+  for (auto& f : key) f.data = (double*)(intptr_t(f.data) - lo);
+  if (int(bt->ws_key.size()) == nb && std::memcmp(bt->ws_key.data(), key.data(), sizeof(amr_fab_t) * nb) == 0)
     return AMR_OK;
+  drop_graphs(bt);  // captured steps write the workspaces being replaced
   HIPCHK(hipStreamSynchronize(bt->ctx->stream));
   for (int w = 0; w < 2; ++w) {
     if (bt->ws_mem[w]) HIPCHK(hipFree(bt->ws_mem[w]));
@@ -1397,7 +1418,7 @@ int ensure_ws(amr_boxtab_s* bt, const amr_fab_t* state) {
     int rc = alloc_mirror(bt, state, &bt->ws_mem[w], bt->ws_fabs[w]);
     if (rc) return rc;
   }
-  bt->ws_key.assign(state, state + nb);
+  bt->ws_key = std::move(key);
   return AMR_OK;
 }
 
@@ -2395,9 +2416,28 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
     return AMR_OK;
   }
   hipStream_t st = bt->ctx->stream;
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  // the events are released and the previous choice restored on every exit
+  // (a failed autotune leaves the tiling in force as it was)
+  struct Guard {
+    amr_boxtab_s* bt;
+    std::string key;
+    bool had;
+    FusedConfig prev;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool done = false;
+    ~Guard() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (done) return;
+      if (had) bt->tuned[key] = prev;
+      else bt->tuned.erase(key);
+    }
+  };
+  auto pit = bt->tuned.find(key);
+  Guard guard{bt, key, pit != bt->tuned.end(), pit != bt->tuned.end() ? pit->second : FusedConfig{}};
+  HIPCHK(hipEventCreate(&guard.e0));
+  HIPCHK(hipEventCreate(&guard.e1));
+  hipEvent_t e0 = guard.e0, e1 = guard.e1;
   float best = 1e30f;
   FusedConfig bestc = cand[0];
   // whole steps (fills and the uncovered-cell copy included) that write only
@@ -2440,9 +2480,8 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
       best = t[i];
       bestc = cand[i];
     }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   bt->tuned[key] = bestc;
+  guard.done = true;
   tiling[0] = bestc.tx;
   tiling[1] = bestc.ty;
   tiling[2] = bestc.tz;
@@ -2486,6 +2525,7 @@ int amr_set_source(amr_boxtab_t bt, const amr_fab_t* source) {
   if (!bt) return fail(AMR_ERR_ARG, "null box table");
   HIPCHK(hipStreamSynchronize(bt->ctx->stream));
   if (!source) {
+    drop_graphs(bt);  // captured steps read the table being freed
     bt->src.reset();
     return AMR_OK;
   }

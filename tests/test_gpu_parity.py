@@ -181,6 +181,35 @@ def test_graph_replay_matches_eager(ctx):
         assert np.array_equal(A.download(), B.download()), f"replay {rep} differs from eager"
 
 
+def test_graph_replay_after_workspace_realloc(ctx):
+    """A step on a state of another layout reallocates the box table's
+    workspaces, which a graph captured on the first state writes: the graph
+    must be dropped and captured again, so every replay still equals eager
+    steps (a stale replay would write freed memory)."""
+    L = _level_mod()
+    cls, g, boxes, u = periodic_setup(seed=12)
+    sch = P.scheme(flux_euler=1, order_keep=4, visc=True)
+    lev = L.Level(g, boxes, cls, sch, ctx=ctx)
+    A = lev.new_state()                 # level layout, graph-captured
+    E = lev.new_state()                 # level layout, eager twin of A
+    B = lev.new_state(layout="boxes")   # per-box arrays: other workspaces
+    for S in (A, E, B):
+        S.upload(u)
+    dt = 2e-6
+    for rep in range(3):
+        print(f"case graph after realloc rep {rep}", flush=True)
+        lev.advance_graph(A, dt, 2)
+        for _ in range(2):
+            lev.advance(B, 0.0, dt)
+        for _ in range(2):
+            lev.advance(E, 0.0, dt)
+        lev.synchronize()
+        a = A.download()
+        assert np.array_equal(a, E.download()), f"replay {rep} differs from eager"
+        err = rel_err(B.download(), a)
+        assert err <= TOL, f"box layout {err:.3e}"
+
+
 def test_tgv_small_conserves(ctx):
     """TGV 32^3 (KEEP4 + viscous, SSPRK3): oracle parity after 3 steps and
     conserved totals to round-off."""
