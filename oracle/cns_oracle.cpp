@@ -18,15 +18,21 @@
  *                       momentum consistency for uniform velocity
  *   WENO5 candidates    exact on quadratic data (any convex weights), ENO
  *                       limits (one smooth sub-stencil selects that candidate)
- *   WENO5 weights       parity unpinned for the exact nonlinear weights
- *                       (beta1 4th-power quirk, eps=1e-40): no reference
- *                       vectors exist and the reference cannot be built here
+ *   WENO5 weights       exact rational WENO5-JS (Jiang & Shu 1996): equal
+ *                       to the textbook scheme where (s1-s3)^2 is 0 or 1,
+ *                       to the quartic beta1 where s1-2s2+s3 = 0, and
+ *                       eps = 1e-40, q = 2, d = {0.1,0.6,0.3} on data whose
+ *                       smoothness indicators are of the size of eps
  *   Riemann             L=R -> exact Euler flux; mirror symmetry; weak waves
  *                       against an independent exact Riemann solver; Sod star
  *                       state (reference notebook values) within the
- *                       two-shock approximation error
- *   Riemann z quirk     presence demonstrated (x/y/z asymmetry); exact form
- *                       parity unpinned
+ *                       two-shock approximation error; symmetric collisions:
+ *                       flux (0,p*,0,0,0) exactly and p* within the error
+ *                       band of the third iterate against the exact
+ *                       two-shock p* (excludes 2 and 4 iterations)
+ *   Riemann z quirk     with given slopes the z face equals the x face of the
+ *                       same data once the left cell's T is T(k+1), bit for
+ *                       bit (riemann_face)
  *   viscous+Sutherland  linear fields -> exact stresses / heat flux;
  *                       Sutherland closed form
  *   SSPRK variants      linear ODE -> stability polynomials; conservation
@@ -1586,6 +1592,28 @@ void orc_riemann_prob(double gamma, double smallp, double rl, double ul, double 
                       double out[5]) {
   riemann_prob(gamma, smallp, rl, ul, pl, ut1l, ut2l, rr, ur, pr, ut1r, ut2r, out[0], out[1], out[2],
                out[3], out[4]);
+}
+
+// One face of cns_riemann_{x,y,z} (riemann_point) from given primitives of
+// the cells (face-1, face, face+1) along `dir` (q3[cell*6 + comp]) and given
+// slopes of the two cells next to the face (dq2[cell*5 + comp], cells
+// face-1 and face): the slopes are inputs here, not recomputed, so a test
+// can vary one primitive field of one cell alone.  f[5] in the state's
+// component order.
+void orc_riemann_face(const orc_closures* c, int dir, const double q3[18], const double dq2[10], double f[5]) {
+  double qb[18], db[15], fb[15];
+  int loq[3] = {0, 0, 0};
+  loq[dir] = -1;
+  int hiq[3] = {0, 0, 0};
+  hiq[dir] = 1;
+  A4 Q = view(qb, loq, hiq), D = view(db, loq, hiq), F = view(fb, loq, hiq);
+  for (int m = -1; m <= 1; ++m) {
+    int ii = dir == 0 ? m : 0, jj = dir == 1 ? m : 0, kk = dir == 2 ? m : 0;
+    for (int n = 0; n < 6; ++n) Q(ii, jj, kk, n) = q3[(m + 1) * 6 + n];
+    for (int n = 0; n < 5; ++n) D(ii, jj, kk, n) = m <= 0 ? dq2[(m + 1) * 5 + n] : 0.0;
+  }
+  riemann_point(0, 0, 0, dir, F, D, Q, *c);
+  for (int n = 0; n < 5; ++n) f[n] = F(0, 0, 0, n);
 }
 
 int orc_box_prims(const orc_closures* c, const int lo[3], const int hi[3], int ng, const double* state,

@@ -54,6 +54,7 @@ double orc_cond(const orc_closures* c, double T);
 void   orc_cons2prim_point(const orc_closures* c, const double u[5], double q[6]);
 double orc_weno5js(const double s[5]);
 double orc_limiter(double dlft, double drgt);
+void   orc_riemann_face(const orc_closures* c, int dir, const double q3[18], const double dq2[10], double f[5]);
 void   orc_riemann_prob(double gamma, double smallp,
                         double rl, double ul, double pl, double ut1l, double ut2l,
                         double rr, double ur, double pr, double ut1r, double ut2r,

@@ -93,6 +93,8 @@ def load(variant=None):
     lib.orc_limiter.restype = ctypes.c_double
     lib.orc_limiter.argtypes = [ctypes.c_double, ctypes.c_double]
     lib.orc_riemann_prob.argtypes = [ctypes.c_double] * 12 + [_DP]
+    lib.orc_riemann_face.restype = None
+    lib.orc_riemann_face.argtypes = [ctypes.POINTER(OClosures), ctypes.c_int, _DP, _DP, _DP]
     lib.orc_box_prims.argtypes = [ctypes.POINTER(OClosures), I3, I3, ctypes.c_int, _DP, _DP]
     lib.orc_box_numflux.argtypes = [ctypes.POINTER(OClosures), ctypes.POINTER(OScheme), I3, I3, ctypes.c_int,
                                     _DP, _DP, _DP, _DP, _DP]
@@ -176,6 +178,17 @@ def weno5js(s):
 
 def limiter(a, b):
     return load().orc_limiter(a, b)
+
+
+def riemann_face(c, d, q3, dq2):
+    """Face flux of cns_riemann_{x,y,z} (d = 0, 1, 2) from primitives q3
+    (3 cells along d: face-1, face, face+1; 6 components each) and the
+    slopes dq2 of the two cells next to the face (5 components each)."""
+    q3 = np.ascontiguousarray(q3, dtype=np.float64).reshape(18)
+    dq2 = np.ascontiguousarray(dq2, dtype=np.float64).reshape(10)
+    out = np.zeros(5)
+    load().orc_riemann_face(ctypes.byref(closures(c)), d, _ptr(q3), _ptr(dq2), _ptr(out))
+    return out
 
 
 def riemann_prob(gamma, rl, ul, pl, ut1l, ut2l, rr, ur, pr, ut1r, ut2r, smallp=1.0e-40):
