@@ -209,6 +209,49 @@ void for_box(const Box& b, F&& f) {
       for (int i = b.lo[0]; i <= b.hi[0]; ++i) f(i, j, k);
 }
 
+// Per-thread scratch arrays of the flux passes, kept between calls (fresh
+// allocations of this size come from mmap, and their page faults on first
+// touch are serial); `slot` distinguishes the arrays one call holds.
+template <class R>
+std::vector<R>& scratch(int slot) {
+  thread_local std::vector<R> v[12];
+  return v[slot];
+}
+
+// Threads for the point passes inside one box (set by a level whose boxes
+// are fewer than its threads, e.g. a single 1D problem).  Used only for
+// passes whose point function writes its own point alone, so the results do
+// not depend on it; the tangent and operation-counting scalar types always
+// run serially.
+thread_local int g_inner_threads = 1;
+
+// v = n zeros (threads as for_box_par)
+template <class R>
+void zero_fill(std::vector<R>& v, long n) {
+  if (!std::is_same<R, double>::value || g_inner_threads <= 1) {
+    v.assign(n, R(0.0));
+    return;
+  }
+  v.resize(n);
+  R* p = v.data();
+#pragma omp parallel for schedule(static) num_threads(g_inner_threads)
+  for (long i = 0; i < n; ++i) p[i] = R(0.0);
+}
+
+template <class R, class F>
+void for_box_par(const Box& b, F&& f) {
+  if (!std::is_same<R, double>::value || g_inner_threads <= 1) {
+    for_box(b, f);
+    return;
+  }
+  const int nj = b.hi[1] - b.lo[1] + 1, nk = b.hi[2] - b.lo[2] + 1;
+#pragma omp parallel for schedule(static) num_threads(g_inner_threads)
+  for (int r = 0; r < nj * nk; ++r) {
+    const int j = b.lo[1] + r % nj, k = b.lo[2] + r / nj;
+    for (int i = b.lo[0]; i <= b.hi[0]; ++i) f(i, j, k);
+  }
+}
+
 // ---------------------------------------------------------------- closures
 // src/cls/Closures.h:103-120 (viscosity), :130-166 (conductivity).  Law 2 is
 // the constant-Prandtl closure k = mu*cp/Pr that Closures.h:145-153 sketches;
@@ -793,10 +836,11 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
   if (!plain && s.rhs_euler && s.art_diss)
     return fail("the tangent (dual-number) path does not cover JST dissipation");
   Box bg = grow(bx, NGHOST);
-  std::vector<R> qbuf(npts(bg.lo, bg.hi) * NPRIM);
+  std::vector<R>& qbuf = scratch<R>(3);
+  zero_fill(qbuf, npts(bg.lo, bg.hi) * NPRIM);
   A4T<R> q = view(qbuf.data(), bg.lo, bg.hi);
   // CNS_advance.cpp:200-208 cons2prim on growntilebox(NGHOST)
-  for_box(bg, [&](int i, int j, int k) { cons2prim(i, j, k, u, q, c); });
+  for_box_par<R>(bg, [&](int i, int j, int k) { cons2prim(i, j, k, u, q, c); });
 
   // CentralKEEP.h:233-286 Flux_2nd_Order_KEEP (called at CNS_advance.cpp:226-229):
   // y faces lo..lo+h-1 next to a lower wall and hi-h+1..hi next to an upper
@@ -828,17 +872,19 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
       wall_order_reduction();
     } else if (s.flux_euler == 2) {
       // High_resolution.h:279-319
-      std::vector<double> pfb[3], lb(npts(bg.lo, bg.hi) * 3);
+      std::vector<double>& lb = scratch<double>(7);
+      zero_fill(lb, npts(bg.lo, bg.hi) * 3);
       A4 pf[3];
       for (int d = 0; d < 3; ++d) {
-        pfb[d].assign(npts(bg.lo, bg.hi) * NCONS, 0.0);
-        pf[d] = view(pfb[d].data(), bg.lo, bg.hi);
+        std::vector<double>& pfb = scratch<double>(4 + d);
+        zero_fill(pfb, npts(bg.lo, bg.hi) * NCONS);
+        pf[d] = view(pfb.data(), bg.lo, bg.hi);
       }
       A4 lam = view(lb.data(), bg.lo, bg.hi);
-      for_box(bg, [&](int i, int j, int k) { cons2eulerflux_lambda(i, j, k, u, pf[0], pf[1], pf[2], lam, c); });
+      for_box_par<R>(bg, [&](int i, int j, int k) { cons2eulerflux_lambda(i, j, k, u, pf[0], pf[1], pf[2], lam, c); });
       Box bn = nodal_all(bx);
       for (int n = 0; n < NCONS; ++n)
-        for_box(bn, [&](int i, int j, int k) {
+        for_box_par<R>(bn, [&](int i, int j, int k) {
           nf[0](i, j, k, n) = weno_face(i, j, k, n, 1, 0, 0, 0, u, pf[0], lam);
           nf[1](i, j, k, n) = weno_face(i, j, k, n, 0, 1, 0, 1, u, pf[1], lam);
           nf[2](i, j, k, n) = weno_face(i, j, k, n, 0, 0, 1, 2, u, pf[2], lam);
@@ -848,16 +894,17 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
       double co[3];
       keep_coeffs(s.order_keep, co);
       int h = s.order_keep / 2;
-      for_box(nodal_all(bx), [&](int i, int j, int k) { keep_point(i, j, k, h, co, q, nf[0], nf[1], nf[2], c); });
+      for_box_par<R>(nodal_all(bx), [&](int i, int j, int k) { keep_point(i, j, k, h, co, q, nf[0], nf[1], nf[2], c); });
       wall_order_reduction();
     } else {
       // Riemann.h:339-408
       Box bg1 = grow(bx, 1);
-      std::vector<double> sb(npts(bg1.lo, bg1.hi) * NCONS, 0.0);
+      std::vector<double>& sb = scratch<double>(8);
+      zero_fill(sb, npts(bg1.lo, bg1.hi) * NCONS);
       A4 sl = view(sb.data(), bg1.lo, bg1.hi);
       for (int d = 0; d < 3; ++d) {
-        for_box(grow_dir(bx, d, 1), [&](int i, int j, int k) { slope_point(i, j, k, d, sl, q, c); });
-        for_box(surrounding_nodes(bx, d), [&](int i, int j, int k) { riemann_point(i, j, k, d, nf[d], sl, q, c); });
+        for_box_par<R>(grow_dir(bx, d, 1), [&](int i, int j, int k) { slope_point(i, j, k, d, sl, q, c); });
+        for_box_par<R>(surrounding_nodes(bx, d), [&](int i, int j, int k) { riemann_point(i, j, k, d, nf[d], sl, q, c); });
       }
     }
   }
@@ -870,13 +917,13 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
     // CNS_advance.cpp:276-353 (no walls: Flux_2nd_Order_KEEP and
     // ViscousWallFluxes apply only to bc=6 y-walls)
     Box b1 = grow(bx, 1);
-    std::vector<R> pb[3];
     A4T<R> pv[3];
     for (int d = 0; d < 3; ++d) {
-      pb[d].assign(npts(b1.lo, b1.hi) * NCONS, R(0.0));
-      pv[d] = view(pb[d].data(), b1.lo, b1.hi);
+      std::vector<R>& pb = scratch<R>(9 + d);
+      zero_fill(pb, npts(b1.lo, b1.hi) * NCONS);
+      pv[d] = view(pb.data(), b1.lo, b1.hi);
     }
-    for_box(b1, [&](int i, int j, int k) { viscous_point(i, j, k, q, pv[0], pv[1], pv[2], dxinv, c); });
+    for_box_par<R>(b1, [&](int i, int j, int k) { viscous_point(i, j, k, q, pv[0], pv[1], pv[2], dxinv, c); });
     // CNS_advance.cpp:303-327: the ghost row outside a bc 6 wall takes
     // ViscousWallFluxes over growntilebox(1)'s x and z extent
     for (int side = 0; side < 2; ++side) {
@@ -887,7 +934,7 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
     }
     // CNS_hydro_K.h:254-261 on grownnodaltilebox(-1,0)
     for (int n = 0; n < NCONS; ++n)
-      for_box(nodal_all(bx), [&](int i, int j, int k) {
+      for_box_par<R>(nodal_all(bx), [&](int i, int j, int k) {
         nf[0](i, j, k, n) += 0.5 * (pv[0](i - 1, j, k, n) + pv[0](i, j, k, n));
         nf[1](i, j, k, n) += 0.5 * (pv[1](i, j - 1, k, n) + pv[1](i, j, k, n));
         nf[2](i, j, k, n) += 0.5 * (pv[2](i, j, k - 1, n) + pv[2](i, j, k, n));
@@ -900,7 +947,7 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
 template <class R>
 void box_assemble(const Box& bx, const A4T<R> nf[3], const double dxinv[3], const A4T<R>& dsdt) {
   for (int n = 0; n < NCONS; ++n)
-    for_box(bx, [&](int i, int j, int k) {
+    for_box_par<R>(bx, [&](int i, int j, int k) {
       dsdt(i, j, k, n) = dxinv[0] * (nf[0](i, j, k, n) - nf[0](i + 1, j, k, n)) +
                          dxinv[1] * (nf[1](i, j, k, n) - nf[1](i, j + 1, k, n)) +
                          dxinv[2] * (nf[2](i, j, k, n) - nf[2](i, j, k + 1, n));
@@ -913,11 +960,11 @@ template <class R>
 int box_rhs(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, const A4T<R>& u,
             const double dxinv[3], const A4T<R>& dsdt, const double* src, const Walls& walls = Walls()) {
   Box bn = grow(nodal_all(bx), NGHOST);
-  std::vector<R> fb[3];
   A4T<R> nf[3];
   for (int d = 0; d < 3; ++d) {
-    fb[d].assign(npts(bn.lo, bn.hi) * NCONS, R(0.0));
-    nf[d] = view(fb[d].data(), bn.lo, bn.hi);
+    std::vector<R>& fb = scratch<R>(d);
+    zero_fill(fb, npts(bn.lo, bn.hi) * NCONS);
+    nf[d] = view(fb.data(), bn.lo, bn.hi);
   }
   int rc = box_flux(c, s, bx, ng, u, dxinv, nf, walls);
   if (rc) return rc;
@@ -1422,8 +1469,7 @@ struct LevelOps {
     ++stage;
     if (rc) return rc;
     int err = 0;
-#pragma omp parallel for schedule(dynamic) num_threads(nthreads) reduction(| : err)
-    for (int b = 0; b < nb; ++b) {
+    auto one = [&](int b) {
       Box gb = grow(vbox[b], ng);
       A4T<R> u = view(state[b], gb.lo, gb.hi);
       A4T<R> d = view(vec[2][b].data(), vbox[b].lo, vbox[b].hi);
@@ -1431,7 +1477,16 @@ struct LevelOps {
       const orc_geom& g = *L->g;
       w.lo = !g.periodic[1] && g.lo_bc[1] == 6 && vbox[b].lo[1] == g.dom_lo[1];
       w.hi = !g.periodic[1] && g.hi_bc[1] == 6 && vbox[b].hi[1] == g.dom_hi[1];
-      if (box_rhs(*c, *s, vbox[b], ng, u, dxinv, d, srcs ? srcs[b] : nullptr, w)) err |= 1;
+      return box_rhs(*c, *s, vbox[b], ng, u, dxinv, d, srcs ? srcs[b] : nullptr, w) ? 1 : 0;
+    };
+    if (nb < nthreads) {
+      // fewer boxes than threads: boxes in turn, the threads inside each
+      g_inner_threads = nthreads;
+      for (int b = 0; b < nb; ++b) err |= one(b);
+      g_inner_threads = 1;
+    } else {
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads) reduction(| : err)
+      for (int b = 0; b < nb; ++b) err |= one(b);
     }
     return err ? -1 : 0;
   }

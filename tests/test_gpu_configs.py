@@ -1,0 +1,384 @@
+"""The benchmark configurations at their stated sizes, GPU against the oracle.
+
+* TGV 256^3 in 64 boxes of 64^3 (KEEP4 + viscous, SSPRK(3,3), dt = 5e-5;
+  exm/TGV/prob.h:92-134 initial state): two autotuned steps against the
+  oracle in the level and the box layout; 100 steps by graph replays equal
+  to 100 eager steps, with mass and energy conserved to 1e-14 relative.
+* Sod 256x4x4 (4 boxes of 64x4x4; exm/Sods/prob.h:98-121), WENO5-JS + LF and
+  MUSCL + Riemann, SSPRK(3,3), CFL 0.3 with the 1.1x growth cap, to t = 0.2:
+  every CFL time step equal to the oracle's, the final state within 1e-12,
+  the density within L1 0.006 of the exact solution whose star state the
+  reference notebook lists (half the 128-cell bound of the CPU pin: the
+  schemes converge at first order at the discontinuities).
+* Batched 1D Riemann problems: all 512 instances (seed 20261202, L in
+  [64, 2048]) to t = 0.15 in one batch; the shortest, the longest and 14
+  seeded instances with L <= 512 run solo on the oracle in host threads:
+  equal time-step sequences and final states within 1e-12.
+* Batched AMR: TGV base 128^3 with two levels (ratio 2, |omega| above its
+  70th percentile, blocking factor 8): one step of each refined level in one
+  launch per kernel against the oracle (level 2 in both layouts), and graph
+  replays of level 2 equal to eager steps.
+* Adjoint: channel 128x64x64, 20 SSPRK(3,3) steps, checkpoints every 5: the
+  gradient against fourth-order central differences of the objective on 32
+  seeded unit directions within 1e-6.
+
+Normalised per-component error as in conftest.rel_err; host threads for the
+oracle: min(16, cpu count).
+"""
+import concurrent.futures as cf
+import math
+import os
+
+import numpy as np
+import pytest
+
+from amrsolver_amd import problems as P
+from conftest import rel_err
+import exact_riemann as ER  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+NT = max(1, min(16, os.cpu_count() or 1))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from amrsolver_amd import level
+    return level.Context(0)
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+# ------------------------------------------------------------------ TGV 256^3
+@pytest.fixture(scope="module")
+def tgv_case():
+    from oracle import oracle as O
+    n = (256, 256, 256)
+    cls = P.tgv_closures()
+    g = P.geom(n, (0, 0, 0), (2 * math.pi,) * 3, (1, 1, 1))
+    boxes = P.decompose(n, 64)
+    u = P.tgv_state(cls, g, n)
+    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
+    say(f"case tgv256 oracle: 2 steps, {NT} threads")
+    ol = O.OracleLevel(g, boxes)
+    ol.set_state(u)
+    for _ in range(2):
+        ol.advance(cls, sch, 5e-5, nthreads=NT)
+    return cls, g, boxes, sch, u, ol.get_state()
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("layout", ["level", "boxes"])
+def test_tgv256_two_steps_match_oracle(ctx, tgv_case, layout):
+    from amrsolver_amd import level as L
+    cls, g, boxes, sch, u, ref = tgv_case
+    lev = L.Level(g, boxes, cls, sch, ctx=ctx)
+    S = lev.new_state(layout=layout)
+    S.upload(u)
+    lev.fill_patch(S)
+    tiling = lev.autotune(S)
+    say(f"case tgv256 {layout}: 64 boxes of 64^3, tiling {tiling}")
+    for _ in range(2):
+        lev.advance(S, 0.0, 5e-5)
+    err = rel_err(S.download(), ref)
+    del S
+    lev.close()
+    assert err <= TOL, f"tgv256 {layout}: {err:.3e}"
+
+
+@pytest.mark.timeout(600)
+def test_tgv256_hundred_steps_graph_and_totals(ctx, tgv_case):
+    import torch
+    from amrsolver_amd import level as L
+    cls, g, boxes, sch, u, _ = tgv_case
+    lev = L.Level(g, boxes, cls, sch, ctx=ctx)
+    A, B = lev.new_state(), lev.new_state()
+    A.upload(u)
+    B.upload(u)
+    lev.fill_patch(A)
+    lev.autotune(A)
+    t0 = np.array(lev.totals(A))
+    say("case tgv256 100 steps: 4 graph replays of 25 steps against 100 eager steps")
+    for _ in range(4):
+        lev.advance_graph(A, 5e-5, 25)
+    for _ in range(100):
+        lev.advance(B, 0.0, 5e-5)
+    lev.synchronize()
+    same = torch.equal(A.download_tensor(), B.download_tensor())
+    t1 = np.array(lev.totals(A))
+    del A, B
+    lev.close()
+    assert same, "graph replays differ from eager steps"
+    for c in (0, 4):
+        drift = abs(t1[c] - t0[c]) / abs(t0[c])
+        say(f"  component {c}: relative drift {drift:.2e}")
+        assert drift <= 1e-14
+
+
+# ------------------------------------------------------------------ Sod
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("flux", [2, 0], ids=["weno", "muscl"])
+def test_sod_256_to_t02(ctx, flux):
+    from amrsolver_amd import level as L
+    from oracle import oracle as O
+    n = (256, 4, 4)
+    cls = P.closures_reference()
+    g = P.geom(n, (0, 0, 0), (1.0, 4 / 256, 4 / 256), (0, 1, 1), lo_bc=(2, 0, 0), hi_bc=(2, 0, 0))
+    boxes = P.decompose(n, (64, 4, 4))
+    sch = P.scheme(flux_euler=flux, visc=False)
+    u0 = P.sod_state(cls, g, n)
+    lev = L.Level(g, boxes, cls, sch, ctx=ctx)
+    S = lev.new_state(layout="boxes")
+    S.upload(u0)
+    ol = O.OracleLevel(g, boxes)
+    ol.set_state(u0)
+    say(f"case sod256 flux {flux}")
+    t, dt, step = 0.0, None, 0
+    while t < 0.2 - 1e-14:
+        est = lev.est_time_step(S, 0.3)
+        est_o = ol.est_dt(cls, 0.3)
+        assert est == est_o, f"step {step}: CFL time step {est!r} != oracle {est_o!r}"
+        dt = L.compute_new_dt(est, dt, t, 0.2, step == 0)
+        lev.advance(S, t, dt)
+        ol.advance(cls, sch, dt, nthreads=4)
+        t += dt
+        step += 1
+        assert step < 2000
+    assert t == pytest.approx(0.2, rel=1e-12)
+    got = S.download()
+    err = rel_err(got, ol.get_state())
+    lev.close()
+    assert err <= TOL, f"sod256 flux {flux}: {err:.3e} after {step} steps"
+    rho = got[0, 0, 0, :]
+    x = (np.arange(256) + 0.5) / 256
+    exact = np.array([ER.sample((xi - 0.5) / 0.2, 1.0, 0.0, 1.0, 0.125, 0.0, 0.1)[0] for xi in x])
+    l1 = float(np.mean(np.abs(rho - exact)))
+    say(f"  {step} steps, L1(rho - exact) = {l1:.4e}")
+    assert l1 < 0.006
+
+
+# ------------------------------------------------------------------ 1D batch
+def _batch_sample(inst):
+    """The shortest, the longest and 14 seeded instances with L <= 512."""
+    Ls = [p["L"] for p in inst]
+    short = min(range(len(inst)), key=lambda b: Ls[b])
+    long_ = max(range(len(inst)), key=lambda b: Ls[b])
+    rng = np.random.default_rng(20261202)
+    pool = [b for b in range(len(inst)) if Ls[b] <= 512 and b not in (short, long_)]
+    pick = sorted(int(b) for b in rng.choice(pool, size=14, replace=False))
+    return [short, long_] + pick
+
+
+def _solo_run(cls, sch, g, u, stop, nthreads):
+    from amrsolver_amd import level as L
+    from oracle import oracle as O
+    ol = O.OracleLevel(g, [(tuple(g.dom_lo), tuple(g.dom_hi))])
+    ol.set_state(u)
+    t, dtp, ests = 0.0, None, []
+    while t < stop - 1e-14:
+        est = ol.est_dt(cls, 0.3)
+        ests.append(est)
+        d = L.compute_new_dt(est, dtp, t, stop, dtp is None)
+        dtp = d
+        ol.advance(cls, sch, d, nthreads=nthreads)
+        t += d
+    return ests, ol.get_state()
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("flux", [2, 0], ids=["weno", "muscl"])
+def test_riemann_batch_512_to_t015(ctx, flux):
+    from amrsolver_amd import level as L
+    cls = P.closures_reference()
+    inst = P.riemann_batch_params()
+    geoms, states = P.riemann_batch_problems(cls, inst)
+    sch = P.scheme(flux_euler=flux, visc=False)
+    sample = _batch_sample(inst)
+    stop = 0.15
+    say(f"case batch512 flux {flux}: sample {[(b, inst[b]['L']) for b in sample]}")
+    # solo oracle runs in host threads (the longest with threads inside its
+    # box), beside the GPU run
+    pool = cf.ThreadPoolExecutor(max_workers=NT)
+    futs = {}
+    for b in sample:
+        inner = max(1, NT // 2) if b == sample[1] else 1
+        futs[b] = pool.submit(_solo_run, cls, sch, geoms[b], states[b], stop, inner)
+    bat = L.Batch(geoms, cls, sch, ctx=ctx)
+    S = bat.new_state()
+    S.upload_problems(states)
+    nb = len(inst)
+    t = [0.0] * nb
+    dtp = [None] * nb
+    ests = {b: [] for b in sample}
+    k = 0
+    while True:
+        est = bat.est_time_steps(S, 0.3)
+        dts = []
+        for b in range(nb):
+            if t[b] >= stop - 1e-14:
+                dts.append(0.0)
+                continue
+            if b in ests:
+                ests[b].append(est[b])
+            d = L.compute_new_dt(est[b], dtp[b], t[b], stop, dtp[b] is None)
+            dtp[b] = d
+            dts.append(d)
+        if not any(dts):
+            break
+        bat.advance_dt(S, dts)
+        for b in range(nb):
+            t[b] += dts[b]
+        k += 1
+        if k % 500 == 0:
+            say(f"  gpu batch step {k}")
+    got = S.download_problems()
+    bat.close()
+    say(f"  gpu batch done after {k} steps; waiting for the oracle")
+    for b in sample:
+        e_o, ref = futs[b].result()
+        say(f"  instance {b} (L={inst[b]['L']}): {len(e_o)} steps")
+        assert ests[b] == e_o, f"instance {b}: time-step sequences differ"
+        err = rel_err(got[b], ref)
+        assert err <= TOL, f"instance {b}: {err:.3e}"
+    pool.shutdown()
+    assert all(abs(tb - stop) <= 1e-12 for tb in t)
+
+
+# ------------------------------------------------------------------ AMR
+@pytest.fixture(scope="module")
+def amr128():
+    return P.tgv_amr_hierarchy(n_base=128, nlev=2)
+
+
+def _level_arrays(lev, cls, g):
+    return [P.tgv_state_box(cls, g, lo, hi, 3) for lo, hi in lev.boxes]
+
+
+def _covered(u, boxes):
+    out = np.zeros_like(u)
+    for lo, hi in boxes:
+        sl = (slice(None), slice(lo[2], hi[2] + 1), slice(lo[1], hi[1] + 1), slice(lo[0], hi[0] + 1))
+        out[sl] = u[sl]
+    return out
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("lev_idx", [1, 2])
+def test_amr_base128_level_step_matches_oracle(ctx, amr128, lev_idx):
+    from amrsolver_amd import level as L
+    from oracle import oracle as O
+    g, boxes = amr128[lev_idx]
+    cls = P.tgv_closures()
+    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
+    dt = 2e-4 / (2 ** lev_idx)
+    lev = L.Level(g, boxes, cls, sch, ctx=ctx)
+    arrs = _level_arrays(lev, cls, g)
+    ncell = sum(int(np.prod([h - l + 1 for l, h in zip(lo, hi)])) for lo, hi in boxes)
+    say(f"case amr128 level {lev_idx}: {len(boxes)} boxes, {ncell} cells; oracle step, {NT} threads")
+    ol = O.OracleLevel(g, boxes)
+    for b, a in enumerate(arrs):
+        ol.fabs[b][...] = a
+    ol.advance(cls, sch, dt, nthreads=NT)
+    ref = ol.get_state()
+    del ol
+    layouts = ["boxes", "level"] if lev_idx == 2 else ["boxes"]
+    for layout in layouts:
+        S = lev.new_state(layout=layout)
+        if layout == "boxes":
+            for b, a in enumerate(arrs):
+                S.upload_box(b, a)
+        else:
+            n = [g.dom_hi[d] + 1 for d in range(3)]
+            S.upload_ghosted(P.tgv_state_box(cls, g, (0, 0, 0), tuple(x - 1 for x in n), 3))
+        lev.fill_patch(S)
+        tiling = lev.autotune(S)
+        say(f"  layout {layout}: tiling {tiling}")
+        lev.advance(S, 0.0, dt)
+        err = rel_err(_covered(S.download(), boxes), ref)
+        del S
+        assert err <= TOL, f"level {lev_idx} {layout}: {err:.3e}"
+    lev.close()
+
+
+@pytest.mark.timeout(600)
+def test_amr_base128_level2_graph_replay(ctx, amr128):
+    import torch
+    from amrsolver_amd import level as L
+    g, boxes = amr128[2]
+    cls = P.tgv_closures()
+    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
+    lev = L.Level(g, boxes, cls, sch, ctx=ctx)
+    n = [g.dom_hi[d] + 1 for d in range(3)]
+    full = P.tgv_state_box(cls, g, (0, 0, 0), tuple(x - 1 for x in n), 3)
+    A, B = lev.new_state(), lev.new_state()
+    A.upload_ghosted(full)
+    B.upload_ghosted(full)
+    del full
+    lev.fill_patch(A)
+    lev.fill_patch(B)
+    lev.autotune(A)
+    say(f"case amr128 level 2 graph: {len(boxes)} boxes")
+    dt = 5e-5
+    for rep in range(3):
+        lev.advance_graph(A, dt, 2)
+        for _ in range(2):
+            lev.advance(B, 0.0, dt)
+        assert torch.equal(A.download_tensor(), B.download_tensor()), f"replay {rep}"
+    del A, B
+    lev.close()
+
+
+# ------------------------------------------------------------------ adjoint
+@pytest.mark.timeout(600)
+def test_adjoint_channel_32_fd_directions(ctx):
+    import torch
+    from amrsolver_amd import level as L
+    n, nsteps, kcheck = (128, 64, 64), 20, 5
+    cls, p0, _ = P.channel_closures()
+    g = P.channel_geom(n)
+    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
+    lev = L.Level(g, P.decompose(n, 64), cls, sch, ctx=ctx)
+    u = P.channel_state(cls, g, n, p0)
+    S0, S = lev.new_state(), lev.new_state()
+    grad = lev.new_state(ng=0)
+    S0.upload(u)
+    lev.fill_patch(S0)
+    dt = lev.est_time_step(S0, 0.3)
+    lev.autotune(S0)
+    J = lev.adjoint_run(S0, dt, nsteps, kcheck, grad)
+    gr = torch.from_numpy(grad.download()).to("cuda")
+    dev = gr.device
+    u_d = torch.from_numpy(np.ascontiguousarray(u)).to(dev)
+    dV = float(np.prod([(g.prob_hi[d] - g.prob_lo[d]) / n[d] for d in range(3)]))
+
+    def J_of(v, h):
+        S.upload_tensor(u_d + h * v)
+        for _ in range(nsteps):
+            lev.advance(S, 0.0, dt)
+        w = S.download_tensor()
+        return float((0.5 * (w[1] ** 2 + w[2] ** 2 + w[3] ** 2) / w[0]).sum()) * dV
+
+    S.upload_tensor(u_d)
+    assert J_of(torch.zeros_like(u_d), 0.0) == pytest.approx(J, rel=1e-12)
+    say(f"case adjoint channel 128x64x64: J = {J:.6e}, 32 directions")
+    rng = np.random.default_rng(20261202)
+    worst = 0.0
+    for k in range(32):
+        v = torch.from_numpy(rng.standard_normal(u.shape)).to(dev)
+        v /= torch.linalg.vector_norm(v)
+        got = float((gr * v).sum())
+        # fourth-order central differences at decreasing steps; keep the
+        # estimate that changes least against the next one
+        hs = [0.04, 0.02, 0.01, 0.005]
+        ests = [(8.0 * (J_of(v, h) - J_of(v, -h)) - (J_of(v, 2 * h) - J_of(v, -2 * h))) / (12.0 * h) for h in hs]
+        i = int(np.argmin([abs(ests[j + 1] - ests[j]) for j in range(len(hs) - 1)]))
+        fd = ests[i + 1]
+        rel = abs(got - fd) / abs(fd)
+        worst = max(worst, rel)
+        if k % 8 == 7:
+            say(f"  {k + 1} directions, worst relative difference {worst:.2e}")
+        assert rel <= 1e-6, f"direction {k}: adjoint {got!r} vs FD {fd!r} ({rel:.2e})"
+    lev.close()
