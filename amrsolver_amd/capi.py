@@ -84,6 +84,16 @@ SIGNATURES = {
                      ctypes.c_double], ctypes.c_int),
     "amr_advance_graph": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF,
                            ctypes.c_double, ctypes.c_int], ctypes.c_int),
+    "amr_clock_set": ([_P, ctypes.c_double], ctypes.c_int),
+    "amr_clock_get": ([_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                       ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+    "amr_advance_cfl": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF, ctypes.c_double,
+                         ctypes.c_double, ctypes.c_int], ctypes.c_int),
+    "amr_advance_cfl_graph": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF, ctypes.c_double,
+                               ctypes.c_double, ctypes.c_int], ctypes.c_int),
+    "amr_new_dt_levels": ([ctypes.c_int, ctypes.POINTER(_P), ctypes.POINTER(Closures), ctypes.POINTER(_PF),
+                           ctypes.POINTER(ctypes.c_int32), ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                           ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "amr_est_dt": ([_P, ctypes.POINTER(Closures), _PF, ctypes.c_double,
                     ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "amr_totals": ([_P, _PF, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),

@@ -243,6 +243,8 @@ struct amr_boxtab_s {
   std::vector<amr_geom_t> geoms;  // per box: its level's or its own problem's geometry
   bool batch = false;             // independent problems, one box each
   double* d_box_dt = nullptr;     // per-box time steps of a batch step
+  Clock* d_clock = nullptr;       // device time-step control (nbox entries; a level uses the first)
+  double* d_dt_levels = nullptr;  // amr_new_dt_levels: per-level steps (MAX_DT_LEVELS + 1)
   double* h_box_dt = nullptr;     // pinned staging for d_box_dt
   hipEvent_t dt_copied = nullptr;
   BoxTable bt;
@@ -1507,6 +1509,47 @@ int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t
   return AMR_OK;
 }
 
+// Every table and workspace a step of `sch` on `state` uses, built before a
+// graph capture (a plain step on a scratch copy would change the state, so
+// the tables are built directly): descriptor sets with their fill tasks,
+// tile tables for box and level tiles, or the pass-structured workspace.
+int prepare_step_tables(amr_boxtab_s* bt, const amr_scheme_t& sch, const amr_fab_t* state) {
+  int rc;
+  if ((rc = ensure_ws(bt, state))) return rc;
+  DescSet* ds = nullptr;
+  if ((rc = get_desc(bt, state, NGHOST, &ds)) || (rc = build_fill(bt, ds))) return rc;
+  // the two-layer fills of the fused stages (as advance_impl decides)
+  const bool narrow = fused_path(bt, sch, nullptr) && sch.order_keep <= 4;
+  if (narrow && (rc = build_fill_narrow(bt, ds))) return rc;
+  for (int w = 0; w < 2; ++w) {
+    DescSet* dw = nullptr;
+    if ((rc = get_desc(bt, bt->ws_fabs[w].data(), NGHOST, &dw)) || (rc = build_fill(bt, dw))) return rc;
+    if (narrow && (rc = build_fill_narrow(bt, dw))) return rc;
+  }
+  FusedConfig cfg = choose_config(bt, sch);
+  const bool fused = cfg.tx > 0 && fused_supported(sch, cfg.tx, cfg.ty) && !bt->walls;
+  TileTable* tt = nullptr;
+  if (fused && fused_fits(bt, cfg) && (rc = get_tiles(bt, cfg, false, &tt))) return rc;
+  if (fused && bt->cover.on && (rc = get_tiles(bt, cfg, true, &tt))) return rc;
+  if (!(fused && fused_fits(bt, cfg)) && (rc = ensure_unfused(bt, sch))) return rc;
+  return AMR_OK;
+}
+
+// One step under device-side time-step control, enqueued only: the per-box
+// CFL minima, the time-step rule into the per-box steps, then the step.
+int cfl_step(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const amr_fab_t* state,
+             double cfl, double stop_time) {
+  int rc;
+  DescSet* ds = nullptr;
+  if ((rc = get_desc(bt, state, NGHOST, &ds))) return rc;
+  hipStream_t st = bt->ctx->stream;
+  HIPCHK(launch_estdt_boxes(bt->bt, ds->d, make_closures(cls), bt->d_partial, bt->d_result, st));
+  HIPCHK(launch_dt_policy(bt->bt.nbox, bt->batch, bt->d_result, bt->d_clock, cfl, stop_time, bt->d_box_dt, st));
+  return advance_impl(bt, cls, sch, state, 1.0, bt->d_box_dt);
+}
+
+int nprob_of(const amr_boxtab_s* bt) { return bt->batch ? bt->bt.nbox : 1; }
+
 // ------------------------------------------------------------ adjoint
 constexpr int ADJ_NE = 16;
 
@@ -1733,6 +1776,7 @@ extern "C" {
 
 const char* amr_last_error(void) { return g_err.c_str(); }
 
+
 int amr_ctx_create(int device, void* stream, amr_ctx_t* out) {
   if (!out) return fail(AMR_ERR_ARG, "null output pointer");
   int ndev = 0;
@@ -1903,6 +1947,12 @@ int finish_boxtab(amr_boxtab_s* bt) {
   if (e == hipSuccess) e = hipMalloc(&bt->d_box_dt, sizeof(double) * nbox);
   if (e == hipSuccess) e = hipHostMalloc(&bt->h_box_dt, sizeof(double) * nbox);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->dt_copied, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipMalloc(&bt->d_clock, sizeof(Clock) * nbox);
+  if (e == hipSuccess) e = hipMalloc(&bt->d_dt_levels, sizeof(double) * (MAX_DT_LEVELS + 1));
+  if (e == hipSuccess) {
+    std::vector<Clock> k(nbox, Clock{0.0, 0.0, 0, 1, 0, 0});
+    e = hipMemcpy(bt->d_clock, k.data(), sizeof(Clock) * nbox, hipMemcpyHostToDevice);
+  }
   if (e == hipSuccess) e = prepare_block_maps(bt->bt);
   if (e != hipSuccess) return fail(AMR_ERR_HIP, std::string("box table allocation: ") + hipGetErrorString(e));
   rc = build_uncovered(bt);
@@ -1944,6 +1994,8 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   if (bt->d_result) (void)hipFree(bt->d_result);
   if (bt->h_result) (void)hipHostFree(bt->h_result);
   if (bt->d_box_dt) (void)hipFree(bt->d_box_dt);
+  if (bt->d_clock) (void)hipFree(bt->d_clock);
+  if (bt->d_dt_levels) (void)hipFree(bt->d_dt_levels);
   if (bt->h_box_dt) (void)hipHostFree(bt->h_box_dt);
   if (bt->dt_copied) (void)hipEventDestroy(bt->dt_copied);
   if (bt->aux) (void)hipStreamDestroy(bt->aux);
@@ -2288,23 +2340,7 @@ int amr_advance_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_sche
     if (g->key == key) ge = g.get();
   hipStream_t st = bt->ctx->stream;
   if (!ge) {
-    // Prepare every table and workspace outside the capture: one plain step
-    // on a scratch copy would change the state, so build tables directly.
-    if ((rc = ensure_ws(bt, state))) return rc;
-    DescSet* ds = nullptr;
-    if ((rc = get_desc(bt, state, NGHOST, &ds)) || (rc = build_fill(bt, ds))) return rc;
-    for (int w = 0; w < 2; ++w) {
-      DescSet* dw = nullptr;
-      if ((rc = get_desc(bt, bt->ws_fabs[w].data(), NGHOST, &dw)) || (rc = build_fill(bt, dw))) return rc;
-    }
-    FusedConfig cfg = choose_config(bt, *sch);
-    // (tables for box tiles, level tiles and the pass-structured kernels, as
-    // the stage arrays may choose)
-    const bool fused = cfg.tx > 0 && fused_supported(*sch, cfg.tx, cfg.ty) && !bt->walls;
-    TileTable* tt = nullptr;
-    if (fused && fused_fits(bt, cfg) && (rc = get_tiles(bt, cfg, false, &tt))) return rc;
-    if (fused && bt->cover.on && (rc = get_tiles(bt, cfg, true, &tt))) return rc;
-    if (!(fused && fused_fits(bt, cfg)) && (rc = ensure_unfused(bt, *sch))) return rc;
+    if ((rc = prepare_step_tables(bt, *sch, state))) return rc;
     HIPCHK(hipStreamSynchronize(st));
     auto g = std::make_unique<GraphEntry>();
     g->key = key;
@@ -2325,6 +2361,145 @@ int amr_advance_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_sche
     bt->graphs.push_back(std::move(g));
   }
   HIPCHK(hipGraphLaunch(ge->exec, st));
+  return AMR_OK;
+}
+
+int amr_clock_set(amr_boxtab_t bt, double time) {
+  if (!bt) return fail(AMR_ERR_ARG, "null box table");
+  if (std::isnan(time)) return fail(AMR_ERR_ARG, "time is NaN");
+  const int n = bt->bt.nbox;
+  std::vector<Clock> k(n, Clock{time, 0.0, 0, 1, 0, 0});
+  hipStream_t st = bt->ctx->stream;
+  HIPCHK(hipMemcpyAsync(bt->d_clock, k.data(), sizeof(Clock) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return AMR_OK;
+}
+
+int amr_clock_get(amr_boxtab_t bt, double* time, double* dt, int32_t* steps) {
+  if (!bt) return fail(AMR_ERR_ARG, "null box table");
+  const int np = nprob_of(bt);
+  std::vector<Clock> k(np);
+  hipStream_t st = bt->ctx->stream;
+  HIPCHK(hipMemcpyAsync(k.data(), bt->d_clock, sizeof(Clock) * np, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  int bad = -1;
+  for (int p = 0; p < np; ++p) {
+    if (time) time[p] = k[p].t;
+    if (dt) dt[p] = k[p].dt;
+    if (steps) steps[p] = k[p].steps;
+    if (k[p].bad && bad < 0) bad = p;
+  }
+  if (bad >= 0)
+    return fail(AMR_ERR_STATE, "non-physical state in problem " + std::to_string(bad) +
+                                   ": CFL time step not positive; its steps stopped there");
+  return AMR_OK;
+}
+
+int amr_advance_cfl(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
+                    double cfl, double stop_time, int nsteps) {
+  if (!bt || !cls || !sch || !state || nsteps < 0 || !(cfl > 0.0)) return fail(AMR_ERR_ARG, "invalid argument");
+  int rc;
+  if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls))) return rc;
+  for (int s = 0; s < nsteps; ++s)
+    if ((rc = cfl_step(bt, *cls, *sch, state, cfl, stop_time))) return rc;
+  return AMR_OK;
+}
+
+int amr_advance_cfl_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch,
+                          const amr_fab_t* state, double cfl, double stop_time, int nsteps) {
+  if (!bt || !cls || !sch || !state || nsteps <= 0 || !(cfl > 0.0)) return fail(AMR_ERR_ARG, "invalid argument");
+  int rc;
+  if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls))) return rc;
+  int nb = bt->bt.nbox;
+  const FusedConfig kcfg = choose_config(bt, *sch);
+  // key: a tag (CFL graph), the state, closures, scheme, cfl, stop time,
+  // step count and the tiling in force
+  const char tag = 'C';
+  std::vector<char> key(1 + sizeof(amr_fab_t) * nb + sizeof(amr_closures_t) + sizeof(amr_scheme_t) +
+                        2 * sizeof(double) + sizeof(int) + sizeof(FusedConfig));
+  char* p = key.data();
+  *p++ = tag;
+  std::memcpy(p, state, sizeof(amr_fab_t) * nb);
+  p += sizeof(amr_fab_t) * nb;
+  std::memcpy(p, cls, sizeof(amr_closures_t));
+  p += sizeof(amr_closures_t);
+  std::memcpy(p, sch, sizeof(amr_scheme_t));
+  p += sizeof(amr_scheme_t);
+  std::memcpy(p, &cfl, sizeof(double));
+  p += sizeof(double);
+  std::memcpy(p, &stop_time, sizeof(double));
+  p += sizeof(double);
+  std::memcpy(p, &nsteps, sizeof(int));
+  p += sizeof(int);
+  std::memcpy(p, &kcfg, sizeof(FusedConfig));
+  GraphEntry* ge = nullptr;
+  for (auto& g : bt->graphs)
+    if (g->key == key) ge = g.get();
+  hipStream_t st = bt->ctx->stream;
+  if (!ge) {
+    if ((rc = prepare_step_tables(bt, *sch, state))) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    auto g = std::make_unique<GraphEntry>();
+    g->key = key;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < nsteps; ++s) {
+      rc = cfl_step(bt, *cls, *sch, state, cfl, stop_time);
+      if (rc) {
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(st, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        return rc;
+      }
+    }
+    HIPCHK(hipStreamEndCapture(st, &g->graph));
+    HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    ge = g.get();
+    if (bt->graphs.size() >= 8) bt->graphs.erase(bt->graphs.begin());
+    bt->graphs.push_back(std::move(g));
+  }
+  HIPCHK(hipGraphLaunch(ge->exec, st));
+  return AMR_OK;
+}
+
+int amr_new_dt_levels(int nlev, const amr_boxtab_t* levels, const amr_closures_t* cls,
+                      const amr_fab_t* const* states, const int32_t* n_cycle, double cfl, double cur_time,
+                      double stop_time, int first, double* dt_level) {
+  if (nlev <= 0 || nlev > MAX_DT_LEVELS || !levels || !cls || !states || !n_cycle || !dt_level || !(cfl > 0.0))
+    return fail(AMR_ERR_ARG, "invalid argument (1 <= nlev <= 8, cfl > 0)");
+  int rc = check_closures(*cls);
+  if (rc) return rc;
+  DtLevels a{};
+  a.nlev = nlev;
+  a.first = first ? 1 : 0;
+  a.cfl = cfl;
+  a.cur_time = cur_time;
+  a.stop_time = stop_time;
+  amr_boxtab_s* b0 = levels[0];
+  if (!b0) return fail(AMR_ERR_ARG, "null box table");
+  hipStream_t st0 = b0->ctx->stream;
+  const Closures c = make_closures(*cls);
+  for (int l = 0; l < nlev; ++l) {
+    amr_boxtab_s* bt = levels[l];
+    if (!bt || !states[l]) return fail(AMR_ERR_ARG, "null box table or state");
+    if (n_cycle[l] <= 0) return fail(AMR_ERR_ARG, "n_cycle entries must be positive");
+    if (bt->ctx->device != b0->ctx->device) return fail(AMR_ERR_ARG, "levels on different devices");
+    DescSet* ds = nullptr;
+    if ((rc = get_desc(bt, states[l], 0, &ds))) return rc;
+    hipStream_t st = bt->ctx->stream;
+    HIPCHK(launch_estdt(bt->bt, ds->d, c, bt->d_partial, bt->d_result, st));
+    if (st != st0) HIPCHK(hipStreamSynchronize(st));
+    a.est[l] = bt->d_result;
+    a.ncycle[l] = n_cycle[l];
+  }
+  double h[MAX_DT_LEVELS + 1];
+  for (int l = 0; l < nlev; ++l) h[l] = dt_level[l];
+  HIPCHK(hipMemcpyAsync(b0->d_dt_levels, h, sizeof(double) * nlev, hipMemcpyHostToDevice, st0));
+  HIPCHK(launch_dt_levels(a, b0->d_dt_levels, st0));
+  HIPCHK(hipMemcpyAsync(h, b0->d_dt_levels, sizeof(double) * (nlev + 1), hipMemcpyDeviceToHost, st0));
+  HIPCHK(hipStreamSynchronize(st0));
+  if (!(h[nlev] > 0.0))
+    return fail(AMR_ERR_STATE, "non-physical state: CFL time step is not positive (negative pressure or NaN)");
+  for (int l = 0; l < nlev; ++l) dt_level[l] = h[l];
   return AMR_OK;
 }
 

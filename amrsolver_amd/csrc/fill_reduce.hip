@@ -387,6 +387,95 @@ hipError_t launch_estdt(const BoxTable& bt, const Fab* st, const Closures& c, do
   return hipGetLastError();
 }
 
+// The time-step rule of CNS::computeInitialDt / computeNewDt
+// (src/CNS.cpp:282-377) for one problem: the estimate on the first step,
+// else min(estimate, 1.1 * previous step) (std::min order), cut to
+// stop_time - t when t + dt > stop_time - 0.001 dt (stop_time >= 0).  A
+// problem within 1e-14 of stop_time takes no step (0); a non-positive or
+// NaN minimum marks the problem bad and stops it.
+__device__ double clock_step(Clock& k, double emin, double cfl, double stop) {
+  if (k.bad) return 0.0;
+  if (stop >= 0.0 && k.t >= stop - 1e-14) return 0.0;
+  if (!(emin > 0.0)) {
+    k.bad = 1;
+    return 0.0;
+  }
+  const double est = emin * cfl;  // CNS::estTimeStep, src/CNS.cpp:391
+  double dt0 = est;
+  if (!k.first) {
+    dt0 = dmin(est, 1.1 * k.dt);
+    const double eps = 0.001 * dt0;
+    if (stop >= 0.0 && (k.t + dt0) > (stop - eps)) dt0 = stop - k.t;
+  }
+  k.t = k.t + dt0;
+  k.dt = dt0;
+  k.steps += 1;
+  k.first = 0;
+  return dt0;
+}
+
+__global__ void __launch_bounds__(RBLK) k_dt_policy(int nbox, int batch, const double* est, Clock* clk,
+                                                    double cfl, double stop, double* box_dt) {
+  if (batch) {
+    const int p = blockIdx.x * RBLK + threadIdx.x;
+    if (p < nbox) box_dt[p] = clock_step(clk[p], est[p], cfl, stop);
+    return;
+  }
+  // one problem: the minimum over its boxes (exact in any order)
+  __shared__ double sm[RBLK];
+  double v = 1.79769313486231570815e+308;
+  for (int b = threadIdx.x; b < nbox; b += RBLK) v = dmin(v, est[b]);
+  const double m = block_min(v, sm);
+  __shared__ double s_dt;
+  if (threadIdx.x == 0) s_dt = clock_step(clk[0], m, cfl, stop);
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbox; b += RBLK) box_dt[b] = s_dt;
+}
+
+hipError_t launch_dt_policy(int nbox, bool batch, const double* est, Clock* clk, double cfl, double stop_time,
+                            double* box_dt, hipStream_t s) {
+  const int grid = batch ? (nbox + RBLK - 1) / RBLK : 1;
+  hipLaunchKernelGGL(k_dt_policy, dim3(grid), dim3(RBLK), 0, s, nbox, batch ? 1 : 0, est, clk, cfl, stop_time,
+                     box_dt);
+  return hipGetLastError();
+}
+
+// computeNewDt over levels (src/CNS.cpp:337-376): per level the estimate,
+// limited by 1.1 times its previous step (computeInitialDt on the first
+// call: the estimate alone); dt0 = min over levels of nfactor_l * dt_l with
+// nfactor_l the product of n_cycle[0..l]; the stop-time cut of dt0; dt_l =
+// dt0 / nfactor_l.  dt_level[nlev] receives the steps, dt_level[nlev] the
+// smallest minimum found (<= 0 or NaN: a non-physical state).
+__global__ void k_dt_levels(DtLevels a, double* dt_level) {
+  if (threadIdx.x != 0) return;
+  double dt0 = 1.79769313486231570815e+308;
+  double worst = dt0;
+  int nfactor = 1;
+  for (int l = 0; l < a.nlev; ++l) {
+    const double m = *a.est[l];
+    worst = (m > 0.0) ? dmin(worst, m) : m;
+    double dmn = m * a.cfl;
+    if (!a.first) dmn = dmin(dmn, 1.1 * dt_level[l]);
+    nfactor *= a.ncycle[l];
+    dt0 = dmin(dt0, nfactor * dmn);
+  }
+  if (!a.first) {
+    const double eps = 0.001 * dt0;
+    if (a.stop_time >= 0.0 && (a.cur_time + dt0) > (a.stop_time - eps)) dt0 = a.stop_time - a.cur_time;
+  }
+  nfactor = 1;
+  for (int l = 0; l < a.nlev; ++l) {
+    nfactor *= a.ncycle[l];
+    dt_level[l] = dt0 / nfactor;
+  }
+  dt_level[a.nlev] = worst;
+}
+
+hipError_t launch_dt_levels(const DtLevels& a, double* dt_level, hipStream_t s) {
+  hipLaunchKernelGGL(k_dt_levels, dim3(1), dim3(64), 0, s, a, dt_level);
+  return hipGetLastError();
+}
+
 int estdt_box_blocks(const BoxTable& bt) {
   int64_t mx = 1;
   for (auto& b : bt.h_boxes)

@@ -260,6 +260,32 @@ hipError_t launch_estdt(const BoxTable& bt, const Fab* st, const Closures& c, do
 int estdt_box_blocks(const BoxTable& bt);
 hipError_t launch_estdt_boxes(const BoxTable& bt, const Fab* st, const Closures& c, double* partial,
                               double* result, hipStream_t s);
+// Device-side time-step control: the clock of each problem of a box table
+// (the level, or each problem of a batch).  first = 1: the next step takes
+// the CFL estimate itself (computeInitialDt); bad = 1: a non-physical state
+// stopped the problem.
+struct Clock {
+  double t, dt;
+  int32_t steps, first, bad, pad_;
+};
+// Per box (est: per-box minima of cns_estdt, without the CFL factor) the
+// time step of its problem (batch: box b is problem b; else one problem)
+// under the policy of CNS::computeNewDt, written to box_dt; clocks advance.
+hipError_t launch_dt_policy(int nbox, bool batch, const double* est, Clock* clk, double cfl, double stop_time,
+                            double* box_dt, hipStream_t s);
+// computeNewDt's multi-level rule (src/CNS.cpp:319-377) for up to
+// MAX_DT_LEVELS levels: est[l] points at level l's minimum of cns_estdt;
+// dt_level (device, nlev entries) holds the previous steps on entry
+// (ignored when first != 0) and the new ones on exit.
+constexpr int MAX_DT_LEVELS = 8;
+struct DtLevels {
+  const double* est[MAX_DT_LEVELS];
+  int32_t ncycle[MAX_DT_LEVELS];
+  int32_t nlev, first;
+  double cfl, cur_time, stop_time;
+};
+hipError_t launch_dt_levels(const DtLevels& a, double* dt_level, hipStream_t s);
+
 hipError_t launch_totals(const BoxTable& bt, const Fab* st, double* partial, double* result,
                          hipStream_t s);
 

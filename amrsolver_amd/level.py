@@ -331,6 +331,31 @@ class Level:
              ctypes.byref(out))
         return out.value
 
+    # --- device-side time-step control ----------------------------------
+    def clock_set(self, time=0.0):
+        """Every problem's device clock to `time`; the next step takes the
+        CFL estimate itself (computeInitialDt)."""
+        call("amr_clock_set", self.h, time)
+
+    def clock_get(self):
+        """(times, last steps, step counts), one entry per problem."""
+        n = len(self.boxes) if isinstance(self, Batch) else 1
+        t = (ctypes.c_double * n)()
+        d = (ctypes.c_double * n)()
+        k = (ctypes.c_int32 * n)()
+        call("amr_clock_get", self.h, t, d, k)
+        return list(t), list(d), list(k)
+
+    def advance_cfl(self, S, cfl, stop_time, nsteps=1):
+        """nsteps steps whose sizes the device picks (CFL estimate, 1.1x
+        growth cap, stop-time cut); no host synchronisation."""
+        call("amr_advance_cfl", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, cfl,
+             stop_time, nsteps)
+
+    def advance_cfl_graph(self, S, cfl, stop_time, nsteps):
+        call("amr_advance_cfl_graph", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs,
+             cfl, stop_time, nsteps)
+
     def est_time_step(self, S, cfl):
         out = ctypes.c_double()
         call("amr_est_dt", self.h, ctypes.byref(self.closures), S.fabs, cfl, ctypes.byref(out))
@@ -343,6 +368,20 @@ class Level:
 
     def synchronize(self):
         self.ctx.synchronize()
+
+
+def new_dt_levels(levels, states, n_cycle, cfl, cur_time, stop_time, dt_prev, first):
+    """computeNewDt / computeInitialDt over a hierarchy, on the device:
+    per-level steps (list) from the levels' states and their previous
+    steps dt_prev."""
+    n = len(levels)
+    hs = (ctypes.c_void_p * n)(*[lv.h.value for lv in levels])
+    ps = (ctypes.POINTER(Fab) * n)(*[ctypes.cast(S.fabs, ctypes.POINTER(Fab)) for S in states])
+    nc = (ctypes.c_int32 * n)(*n_cycle)
+    dt = (ctypes.c_double * n)(*[0.0 if d is None else d for d in dt_prev])
+    call("amr_new_dt_levels", n, hs, ctypes.byref(levels[0].closures), ps, nc, cfl, cur_time, stop_time,
+         1 if first else 0, dt)
+    return list(dt)
 
 
 def compute_new_dt(est, dt_prev, cur_time, stop_time, first):

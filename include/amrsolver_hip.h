@@ -216,6 +216,49 @@ int amr_advance(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* 
 int amr_advance_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch,
                       const amr_fab_t* state, double dt, int nsteps);
 
+/* ---- Device-side time-step control (CNS::computeInitialDt /
+ * CNS::computeNewDt, src/CNS.cpp:282-377, with CNS::estTimeStep,
+ * src/CNS.cpp:380-395).  Every problem of a box table -- the level, or each
+ * problem of a batch -- keeps a clock in device memory: its time, its last
+ * time step and its step count. */
+
+/* Sets every problem's time; the next step of each takes the CFL estimate
+ * itself, as computeInitialDt does.  Synchronous. */
+int amr_clock_set(amr_boxtab_t bt, double time);
+
+/* The clocks (nprob entries: 1 for a level, one per problem of a batch;
+ * any output may be NULL).  AMR_ERR_STATE when a problem met a non-physical
+ * state (CFL estimate not positive or NaN); its steps stopped there.
+ * Synchronous. */
+int amr_clock_get(amr_boxtab_t bt, double* time, double* dt, int32_t* steps);
+
+/* nsteps steps under device-side time-step control, with no host
+ * synchronisation: per step and problem the estimate cfl * min cns_estdt,
+ * the step (the estimate on the first step after amr_clock_set, later
+ * min(estimate, 1.1 * previous step), cut to stop_time - time when time +
+ * step > stop_time - 0.001 step; stop_time < 0: no stop time), then
+ * CNS::advance with that step.  A problem within 1e-14 of stop_time takes
+ * no further steps (its arrays are left untouched). */
+int amr_advance_cfl(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
+                    double cfl, double stop_time, int nsteps);
+
+/* amr_advance_cfl captured into a hipGraph on the first call for a given
+ * (state, closures, scheme, cfl, stop_time, nsteps) and replayed after. */
+int amr_advance_cfl_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch,
+                          const amr_fab_t* state, double cfl, double stop_time, int nsteps);
+
+/* Time steps of a subcycled hierarchy (computeNewDt's multi-level rule,
+ * src/CNS.cpp:319-377; computeInitialDt, :283-316, when first != 0),
+ * evaluated on the device: per level l the estimate cfl * min cns_estdt of
+ * states[l], limited by 1.1 * dt_level[l] (not when first); dt0 = min over
+ * levels of nfactor_l * that, nfactor_l = n_cycle[0] * ... * n_cycle[l];
+ * cut to stop_time - cur_time as above (not when first); dt_level[l] =
+ * dt0 / nfactor_l on exit.  1 <= nlev <= 8, levels on one device.
+ * Synchronous. */
+int amr_new_dt_levels(int nlev, const amr_boxtab_t* levels, const amr_closures_t* cls,
+                      const amr_fab_t* const* states, const int32_t* n_cycle, double cfl, double cur_time,
+                      double stop_time, int first, double* dt_level);
+
 /* CFL time step: cfl * min over valid cells of cns_estdt (src/cls/CNS_K.h:12-47,
  * CNS::estTimeStep src/CNS.cpp:380-395).  Synchronous. */
 int amr_est_dt(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, double cfl,
