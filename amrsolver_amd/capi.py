@@ -96,6 +96,8 @@ SIGNATURES = {
                            ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "amr_est_dt": ([_P, ctypes.POINTER(Closures), _PF, ctypes.c_double,
                     ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "amr_est_dt_noalloc": ([_P, ctypes.POINTER(Closures), _PF, ctypes.c_double,
+                            ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "amr_totals": ([_P, _PF, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "amr_batch_create": ([_P, ctypes.c_int, ctypes.POINTER(Geom), ctypes.POINTER(_P)], ctypes.c_int),
     "amr_batch_advance": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF,

@@ -14,6 +14,7 @@
 #include <sstream>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -121,6 +122,7 @@ Mapping mapping_of(const amr_fab_t& f) {
 struct amr_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
+  void* scratch = nullptr;  // this GPU's no-allocation scratch array (device address)
   bool own_stream = false;
   // the caller's handle plus one per live box table: the context (and its
   // stream) is freed when the last of them goes, in whatever order the
@@ -1776,6 +1778,14 @@ extern "C" {
 
 const char* amr_last_error(void) { return g_err.c_str(); }
 
+namespace {
+// Device address of the no-allocation scratch array of each GPU, taken when
+// the first context on that GPU is created, with that GPU current (each GPU
+// holds its own copy of the array at its own address).
+constexpr int SCRATCH_MAX_DEV = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[SCRATCH_MAX_DEV] = {};
+}  // namespace
 
 int amr_ctx_create(int device, void* stream, amr_ctx_t* out) {
   if (!out) return fail(AMR_ERR_ARG, "null output pointer");
@@ -1783,8 +1793,15 @@ int amr_ctx_create(int device, void* stream, amr_ctx_t* out) {
   HIPCHK(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(AMR_ERR_ARG, "device index out of range");
   HIPCHK(hipSetDevice(device));
+  void* scr = nullptr;
+  if (device < SCRATCH_MAX_DEV) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) HIPCHK(noalloc_scratch_address(&g_scratch_dev[device]));
+    scr = g_scratch_dev[device];
+  }
   auto* c = new amr_ctx_s();
   c->device = device;
+  c->scratch = scr;
   if (stream) {
     c->stream = reinterpret_cast<hipStream_t>(stream);
   } else {
@@ -2675,6 +2692,39 @@ int amr_est_dt(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* stat
   HIPCHK(hipMemcpyAsync(bt->h_result, bt->d_result, sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   double dt = bt->h_result[0];
+  if (!(dt > 0.0) || std::isnan(dt))
+    return fail(AMR_ERR_STATE, "non-physical state: CFL time step is not positive (negative pressure or NaN)");
+  *dt_out = dt * cfl;  // src/CNS.cpp:391
+  return AMR_OK;
+}
+
+int amr_est_dt_noalloc(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, double cfl,
+                       double* dt_out) {
+  if (!bt || !cls || !state || !dt_out) return fail(AMR_ERR_ARG, "null argument");
+  int rc = check_closures(*cls);
+  if (rc) return rc;
+  const int nb = bt->bt.nbox;
+  if (nb > NOALLOC_MAX_BOXES)
+    return fail(AMR_ERR_ARG, "amr_est_dt_noalloc: " + std::to_string(nb) + " boxes, at most " +
+                                 std::to_string(NOALLOC_MAX_BOXES));
+  if ((rc = check_fabs(bt, state, 0))) return rc;
+  void* scr = bt->ctx->scratch;
+  if (!scr) return fail(AMR_ERR_ARG, "amr_est_dt_noalloc: no scratch array on this device");
+  hipStream_t st = bt->ctx->stream;
+  // descriptors into the scratch array through a fixed host buffer
+  Fab chunk[128];
+  for (int b0 = 0; b0 < nb; b0 += 128) {
+    const int m = std::min(128, nb - b0);
+    for (int i = 0; i < m; ++i) chunk[i] = make_fab(state[b0 + i]);
+    HIPCHK(hipMemcpyAsync(static_cast<char*>(scr) + size_t(b0) * sizeof(Fab), chunk, sizeof(Fab) * m,
+                          hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));  // before the buffer is refilled
+  }
+  double* res = nullptr;
+  HIPCHK(launch_estdt_scratch(bt->bt, make_closures(*cls), scr, &res, st));
+  double dt = 0.0;
+  HIPCHK(hipMemcpyAsync(&dt, res, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   if (!(dt > 0.0) || std::isnan(dt))
     return fail(AMR_ERR_STATE, "non-physical state: CFL time step is not positive (negative pressure or NaN)");
   *dt_out = dt * cfl;  // src/CNS.cpp:391

@@ -387,6 +387,31 @@ hipError_t launch_estdt(const BoxTable& bt, const Fab* st, const Closures& c, do
   return hipGetLastError();
 }
 
+// Scratch of the no-allocation CFL reduction: one copy per GPU, held until
+// the process ends.  Host code never passes the array's name to a kernel
+// (on the host it is a stand-in, not a device address): it takes the
+// current device's address once through noalloc_scratch_address() and
+// passes that.  Every call on a GPU shares its copy; nothing in it needs to
+// start at zero (the descriptors are copied in, each partial slot and the
+// result are written before they are read).
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+hipError_t noalloc_scratch_address(void** ptr) { return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch)); }
+
+hipError_t launch_estdt_scratch(const BoxTable& bt, const Closures& c, void* scratch_dev, double** result_out,
+                                hipStream_t s) {
+  char* base = static_cast<char*>(scratch_dev);
+  const Fab* st = reinterpret_cast<const Fab*>(base);
+  double* partial = reinterpret_cast<double*>(base + NOALLOC_DESC_BYTES);
+  double* result = partial + NOALLOC_MAX_BLOCKS;
+  const int nb = reduce_blocks(bt);
+  if (bt.nbox > NOALLOC_MAX_BOXES || nb > NOALLOC_MAX_BLOCKS) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_estdt, dim3(nb), dim3(RBLK), 0, s, bt.d_boxes, bt.nbox, st, c, partial);
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(RBLK), 0, s, partial, nb, result);
+  *result_out = result;
+  return hipGetLastError();
+}
+
 // The time-step rule of CNS::computeInitialDt / computeNewDt
 // (src/CNS.cpp:282-377) for one problem: the estimate on the first step,
 // else min(estimate, 1.1 * previous step) (std::min order), cut to

@@ -286,6 +286,21 @@ struct DtLevels {
 };
 hipError_t launch_dt_levels(const DtLevels& a, double* dt_level, hipStream_t s);
 
+// No-allocation CFL reduction (amr_est_dt_noalloc).  Its scratch is one
+// file-scope device array of NOALLOC_BYTES per GPU: the descriptors of at
+// most NOALLOC_MAX_BOXES boxes, then NOALLOC_MAX_BLOCKS per-block partial
+// minima, then the result.  noalloc_scratch_address() gives the current
+// device's copy of the array (taken once per device at context creation).
+constexpr int NOALLOC_MAX_BOXES = 8192;
+constexpr int NOALLOC_MAX_BLOCKS = 1024;  // reduce_blocks() never exceeds it
+constexpr size_t NOALLOC_DESC_BYTES = size_t(NOALLOC_MAX_BOXES) * sizeof(Fab);
+constexpr size_t NOALLOC_BYTES = NOALLOC_DESC_BYTES + size_t(NOALLOC_MAX_BLOCKS) * sizeof(double) + 256;
+hipError_t noalloc_scratch_address(void** ptr);
+// k_estdt and k_min_final on the descriptors already copied to the start of
+// `scratch`; the minimum (without the CFL factor) lands in the result slot,
+// whose address *result_out receives
+hipError_t launch_estdt_scratch(const BoxTable& bt, const Closures& c, void* scratch, double** result_out,
+                                hipStream_t s);
 hipError_t launch_totals(const BoxTable& bt, const Fab* st, double* partial, double* result,
                          hipStream_t s);
 

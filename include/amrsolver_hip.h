@@ -264,6 +264,18 @@ int amr_new_dt_levels(int nlev, const amr_boxtab_t* levels, const amr_closures_t
 int amr_est_dt(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, double cfl,
                double* dt_out);
 
+/* amr_est_dt without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (tight stepping loops, code
+ * that must not call hipMalloc).  The box descriptors (at most 8192 boxes,
+ * else AMR_ERR_ARG) and the per-block partial minima live in one fixed
+ * device array of the library, about 330 KB per GPU, held until the process
+ * ends, instead of memory the library allocates; the call makes no
+ * hipMalloc/hipFree.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (two host threads, or contexts with different streams)
+ * race.  Same result and errors as amr_est_dt.  Synchronous. */
+int amr_est_dt_noalloc(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, double cfl,
+                       double* dt_out);
+
 /* Batch of `nprob` independent problems advanced together: problem b is
  * the single box geoms[b].dom_lo..dom_hi with its own cell size, periodicity
  * and boundary codes.  Every entry point that takes a box table accepts a
