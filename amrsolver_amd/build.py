@@ -30,7 +30,6 @@ COMMON = [
 DEVICE = [
     f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-munsafe-fp-atomics",
 ]
 
 SOURCES = ["capi.cpp", "unfused.hip", "fused.hip", "fill_reduce.hip", "adjoint.hip", "dir_flux.hip"]

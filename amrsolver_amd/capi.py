@@ -11,6 +11,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "lib", "libamrsolver_hip.so")
 
 AMR_OK = 0
+AMR_ERR_ARG = -1
+AMR_ERR_HIP = -2
+AMR_ERR_STATE = -3
 NCONS = 5
 NGHOST = 3
 
@@ -98,6 +101,10 @@ SIGNATURES = {
                     ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "amr_est_dt_noalloc": ([_P, ctypes.POINTER(Closures), _PF, ctypes.c_double,
                             ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "amr_diagnostics": ([_P, ctypes.POINTER(Closures), _PF, ctypes.c_double] + [ctypes.POINTER(ctypes.c_double)] * 5,
+                        ctypes.c_int),
+    "amr_check_state": ([_P, ctypes.POINTER(Closures), _PF, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+    "amr_set_state_check": ([_P, ctypes.c_int], ctypes.c_int),
     "amr_totals": ([_P, _PF, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "amr_batch_create": ([_P, ctypes.c_int, ctypes.POINTER(Geom), ctypes.POINTER(_P)], ctypes.c_int),
     "amr_batch_advance": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF,

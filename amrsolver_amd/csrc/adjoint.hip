@@ -17,8 +17,10 @@
 //   * cons2prim is transposed per cell.
 // The stage combination is transposed on the valid cells in the same pass;
 // the ghost fill is then transposed (BC tasks in reverse direction order,
-// then the copy tasks) with weighted atomic accumulation into the stage
-// adjoint at the source cells.
+// then the copy tasks) by weighted accumulation into the stage adjoint at
+// the source cells, in a fixed order and without atomics (one lane per
+// boundary line; copy tasks in classes whose sources do not overlap), so
+// that runs repeat bit for bit.
 #include "cns_point.h"
 #include "kernels.h"
 #include "region.h"
@@ -370,45 +372,55 @@ __device__ __forceinline__ double g_weight(const StageArgs& sa, int box) {
   return sc.mode == 0 ? sc.b * sc.d : (sc.mode == 1 ? sc.d : 1.0);
 }
 
-// Transposed physical-BC fill: the ghost's adjoint moves into its source
-// cell (foextrap: boundary cell; reflect: mirror, negated for odd) -- into
-// lam_sin (weighted) when the source is a valid cell, else into the source
-// ghost of G.  The ghost is cleared, because the forward overwrote it.
-__global__ void __launch_bounds__(FB) k_fold_bc(const BcTask* tasks, int ntask, int64_t ncell, const Fab* gfabs,
-                                                const BoxDev* boxes, StageArgs sa, const Fab* lam_sin) {
+// Transposed physical-boundary fill of one direction and side (z, y, x in
+// turn, the reverse of the fill's order): one lane per line along the
+// direction through the task's ghost layers, taken outermost first.  Each
+// ghost cell's adjoint moves to its source cell (the boundary cell for
+// foextrap, the mirror cell for reflections, negated for odd ones) -- into
+// the stage adjoint, weighted, when that cell is valid, else into the ghost
+// adjoint of another direction -- and is cleared.  A line's sources lie on
+// the line inside the box, so no two lanes (or boxes) share one: plain adds
+// in a fixed order, the same bits on every run.
+__global__ void __launch_bounds__(FB) k_fold_bc_lines(const BcTask* tasks, int ntask, int64_t nlines,
+                                                      const Fab* gfabs, const BoxDev* boxes, StageArgs sa,
+                                                      const Fab* lam_sin) {
   int64_t t = int64_t(blockIdx.x) * FB + threadIdx.x;
-  if (t >= ncell) return;
+  if (t >= nlines) return;
   const BcTask& T = tasks[find_task(tasks, ntask, t)];
   if (!box_active(sa, T.box)) return;
-  int64_t r = t - T.start;
-  int nx = T.hi[0] - T.lo[0] + 1, ny = T.hi[1] - T.lo[1] + 1;
+  const int d = T.dir, e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+  const int64_t r = t - T.start;
+  const int n1 = T.hi[e1] - T.lo[e1] + 1;
   int c[3];
-  c[0] = T.lo[0] + int(r % nx);
-  int64_t r2 = r / nx;
-  c[1] = T.lo[1] + int(r2 % ny);
-  c[2] = T.lo[2] + int(r2 / ny);
+  c[e1] = T.lo[e1] + int(r % n1);
+  c[e2] = T.lo[e2] + int(r / n1);
   const Fab f = gfabs[T.box], Li = lam_sin[T.box];
   const BoxDev bx = boxes[T.box];
   const double w = g_weight(sa, T.box);
-  const int cd = c[T.dir];
-  for (int n = 0; n < NCONS; ++n) {
-    int ty = T.type[n];
-    if (ty == 0) continue;
-    int s[3] = {c[0], c[1], c[2]};
-    if (ty == 2) s[T.dir] = T.bnd;
-    else s[T.dir] = T.side == 0 ? 2 * T.bnd - 1 - cd : 2 * T.bnd + 1 - cd;
-    double v = f(c[0], c[1], c[2], n);
-    f(c[0], c[1], c[2], n) = 0.0;
-    if (ty == 4) v = -v;
-    if (in_box(bx, 0, s[0], s[1], s[2])) atomicAdd(&Li(s[0], s[1], s[2], n), w * v);
-    else atomicAdd(&f(s[0], s[1], s[2], n), v);
+  const int first = T.side == 0 ? T.lo[d] : T.hi[d], step = T.side == 0 ? 1 : -1;
+  for (int cd = first; T.side == 0 ? cd <= T.hi[d] : cd >= T.lo[d]; cd += step) {
+    c[d] = cd;
+    for (int n = 0; n < NCONS; ++n) {
+      const int ty = T.type[n];
+      if (ty == 0) continue;
+      int sc[3] = {c[0], c[1], c[2]};
+      if (ty == 2) sc[d] = T.bnd;
+      else sc[d] = T.side == 0 ? 2 * T.bnd - 1 - cd : 2 * T.bnd + 1 - cd;
+      double v = f(c[0], c[1], c[2], n);
+      f(c[0], c[1], c[2], n) = 0.0;
+      if (ty == 4) v = -v;
+      if (in_box(bx, 0, sc[0], sc[1], sc[2])) Li(sc[0], sc[1], sc[2], n) += w * v;
+      else f(sc[0], sc[1], sc[2], n) += v;
+    }
   }
 }
 
-// Transposed same-level / periodic copies: ghost adjoint of `dst` added,
-// weighted, to lam_sin at the valid source cell of `src`.
-__global__ void __launch_bounds__(FB) k_fold_copy(const CopyTask* tasks, int ntask, int64_t ncell,
-                                                  const Fab* gfabs, StageArgs sa, const Fab* lam_sin) {
+// Transposed same-level / periodic copies of one colour class: the ghost
+// adjoint of `dst` added, weighted, to lam_sin at the valid source cell of
+// `src`.  Within a class no two tasks share a source cell, so the adds need
+// no atomics; the classes run one after another in a fixed order.
+__global__ void __launch_bounds__(FB) k_fold_copy_class(const CopyTask* tasks, int ntask, int64_t ncell,
+                                                        const Fab* gfabs, StageArgs sa, const Fab* lam_sin) {
   int64_t t = int64_t(blockIdx.x) * FB + threadIdx.x;
   if (t >= ncell) return;
   const CopyTask& T = tasks[find_task(tasks, ntask, t)];
@@ -422,7 +434,7 @@ __global__ void __launch_bounds__(FB) k_fold_copy(const CopyTask* tasks, int nta
   const Fab d = gfabs[T.dst], s = lam_sin[T.src];
   const double w = g_weight(sa, T.src);
 #pragma unroll
-  for (int n = 0; n < NCONS; ++n) atomicAdd(&s(i - T.sh[0], j - T.sh[1], k - T.sh[2], n), w * d(i, j, k, n));
+  for (int n = 0; n < NCONS; ++n) s(i - T.sh[0], j - T.sh[1], k - T.sh[2], n) += w * d(i, j, k, n);
 }
 
 constexpr int OB = 256;
@@ -551,12 +563,13 @@ hipError_t launch_adj_stage(const BoxTable& bt, const UnfusedWork& w, const AdjW
   else { if (vi) CELL(1, true, true); else CELL(1, true, false); }
 #undef CELL
   for (int d = 2; d >= 0; --d)
-    if (ft.nbc_cells[d] > 0)
-      hipLaunchKernelGGL(k_fold_bc, dim3(unsigned((ft.nbc_cells[d] + FB - 1) / FB)), dim3(FB), 0, st, ft.bc[d],
-                         ft.nbc[d], ft.nbc_cells[d], aw.g, B, sa, lam_sin);
-  if (ft.ncopy_cells > 0)
-    hipLaunchKernelGGL(k_fold_copy, dim3(unsigned((ft.ncopy_cells + FB - 1) / FB)), dim3(FB), 0, st, ft.copies,
-                       ft.ncopy, ft.ncopy_cells, aw.g, sa, lam_sin);
+    for (int side = 0; side < 2; ++side)
+      if (ft.nlines[d][side] > 0)
+        hipLaunchKernelGGL(k_fold_bc_lines, dim3(unsigned((ft.nlines[d][side] + FB - 1) / FB)), dim3(FB), 0, st,
+                           ft.bc[d][side], ft.nbc[d][side], ft.nlines[d][side], aw.g, B, sa, lam_sin);
+  for (int k = 0; k < ft.nclass; ++k)
+    hipLaunchKernelGGL(k_fold_copy_class, dim3(unsigned((ft.class_cells[k] + FB - 1) / FB)), dim3(FB), 0, st,
+                       ft.classes[k], ft.class_tasks[k], ft.class_cells[k], aw.g, sa, lam_sin);
   return hipGetLastError();
 }
 

@@ -149,6 +149,8 @@ struct DescSet {
   int64_t nbc_cells[3] = {0, 0, 0};
   TaskBlock* d_bc_blk[3] = {nullptr, nullptr, nullptr};
   int nbc_blk[3] = {0, 0, 0};
+  std::vector<CopyTask> h_copy;  // host copies of the fill tasks
+  std::vector<BcTask> h_bc[3];
   int64_t last_use = 0;
   int wrap = -1;   // one array over the whole periodic domain (-1 unknown)
   int level = -1;  // one array holding the boxes' bounding box (-1 unknown)
@@ -246,6 +248,9 @@ struct amr_boxtab_s {
   bool batch = false;             // independent problems, one box each
   double* d_box_dt = nullptr;     // per-box time steps of a batch step
   Clock* d_clock = nullptr;       // device time-step control (nbox entries; a level uses the first)
+  double* d_diag = nullptr;       // amr_diagnostics partials (allocated at its first call)
+  int* d_flag = nullptr;          // amr_check_state result
+  bool state_check = false;       // amr_set_state_check: check the state after every step call
   double* d_dt_levels = nullptr;  // amr_new_dt_levels: per-level steps (MAX_DT_LEVELS + 1)
   double* h_box_dt = nullptr;     // pinned staging for d_box_dt
   hipEvent_t dt_copied = nullptr;
@@ -286,6 +291,8 @@ struct amr_boxtab_s {
   AdjWork aw;
   Fab* adj_e_table = nullptr;
   std::unique_ptr<DescSet> adj_g;
+  FoldTasks fold{};                 // transposed fill of adj_g (device task lists)
+  std::vector<void*> fold_mem;
   // domain-wide reverse-mode arrays of a periodic level held as one array
   double* adj_lmem = nullptr;
   Fab* adj_lq_table = nullptr;
@@ -530,6 +537,7 @@ int build_fill(amr_boxtab_s* bt, DescSet* ds) {
   }
   ds->ncopy = int(copies.size());
   ds->ncopy_cells = ncell;
+  ds->h_copy = copies;
   if (!copies.empty()) {
     HIPCHK(hipMalloc(&ds->d_copy, sizeof(CopyTask) * copies.size()));
     HIPCHK(hipMemcpy(ds->d_copy, copies.data(), sizeof(CopyTask) * copies.size(), hipMemcpyHostToDevice));
@@ -592,6 +600,7 @@ int build_fill(amr_boxtab_s* bt, DescSet* ds) {
     }
     ds->nbc[d] = int(tasks.size());
     ds->nbc_cells[d] = nc;
+    ds->h_bc[d] = tasks;
     if (!tasks.empty()) {
       HIPCHK(hipMalloc(&ds->d_bc[d], sizeof(BcTask) * tasks.size()));
       HIPCHK(hipMemcpy(ds->d_bc[d], tasks.data(), sizeof(BcTask) * tasks.size(), hipMemcpyHostToDevice));
@@ -1552,13 +1561,116 @@ int cfl_step(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sc
 
 int nprob_of(const amr_boxtab_s* bt) { return bt->batch ? bt->bt.nbox : 1; }
 
+// Non-physical cells of `state` (NaN or infinite components, rho <= 0 or
+// p <= 0): AMR_ERR_STATE naming the lowest such box.  Synchronous.
+int check_state_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_fab_t* state, int32_t* bad_box) {
+  int rc;
+  DescSet* ds = nullptr;
+  if ((rc = get_desc(bt, state, 0, &ds))) return rc;
+  hipStream_t st = bt->ctx->stream;
+  const int nb = bt->bt.nbox;
+  HIPCHK(hipMemcpyAsync(bt->d_flag, &nb, sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(launch_state_check(bt->bt, ds->d, make_closures(cls), bt->d_flag, st));
+  int first = nb;
+  HIPCHK(hipMemcpyAsync(&first, bt->d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (bad_box) *bad_box = first < nb ? first : -1;
+  if (first < nb)
+    return fail(AMR_ERR_STATE, "non-physical state in box " + std::to_string(first) +
+                                   " (NaN or infinite value, rho <= 0 or p <= 0)");
+  return AMR_OK;
+}
+
+// the check amr_set_state_check asks for, after a step call
+int after_step(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_fab_t* state, int rc) {
+  if (rc || !bt->state_check) return rc;
+  return check_state_impl(bt, cls, state, nullptr);
+}
+
 // ------------------------------------------------------------ adjoint
 constexpr int ADJ_NE = 16;
+
+void free_fold(amr_boxtab_s* bt) {
+  for (void* p : bt->fold_mem) (void)hipFree(p);
+  bt->fold_mem.clear();
+  bt->fold = FoldTasks{};
+}
+
+// The transposed ghost fill of the adjoint workspace without atomics:
+// physical-boundary tasks split by side, one lane per line along the
+// direction; copy tasks greedily coloured so that no two tasks of a class
+// read overlapping source cells of one box (in task order, so the classes
+// and their launch order are fixed).
+int build_fold(amr_boxtab_s* bt, const DescSet* g) {
+  free_fold(bt);
+  FoldTasks& f = bt->fold;
+  auto upload = [&](const void* h, size_t bytes, void** d) -> int {
+    HIPCHK(hipMalloc(d, bytes));
+    bt->fold_mem.push_back(*d);
+    HIPCHK(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
+    return AMR_OK;
+  };
+  int rc;
+  for (int d = 0; d < 3; ++d)
+    for (int side = 0; side < 2; ++side) {
+      std::vector<BcTask> t;
+      int64_t n = 0;
+      for (const BcTask& b : g->h_bc[d]) {
+        if (b.side != side) continue;
+        BcTask c = b;
+        c.start = n;
+        n += cells(b.lo, b.hi) / (b.hi[d] - b.lo[d] + 1);
+        t.push_back(c);
+      }
+      f.nbc[d][side] = int(t.size());
+      f.nlines[d][side] = n;
+      f.bc[d][side] = nullptr;
+      if (!t.empty() && (rc = upload(t.data(), sizeof(BcTask) * t.size(), (void**)&f.bc[d][side]))) return rc;
+    }
+  const std::vector<CopyTask>& cp = g->h_copy;
+  std::vector<int> cls(cp.size(), -1);
+  int ncls = 0;
+  for (size_t i = 0; i < cp.size(); ++i) {
+    unsigned used = 0;
+    for (size_t j = 0; j < i; ++j) {
+      if (cp[j].src != cp[i].src) continue;
+      bool overlap = true;
+      for (int d = 0; d < 3; ++d)
+        if (cp[i].hi[d] - cp[i].sh[d] < cp[j].lo[d] - cp[j].sh[d] ||
+            cp[j].hi[d] - cp[j].sh[d] < cp[i].lo[d] - cp[i].sh[d])
+          overlap = false;
+      if (overlap) used |= 1u << cls[j];
+    }
+    int c = 0;
+    while (c < FOLD_MAX_CLASSES && ((used >> c) & 1u)) ++c;
+    if (c >= FOLD_MAX_CLASSES)
+      return fail(AMR_ERR_ARG, "reverse mode: a cell is the source of more than 32 ghost copies");
+    cls[i] = c;
+    ncls = std::max(ncls, c + 1);
+  }
+  f.nclass = ncls;
+  for (int c = 0; c < ncls; ++c) {
+    std::vector<CopyTask> t;
+    int64_t n = 0;
+    for (size_t i = 0; i < cp.size(); ++i)
+      if (cls[i] == c) {
+        CopyTask x = cp[i];
+        x.start = n;
+        n += cells(x.lo, x.hi);
+        t.push_back(x);
+      }
+    f.class_tasks[c] = int(t.size());
+    f.class_cells[c] = n;
+    if ((rc = upload(t.data(), sizeof(CopyTask) * t.size(), (void**)&f.classes[c]))) return rc;
+  }
+  return AMR_OK;
+}
 
 int ensure_adjoint(amr_boxtab_s* bt, const amr_scheme_t& s) {
   const int visc = s.rhs_visc ? 1 : 0;
   if (bt->adj_mem && bt->adj_visc >= visc) return AMR_OK;
   HIPCHK(hipStreamSynchronize(bt->ctx->stream));
+  free_fold(bt);
   bt->adj_g.reset();
   if (bt->adj_e_table) HIPCHK(hipFree(bt->adj_e_table));
   if (bt->adj_mem) HIPCHK(hipFree(bt->adj_mem));
@@ -1590,6 +1702,7 @@ int ensure_adjoint(amr_boxtab_s* bt, const amr_scheme_t& s) {
   int rc = new_desc(bt, gf.data(), &bt->adj_g);
   if (rc) return rc;
   if ((rc = build_fill(bt, bt->adj_g.get()))) return rc;
+  if ((rc = build_fold(bt, bt->adj_g.get()))) return rc;
   bt->aw.g = bt->adj_g->d;
   if (visc) {
     HIPCHK(hipMalloc(&bt->adj_e_table, sizeof(Fab) * nb));
@@ -1647,8 +1760,7 @@ int adj_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& s
   if ((rc = ensure_unfused(bt, sch))) return rc;
   if ((rc = ensure_adjoint(bt, sch))) return rc;
   const Closures c = make_closures(cls);
-  const DescSet* g = bt->adj_g.get();
-  const FoldTasks ft{g->d_copy, g->ncopy, g->ncopy_cells, g->d_bc, g->nbc, g->nbc_cells};
+  const FoldTasks& ft = bt->fold;
   const WrapArgs wa = wrap_args(bt, sin);
   // the domain-wide path needs lam_out as one periodic array as well
   const bool level = wa.on && wrap_ok(bt, lam_out);
@@ -1663,21 +1775,32 @@ int adj_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& s
 
 // Whether the storage spans of two FAB sets overlap.
 bool fabs_overlap(const amr_fab_t* a, const amr_fab_t* b, int nb) {
-  auto span = [nb](const amr_fab_t* f, intptr_t& lo, intptr_t& hi) {
-    lo = INTPTR_MAX;
-    hi = 0;
+  // per-box address spans (box arrays of two sets may interleave in
+  // memory, so one span per set would report overlaps that do not exist);
+  // sorted by start, an overlap between the sets is a span of one set that
+  // starts before the furthest end reached by the other set's earlier spans
+  struct Span {
+    intptr_t lo, hi;
+    int set;
+  };
+  std::vector<Span> v;
+  v.reserve(2 * size_t(nb));
+  for (int set = 0; set < 2; ++set) {
+    const amr_fab_t* f = set == 0 ? a : b;
     for (int i = 0; i < nb; ++i) {
       int64_t nx = f[i].hi[0] - f[i].lo[0] + 1 + 2 * f[i].ng, ny = f[i].hi[1] - f[i].lo[1] + 1 + 2 * f[i].ng;
       int64_t nz = f[i].hi[2] - f[i].lo[2] + 1 + 2 * f[i].ng;
       int64_t last = (nx - 1) + (ny - 1) * f[i].jstride + (nz - 1) * f[i].kstride + (NCONS - 1) * f[i].nstride;
-      lo = std::min(lo, intptr_t(f[i].data));
-      hi = std::max(hi, intptr_t(f[i].data + last + 1));
+      v.push_back(Span{intptr_t(f[i].data), intptr_t(f[i].data + last + 1), set});
     }
-  };
-  intptr_t alo, ahi, blo, bhi;
-  span(a, alo, ahi);
-  span(b, blo, bhi);
-  return alo < bhi && blo < ahi;
+  }
+  std::sort(v.begin(), v.end(), [](const Span& x, const Span& y) { return x.lo < y.lo; });
+  intptr_t reach[2] = {INTPTR_MIN, INTPTR_MIN};
+  for (const Span& x : v) {
+    if (x.lo < reach[1 - x.set]) return true;
+    reach[x.set] = std::max(reach[x.set], x.hi);
+  }
+  return false;
 }
 
 int check_adjoint(const amr_scheme_t& s) {
@@ -1920,7 +2043,7 @@ int amr_batch_advance(amr_boxtab_t bt, const amr_closures_t* cls, const amr_sche
   std::memcpy(bt->h_box_dt, dt, sizeof(double) * nb);
   HIPCHK(hipMemcpyAsync(bt->d_box_dt, bt->h_box_dt, sizeof(double) * nb, hipMemcpyHostToDevice, st));
   HIPCHK(hipEventRecord(bt->dt_copied, st));
-  return advance_impl(bt, *cls, *sch, state, 1.0, bt->d_box_dt);
+  return after_step(bt, *cls, state, advance_impl(bt, *cls, *sch, state, 1.0, bt->d_box_dt));
 }
 
 int amr_batch_est_dt(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, double cfl,
@@ -1965,6 +2088,7 @@ int finish_boxtab(amr_boxtab_s* bt) {
   if (e == hipSuccess) e = hipHostMalloc(&bt->h_box_dt, sizeof(double) * nbox);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->dt_copied, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&bt->d_clock, sizeof(Clock) * nbox);
+  if (e == hipSuccess) e = hipMalloc(&bt->d_flag, sizeof(int) * 4);
   if (e == hipSuccess) e = hipMalloc(&bt->d_dt_levels, sizeof(double) * (MAX_DT_LEVELS + 1));
   if (e == hipSuccess) {
     std::vector<Clock> k(nbox, Clock{0.0, 0.0, 0, 1, 0, 0});
@@ -1999,6 +2123,7 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   if (bt->d_wall_blk) (void)hipFree(bt->d_wall_blk);
   bt->run.reset();
   bt->src.reset();
+  free_fold(bt);
   bt->adj_g.reset();
   if (bt->adj_e_table) (void)hipFree(bt->adj_e_table);
   if (bt->adj_mem) (void)hipFree(bt->adj_mem);
@@ -2012,6 +2137,8 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   if (bt->h_result) (void)hipHostFree(bt->h_result);
   if (bt->d_box_dt) (void)hipFree(bt->d_box_dt);
   if (bt->d_clock) (void)hipFree(bt->d_clock);
+  if (bt->d_diag) (void)hipFree(bt->d_diag);
+  if (bt->d_flag) (void)hipFree(bt->d_flag);
   if (bt->d_dt_levels) (void)hipFree(bt->d_dt_levels);
   if (bt->h_box_dt) (void)hipHostFree(bt->h_box_dt);
   if (bt->dt_copied) (void)hipEventDestroy(bt->dt_copied);
@@ -2328,7 +2455,7 @@ int amr_advance(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* 
   if (!bt || !cls || !sch || !state) return fail(AMR_ERR_ARG, "null argument");
   int rc;
   if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls))) return rc;
-  return advance_impl(bt, *cls, *sch, state, dt);
+  return after_step(bt, *cls, state, advance_impl(bt, *cls, *sch, state, dt));
 }
 
 int amr_advance_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
@@ -2419,7 +2546,7 @@ int amr_advance_cfl(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme
   if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls))) return rc;
   for (int s = 0; s < nsteps; ++s)
     if ((rc = cfl_step(bt, *cls, *sch, state, cfl, stop_time))) return rc;
-  return AMR_OK;
+  return after_step(bt, *cls, state, AMR_OK);
 }
 
 int amr_advance_cfl_graph(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch,
@@ -2728,6 +2855,48 @@ int amr_est_dt_noalloc(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab
   if (!(dt > 0.0) || std::isnan(dt))
     return fail(AMR_ERR_STATE, "non-physical state: CFL time step is not positive (negative pressure or NaN)");
   *dt_out = dt * cfl;  // src/CNS.cpp:391
+  return AMR_OK;
+}
+
+int amr_check_state(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, int32_t* bad_box) {
+  if (!bt || !cls || !state) return fail(AMR_ERR_ARG, "null argument");
+  int rc = check_closures(*cls);
+  if (rc) return rc;
+  return check_state_impl(bt, *cls, state, bad_box);
+}
+
+int amr_set_state_check(amr_boxtab_t bt, int on) {
+  if (!bt) return fail(AMR_ERR_ARG, "null box table");
+  bt->state_check = on != 0;
+  return AMR_OK;
+}
+
+int amr_diagnostics(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, double dt, double totals[5],
+                    double prim_min[6], double prim_max[6], double wave_max[6], double cfl[3]) {
+  if (!bt || !cls || !state) return fail(AMR_ERR_ARG, "null argument");
+  int rc = check_closures(*cls);
+  if (rc) return rc;
+  DescSet* ds = nullptr;
+  if ((rc = get_desc(bt, state, 0, &ds))) return rc;
+  hipStream_t st = bt->ctx->stream;
+  if (!bt->d_diag) HIPCHK(hipMalloc(&bt->d_diag, sizeof(double) * (21 * 1024 + 32)));
+  double* res = bt->d_diag + 21 * 1024;
+  HIPCHK(launch_diag(bt->bt, ds->d, make_closures(*cls), dt, bt->d_diag, res, st));
+  double h[21];
+  HIPCHK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int m = 0; m < 6; ++m) {
+    if (prim_min) prim_min[m] = h[m];
+    if (prim_max) prim_max[m] = h[6 + m];
+  }
+  for (int d = 0; d < 3; ++d) {
+    if (wave_max) {
+      wave_max[2 * d] = h[12 + 3 * d];
+      wave_max[2 * d + 1] = h[13 + 3 * d];
+    }
+    if (cfl) cfl[d] = h[14 + 3 * d];
+  }
+  if (totals) return amr_totals(bt, state, totals);
   return AMR_OK;
 }
 

@@ -501,6 +501,108 @@ hipError_t launch_dt_levels(const DtLevels& a, double* dt_level, hipStream_t s) 
   return hipGetLastError();
 }
 
+// printTotal's primitive extrema and pointCFL's wave speeds
+// (src/CNS.cpp:544-547, src/cls/CNS_K.h:49-59) over the valid cells: per
+// block 6 minima and 6 maxima of (rho, u, v, w, T, p) from cons2prim, then
+// per direction max |u_d + c| and max |u_d - c| with c = sqrt(gamma R T),
+// and max over cells of max(|u_d + c|, |u_d - c|) * dt / dx_d (the box's
+// dx, so a batch's problems each use their own).  partial: 21 per block.
+constexpr int NDIAG = 21;
+__global__ void __launch_bounds__(RBLK) k_diag(const BoxDev* boxes, int nbox, const Fab* st, Closures c, double dt,
+                                               double* partial) {
+  __shared__ double sm[RBLK];
+  double v[NDIAG];
+  for (int q = 0; q < 6; ++q) {
+    v[q] = 1.79769313486231570815e+308;
+    v[6 + q] = -1.79769313486231570815e+308;
+  }
+  for (int q = 12; q < NDIAG; ++q) v[q] = 0.0;
+  for (int b = 0; b < nbox; ++b) {
+    const BoxDev bx = boxes[b];
+    const Fab u = st[b];
+    int nx = bx.hi[0] - bx.lo[0] + 1, ny = bx.hi[1] - bx.lo[1] + 1, nz = bx.hi[2] - bx.lo[2] + 1;
+    int64_t n = int64_t(nx) * ny * nz;
+    for (int64_t t = int64_t(blockIdx.x) * RBLK + threadIdx.x; t < n; t += int64_t(gridDim.x) * RBLK) {
+      int i = bx.lo[0] + int(t % nx);
+      int64_t r = t / nx;
+      int j = bx.lo[1] + int(r % ny);
+      int k = bx.lo[2] + int(r / ny);
+      double q[NPRIM];
+      cons2prim(u(i, j, k, 0), u(i, j, k, 1), u(i, j, k, 2), u(i, j, k, 3), u(i, j, k, 4), c, q);
+      for (int m = 0; m < 6; ++m) {
+        v[m] = dmin(v[m], q[m]);
+        v[6 + m] = dmax(v[6 + m], q[m]);
+      }
+      const double sos = sqrt(c.gamma * c.Rspec * q[QT]);
+      for (int d = 0; d < 3; ++d) {
+        const double a = fabs(q[QU + d] + sos), bm = fabs(q[QU + d] - sos);
+        v[12 + 3 * d] = dmax(v[12 + 3 * d], a);
+        v[13 + 3 * d] = dmax(v[13 + 3 * d], bm);
+        v[14 + 3 * d] = dmax(v[14 + 3 * d], dmax(a, bm) * dt / bx.dx[d]);
+      }
+    }
+  }
+  for (int q = 0; q < NDIAG; ++q) {
+    sm[threadIdx.x] = v[q];
+    __syncthreads();
+    for (int s = RBLK / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s)
+        sm[threadIdx.x] = q < 6 ? dmin(sm[threadIdx.x], sm[threadIdx.x + s]) : dmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[size_t(NDIAG) * blockIdx.x + q] = sm[0];
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(RBLK) k_diag_final(const double* partial, int n, double* result) {
+  const int q = threadIdx.x;
+  if (q >= NDIAG) return;
+  double v = partial[q];
+  for (int t = 1; t < n; ++t) v = q < 6 ? dmin(v, partial[size_t(NDIAG) * t + q]) : dmax(v, partial[size_t(NDIAG) * t + q]);
+  result[q] = v;
+}
+
+hipError_t launch_diag(const BoxTable& bt, const Fab* st, const Closures& c, double dt, double* partial,
+                       double* result, hipStream_t s) {
+  const int nb = reduce_blocks(bt);
+  hipLaunchKernelGGL(k_diag, dim3(nb), dim3(RBLK), 0, s, bt.d_boxes, bt.nbox, st, c, dt, partial);
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s, partial, nb, result);
+  return hipGetLastError();
+}
+
+// Non-physical states: flag[0] = lowest box with a valid cell whose
+// component is NaN or infinite, whose density is <= 0 or whose pressure
+// (gamma - 1)(E - |m|^2 / (2 rho)) is <= 0 (flag[0] = nbox when none).
+__global__ void __launch_bounds__(RBLK) k_state_check(const BoxDev* boxes, const Fab* st, Closures c, int* flag) {
+  const int b = blockIdx.y;
+  const BoxDev bx = boxes[b];
+  const Fab u = st[b];
+  int nx = bx.hi[0] - bx.lo[0] + 1, ny = bx.hi[1] - bx.lo[1] + 1, nz = bx.hi[2] - bx.lo[2] + 1;
+  int64_t n = int64_t(nx) * ny * nz;
+  bool bad = false;
+  for (int64_t t = int64_t(blockIdx.x) * RBLK + threadIdx.x; t < n && !bad; t += int64_t(gridDim.x) * RBLK) {
+    int i = bx.lo[0] + int(t % nx);
+    int64_t r = t / nx;
+    int j = bx.lo[1] + int(r % ny);
+    int k = bx.lo[2] + int(r / ny);
+    double w[5];
+    for (int m = 0; m < 5; ++m) {
+      w[m] = u(i, j, k, m);
+      if (!isfinite(w[m])) bad = true;
+    }
+    const double p = c.gm1 * (w[4] - 0.5 * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]) / w[0]);
+    if (!(w[0] > 0.0) || !(p > 0.0)) bad = true;
+  }
+  if (bad) atomicMin(flag, b);
+}
+
+hipError_t launch_state_check(const BoxTable& bt, const Fab* st, const Closures& c, int* flag, hipStream_t s) {
+  const int nper = estdt_box_blocks(bt);
+  hipLaunchKernelGGL(k_state_check, dim3(nper, bt.nbox), dim3(RBLK), 0, s, bt.d_boxes, st, c, flag);
+  return hipGetLastError();
+}
+
 int estdt_box_blocks(const BoxTable& bt) {
   int64_t mx = 1;
   for (auto& b : bt.h_boxes)

@@ -188,14 +188,20 @@ struct AdjWork {
   Fab* le = nullptr;
 };
 bool adjoint_supported(const amr_scheme_t& s);
-// Ghost-fill task lists of the adjoint workspace g.
+// Transposed ghost fill of the adjoint workspace g (host-side view of
+// device task lists): physical-boundary tasks per direction and side, their
+// `start` counting lines along the direction; same-level copies in colour
+// classes whose source regions do not overlap, `start` counting cells
+// within the class.
+constexpr int FOLD_MAX_CLASSES = 32;
 struct FoldTasks {
-  const CopyTask* copies;
-  int ncopy;
-  int64_t ncopy_cells;
-  BcTask* const* bc;
-  const int* nbc;
-  const int64_t* nbc_cells;
+  BcTask* bc[3][2];
+  int nbc[3][2];
+  int64_t nlines[3][2];
+  CopyTask* classes[FOLD_MAX_CLASSES];
+  int class_tasks[FOLD_MAX_CLASSES];
+  int64_t class_cells[FOLD_MAX_CLASSES];
+  int nclass;
 };
 // Reverse of one stage (modes as amr_rk_stage) at the ghost-filled input
 // sin: lam_sin (=|+=) stage terms of lam_out through the RHS transpose, the
@@ -301,6 +307,13 @@ hipError_t noalloc_scratch_address(void** ptr);
 // whose address *result_out receives
 hipError_t launch_estdt_scratch(const BoxTable& bt, const Closures& c, void* scratch, double** result_out,
                                 hipStream_t s);
+// printTotal diagnostics: result[0..5] primitive minima, [6..11] maxima,
+// [12 + 3d] max |u_d + c|, [13 + 3d] max |u_d - c|, [14 + 3d] CFL number;
+// partial holds 21 doubles per reduction block.
+hipError_t launch_diag(const BoxTable& bt, const Fab* st, const Closures& c, double dt, double* partial,
+                       double* result, hipStream_t s);
+// flag[0] (device int, preset to nbox) := lowest box holding a non-physical cell
+hipError_t launch_state_check(const BoxTable& bt, const Fab* st, const Closures& c, int* flag, hipStream_t s);
 hipError_t launch_totals(const BoxTable& bt, const Fab* st, double* partial, double* result,
                          hipStream_t s);
 

@@ -361,6 +361,26 @@ class Level:
         call("amr_est_dt", self.h, ctypes.byref(self.closures), S.fabs, cfl, ctypes.byref(out))
         return out.value
 
+    def diagnostics(self, S, dt=0.0):
+        """CNS::printTotal's figures: dict of totals, primitive min/max
+        (rho, u, v, w, T, p), wave speeds max |u_d +- c| and CFL numbers."""
+        D = ctypes.c_double
+        tot, pmin, pmax, wav, cfl = (D * 5)(), (D * 6)(), (D * 6)(), (D * 6)(), (D * 3)()
+        call("amr_diagnostics", self.h, ctypes.byref(self.closures), S.fabs, dt, tot, pmin, pmax, wav, cfl)
+        return {"totals": list(tot), "prim_min": list(pmin), "prim_max": list(pmax),
+                "wave_max": [(wav[2 * d], wav[2 * d + 1]) for d in range(3)], "cfl": list(cfl)}
+
+    def check_state(self, S):
+        """Lowest box with a non-physical cell, or -1 (raises on other errors)."""
+        b = ctypes.c_int32(-1)
+        rc = capi.load().amr_check_state(self.h, ctypes.byref(self.closures), S.fabs, ctypes.byref(b))
+        if rc not in (0, capi.AMR_ERR_STATE):
+            raise capi.AmrError(capi.load().amr_last_error().decode())
+        return b.value
+
+    def set_state_check(self, on=True):
+        call("amr_set_state_check", self.h, 1 if on else 0)
+
     def totals(self, S):
         out = (ctypes.c_double * 5)()
         call("amr_totals", self.h, S.fabs, out)

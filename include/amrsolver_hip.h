@@ -345,8 +345,8 @@ int amr_set_problem(amr_boxtab_t bt, const amr_closures_t* cls, const amr_proble
  * (flux_euler = 1, orders 2/4/6) and viscous right-hand sides of
  * CNS::compute_rhs (src/tim/CNS_advance.cpp:186-427) and every stage scheme
  * of CNS::advance.  Adjoint arrays hold NCONS components on the valid cells
- * (any ghost width).  Ghost-fill transposes accumulate with atomics, so
- * results may differ in the last bits between runs. */
+ * (any ghost width).  The ghost-fill transposes add in a fixed order with no
+ * atomics, so every run gives the same bits. */
 
 /* grad = (dL/dU)^T lam for the map U (valid cells) -> L(fill(U)) of amr_rhs.
  * `state` must be ghost-filled (ng >= 3). */
@@ -379,6 +379,28 @@ int amr_vjp_stage(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t
 int amr_adjoint_run(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state0,
                     double dt, int nsteps, int kcheck, int objective, const amr_fab_t* weights,
                     const amr_fab_t* grad, double* J_out);
+
+/* The rest of CNS::printTotal (src/CNS.cpp:532-621) for `state` (valid
+ * cells): primitive minima and maxima in the order rho, u, v, w, T, p
+ * (cons2prim of `state`; the reference reads the primitives of the step's
+ * last stage input), per direction d wave_max[2d] = max |u_d + c| and
+ * wave_max[2d+1] = max |u_d - c| with c = sqrt(gamma R T), and cfl[d] =
+ * max(|u_d + c|, |u_d - c|) * dt / dx_d over the cells (pointCFL,
+ * src/cls/CNS_K.h:49-59), plus the totals of amr_totals.  Any output may be
+ * NULL.  Synchronous. */
+int amr_diagnostics(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, double dt, double totals[5],
+                    double prim_min[6], double prim_max[6], double wave_max[6], double cfl[3]);
+
+/* AMR_ERR_STATE (and *bad_box = the lowest such box, else -1) when a valid
+ * cell of `state` holds a NaN or infinite value, rho <= 0 or a pressure
+ * (gamma - 1)(E - |m|^2/(2 rho)) <= 0.  No reference counterpart (the
+ * reference clamps silently).  Synchronous. */
+int amr_check_state(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t* state, int32_t* bad_box);
+
+/* on != 0: amr_advance, amr_batch_advance and amr_advance_cfl run
+ * amr_check_state on the state after their steps (making them
+ * synchronous) and return its AMR_ERR_STATE.  Off by default. */
+int amr_set_state_check(amr_boxtab_t bt, int on);
 
 /* Conserved totals sum(U_n) * ProbSize (CNS::printTotal, src/CNS.cpp:537-542).
  * Synchronous. */

@@ -1681,7 +1681,8 @@ int orc_box_prims(const orc_closures* c, const int lo[3], const int hi[3], int n
   Box gb = grow(bx, ng);
   A4 u = view(const_cast<double*>(state), gb.lo, gb.hi);
   A4 q = view(prims, gb.lo, gb.hi);
-  for_box(grow(bx, NGHOST), [&](int i, int j, int k) { cons2prim(i, j, k, u, q, *c); });
+  // every cell of the arrays the caller passed (the box grown by ng)
+  for_box(gb, [&](int i, int j, int k) { cons2prim(i, j, k, u, q, *c); });
   return 0;
 }
 
