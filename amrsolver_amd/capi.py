@@ -105,6 +105,8 @@ SIGNATURES = {
                         ctypes.c_int),
     "amr_check_state": ([_P, ctypes.POINTER(Closures), _PF, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     "amr_set_state_check": ([_P, ctypes.c_int], ctypes.c_int),
+    "amr_set_tiling_form": ([_P, ctypes.POINTER(Scheme), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+                            ctypes.c_int),
     "amr_totals": ([_P, _PF, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "amr_batch_create": ([_P, ctypes.c_int, ctypes.POINTER(Geom), ctypes.POINTER(_P)], ctypes.c_int),
     "amr_batch_advance": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF,

@@ -1391,10 +1391,13 @@ int alloc_mirror(amr_boxtab_s* bt, const amr_fab_t* state, double** mem, std::ve
   for (int b = 1; b < nb && shared; ++b) shared = mapping_of(state[b]) == m0;
   bool mirror = shared || span <= sum + sum / 2;
   if (mirror) {
-    HIPCHK(hipMalloc(mem, sizeof(double) * span));
+    // the mirror keeps each element's offset within 128-byte lines (a
+    // padded level array's rows stay line-aligned in the workspaces)
+    const int64_t shift = int64_t(lo & 127) / 8;
+    HIPCHK(hipMalloc(mem, sizeof(double) * (span + shift)));
     for (int b = 0; b < nb; ++b) {
       fabs[b] = state[b];
-      fabs[b].data = *mem + (intptr_t(state[b].data) - lo) / 8;
+      fabs[b].data = *mem + shift + (intptr_t(state[b].data) - lo) / 8;
     }
   } else {
     HIPCHK(hipMalloc(mem, sizeof(double) * sum));
@@ -2665,6 +2668,19 @@ int amr_set_tiling(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, int t
   return AMR_OK;
 }
 
+int amr_set_tiling_form(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, int tz, int form) {
+  if (!ctx || !sch) return fail(AMR_ERR_ARG, "null argument");
+  if (form == 0) return amr_set_tiling(ctx, sch, tx, ty, tz);
+  if (form != 1) return fail(AMR_ERR_ARG, "form must be 0 or 1");
+  if (tz <= 0) return fail(AMR_ERR_ARG, "tz must be positive");
+  if (!fused_self_supported(*sch, tx, ty))
+    return fail(AMR_ERR_ARG, "the self-loading form takes 16x16 tiles of a viscous KEEP 2/4 scheme");
+  FusedConfig f{tx, ty, tz};
+  f.form = form;
+  ctx->tuned[scheme_key(*sch)] = f;
+  return AMR_OK;
+}
+
 int amr_set_box_tiling(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, int tz) {
   if (!ctx || !sch) return fail(AMR_ERR_ARG, "null argument");
   if (tz <= 0) return fail(AMR_ERR_ARG, "tz must be positive");
@@ -2674,7 +2690,7 @@ int amr_set_box_tiling(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, i
   return AMR_OK;
 }
 
-int amr_step_plan(amr_boxtab_t bt, const amr_scheme_t* sch, const amr_fab_t* state, int32_t info[5]) {
+int amr_step_plan(amr_boxtab_t bt, const amr_scheme_t* sch, const amr_fab_t* state, int32_t info[6]) {
   if (!bt || !sch || !state || !info) return fail(AMR_ERR_ARG, "null argument");
   int rc;
   if ((rc = check_scheme(*sch))) return rc;
@@ -2691,6 +2707,7 @@ int amr_step_plan(amr_boxtab_t bt, const amr_scheme_t* sch, const amr_fab_t* sta
   info[2] = fused ? cfg.tz : 0;
   info[3] = fused ? (level ? 2 : 1) : 0;
   info[4] = 0;
+  info[5] = fused ? cfg.form : 0;
   int nst = 0;
   StageCoef sc;
   if ((rc = stage_coeffs(sch->order_rk, sch->stages_rk, 1.0, 0, &nst, &sc))) return rc;
@@ -2726,7 +2743,13 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
       for (int tz : tzs) {
         FusedConfig c{sh[0], sh[1], tz, box_tiles};
         if (tz > 2 * (box_tiles ? maxz : maxz_level) && tz != 16) continue;
-        if (fused_supported(*sch, c.tx, c.ty) && fused_fits(bt, c, !box_tiles)) cand.push_back(c);
+        if (!(fused_supported(*sch, c.tx, c.ty) && fused_fits(bt, c, !box_tiles))) continue;
+        cand.push_back(c);
+        // the self-loading form of the same tiling
+        if (fused_self_supported(*sch, c.tx, c.ty)) {
+          c.form = 1;
+          cand.push_back(c);
+        }
       }
   if (cand.empty()) {
     bt->tuned[key] = FusedConfig{-1, 0, 0};

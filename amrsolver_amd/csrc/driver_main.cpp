@@ -195,7 +195,7 @@ struct Hierarchy {
       return;
     }
     if (ngrow[l] == 0) {  // the fused stage kernel reads two ghost layers, the others three
-      int32_t info[5];
+      int32_t info[6];
       check(amr_step_plan(lev[l].bt, &sch, S.fabs.data(), info), "amr_step_plan");
       ngrow[l] = info[3] != 0 ? 2 : 3;
     }

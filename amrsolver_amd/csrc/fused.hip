@@ -18,56 +18,13 @@
 // HBM traffic per cell and stage: the input state (plus tile halos, mostly
 // served from L2), U^n when the stage needs it, and the output.
 #include "kernels.h"
+#include "fused_common.h"
 
 namespace amr {
 
 namespace {
 
-// Global-address-space views: loads and stores then compile to global_*
-// instructions (flat ones would share the LDS wait counter and serialise
-// the plane prefetch against every LDS wait).
-typedef __attribute__((address_space(1))) double gdouble;
-typedef __attribute__((address_space(1))) const unsigned guint;
-__device__ __forceinline__ gdouble* gptr(double* p) { return (gdouble*)p; }
-
-struct VDer {
-  double ux, uy, uz, T;
-  double dudx, dvdx, dwdx, dTdx, dudy, dvdy, dwdy, dTdy, dudz, dvdz, dwdz, dTdz;
-};
-
-// ViscousFluxes (CNS_hydro_K.h:133-171), components UMX..UET (URHO is 0)
-template <bool WX, bool WY, bool WZ>
-__device__ __forceinline__ void visc_flux(const VDer& d, const Closures& c, double fx[4], double fy[4],
-                                          double fz[4]) {
-  double div = d.dudx + d.dvdy + d.dwdz;
-  double mu = visc(c, d.T);
-  double lambda = cond(c, d.T, mu);
-  double r1_3 = 1.0 / 3.0;
-  double tauxx = 2.0 * mu * (d.dudx - r1_3 * div);
-  double tauxy = mu * (d.dudy + d.dvdx);
-  double tauxz = mu * (d.dudz + d.dwdx);
-  double tauyy = 2.0 * mu * (d.dvdy - r1_3 * div);
-  double tauyz = mu * (d.dvdz + d.dwdy);
-  double tauzz = 2.0 * mu * (d.dwdz - r1_3 * div);
-  if (WX) {
-    fx[0] = -tauxx;
-    fx[1] = -tauxy;
-    fx[2] = -tauxz;
-    fx[3] = -lambda * d.dTdx - tauxx * d.ux - tauxy * d.uy - tauxz * d.uz;
-  }
-  if (WY) {
-    fy[0] = -tauxy;
-    fy[1] = -tauyy;
-    fy[2] = -tauyz;
-    fy[3] = -lambda * d.dTdy - tauxy * d.ux - tauyy * d.uy - tauyz * d.uz;
-  }
-  if (WZ) {
-    fz[0] = -tauxz;
-    fz[1] = -tauyz;
-    fz[2] = -tauzz;
-    fz[3] = -lambda * d.dTdz - tauxz * d.ux - tauyz * d.uy - tauzz * d.uz;
-  }
-}
+using namespace fusedk;
 
 template <int H, bool VISC, int TX, int TY>
 struct Cfg {
@@ -80,56 +37,75 @@ struct Cfg {
   static constexpr int PL = NREG;  // LDS plane stride per primitive
   static constexpr int NLOAD = (NREG + NT - 1) / NT;
   static constexpr int NRING = 2 * TX + 2 * TY;
-  // threads of the extra x faces (i = TX) and y faces (j = TY): waves of
-  // their own (the ring fluxes occupy wave 0) where the workgroup has them
+  // 16x16 viscous tiles (four waves) spread the work beyond the own cells
+  // over the waves: the ring's viscous fluxes on wave 3 (one pass for both
+  // directions), the KEEP part of the extra faces at i = TX and j = TY on
+  // wave 0 before the viscous barrier (one 32-lane pass for both
+  // directions), their viscous part on wave 1 after it
+  static constexpr bool EARLY = VISC && NT == 256 && NRING == 64;
+  static constexpr int VA = 64;  // first thread of the extra faces' viscous part
+  // other shapes: threads of the extra x faces (i = TX) and y faces (j = TY),
+  // waves of their own (the ring fluxes occupy wave 0) where the workgroup
+  // has them
   static constexpr int XB = NT >= 128 ? 64 * ((NRING + 63) / 64) : (VISC ? NRING : 0);
   static constexpr int YB = NT >= XB + 128 ? XB + 64 : XB + TY;
-  // first ring thread: the last wave where it holds the whole ring and the
-  // extra faces sit on waves 1 and 2, else thread 0
-  static constexpr int RB = (NT >= 256 && NRING <= 64 && XB == 64) ? NT - 64 : 0;
-  // the KEEP part of the extra faces is computed before the viscous barrier
-  // (balancing the phases) where the extra-face and ring waves differ
-  static constexpr bool EARLY = VISC && RB > 0;
+  // first ring thread
+  static constexpr int RB = EARLY ? NT - 64 : 0;
+  // halo cells that are not ring cells (outer ring and the ring's corners)
+  static constexpr int NOUT = NREG - NT - NRING;
 };
 
-// KEEP face flux in direction NRM at the face below LDS cell `idx`; cells
-// idx + m*st, m = -H..H-1 (CentralKEEP.h:58-142 accumulation order).
-template <int H, int NRM, int PL>
-__device__ __forceinline__ void keep_face(const double* sq, int idx, int st, const double co[3], double cv,
-                                          double f[5]) {
-#pragma unroll
-  for (int n = 0; n < 5; ++n) f[n] = 0.0;
-#pragma unroll
-  for (int l = 1; l <= H; ++l) {
-#pragma unroll
-    for (int m = 0; m <= l - 1; ++m) {
-      const int a = idx + m * st, b = idx + (m - l) * st;
-      keep_pair<NRM>(co[l - 1], sq[QRHO * PL + a], sq[QU * PL + a], sq[QV * PL + a], sq[QW * PL + a],
-                     cv * sq[QT * PL + a], sq[QPRES * PL + a], sq[QRHO * PL + b], sq[QU * PL + b],
-                     sq[QV * PL + b], sq[QW * PL + b], cv * sq[QT * PL + b], sq[QPRES * PL + b], f);
-    }
+// LDS cell of a thread's second load slot in the self-loading form (16x16
+// viscous tiles): the ring threads load their ring cell, threads 64 ..
+// 64 + NOUT - 1 (waves 1 and 2) the other halo cells (rows 0 and RY-1,
+// columns 0 and RX-1, then the ring's four corners); -1 for threads with no
+// second cell
+template <class C, int TX>
+__device__ __forceinline__ int self_halo_cell(int tid, int ring_idx) {
+  constexpr int RX = C::RX, RY = C::RY;
+  constexpr int OB = 64;  // first thread of the other halo cells
+  static_assert(C::EARLY && C::RB + C::NRING == C::NT && OB + C::NOUT <= C::RB,
+                "self-loading layout needs the ring on the last wave");
+  if (tid >= C::RB) return ring_idx;
+  if (tid < OB || tid >= OB + C::NOUT) return -1;
+  const int o = tid - OB;
+  int i, j;
+  if (o < RX) { i = o; j = 0; }
+  else if (o < 2 * RX) { i = o - RX; j = RY - 1; }
+  else if (o < 2 * RX + RY - 2) { i = 0; j = o - 2 * RX + 1; }
+  else if (o < 2 * RX + 2 * (RY - 2)) { i = RX - 1; j = o - 2 * RX - (RY - 2) + 1; }
+  else {
+    const int q = o - 2 * RX - 2 * (RY - 2);
+    i = (q & 1) ? RX - 2 : 1;
+    j = (q & 2) ? RY - 2 : 1;
   }
+  return j * RX + i;
 }
 
-// Terms of the z-direction KEEP pair (cell 1 = q1, cell 2 = q2).
-__device__ __forceinline__ void zterms(const double q1[NPRIM], const double q2[NPRIM], double cv, double t[5]) {
-  keep_terms<2>(q1[QRHO], q1[QU], q1[QV], q1[QW], cv * q1[QT], q1[QPRES], q2[QRHO], q2[QU], q2[QV], q2[QW],
-                cv * q2[QT], q2[QPRES], t);
-}
-
-template <int H, bool VISC, int TX, int TY, bool UCS>
-__global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ tiles, const BoxDev* __restrict__ boxes,
+// SELF: the self-loading form (every cell a thread reads from the plane
+// above its own is one it stored, so storing that plane needs no barrier;
+// 16x16 viscous tiles).  SRC: the stage may carry a user_source array or
+// a body force (the code for them is compiled out otherwise).
+// At least two waves per SIMD: the compiler would otherwise take registers
+// past 256 (into AGPRs) rather than reuse them, halving the occupancy.
+template <int H, bool VISC, int TX, int TY, bool UCS, bool SELF, bool SRC>
+__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) k_fused(const TileDev* __restrict__ tiles, const BoxDev* __restrict__ boxes,
                                                      Closures c, double co0, double co1, double co2, StageArgs sa,
                                                      WrapArgs wa, Cover cov, UncovSrc us, const Fab* __restrict__ s1f,
                                                      const Fab* __restrict__ sinf, const Fab* __restrict__ outf) {
   using C = Cfg<H, VISC, TX, TY>;
   constexpr int RX = C::RX, HX = C::HX, PL = C::PL, NT = C::NT;
+  static_assert(!SELF || (VISC && TX == 16 && TY == 16 && C::NLOAD == 2 && C::EARLY),
+                "the self-loading form takes 16x16 viscous tiles");
   __shared__ double s_q[2][NPRIM * PL];
   __shared__ double s_px[VISC ? 4 : 1][TY][TX + 2];
   __shared__ double s_py[VISC ? 4 : 1][TY + 2][TX];
   __shared__ double s_fx[5][TY][TX + 1];
   __shared__ double s_fy[5][TY + 1][TX];
   __shared__ unsigned s_um[UCS ? C::NLOAD * NT : 1];
+  // KEEP part of the extra faces, across barrier (c) (self-loading form:
+  // s_fx/s_fy column TX / row TY may still be read for the plane below)
+  __shared__ double s_fe[SELF ? 5 : 1][SELF ? TX + TY : 1];
 
   const TileDev T = tiles[blockIdx.x];
   if (T.nz <= 0 || !box_active(sa, T.box)) return;  // uniform: the whole workgroup leaves
@@ -142,8 +118,8 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
   // loads): every iteration then issues the same loads in the same order,
   // and the compiler's load counters never wait on the plane prefetch
   const Fab S1 = use_s1 ? s1f[T.box] : S;
-  const bool has_src = sa.src != nullptr;
-  const Fab SRC = has_src ? sa.src[T.box] : S;  // user_source FAB (valid cells)
+  const bool has_src = SRC && sa.src != nullptr;
+  const Fab SR = has_src ? sa.src[T.box] : S;  // user_source FAB (valid cells)
   const int tid = threadIdx.x;
   const int ti = tid % TX, tj = tid / TX;
   const int gi = T.i0 + ti, gj = T.j0 + tj;
@@ -177,13 +153,25 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
     ring_idx = (ring_b + HX) * RX + (ring_a + HX);
   }
 
+  // LDS cell of each load slot (-1: loaded, not stored).  First form: slot
+  // s holds cell tid + s*NT.  Self-loading form: slot 0 is the own cell,
+  // slot 1 the cell self_halo_cell gives.
+  int scell[C::NLOAD];
+#pragma unroll
+  for (int s = 0; s < C::NLOAD; ++s) {
+    if constexpr (SELF) scell[s] = s == 0 ? own : self_halo_cell<C, TX>(tid, ring_idx);
+    else scell[s] = tid + s * NT < C::NREG ? tid + s * NT : -1;
+  }
+  // cell a slot loads: lanes with nothing to store load a region cell
+  auto load_cell = [&](int s) { return scell[s] >= 0 ? scell[s] : (SELF ? own : C::NREG - 1); };
+
   // per-lane element offsets of the loaded plane cells; plane bases are
   // wave-uniform
   const gdouble* sbase = gptr(S.p) + S.off;
   int loff[C::NLOAD];
 #pragma unroll
   for (int s = 0; s < C::NLOAD; ++s) {
-    const int cell = min(tid + s * NT, C::NREG - 1);  // lanes past the region load a region cell
+    const int cell = load_cell(s);
     int ii = cell % RX, jj = cell / RX;
     int x = T.i0 - HX + ii, y = T.j0 - HX + jj;
     if (wa.on) {
@@ -208,7 +196,7 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
     guint* ubits = (guint*)us.bits;
 #pragma unroll
     for (int s = 0; s < C::NLOAD; ++s) {
-      const int cell = min(tid + s * NT, C::NREG - 1);
+      const int cell = load_cell(s);
       const int bx = (min(T.i0 - HX + cell % RX, xcap) - us.lo[0]) >> us.sh;
       const int by = (min(T.j0 - HX + cell / RX, ycap) - us.lo[1]) >> us.sh;
       const bool inxy = bx >= 0 && bx < us.nb[0] && by >= 0 && by < us.nb[1];
@@ -256,8 +244,8 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
     double* sq = s_q[p & 1];
 #pragma unroll
     for (int s = 0; s < C::NLOAD; ++s) {
-      int cell = tid + s * NT;
-      if (cell < C::NREG) {
+      const int cell = SELF ? scell[s] : tid + s * NT;
+      if (SELF ? cell >= 0 : cell < C::NREG) {
         double q[6];
         cons2prim(pre[s][0], pre[s][1], pre[s][2], pre[s][3], pre[s][4], c, q);
 #pragma unroll
@@ -319,7 +307,10 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
     if (VISC && ring_kind >= 0)
 #pragma unroll
       for (int n = 0; n < 4; ++n) rs[n] = sq[(QU + n) * PL + ring_idx];
-    __syncthreads();  // (b)
+    // (b) plane c+1 visible.  The self-loading form reads plane c+1 only at
+    // cells the thread stored itself; it keeps the barrier while loading the
+    // planes below the tile, whose iterations have no other one
+    if (!SELF || cpl <= k0 - 1) __syncthreads();
     if (cpl < k0 - 1 || (!VISC && H == 1 && cpl < k0)) {  // uniform: only loading planes below the tile
 #pragma unroll
       for (int n = 0; n < NPRIM; ++n) qm1[n] = qs[n];
@@ -395,31 +386,31 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
         d.dvdz = (sqn[QV * PL + r] - rm1[1]) * hz;
         d.dwdz = (sqn[QW * PL + r] - rm1[2]) * hz;
         d.dTdz = (sqn[QT * PL + r] - rm1[3]) * hz;
+        // both directions in one pass (each component is computed as the
+        // one-direction flux computes it), stored per lane
         double fx[4], fy[4], fz[4];
-        if (ring_kind == 0) {
-          visc_flux<true, false, false>(d, c, fx, fy, fz);
+        visc_flux<true, true, false>(d, c, fx, fy, fz);
+        const bool rx = ring_kind == 0;
+        double* dst = rx ? &s_px[0][ring_b][ring_a + 1] : &s_py[0][ring_b + 1][ring_a];
+        const int cs = rx ? TY * (TX + 2) : (TY + 2) * TX;
 #pragma unroll
-          for (int n = 0; n < 4; ++n) s_px[n][ring_b][ring_a + 1] = fx[n];
-        } else {
-          visc_flux<false, true, false>(d, c, fx, fy, fz);
-#pragma unroll
-          for (int n = 0; n < 4; ++n) s_py[n][ring_b + 1][ring_a] = fy[n];
-        }
+        for (int n = 0; n < 4; ++n) dst[n * cs] = rx ? fx[n] : fy[n];
       }
-      // KEEP part of the faces at i = TX (wave 1) and j = TY (wave 2); their
-      // viscous part is added after the barrier
-      if (C::EARLY && inner) {
+      // KEEP part of the faces at i = TX (threads 0 .. TY-1) and j = TY
+      // (threads TY .. TY+TX-1) in one pass; their viscous part is added
+      // after the barrier by other threads
+      if (C::EARLY && inner && tid < TX + TY) {
+        const bool isx = tid < TY;
+        const int e = isx ? tid : tid - TY;
         double f[5];
-        if (tid >= C::XB && tid < C::XB + TY) {
-          const int jj = tid - C::XB;
-          keep_face<H, 0, PL>(sq, (jj + HX) * RX + (TX + HX), 1, co, cv, f);
+        keep_face_xy<H, PL>(sq, isx ? (e + HX) * RX + (TX + HX) : (TY + HX) * RX + (e + HX), isx ? 1 : RX, isx, co,
+                            cv, f);
+        double* dst = isx ? &s_fx[0][e][TX] : &s_fy[0][TY][e];
+        const int cs = isx ? TY * (TX + 1) : (TY + 1) * TX;
 #pragma unroll
-          for (int n = 0; n < 5; ++n) s_fx[n][jj][TX] = f[n];
-        } else if (tid >= C::YB && tid < C::YB + TX) {
-          const int ii = tid - C::YB;
-          keep_face<H, 1, PL>(sq, (TY + HX) * RX + (ii + HX), RX, co, cv, f);
-#pragma unroll
-          for (int n = 0; n < 5; ++n) s_fy[n][TY][ii] = f[n];
+        for (int n = 0; n < 5; ++n) {
+          if (SELF) s_fe[n][tid] = f[n];
+          else dst[n * cs] = f[n];
         }
       }
       // (c) viscous x/y fluxes visible; below the tile this barrier instead
@@ -447,14 +438,27 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
 #pragma unroll
       for (int n = 0; n < 5; ++n) s_fy[n][tj][ti] = f[n];
       // the faces at i = TX and j = TY
-      if (tid >= C::XB && tid < C::XB + TY) {
-        const int jj = tid - C::XB;
-        if (C::EARLY) {
+      if (C::EARLY) {
+        if (tid >= C::VA && tid < C::VA + TX + TY) {
+          const int q = tid - C::VA;
+          const bool isx = q < TY;
+          const int e = isx ? q : q - TY;
+          double* dst = isx ? &s_fx[0][e][TX] : &s_fy[0][TY][e];
+          const int cs = isx ? TY * (TX + 1) : (TY + 1) * TX;
+          // viscous cell fluxes on either side of the face
+          const double* pv = isx ? &s_px[0][e][TX] : &s_py[0][TY][e];
+          const int pcs = isx ? TY * (TX + 2) : (TY + 2) * TX, pst = isx ? 1 : TX;
 #pragma unroll
-          for (int n = 0; n < 5; ++n) f[n] = s_fx[n][jj][TX];
-        } else {
-          keep_face<H, 0, PL>(sq, (jj + HX) * RX + (TX + HX), 1, co, cv, f);
+          for (int n = 0; n < 5; ++n) f[n] = SELF ? s_fe[n][q] : dst[n * cs];
+          f[0] += 0.5 * (0.0 + 0.0);
+#pragma unroll
+          for (int n = 0; n < 4; ++n) f[n + 1] += 0.5 * (pv[n * pcs] + pv[n * pcs + pst]);
+#pragma unroll
+          for (int n = 0; n < 5; ++n) dst[n * cs] = f[n];
         }
+      } else if (tid >= C::XB && tid < C::XB + TY) {
+        const int jj = tid - C::XB;
+        keep_face<H, 0, PL>(sq, (jj + HX) * RX + (TX + HX), 1, co, cv, f);
         if (VISC) {
           f[0] += 0.5 * (0.0 + 0.0);
 #pragma unroll
@@ -464,12 +468,7 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
         for (int n = 0; n < 5; ++n) s_fx[n][jj][TX] = f[n];
       } else if (tid >= C::YB && tid < C::YB + TX) {
         const int ii = tid - C::YB;
-        if (C::EARLY) {
-#pragma unroll
-          for (int n = 0; n < 5; ++n) f[n] = s_fy[n][TY][ii];
-        } else {
-          keep_face<H, 1, PL>(sq, (TY + HX) * RX + (ii + HX), RX, co, cv, f);
-        }
+        keep_face<H, 1, PL>(sq, (TY + HX) * RX + (ii + HX), RX, co, cv, f);
         if (VISC) {
           f[0] += 0.5 * (0.0 + 0.0);
 #pragma unroll
@@ -515,9 +514,9 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
 #pragma unroll
         for (int n = 0; n < 5; ++n) {
           L[n] = xy_prev[n] + dxi2 * (fz_prev[n] - fz[n]);
-          if (has_src) L[n] = L[n] + SRC(gi, gj, cpl - 1, n);  // user_source
+          if (has_src) L[n] = L[n] + SR(gi, gj, cpl - 1, n);  // user_source
         }
-        if (sa.bfx != 0.0) {  // body-force user_source (exm/Channel/prob.h:189-193)
+        if (SRC && sa.bfx != 0.0) {  // body-force user_source (exm/Channel/prob.h:189-193)
           L[UMX] = L[UMX] + sv[URHO] * sa.bfx;
           L[UET] = L[UET] + sv[UMX] * sa.bfx;
         }
@@ -548,17 +547,24 @@ __global__ void __launch_bounds__(TX* TY) k_fused(const TileDev* __restrict__ ti
   }
 }
 
-template <int H, bool VISC, int TX, int TY>
+#define AMR_FUSED_LAUNCH(UCS_, SELF_, SRC_)                                                                   \
+  hipLaunchKernelGGL((k_fused<H, VISC, TX, TY, UCS_, SELF_, SRC_>), dim3(ntiles), dim3(TX * TY), 0, st, tiles, boxes, \
+                     c, co[0], co[1], co[2], sa, wa, cov, us, s1, sin, out)
+
+template <int H, bool VISC, int TX, int TY, bool SELF>
 void launch_t(const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c, const double co[3],
               const StageArgs& sa, const WrapArgs& wa, const Cover& cov, const UncovSrc& us, const Fab* s1,
               const Fab* sin, const Fab* out, hipStream_t st) {
-  if (us.on)
-    hipLaunchKernelGGL((k_fused<H, VISC, TX, TY, true>), dim3(ntiles), dim3(TX * TY), 0, st, tiles, boxes, c, co[0],
-                       co[1], co[2], sa, wa, cov, us, s1, sin, out);
-  else
-    hipLaunchKernelGGL((k_fused<H, VISC, TX, TY, false>), dim3(ntiles), dim3(TX * TY), 0, st, tiles, boxes, c, co[0],
-                       co[1], co[2], sa, wa, cov, us, s1, sin, out);
+  const bool src = sa.src != nullptr || sa.bfx != 0.0;
+  if (us.on) {
+    if (src) AMR_FUSED_LAUNCH(true, SELF, true);
+    else AMR_FUSED_LAUNCH(true, SELF, false);
+  } else {
+    if (src) AMR_FUSED_LAUNCH(false, SELF, true);
+    else AMR_FUSED_LAUNCH(false, SELF, false);
+  }
 }
+#undef AMR_FUSED_LAUNCH
 
 template <int TX, int TY>
 bool launch_shape(const amr_scheme_t& s, const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c,
@@ -566,11 +572,11 @@ bool launch_shape(const amr_scheme_t& s, const TileDev* tiles, int ntiles, const
                   const UncovSrc& us, const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
   const bool v = s.rhs_visc != 0;
   if (s.order_keep == 4) {
-    if (v) launch_t<2, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
-    else launch_t<2, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    if (v) launch_t<2, true, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    else launch_t<2, false, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
   } else if (s.order_keep == 2) {
-    if (v) launch_t<1, true, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
-    else launch_t<1, false, TX, TY>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    if (v) launch_t<1, true, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    else launch_t<1, false, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
   } else {
     return false;
   }
@@ -586,6 +592,10 @@ bool fused_supported(const amr_scheme_t& s, int tx, int ty) {
   return (tx == 16 && ty == 16) || (tx == 8 && ty == 8) || (tx == 32 && ty == 8);
 }
 
+bool fused_self_supported(const amr_scheme_t& s, int tx, int ty) {
+  return fused_supported(s, tx, ty) && s.rhs_visc && tx == 16 && ty == 16;
+}
+
 hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const WrapArgs& wa, const Cover& cov,
                               const UncovSrc& us, const TileDev* tiles, int ntiles, const BoxDev* boxes,
                               const Closures& c, const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* out,
@@ -593,7 +603,12 @@ hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, con
   double co[3];
   keep_coefficients(s.order_keep, co);
   bool ok = false;
-  if (cfg.tx == 16 && cfg.ty == 16)
+  if (cfg.form == 1) {
+    if (!fused_self_supported(s, cfg.tx, cfg.ty)) return hipErrorInvalidValue;
+    if (s.order_keep == 4) launch_t<2, true, 16, 16, true>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    else launch_t<1, true, 16, 16, true>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    ok = true;
+  } else if (cfg.tx == 16 && cfg.ty == 16)
     ok = launch_shape<16, 16>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
   else if (cfg.tx == 8 && cfg.ty == 8)
     ok = launch_shape<8, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);

@@ -326,6 +326,7 @@ struct TileDev {
 struct FusedConfig {
   int tx, ty, tz;      // tile extents
   int box_tiles = 0;   // 1: tiles per box even where level tiles would apply
+  int form = 0;        // 0 first form, 1 self-loading form (16x16 viscous; no barrier after the plane store)
 };
 // Ghost cells of a level that no box covers (coarse-fine ghost cells) keep
 // the caller's values of U^n through a step.  Where every stage array is one
@@ -345,6 +346,8 @@ struct UncovSrc {
 // mask).
 constexpr int UNCOV_MAX_ZBLK = 32;
 bool fused_supported(const amr_scheme_t& s, int tx, int ty);
+// the self-loading form of the fused kernel (FusedConfig::form 1) applies
+bool fused_self_supported(const amr_scheme_t& s, int tx, int ty);
 hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const WrapArgs& wa, const Cover& cov,
                               const UncovSrc& us, const TileDev* tiles, int ntiles, const BoxDev* boxes,
                               const Closures& c, const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* out,

@@ -59,6 +59,9 @@ class Context:
             pass
 
 
+LEVEL_PAD = 16  # x offset of cell 0 in a level-wide array's rows (elements)
+
+
 class StateArrays:
     """Device arrays of one MultiFab: either one level-wide array (boxes are
     windows into it) or one array per box."""
@@ -74,10 +77,16 @@ class StateArrays:
         dev = torch.device("cuda", level.ctx.device)
         if layout == "level":
             N = level.n
-            shape = (ncomp, N[2] + 2 * ng, N[1] + 2 * ng, N[0] + 2 * ng)
-            t = torch.zeros(shape, dtype=torch.float64, device=dev)
+            # rows padded to a multiple of 16 elements with x = 0 at element
+            # LEVEL_PAD of a row: the rows of 16-cell tiles start on 128-byte
+            # lines (the allocation is 512-byte aligned)
+            if ng > LEVEL_PAD:
+                raise ValueError(f"ng {ng} > {LEVEL_PAD}")
+            rowlen = -(-(LEVEL_PAD + N[0] + ng) // 16) * 16
+            store = torch.zeros((ncomp, N[2] + 2 * ng, N[1] + 2 * ng, rowlen), dtype=torch.float64, device=dev)
+            t = store[:, :, :, LEVEL_PAD - ng:LEVEL_PAD + N[0] + ng]  # (ncomp, nz+2g, ny+2g, nx+2g) view
             self.tensors.append(t)
-            JS, KS, NS = shape[3], shape[3] * shape[2], shape[3] * shape[2] * shape[1]
+            JS, KS, NS = store.stride(2), store.stride(1), store.stride(0)
             base = t.data_ptr()
             for b, (lo, hi) in enumerate(level.boxes):
                 f = self.fabs[b]
@@ -273,10 +282,14 @@ class Level:
         call("amr_advance_graph", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, dt,
              nsteps)
 
-    def set_tiling(self, tx, ty=0, tz=0):
+    def set_tiling(self, tx, ty=0, tz=0, form=0):
         """Force the fused-kernel tiling for this level's scheme (0,0,0 = auto,
-        -1 = pass-structured kernels)."""
-        call("amr_set_tiling", self.ctx.h, ctypes.byref(self.scheme), tx, ty, tz)
+        -1 = pass-structured kernels); form 1 picks the fused kernel's
+        self-loading form (16x16 tiles, viscous KEEP 2/4)."""
+        if form:
+            call("amr_set_tiling_form", self.ctx.h, ctypes.byref(self.scheme), tx, ty, tz, form)
+        else:
+            call("amr_set_tiling", self.ctx.h, ctypes.byref(self.scheme), tx, ty, tz)
 
     def set_box_tiling(self, tx, ty, tz):
         """Force the fused-kernel tiling with tiles per box, also on a
@@ -286,11 +299,12 @@ class Level:
     def step_plan(self, S):
         """How a step on S runs: tile, tile kind ("passes", "boxes" or
         "level") and whether the stages after the first read the ghost
-        cells no box covers straight from U^n (no copy pass)."""
-        out = (ctypes.c_int32 * 5)()
+        cells no box covers straight from U^n (no copy pass), and the fused
+        kernel's form (0 first, 1 self-loading)."""
+        out = (ctypes.c_int32 * 6)()
         call("amr_step_plan", self.h, ctypes.byref(self.scheme), S.fabs, out)
         return {"tiling": (out[0], out[1], out[2]), "tiles": ("passes", "boxes", "level")[out[3]],
-                "uncovered_from_un": bool(out[4])}
+                "uncovered_from_un": bool(out[4]), "form": out[5]}
 
     def autotune(self, S):
         out = (ctypes.c_int32 * 3)()

@@ -54,7 +54,8 @@ def parse():
                     choices=["tgv", "sod-w", "sod-r", "riemann-batch-w", "riemann-batch-r", "adjoint", "amr",
                              "amr-hierarchy", "channel"])
     ap.add_argument("--tiling", type=str, default="auto",
-                    help="fused-kernel tiling: auto (autotune), default, or TX,TY,TZ")
+                    help="fused-kernel tiling: auto (autotune), default, or TX,TY,TZ[,FORM] "
+                         "(FORM 1: the self-loading form of 16x16 tiles)")
     ap.add_argument("--traffic", type=str, default=os.path.join(ROOT, "profiles", "traffic_tgv256_keep4v.json"))
     return ap.parse_args()
 
@@ -649,7 +650,9 @@ def main():
         tiling = None
     else:
         tiling = tuple(int(x) for x in a.tiling.split(","))
-        lev.set_tiling(*tiling)
+        lev.set_tiling(*tiling)  # TX,TY,TZ[,FORM]
+        tiling = tiling[:3]
+    form = lev.step_plan(S)["form"]
     t0 = np.array(lev.totals(S))
     # timed region: whole CNS::advance steps (ghost fill + three fused stages)
     for _ in range(a.warmup):
@@ -695,7 +698,8 @@ def main():
     traffic = None
     if os.path.exists(a.traffic) and tiling:
         try:
-            entry = json.load(open(a.traffic)).get("by_tiling", {}).get(",".join(str(x) for x in tiling))
+            key = ",".join(str(x) for x in tiling) + (f",{form}" if form else "")
+            entry = json.load(open(a.traffic)).get("by_tiling", {}).get(key)
             traffic = entry["bytes_per_launch"] if entry else None
         except (OSError, ValueError, KeyError):
             traffic = None
@@ -720,6 +724,7 @@ def main():
                  "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": fp64_tflops / FP64_PEAK_TFLOPS},
         "stage_kernel_ms": [float(x * 1e3) for x in dur],
         "tiling": tiling,
+        "kernel_form": form,
         "totals_drift": {"mass": drift[0], "energy": drift[1]},
     }
     if not a.no_adjoint:

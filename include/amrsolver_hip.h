@@ -302,6 +302,13 @@ int amr_batch_est_dt(amr_boxtab_t bt, const amr_closures_t* cls, const amr_fab_t
  * counterpart: AMReX picks launch shapes itself (ParallelFor). */
 int amr_set_tiling(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, int tz);
 
+/* As amr_set_tiling, choosing also the form of the fused kernel: 0 the
+ * first, 1 the self-loading one (16x16 tiles of a viscous KEEP 2/4 scheme:
+ * each thread converts the cells of the next plane it reads itself, which
+ * saves one workgroup barrier per plane).  The forms give the same bits.
+ * No reference counterpart. */
+int amr_set_tiling_form(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, int tz, int form);
+
 /* As amr_set_tiling with tx > 0, but the fused kernel keeps to tiles per box
  * also where the state is one array over the level (no reference
  * counterpart). */
@@ -312,8 +319,10 @@ int amr_set_box_tiling(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, i
  * info[3] 0 pass-structured, 1 tiles per box, 2 tiles over the union of the
  * boxes, info[4] 1 when the stages after the first read the ghost cells no
  * box covers straight from U^n (else a copy pass per step into the stage
- * workspaces, where there are such cells).  No reference counterpart. */
-int amr_step_plan(amr_boxtab_t bt, const amr_scheme_t* sch, const amr_fab_t* state, int32_t info[5]);
+ * workspaces, where there are such cells), info[5] the fused kernel's form
+ * (0 first, 1 self-loading; see amr_set_tiling_form).  No reference
+ * counterpart. */
+int amr_step_plan(amr_boxtab_t bt, const amr_scheme_t* sch, const amr_fab_t* state, int32_t info[6]);
 
 /* Time a whole step (fills and stages, the last one written to a workspace)
  * on `state` (ghost-filled, ng >= 3) under every compiled tiling that fits
