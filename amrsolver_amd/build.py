@@ -32,7 +32,7 @@ DEVICE = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-SOURCES = ["capi.cpp", "unfused.hip", "fused.hip", "fill_reduce.hip", "adjoint.hip", "dir_flux.hip"]
+SOURCES = ["capi.cpp", "unfused.hip", "fused.hip", "fill_reduce.hip", "adjoint.hip", "dir_flux.hip", "regrid.hip"]
 
 
 def _compile(src):

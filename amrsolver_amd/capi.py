@@ -47,6 +47,16 @@ class Problem(ctypes.Structure):
     _fields_ = [("wall_T", ctypes.c_double), ("body_force_x", ctypes.c_double)]
 
 
+AMR_TAG_DENGRAD = 1
+AMR_TAG_WALLBAND = 2
+AMR_TAG_VORT = 3
+
+
+class Tagger(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("pad_", ctypes.c_int32), ("threshold", ctypes.c_double),
+                ("h", ctypes.c_double), ("factor", ctypes.c_double)]
+
+
 class Fab(ctypes.Structure):
     _fields_ = [
         ("lo", ctypes.c_int32 * 3), ("hi", ctypes.c_int32 * 3), ("ng", ctypes.c_int32),
@@ -70,6 +80,10 @@ SIGNATURES = {
     "amr_fill_two_levels": ([_P, _P, ctypes.c_int, ctypes.c_double, _PF, ctypes.c_double, _PF, ctypes.c_int,
                              _PF], ctypes.c_int),
     "amr_average_down": ([_P, _P, ctypes.c_int, _PF, _PF], ctypes.c_int),
+    "amr_tag_cells": ([_P, ctypes.POINTER(Tagger), _PF, _P], ctypes.c_int),
+    "amr_tag_blocks": ([_P, _P, ctypes.c_int, _P], ctypes.c_int),
+    "amr_regrid_fill": ([_P, _P, _PF, _P, ctypes.c_int, ctypes.c_double, _PF, ctypes.c_double, _PF, _PF],
+                        ctypes.c_int),
     "amr_plot_vars": ([_P, ctypes.POINTER(Closures), _PF, ctypes.c_int, ctypes.c_int, _PF], ctypes.c_int),
     "amr_write_plotfile": ([ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_P), ctypes.POINTER(_PF), ctypes.c_int,
                             ctypes.POINTER(ctypes.c_char_p), ctypes.c_double, ctypes.POINTER(ctypes.c_int32),

@@ -1900,6 +1900,116 @@ int finish_boxtab(amr_boxtab_s* bt);
 }  // namespace
 
 // ======================================================================= ABI
+namespace {
+
+TagArgs tag_args(const amr_boxtab_s* bt) {
+  const amr_geom_t& g = bt->geoms[0];
+  TagArgs ta;
+  std::memset(&ta, 0, sizeof ta);
+  for (int d = 0; d < 3; ++d) {
+    ta.dlo[d] = g.dom_lo[d];
+    ta.n[d] = g.dom_hi[d] - g.dom_lo[d] + 1;
+    ta.dx[d] = g.dx[d];
+  }
+  ta.prob_hi1 = g.prob_hi[1];
+  return ta;
+}
+
+// Cells of box `nb` (of the new level) that no box of the old level holds, as
+// rectangles: x-runs of each row, merged over consecutive rows and planes
+// with the same extents.
+void uncovered_by(const BoxDev& nb, int dst, const std::vector<BoxDev>* old, std::vector<CopyTask>& out) {
+  const int nx = nb.hi[0] - nb.lo[0] + 1, ny = nb.hi[1] - nb.lo[1] + 1, nz = nb.hi[2] - nb.lo[2] + 1;
+  std::vector<unsigned char> cov(size_t(nx) * ny * nz, 0);
+  if (old)
+    for (const BoxDev& o : *old) {
+      int lo[3], hi[3];
+      bool empty = false;
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::max(nb.lo[d], o.lo[d]);
+        hi[d] = std::min(nb.hi[d], o.hi[d]);
+        empty = empty || lo[d] > hi[d];
+      }
+      if (empty) continue;
+      for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+          for (int i = lo[0]; i <= hi[0]; ++i)
+            cov[(size_t(k - nb.lo[2]) * ny + (j - nb.lo[1])) * nx + (i - nb.lo[0])] = 1;
+    }
+  // runs of each row, then rectangles grown over j (keyed by plane and
+  // x-extent), then over k
+  struct Rect {
+    int lo[3], hi[3];
+  };
+  std::vector<Rect> plane_rects;
+  std::vector<Rect> done;
+  for (int k = nb.lo[2]; k <= nb.hi[2]; ++k) {
+    std::map<std::pair<int, int>, Rect> open;
+    for (int j = nb.lo[1]; j <= nb.hi[1]; ++j) {
+      std::map<std::pair<int, int>, Rect> next;
+      const unsigned char* row = &cov[(size_t(k - nb.lo[2]) * ny + (j - nb.lo[1])) * nx];
+      for (int i = 0; i < nx;) {
+        if (row[i]) {
+          ++i;
+          continue;
+        }
+        int e = i;
+        while (e + 1 < nx && !row[e + 1]) ++e;
+        const std::pair<int, int> key{nb.lo[0] + i, nb.lo[0] + e};
+        auto it = open.find(key);
+        Rect r;
+        if (it != open.end()) {
+          r = it->second;
+          r.hi[1] = j;
+          open.erase(it);
+        } else {
+          r = Rect{{key.first, j, k}, {key.second, j, k}};
+        }
+        next[key] = r;
+        i = e + 1;
+      }
+      for (auto& kv : open) plane_rects.push_back(kv.second);
+      open.swap(next);
+    }
+    for (auto& kv : open) plane_rects.push_back(kv.second);
+  }
+  // merge plane rectangles with equal x/y extents over consecutive k
+  std::map<std::array<int, 4>, Rect> zopen;
+  std::sort(plane_rects.begin(), plane_rects.end(), [](const Rect& a, const Rect& b) { return a.lo[2] < b.lo[2]; });
+  for (const Rect& r : plane_rects) {
+    const std::array<int, 4> key{r.lo[0], r.hi[0], r.lo[1], r.hi[1]};
+    auto it = zopen.find(key);
+    if (it != zopen.end() && it->second.hi[2] == r.lo[2] - 1) {
+      it->second.hi[2] = r.hi[2];
+    } else {
+      if (it != zopen.end()) done.push_back(it->second);
+      zopen[key] = r;
+    }
+  }
+  for (auto& kv : zopen) done.push_back(kv.second);
+  for (const Rect& r : done) {
+    CopyTask t;
+    t.dst = t.src = dst;
+    for (int d = 0; d < 3; ++d) {
+      t.lo[d] = r.lo[d];
+      t.hi[d] = r.hi[d];
+      t.sh[d] = 0;
+    }
+    t.start = 0;
+    out.push_back(t);
+  }
+}
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
 extern "C" {
 
 const char* amr_last_error(void) { return g_err.c_str(); }
@@ -2196,13 +2306,14 @@ CfTable* cf_table(amr_boxtab_s* f, amr_boxtab_s* c, int r, int ngrow) {
   return t.get();
 }
 
-// Fine ghost regions no fine box covers (inside the domain, periodic images
-// unwrapped), split by periodic shift and by the coarse box holding their
-// coarse cells.  The seven-point coarse stencil of every such cell must lie
-// in the coarse level (valid cells, periodic images) or outside the domain
-// across a physical boundary that is not ext_dir.
-int build_cf_interp(amr_boxtab_s* F, amr_boxtab_s* C, int r, CfTable* t) {
-  if (t->interp_built) return AMR_OK;
+// Interpolation tasks for fine regions `pieces` (task dst = fine box, region
+// lo/hi, possibly outside the domain in periodic directions): each region is
+// split by periodic shift and by the coarse box holding its coarse cells.  The
+// seven-point coarse stencil of every cell must lie in the coarse level (valid
+// cells, periodic images) or outside the domain across a physical boundary
+// that is not ext_dir.
+int cf_interp_tasks(amr_boxtab_s* F, amr_boxtab_s* C, int r, const std::vector<CopyTask>& pieces,
+                    std::vector<InterpTask>& tasks, int64_t* ncell_out) {
   const amr_geom_t& g = F->geoms[0];
   const amr_geom_t& gc = C->geoms[0];
   int len[3], lenc[3];
@@ -2231,9 +2342,7 @@ int build_cf_interp(amr_boxtab_s* F, amr_boxtab_s* C, int r, CfTable* t) {
         }
     return true;
   };
-  std::vector<InterpTask> tasks;
-  int64_t ncell = 0, nun = 0;
-  const std::vector<CopyTask> pieces = uncovered_pieces(F, t->ngrow, &nun);
+  int64_t ncell = 0;
   for (const CopyTask& p : pieces) {
     struct Part {
       IBox x;
@@ -2310,16 +2419,29 @@ int build_cf_interp(amr_boxtab_s* F, amr_boxtab_s* C, int r, CfTable* t) {
         tasks.push_back(tk);
       }
       if (got != need)
-        return fail(AMR_ERR_ARG, "coarse-fine interpolation: fine level not properly nested (ghost cells over no "
+        return fail(AMR_ERR_ARG, "coarse-fine interpolation: fine level not properly nested (fine cells over no "
                                  "coarse box)");
     }
   }
+  *ncell_out = ncell;
+  return AMR_OK;
+}
+
+// Fine ghost regions no fine box covers (inside the domain, periodic images
+// unwrapped), as interpolation tasks.
+int build_cf_interp(amr_boxtab_s* F, amr_boxtab_s* C, int r, CfTable* t) {
+  if (t->interp_built) return AMR_OK;
+  std::vector<InterpTask> tasks;
+  int64_t ncell = 0, nun = 0;
+  const std::vector<CopyTask> pieces = uncovered_pieces(F, t->ngrow, &nun);
+  int rc = cf_interp_tasks(F, C, r, pieces, tasks, &ncell);
+  if (rc) return rc;
   if (!tasks.empty()) {
     HIPCHK(hipMalloc(&t->d_interp, sizeof(InterpTask) * tasks.size()));
     HIPCHK(hipMemcpy(t->d_interp, tasks.data(), sizeof(InterpTask) * tasks.size(), hipMemcpyHostToDevice));
   }
   t->ninterp_cells = ncell;
-  int rc = upload_blocks(tasks, &t->d_interp_blk, &t->ninterp_blk);
+  rc = upload_blocks(tasks, &t->d_interp_blk, &t->ninterp_blk);
   if (rc) return rc;
   t->interp_built = true;
   return AMR_OK;
@@ -2408,6 +2530,116 @@ int amr_average_down(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, const amr_
   if ((rc = get_desc(fine, fine_state, 0, &df))) return rc;
   if ((rc = get_desc(crse, crse_state, 0, &dc))) return rc;
   HIPCHK(launch_avg_down(t->d_avg, t->d_avg_blk, t->navg_blk, df->d, dc->d, ratio, fine->ctx->stream));
+  return AMR_OK;
+}
+
+int amr_tag_cells(amr_boxtab_t bt, const amr_tagger_t* tagger, const amr_fab_t* state, uint8_t* tags) {
+  if (!bt || !tagger || !state || !tags) return fail(AMR_ERR_ARG, "null argument");
+  if (bt->batch) return fail(AMR_ERR_ARG, "a batch of problems has no level to tag");
+  const int kind = tagger->kind;
+  if (kind != AMR_TAG_DENGRAD && kind != AMR_TAG_WALLBAND && kind != AMR_TAG_VORT)
+    return fail(AMR_ERR_ARG, "unknown tagger kind " + std::to_string(kind));
+  if (!std::isfinite(tagger->threshold) || !std::isfinite(tagger->h) || !std::isfinite(tagger->factor))
+    return fail(AMR_ERR_ARG, "tagger parameters must be finite");
+  int rc;
+  DescSet* ds = nullptr;
+  if ((rc = get_desc(bt, state, kind == AMR_TAG_WALLBAND ? 0 : 1, &ds))) return rc;
+  TagArgs ta = tag_args(bt);
+  ta.kind = kind;
+  ta.thr = tagger->threshold;
+  ta.h = tagger->h;
+  ta.factor = tagger->factor;
+  HIPCHK(launch_tag_cells(bt->bt, ds->d, ta, tags, bt->ctx->stream));
+  return AMR_OK;
+}
+
+int amr_tag_blocks(amr_boxtab_t bt, const uint8_t* tags, int cb, uint8_t* out) {
+  if (!bt || !tags || !out) return fail(AMR_ERR_ARG, "null argument");
+  if (bt->batch) return fail(AMR_ERR_ARG, "a batch of problems has no level to tag");
+  if (cb < 1) return fail(AMR_ERR_ARG, "block size must be positive");
+  const TagArgs ta = tag_args(bt);
+  for (int d = 0; d < 3; ++d)
+    if (ta.n[d] % cb != 0) return fail(AMR_ERR_ARG, "domain extents must be multiples of the block size");
+  HIPCHK(launch_tag_blocks(tags, ta, cb, out, bt->ctx->stream));
+  return AMR_OK;
+}
+
+int amr_regrid_fill(amr_boxtab_t fine_new, amr_boxtab_t fine_old, const amr_fab_t* old_state, amr_boxtab_t crse,
+                    int ratio, double a_old, const amr_fab_t* crse_old, double b_new, const amr_fab_t* crse_new,
+                    const amr_fab_t* new_state) {
+  if (!fine_new || !crse || !new_state) return fail(AMR_ERR_ARG, "null argument");
+  if (fine_old && !old_state) return fail(AMR_ERR_ARG, "old_state is required with fine_old");
+  if (!std::isfinite(a_old) || !std::isfinite(b_new)) return fail(AMR_ERR_ARG, "time weights must be finite");
+  const bool use_old = !(a_old == 0.0 && b_new != 0.0), use_new = b_new != 0.0;
+  if (use_old && !crse_old) return fail(AMR_ERR_ARG, "crse_old is required unless a_old == 0 and b_new != 0");
+  if (use_new && !crse_new) return fail(AMR_ERR_ARG, "crse_new is required when b_new != 0");
+  int rc;
+  if ((rc = check_two_levels(fine_new, crse, ratio))) return rc;
+  if (fine_old) {
+    if (fine_old->batch || fine_old->ctx->device != fine_new->ctx->device)
+      return fail(AMR_ERR_ARG, "the old level must be a level on the same device");
+    const amr_geom_t &gn = fine_new->geoms[0], &go = fine_old->geoms[0];
+    for (int d = 0; d < 3; ++d)
+      if (gn.dom_lo[d] != go.dom_lo[d] || gn.dom_hi[d] != go.dom_hi[d])
+        return fail(AMR_ERR_ARG, "the old and new levels have different domains");
+  }
+  DescSet *dn = nullptr, *dold = nullptr, *d0 = nullptr, *d1 = nullptr;
+  if ((rc = get_desc(fine_new, new_state, 0, &dn))) return rc;
+  if (fine_old && (rc = get_desc(fine_old, old_state, 0, &dold))) return rc;
+  if (use_old && (rc = get_desc(crse, crse_old, 1, &d0))) return rc;
+  if (use_new && (rc = get_desc(crse, crse_new, 1, &d1))) return rc;
+  // copies from the old level's boxes, the rest from the coarse level
+  std::vector<CopyTask> copies, pieces;
+  int64_t ncopy = 0;
+  for (int b = 0; b < fine_new->bt.nbox; ++b) {
+    const BoxDev& N = fine_new->bt.h_boxes[b];
+    if (fine_old)
+      for (int o = 0; o < fine_old->bt.nbox; ++o) {
+        const BoxDev& O = fine_old->bt.h_boxes[o];
+        CopyTask t;
+        bool empty = false;
+        for (int d = 0; d < 3; ++d) {
+          t.lo[d] = std::max(N.lo[d], O.lo[d]);
+          t.hi[d] = std::min(N.hi[d], O.hi[d]);
+          t.sh[d] = 0;
+          empty = empty || t.lo[d] > t.hi[d];
+        }
+        if (empty) continue;
+        t.dst = b;
+        t.src = o;
+        t.start = ncopy;
+        ncopy += cells(t.lo, t.hi);
+        copies.push_back(t);
+      }
+    uncovered_by(N, b, fine_old ? &fine_old->bt.h_boxes : nullptr, pieces);
+  }
+  std::vector<InterpTask> itasks;
+  int64_t ninterp = 0;
+  if ((rc = cf_interp_tasks(fine_new, crse, ratio, pieces, itasks, &ninterp))) return rc;
+  DevBuf<CopyTask> dcopy;
+  DevBuf<TaskBlock> dcopy_blk, dint_blk;
+  DevBuf<InterpTask> dint;
+  int ncopy_blk = 0, nint_blk = 0;
+  if (!copies.empty()) {
+    HIPCHK(hipMalloc(&dcopy.p, sizeof(CopyTask) * copies.size()));
+    HIPCHK(hipMemcpy(dcopy.p, copies.data(), sizeof(CopyTask) * copies.size(), hipMemcpyHostToDevice));
+  }
+  if ((rc = upload_blocks(copies, &dcopy_blk.p, &ncopy_blk))) return rc;
+  if (!itasks.empty()) {
+    HIPCHK(hipMalloc(&dint.p, sizeof(InterpTask) * itasks.size()));
+    HIPCHK(hipMemcpy(dint.p, itasks.data(), sizeof(InterpTask) * itasks.size(), hipMemcpyHostToDevice));
+  }
+  if ((rc = upload_blocks(itasks, &dint_blk.p, &nint_blk))) return rc;
+  hipStream_t st = fine_new->ctx->stream;
+  if (ncopy_blk) HIPCHK(launch_copy_regions(dcopy.p, dcopy_blk.p, ncopy_blk, dold->d, dn->d, nullptr, st));
+  if (nint_blk) {
+    const bool blend = use_old && use_new;
+    const Fab* c0 = use_old ? d0->d : d1->d;
+    HIPCHK(launch_interp_cf(dint.p, dint_blk.p, nint_blk, c0, blend ? d1->d : c0, blend, a_old, b_new, ratio, dn->d,
+                            st));
+  }
+  // the task lists are freed on return
+  HIPCHK(hipStreamSynchronize(st));
   return AMR_OK;
 }
 

@@ -314,6 +314,22 @@ hipError_t launch_diag(const BoxTable& bt, const Fab* st, const Closures& c, dou
                        double* result, hipStream_t s);
 // flag[0] (device int, preset to nbox) := lowest box holding a non-physical cell
 hipError_t launch_state_check(const BoxTable& bt, const Fab* st, const Closures& c, int* flag, hipStream_t s);
+
+// Cell tagging (amr_tag_cells): a byte per cell of the level's domain
+// [dlo, dlo + n), x fastest.
+struct TagArgs {
+  int kind;
+  int dlo[3], n[3];
+  double thr, h, factor, prob_hi1;
+  double dx[3];
+};
+// Clears the map, then tags from every box's valid cells (ghost cells of the
+// state filled, one layer read).
+hipError_t launch_tag_cells(const BoxTable& bt, const Fab* st, const TagArgs& ta, unsigned char* tags,
+                            hipStream_t s);
+// out[block] = any tag in the block of cb^3 cells (blocks x fastest).
+hipError_t launch_tag_blocks(const unsigned char* tags, const TagArgs& ta, int cb, unsigned char* out,
+                             hipStream_t s);
 hipError_t launch_totals(const BoxTable& bt, const Fab* st, double* partial, double* result,
                          hipStream_t s);
 

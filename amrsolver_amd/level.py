@@ -278,6 +278,35 @@ class Level:
         """CNS::avgDown: coarse cells of Sc under this level := mean of S."""
         call("amr_average_down", self.h, coarse.h, ratio, S.fabs, Sc.fabs)
 
+    def tag_cells(self, S, tagger, out=None):
+        """CNS::errorEst on the ghost-filled state S: a (nz, ny, nx) uint8
+        device tensor over the domain, 1 where `tagger` (a capi.Tagger)
+        tags the cell."""
+        torch = _torch_mod()
+        if out is None:
+            out = torch.empty((self.n[2], self.n[1], self.n[0]), dtype=torch.uint8,
+                              device=torch.device("cuda", self.ctx.device))
+        call("amr_tag_cells", self.h, ctypes.byref(tagger), S.fabs, ctypes.c_void_p(out.data_ptr()))
+        return out
+
+    def tag_blocks(self, tags, cb):
+        """The tag map reduced to blocks of cb^3 cells: a (nb0, nb1, nb2)
+        numpy bool array indexed [bi, bj, bk]."""
+        torch = _torch_mod()
+        nb = [self.n[d] // cb for d in range(3)]
+        out = torch.empty((nb[2], nb[1], nb[0]), dtype=torch.uint8, device=tags.device)
+        call("amr_tag_blocks", self.h, ctypes.c_void_p(tags.data_ptr()), cb, ctypes.c_void_p(out.data_ptr()))
+        self.ctx.synchronize()
+        return out.cpu().numpy().astype(bool).transpose(2, 1, 0)
+
+    def regrid_fill(self, S, old, S_old, coarse, crse_old, crse_new, a, b, ratio=2):
+        """CNS::init(old) of this (regridded) level: S's valid cells from
+        the old level (`old`, `S_old`; None: none) where it holds them, else
+        from a*crse_old + b*crse_new of `coarse`, interpolated."""
+        call("amr_regrid_fill", self.h, None if old is None else old.h, None if S_old is None else S_old.fabs,
+             coarse.h, ratio, a, None if crse_old is None else crse_old.fabs, b,
+             None if crse_new is None else crse_new.fabs, S.fabs)
+
     def advance_graph(self, S, dt, nsteps):
         call("amr_advance_graph", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, dt,
              nsteps)
@@ -567,6 +596,44 @@ class Hierarchy:
     def advance(self, time, dt):
         """One coarse step of dt (level l steps by dt / r^l)."""
         self._step(0, time, dt)
+
+    def regrid(self, lev, tagger, blocking=8, max_grid=64, eff=0.7, nest=3, autotune=True):
+        """Amr::regrid of level lev+1 at the time the hierarchy has reached
+        (every level at the end of its last step): CNS::errorEst tags on
+        level lev's ghost-filled state (amr_tag_cells), reduced to blocks of
+        blocking/ratio coarse cells, clustered into boxes of at most
+        max_grid fine cells properly nested `nest` coarse cells inside level
+        lev (problems.boxes_from_block_tags), and the level rebuilt on them
+        by CNS::init(old) (amr_regrid_fill) with a new box table (autotuned).
+        Returns the new boxes (the level is kept when they are unchanged)."""
+        from . import problems as P
+        C = self.levels[lev]
+        S = self.cur[lev]
+        if lev == 0:
+            C.fill_patch(S)
+        else:
+            self._fill(lev, S, self.tint[lev][1])
+        cb = blocking // self.ratio
+        tags = C.tag_blocks(C.tag_cells(S, tagger), cb)
+        boxes = P.boxes_from_block_tags(tags, C.boxes, C.n, cb, self.ratio, max_grid // blocking, eff, nest)
+        old = self.levels[lev + 1]
+        if [tuple(map(tuple, b)) for b in boxes] == [tuple(map(tuple, b)) for b in old.boxes]:
+            return boxes
+        if not boxes:
+            raise ValueError("regrid: no tagged cells (removing a level is not supported)")
+        new = Level(old.geom, boxes, old.closures, old.scheme, ctx=old.ctx)
+        layout = self.cur[lev + 1].layout
+        cur = new.new_state(layout=layout)
+        new.regrid_fill(cur, old, self.cur[lev + 1], C, None, S, 0.0, 1.0, self.ratio)
+        self.levels[lev + 1] = new
+        self.cur[lev + 1] = cur
+        self.old[lev + 1] = new.new_state(layout=layout)
+        self.tmp[lev + 1] = new.new_state(layout=layout)
+        self.ngrow[lev + 1] = None
+        old.close()
+        if autotune:
+            new.autotune(cur)
+        return boxes
 
 
 class Batch(Level):

@@ -294,28 +294,41 @@ def tgv_amr_hierarchy(n_base=128, nlev=2, ratio=2, pct=70.0, blocking=8, max_gri
             blk = t.reshape(sh[0], cb, sh[1], cb, sh[2], cb).any(axis=(1, 3, 5))
             tags[lo[0] // cb:lo[0] // cb + sh[0], lo[1] // cb:lo[1] // cb + sh[1],
                  lo[2] // cb:lo[2] // cb + sh[2]] |= blk
-        allowed = None
-        if nest > 0:
-            cover = np.zeros(n, dtype=bool)
-            for lo, hi in boxes:
-                cover[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1] = True
-            inner = _erode_periodic(cover, nest)
-            allowed = inner.reshape(nb[0], cb, nb[1], cb, nb[2], cb).all(axis=(1, 3, 5))
-            tags &= allowed
-        clusters = cluster_blocks(tags, mb, eff, allowed)
-        fine = []
-        for blo, bhi in clusters:
-            clo = tuple(blo[d] * cb for d in range(3))
-            chi = tuple((bhi[d] + 1) * cb - 1 for d in range(3))
-            for B in boxes:
-                x = _intersect((clo, chi), B)
-                if x is not None:
-                    fine.append((tuple(x[0][d] * ratio for d in range(3)),
-                                 tuple((x[1][d] + 1) * ratio - 1 for d in range(3))))
+        fine = boxes_from_block_tags(tags, boxes, n, cb, ratio, mb, eff, nest)
         nf = [n[d] * ratio for d in range(3)]
         gf = geom(nf, (0, 0, 0), (2 * math.pi,) * 3, (1, 1, 1))
-        levels.append((gf, sorted(fine, key=lambda b: (b[0][2], b[0][1], b[0][0]))))
+        levels.append((gf, fine))
     return levels
+
+
+def boxes_from_block_tags(tags, boxes, n, cb, ratio=2, max_blocks=8, eff=0.7, nest=0):
+    """Fine boxes from a coarse level's tags reduced to blocks of cb^3 coarse
+    cells (tags[bi, bj, bk], bool): with nest > 0 tags outside the coarse
+    union eroded by `nest` cells are dropped and clusters are cut until they
+    fit inside it; clusters (cluster_blocks, at most max_blocks blocks a
+    side, tagged fraction >= eff) are clipped to the coarse boxes and
+    refined by `ratio`.  Sorted by (k, j, i) of their low corners."""
+    nb = [n[d] // cb for d in range(3)]
+    tags = np.array(tags, dtype=bool)
+    allowed = None
+    if nest > 0:
+        cover = np.zeros(n, dtype=bool)
+        for lo, hi in boxes:
+            cover[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1] = True
+        inner = _erode_periodic(cover, nest)
+        allowed = inner.reshape(nb[0], cb, nb[1], cb, nb[2], cb).all(axis=(1, 3, 5))
+        tags &= allowed
+    clusters = cluster_blocks(tags, max_blocks, eff, allowed)
+    fine = []
+    for blo, bhi in clusters:
+        clo = tuple(blo[d] * cb for d in range(3))
+        chi = tuple((bhi[d] + 1) * cb - 1 for d in range(3))
+        for B in boxes:
+            x = _intersect((clo, chi), B)
+            if x is not None:
+                fine.append((tuple(x[0][d] * ratio for d in range(3)),
+                             tuple((x[1][d] + 1) * ratio - 1 for d in range(3))))
+    return sorted(fine, key=lambda b: (b[0][2], b[0][1], b[0][0]))
 
 
 # ---------------------------------------------------------------- channel-like box

@@ -148,6 +148,55 @@ int amr_fill_two_levels(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, double 
 int amr_average_down(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, const amr_fab_t* fine_state,
                      const amr_fab_t* crse_state);
 
+/* Error estimation (CNS::errorEst, src/CNS.cpp:469-507: the problem's
+ * user_tagging on every valid cell of every box of a FillPatch'ed state).
+ * kind AMR_TAG_DENGRAD: exm/Sods/prob.h:187-197, |rho(i+1) - rho(i-1)| /
+ * rho(i) > threshold tags cells i .. i+3 (the reference's dengrad_threshold
+ * is 0.5; tags past the domain's hi end are dropped, where the reference
+ * writes past its TagBox).  AMR_TAG_WALLBAND: exm/Channel/prob.h:199-209,
+ * cells with y = (j + 1/2) dy below factor*h or above prob_hi[1] - factor*h
+ * (the reference: factor 0.5, h = ProbParm::h).  AMR_TAG_VORT: |curl u| >
+ * threshold, with u = (rho u)/rho and central differences (the criterion of
+ * the TGV workload's refined levels; the reference's TGV tags nothing). */
+#define AMR_TAG_DENGRAD 1
+#define AMR_TAG_WALLBAND 2
+#define AMR_TAG_VORT 3
+typedef struct {
+  int32_t kind;
+  int32_t pad_;
+  double threshold; /* DENGRAD, VORT */
+  double h;         /* WALLBAND */
+  double factor;    /* WALLBAND */
+} amr_tagger_t;
+
+/* Tags of one level into `tags`, a device byte array over the level's
+ * domain (dom_lo .. dom_hi, x fastest): 1 tagged, 0 not (the array is
+ * cleared first).  `state` needs ng >= 1 with its ghost cells filled
+ * (amr_fill_boundary / amr_fill_two_levels, as errorEst's FillPatch). */
+int amr_tag_cells(amr_boxtab_t bt, const amr_tagger_t* tagger, const amr_fab_t* state, uint8_t* tags);
+
+/* The tag map reduced to blocks of cb^3 cells (the clustering unit, the
+ * blocking factor over the refinement ratio in coarse cells): out[b] = 1
+ * when any cell of block b is tagged; out is a device byte array of
+ * (n0/cb)*(n1/cb)*(n2/cb) blocks, x fastest; the domain extents must be
+ * multiples of cb. */
+int amr_tag_blocks(amr_boxtab_t bt, const uint8_t* tags, int cb, uint8_t* out);
+
+/* CNS::init(AmrLevel& old) (src/CNS.cpp:155-167): the state of a regridded
+ * level, FillPatch(old, S_new, 0 ghost cells, cur_time).  Every valid cell of
+ * `new_state` (boxes of `fine_new`) takes the value of the old level's box
+ * that holds it (`fine_old`, `old_state`); the others are interpolated from
+ * the coarse level at the fill time as amr_fill_two_levels does (weights
+ * a_old, b_new; conservative linear interpolation).  fine_old NULL: every
+ * cell from the coarse level (CNS::init(), FillCoarsePatch, src/CNS.cpp:169-
+ * 179).  The coarse states need ng >= 1 with ghost cells filled and every
+ * interpolated cell's coarse stencil in the coarse level; ghost cells of
+ * new_state are not written.  No reference counterpart beyond the calls
+ * cited: Amr::regrid drives it. */
+int amr_regrid_fill(amr_boxtab_t fine_new, amr_boxtab_t fine_old, const amr_fab_t* old_state, amr_boxtab_t crse,
+                    int ratio, double a_old, const amr_fab_t* crse_old, double b_new, const amr_fab_t* crse_new,
+                    const amr_fab_t* new_state);
+
 /* Plot variables of one level (the plotMF of CNS::writePlotFile,
  * src/CNS.cpp:644-874): into `out` (valid cells), the five state components
  * when state_vars != 0, then the derived variables whose bits are set in

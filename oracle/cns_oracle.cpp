@@ -1783,6 +1783,115 @@ int orc_average_down(int nbf, const int* boxes_f, int ngf, const double* const* 
   return 0;
 }
 
+// Error estimation (CNS::errorEst, src/CNS.cpp:469-507: user_tagging on
+// every valid cell of every box of the ghost-filled state) into a byte map
+// over the domain, x fastest.
+//  kind 1, exm/Sods/prob.h:187-197: drhox = |rho(i+1) - rho(i-1)| / rho(i);
+//    drhox > thr tags i, i+1, i+2, i+3 (cells past the domain's hi end are
+//    dropped; the reference writes them past its TagBox);
+//  kind 2, exm/Channel/prob.h:199-209: y = (j + 0.5) dy; tagged where
+//    y < h*factor or y > prob_hi[1] - h*factor;
+//  kind 3: |curl u| > thr, u = (rho u)/rho, central differences
+//    (u(+1) - u(-1)) * (0.5/dx) per derivative, the magnitude
+//    sqrt(wx^2 + wy^2 + wz^2) with wx = dw/dy - dv/dz, wy = du/dz - dw/dx,
+//    wz = dv/dx - du/dy (no reference tagger: the criterion of the refined
+//    levels of the TGV workload).
+int orc_tag_cells(const orc_geom* g, int nbox, const int* boxes, int ng, const double* const* state, int kind,
+                  double thr, double h, double factor, unsigned char* tags) {
+  if (kind < 1 || kind > 3) return fail("unknown tagger kind");
+  if (kind != 2 && ng < 1) return fail("the tagger reads one ghost layer");
+  Level L = make_level(g, nbox, boxes, ng);
+  int n[3];
+  for (int d = 0; d < 3; ++d) n[d] = g->dom_hi[d] - g->dom_lo[d] + 1;
+  std::fill(tags, tags + size_t(n[0]) * n[1] * n[2], (unsigned char)0);
+  auto at = [&](int i, int j, int k) -> unsigned char& {
+    return tags[(size_t(k - g->dom_lo[2]) * n[1] + (j - g->dom_lo[1])) * n[0] + (i - g->dom_lo[0])];
+  };
+  for (int b = 0; b < L.nbox; ++b) {
+    const Box& bx = L.boxes[b];
+    Box gb = grow(bx, ng);
+    A4 u = view(const_cast<double*>(state[b]), gb.lo, gb.hi);
+    for_box(bx, [&](int i, int j, int k) {
+      if (kind == 1) {
+        const double drhox = std::abs(u(i + 1, j, k, URHO) - u(i - 1, j, k, URHO)) / u(i, j, k, URHO);
+        if (drhox > thr)
+          for (int m = 0; m < 4; ++m)
+            if (i + m <= g->dom_hi[0]) at(i + m, j, k) = 1;
+      } else if (kind == 2) {
+        const double y = (j + 0.5) * g->dx[1];
+        if (y < h * factor || y > g->prob_hi[1] - h * factor) at(i, j, k) = 1;
+      } else {
+        auto vel = [&](int a, int bb, int c, int m) { return u(a, bb, c, m) / u(a, bb, c, URHO); };
+        const double h0 = 0.5 / g->dx[0], h1 = 0.5 / g->dx[1], h2 = 0.5 / g->dx[2];
+        const double dwdy = (vel(i, j + 1, k, UMZ) - vel(i, j - 1, k, UMZ)) * h1;
+        const double dvdz = (vel(i, j, k + 1, UMY) - vel(i, j, k - 1, UMY)) * h2;
+        const double dudz = (vel(i, j, k + 1, UMX) - vel(i, j, k - 1, UMX)) * h2;
+        const double dwdx = (vel(i + 1, j, k, UMZ) - vel(i - 1, j, k, UMZ)) * h0;
+        const double dvdx = (vel(i + 1, j, k, UMY) - vel(i - 1, j, k, UMY)) * h0;
+        const double dudy = (vel(i, j + 1, k, UMX) - vel(i, j - 1, k, UMX)) * h1;
+        const double wx = dwdy - dvdz, wy = dudz - dwdx, wz = dvdx - dudy;
+        if (std::sqrt(wx * wx + wy * wy + wz * wz) > thr) at(i, j, k) = 1;
+      }
+    });
+  }
+  return 0;
+}
+
+// CNS::init(AmrLevel& old) (src/CNS.cpp:155-167, FillPatch(old, S_new, 0,
+// cur_time)) and, with no old level, CNS::init() (FillCoarsePatch,
+// src/CNS.cpp:169-179): every valid cell of the new boxes takes the old
+// level's value where an old box holds it, else the coarse level's data
+// a*crse0 + b*crse1 interpolated as the coarse-fine ghost cells are.
+int orc_regrid_fill(const orc_geom* gf, int nb_new, const int* boxes_new, int ng_new, double* const* fabs_new,
+                    int nb_old, const int* boxes_old, int ng_old, const double* const* fabs_old,
+                    const orc_geom* gc, int nbc, const int* boxes_c, int ngc, const double* const* crse0,
+                    const double* const* crse1, double a, double b, int ratio) {
+  if (int rc = check_two_levels(*gf, *gc, ratio)) return rc;
+  if ((b != 0.0 && !crse1) || (a != 0.0 && !crse0)) return fail("missing coarse data for nonzero weight");
+  Level N = make_level(gf, nb_new, boxes_new, ng_new);
+  Level O = make_level(gf, nb_old, boxes_old, ng_old);
+  Level C = make_level(gc, nbc, boxes_c, ngc);
+  OwnerMap omap, cmap;
+  omap.build(O);
+  cmap.build(C);
+  CoarseFine cf;
+  cf.C = &C;
+  cf.u0 = crse0;
+  cf.u1 = crse1;
+  cf.a = a;
+  cf.b = b;
+  cf.ratio = ratio;
+  cf.cmap = &cmap;
+  std::vector<A4> ov(O.nbox);
+  for (int o = 0; o < O.nbox; ++o) {
+    Box gb = grow(O.boxes[o], ng_old);
+    ov[o] = view(const_cast<double*>(fabs_old[o]), gb.lo, gb.hi);
+  }
+  for (int nb = 0; nb < N.nbox; ++nb) {
+    const Box& bx = N.boxes[nb];
+    Box gb = grow(bx, ng_new);
+    A4 v = view(fabs_new[nb], gb.lo, gb.hi);
+    int err = 0;
+    for_box(bx, [&](int i, int j, int k) {
+      if (err) return;
+      const int c[3] = {i, j, k};
+      const int o = omap.at(c);
+      if (o >= 0) {
+        for (int n = 0; n < NCONS; ++n) v(i, j, k, n) = ov[o](i, j, k, n);
+        return;
+      }
+      double val[NCONS];
+      if (interp_cell(cf, c, val)) {
+        err = 1;
+        return;
+      }
+      for (int n = 0; n < NCONS; ++n) v(i, j, k, n) = val[n];
+    });
+    if (err) return -1;
+  }
+  return 0;
+}
+
 int orc_advance_two_levels(const orc_geom* g, const orc_closures* c, const orc_scheme* s, int nbox, const int* boxes,
                            int ng, double* const* state, const double* const* src, double time, double dt,
                            int nthreads, const orc_geom* gc, int nbc, const int* boxes_c, int ngc,

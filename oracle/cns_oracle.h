@@ -96,6 +96,12 @@ int orc_fill_two_levels(const orc_geom* gf, int nbf, const int* boxes_f, int ngf
 int orc_average_down(int nbf, const int* boxes_f, int ngf, const double* const* fabs_f,
                      int nbc, const int* boxes_c, int ngc, double* const* fabs_c,
                      int ratio);
+int orc_tag_cells(const orc_geom* g, int nbox, const int* boxes, int ng, const double* const* state, int kind,
+                  double thr, double h, double factor, unsigned char* tags);
+int orc_regrid_fill(const orc_geom* gf, int nb_new, const int* boxes_new, int ng_new, double* const* fabs_new,
+                    int nb_old, const int* boxes_old, int ng_old, const double* const* fabs_old,
+                    const orc_geom* gc, int nbc, const int* boxes_c, int ngc, const double* const* crse0,
+                    const double* const* crse1, double a, double b, int ratio);
 int orc_advance_two_levels(const orc_geom* g, const orc_closures* c, const orc_scheme* s,
                            int nbox, const int* boxes, int ng, double* const* state,
                            const double* const* src, double time, double dt, int nthreads,

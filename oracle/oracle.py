@@ -138,6 +138,13 @@ def load(variant=None):
                                            ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                            ctypes.POINTER(OGeom), ctypes.c_int, IP, ctypes.c_int, PP, PP,
                                            ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    lib.orc_tag_cells.argtypes = [ctypes.POINTER(OGeom), ctypes.c_int, IP, ctypes.c_int, PP, ctypes.c_int,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  ctypes.POINTER(ctypes.c_ubyte)]
+    lib.orc_regrid_fill.argtypes = [ctypes.POINTER(OGeom), ctypes.c_int, IP, ctypes.c_int, PP,
+                                    ctypes.c_int, IP, ctypes.c_int, PP,
+                                    ctypes.POINTER(OGeom), ctypes.c_int, IP, ctypes.c_int, PP, PP,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_int]
     if variant is None:
         _lib = lib
     return lib
@@ -391,6 +398,28 @@ class OracleLevel:
                                           None if crse0 is None else coarse._pp(crse0),
                                           None if crse1 is None else coarse._pp(crse1), a, b, ratio))
 
+    def tag_cells(self, kind, threshold=0.0, h=0.0, factor=0.0):
+        """CNS::errorEst on this level's ghost-filled FABs: a (nz, ny, nx)
+        uint8 map over the domain (kinds as amr_tagger_t: 1 density
+        gradient, 2 wall band, 3 vorticity magnitude)."""
+        tags = np.zeros((self.n[2], self.n[1], self.n[0]), dtype=np.uint8)
+        _check(load().orc_tag_cells(ctypes.byref(self.g), len(self.boxes), self.barr, self.ng, self._pp(), kind,
+                                    threshold, h, factor, tags.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))))
+        return tags
+
+    def regridded(self, boxes, coarse, crse0, crse1, a, b, ratio=2):
+        """A level on `boxes` whose valid cells are filled as CNS::init(old)
+        fills a regridded level: this level's value where one of its boxes
+        holds the cell, else interpolated from a*crse0 + b*crse1 (ghosted
+        coarse FABs, ghost cells filled)."""
+        new = OracleLevel(self.g, boxes, self.ng)
+        _check(load().orc_regrid_fill(ctypes.byref(self.g), len(new.boxes), new.barr, new.ng, new._pp(),
+                                      len(self.boxes), self.barr, self.ng, self._pp(),
+                                      ctypes.byref(coarse.g), len(coarse.boxes), coarse.barr, coarse.ng,
+                                      None if crse0 is None else coarse._pp(crse0),
+                                      None if crse1 is None else coarse._pp(crse1), a, b, ratio))
+        return new
+
     def average_down(self, coarse, ratio=2):
         """Coarse cells under this level's boxes := mean of their children."""
         _check(load().orc_average_down(len(self.boxes), self.barr, self.ng, self._pp(), len(coarse.boxes),
@@ -468,6 +497,14 @@ class OracleHierarchy:
     def advance(self, c, s, time, dt, nthreads=1):
         """One coarse step of dt (level l steps by dt / r^l)."""
         self._step(0, c, s, time, dt, nthreads)
+
+    def regrid(self, lev, boxes):
+        """Level lev+1 rebuilt on `boxes` at the time the hierarchy has
+        reached (every level at the end of its last step): CNS::init(old)
+        from the old level and level lev's new data."""
+        C = self.levels[lev]
+        C.fill_patch() if lev == 0 else self._fill(lev, self.tint[lev][1])
+        self.levels[lev + 1] = self.levels[lev + 1].regridded(boxes, C, None, C.fabs, 0.0, 1.0, self.ratio)
 
 
 def set_problem(c, wall_T=0.0, body_force_x=0.0):
