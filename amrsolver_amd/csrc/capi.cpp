@@ -256,6 +256,7 @@ struct amr_boxtab_s {
   hipEvent_t dt_copied = nullptr;
   BoxTable bt;
   std::vector<std::unique_ptr<DescSet>> descs;
+  std::vector<std::unique_ptr<DescSet>> face_descs;  // face-flux arrays (amr_face_fluxes)
   int64_t use_clock = 0;
   // pass-structured workspace
   double* uw_mem = nullptr;
@@ -2677,6 +2678,53 @@ int amr_rk_stage(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
       return fail(AMR_ERR_ARG, "out must not alias sin");
   StageArgs sa{sc, nullptr, 0, 0, 0};
   return do_stage(bt, *cls, *sch, sa, ds1, dsin, dout);
+}
+
+int amr_face_fluxes(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
+                    const amr_fab_t* flux_x, const amr_fab_t* flux_y, const amr_fab_t* flux_z) {
+  if (!bt || !cls || !sch || !state || !flux_x || !flux_y || !flux_z) return fail(AMR_ERR_ARG, "null argument");
+  int rc;
+  if ((rc = check_scheme(*sch)) || (rc = check_closures(*cls))) return rc;
+  DescSet* dsin = nullptr;
+  if ((rc = get_desc(bt, state, NGHOST, &dsin))) return rc;
+  const amr_fab_t* fl[3] = {flux_x, flux_y, flux_z};
+  const Fab* faces[3] = {nullptr, nullptr, nullptr};
+  const int nb = bt->bt.nbox;
+  for (int d = 0; d < 3; ++d) {
+    // face arrays: the box's faces normal to d, lo .. hi + e_d
+    for (int b = 0; b < nb; ++b) {
+      const amr_fab_t& f = fl[d][b];
+      const BoxDev& B = bt->bt.h_boxes[b];
+      for (int e = 0; e < 3; ++e)
+        if (f.lo[e] != B.lo[e] || f.hi[e] != B.hi[e] + (e == d ? 1 : 0))
+          return fail(AMR_ERR_ARG, "face fab " + std::to_string(b) + " of direction " + std::to_string(d) +
+                                       " is not the box's face box (hi + 1 in that direction)");
+      if (f.ng < 0 || !f.data) return fail(AMR_ERR_ARG, "face fab " + std::to_string(b) + " has no data");
+      const int64_t nx = f.hi[0] - f.lo[0] + 1 + 2 * f.ng, ny = f.hi[1] - f.lo[1] + 1 + 2 * f.ng;
+      const int64_t nz = f.hi[2] - f.lo[2] + 1 + 2 * f.ng;
+      if (f.jstride < nx || f.kstride < f.jstride * ny || f.nstride < f.kstride * nz)
+        return fail(AMR_ERR_ARG, "face fab " + std::to_string(b) + " strides do not cover its face box");
+    }
+    DescSet* ds = nullptr;
+    for (auto& x : bt->face_descs)
+      if (std::memcmp(x->host.data(), fl[d], sizeof(amr_fab_t) * nb) == 0) ds = x.get();
+    if (!ds) {
+      if (bt->face_descs.size() >= 12) {
+        HIPCHK(hipStreamSynchronize(bt->ctx->stream));
+        bt->face_descs.erase(bt->face_descs.begin());
+      }
+      std::unique_ptr<DescSet> n;
+      if ((rc = new_desc(bt, fl[d], &n))) return rc;
+      ds = n.get();
+      bt->face_descs.push_back(std::move(n));
+    }
+    faces[d] = ds->d;
+  }
+  if ((rc = ensure_unfused(bt, *sch))) return rc;
+  StageArgs sa{StageCoef{2, 0.0, 0.0, 0.0, 0.0}, nullptr, 0, 0, 0};
+  HIPCHK(launch_unfused_stage(bt->bt, bt->uw, make_closures(*cls), *sch, sa, bt->wall_args, nullptr, dsin->d, nullptr,
+                              bt->ctx->stream, faces));
+  return AMR_OK;
 }
 
 int amr_rhs(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,

@@ -164,7 +164,9 @@ inline void keep_coefficients(int order, double co[3]) {
 hipError_t prepare_block_maps(const BoxTable& bt);
 hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const Closures& c,
                                 const amr_scheme_t& s, const StageArgs& sa, const WallArgs& wa, const Fab* s1,
-                                const Fab* sin, const Fab* out, hipStream_t st);
+                                const Fab* sin, const Fab* out, hipStream_t st, const Fab* const* faces = nullptr);
+// (faces non-null: the face fluxes of every direction into faces[d] over the
+// boxes' faces normal to d, instead of their assembly into `out`)
 
 // cons2prim of `sin` into w.prims on grow(bx,3) for every box (halo cells
 // from their periodic images when wa.on).

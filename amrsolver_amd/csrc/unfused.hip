@@ -451,9 +451,25 @@ hipError_t prepare_block_maps(const BoxTable& bt) {
 }
 
 // Flux phase + assembly + stage combination, reference pass structure.
+// dst(i,j,k,n) = src(i,j,k,n) over region reg of every box (zero when src is
+// null): the face fluxes of one direction out of the workspace.
+__global__ void __launch_bounds__(BLK) k_copy_faces(const BoxDev* boxes, const BlockRef* bmap, Region reg,
+                                                    const double* box_dt, const Fab* src, const Fab* dst) {
+  REGION_LOOP_BEGIN
+  const Fab o = dst[b];
+  if (src) {
+    const Fab f = src[b];
+#pragma unroll
+    for (int n = 0; n < NCONS; ++n) o(i, j, k, n) = f(i, j, k, n);
+  } else {
+#pragma unroll
+    for (int n = 0; n < NCONS; ++n) o(i, j, k, n) = 0.0;
+  }
+}
+
 hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const Closures& c,
                                 const amr_scheme_t& s, const StageArgs& sa, const WallArgs& wa, const Fab* s1,
-                                const Fab* sin, const Fab* out, hipStream_t st) {
+                                const Fab* sin, const Fab* out, hipStream_t st, const Fab* const* faces) {
   const BoxDev* B = bt.d_boxes;
   const double* bdt = sa.box_dt;
 #define LAUNCH(KERNEL, REG, ...)                                                                    \
@@ -465,7 +481,7 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
   const Region g3 = grown(NGHOST), nod = nodal(), g1 = grown(1), g0 = grown(0);
   // WENO or MUSCL without viscous fluxes: directional passes from the
   // conserved state, each forming what it needs of its pencil in LDS
-  if (s.rhs_euler && (s.flux_euler == 2 || s.flux_euler == 0) && !s.rhs_visc && !s.art_diss &&
+  if (!faces && s.rhs_euler && (s.flux_euler == 2 || s.flux_euler == 0) && !s.rhs_visc && !s.art_diss &&
       w.weno.d[0] != nullptr)
     return launch_dir_flux(s.flux_euler, w.weno, B, c, sa, sin, w.nf[0], s1, out, st);
   LAUNCH(k_prims, g3, sin, w.prims, c, WrapArgs{0, {0, 0, 0}, {1, 1, 1}});
@@ -513,6 +529,10 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
       hipLaunchKernelGGL(k_visc_wall, dim3(unsigned(wa.nvisc_blk)), dim3(BLK), 0, st, wa.visc, wa.visc_blk, B,
                          w.prims, w.pv[0], w.pv[1], w.pv[2], c);
     LAUNCH(k_visc_add, nod, w.pv[0], w.pv[1], w.pv[2], w.nf[0], w.nf[1], w.nf[2]);
+  }
+  if (faces) {  // the face fluxes themselves instead of their assembly
+    for (int d = 0; d < 3; ++d) LAUNCH(k_copy_faces, face_region(d), s.rhs_euler ? w.nf[d] : nullptr, faces[d]);
+    return hipGetLastError();
   }
   LAUNCH(k_assemble, g0, w.nf[0], w.nf[1], w.nf[2], sa, s1, sin, out);
 #undef LAUNCH

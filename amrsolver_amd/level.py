@@ -211,6 +211,35 @@ class StateArrays:
         return t[:, lo[2]:hi[2] + 1 + 2 * g, lo[1]:hi[1] + 1 + 2 * g, lo[0]:hi[0] + 1 + 2 * g].cpu().numpy().copy()
 
 
+class FaceArrays:
+    """Face-flux arrays of a level (AMReX's face-centred numflxmf): per
+    direction d one array per box over its faces normal to d (the box with
+    hi + 1 in d)."""
+
+    def __init__(self, level, ncomp=5):
+        torch = _torch_mod()
+        dev = torch.device("cuda", level.ctx.device)
+        nb = len(level.boxes)
+        self.tensors = [[], [], []]
+        self.fabs = [(Fab * nb)() for _ in range(3)]
+        for d in range(3):
+            for b, (lo, hi) in enumerate(level.boxes):
+                n = [hi[e] - lo[e] + 1 + (1 if e == d else 0) for e in range(3)]
+                t = torch.zeros((ncomp, n[2], n[1], n[0]), dtype=torch.float64, device=dev)
+                self.tensors[d].append(t)
+                f = self.fabs[d][b]
+                for e in range(3):
+                    f.lo[e], f.hi[e] = lo[e], hi[e] + (1 if e == d else 0)
+                f.ng = 0
+                f.pad_ = 0
+                f.data = t.data_ptr()
+                f.jstride, f.kstride, f.nstride = n[0], n[0] * n[1], n[0] * n[1] * n[2]
+
+    def download(self, d, b):
+        """Faces of box b normal to d, (ncomp, nz, ny, nx) with one more in d."""
+        return self.tensors[d][b].cpu().numpy().copy()
+
+
 class Level:
     """One AMR level on one GPU (mirror of the CNS AmrLevel's hot path)."""
 
@@ -247,6 +276,12 @@ class Level:
     # --- reference interface -------------------------------------------
     def fill_patch(self, S):
         call("amr_fill_boundary", self.h, S.fabs)
+
+    def face_fluxes(self, Sborder, F):
+        """numflxmf of compute_rhs for the ghost-filled state into the
+        FaceArrays F."""
+        call("amr_face_fluxes", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), Sborder.fabs,
+             F.fabs[0], F.fabs[1], F.fabs[2])
 
     def compute_rhs(self, Sborder, dSdt):
         call("amr_rhs", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), Sborder.fabs, dSdt.fabs)

@@ -197,6 +197,23 @@ int amr_regrid_fill(amr_boxtab_t fine_new, amr_boxtab_t fine_old, const amr_fab_
                     int ratio, double a_old, const amr_fab_t* crse_old, double b_new, const amr_fab_t* crse_new,
                     const amr_fab_t* new_state);
 
+/* The numerical face fluxes numflxmf[3] that CNS::compute_rhs assembles
+ * (src/tim/CNS_advance.cpp:218-346): per direction d the scheme's Euler flux
+ * (CentralKEEP::FluxKEEP with Flux_2nd_Order_KEEP next to bc 6 walls,
+ * HiRes::FluxWENO or Riemann::Flux; with JSTflux when art_diss = 1)
+ * plus ViscousNumericalFluxes (with ViscousWallFluxes at
+ * walls), at every face of every box normal to d.  flux_d[b] describes box
+ * b's face box: lo = the box's lo, hi = its hi + 1 in direction d (as the
+ * Array4 of AMReX's face-centred numflxmf); ghost layers allowed, none
+ * written.  `state` needs its ghost cells filled (ng >= 3).  The assembly
+ * dSdt = dxinv[0]*(fx(i) - fx(i+1)) + dxinv[1]*(fy(j) - fy(j+1)) +
+ * dxinv[2]*(fz(k) - fz(k+1)) of these faces is amr_rhs's result before the
+ * source, bit for bit.  For a caller that assembles itself or feeds flux
+ * registers (the reference's commented-out refluxing,
+ * src/tim/CNS_advance.cpp:356-371). */
+int amr_face_fluxes(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t* sch, const amr_fab_t* state,
+                    const amr_fab_t* flux_x, const amr_fab_t* flux_y, const amr_fab_t* flux_z);
+
 /* Plot variables of one level (the plotMF of CNS::writePlotFile,
  * src/CNS.cpp:644-874): into `out` (valid cells), the five state components
  * when state_vars != 0, then the derived variables whose bits are set in
