@@ -32,7 +32,8 @@ DEVICE = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-SOURCES = ["capi.cpp", "unfused.hip", "fused.hip", "fill_reduce.hip", "adjoint.hip", "dir_flux.hip", "regrid.hip"]
+SOURCES = ["capi.cpp", "unfused.hip", "fused.hip", "fill_reduce.hip", "adjoint.hip", "dir_flux.hip", "regrid.hip",
+           "user_source.cpp"]
 
 
 def _compile(src):
@@ -53,7 +54,7 @@ def build_library(jobs=4, verbose=True):
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIBNAME + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-z,defs", "-o", tmp] + objs
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-z,defs", "-o", tmp] + objs + ["-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -22,6 +22,7 @@
 
 #include "amrsolver_hip.h"
 #include "kernels.h"
+#include "user_source.h"
 
 using namespace amr;
 
@@ -257,6 +258,12 @@ struct amr_boxtab_s {
   BoxTable bt;
   std::vector<std::unique_ptr<DescSet>> descs;
   std::vector<std::unique_ptr<DescSet>> face_descs;  // face-flux arrays (amr_face_fluxes)
+  // run-time compiled user_source (amr_set_user_source) and its dSdt buffer
+  hipModule_t usrc_mod = nullptr;
+  hipFunction_t usrc_fn = nullptr;
+  double* usrc_params = nullptr;
+  double* usrc_tmp = nullptr;
+  Fab* usrc_tmp_fabs = nullptr;
   int64_t use_clock = 0;
   // pass-structured workspace
   double* uw_mem = nullptr;
@@ -1106,6 +1113,7 @@ bool fused_path(const amr_boxtab_s* bt, const amr_scheme_t& sch, FusedConfig* ou
   FusedConfig cfg = choose_config(bt, sch);
   if (out) *out = cfg;
   if (bt->walls) return false;  // the wall flux corrections run in the pass-structured kernels
+  if (sch.rhs_source && bt->usrc_fn) return false;  // the user's source runs between assembly and combination
   return cfg.tx > 0 && fused_supported(sch, cfg.tx, cfg.ty) && fused_fits(bt, cfg, level);
 }
 
@@ -1326,9 +1334,69 @@ bool use_level_tiles(amr_boxtab_s* bt, const amr_scheme_t& sch, DescSet* ds1, De
   return !choose_config(bt, sch).box_tiles && level_possible(bt, sch, ds1, dsin, dout);
 }
 
+void free_user_source(amr_boxtab_s* bt) {
+  if (bt->usrc_mod) (void)hipModuleUnload(bt->usrc_mod);
+  if (bt->usrc_params) (void)hipFree(bt->usrc_params);
+  if (bt->usrc_tmp) (void)hipFree(bt->usrc_tmp);
+  if (bt->usrc_tmp_fabs) (void)hipFree(bt->usrc_tmp_fabs);
+  bt->usrc_mod = nullptr;
+  bt->usrc_fn = nullptr;
+  bt->usrc_params = nullptr;
+  bt->usrc_tmp = nullptr;
+  bt->usrc_tmp_fabs = nullptr;
+}
+
+// A stage whose source is the run-time compiled user_source: dSdt assembled
+// by the pass-structured kernels into a buffer, the user's source added to
+// it cell by cell, then the stage combination (CNS::compute_rhs calls
+// user_source after the assembly, src/tim/CNS_advance.cpp:375-407).
+int user_source_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageArgs& sa_in,
+                      DescSet* ds1, DescSet* dsin, DescSet* dout, hipStream_t st) {
+  int rc;
+  const int nb = bt->bt.nbox;
+  if (!bt->usrc_tmp) {
+    int64_t total = 0;
+    for (auto& B : bt->bt.h_boxes) total += compact_len(B.lo, B.hi, 0, NCONS);
+    HIPCHK(hipMalloc(&bt->usrc_tmp, sizeof(double) * std::max<int64_t>(total, 1)));
+    std::vector<Fab> h(nb);
+    int64_t off = 0;
+    for (int b = 0; b < nb; ++b) {
+      const BoxDev& B = bt->bt.h_boxes[b];
+      h[b] = make_fab(compact_fab(B.lo, B.hi, 0, bt->usrc_tmp + off));
+      off += compact_len(B.lo, B.hi, 0, NCONS);
+    }
+    HIPCHK(hipMalloc(&bt->usrc_tmp_fabs, sizeof(Fab) * nb));
+    HIPCHK(hipMemcpy(bt->usrc_tmp_fabs, h.data(), sizeof(Fab) * nb, hipMemcpyHostToDevice));
+  }
+  if ((rc = ensure_unfused(bt, sch))) return rc;
+  StageArgs sa = sa_in;
+  sa.src = nullptr;
+  sa.bfx = 0.0;
+  sa.rhs_only = 1;
+  HIPCHK(launch_unfused_stage(bt->bt, bt->uw, make_closures(cls), sch, sa, bt->wall_args, nullptr, dsin->d,
+                              bt->usrc_tmp_fabs, st));
+  int64_t mx = 1;
+  for (auto& B : bt->bt.h_boxes) mx = std::max<int64_t>(mx, cells(B.lo, B.hi));
+  const unsigned gx = unsigned(std::min<int64_t>((mx + 255) / 256, 1024));
+  const BoxDev* boxes = bt->bt.d_boxes;
+  const Fab* sfab = dsin->d;
+  const Fab* rfab = bt->usrc_tmp_fabs;
+  const double* pp = bt->usrc_params;
+  amr_closures_t c = cls;
+  const double* bdt = sa_in.box_dt;
+  void* args[] = {&boxes, &sfab, &rfab, &pp, &c, &bdt};
+  HIPCHK(hipModuleLaunchKernel(bt->usrc_fn, gx, unsigned(nb), 1, 256, 1, 1, 0, st, args, nullptr));
+  HIPCHK(launch_combine(bt->bt, sa_in, ds1 ? ds1->d : nullptr, dsin->d, bt->usrc_tmp_fabs, dout->d, st));
+  return AMR_OK;
+}
+
 int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sch, const StageArgs& sa_in,
              DescSet* ds1, DescSet* dsin, DescSet* dout, const Fab* uncov_u0 = nullptr,
              hipStream_t stream = nullptr) {
+  if (sch.rhs_source && bt->usrc_fn) {
+    if (uncov_u0) return fail(AMR_ERR_ARG, "internal: uncovered ghost cells from U^n with a user source");
+    return user_source_stage(bt, cls, sch, sa_in, ds1, dsin, dout, stream ? stream : bt->ctx->stream);
+  }
   StageArgs sa = sa_in;
   if (sch.rhs_source) {
     const bool body = bt->has_problem && bt->problem.body_force_x != 0.0;
@@ -2237,6 +2305,8 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   if (bt->d_wall_blk) (void)hipFree(bt->d_wall_blk);
   bt->run.reset();
   bt->src.reset();
+  bt->face_descs.clear();
+  free_user_source(bt);
   free_fold(bt);
   bt->adj_g.reset();
   if (bt->adj_e_table) (void)hipFree(bt->adj_e_table);
@@ -3232,6 +3302,24 @@ int amr_set_source(amr_boxtab_t bt, const amr_fab_t* source) {
   if ((rc = new_desc(bt, source, &ds))) return rc;
   bt->src = std::move(ds);
   bt->graphs.clear();  // captured steps refer to the previous table
+  return AMR_OK;
+}
+
+int amr_set_user_source(amr_boxtab_t bt, const char* code, const double* params, int nparams) {
+  if (!bt) return fail(AMR_ERR_ARG, "null box table");
+  if (nparams < 0 || (nparams > 0 && !params)) return fail(AMR_ERR_ARG, "params: nparams values required");
+  drop_graphs(bt);  // captured stages launch the kernel being replaced
+  HIPCHK(hipStreamSynchronize(bt->ctx->stream));
+  free_user_source(bt);
+  if (!code) return AMR_OK;
+  HIPCHK(hipSetDevice(bt->ctx->device));
+  std::string log;
+  if (compile_user_source(code, bt->ctx->device, &bt->usrc_mod, &bt->usrc_fn, &log)) {
+    free_user_source(bt);
+    return fail(AMR_ERR_ARG, "user_source does not compile: " + log);
+  }
+  HIPCHK(hipMalloc(&bt->usrc_params, sizeof(double) * std::max(nparams, 1)));
+  if (nparams) HIPCHK(hipMemcpy(bt->usrc_params, params, sizeof(double) * nparams, hipMemcpyHostToDevice));
   return AMR_OK;
 }
 

@@ -402,6 +402,7 @@ struct StageArgs {
   int order_rk, stages_rk, stage;
   const Fab* src = nullptr;  // per-box source added to dSdt (rhs_source = 1)
   double bfx = 0.0;          // x body force of the source (0: none): rho*fx to rho*u, (rho*u)*fx to rho*E
+  int rhs_only = 0;          // write dSdt itself (the stage combination follows in a later pass)
 };
 
 __device__ __forceinline__ bool box_active(const StageArgs& sa, int b) {

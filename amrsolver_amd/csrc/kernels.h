@@ -167,6 +167,10 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
                                 const Fab* sin, const Fab* out, hipStream_t st, const Fab* const* faces = nullptr);
 // (faces non-null: the face fluxes of every direction into faces[d] over the
 // boxes' faces normal to d, instead of their assembly into `out`)
+// out = the stage combination (StageArgs) of s1, sin and the dSdt in `rhs`,
+// on the valid cells of every box.
+hipError_t launch_combine(const BoxTable& bt, const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* rhs,
+                          const Fab* out, hipStream_t st);
 
 // cons2prim of `sin` into w.prims on grow(bx,3) for every box (halo cells
 // from their periodic images when wa.on).

@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(BLK) k_assemble(const BoxDev* boxes, const Blo
     if (sa.bfx != 0.0 && n == UMX) L = L + sin[b](i, j, k, URHO) * sa.bfx;
     if (sa.bfx != 0.0 && n == UET) L = L + sin[b](i, j, k, UMX) * sa.bfx;
     double v;
-    if (sc.mode == 2) {
+    if (sc.mode == 2 || sa.rhs_only) {
       v = L;
     } else {
       double a = (sc.a != 0.0) ? s1[b](i, j, k, n) : 0.0;
@@ -451,6 +451,28 @@ hipError_t prepare_block_maps(const BoxTable& bt) {
 }
 
 // Flux phase + assembly + stage combination, reference pass structure.
+// out = the stage combination of s1, sin and the dSdt held in `rhs` (the
+// combination k_assemble applies, after a pass that changed dSdt).
+__global__ void __launch_bounds__(BLK) k_combine(const BoxDev* boxes, const BlockRef* bmap, Region reg,
+                                                 const double* box_dt, StageArgs sa, const Fab* s1, const Fab* sin,
+                                                 const Fab* rhs, const Fab* out) {
+  REGION_LOOP_BEGIN
+  const StageCoef sc = box_stage(sa, b);
+  const Fab r = rhs[b], o = out[b];
+#pragma unroll
+  for (int n = 0; n < NCONS; ++n) {
+    const double L = r(i, j, k, n);
+    double v;
+    if (sc.mode == 2) {
+      v = L;
+    } else {
+      double a = (sc.a != 0.0) ? s1[b](i, j, k, n) : 0.0;
+      v = stage_combine(sc, a, sin[b](i, j, k, n), L);
+    }
+    o(i, j, k, n) = v;
+  }
+}
+
 // dst(i,j,k,n) = src(i,j,k,n) over region reg of every box (zero when src is
 // null): the face fluxes of one direction out of the workspace.
 __global__ void __launch_bounds__(BLK) k_copy_faces(const BoxDev* boxes, const BlockRef* bmap, Region reg,
@@ -467,6 +489,15 @@ __global__ void __launch_bounds__(BLK) k_copy_faces(const BoxDev* boxes, const B
   }
 }
 
+hipError_t launch_combine(const BoxTable& bt, const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* rhs,
+                          const Fab* out, hipStream_t st) {
+  Launch L_ = grid_for(bt, grown(0));
+  if (!L_.map) return hipErrorNotInitialized;
+  hipLaunchKernelGGL(k_combine, L_.grid, dim3(BLK), 0, st, bt.d_boxes, L_.map, grown(0), sa.box_dt, sa, s1, sin,
+                     rhs, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const Closures& c,
                                 const amr_scheme_t& s, const StageArgs& sa, const WallArgs& wa, const Fab* s1,
                                 const Fab* sin, const Fab* out, hipStream_t st, const Fab* const* faces) {
@@ -481,8 +512,8 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
   const Region g3 = grown(NGHOST), nod = nodal(), g1 = grown(1), g0 = grown(0);
   // WENO or MUSCL without viscous fluxes: directional passes from the
   // conserved state, each forming what it needs of its pencil in LDS
-  if (!faces && s.rhs_euler && (s.flux_euler == 2 || s.flux_euler == 0) && !s.rhs_visc && !s.art_diss &&
-      w.weno.d[0] != nullptr)
+  if (!faces && !sa.rhs_only && s.rhs_euler && (s.flux_euler == 2 || s.flux_euler == 0) && !s.rhs_visc &&
+      !s.art_diss && w.weno.d[0] != nullptr)
     return launch_dir_flux(s.flux_euler, w.weno, B, c, sa, sin, w.nf[0], s1, out, st);
   LAUNCH(k_prims, g3, sin, w.prims, c, WrapArgs{0, {0, 0, 0}, {1, 1, 1}});
   if (s.rhs_euler) {

@@ -380,6 +380,13 @@ class Level:
         for schemes with rhs_source = 1."""
         call("amr_set_source", self.h, src.fabs if src is not None else None)
 
+    def set_user_source(self, code, params=()):
+        """The problem's user_source as HIP source text (compiled at run time
+        for this GPU), applied after the flux assembly for schemes with
+        rhs_source = 1; None removes it."""
+        arr = (ctypes.c_double * max(len(params), 1))(*params)
+        call("amr_set_user_source", self.h, None if code is None else code.encode(), arr, len(params))
+
     def set_problem(self, wall_T=0.0, body_force_x=0.0, clear=False):
         """Problem parameters: the bc 6 wall temperature and the x body force
         of the channel's user_source (added when rhs_source = 1)."""

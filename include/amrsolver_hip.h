@@ -408,6 +408,28 @@ int amr_autotune(amr_boxtab_t bt, const amr_closures_t* cls, const amr_scheme_t*
  * the caller may update them in place; NULL unregisters.  Synchronous. */
 int amr_set_source(amr_boxtab_t bt, const amr_fab_t* source);
 
+/* A problem's user_source as HIP source text (CNS::compute_rhs calls
+ * user_source(i, j, k, statefab, dsdtfab, probparm, closures, dx) on every
+ * valid cell after the flux assembly, src/tim/CNS_advance.cpp:397-407; e.g.
+ * exm/Channel/prob.h:189-193), compiled at run time for this GPU (hipRTC,
+ * -ffp-contract=off).  `code` defines
+ *   __device__ void user_source(int i, int j, int k, const StateView& state,
+ *                               const RhsView& rhs, const double* pparm,
+ *                               const amr_closures_t& cls, const double* dx);
+ * state(i,j,k,n) reads the stage input (ghost cells filled, as the
+ * reference's statefab), rhs(i,j,k,n) is the cell's dSdt (a double&), pparm
+ * a device copy of params[nparams] (the ProbParm values the code reads), dx
+ * the level's cell size; URHO, UMX, UMY, UMZ, UET name the components and
+ * amr_closures_t is this header's struct.  With a user source set, schemes
+ * with rhs_source = 1 add it (in place of amr_set_source's array and the
+ * problem's body force) in amr_rhs, amr_rk_stage, amr_advance and the
+ * captured steps of amr_advance_graph, on the pass-structured kernels: dSdt
+ * is assembled, the source applied, then the stage combined.  code NULL
+ * removes it.  A compile error returns AMR_ERR_ARG with the compiler's log in
+ * amr_last_error().  No reference counterpart beyond the hook: the
+ * reference compiles user_source into its build. */
+int amr_set_user_source(amr_boxtab_t bt, const char* code, const double* params, int nparams);
+
 /* Problem parameters of a box table (walls, body force) and the closures
  * the wall fill uses (gamma, Rspec: bcnormal's ProbClosures); problem NULL
  * clears them.  With rhs_source = 1 the body force is added after the

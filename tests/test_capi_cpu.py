@@ -53,3 +53,16 @@ def test_errors_are_reported_without_device():
     import pytest
     with pytest.raises(capi.AmrError, match="unsupported order_rk"):
         capi.stage_coeffs(4, 10, 0.1, 0)
+
+
+def test_user_source_prelude_repeats_the_closures_struct():
+    """The run-time compiled user_source sees amr_closures_t through a copy
+    of the header's definition: the two texts must agree (layout of the
+    struct passed by value to the compiled kernel)."""
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "amrsolver_hip.h")).read()
+    src = open(os.path.join(root, "amrsolver_amd", "csrc", "user_source.cpp")).read()
+    pat = re.compile(r"typedef struct \{(.*?)\} amr_closures_t;", re.S)
+    norm = lambda t: " ".join(t.split())
+    assert norm(pat.search(hdr).group(1)) == norm(pat.search(src).group(1))
