@@ -26,6 +26,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (fused multiply-add counted as two)
+# FP64 operations per cell-stage of the Sod workloads (4 boxes of 64x4x4,
+# SSPRK(3,3), the Sod initial state; MUSCL + Riemann depends on the data
+# through its branches), counted as the TGV constant is; the CPU test suite
+# recounts both
+FP64_OPS_SOD_W = 5222.2
+FP64_OPS_SOD_R = 1307.0
 # FP64 operations of the reference arithmetic per cell-stage for this
 # workload (KEEP4 + Sutherland viscous, SSPRK(3,3), 64^3 boxes), counted by
 # running the CPU restatement on an operation-counting scalar; the CPU test
@@ -159,9 +165,42 @@ def base_line(metric_value, elapsed_ms_per_step, steps, warmup, workload, extra)
     return out
 
 
+def _fp64(ops, rate):
+    """Algorithmic FP64 rate (ops per cell-stage times cell-stages/s)
+    against the FP64 vector peak."""
+    ach = ops * rate / 1e12
+    return {"ops_per_cell_stage": ops, "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / FP64_PEAK_TFLOPS}
+
+
+def _oracle_cfl_run(cls, sch, g, boxes, u, stop, nthreads):
+    """The CPU oracle to `stop` with the CFL 0.3 policy (computeNewDt on the
+    host): (steps, seconds)."""
+    from amrsolver_amd import level as LV
+    from oracle import oracle as O
+    ol = O.OracleLevel(g, boxes)
+    ol.set_state(u)
+    t, dtp, k = 0.0, None, 0
+    t0 = time.perf_counter()
+    while t < stop - 1e-14:
+        d = LV.compute_new_dt(ol.est_dt(cls, 0.3), dtp, t, stop, dtp is None)
+        dtp = d
+        ol.advance(cls, sch, d, nthreads=nthreads)
+        t += d
+        k += 1
+    return k, time.perf_counter() - t0
+
+
+def _host_threads():
+    import os
+    return len(os.sched_getaffinity(0))
+
+
 def run_sod(a, flux):
     """Sod tube 256x4x4 (4 boxes of 64x4x4), outflow in x, periodic y/z,
-    SSPRK(3,3), CFL 0.3 with the 1.1x growth cap, to t = 0.2."""
+    SSPRK(3,3), CFL 0.3 with the 1.1x growth cap, to t = 0.2, with the time
+    step chosen on the device (amr_advance_cfl_graph: captured chunks of
+    steps, the clock read back once per chunk)."""
     import torch
     from amrsolver_amd import level as LV
     from amrsolver_amd import problems as P
@@ -173,31 +212,47 @@ def run_sod(a, flux):
     lev = LV.Level(g, boxes, cls, sch)
     S = lev.new_state(layout="boxes")
     u0 = P.sod_state(cls, g, n)
+    chunk = 32
 
-    def run(max_steps):
+    def run():
         S.upload(u0)
-        t, dt, k = 0.0, None, 0
-        while t < 0.2 - 1e-14 and k < max_steps:
-            dt = LV.compute_new_dt(lev.est_time_step(S, 0.3), dt, t, 0.2, k == 0)
-            lev.advance(S, t, dt)
-            t += dt
-            k += 1
+        lev.clock_set(0.0)
+        while True:
+            lev.advance_cfl_graph(S, 0.3, 0.2, chunk)
+            t, _, k = lev.clock_get()
+            if t[0] >= 0.2 - 1e-14:
+                break
         torch.cuda.synchronize()
-        return k
+        return k[0]
 
-    run(a.warmup)
+    for _ in range(max(1, a.warmup)):
+        run()
     t0 = time.perf_counter()
-    k = run(10 ** 9)
+    k = run()
     el = time.perf_counter() - t0
+    rate = n[0] * n[1] * n[2] * 3 * k / el
     name = "Sod-W (WENO5-JS + LF)" if flux == 2 else "Sod-R (MUSCL + Riemann)"
-    return base_line(n[0] * n[1] * n[2] * 3 * k / el, el / k * 1e3, k, a.warmup,
-                     f"{name} 256x4x4, 4 boxes, SSPRK(3,3), CFL 0.3, t=0.2", {"data": "synthetic (Sod initial state)"})
+    ops = FP64_OPS_SOD_W if flux == 2 else FP64_OPS_SOD_R
+    extra = {"data": "synthetic (Sod initial state)", "fp64": _fp64(ops, rate)}
+    if not a.no_cpu:
+        # the whole run on the CPU oracle: one thread, then all host threads
+        ncell = n[0] * n[1] * n[2]
+        k1, e1 = _oracle_cfl_run(cls, sch, g, boxes, u0, 0.2, 1)
+        extra["cpu_baseline"] = {"value": ncell * 3 * k1 / e1, "unit": "cell-stage/s", "cores": 1, "kind": "port",
+                                 "sample": f"the whole run to t=0.2 ({k1} steps), 1 thread, {e1:.1f} s"}
+        nt = _host_threads()
+        k2, e2 = _oracle_cfl_run(cls, sch, g, boxes, u0, 0.2, nt)
+        extra["cpu_all_cores"] = {"value": ncell * 3 * k2 / e2, "unit": "cell-stage/s", "cores": nt,
+                                  "sample": f"the whole run to t=0.2 ({k2} steps), {nt} threads, {e2:.1f} s"}
+    return base_line(rate, el / k * 1e3, k, a.warmup,
+                     f"{name} 256x4x4, 4 boxes, SSPRK(3,3), CFL 0.3 on the device, t=0.2", extra)
 
 
 def run_riemann_batch(a, flux):
     """512 independent 1D Riemann problems, L uniform in [64, 2048] cells (x4x4),
     seed 20261202, SSPRK(3,3), CFL 0.3 per instance, to t = 0.15; one launch
-    per kernel for all instances."""
+    per kernel for all instances, each problem's time step chosen on the
+    device (amr_advance_cfl_graph, the clocks read back once per chunk)."""
     import torch
     from amrsolver_amd import level as LV
     from amrsolver_amd import problems as P
@@ -209,44 +264,59 @@ def run_riemann_batch(a, flux):
     S = bat.new_state()
     cells = [p["L"] * 16 for p in inst]
     stop = 0.15
+    chunk = 32
 
-    def run(max_steps):
+    def run():
         S.upload_problems(states)
-        nb = len(inst)
-        t = [0.0] * nb
-        dtp = [None] * nb
-        done = 0
-        k = 0
-        while k < max_steps:
-            est = bat.est_time_steps(S, 0.3)
-            dts = []
-            for b in range(nb):
-                if t[b] >= stop - 1e-14:
-                    dts.append(0.0)
-                    continue
-                d = LV.compute_new_dt(est[b], dtp[b], t[b], stop, dtp[b] is None)
-                dtp[b] = d
-                dts.append(d)
-            if not any(dts):
+        bat.clock_set(0.0)
+        while True:
+            bat.advance_cfl_graph(S, 0.3, stop, chunk)
+            t, _, k = bat.clock_get()
+            if min(t) >= stop - 1e-14:
                 break
-            bat.advance_dt(S, dts)
-            for b in range(nb):
-                if dts[b] > 0.0:
-                    t[b] += dts[b]
-                    done += cells[b] * 3
-            k += 1
         torch.cuda.synchronize()
-        return k, done
+        return k
 
-    run(a.warmup)
+    for _ in range(max(1, a.warmup)):
+        run()
     t0 = time.perf_counter()
-    k, done = run(10 ** 9)
+    k = run()
     el = time.perf_counter() - t0
+    done = sum(c * 3 * kb for c, kb in zip(cells, k))
     name = "WENO5-JS + LF" if flux == 2 else "MUSCL + Riemann"
-    return base_line(done / el, el / k * 1e3, k, a.warmup,
-                     f"batched 1D Riemann, 512 instances L in [64,2048] x4x4, {name}, SSPRK(3,3), CFL 0.3, t=0.15",
-                     {"data": "synthetic (seed 20261202, SplitMix64)",
-                      "cell_stages": done, "batch_steps": k})
+    ops = FP64_OPS_SOD_W if flux == 2 else FP64_OPS_SOD_R
+    extra = {"data": "synthetic (seed 20261202, SplitMix64)", "cell_stages": done, "batch_steps": max(k),
+             "fp64": _fp64(ops, done / el),
+             "fp64_note": "ops per cell-stage counted on the Sod state with 64x4x4 boxes"}
+    if not a.no_cpu:
+        # the instances with L <= 160 solo on the CPU oracle: one after another
+        # on one thread, then side by side on all host threads
+        import concurrent.futures as cf
+        sample = [b for b, p in enumerate(inst) if p["L"] <= 160]
+
+        def solo(b):
+            g = geoms[b]
+            kk, ee = _oracle_cfl_run(cls, sch, g, [(tuple(g.dom_lo), tuple(g.dom_hi))], states[b], stop, 1)
+            return cells[b] * 3 * kk, ee
+
+        t0 = time.perf_counter()
+        res = [solo(b) for b in sample]
+        e1 = time.perf_counter() - t0
+        work = sum(w for w, _ in res)
+        extra["cpu_baseline"] = {"value": work / e1, "unit": "cell-stage/s", "cores": 1, "kind": "port",
+                                 "sample": f"{len(sample)} instances with L <= 160 to t=0.15, one after another, "
+                                           f"1 thread, {e1:.1f} s"}
+        nt = _host_threads()
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(max_workers=nt) as pool:
+            list(pool.map(solo, sample))
+        e2 = time.perf_counter() - t0
+        extra["cpu_all_cores"] = {"value": work / e2, "unit": "cell-stage/s", "cores": nt,
+                                  "sample": f"the same {len(sample)} instances side by side, {nt} threads, "
+                                            f"{e2:.1f} s"}
+    return base_line(done / el, el / max(k) * 1e3, max(k), a.warmup,
+                     f"batched 1D Riemann, 512 instances L in [64,2048] x4x4, {name}, SSPRK(3,3), "
+                     f"CFL 0.3 on the device, t=0.15", extra)
 
 
 def adjoint_timing(a, fd_dirs=0):

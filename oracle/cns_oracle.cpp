@@ -145,6 +145,25 @@ inline Counted max_of(const Counted& a, double b) {
   return a.v < b ? Counted(b) : a;
 }
 
+// Comparisons, min/max (std::max/std::min semantics), magnitude and sign of
+// the WENO and Riemann paths for plain doubles and for counted scalars
+// (comparisons are counted; |x| and copysign are sign-bit operations).
+inline bool operator<(const Counted& a, const Counted& b) { ++g_flops.cmp; return a.v < b.v; }
+inline bool operator<(const Counted& a, double b) { ++g_flops.cmp; return a.v < b; }
+inline bool operator>(const Counted& a, double b) { ++g_flops.cmp; return a.v > b; }
+inline bool operator>=(const Counted& a, const Counted& b) { ++g_flops.cmp; return a.v >= b.v; }
+inline bool operator>=(const Counted& a, double b) { ++g_flops.cmp; return a.v >= b; }
+inline double gmax(double a, double b) { return std::max(a, b); }
+inline double gmin(double a, double b) { return std::min(a, b); }
+inline double gabs(double a) { return std::abs(a); }
+inline double gsqrt(double a) { return std::sqrt(a); }
+inline double gcopysign(double a, double b) { return std::copysign(a, b); }
+inline Counted gmax(const Counted& a, const Counted& b) { return a < b ? b : a; }
+inline Counted gmin(const Counted& a, const Counted& b) { return b < a ? b : a; }
+inline Counted gabs(const Counted& a) { return Counted(std::abs(a.v)); }
+inline Counted gsqrt(const Counted& a) { return sqrt(a); }
+inline Counted gcopysign(double a, const Counted& b) { return Counted(std::copysign(a, b.v)); }
+
 // Fortran-order view, component outermost (AMReX Array4 convention).
 template <class R>
 struct A4T {
@@ -384,15 +403,16 @@ void keepy_point(int i, int j, int k, int halfsten, const double co[3], const A4
 
 // ---------------------------------------------------------------- WENO5
 // src/rhs/High_resolution.h:83-121
-void cons2eulerflux_lambda(int i, int j, int k, const A4& u, const A4& fx, const A4& fy,
-                           const A4& fz, const A4& lam, const orc_closures& c) {
-  double rhoinv = 1.0 / u(i, j, k, URHO);
-  double momx = u(i, j, k, UMX), momy = u(i, j, k, UMY), momz = u(i, j, k, UMZ);
-  double rhoet = u(i, j, k, UET);
-  double ux = momx * rhoinv, uy = momy * rhoinv, uz = momz * rhoinv;
-  double rhoekin = 0.5 * rhoinv * (momx * momx + momy * momy + momz * momz);
-  double rhoeint = rhoet - rhoekin;
-  double P = (c.gamma - 1.0) * rhoeint;
+template <class R>
+void cons2eulerflux_lambda(int i, int j, int k, const A4T<R>& u, const A4T<R>& fx, const A4T<R>& fy,
+                           const A4T<R>& fz, const A4T<R>& lam, const orc_closures& c) {
+  R rhoinv = 1.0 / u(i, j, k, URHO);
+  R momx = u(i, j, k, UMX), momy = u(i, j, k, UMY), momz = u(i, j, k, UMZ);
+  R rhoet = u(i, j, k, UET);
+  R ux = momx * rhoinv, uy = momy * rhoinv, uz = momz * rhoinv;
+  R rhoekin = 0.5 * rhoinv * (momx * momx + momy * momy + momz * momz);
+  R rhoeint = rhoet - rhoekin;
+  R P = (c.gamma - 1.0) * rhoeint;
   fx(i, j, k, URHO) = momx;
   fx(i, j, k, UMX) = momx * ux + P;
   fx(i, j, k, UMY) = momy * ux;
@@ -408,28 +428,29 @@ void cons2eulerflux_lambda(int i, int j, int k, const A4& u, const A4& fx, const
   fz(i, j, k, UMY) = momy * uz;
   fz(i, j, k, UMZ) = momz * uz + P;
   fz(i, j, k, UET) = (rhoet + P) * uz;
-  double cs = std::sqrt(c.gamma * P * rhoinv);
-  lam(i, j, k, 0) = std::abs(std::max(std::max(ux + cs, ux - cs), ux));
-  lam(i, j, k, 1) = std::abs(std::max(std::max(uy + cs, uy - cs), uy));
-  lam(i, j, k, 2) = std::abs(std::max(std::max(uz + cs, uz - cs), uz));
+  R cs = gsqrt(c.gamma * P * rhoinv);
+  lam(i, j, k, 0) = gabs(gmax(gmax(ux + cs, ux - cs), ux));
+  lam(i, j, k, 1) = gabs(gmax(gmax(uy + cs, uy - cs), uy));
+  lam(i, j, k, 2) = gabs(gmax(gmax(uz + cs, uz - cs), uz));
 }
 
 // src/rhs/High_resolution.h:124-159 (beta[1] uses the 4th power of
 // (s1-s3), eps = 1e-40, q = 2; pow(x,2) is written x*x)
-double weno5js(const double s[5]) {
+template <class R>
+R weno5js(const R s[5]) {
   const double eps = 1e-40;
-  double beta[3], alpha[3], vl[3];
+  R beta[3], alpha[3], vl[3];
   beta[0] = (13.0 / 12.0) * (s[0] - 2.0 * s[1] + s[2]) * (s[0] - 2.0 * s[1] + s[2]) +
             0.25 * (s[0] - 4.0 * s[1] + 3.0 * s[2]) * (s[0] - 4.0 * s[1] + 3.0 * s[2]);
   beta[1] = (13.0 / 12.0) * (s[1] - 2.0 * s[2] + s[3]) * (s[1] - 2.0 * s[2] + s[3]) +
             0.25 * (s[1] - s[3]) * (s[1] - s[3]) * (s[1] - s[3]) * (s[1] - s[3]);
   beta[2] = (13.0 / 12.0) * (s[2] - 2.0 * s[3] + s[4]) * (s[2] - 2.0 * s[3] + s[4]) +
             0.25 * (3.0 * s[2] - 4.0 * s[3] + s[4]) * (3.0 * s[2] - 4.0 * s[3] + s[4]);
-  double t0 = eps + beta[0], t1 = eps + beta[1], t2 = eps + beta[2];
+  R t0 = eps + beta[0], t1 = eps + beta[1], t2 = eps + beta[2];
   alpha[0] = 0.1 / (t0 * t0);
   alpha[1] = 0.6 / (t1 * t1);
   alpha[2] = 0.3 / (t2 * t2);
-  double sum = 1.0 / (alpha[0] + alpha[1] + alpha[2]);
+  R sum = 1.0 / (alpha[0] + alpha[1] + alpha[2]);
   vl[0] = 2.0 * s[0] - 7.0 * s[1] + 11.0 * s[2];
   vl[1] = -s[1] + 5.0 * s[2] + 2.0 * s[3];
   vl[2] = 2.0 * s[2] + 5.0 * s[3] - s[4];
@@ -438,15 +459,16 @@ double weno5js(const double s[5]) {
 
 // src/rhs/High_resolution.h:162-246; one direction (unit vector di,dj,dk,
 // eigenvalue component `d`), component n
-double weno_face(int i, int j, int k, int n, int di, int dj, int dk, int d, const A4& u,
-                 const A4& pf, const A4& lam) {
+template <class R>
+R weno_face(int i, int j, int k, int n, int di, int dj, int dk, int d, const A4T<R>& u, const A4T<R>& pf,
+            const A4T<R>& lam) {
   auto at = [&](int o) { return std::array<int, 3>{i + o * di, j + o * dj, k + o * dk}; };
-  double me = lam(i - 3 * di, j - 3 * dj, k - 3 * dk, d);
+  R me = lam(i - 3 * di, j - 3 * dj, k - 3 * dk, d);
   for (int o = -2; o <= 2; ++o) {
     auto c = at(o);
-    me = std::max(me, lam(c[0], c[1], c[2], d));
+    me = gmax(me, lam(c[0], c[1], c[2], d));
   }
-  double df[6];
+  R df[6];
   for (int o = -3; o <= 2; ++o) {
     auto c = at(o);
     df[o + 3] = me * u(c[0], c[1], c[2], n);
@@ -455,13 +477,13 @@ double weno_face(int i, int j, int k, int n, int di, int dj, int dk, int d, cons
     auto c = at(o);
     return pf(c[0], c[1], c[2], n);
   };
-  double st[5];
+  R st[5];
   st[0] = 0.5 * (P(-3) + df[0]);
   st[1] = 0.5 * (P(-2) + df[1]);
   st[2] = 0.5 * (P(-1) + df[2]);
   st[3] = 0.5 * (P(0) + df[3]);
   st[4] = 0.5 * (P(1) + df[4]);
-  double flx = weno5js(st);
+  R flx = weno5js(st);
   st[4] = 0.5 * (P(-2) - df[1]);
   st[3] = 0.5 * (P(-1) - df[2]);
   st[2] = 0.5 * (P(0) - df[3]);
@@ -473,41 +495,43 @@ double weno_face(int i, int j, int k, int n, int di, int dj, int dk, int d, cons
 
 // ---------------------------------------------------------------- Riemann
 // src/rhs/Riemann.h:9-17
-double limiter(double dlft, double drgt) {
-  double dcen = 0.5 * (dlft + drgt);
-  double dsgn = std::copysign(1.0, dcen);
-  double slop = 2.0 * std::min(std::abs(dlft), std::abs(drgt));
-  double dlim = (dlft * drgt >= 0.0) ? slop : 0.0;
-  return dsgn * std::min(dlim, std::abs(dcen));
+template <class R>
+R limiter(R dlft, R drgt) {
+  R dcen = 0.5 * (dlft + drgt);
+  R dsgn = gcopysign(1.0, dcen);
+  R slop = 2.0 * gmin(gabs(dlft), gabs(drgt));
+  R dlim = (dlft * drgt >= 0.0) ? slop : R(0.0);
+  return dsgn * gmin(dlim, gabs(dcen));
 }
 
 // src/rhs/Riemann.h:19-120.  dir: 0,1,2; the normal velocity is QU+dir and
 // the two transverse velocities are (QV,QW) for x, (QU,QW) for y, (QU,QV)
 // for z.  The x slope adds 2e-40 to the sound speed, y and z add 1e-40.
-void slope_point(int i, int j, int k, int dir, const A4& dq, const A4& q, const orc_closures& c) {
+template <class R>
+void slope_point(int i, int j, int k, int dir, const A4T<R>& dq, const A4T<R>& q, const orc_closures& c) {
   int di = dir == 0, dj = dir == 1, dk = dir == 2;
   int qn = QU + dir;
   int qt1 = dir == 0 ? QV : QU;
   int qt2 = dir == 2 ? QV : QW;
   double add = dir == 0 ? 2.e-40 : 1.e-40;
   auto Q = [&](int o, int comp) { return q(i + o * di, j + o * dj, k + o * dk, comp); };
-  double cspeed = std::sqrt(c.gamma * c.Rspec * Q(0, QT)) + add;
-  double dlft = 0.5 * (Q(0, QPRES) - Q(-1, QPRES)) / cspeed - 0.5 * Q(0, QRHO) * (Q(0, qn) - Q(-1, qn));
-  double drgt = 0.5 * (Q(1, QPRES) - Q(0, QPRES)) / cspeed - 0.5 * Q(0, QRHO) * (Q(1, qn) - Q(0, qn));
-  double d0 = limiter(dlft, drgt);
-  double cs2 = cspeed * cspeed;
+  R cspeed = gsqrt(c.gamma * c.Rspec * Q(0, QT)) + add;
+  R dlft = 0.5 * (Q(0, QPRES) - Q(-1, QPRES)) / cspeed - 0.5 * Q(0, QRHO) * (Q(0, qn) - Q(-1, qn));
+  R drgt = 0.5 * (Q(1, QPRES) - Q(0, QPRES)) / cspeed - 0.5 * Q(0, QRHO) * (Q(1, qn) - Q(0, qn));
+  R d0 = limiter(dlft, drgt);
+  R cs2 = cspeed * cspeed;
   dlft = (Q(0, QRHO) - Q(-1, QRHO)) - (Q(0, QPRES) - Q(-1, QPRES)) / cs2;
   drgt = (Q(1, QRHO) - Q(0, QRHO)) - (Q(1, QPRES) - Q(0, QPRES)) / cs2;
-  double d1 = limiter(dlft, drgt);
+  R d1 = limiter(dlft, drgt);
   dlft = 0.5 * (Q(0, QPRES) - Q(-1, QPRES)) / cspeed + 0.5 * Q(0, QRHO) * (Q(0, qn) - Q(-1, qn));
   drgt = 0.5 * (Q(1, QPRES) - Q(0, QPRES)) / cspeed + 0.5 * Q(0, QRHO) * (Q(1, qn) - Q(0, qn));
-  double d2 = limiter(dlft, drgt);
+  R d2 = limiter(dlft, drgt);
   dlft = Q(0, qt1) - Q(-1, qt1);
   drgt = Q(1, qt1) - Q(0, qt1);
-  double d3 = limiter(dlft, drgt);
+  R d3 = limiter(dlft, drgt);
   dlft = Q(0, qt2) - Q(-1, qt2);
   drgt = Q(1, qt2) - Q(0, qt2);
-  double d4 = limiter(dlft, drgt);
+  R d4 = limiter(dlft, drgt);
   dq(i, j, k, 0) = d0;
   dq(i, j, k, 1) = d1;
   dq(i, j, k, 2) = d2;
@@ -516,52 +540,52 @@ void slope_point(int i, int j, int k, int dir, const A4& dq, const A4& q, const 
 }
 
 // src/rhs/Riemann.h:123-249
-void riemann_prob(double gamma, double smallp, double rl, double ul, double pl, double ut1l,
-                  double ut2l, double rr, double ur, double pr, double ut1r, double ut2r,
-                  double& flxrho, double& flxu, double& flxut, double& flxutt, double& flxe) {
+template <class R>
+void riemann_prob(double gamma, double smallp, R rl, R ul, R pl, R ut1l, R ut2l, R rr, R ur, R pr, R ut1r, R ut2r,
+                  R& flxrho, R& flxu, R& flxut, R& flxutt, R& flxe) {
   const double weakwv = 1.e-3;
   const double small = 1.e-6;
-  double clsql = gamma * pl * rl;
-  double clsqr = gamma * pr * rr;
-  double wl = std::sqrt(clsql);
-  double wr = std::sqrt(clsqr);
-  double cleft = wl / rl;
-  double cright = wr / rr;
-  double ccsmall = small * (cleft + cright);
-  double pstar = (wl * pr + wr * pl - wr * wl * (ur - ul)) / (wl + wr);
-  pstar = std::max(pstar, smallp);
-  double pstnm1 = pstar;
-  double wlsq = (0.5 * (gamma - 1.) * (pstar + pl) + pstar) * rl;
-  double wrsq = (0.5 * (gamma - 1.) * (pstar + pr) + pstar) * rr;
-  wl = std::sqrt(wlsq);
-  wr = std::sqrt(wrsq);
-  double ustarp = ul - (pstar - pl) / wl;
-  double ustarm = ur + (pstar - pr) / wr;
+  R clsql = gamma * pl * rl;
+  R clsqr = gamma * pr * rr;
+  R wl = gsqrt(clsql);
+  R wr = gsqrt(clsqr);
+  R cleft = wl / rl;
+  R cright = wr / rr;
+  R ccsmall = small * (cleft + cright);
+  R pstar = (wl * pr + wr * pl - wr * wl * (ur - ul)) / (wl + wr);
+  pstar = gmax(pstar, R(smallp));
+  R pstnm1 = pstar;
+  R wlsq = (0.5 * (gamma - 1.) * (pstar + pl) + pstar) * rl;
+  R wrsq = (0.5 * (gamma - 1.) * (pstar + pr) + pstar) * rr;
+  wl = gsqrt(wlsq);
+  wr = gsqrt(wrsq);
+  R ustarp = ul - (pstar - pl) / wl;
+  R ustarm = ur + (pstar - pr) / wr;
   pstar = (wl * pr + wr * pl - wr * wl * (ur - ul)) / (wl + wr);
-  pstar = std::max(pstar, smallp);
-  double ustar = 0.0;
+  pstar = gmax(pstar, R(smallp));
+  R ustar = 0.0;
   for (int iter = 0; iter < 3; ++iter) {
     wlsq = (0.5 * (gamma - 1.) * (pstar + pl) + pstar) * rl;
     wrsq = (0.5 * (gamma - 1.) * (pstar + pr) + pstar) * rr;
-    wl = 1. / std::sqrt(wlsq);
-    wr = 1. / std::sqrt(wrsq);
-    double ustnm1 = ustarm;
-    double ustnp1 = ustarp;
+    wl = 1. / gsqrt(wlsq);
+    wr = 1. / gsqrt(wrsq);
+    R ustnm1 = ustarm;
+    R ustnp1 = ustarp;
     ustarm = ur - (pr - pstar) * wr;
     ustarp = ul + (pl - pstar) * wl;
-    double dpditer = std::abs(pstnm1 - pstar);
-    double zp = std::abs(ustarp - ustnp1);
+    R dpditer = gabs(pstnm1 - pstar);
+    R zp = gabs(ustarp - ustnp1);
     if (zp - weakwv * cleft < 0.0) zp = dpditer * wl;
-    double zm = std::abs(ustarm - ustnm1);
+    R zm = gabs(ustarm - ustnm1);
     if (zm - weakwv * cright < 0.0) zm = dpditer * wr;
-    double zz = zp + zm;
-    double denom = dpditer / std::max(zz, ccsmall);
+    R zz = zp + zm;
+    R denom = dpditer / gmax(zz, ccsmall);
     pstnm1 = pstar;
     pstar = pstar - denom * (ustarm - ustarp);
-    pstar = std::max(pstar, smallp);
+    pstar = gmax(pstar, R(smallp));
     ustar = 0.5 * (ustarm + ustarp);
   }
-  double ro, uo, po, sgnm, utrans1, utrans2;
+  R ro, uo, po, sgnm, utrans1, utrans2;
   if (ustar > 0.) {
     ro = rl; uo = ul; po = pl; sgnm = 1.; utrans1 = ut1l; utrans2 = ut2l;
   } else if (ustar < 0.) {
@@ -574,21 +598,21 @@ void riemann_prob(double gamma, double smallp, double rl, double ul, double pl, 
     utrans1 = 0.5 * (ut1l + ut1r);
     utrans2 = 0.5 * (ut2l + ut2r);
   }
-  double wosq = (0.5 * (gamma - 1.) * (pstar + po) + pstar) * ro;
-  double co = std::sqrt(gamma * po / ro);
-  double wo = std::sqrt(wosq);
-  double dpjmp = pstar - po;
-  double rstar = ro / (1. - ro * dpjmp / wosq);
-  double cstar = std::sqrt(gamma * pstar / rstar);
-  double spout = co - sgnm * uo;
-  double spin = cstar - sgnm * uo;
+  R wosq = (0.5 * (gamma - 1.) * (pstar + po) + pstar) * ro;
+  R co = gsqrt(gamma * po / ro);
+  R wo = gsqrt(wosq);
+  R dpjmp = pstar - po;
+  R rstar = ro / (1. - ro * dpjmp / wosq);
+  R cstar = gsqrt(gamma * pstar / rstar);
+  R spout = co - sgnm * uo;
+  R spin = cstar - sgnm * uo;
   if (pstar >= po) {
     spin = wo / ro - sgnm * uo;
     spout = spin;
   }
-  double ss = std::max(spout - spin, spout + spin);
-  double frac = 0.5 * (1. + (spin + spout) / std::max(ss, ccsmall));
-  double rgdnv, ugdnv, pgdnv;
+  R ss = gmax(spout - spin, spout + spin);
+  R frac = 0.5 * (1. + (spin + spout) / gmax(ss, ccsmall));
+  R rgdnv, ugdnv, pgdnv;
   if (spout < 0.) {
     rgdnv = ro; ugdnv = uo; pgdnv = po;
   } else if (spin >= 0.) {
@@ -610,33 +634,34 @@ void riemann_prob(double gamma, double smallp, double rl, double ul, double pl, 
 // takes the left-state sound speed from T at k+1 (Riemann.h:316).  The
 // flux components are written normal-first and permuted back per direction
 // (Riemann.h:278, :306, :336).
-void riemann_point(int i, int j, int k, int dir, const A4& f, const A4& dq, const A4& q,
+template <class R>
+void riemann_point(int i, int j, int k, int dir, const A4T<R>& f, const A4T<R>& dq, const A4T<R>& q,
                    const orc_closures& c) {
   int di = dir == 0, dj = dir == 1, dk = dir == 2;
   int qn = QU + dir;
   int qt1 = dir == 0 ? QV : QU;
   int qt2 = dir == 2 ? QV : QW;
   int il = i - di, jl = j - dj, kl = k - dk;
-  double Tl = dir == 2 ? q(i, j, k + 1, QT) : q(il, jl, kl, QT);
-  double cspeed = std::sqrt(c.gamma * c.Rspec * Tl) + 1.e-40;
-  double rl = q(il, jl, kl, QRHO) + 0.5 * ((dq(il, jl, kl, 0) + dq(il, jl, kl, 2)) / cspeed + dq(il, jl, kl, 1));
-  rl = std::max(rl, 1.0e-40);
-  double ul = q(il, jl, kl, qn) + 0.5 * ((dq(il, jl, kl, 2) - dq(il, jl, kl, 0)) / q(il, jl, kl, QRHO));
-  double pl = q(il, jl, kl, QPRES) + 0.5 * (dq(il, jl, kl, 0) + dq(il, jl, kl, 2)) * cspeed;
-  pl = std::max(pl, 1.0e-40);
-  double ut1l = q(il, jl, kl, qt1) + 0.5 * dq(il, jl, kl, 3);
-  double ut2l = q(il, jl, kl, qt2) + 0.5 * dq(il, jl, kl, 4);
+  R Tl = dir == 2 ? q(i, j, k + 1, QT) : q(il, jl, kl, QT);
+  R cspeed = gsqrt(c.gamma * c.Rspec * Tl) + 1.e-40;
+  R rl = q(il, jl, kl, QRHO) + 0.5 * ((dq(il, jl, kl, 0) + dq(il, jl, kl, 2)) / cspeed + dq(il, jl, kl, 1));
+  rl = gmax(rl, R(1.0e-40));
+  R ul = q(il, jl, kl, qn) + 0.5 * ((dq(il, jl, kl, 2) - dq(il, jl, kl, 0)) / q(il, jl, kl, QRHO));
+  R pl = q(il, jl, kl, QPRES) + 0.5 * (dq(il, jl, kl, 0) + dq(il, jl, kl, 2)) * cspeed;
+  pl = gmax(pl, R(1.0e-40));
+  R ut1l = q(il, jl, kl, qt1) + 0.5 * dq(il, jl, kl, 3);
+  R ut2l = q(il, jl, kl, qt2) + 0.5 * dq(il, jl, kl, 4);
 
-  cspeed = std::sqrt(c.gamma * c.Rspec * q(i, j, k, QT)) + 1.e-40;
-  double rr = q(i, j, k, QRHO) - 0.5 * ((dq(i, j, k, 0) + dq(i, j, k, 2)) / cspeed + dq(i, j, k, 1));
-  rr = std::max(rr, 1.0e-40);
-  double ur = q(i, j, k, qn) - 0.5 * ((dq(i, j, k, 2) - dq(i, j, k, 0)) / q(i, j, k, QRHO));
-  double pr = q(i, j, k, QPRES) - 0.5 * (dq(i, j, k, 0) + dq(i, j, k, 2)) * cspeed;
-  pr = std::max(pr, 1.0e-40);
-  double ut1r = q(i, j, k, qt1) - 0.5 * dq(i, j, k, 3);
-  double ut2r = q(i, j, k, qt2) - 0.5 * dq(i, j, k, 4);
+  cspeed = gsqrt(c.gamma * c.Rspec * q(i, j, k, QT)) + 1.e-40;
+  R rr = q(i, j, k, QRHO) - 0.5 * ((dq(i, j, k, 0) + dq(i, j, k, 2)) / cspeed + dq(i, j, k, 1));
+  rr = gmax(rr, R(1.0e-40));
+  R ur = q(i, j, k, qn) - 0.5 * ((dq(i, j, k, 2) - dq(i, j, k, 0)) / q(i, j, k, QRHO));
+  R pr = q(i, j, k, QPRES) - 0.5 * (dq(i, j, k, 0) + dq(i, j, k, 2)) * cspeed;
+  pr = gmax(pr, R(1.0e-40));
+  R ut1r = q(i, j, k, qt1) - 0.5 * dq(i, j, k, 3);
+  R ut2r = q(i, j, k, qt2) - 0.5 * dq(i, j, k, 4);
 
-  double frho, fu, fut, futt, fe;
+  R frho, fu, fut, futt, fe;
   riemann_prob(c.gamma, 1.0e-40, rl, ul, pl, ut1l, ut2l, rr, ur, pr, ut1r, ut2r, frho, fu, fut,
                futt, fe);
   int cn = UMX + dir;
@@ -831,10 +856,11 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
   int rc = check_args(s, ng);
   if (rc) return rc;
   constexpr bool plain = std::is_same<R, double>::value;
-  if (!plain && s.rhs_euler && s.flux_euler != 1)
+  constexpr bool dual = std::is_same<R, Dual>::value;
+  if (dual && s.rhs_euler && s.flux_euler != 1)
     return fail("the tangent (dual-number) path covers the KEEP and viscous right-hand sides");
   if (!plain && s.rhs_euler && s.art_diss)
-    return fail("the tangent (dual-number) path does not cover JST dissipation");
+    return fail("the tangent and counting paths do not cover JST dissipation");
   Box bg = grow(bx, NGHOST);
   std::vector<R>& qbuf = scratch<R>(3);
   zero_fill(qbuf, npts(bg.lo, bg.hi) * NPRIM);
@@ -864,7 +890,7 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
     }
   };
   if (s.rhs_euler) {
-    if constexpr (!plain) {
+    if constexpr (dual) {
       double co[3];
       keep_coeffs(s.order_keep, co);
       int h = s.order_keep / 2;
@@ -872,15 +898,15 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
       wall_order_reduction();
     } else if (s.flux_euler == 2) {
       // High_resolution.h:279-319
-      std::vector<double>& lb = scratch<double>(7);
+      std::vector<R>& lb = scratch<R>(7);
       zero_fill(lb, npts(bg.lo, bg.hi) * 3);
-      A4 pf[3];
+      A4T<R> pf[3];
       for (int d = 0; d < 3; ++d) {
-        std::vector<double>& pfb = scratch<double>(4 + d);
+        std::vector<R>& pfb = scratch<R>(4 + d);
         zero_fill(pfb, npts(bg.lo, bg.hi) * NCONS);
         pf[d] = view(pfb.data(), bg.lo, bg.hi);
       }
-      A4 lam = view(lb.data(), bg.lo, bg.hi);
+      A4T<R> lam = view(lb.data(), bg.lo, bg.hi);
       for_box_par<R>(bg, [&](int i, int j, int k) { cons2eulerflux_lambda(i, j, k, u, pf[0], pf[1], pf[2], lam, c); });
       Box bn = nodal_all(bx);
       for (int n = 0; n < NCONS; ++n)
@@ -899,9 +925,9 @@ int box_flux(const orc_closures& c, const orc_scheme& s, const Box& bx, int ng, 
     } else {
       // Riemann.h:339-408
       Box bg1 = grow(bx, 1);
-      std::vector<double>& sb = scratch<double>(8);
+      std::vector<R>& sb = scratch<R>(8);
       zero_fill(sb, npts(bg1.lo, bg1.hi) * NCONS);
-      A4 sl = view(sb.data(), bg1.lo, bg1.hi);
+      A4T<R> sl = view(sb.data(), bg1.lo, bg1.hi);
       for (int d = 0; d < 3; ++d) {
         for_box_par<R>(grow_dir(bx, d, 1), [&](int i, int j, int k) { slope_point(i, j, k, d, sl, q, c); });
         for_box_par<R>(surrounding_nodes(bx, d), [&](int i, int j, int k) { riemann_point(i, j, k, d, nf[d], sl, q, c); });

@@ -175,3 +175,27 @@ def test_bench_operation_count_constant():
     c = ol.advance_flops(cls, sch, 5e-5)
     ops = (c["add"] + c["mul"] + c["div"] + c["sqrt"]) / (n[0] * n[1] * n[2] * 3)
     assert abs(ops - bench.FP64_OPS_PER_CELL_STAGE) <= 0.5, ops
+
+
+@pytest.mark.parametrize("flux,name", [(2, "FP64_OPS_SOD_W"), (0, "FP64_OPS_SOD_R")], ids=["weno", "muscl"])
+def test_counting_weno_and_riemann(flux, name):
+    """The counting instantiation of the WENO and MUSCL + Riemann paths
+    returns the plain path's values bit for bit, and bench.py's operations
+    per cell-stage of the Sod workloads are its counts."""
+    import bench
+    cls = P.closures_reference()
+    n = (256, 4, 4)
+    g = P.geom(n, (0, 0, 0), (1.0, 4 / 256, 4 / 256), (0, 1, 1), lo_bc=(2, 0, 0), hi_bc=(2, 0, 0))
+    boxes = P.decompose(n, (64, 4, 4))
+    u = P.sod_state(cls, g, n)
+    sch = P.scheme(flux_euler=flux, visc=False)
+    a = O.OracleLevel(g, boxes)
+    a.set_state(u)
+    c = a.advance_flops(cls, sch, 1e-3)
+    b = O.OracleLevel(g, boxes)
+    b.set_state(u)
+    b.advance(cls, sch, 1e-3)
+    assert np.array_equal(a.get_state(), b.get_state())
+    assert c["div"] > 0 and c["sqrt"] > 0 and c["cmp"] > 0
+    ops = (c["add"] + c["mul"] + c["div"] + c["sqrt"]) / (n[0] * n[1] * n[2] * 3)
+    assert abs(ops - getattr(bench, name)) <= 0.5, ops
