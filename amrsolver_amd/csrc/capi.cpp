@@ -1054,7 +1054,7 @@ int get_tiles(amr_boxtab_s* bt, const FusedConfig& cfg, bool level, TileTable** 
     for (const TileDev& t : logical)
       if (t.nz > 0) {
         const int lo2 = bt->srcmap.lo[2], sh = bt->srcmap.sh;
-        max_zblk = std::max(max_zblk, ((t.k0 + t.nz + 1 - lo2) >> sh) - ((t.k0 - 2 - lo2) >> sh) + 1);
+        max_zblk = std::max(max_zblk, ((t.k0 + t.nz + 2 - lo2) >> sh) - ((t.k0 - 3 - lo2) >> sh) + 1);
       }
   std::vector<int> order;
   order.reserve(n);

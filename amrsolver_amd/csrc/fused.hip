@@ -1,4 +1,4 @@
-// Fused KEEP (order 2/4) + viscous RHS and SSPRK stage update.
+// Fused KEEP (order 2/4/6) + viscous RHS and SSPRK stage update.
 //
 // One workgroup owns a TX x TY column tile of one box and marches up a
 // run of planes (2.5D blocking).  Per plane it
@@ -12,7 +12,9 @@
 //      CentralKEEP.h:40-145, plus the viscous face average,
 //      CNS_hydro_K.h:254-261) into LDS;
 //   4. computes the z face below the plane from a per-thread register queue
-//      of its own column and finalises the plane below: flux difference
+//      of its own column (order 6 also reads the column two planes up, a
+//      load issued one plane ahead) and finalises the plane below: flux
+//      difference
 //      (CNS_advance.cpp:384-390) and stage combination (CNS::advance
 //      LinComb/Saxpy/Xpay) written straight to the output.
 // HBM traffic per cell and stage: the input state (plus tile halos, mostly
@@ -86,10 +88,13 @@ __device__ __forceinline__ int self_halo_cell(int tid, int ring_idx) {
 // above its own is one it stored, so storing that plane needs no barrier;
 // 16x16 viscous tiles).  SRC: the stage may carry a user_source array or
 // a body force (the code for them is compiled out otherwise).
-// At least two waves per SIMD: the compiler would otherwise take registers
-// past 256 (into AGPRs) rather than reuse them, halving the occupancy.
+// At least two waves per SIMD for orders 2 and 4: the compiler would
+// otherwise take registers past 256 (into AGPRs) rather than reuse them,
+// halving the occupancy.  Order 6 holds more (six pairs per x/y face, two
+// more z carries): one wave per SIMD with up to 512 registers rather than
+// spills to scratch memory.
 template <int H, bool VISC, int TX, int TY, bool UCS, bool SELF, bool SRC>
-__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) k_fused(const TileDev* __restrict__ tiles, const BoxDev* __restrict__ boxes,
+__global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(H == 3 ? 1 : 2))) k_fused(const TileDev* __restrict__ tiles, const BoxDev* __restrict__ boxes,
                                                      Closures c, double co0, double co1, double co2, StageArgs sa,
                                                      WrapArgs wa, Cover cov, UncovSrc us, const Fab* __restrict__ s1f,
                                                      const Fab* __restrict__ sinf, const Fab* __restrict__ outf) {
@@ -189,10 +194,11 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))
   // UNCOV_MAX_ZBLK blocks per tile
   const int zb0 = (kfirst - us.lo[2]) >> us.sh;
   const gdouble* u0base = sbase;
+  unsigned um_own = 0u;  // the same mask for the own column (order 6)
   if (UCS) {
     const Fab U0 = us.u0[T.box];
     u0base = gptr(U0.p) + U0.off;
-    const int zb1 = (k0 + nz + 1 - us.lo[2]) >> us.sh;
+    const int zb1 = (k0 + nz + (H == 3 ? 2 : 1) - us.lo[2]) >> us.sh;
     guint* ubits = (guint*)us.bits;
 #pragma unroll
     for (int s = 0; s < C::NLOAD; ++s) {
@@ -208,6 +214,16 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))
         }
       }
       s_um[s * NT + tid] = m;
+    }
+    if (H == 3) {
+      const int bx = (min(gi, xhi) - us.lo[0]) >> us.sh, by = (min(gj, yhi) - us.lo[1]) >> us.sh;
+      const bool inxy = bx >= 0 && bx < us.nb[0] && by >= 0 && by < us.nb[1];
+      for (int zb = zb0; zb <= zb1; ++zb) {
+        if (inxy && zb >= 0 && zb < us.nb[2]) {
+          const int bit = (zb * us.nb[1] + by) * us.nb[0] + bx;
+          if ((ubits[bit >> 5] >> (bit & 31)) & 1u) um_own |= 1u << (zb - zb0);
+        }
+      }
     }
   }
   // own column for the epilogue loads, clamped into the valid region (s1
@@ -240,6 +256,16 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))
         for (int n = 0; n < 5; ++n) pre[s][n] = pb[n * S.ns + loff[s]];
     }
   };
+  // order 6: the own column's conserved state two planes above an
+  // iteration's plane c, loaded one iteration ahead into o2n
+  double o2n[5], o2[5];
+  auto load_own = [&](int p) {
+    if (wa.on) p = p < wa.lo[2] ? p + wa.len[2] : (p >= wa.lo[2] + wa.len[2] ? p - wa.len[2] : p);
+    const bool from_u0 = UCS && ((um_own >> (((p - us.lo[2]) >> us.sh) - zb0)) & 1u);
+    const gdouble* pb = (from_u0 ? u0base : sbase) + int64_t(p) * S.ks;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) o2n[n] = pb[n * S.ns + soff];
+  };
   auto store_plane = [&](int p) {
     double* sq = s_q[p & 1];
 #pragma unroll
@@ -263,6 +289,8 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))
   double pfz_prev[4] = {0, 0, 0, 0}, pfz_cur[4] = {0, 0, 0, 0};
   double fz_prev[5] = {0, 0, 0, 0, 0};
   double zcarry[5] = {0, 0, 0, 0, 0};  // z pair (c+1, c-1) terms for the next plane
+  // order 6: z pairs (c+2, c-1) of the previous plane and of the one before
+  double z3p[5] = {0, 0, 0, 0, 0}, z3pp[5] = {0, 0, 0, 0, 0};
   double xy_prev[5] = {0, 0, 0, 0, 0};
 #pragma unroll
   for (int n = 0; n < NPRIM; ++n) qm1[n] = 0.0;
@@ -273,6 +301,7 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))
   load_plane(kfirst);
   store_plane(kfirst);
   load_plane(kfirst + 1);
+  if (H == 3) load_own(kfirst + 2);
   __syncthreads();
 
   for (int cpl = kfirst; cpl <= k0 + nz; ++cpl) {
@@ -300,6 +329,13 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))
     // tile reads lies within two planes of its own)
     store_plane(pn);
     load_plane(min(pn + 1, k0 + nz + 1));
+    if (H == 3) {
+      // plane c+2 of the own column (issued one iteration ago, after that
+      // iteration's prefetch), then plane c+3 for the next iteration
+#pragma unroll
+      for (int n = 0; n < 5; ++n) o2[n] = o2n[n];
+      load_own(min(cpl + 3, k0 + nz + 2));
+    }
     const double* sq = s_q[cpl & 1];        // plane c
     const double* sqn = s_q[(cpl + 1) & 1];  // plane c+1 (stored above)
 #pragma unroll
@@ -312,18 +348,31 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))
     // planes below the tile, whose iterations have no other one
     if (!SELF || cpl <= k0 - 1) __syncthreads();
     if (cpl < k0 - 1 || (!VISC && H == 1 && cpl < k0)) {  // uniform: only loading planes below the tile
+      if (H == 3 && cpl == k0 - 2) {
+        // (k0, k0-3): the first face's (c, c-3) pair
+        double q2[NPRIM];
+        cons2prim(o2[0], o2[1], o2[2], o2[3], o2[4], c, q2);
+        zterms(q2, qm1, cv, z3p);
+      }
 #pragma unroll
       for (int n = 0; n < NPRIM; ++n) qm1[n] = qs[n];
 #pragma unroll
       for (int n = 0; n < 4; ++n) rm1[n] = rs[n];
       continue;
     }
-    // z pair (k0, k0-2) for the first face
-    if (H == 2 && cpl == k0 - 1) {
+    // z pair (k0, k0-2) for the first face; order 6 also (k0+1, k0-2)
+    if (H >= 2 && cpl == k0 - 1) {
       double qn[NPRIM];
 #pragma unroll
       for (int n = 0; n < NPRIM; ++n) qn[n] = sqn[n * PL + own];
       zterms(qn, qm1, cv, zcarry);
+      if (H == 3) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) z3pp[n] = z3p[n];
+        double q2[NPRIM];
+        cons2prim(o2[0], o2[1], o2[2], o2[3], o2[4], c, q2);
+        zterms(q2, qm1, cv, z3p);
+      }
       if (!VISC) {
 #pragma unroll
         for (int n = 0; n < NPRIM; ++n) qm1[n] = qs[n];
@@ -494,7 +543,7 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))
       double t[5];
       zterms(qs, qm1, cv, t);  // (c, c-1)
       keep_accum(co[0], t, fz);
-      if (H == 2) {
+      if (H >= 2) {
         // (c, c-2) is the previous plane's (c+1, c-1), carried in registers
         keep_accum(co[1], zcarry, fz);
         double qn[NPRIM];
@@ -502,6 +551,18 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))
         for (int n = 0; n < NPRIM; ++n) qn[n] = sqn[n * PL + own];
         zterms(qn, qm1, cv, zcarry);  // (c+1, c-1)
         keep_accum(co[1], zcarry, fz);
+      }
+      if (H == 3) {
+        // (c, c-3) and (c+1, c-2) are the (c+2, c-1) pairs of the planes
+        // two and one below
+        keep_accum(co[2], z3pp, fz);
+        keep_accum(co[2], z3p, fz);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) z3pp[n] = z3p[n];
+        double q2[NPRIM];
+        cons2prim(o2[0], o2[1], o2[2], o2[3], o2[4], c, q2);
+        zterms(q2, qm1, cv, z3p);  // (c+2, c-1)
+        keep_accum(co[2], z3p, fz);
       }
       if (VISC) {
         fz[0] += 0.5 * (0.0 + 0.0);
@@ -571,7 +632,10 @@ bool launch_shape(const amr_scheme_t& s, const TileDev* tiles, int ntiles, const
                   const double co[3], const StageArgs& sa, const WrapArgs& wa, const Cover& cov,
                   const UncovSrc& us, const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
   const bool v = s.rhs_visc != 0;
-  if (s.order_keep == 4) {
+  if (s.order_keep == 6) {
+    if (v) launch_t<3, true, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    else launch_t<3, false, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+  } else if (s.order_keep == 4) {
     if (v) launch_t<2, true, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
     else launch_t<2, false, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
   } else if (s.order_keep == 2) {
@@ -588,12 +652,12 @@ bool launch_shape(const amr_scheme_t& s, const TileDev* tiles, int ntiles, const
 bool fused_supported(const amr_scheme_t& s, int tx, int ty) {
   // any box shape: tiles overhanging a box are masked
   if (!s.rhs_euler || s.flux_euler != 1 || s.art_diss) return false;
-  if (!(s.order_keep == 2 || s.order_keep == 4)) return false;
+  if (!(s.order_keep == 2 || s.order_keep == 4 || s.order_keep == 6)) return false;
   return (tx == 16 && ty == 16) || (tx == 8 && ty == 8) || (tx == 32 && ty == 8);
 }
 
 bool fused_self_supported(const amr_scheme_t& s, int tx, int ty) {
-  return fused_supported(s, tx, ty) && s.rhs_visc && tx == 16 && ty == 16;
+  return fused_supported(s, tx, ty) && s.order_keep <= 4 && s.rhs_visc && tx == 16 && ty == 16;
 }
 
 hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const WrapArgs& wa, const Cover& cov,

@@ -37,6 +37,8 @@ FP64_OPS_SOD_R = 1307.0
 # running the CPU restatement on an operation-counting scalar; the CPU test
 # suite recounts it and checks this constant
 FP64_OPS_PER_CELL_STAGE = 754.0
+# the same count for KEEP6 + viscous
+FP64_OPS_KEEP6V = 1282.0
 N = 256
 BOX = 64
 DT = 5.0e-5
@@ -63,41 +65,43 @@ def parse():
                     help="fused-kernel tiling: auto (autotune), default, or TX,TY,TZ[,FORM] "
                          "(FORM 1: the self-loading form of 16x16 tiles)")
     ap.add_argument("--traffic", type=str, default=os.path.join(ROOT, "profiles", "traffic_tgv256_keep4v.json"))
+    ap.add_argument("--keep", type=int, default=4, choices=[4, 6],
+                    help="KEEP order of the TGV workload (the headline is order 4)")
     return ap.parse_args()
 
 
-def cpu_baseline():
+def cpu_baseline(keep=4):
     """The CPU oracle (a restatement of the reference's unfused passes) on a
     bounded sample of the same workload: TGV 128^3 in 8 boxes of 64^3, same
-    closure and scheme, 12 SSPRK(3,3) steps (about 10 s), one thread."""
+    closure and scheme, 12 SSPRK(3,3) steps (about 10 s at order 4), one
+    thread."""
     from amrsolver_amd import problems as P
     from oracle import oracle as O
     n = (128, 128, 128)
     cls = P.tgv_closures()
     g = P.geom(n, (0, 0, 0), (2 * math.pi,) * 3, (1, 1, 1))
-    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
+    sch = P.scheme(flux_euler=1, order_keep=keep, visc=True, order_rk=3, stages_rk=3)
     ol = O.OracleLevel(g, P.decompose(n, BOX))
     ol.set_state(P.tgv_state(cls, g, n))
-    steps = 12
+    steps = 12 if keep == 4 else 8
     t0 = time.perf_counter()
     for _ in range(steps):
         ol.advance(cls, sch, DT, nthreads=1)
     el = time.perf_counter() - t0
     return {"value": n[0] * n[1] * n[2] * 3 * steps / el, "unit": "cell-stage/s", "cores": 1, "kind": "port",
-            "sample": f"TGV 128^3 (8 boxes of 64^3), KEEP4+viscous, {steps} SSPRK(3,3) steps, 1 thread, "
+            "sample": f"TGV 128^3 (8 boxes of 64^3), KEEP{keep}+viscous, {steps} SSPRK(3,3) steps, 1 thread, "
                       f"{el:.1f} s"}
 
 
-def cpu_extra():
+def cpu_extra(keep=4):
     """The same oracle on all host cores the process may use (OpenMP over
     boxes, TGV 256^3 in 64 boxes of 64^3, 2 steps) and, one thread, built for
     x86-64-v3 (TGV 128^3, 6 steps); plus the host's CPU model."""
-    import os
     from amrsolver_amd import problems as P
     from oracle import oracle as O
     cls = P.tgv_closures()
-    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
-    threads = len(os.sched_getaffinity(0))
+    sch = P.scheme(flux_euler=1, order_keep=keep, visc=True, order_rk=3, stages_rk=3)
+    threads = _host_threads()
     model = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -192,8 +196,10 @@ def _oracle_cfl_run(cls, sch, g, boxes, u, stop, nthreads):
 
 
 def _host_threads():
+    """Host threads for the all-core CPU leg: the cores this process may
+    run on, at most 16 (one GPU's share of a shared host)."""
     import os
-    return len(os.sched_getaffinity(0))
+    return min(16, len(os.sched_getaffinity(0)))
 
 
 def run_sod(a, flux):
@@ -531,7 +537,7 @@ def cpu_hierarchy_baseline(n_base=64):
     cls = P.tgv_closures()
     sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
     levels = P.tgv_amr_hierarchy(n_base=n_base, nlev=2, nest=3, base_grid=32)  # 8 base boxes: the
-    threads = len(os.sched_getaffinity(0))                                        # oracle threads over boxes
+    threads = _host_threads()                                     # oracle threads over boxes
     H = O.OracleHierarchy(levels)
     H.set_state([P.tgv_state(cls, g, tuple(g.dom_hi[d] + 1 for d in range(3))) for g, _ in levels])
     dt0 = DT * 256 / n_base
@@ -703,7 +709,7 @@ def main():
     cls = P.tgv_closures()
     g = P.geom(n, (0, 0, 0), (2 * math.pi,) * 3, (1, 1, 1))
     boxes = P.decompose(n, BOX)
-    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
+    sch = P.scheme(flux_euler=1, order_keep=a.keep, visc=True, order_rk=3, stages_rk=3)
     ctx = LV.Context(0)
     lev = LV.Level(g, boxes, cls, sch, ctx=ctx)
     S = lev.new_state(layout="level")
@@ -762,11 +768,12 @@ def main():
     alg_bytes = sum(ALG_BYTES) / 3 * ncell
     mean_dur = float(dur.mean())
     achieved = alg_bytes / mean_dur / 1e9
-    fp64_tflops = FP64_OPS_PER_CELL_STAGE * ncell / mean_dur / 1e12
+    ops = FP64_OPS_PER_CELL_STAGE if a.keep == 4 else FP64_OPS_KEEP6V
+    fp64_tflops = ops * ncell / mean_dur / 1e12
     # HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE for this
     # tiling (null where no measurement of the tiling in force exists)
     traffic = None
-    if os.path.exists(a.traffic) and tiling:
+    if os.path.exists(a.traffic) and tiling and a.keep == 4:
         try:
             key = ",".join(str(x) for x in tiling) + (f",{form}" if form else "")
             entry = json.load(open(a.traffic)).get("by_tiling", {}).get(key)
@@ -786,22 +793,22 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (analytic TGV initial state)",
-        "config": {"workload": f"TGV {n[0]}^3 periodic, {len(boxes)} boxes of {BOX}^3, KEEP4+viscous, "
+        "config": {"workload": f"TGV {n[0]}^3 periodic, {len(boxes)} boxes of {BOX}^3, KEEP{a.keep}+viscous, "
                                f"SSPRK(3,3), dt={DT}", "cells": ncell, "stages_per_step": 3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
-        "fp64": {"ops_per_cell_stage": FP64_OPS_PER_CELL_STAGE, "achieved": fp64_tflops,
+        "fp64": {"ops_per_cell_stage": ops, "achieved": fp64_tflops,
                  "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": fp64_tflops / FP64_PEAK_TFLOPS},
         "stage_kernel_ms": [float(x * 1e3) for x in dur],
         "tiling": tiling,
         "kernel_form": form,
         "totals_drift": {"mass": drift[0], "energy": drift[1]},
     }
-    if not a.no_adjoint:
+    if not a.no_adjoint and a.keep == 4:
         out["adjoint"] = adjoint_timing(a)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
-        out["cpu_other"] = cpu_extra()
+        out["cpu_baseline"] = cpu_baseline(a.keep)
+        out["cpu_other"] = cpu_extra(a.keep)
     print(json.dumps(out))
     return 0
 

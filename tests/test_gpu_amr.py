@@ -130,8 +130,9 @@ def _irregular_boxes(n, seed=7, keep=0.7):
     return boxes
 
 
+@pytest.mark.parametrize("order", [4, 6], ids=["keep4v", "keep6v"])
 @pytest.mark.parametrize("case", ["irregular", "hierarchy"])
-def test_level_tiling_matches_oracle(ctx, hierarchy, case):
+def test_level_tiling_matches_oracle(ctx, hierarchy, case, order):
     """Level layout, fused kernel with tiles laid over the union of the boxes
     (crossing box boundaries; only covered cells written) and with tiles per
     box, for several tile shapes, the stages after the first reading the
@@ -149,7 +150,7 @@ def test_level_tiling_matches_oracle(ctx, hierarchy, case):
         g, boxes = hierarchy[2]
         dt = 5e-5
     cls = P.tgv_closures()
-    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=3, stages_rk=3)
+    sch = P.scheme(flux_euler=1, order_keep=order, visc=True, order_rk=3, stages_rk=3)
     lev = L.Level(g, boxes, cls, sch, ctx=ctx)
     ol = O.OracleLevel(g, boxes)
     S = lev.new_state(layout="level")
@@ -162,7 +163,7 @@ def test_level_tiling_matches_oracle(ctx, hierarchy, case):
     results, in_kernel = [], set()
     for kind in ("level", "boxes"):
         for tiling in [(16, 16, 16), (16, 16, 128), (32, 8, 32), (8, 8, 64), (8, 8, 16)]:
-            print(f"case level tiling {case} boxes {len(boxes)} tiles {kind} tiling {tiling}", flush=True)
+            print(f"case level tiling keep{order} {case} boxes {len(boxes)} tiles {kind} tiling {tiling}", flush=True)
             _boxes_state(lev, S, cls, g)
             lev.fill_patch(S)
             if kind == "level":

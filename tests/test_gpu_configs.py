@@ -84,6 +84,7 @@ def test_tgv256_two_steps_match_oracle(ctx, tgv_case, layout):
     for _ in range(2):
         lev.advance(S, 0.0, 5e-5)
     err = rel_err(S.download(), ref)
+    say(f"  max normalised error {err:.3e}")
     del S
     lev.close()
     assert err <= TOL, f"tgv256 {layout}: {err:.3e}"
@@ -150,6 +151,7 @@ def test_sod_256_to_t02(ctx, flux):
     assert t == pytest.approx(0.2, rel=1e-12)
     got = S.download()
     err = rel_err(got, ol.get_state())
+    say(f"  max normalised error {err:.3e}")
     lev.close()
     assert err <= TOL, f"sod256 flux {flux}: {err:.3e} after {step} steps"
     rho = got[0, 0, 0, :]
@@ -242,6 +244,7 @@ def test_riemann_batch_512_to_t015(ctx, flux):
         say(f"  instance {b} (L={inst[b]['L']}): {len(e_o)} steps")
         assert ests[b] == e_o, f"instance {b}: time-step sequences differ"
         err = rel_err(got[b], ref)
+        say(f"  instance {b}: max normalised error {err:.3e}")
         assert err <= TOL, f"instance {b}: {err:.3e}"
     pool.shutdown()
     assert all(abs(tb - stop) <= 1e-12 for tb in t)
@@ -298,6 +301,7 @@ def test_amr_base128_level_step_matches_oracle(ctx, amr128, lev_idx):
         say(f"  layout {layout}: tiling {tiling}")
         lev.advance(S, 0.0, dt)
         err = rel_err(_covered(S.download(), boxes), ref)
+        say(f"  max normalised error {err:.3e}")
         del S
         assert err <= TOL, f"level {lev_idx} {layout}: {err:.3e}"
     lev.close()

@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 TILINGS = [(16, 16, 64), (16, 16, 7), (8, 8, 5), (32, 8, 16)]
-KEEP = [("keep2", 2, False), ("keep4", 4, False), ("keep2v", 2, True), ("keep4v", 4, True)]
+KEEP = [("keep2", 2, False), ("keep4", 4, False), ("keep6", 6, False), ("keep2v", 2, True), ("keep4v", 4, True),
+        ("keep6v", 6, True)]
 
 
 @pytest.fixture(scope="module")
@@ -54,14 +55,15 @@ def test_fused_rhs(ctx, tiling, name, order, visc):
     assert rel_err(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("order", [4, 6], ids=["keep4v", "keep6v"])
 @pytest.mark.parametrize("order_rk,stages_rk", [(3, 3), (3, 4), (2, 3)], ids=["ssprk33", "ssprk43", "ssprk32"])
 @pytest.mark.parametrize("tiling", TILINGS[:3], ids=["x".join(map(str, t)) for t in TILINGS[:3]])
-def test_fused_advance(ctx, tiling, order_rk, stages_rk):
+def test_fused_advance(ctx, tiling, order_rk, stages_rk, order):
     from amrsolver_amd import level as L
     from oracle import oracle as O
     cls, g, boxes, u = setup(9)
-    sch = P.scheme(flux_euler=1, order_keep=4, visc=True, order_rk=order_rk, stages_rk=stages_rk)
-    print(f"case fused advance {tiling} {order_rk},{stages_rk}", flush=True)
+    sch = P.scheme(flux_euler=1, order_keep=order, visc=True, order_rk=order_rk, stages_rk=stages_rk)
+    print(f"case fused advance keep{order} {tiling} {order_rk},{stages_rk}", flush=True)
     lev = L.Level(g, boxes, cls, sch, ctx=ctx)
     lev.set_tiling(*tiling)
     A = lev.new_state()
@@ -98,9 +100,10 @@ def test_autotune_picks_a_fitting_tiling(ctx):
     lev.set_tiling(0, 0, 0)
 
 
+@pytest.mark.parametrize("order", [4, 6], ids=["keep4v", "keep6v"])
 @pytest.mark.parametrize("layout", ["level", "boxes"])
 @pytest.mark.parametrize("tiling", [(16, 16, 64), (32, 8, 8), (8, 8, 16)], ids=["16x16", "32x8", "8x8"])
-def test_fused_partial_tiles(ctx, tiling, layout):
+def test_fused_partial_tiles(ctx, tiling, layout, order):
     """Boxes whose x/y extents are not multiples of the tile (tiles overhang
     the box and are masked), including boxes narrower than one tile."""
     from amrsolver_amd import level as L
@@ -111,8 +114,8 @@ def test_fused_partial_tiles(ctx, tiling, layout):
     boxes = [((0, 0, 0), (23, 11, 11)), ((24, 0, 0), (39, 11, 11)), ((0, 12, 0), (7, 27, 11)),
              ((8, 12, 0), (39, 27, 5)), ((8, 12, 6), (39, 27, 11))]
     u = P.smooth_random_state(cls, n, 17)
-    sch = P.scheme(flux_euler=1, order_keep=4, visc=True)
-    print(f"case fused partial {tiling} {layout}", flush=True)
+    sch = P.scheme(flux_euler=1, order_keep=order, visc=True)
+    print(f"case fused partial keep{order} {tiling} {layout}", flush=True)
     lev = L.Level(g, boxes, cls, sch, ctx=ctx)
     lev.set_tiling(*tiling)
     S = lev.new_state(layout=layout)
@@ -126,14 +129,15 @@ def test_fused_partial_tiles(ctx, tiling, layout):
     assert rel_err(S.download(), ol.get_state()) <= TOL
 
 
-def test_fused_periodic_level_reads_halo_images(ctx):
+@pytest.mark.parametrize("order", [4, 6], ids=["keep4v", "keep6v"])
+def test_fused_periodic_level_reads_halo_images(ctx, order):
     """A periodic level held as one array: the fused kernel reads halo cells
     from their periodic images, so the result does not depend on the ghost
     cells (left unfilled here)."""
     from amrsolver_amd import level as L
     from oracle import oracle as O
     cls, g, boxes, u = setup(23)
-    sch = P.scheme(flux_euler=1, order_keep=4, visc=True)
+    sch = P.scheme(flux_euler=1, order_keep=order, visc=True)
     lev = L.Level(g, boxes, cls, sch, ctx=ctx)
     lev.set_tiling(16, 16, 16)
     S = lev.new_state()
