@@ -1,6 +1,8 @@
-// Euler fluxes without viscous terms, one pass per direction: WENO5-JS with
-// global Lax-Friedrichs splitting (flux_euler = 2) and MUSCL slopes with the
-// two-shock Riemann solver (flux_euler = 0).
+// Euler fluxes, one pass per direction: WENO5-JS with global Lax-Friedrichs
+// splitting (flux_euler = 2) and MUSCL slopes with the two-shock Riemann
+// solver (flux_euler = 0); with viscous terms each pass adds the face average
+// of the cell-centred viscous fluxes of its direction (computed beforehand by
+// a pass over the box grown by one cell) to its faces before differencing.
 //
 // The reference runs each scheme as full-box passes with every intermediate
 // in HBM: cons2eulerflux_lambda over the grown box and one
@@ -16,7 +18,10 @@
 //   evaluations per component) from LDS;
 // - MUSCL: primitives per cell, then the characteristic slopes of the cells
 //   next to a face, then one Riemann problem per face, all from LDS.
-// After a barrier each thread forms its cell's flux difference.  Pass x
+// With viscous terms the face flux is the Euler flux plus the average of the
+// two cells' viscous fluxes (ViscousNumericalFluxes, CNS_hydro_K.h:254-261,
+// read from HBM).  After a barrier each thread forms its cell's flux
+// difference.  Pass x
 // writes dxinv_x (F(i) - F(i+1)) to an accumulator, pass y adds its term,
 // pass z adds its term, the source and the stage combination and writes the
 // stage output, so the sum keeps the reference's order (x + y) + z.
@@ -100,11 +105,30 @@ __device__ __forceinline__ void dir_epilogue(const WenoBrick& W, const BoxDev& b
   }
 }
 
+// Viscous part of face f of a brick (f % FX along x, f / FX along the second
+// axis; the face lies below its cell in direction D): F += 0.5 (Fv(left) +
+// Fv(right)) per component (CNS_hydro_K.h:254-261, the reference's operand
+// order).  Cells beyond the box (faces of overhanging lanes, whose fluxes
+// are never used) are clamped into the viscous fluxes' grown-by-one extent.
+template <int D, int FX>
+__device__ __forceinline__ void add_visc_face(const WenoBrick& W, const BoxDev& bx, const Fab& P, int f,
+                                              double F[5]) {
+  constexpr int BA = D == 2 ? 2 : 1;
+  const int fx = f % FX, fb = f / FX;
+  int xr = min(W.x0 + fx, bx.hi[0] + 1), br = min(W.b0 + fb, bx.hi[BA] + 1);
+  const int xl = D == 0 ? xr - 1 : xr, bl = D == 0 ? br : br - 1;
+  int il, jl, kl, ir, jr, kr;
+  brick_ijk<D>(xl, bl, W.c, il, jl, kl);
+  brick_ijk<D>(xr, br, W.c, ir, jr, kr);
+#pragma unroll
+  for (int n = 0; n < 5; ++n) F[n] += 0.5 * (P(il, jl, kl, n) + P(ir, jr, kr, n));
+}
+
 template <int D, int LAST>
 __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ bricks, const BoxDev* __restrict__ boxes,
                                                   Closures c, StageArgs sa, const Fab* __restrict__ sinf,
                                                   const Fab* __restrict__ accf, const Fab* __restrict__ s1f,
-                                                  const Fab* __restrict__ outf) {
+                                                  const Fab* __restrict__ outf, const Fab* __restrict__ pvf) {
   using G = WGeo<D>;
   __shared__ double sv[G::NV][G::NL];
   const WenoBrick W = bricks[blockIdx.x];
@@ -183,6 +207,7 @@ __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ 
                        0.5 * (Pf[2] - df[2]), 0.5 * (Pf[1] - df[1]));
         F[r][n] = flx;
       }
+      if (pvf) add_visc_face<D, G::FX>(W, bx, pvf[W.box], f, F[r]);
     }
   }
   __syncthreads();  // every stencil read done: the faces reuse the cell slots
@@ -215,7 +240,8 @@ template <int D, int LAST>
 __global__ void __launch_bounds__(WNT) k_muscl_dir(const WenoBrick* __restrict__ bricks,
                                                    const BoxDev* __restrict__ boxes, Closures c, StageArgs sa,
                                                    const Fab* __restrict__ sinf, const Fab* __restrict__ accf,
-                                                   const Fab* __restrict__ s1f, const Fab* __restrict__ outf) {
+                                                   const Fab* __restrict__ s1f, const Fab* __restrict__ outf,
+                                                   const Fab* __restrict__ pvf) {
   using G = MGeo<D>;
   __shared__ double sq[NPRIM][G::NL];
   __shared__ double sd[5][G::NL];
@@ -276,6 +302,7 @@ __global__ void __launch_bounds__(WNT) k_muscl_dir(const WenoBrick* __restrict__
       auto qr = [&](int n) { return sq[n][lR]; };
       auto dqr = [&](int n) { return sd[n][lR]; };
       muscl_face<D>(ql, dql, qr, dqr, Tl, c, F[r]);
+      if (pvf) add_visc_face<D, G::FX>(W, bx, pvf[W.box], f, F[r]);
     }
   }
   __syncthreads();  // every pencil read done: the faces reuse the primitive slots
@@ -317,27 +344,30 @@ void weno_bricks(const BoxTable& bt, int d, const int* wrap, WenoBrick* out) {
 
 hipError_t launch_dir_flux(int flux_euler, const WenoWork& ww, const BoxDev* boxes, const Closures& c,
                            const StageArgs& sa, const Fab* sin, const Fab* acc, const Fab* s1, const Fab* out,
-                           hipStream_t st) {
+                           hipStream_t st, const Fab* const* pv) {
+  const Fab* pv0 = pv ? pv[0] : nullptr;
+  const Fab* pv1 = pv ? pv[1] : nullptr;
+  const Fab* pv2 = pv ? pv[2] : nullptr;
   if (flux_euler == 2) {
     if (ww.n[0] > 0)
       hipLaunchKernelGGL((k_weno_dir<0, 0>), dim3(unsigned(ww.n[0])), dim3(WNT), 0, st, ww.d[0], boxes, c, sa, sin,
-                         acc, s1, out);
+                         acc, s1, out, pv0);
     if (ww.n[1] > 0)
       hipLaunchKernelGGL((k_weno_dir<1, 0>), dim3(unsigned(ww.n[1])), dim3(WNT), 0, st, ww.d[1], boxes, c, sa, sin,
-                         acc, s1, out);
+                         acc, s1, out, pv1);
     if (ww.n[2] > 0)
       hipLaunchKernelGGL((k_weno_dir<2, 1>), dim3(unsigned(ww.n[2])), dim3(WNT), 0, st, ww.d[2], boxes, c, sa, sin,
-                         acc, s1, out);
+                         acc, s1, out, pv2);
   } else if (flux_euler == 0) {
     if (ww.n[0] > 0)
       hipLaunchKernelGGL((k_muscl_dir<0, 0>), dim3(unsigned(ww.n[0])), dim3(WNT), 0, st, ww.d[0], boxes, c, sa, sin,
-                         acc, s1, out);
+                         acc, s1, out, pv0);
     if (ww.n[1] > 0)
       hipLaunchKernelGGL((k_muscl_dir<1, 0>), dim3(unsigned(ww.n[1])), dim3(WNT), 0, st, ww.d[1], boxes, c, sa, sin,
-                         acc, s1, out);
+                         acc, s1, out, pv1);
     if (ww.n[2] > 0)
       hipLaunchKernelGGL((k_muscl_dir<2, 1>), dim3(unsigned(ww.n[2])), dim3(WNT), 0, st, ww.d[2], boxes, c, sa, sin,
-                         acc, s1, out);
+                         acc, s1, out, pv2);
   } else {
     return hipErrorInvalidValue;
   }

@@ -172,14 +172,16 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
 hipError_t launch_combine(const BoxTable& bt, const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* rhs,
                           const Fab* out, hipStream_t st);
 
-// cons2prim of `sin` into w.prims on grow(bx,3) for every box (halo cells
-// from their periodic images when wa.on).
 // The three directional passes of a stage (flux_euler 2: WENO, 0: MUSCL):
 // pass x and y accumulate the flux differences in `acc`, pass z adds its
-// own, the source and the stage combination and writes `out`.
+// own, the source and the stage combination and writes `out`.  With `pv`
+// (the cell-centred viscous fluxes per direction on grow(bx,1)) each pass
+// adds the face averages of its direction's viscous fluxes to its faces.
 hipError_t launch_dir_flux(int flux_euler, const WenoWork& ww, const BoxDev* boxes, const Closures& c,
                            const StageArgs& sa, const Fab* sin, const Fab* acc, const Fab* s1, const Fab* out,
-                           hipStream_t st);
+                           hipStream_t st, const Fab* const* pv = nullptr);
+// cons2prim of `sin` into w.prims on grow(bx,3) for every box (halo cells
+// from their periodic images when wa.on).
 hipError_t launch_prims(const BoxTable& bt, const UnfusedWork& w, const Closures& c, const double* box_dt,
                         const Fab* sin, const WrapArgs& wa, hipStream_t st);
 

@@ -510,11 +510,23 @@ hipError_t launch_unfused_stage(const BoxTable& bt, const UnfusedWork& w, const 
     hipLaunchKernelGGL(KERNEL, L_.grid, dim3(BLK), 0, st, B, L_.map, REG, bdt, __VA_ARGS__);     \
   } while (0)
   const Region g3 = grown(NGHOST), nod = nodal(), g1 = grown(1), g0 = grown(0);
-  // WENO or MUSCL without viscous fluxes: directional passes from the
-  // conserved state, each forming what it needs of its pencil in LDS
-  if (!faces && !sa.rhs_only && s.rhs_euler && (s.flux_euler == 2 || s.flux_euler == 0) && !s.rhs_visc &&
-      !s.art_diss && w.weno.d[0] != nullptr)
-    return launch_dir_flux(s.flux_euler, w.weno, B, c, sa, sin, w.nf[0], s1, out, st);
+  // WENO or MUSCL: directional passes from the conserved state, each
+  // forming what it needs of its pencil in LDS; with viscous terms the
+  // cell-centred viscous fluxes are computed first (primitives on grow(bx,3),
+  // ViscousFluxes on grow(bx,1), the wall rows' own fluxes next to bc 6
+  // walls) and each pass adds their face averages
+  if (!faces && !sa.rhs_only && s.rhs_euler && (s.flux_euler == 2 || s.flux_euler == 0) && !s.art_diss &&
+      w.weno.d[0] != nullptr) {
+    if (!s.rhs_visc) return launch_dir_flux(s.flux_euler, w.weno, B, c, sa, sin, w.nf[0], s1, out, st);
+    if (w.pv[0] != nullptr) {
+      LAUNCH(k_prims, g3, sin, w.prims, c, WrapArgs{0, {0, 0, 0}, {1, 1, 1}});
+      LAUNCH(k_visc_pf, g1, w.prims, w.pv[0], w.pv[1], w.pv[2], c);
+      if (wa.nvisc_blk > 0)
+        hipLaunchKernelGGL(k_visc_wall, dim3(unsigned(wa.nvisc_blk)), dim3(BLK), 0, st, wa.visc, wa.visc_blk, B,
+                           w.prims, w.pv[0], w.pv[1], w.pv[2], c);
+      return launch_dir_flux(s.flux_euler, w.weno, B, c, sa, sin, w.nf[0], s1, out, st, w.pv);
+    }
+  }
   LAUNCH(k_prims, g3, sin, w.prims, c, WrapArgs{0, {0, 0, 0}, {1, 1, 1}});
   if (s.rhs_euler) {
     if (s.flux_euler == 1) {
