@@ -79,6 +79,8 @@ SIGNATURES = {
     "amr_fill_boundary": ([_P, _PF], ctypes.c_int),
     "amr_fill_two_levels": ([_P, _P, ctypes.c_int, ctypes.c_double, _PF, ctypes.c_double, _PF, ctypes.c_int,
                              _PF], ctypes.c_int),
+    "amr_fill_two_levels_cf": ([_P, _P, ctypes.c_int, ctypes.c_double, _PF, ctypes.c_double, _PF, ctypes.c_int,
+                                _PF], ctypes.c_int),
     "amr_average_down": ([_P, _P, ctypes.c_int, _PF, _PF], ctypes.c_int),
     "amr_set_user_source": ([_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int], ctypes.c_int),
     "amr_face_fluxes": ([_P, ctypes.POINTER(Closures), ctypes.POINTER(Scheme), _PF, _PF, _PF, _PF], ctypes.c_int),

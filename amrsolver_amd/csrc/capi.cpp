@@ -283,6 +283,8 @@ struct amr_boxtab_s {
   Cover cover{};                      // level tiling (on = 0: not available)
   std::vector<unsigned> cover_bits;   // host copy of cover.bits
   unsigned* d_cover_bits = nullptr;
+  NbMap nbmap{};            // owner map for direct neighbour reads (on = 0: not available)
+  int* d_owner = nullptr;
   unsigned* d_all_covered = nullptr;
   // problem parameters (bc 6 walls, body force) and the wall flux tasks
   bool has_problem = false;
@@ -1172,6 +1174,57 @@ int build_cover(amr_boxtab_s* bt) {
   return AMR_OK;
 }
 
+// Owner map of the level's domain for the fused kernel's direct neighbour
+// reads (NbMap): blocks of 2^sh cells a side, sh the largest power of two
+// (at most 2^6) dividing every box boundary's and every periodic length's
+// offset from the domain's lower corner.  Built with the box table for a
+// level whose boxes keep NGHOST cells away from every non-periodic domain
+// boundary (their halos are then inside the domain or periodic images);
+// off for batches and for maps over 2^24 blocks.
+int build_nbmap(amr_boxtab_s* bt) {
+  bt->nbmap = NbMap{};
+  if (bt->batch || bt->bt.nbox == 0) return AMR_OK;
+  const amr_geom_t& g = bt->geom;
+  const auto& B = bt->bt.h_boxes;
+  NbMap m{};
+  unsigned msk = 1u << 6;
+  for (int d = 0; d < 3; ++d) {
+    m.dlo[d] = g.dom_lo[d];
+    m.len[d] = g.dom_hi[d] - g.dom_lo[d] + 1;
+    m.per[d] = g.periodic[d] ? 1 : 0;
+    if (m.per[d] && m.len[d] < 2 * NGHOST) return AMR_OK;  // a halo would wrap more than once
+    if (m.per[d]) msk |= unsigned(m.len[d]);
+    for (const BoxDev& b : B) {
+      if (b.lo[d] < g.dom_lo[d] || b.hi[d] > g.dom_hi[d]) return AMR_OK;
+      if (!m.per[d] && (b.lo[d] - g.dom_lo[d] < NGHOST || g.dom_hi[d] - b.hi[d] < NGHOST)) return AMR_OK;
+      msk |= unsigned(b.lo[d] - g.dom_lo[d]) | unsigned(b.hi[d] + 1 - g.dom_lo[d]);
+    }
+  }
+  m.sh = __builtin_ctz(msk);
+  int64_t nblk = 1;
+  for (int d = 0; d < 3; ++d) {
+    m.nb[d] = (m.len[d] + (1 << m.sh) - 1) >> m.sh;
+    nblk *= m.nb[d];
+  }
+  if (nblk > (int64_t(1) << 24)) return AMR_OK;
+  std::vector<int> own(size_t(nblk), -1);
+  for (int i = 0; i < bt->bt.nbox; ++i) {
+    const BoxDev& b = B[i];
+    for (int bz = (b.lo[2] - m.dlo[2]) >> m.sh; bz <= (b.hi[2] - m.dlo[2]) >> m.sh; ++bz)
+      for (int by = (b.lo[1] - m.dlo[1]) >> m.sh; by <= (b.hi[1] - m.dlo[1]) >> m.sh; ++by)
+        for (int bx = (b.lo[0] - m.dlo[0]) >> m.sh; bx <= (b.hi[0] - m.dlo[0]) >> m.sh; ++bx)
+          own[size_t((int64_t(bz) * m.nb[1] + by) * m.nb[0] + bx)] = i;
+  }
+  if (bt->d_owner) HIPCHK(hipFree(bt->d_owner));
+  bt->d_owner = nullptr;
+  HIPCHK(hipMalloc(&bt->d_owner, sizeof(int) * own.size()));
+  HIPCHK(hipMemcpy(bt->d_owner, own.data(), sizeof(int) * own.size(), hipMemcpyHostToDevice));
+  m.owner = bt->d_owner;
+  m.on = 1;
+  bt->nbmap = m;
+  return AMR_OK;
+}
+
 // Block map of the ghost cells no box covers (UncovSrc): blocks of 2^sh
 // cells, sh the cover's block exponent lowered until the domain boundaries
 // fall on block boundaries too (then every cell of a block shares the block's
@@ -1334,6 +1387,13 @@ bool use_level_tiles(amr_boxtab_s* bt, const amr_scheme_t& sch, DescSet* ds1, De
   return !choose_config(bt, sch).box_tiles && level_possible(bt, sch, ds1, dsin, dout);
 }
 
+// Whether the fused stages with tiles per box read the halo cells other
+// boxes hold from those boxes directly (the stage inputs then need no
+// same-level ghost copies): KEEP order 2 or 4 on a level with an owner map.
+bool nb_direct(const amr_boxtab_s* bt, const amr_scheme_t& sch, bool level) {
+  return !level && bt->nbmap.on && sch.order_keep <= 4 && fused_path(bt, sch, nullptr, level);
+}
+
 void free_user_source(amr_boxtab_s* bt) {
   if (bt->usrc_mod) (void)hipModuleUnload(bt->usrc_mod);
   if (bt->usrc_params) (void)hipFree(bt->usrc_params);
@@ -1423,8 +1483,9 @@ int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sc
       us.u0 = uncov_u0;
     }
     const Cover cov = kernel_cover(bt, level);
+    const NbMap nbm = !uncov_u0 && nb_direct(bt, sch, level) ? bt->nbmap : NbMap{};
     HIPCHK(launch_fused_stage(sch, cfg, wa, cov, us, tt->d, tt->n, bt->bt.d_boxes, c, sa, s1, dsin->d, dout->d,
-                              st));
+                              st, nbm));
     return AMR_OK;
   }
   if (uncov_u0) return fail(AMR_ERR_ARG, "internal: uncovered ghost cells from U^n requested off the fused path");
@@ -1524,6 +1585,7 @@ int uncovered_from_un(amr_boxtab_s* bt, const amr_scheme_t& sch, DescSet* dst, D
   if (!bt->srcmap.on || wrap_ok(bt, dst)) return AMR_OK;
   const bool level = use_level_tiles(bt, sch, dst, dst, dw[0]);
   if (level && !level_ok(bt, dw[1])) return AMR_OK;
+  if (nb_direct(bt, sch, level)) return AMR_OK;  // those stages read uncovered ghosts from their own arrays
   FusedConfig cfg;
   if (!fused_path(bt, sch, &cfg, level)) return AMR_OK;
   TileTable* tt = nullptr;
@@ -1579,7 +1641,9 @@ int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t
     // directional flux passes read the periodic images of a batch's problems
     // themselves
     const bool fused = fused_path(bt, sch, nullptr);
-    if (!(fused && wrap_ok(bt, sin)) && !(scratch_out && s == 0))
+    // nor do fused stages whose tiles read the other boxes' cells directly
+    const bool direct_nb = nb_direct(bt, sch, use_level_tiles(bt, sch, dst, sin, out));
+    if (!(fused && wrap_ok(bt, sin)) && !direct_nb && !(scratch_out && s == 0))
       if ((rc = enqueue_fill(bt, sin, bt->batch && !fused && dir_flux_scheme(sch), nullptr, narrow && fused ? 2 : 0)))
         return rc;
     StageArgs sa{sc, d_box_dt, sch.order_rk, sch.stages_rk, s};
@@ -2280,6 +2344,7 @@ int finish_boxtab(amr_boxtab_s* bt) {
   if (e != hipSuccess) return fail(AMR_ERR_HIP, std::string("box table allocation: ") + hipGetErrorString(e));
   rc = build_uncovered(bt);
   if (rc == AMR_OK) rc = build_cover(bt);
+  if (rc == AMR_OK) rc = build_nbmap(bt);
   if (rc == AMR_OK) rc = build_srcmap(bt);
   if (rc == AMR_OK) rc = build_walls(bt);
   return rc;
@@ -2299,6 +2364,7 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   if (bt->d_uncov) (void)hipFree(bt->d_uncov);
   if (bt->d_uncov_blk) (void)hipFree(bt->d_uncov_blk);
   if (bt->d_cover_bits) (void)hipFree(bt->d_cover_bits);
+  if (bt->d_owner) (void)hipFree(bt->d_owner);
   if (bt->d_srcmap_bits) (void)hipFree(bt->d_srcmap_bits);
   if (bt->d_all_covered) (void)hipFree(bt->d_all_covered);
   if (bt->d_wall_tasks) (void)hipFree(bt->d_wall_tasks);
@@ -2559,8 +2625,10 @@ int build_cf_avg(amr_boxtab_s* F, amr_boxtab_s* C, int r, CfTable* t) {
 
 }  // namespace
 
-int amr_fill_two_levels(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, double a_old, const amr_fab_t* crse_old,
-                        double b_new, const amr_fab_t* crse_new, int ngrow, const amr_fab_t* fine_state) {
+namespace {
+int fill_two_levels(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, double a_old, const amr_fab_t* crse_old,
+                    double b_new, const amr_fab_t* crse_new, int ngrow, const amr_fab_t* fine_state,
+                    bool same_level) {
   if (!fine || !crse || !fine_state) return fail(AMR_ERR_ARG, "null argument");
   if (ngrow < 1 || ngrow > NGHOST) return fail(AMR_ERR_ARG, "ngrow must be 1, 2 or 3");
   if (!std::isfinite(a_old) || !std::isfinite(b_new)) return fail(AMR_ERR_ARG, "time weights must be finite");
@@ -2582,12 +2650,27 @@ int amr_fill_two_levels(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, double 
   hipStream_t st = fine->ctx->stream;
   if (fine->walls && !fine->has_problem)
     return fail(AMR_ERR_ARG, "bc code 6 walls need amr_set_problem (wall temperature) first");
-  HIPCHK(launch_fill_copy(tk->d_copy, tk->ncopy, tk->d_copy_blk, tk->ncopy_blk, dsf->d, st));
+  if (same_level) HIPCHK(launch_fill_copy(tk->d_copy, tk->ncopy, tk->d_copy_blk, tk->ncopy_blk, dsf->d, st));
   const bool blend = use_old && use_new;
   const Fab* c0 = use_old ? d0->d : d1->d;
   HIPCHK(launch_interp_cf(t->d_interp, t->d_interp_blk, t->ninterp_blk, c0, blend ? d1->d : c0, blend, a_old,
                           b_new, ratio, dsf->d, st));
   return enqueue_fill(fine, dsf, true, nullptr, width);
+}
+}  // namespace
+
+int amr_fill_two_levels(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, double a_old, const amr_fab_t* crse_old,
+                        double b_new, const amr_fab_t* crse_new, int ngrow, const amr_fab_t* fine_state) {
+  return fill_two_levels(fine, crse, ratio, a_old, crse_old, b_new, crse_new, ngrow, fine_state, true);
+}
+
+int amr_fill_two_levels_cf(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, double a_old,
+                           const amr_fab_t* crse_old, double b_new, const amr_fab_t* crse_new, int ngrow,
+                           const amr_fab_t* fine_state) {
+  // without an owner map the level may have physical boundaries, whose
+  // ghost cells next to other boxes depend on the same-level copies
+  const bool same = !(fine && fine->nbmap.on);
+  return fill_two_levels(fine, crse, ratio, a_old, crse_old, b_new, crse_new, ngrow, fine_state, same);
 }
 
 int amr_average_down(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, const amr_fab_t* fine_state,
@@ -3055,7 +3138,7 @@ int amr_step_plan(amr_boxtab_t bt, const amr_scheme_t* sch, const amr_fab_t* sta
   info[0] = fused ? cfg.tx : -1;
   info[1] = fused ? cfg.ty : 0;
   info[2] = fused ? cfg.tz : 0;
-  info[3] = fused ? (level ? 2 : 1) : 0;
+  info[3] = fused ? (level ? 2 : (nb_direct(bt, *sch, level) ? 3 : 1)) : 0;
   info[4] = 0;
   info[5] = fused ? cfg.form : 0;
   int nst = 0;

@@ -87,18 +87,24 @@ __device__ __forceinline__ int self_halo_cell(int tid, int ring_idx) {
 // SELF: the self-loading form (every cell a thread reads from the plane
 // above its own is one it stored, so storing that plane needs no barrier;
 // 16x16 viscous tiles).  SRC: the stage may carry a user_source array or
-// a body force (the code for them is compiled out otherwise).
+// a body force (the code for them is compiled out otherwise).  NB: tiles
+// per box read every halo cell that another box of the level holds (its
+// periodic image included) from that box's array, through an owner map of
+// the level's blocks; halo cells no box holds come from the box's own ghost
+// cells.  The stage inputs then need no same-level ghost copies.
 // At least two waves per SIMD for orders 2 and 4: the compiler would
 // otherwise take registers past 256 (into AGPRs) rather than reuse them,
 // halving the occupancy.  Order 6 holds more (six pairs per x/y face, two
 // more z carries): one wave per SIMD with up to 512 registers rather than
 // spills to scratch memory.
-template <int H, bool VISC, int TX, int TY, bool UCS, bool SELF, bool SRC>
+template <int H, bool VISC, int TX, int TY, bool UCS, bool SELF, bool SRC, bool NB>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(H == 3 ? 1 : 2))) k_fused(const TileDev* __restrict__ tiles, const BoxDev* __restrict__ boxes,
                                                      Closures c, double co0, double co1, double co2, StageArgs sa,
-                                                     WrapArgs wa, Cover cov, UncovSrc us, const Fab* __restrict__ s1f,
-                                                     const Fab* __restrict__ sinf, const Fab* __restrict__ outf) {
+                                                     WrapArgs wa, Cover cov, UncovSrc us, NbMap nbm,
+                                                     const Fab* __restrict__ s1f, const Fab* __restrict__ sinf,
+                                                     const Fab* __restrict__ outf) {
   using C = Cfg<H, VISC, TX, TY>;
+  static_assert(!NB || (!UCS && H <= 2), "neighbour reads take tiles per box of KEEP order 2 or 4");
   constexpr int RX = C::RX, HX = C::HX, PL = C::PL, NT = C::NT;
   static_assert(!SELF || (VISC && TX == 16 && TY == 16 && C::NLOAD == 2 && C::EARLY),
                 "the self-loading form takes 16x16 viscous tiles");
@@ -188,6 +194,43 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(H =
     }
     loff[s] = x + y * int(S.js);
   }
+  // NB: per slot, the element address of its column in the array that
+  // holds it (plane 0, the periodic shift of the current block folded in)
+  // and that array's plane and component strides; refreshed when the
+  // plane enters another block of the owner map (uniform over the tile)
+  const gdouble* nsrc[NB ? C::NLOAD : 1];
+  int nks[NB ? C::NLOAD : 1], nns[NB ? C::NLOAD : 1];
+  int nzb = INT_MIN;
+  auto nb_refresh = [&](int p) {
+    const int zb = (p - nbm.dlo[2]) >> nbm.sh;
+    if (zb == nzb) return;
+    nzb = zb;
+    int pw = p, zbw = zb;
+    if (nbm.per[2]) {
+      pw = p < nbm.dlo[2] ? p + nbm.len[2] : (p >= nbm.dlo[2] + nbm.len[2] ? p - nbm.len[2] : p);
+      zbw = (pw - nbm.dlo[2]) >> nbm.sh;
+    }
+#pragma unroll
+    for (int s = 0; s < C::NLOAD; ++s) {
+      const int cell = load_cell(s);
+      const int x = min(T.i0 - HX + cell % RX, xcap), y = min(T.j0 - HX + cell / RX, ycap);
+      int xw = x, yw = y;
+      if (nbm.per[0]) xw = x < nbm.dlo[0] ? x + nbm.len[0] : (x >= nbm.dlo[0] + nbm.len[0] ? x - nbm.len[0] : x);
+      if (nbm.per[1]) yw = y < nbm.dlo[1] ? y + nbm.len[1] : (y >= nbm.dlo[1] + nbm.len[1] ? y - nbm.len[1] : y);
+      const int ob = nbm.owner[(int64_t(zbw) * nbm.nb[1] + ((yw - nbm.dlo[1]) >> nbm.sh)) * nbm.nb[0] +
+                               ((xw - nbm.dlo[0]) >> nbm.sh)];
+      if (ob >= 0) {
+        const Fab F = sinf[ob];
+        nsrc[s] = gptr(F.p) + F.off + xw + int64_t(yw) * F.js + int64_t(pw - p) * F.ks;
+        nks[s] = int(F.ks);
+        nns[s] = int(F.ns);
+      } else {
+        nsrc[s] = sbase + x + int64_t(y) * S.js;
+        nks[s] = int(S.ks);
+        nns[s] = int(S.ns);
+      }
+    }
+  };
   // ghost cells no box covers come from U^n (UCS): bit b of the lane's mask
   // for load cell s (in LDS, which has room where registers have not) is
   // set when the cell is such a ghost cell in plane block zb0 + b; at most
@@ -233,6 +276,16 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(H =
   const int s1off = min(gi, xhi) + min(gj, yhi) * int(S1.js);
   double pre[C::NLOAD][5];
   auto load_plane = [&](int p) {
+    if (NB) {
+      nb_refresh(p);
+#pragma unroll
+      for (int s = 0; s < C::NLOAD; ++s) {
+        const gdouble* pb = nsrc[s] + int64_t(p) * nks[s];
+#pragma unroll
+        for (int n = 0; n < 5; ++n) pre[s][n] = pb[n * nns[s]];
+      }
+      return;
+    }
     if (wa.on) p = p < wa.lo[2] ? p + wa.len[2] : (p >= wa.lo[2] + wa.len[2] ? p - wa.len[2] : p);
     if (UCS) {
       const int b = ((p - us.lo[2]) >> us.sh) - zb0;
@@ -608,21 +661,28 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(H =
   }
 }
 
-#define AMR_FUSED_LAUNCH(UCS_, SELF_, SRC_)                                                                   \
-  hipLaunchKernelGGL((k_fused<H, VISC, TX, TY, UCS_, SELF_, SRC_>), dim3(ntiles), dim3(TX * TY), 0, st, tiles, boxes, \
-                     c, co[0], co[1], co[2], sa, wa, cov, us, s1, sin, out)
+#define AMR_FUSED_LAUNCH(UCS_, SELF_, SRC_, NB_)                                                              \
+  hipLaunchKernelGGL((k_fused<H, VISC, TX, TY, UCS_, SELF_, SRC_, NB_>), dim3(ntiles), dim3(TX * TY), 0, st, tiles, \
+                     boxes, c, co[0], co[1], co[2], sa, wa, cov, us, nbm, s1, sin, out)
 
 template <int H, bool VISC, int TX, int TY, bool SELF>
 void launch_t(const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c, const double co[3],
-              const StageArgs& sa, const WrapArgs& wa, const Cover& cov, const UncovSrc& us, const Fab* s1,
-              const Fab* sin, const Fab* out, hipStream_t st) {
+              const StageArgs& sa, const WrapArgs& wa, const Cover& cov, const UncovSrc& us, const NbMap& nbm,
+              const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
   const bool src = sa.src != nullptr || sa.bfx != 0.0;
+  if constexpr (H <= 2) {
+    if (nbm.on) {
+      if (src) AMR_FUSED_LAUNCH(false, SELF, true, true);
+      else AMR_FUSED_LAUNCH(false, SELF, false, true);
+      return;
+    }
+  }
   if (us.on) {
-    if (src) AMR_FUSED_LAUNCH(true, SELF, true);
-    else AMR_FUSED_LAUNCH(true, SELF, false);
+    if (src) AMR_FUSED_LAUNCH(true, SELF, true, false);
+    else AMR_FUSED_LAUNCH(true, SELF, false, false);
   } else {
-    if (src) AMR_FUSED_LAUNCH(false, SELF, true);
-    else AMR_FUSED_LAUNCH(false, SELF, false);
+    if (src) AMR_FUSED_LAUNCH(false, SELF, true, false);
+    else AMR_FUSED_LAUNCH(false, SELF, false, false);
   }
 }
 #undef AMR_FUSED_LAUNCH
@@ -630,17 +690,18 @@ void launch_t(const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closu
 template <int TX, int TY>
 bool launch_shape(const amr_scheme_t& s, const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closures& c,
                   const double co[3], const StageArgs& sa, const WrapArgs& wa, const Cover& cov,
-                  const UncovSrc& us, const Fab* s1, const Fab* sin, const Fab* out, hipStream_t st) {
+                  const UncovSrc& us, const NbMap& nbm, const Fab* s1, const Fab* sin, const Fab* out,
+                  hipStream_t st) {
   const bool v = s.rhs_visc != 0;
   if (s.order_keep == 6) {
-    if (v) launch_t<3, true, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
-    else launch_t<3, false, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    if (v) launch_t<3, true, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, nbm, s1, sin, out, st);
+    else launch_t<3, false, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, nbm, s1, sin, out, st);
   } else if (s.order_keep == 4) {
-    if (v) launch_t<2, true, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
-    else launch_t<2, false, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    if (v) launch_t<2, true, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, nbm, s1, sin, out, st);
+    else launch_t<2, false, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, nbm, s1, sin, out, st);
   } else if (s.order_keep == 2) {
-    if (v) launch_t<1, true, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
-    else launch_t<1, false, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    if (v) launch_t<1, true, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, nbm, s1, sin, out, st);
+    else launch_t<1, false, TX, TY, false>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, nbm, s1, sin, out, st);
   } else {
     return false;
   }
@@ -663,21 +724,22 @@ bool fused_self_supported(const amr_scheme_t& s, int tx, int ty) {
 hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const WrapArgs& wa, const Cover& cov,
                               const UncovSrc& us, const TileDev* tiles, int ntiles, const BoxDev* boxes,
                               const Closures& c, const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* out,
-                              hipStream_t st) {
+                              hipStream_t st, const NbMap& nbm) {
+  if (nbm.on && (us.on || s.order_keep > 4)) return hipErrorInvalidValue;
   double co[3];
   keep_coefficients(s.order_keep, co);
   bool ok = false;
   if (cfg.form == 1) {
     if (!fused_self_supported(s, cfg.tx, cfg.ty)) return hipErrorInvalidValue;
-    if (s.order_keep == 4) launch_t<2, true, 16, 16, true>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
-    else launch_t<1, true, 16, 16, true>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    if (s.order_keep == 4) launch_t<2, true, 16, 16, true>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, nbm, s1, sin, out, st);
+    else launch_t<1, true, 16, 16, true>(tiles, ntiles, boxes, c, co, sa, wa, cov, us, nbm, s1, sin, out, st);
     ok = true;
   } else if (cfg.tx == 16 && cfg.ty == 16)
-    ok = launch_shape<16, 16>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    ok = launch_shape<16, 16>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, us, nbm, s1, sin, out, st);
   else if (cfg.tx == 8 && cfg.ty == 8)
-    ok = launch_shape<8, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    ok = launch_shape<8, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, us, nbm, s1, sin, out, st);
   else if (cfg.tx == 32 && cfg.ty == 8)
-    ok = launch_shape<32, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, us, s1, sin, out, st);
+    ok = launch_shape<32, 8>(s, tiles, ntiles, boxes, c, co, sa, wa, cov, us, nbm, s1, sin, out, st);
   if (!ok) return hipErrorInvalidValue;
   return hipGetLastError();
 }

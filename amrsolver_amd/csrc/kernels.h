@@ -85,6 +85,20 @@ struct Cover {
   const unsigned* bits;
 };
 
+// Owner map of a level for direct neighbour reads by the fused kernel's
+// tiles per box: the domain [dlo, dlo+len) in blocks of 2^sh cells a side
+// (nb per direction, x fastest), owner[block] = the box holding the block or
+// -1; per[d] marks periodic directions (halo coordinates wrap there).  Every
+// box boundary lies on a block boundary, and no box lies within NGHOST cells
+// of a non-periodic domain boundary (its halo cells are all inside the
+// domain or periodic images).
+struct NbMap {
+  int on;
+  int dlo[3], len[3], per[3], nb[3];
+  int sh;
+  const int* owner;
+};
+
 // Ghost-fill tasks.  Copy: dst(i,j,k) = src(i - sh) for (i,j,k) in [lo,hi].
 struct CopyTask {
   int dst, src;
@@ -375,6 +389,6 @@ bool fused_self_supported(const amr_scheme_t& s, int tx, int ty);
 hipError_t launch_fused_stage(const amr_scheme_t& s, const FusedConfig& cfg, const WrapArgs& wa, const Cover& cov,
                               const UncovSrc& us, const TileDev* tiles, int ntiles, const BoxDev* boxes,
                               const Closures& c, const StageArgs& sa, const Fab* s1, const Fab* sin, const Fab* out,
-                              hipStream_t st);
+                              hipStream_t st, const NbMap& nbm = NbMap{});
 
 }  // namespace amr

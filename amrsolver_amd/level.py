@@ -294,12 +294,15 @@ class Level:
         call("amr_advance", self.h, ctypes.byref(self.closures), ctypes.byref(self.scheme), S.fabs, time, dt)
 
     # --- coarse-fine coupling (this level is the fine one) ---------------
-    def fill_two_levels(self, S, coarse, crse_old, crse_new, a, b, ratio=2, ngrow=3):
+    def fill_two_levels(self, S, coarse, crse_old, crse_new, a, b, ratio=2, ngrow=3, same_level=True):
         """FillPatch of a fine level (its `ngrow` inner ghost layers):
         same-level copies, coarse-fine ghost cells interpolated from
         a*crse_old + b*crse_new (states of `coarse` with filled ghost cells),
-        physical BCs."""
-        call("amr_fill_two_levels", self.h, coarse.h, ratio, a, None if crse_old is None else crse_old.fabs, b,
+        physical BCs.  same_level=False leaves out the same-level copies
+        where the level's fused stages read the other boxes' cells directly
+        (amr_fill_two_levels_cf)."""
+        name = "amr_fill_two_levels" if same_level else "amr_fill_two_levels_cf"
+        call(name, self.h, coarse.h, ratio, a, None if crse_old is None else crse_old.fabs, b,
              None if crse_new is None else crse_new.fabs, ngrow, S.fabs)
 
     def plot_vars(self, S, out, state_vars=True, derive=()):
@@ -362,13 +365,15 @@ class Level:
 
     def step_plan(self, S):
         """How a step on S runs: tile, tile kind ("passes", "boxes" or
-        "level") and whether the stages after the first read the ghost
-        cells no box covers straight from U^n (no copy pass), and the fused
-        kernel's form (0 first, 1 self-loading)."""
+        "level"), whether tiles per box read the cells other boxes hold from
+        those boxes' arrays (no same-level ghost copies), whether the stages
+        after the first read the ghost cells no box covers straight from U^n
+        (no copy pass), and the fused kernel's form (0 first, 1
+        self-loading)."""
         out = (ctypes.c_int32 * 6)()
         call("amr_step_plan", self.h, ctypes.byref(self.scheme), S.fabs, out)
-        return {"tiling": (out[0], out[1], out[2]), "tiles": ("passes", "boxes", "level")[out[3]],
-                "uncovered_from_un": bool(out[4]), "form": out[5]}
+        return {"tiling": (out[0], out[1], out[2]), "tiles": ("passes", "boxes", "level", "boxes")[out[3]],
+                "direct_neighbours": out[3] == 3, "uncovered_from_un": bool(out[4]), "form": out[5]}
 
     def autotune(self, S):
         out = (ctypes.c_int32 * 3)()
@@ -593,6 +598,7 @@ class Hierarchy:
         self.tmp = [lev.new_state(layout=layout) for lev in self.levels]
         self.tint = [(0.0, 0.0)] * len(self.levels)
         self.ngrow = [None] * len(self.levels)
+        self.direct = [False] * len(self.levels)
 
     def upload(self, us):
         for S, u in zip(self.cur, us):
@@ -601,18 +607,27 @@ class Hierarchy:
     def download(self, lev):
         return self.cur[lev].download()
 
-    def _fill(self, lev, S, t):
+    def _fill(self, lev, S, t, stage=False):
+        """Ghost cells of level lev's state S at time t; a stage input of a
+        level whose fused stages read the other boxes' cells directly gets
+        no same-level copies."""
         L = self.levels[lev]
-        if lev == 0:
-            L.fill_patch(S)
-            return
         if self.ngrow[lev] is None:
-            # the fused stage kernel reads two ghost layers, the others three
-            self.ngrow[lev] = 2 if L.step_plan(S)["tiles"] != "passes" else 3
+            # the fused stage kernel reads two ghost layers, the others three;
+            # tiles per box may read the other boxes' cells themselves
+            plan = L.step_plan(S)
+            self.ngrow[lev] = 2 if plan["tiles"] != "passes" else 3
+            self.direct[lev] = plan["direct_neighbours"]
+        if lev == 0:
+            # the base level's ghost cells are same-level copies and physical
+            # boundaries only (none where its stages read neighbours directly)
+            if not (stage and self.direct[0]):
+                L.fill_patch(S)
+            return
         t0, t1 = self.tint[lev - 1]
         a, b = cf_weights(t0, t1, t)
         L.fill_two_levels(S, self.levels[lev - 1], self.old[lev - 1], self.cur[lev - 1], a, b, self.ratio,
-                          self.ngrow[lev])
+                          self.ngrow[lev], same_level=not (stage and self.direct[lev]))
 
     def _step(self, lev, time, dt):
         L = self.levels[lev]
@@ -621,7 +636,7 @@ class Hierarchy:
         A, B, C = self.cur[lev], self.old[lev], self.tmp[lev]
         sin = A
         for s, t in enumerate(times):
-            self._fill(lev, sin, t)
+            self._fill(lev, sin, t, stage=True)
             _, mode, a, b, c, d = capi.stage_coeffs(sch.order_rk, sch.stages_rk, dt, s)
             out = B if s % 2 == 0 else C
             L.rk_stage(mode, a, b, c, d, A, sin, out)
@@ -672,6 +687,7 @@ class Hierarchy:
         self.old[lev + 1] = new.new_state(layout=layout)
         self.tmp[lev + 1] = new.new_state(layout=layout)
         self.ngrow[lev + 1] = None
+        self.direct[lev + 1] = False
         old.close()
         if autotune:
             new.autotune(cur)
