@@ -139,6 +139,8 @@ namespace {
 struct DescSet {
   std::vector<amr_fab_t> host;
   Fab* d = nullptr;
+  int ustride = -1;  // every box's array has the same strides (-1: not yet known)
+  int span32 = -1;   // all the boxes' arrays lie within 2^31 elements (-1: not yet known)
   bool fill_built = false;
   CopyTask* d_copy = nullptr;
   int ncopy = 0;
@@ -1387,11 +1389,41 @@ bool use_level_tiles(amr_boxtab_s* bt, const amr_scheme_t& sch, DescSet* ds1, De
   return !choose_config(bt, sch).box_tiles && level_possible(bt, sch, ds1, dsin, dout);
 }
 
+// Whether every box's array of `ds` has the same strides, and whether all of
+// them lie within 2^31 elements of each other (the fused kernel's direct
+// reads address a neighbour's cells by a 32-bit element offset from the own
+// box's array).
+void desc_layout(DescSet* ds) {
+  if (ds->ustride < 0) {
+    ds->ustride = 1;
+    intptr_t lo = INTPTR_MAX, hi = 0;
+    for (const amr_fab_t& f : ds->host) {
+      if (f.jstride != ds->host[0].jstride || f.kstride != ds->host[0].kstride ||
+          f.nstride != ds->host[0].nstride)
+        ds->ustride = 0;
+      const int64_t nx = f.hi[0] - f.lo[0] + 1 + 2 * f.ng, ny = f.hi[1] - f.lo[1] + 1 + 2 * f.ng;
+      const int64_t nz = f.hi[2] - f.lo[2] + 1 + 2 * f.ng;
+      const int64_t last = (nx - 1) + (ny - 1) * f.jstride + (nz - 1) * f.kstride + (NCONS - 1) * f.nstride;
+      // the array's index origin (cell (0,0,0), which the kernels address
+      // from) and its last element
+      const int64_t org = -(int64_t(f.lo[0] - f.ng) + int64_t(f.lo[1] - f.ng) * f.jstride +
+                            int64_t(f.lo[2] - f.ng) * f.kstride);
+      lo = std::min({lo, intptr_t(f.data), intptr_t(f.data + org)});
+      hi = std::max({hi, intptr_t(f.data + last + 1), intptr_t(f.data + org)});
+    }
+    ds->span32 = int64_t(hi - lo) / int64_t(sizeof(double)) < (int64_t(1) << 31) - 1 ? 1 : 0;
+  }
+}
+
 // Whether the fused stages with tiles per box read the halo cells other
 // boxes hold from those boxes directly (the stage inputs then need no
-// same-level ghost copies): KEEP order 2 or 4 on a level with an owner map.
-bool nb_direct(const amr_boxtab_s* bt, const amr_scheme_t& sch, bool level) {
-  return !level && bt->nbmap.on && sch.order_keep <= 4 && fused_path(bt, sch, nullptr, level);
+// same-level ghost copies): KEEP order 2 or 4 on a level with an owner map,
+// stage inputs other than one periodic array over the domain (whose tiles
+// read the periodic images of its own cells, which costs less).
+bool nb_direct(amr_boxtab_s* bt, const amr_scheme_t& sch, bool level, DescSet* dsin) {
+  desc_layout(dsin);
+  return !level && bt->nbmap.on && sch.order_keep <= 4 && dsin->span32 == 1 &&
+         fused_path(bt, sch, nullptr, level) && !wrap_ok(bt, dsin);
 }
 
 void free_user_source(amr_boxtab_s* bt) {
@@ -1483,7 +1515,8 @@ int do_stage(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sc
       us.u0 = uncov_u0;
     }
     const Cover cov = kernel_cover(bt, level);
-    const NbMap nbm = !uncov_u0 && nb_direct(bt, sch, level) ? bt->nbmap : NbMap{};
+    NbMap nbm = !uncov_u0 && nb_direct(bt, sch, level, dsin) ? bt->nbmap : NbMap{};
+    nbm.ustride = nbm.on && dsin->ustride == 1 ? 1 : 0;
     HIPCHK(launch_fused_stage(sch, cfg, wa, cov, us, tt->d, tt->n, bt->bt.d_boxes, c, sa, s1, dsin->d, dout->d,
                               st, nbm));
     return AMR_OK;
@@ -1585,7 +1618,9 @@ int uncovered_from_un(amr_boxtab_s* bt, const amr_scheme_t& sch, DescSet* dst, D
   if (!bt->srcmap.on || wrap_ok(bt, dst)) return AMR_OK;
   const bool level = use_level_tiles(bt, sch, dst, dst, dw[0]);
   if (level && !level_ok(bt, dw[1])) return AMR_OK;
-  if (nb_direct(bt, sch, level)) return AMR_OK;  // those stages read uncovered ghosts from their own arrays
+  // stages that read the other boxes' cells directly read uncovered ghost
+  // cells from their own arrays (the copy pass fills them)
+  if (nb_direct(bt, sch, level, dw[0]) || nb_direct(bt, sch, level, dw[1])) return AMR_OK;
   FusedConfig cfg;
   if (!fused_path(bt, sch, &cfg, level)) return AMR_OK;
   TileTable* tt = nullptr;
@@ -1642,12 +1677,14 @@ int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t
     // themselves
     const bool fused = fused_path(bt, sch, nullptr);
     // nor do fused stages whose tiles read the other boxes' cells directly
-    const bool direct_nb = nb_direct(bt, sch, use_level_tiles(bt, sch, dst, sin, out));
+    // (do_stage's choice: not with uncovered ghost cells read from U^n)
+    const Fab* u0 = direct && s > 0 ? dst->d : nullptr;
+    const bool direct_nb = !u0 && nb_direct(bt, sch, use_level_tiles(bt, sch, dst, sin, out), sin);
     if (!(fused && wrap_ok(bt, sin)) && !direct_nb && !(scratch_out && s == 0))
       if ((rc = enqueue_fill(bt, sin, bt->batch && !fused && dir_flux_scheme(sch), nullptr, narrow && fused ? 2 : 0)))
         return rc;
     StageArgs sa{sc, d_box_dt, sch.order_rk, sch.stages_rk, s};
-    if ((rc = do_stage(bt, cls, sch, sa, dst, sin, out, direct && s > 0 ? dst->d : nullptr))) return rc;
+    if ((rc = do_stage(bt, cls, sch, sa, dst, sin, out, u0))) return rc;
   }
   if (nst == 1 && !scratch_out) {
     // single-stage schemes wrote into workspace 0; copy its valid cells back
@@ -3138,7 +3175,7 @@ int amr_step_plan(amr_boxtab_t bt, const amr_scheme_t* sch, const amr_fab_t* sta
   info[0] = fused ? cfg.tx : -1;
   info[1] = fused ? cfg.ty : 0;
   info[2] = fused ? cfg.tz : 0;
-  info[3] = fused ? (level ? 2 : (nb_direct(bt, *sch, level) ? 3 : 1)) : 0;
+  info[3] = fused ? (level ? 2 : (nb_direct(bt, *sch, level, dst) ? 3 : 1)) : 0;
   info[4] = 0;
   info[5] = fused ? cfg.form : 0;
   int nst = 0;

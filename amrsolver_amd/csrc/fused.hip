@@ -91,13 +91,17 @@ __device__ __forceinline__ int self_halo_cell(int tid, int ring_idx) {
 // per box read every halo cell that another box of the level holds (its
 // periodic image included) from that box's array, through an owner map of
 // the level's blocks; halo cells no box holds come from the box's own ghost
-// cells.  The stage inputs then need no same-level ghost copies.
+// cells.  The stage inputs then need no same-level ghost copies.  The
+// boxes' arrays lie within 2^31 elements of each other, so a slot's source
+// is an element offset from the own array; NB 2: they share their strides
+// (the registers of the plain form); NB 1: plane and component strides per
+// slot too.
 // At least two waves per SIMD for orders 2 and 4: the compiler would
 // otherwise take registers past 256 (into AGPRs) rather than reuse them,
 // halving the occupancy.  Order 6 holds more (six pairs per x/y face, two
 // more z carries): one wave per SIMD with up to 512 registers rather than
 // spills to scratch memory.
-template <int H, bool VISC, int TX, int TY, bool UCS, bool SELF, bool SRC, bool NB>
+template <int H, bool VISC, int TX, int TY, bool UCS, bool SELF, bool SRC, int NB>
 __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(H == 3 ? 1 : 2))) k_fused(const TileDev* __restrict__ tiles, const BoxDev* __restrict__ boxes,
                                                      Closures c, double co0, double co1, double co2, StageArgs sa,
                                                      WrapArgs wa, Cover cov, UncovSrc us, NbMap nbm,
@@ -194,12 +198,13 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(H =
     }
     loff[s] = x + y * int(S.js);
   }
-  // NB: per slot, the element address of its column in the array that
-  // holds it (plane 0, the periodic shift of the current block folded in)
-  // and that array's plane and component strides; refreshed when the
-  // plane enters another block of the owner map (uniform over the tile)
-  const gdouble* nsrc[NB ? C::NLOAD : 1];
-  int nks[NB ? C::NLOAD : 1], nns[NB ? C::NLOAD : 1];
+  // NB: per slot, the element offset from sbase of its column in the array
+  // that holds it (plane 0, the periodic shift of the current block folded
+  // in) and, for NB 1, that array's plane and component strides; refreshed
+  // when the plane enters another block of the owner map (uniform over the
+  // tile)
+  constexpr int NBS = NB == 1 ? C::NLOAD : 1;
+  int nks[NBS], nns[NBS];
   int nzb = INT_MIN;
   auto nb_refresh = [&](int p) {
     const int zb = (p - nbm.dlo[2]) >> nbm.sh;
@@ -221,13 +226,17 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(H =
                                ((xw - nbm.dlo[0]) >> nbm.sh)];
       if (ob >= 0) {
         const Fab F = sinf[ob];
-        nsrc[s] = gptr(F.p) + F.off + xw + int64_t(yw) * F.js + int64_t(pw - p) * F.ks;
-        nks[s] = int(F.ks);
-        nns[s] = int(F.ns);
+        loff[s] = int((F.p - S.p) + (F.off - S.off) + xw + int64_t(yw) * F.js + int64_t(pw - p) * F.ks);
+        if constexpr (NB == 1) {
+          nks[s] = int(F.ks);
+          nns[s] = int(F.ns);
+        }
       } else {
-        nsrc[s] = sbase + x + int64_t(y) * S.js;
-        nks[s] = int(S.ks);
-        nns[s] = int(S.ns);
+        loff[s] = x + y * int(S.js);
+        if constexpr (NB == 1) {
+          nks[s] = int(S.ks);
+          nns[s] = int(S.ns);
+        }
       }
     }
   };
@@ -276,17 +285,18 @@ __global__ void __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(H =
   const int s1off = min(gi, xhi) + min(gj, yhi) * int(S1.js);
   double pre[C::NLOAD][5];
   auto load_plane = [&](int p) {
-    if (NB) {
+    if constexpr (NB == 1) {
       nb_refresh(p);
 #pragma unroll
       for (int s = 0; s < C::NLOAD; ++s) {
-        const gdouble* pb = nsrc[s] + int64_t(p) * nks[s];
+        const int o = loff[s] + p * nks[s];
 #pragma unroll
-        for (int n = 0; n < 5; ++n) pre[s][n] = pb[n * nns[s]];
+        for (int n = 0; n < 5; ++n) pre[s][n] = sbase[int64_t(o) + n * nns[s]];
       }
       return;
     }
-    if (wa.on) p = p < wa.lo[2] ? p + wa.len[2] : (p >= wa.lo[2] + wa.len[2] ? p - wa.len[2] : p);
+    if constexpr (NB == 2) nb_refresh(p);  // then the plain form with the refreshed offsets
+    if (!NB && wa.on) p = p < wa.lo[2] ? p + wa.len[2] : (p >= wa.lo[2] + wa.len[2] ? p - wa.len[2] : p);
     if (UCS) {
       const int b = ((p - us.lo[2]) >> us.sh) - zb0;
       const gdouble* pb = sbase + int64_t(p) * S.ks;
@@ -672,17 +682,22 @@ void launch_t(const TileDev* tiles, int ntiles, const BoxDev* boxes, const Closu
   const bool src = sa.src != nullptr || sa.bfx != 0.0;
   if constexpr (H <= 2) {
     if (nbm.on) {
-      if (src) AMR_FUSED_LAUNCH(false, SELF, true, true);
-      else AMR_FUSED_LAUNCH(false, SELF, false, true);
+      if (nbm.ustride) {
+        if (src) AMR_FUSED_LAUNCH(false, SELF, true, 2);
+        else AMR_FUSED_LAUNCH(false, SELF, false, 2);
+      } else {
+        if (src) AMR_FUSED_LAUNCH(false, SELF, true, 1);
+        else AMR_FUSED_LAUNCH(false, SELF, false, 1);
+      }
       return;
     }
   }
   if (us.on) {
-    if (src) AMR_FUSED_LAUNCH(true, SELF, true, false);
-    else AMR_FUSED_LAUNCH(true, SELF, false, false);
+    if (src) AMR_FUSED_LAUNCH(true, SELF, true, 0);
+    else AMR_FUSED_LAUNCH(true, SELF, false, 0);
   } else {
-    if (src) AMR_FUSED_LAUNCH(false, SELF, true, false);
-    else AMR_FUSED_LAUNCH(false, SELF, false, false);
+    if (src) AMR_FUSED_LAUNCH(false, SELF, true, 0);
+    else AMR_FUSED_LAUNCH(false, SELF, false, 0);
   }
 }
 #undef AMR_FUSED_LAUNCH

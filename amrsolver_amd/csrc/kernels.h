@@ -97,6 +97,7 @@ struct NbMap {
   int dlo[3], len[3], per[3], nb[3];
   int sh;
   const int* owner;
+  int ustride;  // set per launch: every box's stage-input array has the same strides
 };
 
 // Ghost-fill tasks.  Copy: dst(i,j,k) = src(i - sh) for (i,j,k) in [lo,hi].

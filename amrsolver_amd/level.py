@@ -98,10 +98,22 @@ class StateArrays:
                 f.data = base + 8 * off
                 f.jstride, f.kstride, f.nstride = JS, KS, NS
         elif layout == "boxes":
-            for b, (lo, hi) in enumerate(level.boxes):
+            # one compact ghosted array per box, carved from one allocation
+            # as an AMReX arena hands out a MultiFab's FABs (each array
+            # starts on a 512-byte boundary)
+            shapes = []
+            for lo, hi in level.boxes:
                 n = [hi[d] - lo[d] + 1 for d in range(3)]
-                shape = (ncomp, n[2] + 2 * ng, n[1] + 2 * ng, n[0] + 2 * ng)
-                t = torch.zeros(shape, dtype=torch.float64, device=dev)
+                shapes.append((ncomp, n[2] + 2 * ng, n[1] + 2 * ng, n[0] + 2 * ng))
+            sizes = [int(np.prod(sh)) for sh in shapes]
+            offs, total = [], 0
+            for sz in sizes:
+                offs.append(total)
+                total += -(-sz // 64) * 64
+            self.arena = torch.zeros(max(total, 1), dtype=torch.float64, device=dev)
+            for b, (lo, hi) in enumerate(level.boxes):
+                shape = shapes[b]
+                t = self.arena[offs[b]:offs[b] + sizes[b]].view(shape)
                 self.tensors.append(t)
                 f = self.fabs[b]
                 for d in range(3):
@@ -598,7 +610,7 @@ class Hierarchy:
         self.tmp = [lev.new_state(layout=layout) for lev in self.levels]
         self.tint = [(0.0, 0.0)] * len(self.levels)
         self.ngrow = [None] * len(self.levels)
-        self.direct = [False] * len(self.levels)
+        self.direct = {}  # (level, id of a state): its stages read the other boxes' cells directly
 
     def upload(self, us):
         for S, u in zip(self.cur, us):
@@ -612,22 +624,25 @@ class Hierarchy:
         level whose fused stages read the other boxes' cells directly gets
         no same-level copies."""
         L = self.levels[lev]
-        if self.ngrow[lev] is None:
+        key = (lev, id(S))
+        if key not in self.direct:
             # the fused stage kernel reads two ghost layers, the others three;
-            # tiles per box may read the other boxes' cells themselves
+            # tiles per box may read the other boxes' cells themselves (decided
+            # per stage input)
             plan = L.step_plan(S)
             self.ngrow[lev] = 2 if plan["tiles"] != "passes" else 3
-            self.direct[lev] = plan["direct_neighbours"]
+            self.direct[key] = plan["direct_neighbours"]
+        direct = stage and self.direct[key]
         if lev == 0:
             # the base level's ghost cells are same-level copies and physical
             # boundaries only (none where its stages read neighbours directly)
-            if not (stage and self.direct[0]):
+            if not direct:
                 L.fill_patch(S)
             return
         t0, t1 = self.tint[lev - 1]
         a, b = cf_weights(t0, t1, t)
         L.fill_two_levels(S, self.levels[lev - 1], self.old[lev - 1], self.cur[lev - 1], a, b, self.ratio,
-                          self.ngrow[lev], same_level=not (stage and self.direct[lev]))
+                          self.ngrow[lev], same_level=not direct)
 
     def _step(self, lev, time, dt):
         L = self.levels[lev]
@@ -687,7 +702,7 @@ class Hierarchy:
         self.old[lev + 1] = new.new_state(layout=layout)
         self.tmp[lev + 1] = new.new_state(layout=layout)
         self.ngrow[lev + 1] = None
-        self.direct[lev + 1] = False
+        self.direct = {k: v for k, v in self.direct.items() if k[0] != lev + 1}
         old.close()
         if autotune:
             new.autotune(cur)

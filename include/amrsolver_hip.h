@@ -142,12 +142,12 @@ int amr_fill_two_levels(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, double 
                         const amr_fab_t* fine_state);
 
 /* amr_fill_two_levels without the same-level copies (the coarse-fine
- * interpolation and physical BCs only), for the stage inputs of a step whose
- * fused kernel reads the cells other boxes hold straight from their arrays
- * (amr_step_plan info[3] == 3; src/tim/CNS_advance.cpp:54-161 fills those
- * ghost cells by copies).  Where the level has no owner map for such reads
- * (info[3] != 3 for every scheme) it is amr_fill_two_levels.  No reference
- * counterpart. */
+ * interpolation and physical BCs only), for a stage input whose stage's fused
+ * kernel reads the cells other boxes hold straight from their arrays
+ * (amr_step_plan on that input: info[3] == 3; src/tim/CNS_advance.cpp:54-161
+ * fills those ghost cells by copies).  Other stage inputs need
+ * amr_fill_two_levels.  On a level with no owner map for such reads it is
+ * amr_fill_two_levels.  No reference counterpart. */
 int amr_fill_two_levels_cf(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, double a_old,
                            const amr_fab_t* crse_old, double b_new, const amr_fab_t* crse_new, int ngrow,
                            const amr_fab_t* fine_state);
@@ -397,7 +397,9 @@ int amr_set_box_tiling(amr_ctx_t ctx, const amr_scheme_t* sch, int tx, int ty, i
  * boxes, 3 tiles per box that read the cells other boxes hold (periodic
  * images included) straight from those boxes' arrays, so that a step fills
  * no same-level ghost cells (levels whose boxes keep 3 cells away from every
- * non-periodic domain boundary, KEEP order 2 or 4), info[4] 1 when the stages after the first read the ghost cells no
+ * non-periodic domain boundary, KEEP order 2 or 4, the boxes' arrays within
+ * 2^31 elements of each other and not one periodic array over the domain),
+ * info[4] 1 when the stages after the first read the ghost cells no
  * box covers straight from U^n (else a copy pass per step into the stage
  * workspaces, where there are such cells), info[5] the fused kernel's form
  * (0 first, 1 self-loading; see amr_set_tiling_form).  No reference
