@@ -2705,8 +2705,16 @@ int amr_fill_two_levels_cf(amr_boxtab_t fine, amr_boxtab_t crse, int ratio, doub
                            const amr_fab_t* crse_old, double b_new, const amr_fab_t* crse_new, int ngrow,
                            const amr_fab_t* fine_state) {
   // without an owner map the level may have physical boundaries, whose
-  // ghost cells next to other boxes depend on the same-level copies
-  const bool same = !(fine && fine->nbmap.on);
+  // ghost cells next to other boxes depend on the same-level copies; arrays
+  // too far apart for the kernel's 32-bit offsets keep the copies too
+  bool same = !(fine && fine->nbmap.on);
+  if (!same && fine_state) {
+    DescSet* ds = nullptr;
+    int rc = get_desc(fine, fine_state, NGHOST, &ds);
+    if (rc) return rc;
+    desc_layout(ds);
+    same = ds->span32 != 1;
+  }
   return fill_two_levels(fine, crse, ratio, a_old, crse_old, b_new, crse_new, ngrow, fine_state, same);
 }
 
