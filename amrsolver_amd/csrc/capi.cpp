@@ -287,6 +287,7 @@ struct amr_boxtab_s {
   unsigned* d_cover_bits = nullptr;
   NbMap nbmap{};            // owner map for direct neighbour reads (on = 0: not available)
   int* d_owner = nullptr;
+  XSide* d_xside = nullptr;  // x sides of the directional passes (null: they need ghost fills)
   unsigned* d_all_covered = nullptr;
   // problem parameters (bc 6 walls, body force) and the wall flux tasks
   bool has_problem = false;
@@ -934,6 +935,7 @@ int ensure_unfused(amr_boxtab_s* bt, const amr_scheme_t& s) {
       bt->uw.weno.d[d] = dv;
       bt->uw.weno.n[d] = n;
     }
+    bt->uw.weno.xside = bt->d_xside;
   }
   bt->uw_kinds = need;
   return AMR_OK;
@@ -1224,6 +1226,61 @@ int build_nbmap(amr_boxtab_s* bt) {
   m.owner = bt->d_owner;
   m.on = 1;
   bt->nbmap = m;
+  return AMR_OK;
+}
+
+// Side tables of the directional x pass (XSide) for a table whose boxes all
+// span their problem's periodic y and z extents (the y and z passes read
+// periodic images of their own cells): each x end of a box is a periodic
+// image of the box itself (the pass wraps), one neighbour box covering the
+// NGHOST cells beyond it (or its periodic image), or a physical boundary
+// whose rule is foextrap or reflect for every component.  Otherwise no
+// table: the passes read ghost cells.  Built with the box table.
+int build_xsides(amr_boxtab_s* bt) {
+  const int nb = bt->bt.nbox;
+  const auto& B = bt->bt.h_boxes;
+  std::vector<XSide> xs(size_t(2 * nb));
+  for (int b = 0; b < nb; ++b) {
+    const amr_geom_t& g = bt->geoms[b];
+    for (int d = 1; d < 3; ++d)
+      if (!g.periodic[d] || B[b].lo[d] != g.dom_lo[d] || B[b].hi[d] != g.dom_hi[d]) return AMR_OK;
+    const int len = g.dom_hi[0] - g.dom_lo[0] + 1;
+    if (g.periodic[0] && B[b].lo[0] == g.dom_lo[0] && B[b].hi[0] == g.dom_hi[0]) continue;  // wraps
+    if (B[b].hi[0] - B[b].lo[0] + 1 < NGHOST) return AMR_OK;
+    for (int side = 0; side < 2; ++side) {
+      XSide& X = xs[size_t(2 * b + side)];
+      // the cells beyond this end: [e0, e1]
+      int e0 = side == 0 ? B[b].lo[0] - NGHOST : B[b].hi[0] + 1;
+      int e1 = e0 + NGHOST - 1;
+      const bool outside = side == 0 ? e1 < g.dom_lo[0] : e0 > g.dom_hi[0];
+      if (outside && !g.periodic[0]) {
+        const int code = side == 0 ? g.lo_bc[0] : g.hi_bc[0];
+        X.kind = 2;
+        X.bnd = side == 0 ? g.dom_lo[0] : g.dom_hi[0];
+        for (int n = 0; n < 5; ++n) {
+          X.type[n] = bc_type(code, n, 0);
+          if (X.type[n] < 2 || X.type[n] > 4) return AMR_OK;
+        }
+        continue;
+      }
+      X.shift = 0;
+      if (outside) {  // periodic image
+        X.shift = side == 0 ? len : -len;
+        e0 += X.shift;
+        e1 += X.shift;
+      }
+      if (e0 < g.dom_lo[0] || e1 > g.dom_hi[0]) return AMR_OK;
+      X.src = -1;
+      for (int o = 0; o < nb && !bt->batch; ++o)
+        if (B[o].lo[0] <= e0 && B[o].hi[0] >= e1 && B[o].lo[1] == B[b].lo[1] && B[o].hi[1] == B[b].hi[1] &&
+            B[o].lo[2] == B[b].lo[2] && B[o].hi[2] == B[b].hi[2])
+          X.src = o;
+      if (X.src < 0) return AMR_OK;
+      X.kind = 1;
+    }
+  }
+  HIPCHK(hipMalloc(&bt->d_xside, sizeof(XSide) * xs.size()));
+  HIPCHK(hipMemcpy(bt->d_xside, xs.data(), sizeof(XSide) * xs.size(), hipMemcpyHostToDevice));
   return AMR_OK;
 }
 
@@ -1677,9 +1734,13 @@ int advance_impl(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t
     // themselves
     const bool fused = fused_path(bt, sch, nullptr);
     // nor do fused stages whose tiles read the other boxes' cells directly
-    // (do_stage's choice: not with uncovered ghost cells read from U^n)
+    // (do_stage's choice: not with uncovered ghost cells read from U^n), nor
+    // the inviscid directional passes where they read x ends through the
+    // side tables
     const Fab* u0 = direct && s > 0 ? dst->d : nullptr;
-    const bool direct_nb = !u0 && nb_direct(bt, sch, use_level_tiles(bt, sch, dst, sin, out), sin);
+    const bool direct_nb = (!u0 && nb_direct(bt, sch, use_level_tiles(bt, sch, dst, sin, out), sin)) ||
+                           (!fused && dir_flux_scheme(sch) && bt->d_xside != nullptr && !bt->walls &&
+                            !(sch.rhs_source && bt->usrc_fn));
     if (!(fused && wrap_ok(bt, sin)) && !direct_nb && !(scratch_out && s == 0))
       if ((rc = enqueue_fill(bt, sin, bt->batch && !fused && dir_flux_scheme(sch), nullptr, narrow && fused ? 2 : 0)))
         return rc;
@@ -1727,8 +1788,11 @@ int cfl_step(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sc
   DescSet* ds = nullptr;
   if ((rc = get_desc(bt, state, NGHOST, &ds))) return rc;
   hipStream_t st = bt->ctx->stream;
-  HIPCHK(launch_estdt_boxes(bt->bt, ds->d, make_closures(cls), bt->d_partial, bt->d_result, st));
-  HIPCHK(launch_dt_policy(bt->bt.nbox, bt->batch, bt->d_result, bt->d_clock, cfl, stop_time, bt->d_box_dt, st));
+  // per-box partial minima straight into the policy kernel (no separate
+  // per-box reduction)
+  HIPCHK(launch_estdt_boxes(bt->bt, ds->d, make_closures(cls), bt->d_partial, nullptr, st));
+  HIPCHK(launch_dt_policy(bt->bt.nbox, bt->batch, bt->d_partial, estdt_box_blocks(bt->bt), bt->d_clock, cfl,
+                          stop_time, bt->d_box_dt, st));
   return advance_impl(bt, cls, sch, state, 1.0, bt->d_box_dt);
 }
 
@@ -2382,6 +2446,7 @@ int finish_boxtab(amr_boxtab_s* bt) {
   rc = build_uncovered(bt);
   if (rc == AMR_OK) rc = build_cover(bt);
   if (rc == AMR_OK) rc = build_nbmap(bt);
+  if (rc == AMR_OK) rc = build_xsides(bt);
   if (rc == AMR_OK) rc = build_srcmap(bt);
   if (rc == AMR_OK) rc = build_walls(bt);
   return rc;
@@ -2402,6 +2467,7 @@ int amr_boxtab_destroy(amr_boxtab_t bt) {
   if (bt->d_uncov_blk) (void)hipFree(bt->d_uncov_blk);
   if (bt->d_cover_bits) (void)hipFree(bt->d_cover_bits);
   if (bt->d_owner) (void)hipFree(bt->d_owner);
+  if (bt->d_xside) (void)hipFree(bt->d_xside);
   if (bt->d_srcmap_bits) (void)hipFree(bt->d_srcmap_bits);
   if (bt->d_all_covered) (void)hipFree(bt->d_all_covered);
   if (bt->d_wall_tasks) (void)hipFree(bt->d_wall_tasks);

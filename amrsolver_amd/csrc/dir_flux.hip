@@ -32,6 +32,10 @@ namespace amr {
 
 namespace {
 
+// One thread per cell of a WX x WB brick; the faces beyond (65 x 4 in the x
+// pass, 64 x 5 in the others) take a second round on the first threads
+// (five-wave workgroups with one face per thread ran the 1D batch 18%
+// slower).
 constexpr int WX = 64, WB = 4, WNT = WX * WB;
 
 // Brick geometry of pass D: loaded cells LX x LB, faces FX x FB (x fastest);
@@ -69,6 +73,7 @@ __device__ __forceinline__ void dir_epilogue(const WenoBrick& W, const BoxDev& b
   constexpr int BA = D == 2 ? 2 : 1;
   const int tid = threadIdx.x;
   // flux difference of the own cell (CNS_advance.cpp:384-390)
+  if (tid >= WX * WB) return;
   const int cx = tid % WX, cb = tid / WX;
   const int x = W.x0 + cx, b = W.b0 + cb;
   if (x > bx.hi[0] || b > bx.hi[BA]) return;
@@ -105,6 +110,39 @@ __device__ __forceinline__ void dir_epilogue(const WenoBrick& W, const BoxDev& b
   }
 }
 
+// The conserved state of pencil cell (x, b) of pass D (x wrapped or clamped
+// by the caller except, with side tables, beyond the box's x ends in the x
+// pass: there the neighbour's cell or the boundary rule of the side).
+template <int D>
+__device__ __forceinline__ void pencil_cell(const WenoBrick& W, const BoxDev& bx, const Fab& S, const Fab* sinf,
+                                            const XSide* xs, int x, int b, double u[5]) {
+  int i, j, k;
+  brick_ijk<D>(x, b, W.c, i, j, k);
+  if (D == 0 && xs != nullptr && (x < bx.lo[0] || x > bx.hi[0])) {
+    const XSide& X = xs[2 * W.box + (x > bx.hi[0] ? 1 : 0)];
+    if (X.kind == 1) {
+      const Fab F = sinf[X.src];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) u[n] = F(i + X.shift, j, k, n);
+      return;
+    }
+    if (X.kind == 2) {
+      // k_fill_bc's rule (AMReX foextrap / reflect_even / reflect_odd)
+      const int lo_side = x < bx.lo[0];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        const int ty = X.type[n];
+        const int sx = ty == 2 ? X.bnd : (lo_side ? 2 * X.bnd - 1 - i : 2 * X.bnd + 1 - i);
+        const double v = S(sx, j, k, n);
+        u[n] = ty == 4 ? -v : v;
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < 5; ++n) u[n] = S(i, j, k, n);
+}
+
 // Viscous part of face f of a brick (f % FX along x, f / FX along the second
 // axis; the face lies below its cell in direction D): F += 0.5 (Fv(left) +
 // Fv(right)) per component (CNS_hydro_K.h:254-261, the reference's operand
@@ -128,7 +166,8 @@ template <int D, int LAST>
 __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ bricks, const BoxDev* __restrict__ boxes,
                                                   Closures c, StageArgs sa, const Fab* __restrict__ sinf,
                                                   const Fab* __restrict__ accf, const Fab* __restrict__ s1f,
-                                                  const Fab* __restrict__ outf, const Fab* __restrict__ pvf) {
+                                                  const Fab* __restrict__ outf, const Fab* __restrict__ pvf,
+                                                  const XSide* __restrict__ xs) {
   using G = WGeo<D>;
   __shared__ double sv[G::NV][G::NL];
   const WenoBrick W = bricks[blockIdx.x];
@@ -150,10 +189,10 @@ __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ 
     else x = min(x, xmax);
     if (D != 0 && W.wrap > 0) b = dlo + ((b - dlo) % W.wrap + W.wrap) % W.wrap;
     else b = min(b, bmax);
-    int i, j, k;
-    brick_ijk<D>(x, b, W.c, i, j, k);
-    const double rho = S(i, j, k, URHO), momx = S(i, j, k, UMX), momy = S(i, j, k, UMY);
-    const double momz = S(i, j, k, UMZ), rhoet = S(i, j, k, UET);
+    double uc[5];
+    pencil_cell<D>(W, bx, S, sinf, xs, x, b, uc);
+    const double rho = uc[URHO], momx = uc[UMX], momy = uc[UMY];
+    const double momz = uc[UMZ], rhoet = uc[UET];
     const double rhoinv = 1.0 / rho;
     const double ux = momx * rhoinv, uy = momy * rhoinv, uz = momz * rhoinv;
     const double rhoekin = 0.5 * rhoinv * (momx * momx + momy * momy + momz * momz);
@@ -241,7 +280,7 @@ __global__ void __launch_bounds__(WNT) k_muscl_dir(const WenoBrick* __restrict__
                                                    const BoxDev* __restrict__ boxes, Closures c, StageArgs sa,
                                                    const Fab* __restrict__ sinf, const Fab* __restrict__ accf,
                                                    const Fab* __restrict__ s1f, const Fab* __restrict__ outf,
-                                                   const Fab* __restrict__ pvf) {
+                                                   const Fab* __restrict__ pvf, const XSide* __restrict__ xs) {
   using G = MGeo<D>;
   __shared__ double sq[NPRIM][G::NL];
   __shared__ double sd[5][G::NL];
@@ -264,10 +303,10 @@ __global__ void __launch_bounds__(WNT) k_muscl_dir(const WenoBrick* __restrict__
     else x = min(x, xmax);
     if (D != 0 && W.wrap > 0) b = dlo + ((b - dlo) % W.wrap + W.wrap) % W.wrap;
     else b = min(b, bmax);
-    int i, j, k;
-    brick_ijk<D>(x, b, W.c, i, j, k);
+    double uc[5];
+    pencil_cell<D>(W, bx, S, sinf, xs, x, b, uc);
     double q[NPRIM];
-    cons2prim(S(i, j, k, URHO), S(i, j, k, UMX), S(i, j, k, UMY), S(i, j, k, UMZ), S(i, j, k, UET), c, q);
+    cons2prim(uc[0], uc[1], uc[2], uc[3], uc[4], c, q);
 #pragma unroll
     for (int n = 0; n < NPRIM; ++n) sq[n][l] = q[n];
   }
@@ -351,23 +390,23 @@ hipError_t launch_dir_flux(int flux_euler, const WenoWork& ww, const BoxDev* box
   if (flux_euler == 2) {
     if (ww.n[0] > 0)
       hipLaunchKernelGGL((k_weno_dir<0, 0>), dim3(unsigned(ww.n[0])), dim3(WNT), 0, st, ww.d[0], boxes, c, sa, sin,
-                         acc, s1, out, pv0);
+                         acc, s1, out, pv0, ww.xside);
     if (ww.n[1] > 0)
       hipLaunchKernelGGL((k_weno_dir<1, 0>), dim3(unsigned(ww.n[1])), dim3(WNT), 0, st, ww.d[1], boxes, c, sa, sin,
-                         acc, s1, out, pv1);
+                         acc, s1, out, pv1, ww.xside);
     if (ww.n[2] > 0)
       hipLaunchKernelGGL((k_weno_dir<2, 1>), dim3(unsigned(ww.n[2])), dim3(WNT), 0, st, ww.d[2], boxes, c, sa, sin,
-                         acc, s1, out, pv2);
+                         acc, s1, out, pv2, ww.xside);
   } else if (flux_euler == 0) {
     if (ww.n[0] > 0)
       hipLaunchKernelGGL((k_muscl_dir<0, 0>), dim3(unsigned(ww.n[0])), dim3(WNT), 0, st, ww.d[0], boxes, c, sa, sin,
-                         acc, s1, out, pv0);
+                         acc, s1, out, pv0, ww.xside);
     if (ww.n[1] > 0)
       hipLaunchKernelGGL((k_muscl_dir<1, 0>), dim3(unsigned(ww.n[1])), dim3(WNT), 0, st, ww.d[1], boxes, c, sa, sin,
-                         acc, s1, out, pv1);
+                         acc, s1, out, pv1, ww.xside);
     if (ww.n[2] > 0)
       hipLaunchKernelGGL((k_muscl_dir<2, 1>), dim3(unsigned(ww.n[2])), dim3(WNT), 0, st, ww.d[2], boxes, c, sa, sin,
-                         acc, s1, out, pv2);
+                         acc, s1, out, pv2, ww.xside);
   } else {
     return hipErrorInvalidValue;
   }

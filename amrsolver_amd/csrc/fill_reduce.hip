@@ -439,17 +439,23 @@ __device__ double clock_step(Clock& k, double emin, double cfl, double stop) {
   return dt0;
 }
 
-__global__ void __launch_bounds__(RBLK) k_dt_policy(int nbox, int batch, const double* est, Clock* clk,
+// est holds nper partial minima per box (k_estdt_boxes' blocks); a box's
+// estimate is their minimum, exact in any order
+__global__ void __launch_bounds__(RBLK) k_dt_policy(int nbox, int batch, const double* est, int nper, Clock* clk,
                                                     double cfl, double stop, double* box_dt) {
   if (batch) {
     const int p = blockIdx.x * RBLK + threadIdx.x;
-    if (p < nbox) box_dt[p] = clock_step(clk[p], est[p], cfl, stop);
+    if (p < nbox) {
+      double m = est[size_t(p) * nper];
+      for (int t = 1; t < nper; ++t) m = dmin(m, est[size_t(p) * nper + t]);
+      box_dt[p] = clock_step(clk[p], m, cfl, stop);
+    }
     return;
   }
   // one problem: the minimum over its boxes (exact in any order)
   __shared__ double sm[RBLK];
   double v = 1.79769313486231570815e+308;
-  for (int b = threadIdx.x; b < nbox; b += RBLK) v = dmin(v, est[b]);
+  for (int b = threadIdx.x; b < nbox * nper; b += RBLK) v = dmin(v, est[b]);
   const double m = block_min(v, sm);
   __shared__ double s_dt;
   if (threadIdx.x == 0) s_dt = clock_step(clk[0], m, cfl, stop);
@@ -457,10 +463,10 @@ __global__ void __launch_bounds__(RBLK) k_dt_policy(int nbox, int batch, const d
   for (int b = threadIdx.x; b < nbox; b += RBLK) box_dt[b] = s_dt;
 }
 
-hipError_t launch_dt_policy(int nbox, bool batch, const double* est, Clock* clk, double cfl, double stop_time,
-                            double* box_dt, hipStream_t s) {
+hipError_t launch_dt_policy(int nbox, bool batch, const double* est, int nper, Clock* clk, double cfl,
+                            double stop_time, double* box_dt, hipStream_t s) {
   const int grid = batch ? (nbox + RBLK - 1) / RBLK : 1;
-  hipLaunchKernelGGL(k_dt_policy, dim3(grid), dim3(RBLK), 0, s, nbox, batch ? 1 : 0, est, clk, cfl, stop_time,
+  hipLaunchKernelGGL(k_dt_policy, dim3(grid), dim3(RBLK), 0, s, nbox, batch ? 1 : 0, est, nper, clk, cfl, stop_time,
                      box_dt);
   return hipGetLastError();
 }
@@ -615,7 +621,7 @@ hipError_t launch_estdt_boxes(const BoxTable& bt, const Fab* st, const Closures&
                               hipStream_t s) {
   int nper = estdt_box_blocks(bt);
   hipLaunchKernelGGL(k_estdt_boxes, dim3(nper, bt.nbox), dim3(RBLK), 0, s, bt.d_boxes, st, c, partial);
-  hipLaunchKernelGGL(k_min_boxes, dim3(bt.nbox), dim3(RBLK), 0, s, partial, nper, result);
+  if (result) hipLaunchKernelGGL(k_min_boxes, dim3(bt.nbox), dim3(RBLK), 0, s, partial, nper, result);
   return hipGetLastError();
 }
 

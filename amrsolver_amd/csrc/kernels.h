@@ -43,9 +43,21 @@ struct BoxTable {
 struct WenoBrick {
   int box, x0, b0, c, wrap;
 };
+// Where the x pass of a box reads the cells beyond its x ends (2 per box,
+// left then right): kind 0 its ghost cells; 1 box `src` at x + shift (a
+// neighbour, or a periodic image); 2 the physical boundary's rule per
+// component (type 2 foextrap, 3 reflect even, 4 reflect odd, about the
+// boundary cell bnd, from the box's own valid cells).
+struct XSide {
+  int kind, src, shift, bnd;
+  int type[5];
+};
 struct WenoWork {
   WenoBrick* d[3] = {nullptr, nullptr, nullptr};
   int n[3] = {0, 0, 0};
+  // set when every box spans its periodic y and z extents and both x sides
+  // have kind 1 or 2: the passes then read no ghost cell (no fill needed)
+  const XSide* xside = nullptr;
 };
 int weno_brick_count(const BoxTable& bt, int d);
 void weno_bricks(const BoxTable& bt, int d, const int* wrap, WenoBrick* out);
@@ -285,7 +297,8 @@ hipError_t launch_pack_valid(const BoxTable& bt, const Fab* src, int ncomp, cons
 int reduce_blocks(const BoxTable& bt);
 hipError_t launch_estdt(const BoxTable& bt, const Fab* st, const Closures& c, double* partial,
                         double* result, hipStream_t s);
-// per-box minimum of cns_estdt: result[b] for every box
+// per-box minimum of cns_estdt: result[b] for every box (estdt_box_blocks
+// partial minima per box in `partial`; result may be null)
 int estdt_box_blocks(const BoxTable& bt);
 hipError_t launch_estdt_boxes(const BoxTable& bt, const Fab* st, const Closures& c, double* partial,
                               double* result, hipStream_t s);
@@ -297,11 +310,12 @@ struct Clock {
   double t, dt;
   int32_t steps, first, bad, pad_;
 };
-// Per box (est: per-box minima of cns_estdt, without the CFL factor) the
-// time step of its problem (batch: box b is problem b; else one problem)
-// under the policy of CNS::computeNewDt, written to box_dt; clocks advance.
-hipError_t launch_dt_policy(int nbox, bool batch, const double* est, Clock* clk, double cfl, double stop_time,
-                            double* box_dt, hipStream_t s);
+// Per box (est: nper partial minima per box of cns_estdt, without the CFL
+// factor, as launch_estdt_boxes writes them to `partial`) the time step of
+// its problem (batch: box b is problem b; else one problem) under the policy
+// of CNS::computeNewDt, written to box_dt; clocks advance.
+hipError_t launch_dt_policy(int nbox, bool batch, const double* est, int nper, Clock* clk, double cfl,
+                            double stop_time, double* box_dt, hipStream_t s);
 // computeNewDt's multi-level rule (src/CNS.cpp:319-377) for up to
 // MAX_DT_LEVELS levels: est[l] points at level l's minimum of cns_estdt;
 // dt_level (device, nlev entries) holds the previous steps on entry
