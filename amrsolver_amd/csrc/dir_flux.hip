@@ -110,6 +110,23 @@ __device__ __forceinline__ void dir_epilogue(const WenoBrick& W, const BoxDev& b
   }
 }
 
+// Faces 0 .. nf-1 of the thread's rounds into sF ([5][NF]); with repeated
+// rows (rep) face row 0 also goes to row WB.
+template <int FX, int NF, int NFT>
+__device__ __forceinline__ void store_faces(const double (&F)[NFT][5], double* sF, int nf, bool rep) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < NFT; ++r) {
+    const int f = tid + r * WNT;
+    if (f < nf)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) sF[n * NF + f] = F[r][n];
+  }
+  if (rep && tid < FX)
+#pragma unroll
+    for (int n = 0; n < 5; ++n) sF[n * NF + WB * FX + tid] = F[0][n];
+}
+
 // The conserved state of pencil cell (x, b) of pass D (x wrapped or clamped
 // by the caller except, with side tables, beyond the box's x ends in the x
 // pass: there the neighbour's cell or the boundary rule of the side).
@@ -141,6 +158,15 @@ __device__ __forceinline__ void pencil_cell(const WenoBrick& W, const BoxDev& bx
   }
 #pragma unroll
   for (int n = 0; n < 5; ++n) u[n] = S(i, j, k, n);
+}
+
+// Primitives of pencil cell (x, b) of pass D.
+template <int D>
+__device__ __forceinline__ void cons2prim_cell(const WenoBrick& W, const BoxDev& bx, const Fab& S, const Fab* sinf,
+                                               const XSide* xs, int x, int b, const Closures& c, double q[NPRIM]) {
+  double uc[5];
+  pencil_cell<D>(W, bx, S, sinf, xs, x, b, uc);
+  cons2prim(uc[0], uc[1], uc[2], uc[3], uc[4], c, q);
 }
 
 // Viscous part of face f of a brick (f % FX along x, f / FX along the second
@@ -181,14 +207,7 @@ __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ 
 
   // 1. stage the cells (High_resolution.h:83-121, the pass direction's part)
   const int dlo = bx.lo[D];
-  for (int l = tid; l < G::NL; l += WNT) {
-    const int lx = l % G::LX, lb = l / G::LX;
-    int x = W.x0 + lx - (D == 0 ? 3 : 0);
-    int b = W.b0 + lb - (D == 0 ? 0 : 3);
-    if (D == 0 && W.wrap > 0) x = dlo + ((x - dlo) % W.wrap + W.wrap) % W.wrap;
-    else x = min(x, xmax);
-    if (D != 0 && W.wrap > 0) b = dlo + ((b - dlo) % W.wrap + W.wrap) % W.wrap;
-    else b = min(b, bmax);
+  auto stage_cell = [&](int x, int b, double v[G::NV]) {
     double uc[5];
     pencil_cell<D>(W, bx, S, sinf, xs, x, b, uc);
     const double rho = uc[URHO], momx = uc[UMX], momy = uc[UMY];
@@ -200,14 +219,42 @@ __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ 
     const double P = c.gm1 * rhoeint;
     const double cs = sqrt(c.gamma * P * rhoinv);
     const double ud = D == 0 ? ux : (D == 1 ? uy : uz);
-    sv[0][l] = rho;
-    sv[1][l] = momx;
-    sv[2][l] = momy;
-    sv[3][l] = momz;
-    sv[4][l] = rhoet;
-    sv[5][l] = ud;
-    sv[6][l] = P;
-    sv[7][l] = fabs(dmax(dmax(ud + cs, ud - cs), ud));
+    v[0] = rho;
+    v[1] = momx;
+    v[2] = momy;
+    v[3] = momz;
+    v[4] = rhoet;
+    v[5] = ud;
+    v[6] = P;
+    v[7] = fabs(dmax(dmax(ud + cs, ud - cs), ud));
+  };
+  // A box WB cells deep and periodic along D (the 1D problems' x4x4 boxes):
+  // the pencil's rows repeat the box's WB rows and face row WB is face row
+  // 0 (the same six cells in the same order), so each thread stages one cell
+  // into every row holding it and the faces of rows 0 .. WB-1 are computed
+  // once.  Uniform over the workgroup.
+  const bool rep = D != 0 && W.wrap == WB && pvf == nullptr;
+  if (rep) {
+    const int lx = tid % G::LX, r = tid / G::LX;
+    double v[G::NV];
+    stage_cell(min(W.x0 + lx, xmax), dlo + r, v);
+    for (int lb = (r + 3) % WB; lb < G::LB; lb += WB)
+#pragma unroll
+      for (int n = 0; n < G::NV; ++n) sv[n][lb * G::LX + lx] = v[n];
+  } else {
+    for (int l = tid; l < G::NL; l += WNT) {
+      const int lx = l % G::LX, lb = l / G::LX;
+      int x = W.x0 + lx - (D == 0 ? 3 : 0);
+      int b = W.b0 + lb - (D == 0 ? 0 : 3);
+      if (D == 0 && W.wrap > 0) x = dlo + ((x - dlo) % W.wrap + W.wrap) % W.wrap;
+      else x = min(x, xmax);
+      if (D != 0 && W.wrap > 0) b = dlo + ((b - dlo) % W.wrap + W.wrap) % W.wrap;
+      else b = min(b, bmax);
+      double v[G::NV];
+      stage_cell(x, b, v);
+#pragma unroll
+      for (int n = 0; n < G::NV; ++n) sv[n][l] = v[n];
+    }
   }
   __syncthreads();
 
@@ -215,11 +262,12 @@ __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ 
   // f sits below stencil cell 3 of cells l0 .. l0+5 (stride st along D)
   constexpr int st = D == 0 ? 1 : G::LX;
   constexpr int NFT = (G::NF + WNT - 1) / WNT;
+  const int nf = rep ? WB * G::FX : G::NF;
   double F[NFT][5];
 #pragma unroll
   for (int r = 0; r < NFT; ++r) {
     const int f = tid + r * WNT;
-    if (f < G::NF) {
+    if (f < nf) {
       const int fx = f % G::FX, fb = f / G::FX;
       const int l0 = fb * G::LX + fx;
       double me = sv[7][l0];
@@ -251,13 +299,7 @@ __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ 
   }
   __syncthreads();  // every stencil read done: the faces reuse the cell slots
   double* sF = &sv[0][0];  // [5][NF]
-#pragma unroll
-  for (int r = 0; r < NFT; ++r) {
-    const int f = tid + r * WNT;
-    if (f < G::NF)
-#pragma unroll
-      for (int n = 0; n < 5; ++n) sF[n * G::NF + f] = F[r][n];
-  }
+  store_faces<G::FX, G::NF, NFT>(F, sF, nf, rep);
   __syncthreads();
 
   dir_epilogue<D, LAST, G::FX, G::NF>(W, bx, sa, sF, S, accf, s1f, outf);
@@ -294,44 +336,72 @@ __global__ void __launch_bounds__(WNT) k_muscl_dir(const WenoBrick* __restrict__
   const int bmax = bx.hi[BA] + (D == 0 ? 0 : NGHOST);
   const int dlo = bx.lo[D];
 
-  // 1. primitives of the pencil (cons2prim, CNS_hydro_K.h:15-34)
-  for (int l = tid; l < G::NL; l += WNT) {
-    const int lx = l % G::LX, lb = l / G::LX;
-    int x = W.x0 + lx - (D == 0 ? 2 : 0);
-    int b = W.b0 + lb - (D == 0 ? 0 : 2);
-    if (D == 0 && W.wrap > 0) x = dlo + ((x - dlo) % W.wrap + W.wrap) % W.wrap;
-    else x = min(x, xmax);
-    if (D != 0 && W.wrap > 0) b = dlo + ((b - dlo) % W.wrap + W.wrap) % W.wrap;
-    else b = min(b, bmax);
-    double uc[5];
-    pencil_cell<D>(W, bx, S, sinf, xs, x, b, uc);
+  // 1. primitives of the pencil (cons2prim, CNS_hydro_K.h:15-34).  A box WB
+  // cells deep and periodic along D: the pencil's rows repeat the box's WB
+  // rows, so are the slopes of rows 1 .. LB-2, and face row WB is face row 0
+  // (the same cells in the same order); each thread converts one cell and
+  // computes one slope for every row holding them, the faces of rows
+  // 0 .. WB-1 are computed once.  Uniform over the workgroup.
+  const bool rep = D != 0 && W.wrap == WB && pvf == nullptr;
+  if (rep) {
+    const int lx = tid % G::LX, r = tid / G::LX;
     double q[NPRIM];
-    cons2prim(uc[0], uc[1], uc[2], uc[3], uc[4], c, q);
+    cons2prim_cell<D>(W, bx, S, sinf, xs, min(W.x0 + lx, xmax), dlo + r, c, q);
+    for (int lb = (r + 2) % WB; lb < G::LB; lb += WB)
 #pragma unroll
-    for (int n = 0; n < NPRIM; ++n) sq[n][l] = q[n];
+      for (int n = 0; n < NPRIM; ++n) sq[n][lb * G::LX + lx] = q[n];
+  } else {
+    for (int l = tid; l < G::NL; l += WNT) {
+      const int lx = l % G::LX, lb = l / G::LX;
+      int x = W.x0 + lx - (D == 0 ? 2 : 0);
+      int b = W.b0 + lb - (D == 0 ? 0 : 2);
+      if (D == 0 && W.wrap > 0) x = dlo + ((x - dlo) % W.wrap + W.wrap) % W.wrap;
+      else x = min(x, xmax);
+      if (D != 0 && W.wrap > 0) b = dlo + ((b - dlo) % W.wrap + W.wrap) % W.wrap;
+      else b = min(b, bmax);
+      double q[NPRIM];
+      cons2prim_cell<D>(W, bx, S, sinf, xs, x, b, c, q);
+#pragma unroll
+      for (int n = 0; n < NPRIM; ++n) sq[n][l] = q[n];
+    }
   }
   __syncthreads();
 
   // 2. slopes of the cells with both neighbours in the pencil (Riemann.h:19-120)
   constexpr int st = D == 0 ? 1 : G::LX;
-  for (int l = tid; l < G::NL; l += WNT) {
-    const int ld = D == 0 ? l % G::LX : l / G::LX;
-    if (ld < 1 || ld > (D == 0 ? G::LX : G::LB) - 2) continue;
+  if (rep) {
+    // row r + 2 (rows 2 .. 5, neighbours within 1 .. 6), stored in every row
+    // of 1 .. LB-2 that repeats it
+    const int lx = tid % G::LX, r = tid / G::LX;
+    const int l = (r + 2) * G::LX + lx;
     auto Q = [&](int o, int comp) { return sq[comp][l + o * st]; };
     double d[5];
     muscl_slopes<D>(Q, c, d);
+    for (int ld = (r + 2) % WB; ld <= G::LB - 2; ld += WB)
+      if (ld >= 1)
 #pragma unroll
-    for (int n = 0; n < 5; ++n) sd[n][l] = d[n];
+        for (int n = 0; n < 5; ++n) sd[n][ld * G::LX + lx] = d[n];
+  } else {
+    for (int l = tid; l < G::NL; l += WNT) {
+      const int ld = D == 0 ? l % G::LX : l / G::LX;
+      if (ld < 1 || ld > (D == 0 ? G::LX : G::LB) - 2) continue;
+      auto Q = [&](int o, int comp) { return sq[comp][l + o * st]; };
+      double d[5];
+      muscl_slopes<D>(Q, c, d);
+#pragma unroll
+      for (int n = 0; n < 5; ++n) sd[n][l] = d[n];
+    }
   }
   __syncthreads();
 
   // 3. faces (Riemann.h:251-337): face f between pencil cells lL and lL + st
   constexpr int NFT = (G::NF + WNT - 1) / WNT;
+  const int nf = rep ? WB * G::FX : G::NF;
   double F[NFT][5];
 #pragma unroll
   for (int r = 0; r < NFT; ++r) {
     const int f = tid + r * WNT;
-    if (f < G::NF) {
+    if (f < nf) {
       const int fx = f % G::FX, fb = f / G::FX;
       const int lL = D == 0 ? fb * G::LX + fx + 1 : (fb + 1) * G::LX + fx;
       const int lR = lL + st;
@@ -346,13 +416,7 @@ __global__ void __launch_bounds__(WNT) k_muscl_dir(const WenoBrick* __restrict__
   }
   __syncthreads();  // every pencil read done: the faces reuse the primitive slots
   double* sF = &sq[0][0];  // [5][NF]
-#pragma unroll
-  for (int r = 0; r < NFT; ++r) {
-    const int f = tid + r * WNT;
-    if (f < G::NF)
-#pragma unroll
-      for (int n = 0; n < 5; ++n) sF[n * G::NF + f] = F[r][n];
-  }
+  store_faces<G::FX, G::NF, NFT>(F, sF, nf, rep);
   __syncthreads();
   dir_epilogue<D, LAST, G::FX, G::NF>(W, bx, sa, sF, S, accf, s1f, outf);
 }
