@@ -127,29 +127,48 @@ __device__ __forceinline__ void store_faces(const double (&F)[NFT][5], double* s
     for (int n = 0; n < 5; ++n) sF[n * NF + WB * FX + tid] = F[0][n];
 }
 
-// The conserved state of pencil cell (x, b) of pass D (x wrapped or clamped
-// by the caller except, with side tables, beyond the box's x ends in the x
-// pass: there the neighbour's cell or the boundary rule of the side).
+// The x ends of a brick's box as the x pass reads them with side tables:
+// both entries and, for a neighbour side, that box's array, loaded once per
+// brick (uniform scalar loads) instead of per cell.
+struct XEnds {
+  XSide s[2];
+  Fab f[2];
+};
+
 template <int D>
-__device__ __forceinline__ void pencil_cell(const WenoBrick& W, const BoxDev& bx, const Fab& S, const Fab* sinf,
-                                            const XSide* xs, int x, int b, double u[5]) {
+__device__ __forceinline__ bool load_xends(const WenoBrick& W, const Fab* sinf, const XSide* xs, XEnds& e) {
+  if (D != 0 || xs == nullptr) return false;
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    e.s[side] = xs[2 * W.box + side];
+    if (e.s[side].kind == 1) e.f[side] = sinf[e.s[side].src];
+  }
+  return true;
+}
+
+// The conserved state of pencil cell (x, b) of pass D (x wrapped or clamped
+// by the caller except, with side tables (xon), beyond the box's x ends in
+// the x pass: there the neighbour's cell or the boundary rule of the side).
+template <int D>
+__device__ __forceinline__ void pencil_cell(const WenoBrick& W, const BoxDev& bx, const Fab& S, bool xon,
+                                            const XEnds& xe, int x, int b, double u[5]) {
   int i, j, k;
   brick_ijk<D>(x, b, W.c, i, j, k);
-  if (D == 0 && xs != nullptr && (x < bx.lo[0] || x > bx.hi[0])) {
-    const XSide& X = xs[2 * W.box + (x > bx.hi[0] ? 1 : 0)];
+  if (D == 0 && xon && (x < bx.lo[0] || x > bx.hi[0])) {
+    const int side = x > bx.hi[0] ? 1 : 0;
+    const XSide& X = xe.s[side];
     if (X.kind == 1) {
-      const Fab F = sinf[X.src];
+      const Fab& F = xe.f[side];
 #pragma unroll
       for (int n = 0; n < 5; ++n) u[n] = F(i + X.shift, j, k, n);
       return;
     }
     if (X.kind == 2) {
       // k_fill_bc's rule (AMReX foextrap / reflect_even / reflect_odd)
-      const int lo_side = x < bx.lo[0];
 #pragma unroll
       for (int n = 0; n < 5; ++n) {
         const int ty = X.type[n];
-        const int sx = ty == 2 ? X.bnd : (lo_side ? 2 * X.bnd - 1 - i : 2 * X.bnd + 1 - i);
+        const int sx = ty == 2 ? X.bnd : (side == 0 ? 2 * X.bnd - 1 - i : 2 * X.bnd + 1 - i);
         const double v = S(sx, j, k, n);
         u[n] = ty == 4 ? -v : v;
       }
@@ -158,15 +177,6 @@ __device__ __forceinline__ void pencil_cell(const WenoBrick& W, const BoxDev& bx
   }
 #pragma unroll
   for (int n = 0; n < 5; ++n) u[n] = S(i, j, k, n);
-}
-
-// Primitives of pencil cell (x, b) of pass D.
-template <int D>
-__device__ __forceinline__ void cons2prim_cell(const WenoBrick& W, const BoxDev& bx, const Fab& S, const Fab* sinf,
-                                               const XSide* xs, int x, int b, const Closures& c, double q[NPRIM]) {
-  double uc[5];
-  pencil_cell<D>(W, bx, S, sinf, xs, x, b, uc);
-  cons2prim(uc[0], uc[1], uc[2], uc[3], uc[4], c, q);
 }
 
 // Viscous part of face f of a brick (f % FX along x, f / FX along the second
@@ -207,9 +217,9 @@ __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ 
 
   // 1. stage the cells (High_resolution.h:83-121, the pass direction's part)
   const int dlo = bx.lo[D];
-  auto stage_cell = [&](int x, int b, double v[G::NV]) {
-    double uc[5];
-    pencil_cell<D>(W, bx, S, sinf, xs, x, b, uc);
+  XEnds xe;
+  const bool xon = load_xends<D>(W, sinf, xs, xe);
+  auto stage_cell = [&](const double uc[5], double v[G::NV]) {
     const double rho = uc[URHO], momx = uc[UMX], momy = uc[UMY];
     const double momz = uc[UMZ], rhoet = uc[UET];
     const double rhoinv = 1.0 / rho;
@@ -236,24 +246,39 @@ __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ 
   const bool rep = D != 0 && W.wrap == WB && pvf == nullptr;
   if (rep) {
     const int lx = tid % G::LX, r = tid / G::LX;
-    double v[G::NV];
-    stage_cell(min(W.x0 + lx, xmax), dlo + r, v);
+    double uc[5], v[G::NV];
+    pencil_cell<D>(W, bx, S, xon, xe, min(W.x0 + lx, xmax), dlo + r, uc);
+    stage_cell(uc, v);
     for (int lb = (r + 3) % WB; lb < G::LB; lb += WB)
 #pragma unroll
       for (int n = 0; n < G::NV; ++n) sv[n][lb * G::LX + lx] = v[n];
   } else {
-    for (int l = tid; l < G::NL; l += WNT) {
-      const int lx = l % G::LX, lb = l / G::LX;
-      int x = W.x0 + lx - (D == 0 ? 3 : 0);
-      int b = W.b0 + lb - (D == 0 ? 0 : 3);
-      if (D == 0 && W.wrap > 0) x = dlo + ((x - dlo) % W.wrap + W.wrap) % W.wrap;
-      else x = min(x, xmax);
-      if (D != 0 && W.wrap > 0) b = dlo + ((b - dlo) % W.wrap + W.wrap) % W.wrap;
-      else b = min(b, bmax);
-      double v[G::NV];
-      stage_cell(x, b, v);
+    // every round's loads first, then the conversions
+    constexpr int NR = (G::NL + WNT - 1) / WNT;
+    double uc[NR][5];
 #pragma unroll
-      for (int n = 0; n < G::NV; ++n) sv[n][l] = v[n];
+    for (int r = 0; r < NR; ++r) {
+      const int l = tid + r * WNT;
+      if (l < G::NL) {
+        const int lx = l % G::LX, lb = l / G::LX;
+        int x = W.x0 + lx - (D == 0 ? 3 : 0);
+        int b = W.b0 + lb - (D == 0 ? 0 : 3);
+        if (D == 0 && W.wrap > 0) x = dlo + ((x - dlo) % W.wrap + W.wrap) % W.wrap;
+        else x = min(x, xmax);
+        if (D != 0 && W.wrap > 0) b = dlo + ((b - dlo) % W.wrap + W.wrap) % W.wrap;
+        else b = min(b, bmax);
+        pencil_cell<D>(W, bx, S, xon, xe, x, b, uc[r]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int l = tid + r * WNT;
+      if (l < G::NL) {
+        double v[G::NV];
+        stage_cell(uc[r], v);
+#pragma unroll
+        for (int n = 0; n < G::NV; ++n) sv[n][l] = v[n];
+      }
     }
   }
   __syncthreads();
@@ -342,27 +367,44 @@ __global__ void __launch_bounds__(WNT) k_muscl_dir(const WenoBrick* __restrict__
   // (the same cells in the same order); each thread converts one cell and
   // computes one slope for every row holding them, the faces of rows
   // 0 .. WB-1 are computed once.  Uniform over the workgroup.
+  XEnds xe;
+  const bool xon = load_xends<D>(W, sinf, xs, xe);
   const bool rep = D != 0 && W.wrap == WB && pvf == nullptr;
   if (rep) {
     const int lx = tid % G::LX, r = tid / G::LX;
-    double q[NPRIM];
-    cons2prim_cell<D>(W, bx, S, sinf, xs, min(W.x0 + lx, xmax), dlo + r, c, q);
+    double uc[5], q[NPRIM];
+    pencil_cell<D>(W, bx, S, xon, xe, min(W.x0 + lx, xmax), dlo + r, uc);
+    cons2prim(uc[0], uc[1], uc[2], uc[3], uc[4], c, q);
     for (int lb = (r + 2) % WB; lb < G::LB; lb += WB)
 #pragma unroll
       for (int n = 0; n < NPRIM; ++n) sq[n][lb * G::LX + lx] = q[n];
   } else {
-    for (int l = tid; l < G::NL; l += WNT) {
-      const int lx = l % G::LX, lb = l / G::LX;
-      int x = W.x0 + lx - (D == 0 ? 2 : 0);
-      int b = W.b0 + lb - (D == 0 ? 0 : 2);
-      if (D == 0 && W.wrap > 0) x = dlo + ((x - dlo) % W.wrap + W.wrap) % W.wrap;
-      else x = min(x, xmax);
-      if (D != 0 && W.wrap > 0) b = dlo + ((b - dlo) % W.wrap + W.wrap) % W.wrap;
-      else b = min(b, bmax);
-      double q[NPRIM];
-      cons2prim_cell<D>(W, bx, S, sinf, xs, x, b, c, q);
+    // every round's loads first, then the conversions
+    constexpr int NR = (G::NL + WNT - 1) / WNT;
+    double uc[NR][5];
 #pragma unroll
-      for (int n = 0; n < NPRIM; ++n) sq[n][l] = q[n];
+    for (int r = 0; r < NR; ++r) {
+      const int l = tid + r * WNT;
+      if (l < G::NL) {
+        const int lx = l % G::LX, lb = l / G::LX;
+        int x = W.x0 + lx - (D == 0 ? 2 : 0);
+        int b = W.b0 + lb - (D == 0 ? 0 : 2);
+        if (D == 0 && W.wrap > 0) x = dlo + ((x - dlo) % W.wrap + W.wrap) % W.wrap;
+        else x = min(x, xmax);
+        if (D != 0 && W.wrap > 0) b = dlo + ((b - dlo) % W.wrap + W.wrap) % W.wrap;
+        else b = min(b, bmax);
+        pencil_cell<D>(W, bx, S, xon, xe, x, b, uc[r]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int l = tid + r * WNT;
+      if (l < G::NL) {
+        double q[NPRIM];
+        cons2prim(uc[r][0], uc[r][1], uc[r][2], uc[r][3], uc[r][4], c, q);
+#pragma unroll
+        for (int n = 0; n < NPRIM; ++n) sq[n][l] = q[n];
+      }
     }
   }
   __syncthreads();
