@@ -1244,6 +1244,17 @@ int build_xsides(amr_boxtab_s* bt) {
     const amr_geom_t& g = bt->geoms[b];
     for (int d = 1; d < 3; ++d)
       if (!g.periodic[d] || B[b].lo[d] != g.dom_lo[d] || B[b].hi[d] != g.dom_hi[d]) return AMR_OK;
+  }
+  // a level's boxes now all span the same y and z extents: disjoint x
+  // intervals, searched by their lower ends
+  std::vector<int> by_lo;
+  if (!bt->batch) {
+    by_lo.resize(size_t(nb));
+    for (int b = 0; b < nb; ++b) by_lo[size_t(b)] = b;
+    std::sort(by_lo.begin(), by_lo.end(), [&](int a, int c) { return B[a].lo[0] < B[c].lo[0]; });
+  }
+  for (int b = 0; b < nb; ++b) {
+    const amr_geom_t& g = bt->geoms[b];
     const int len = g.dom_hi[0] - g.dom_lo[0] + 1;
     if (g.periodic[0] && B[b].lo[0] == g.dom_lo[0] && B[b].hi[0] == g.dom_hi[0]) continue;  // wraps
     if (B[b].hi[0] - B[b].lo[0] + 1 < NGHOST) return AMR_OK;
@@ -1271,10 +1282,11 @@ int build_xsides(amr_boxtab_s* bt) {
       }
       if (e0 < g.dom_lo[0] || e1 > g.dom_hi[0]) return AMR_OK;
       X.src = -1;
-      for (int o = 0; o < nb && !bt->batch; ++o)
-        if (B[o].lo[0] <= e0 && B[o].hi[0] >= e1 && B[o].lo[1] == B[b].lo[1] && B[o].hi[1] == B[b].hi[1] &&
-            B[o].lo[2] == B[b].lo[2] && B[o].hi[2] == B[b].hi[2])
-          X.src = o;
+      if (!bt->batch) {
+        // the last box starting at or before e0
+        auto it = std::upper_bound(by_lo.begin(), by_lo.end(), e0, [&](int v, int o) { return v < B[o].lo[0]; });
+        if (it != by_lo.begin() && B[*(it - 1)].hi[0] >= e1) X.src = *(it - 1);
+      }
       if (X.src < 0) return AMR_OK;
       X.kind = 1;
     }

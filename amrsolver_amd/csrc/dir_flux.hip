@@ -286,46 +286,79 @@ __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ 
   // 2. faces (numericalflux_globallaxsplit, High_resolution.h:162-246): face
   // f sits below stencil cell 3 of cells l0 .. l0+5 (stride st along D)
   constexpr int st = D == 0 ? 1 : G::LX;
-  constexpr int NFT = (G::NF + WNT - 1) / WNT;
-  const int nf = rep ? WB * G::FX : G::NF;
-  double F[NFT][5];
+  // wave-speed maximum over the stencil of the face at l0, and component n
+  // of its flux
+  auto face_me = [&](int l0) {
+    double me = sv[7][l0];
 #pragma unroll
-  for (int r = 0; r < NFT; ++r) {
-    const int f = tid + r * WNT;
-    if (f < nf) {
-      const int fx = f % G::FX, fb = f / G::FX;
-      const int l0 = fb * G::LX + fx;
-      double me = sv[7][l0];
+    for (int o = 1; o < 6; ++o) me = dmax(me, sv[7][l0 + o * st]);
+    return me;
+  };
+  auto face_comp = [&](int l0, double me, int n) {
+    double df[6], Pf[6];
 #pragma unroll
-      for (int o = 1; o < 6; ++o) me = dmax(me, sv[7][l0 + o * st]);
-#pragma unroll
-      for (int n = 0; n < 5; ++n) {
-        double df[6], Pf[6];
-#pragma unroll
-        for (int o = 0; o < 6; ++o) {
-          const int l = l0 + o * st;
-          const double ud = sv[5][l];
-          double flux;
-          if (n == URHO) flux = sv[1 + D][l];
-          else if (n == UET) flux = (sv[4][l] + sv[6][l]) * ud;
-          else if (n == 1 + D) flux = sv[n][l] * ud + sv[6][l];
-          else flux = sv[n][l] * ud;
-          df[o] = me * sv[n][l];
-          Pf[o] = flux;
-        }
-        double flx = weno5js(0.5 * (Pf[0] + df[0]), 0.5 * (Pf[1] + df[1]), 0.5 * (Pf[2] + df[2]),
-                             0.5 * (Pf[3] + df[3]), 0.5 * (Pf[4] + df[4]));
-        flx += weno5js(0.5 * (Pf[5] - df[5]), 0.5 * (Pf[4] - df[4]), 0.5 * (Pf[3] - df[3]),
-                       0.5 * (Pf[2] - df[2]), 0.5 * (Pf[1] - df[1]));
-        F[r][n] = flx;
-      }
-      if (pvf) add_visc_face<D, G::FX>(W, bx, pvf[W.box], f, F[r]);
+    for (int o = 0; o < 6; ++o) {
+      const int l = l0 + o * st;
+      const double ud = sv[5][l];
+      double flux;
+      if (n == URHO) flux = sv[1 + D][l];
+      else if (n == UET) flux = (sv[4][l] + sv[6][l]) * ud;
+      else if (n == 1 + D) flux = sv[n][l] * ud + sv[6][l];
+      else flux = sv[n][l] * ud;
+      df[o] = me * sv[n][l];
+      Pf[o] = flux;
     }
-  }
-  __syncthreads();  // every stencil read done: the faces reuse the cell slots
+    double flx = weno5js(0.5 * (Pf[0] + df[0]), 0.5 * (Pf[1] + df[1]), 0.5 * (Pf[2] + df[2]),
+                         0.5 * (Pf[3] + df[3]), 0.5 * (Pf[4] + df[4]));
+    flx += weno5js(0.5 * (Pf[5] - df[5]), 0.5 * (Pf[4] - df[4]), 0.5 * (Pf[3] - df[3]),
+                   0.5 * (Pf[2] - df[2]), 0.5 * (Pf[1] - df[1]));
+    return flx;
+  };
   double* sF = &sv[0][0];  // [5][NF]
-  store_faces<G::FX, G::NF, NFT>(F, sF, nf, rep);
-  __syncthreads();
+  if (D == 0 && pvf == nullptr) {
+    // one (face, component) pair per thread and round, u = n * NF + f: the
+    // 65th face of each row costs a fifth of a round, not a second round of
+    // all five components on one wave
+    constexpr int NU = 5 * G::NF;
+    constexpr int NUT = (NU + WNT - 1) / WNT;
+    double Fu[NUT];
+#pragma unroll
+    for (int r = 0; r < NUT; ++r) {
+      const int u = tid + r * WNT;
+      if (u < NU) {
+        const int n = u / G::NF, f = u - n * G::NF;
+        const int fx = f % G::FX, fb = f / G::FX;
+        const int l0 = fb * G::LX + fx;
+        Fu[r] = face_comp(l0, face_me(l0), n);
+      }
+    }
+    __syncthreads();  // every stencil read done: the faces reuse the cell slots
+#pragma unroll
+    for (int r = 0; r < NUT; ++r) {
+      const int u = tid + r * WNT;
+      if (u < NU) sF[u] = Fu[r];
+    }
+    __syncthreads();
+  } else {
+    constexpr int NFT = (G::NF + WNT - 1) / WNT;
+    const int nf = rep ? WB * G::FX : G::NF;
+    double F[NFT][5];
+#pragma unroll
+    for (int r = 0; r < NFT; ++r) {
+      const int f = tid + r * WNT;
+      if (f < nf) {
+        const int fx = f % G::FX, fb = f / G::FX;
+        const int l0 = fb * G::LX + fx;
+        const double me = face_me(l0);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) F[r][n] = face_comp(l0, me, n);
+        if (pvf) add_visc_face<D, G::FX>(W, bx, pvf[W.box], f, F[r]);
+      }
+    }
+    __syncthreads();  // every stencil read done: the faces reuse the cell slots
+    store_faces<G::FX, G::NF, NFT>(F, sF, nf, rep);
+    __syncthreads();
+  }
 
   dir_epilogue<D, LAST, G::FX, G::NF>(W, bx, sa, sF, S, accf, s1f, outf);
 }
