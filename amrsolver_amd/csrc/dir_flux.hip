@@ -64,21 +64,54 @@ __device__ __forceinline__ void brick_ijk(int x, int b, int c, int& i, int& j, i
   }
 }
 
+// The epilogue operands of the thread's cell, loaded when the kernel starts
+// so that their latency hides behind the staging and the faces: the sum of
+// the earlier passes (passes y and z) and, in the last pass, s1 where the
+// stage combination reads it.
+template <int D, int LAST>
+struct EpiOps {
+  bool on;
+  int i, j, k;
+  double a[5], s1[5];
+};
+
+template <int D, int LAST>
+__device__ __forceinline__ void epi_load(const WenoBrick& W, const BoxDev& bx, const StageArgs& sa,
+                                         const Fab* __restrict__ accf, const Fab* __restrict__ s1f,
+                                         EpiOps<D, LAST>& e) {
+  constexpr int BA = D == 2 ? 2 : 1;
+  const int tid = threadIdx.x;
+  const int cx = tid % WX, cb = tid / WX;
+  const int x = W.x0 + cx, b = W.b0 + cb;
+  e.on = tid < WX * WB && x <= bx.hi[0] && b <= bx.hi[BA];
+  if (!e.on) return;
+  brick_ijk<D>(x, b, W.c, e.i, e.j, e.k);
+  if (D != 0) {
+    const Fab A = accf[W.box];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) e.a[n] = A(e.i, e.j, e.k, n);
+  }
+  if (LAST) {
+    const StageCoef sc = box_stage(sa, W.box);
+    if (sc.mode != 2 && sc.a != 0.0) {
+      const Fab F = s1f[W.box];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) e.s1[n] = F(e.i, e.j, e.k, n);
+    }
+  }
+}
+
 // Flux difference of the thread's cell from the faces in sF ([5][NF], FX
 // faces per row) and, in the last pass, source and stage combination.
 template <int D, int LAST, int FX, int NF>
 __device__ __forceinline__ void dir_epilogue(const WenoBrick& W, const BoxDev& bx, const StageArgs& sa,
                                              const double* sF, const Fab& S, const Fab* __restrict__ accf,
-                                             const Fab* __restrict__ s1f, const Fab* __restrict__ outf) {
-  constexpr int BA = D == 2 ? 2 : 1;
+                                             const EpiOps<D, LAST>& e, const Fab* __restrict__ outf) {
   const int tid = threadIdx.x;
   // flux difference of the own cell (CNS_advance.cpp:384-390)
-  if (tid >= WX * WB) return;
+  if (!e.on) return;
+  const int i = e.i, j = e.j, k = e.k;
   const int cx = tid % WX, cb = tid / WX;
-  const int x = W.x0 + cx, b = W.b0 + cb;
-  if (x > bx.hi[0] || b > bx.hi[BA]) return;
-  int i, j, k;
-  brick_ijk<D>(x, b, W.c, i, j, k);
   const int f0 = cb * FX + cx;
   const int f1 = D == 0 ? f0 + 1 : f0 + FX;
   const double dxi = bx.dxi[D];
@@ -87,7 +120,7 @@ __device__ __forceinline__ void dir_epilogue(const WenoBrick& W, const BoxDev& b
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
       const double t = dxi * (sF[n * NF + f0] - sF[n * NF + f1]);
-      A(i, j, k, n) = D == 0 ? t : A(i, j, k, n) + t;
+      A(i, j, k, n) = D == 0 ? t : e.a[n] + t;
     }
     return;
   }
@@ -95,7 +128,7 @@ __device__ __forceinline__ void dir_epilogue(const WenoBrick& W, const BoxDev& b
   const Fab O = outf[W.box];
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
-    double L = A(i, j, k, n) + dxi * (sF[n * NF + f0] - sF[n * NF + f1]);
+    double L = e.a[n] + dxi * (sF[n * NF + f0] - sF[n * NF + f1]);
     if (sa.src) L = L + sa.src[W.box](i, j, k, n);  // user_source, CNS_advance.cpp:397-407
     if (sa.bfx != 0.0 && n == UMX) L = L + S(i, j, k, URHO) * sa.bfx;
     if (sa.bfx != 0.0 && n == UET) L = L + S(i, j, k, UMX) * sa.bfx;
@@ -103,7 +136,7 @@ __device__ __forceinline__ void dir_epilogue(const WenoBrick& W, const BoxDev& b
     if (sc.mode == 2) {
       v = L;
     } else {
-      const double a = (sc.a != 0.0) ? s1f[W.box](i, j, k, n) : 0.0;
+      const double a = (sc.a != 0.0) ? e.s1[n] : 0.0;
       v = stage_combine(sc, a, S(i, j, k, n), L);
     }
     O(i, j, k, n) = v;
@@ -211,6 +244,8 @@ __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ 
   const BoxDev bx = boxes[W.box];
   const Fab S = sinf[W.box];
   const int tid = threadIdx.x;
+  EpiOps<D, LAST> epi;
+  epi_load<D, LAST>(W, bx, sa, accf, s1f, epi);
   constexpr int BA = D == 2 ? 2 : 1;  // axis of the brick's second coordinate
   const int xmax = bx.hi[0] + (D == 0 ? NGHOST : 0);
   const int bmax = bx.hi[BA] + (D == 0 ? 0 : NGHOST);
@@ -360,7 +395,7 @@ __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ 
     __syncthreads();
   }
 
-  dir_epilogue<D, LAST, G::FX, G::NF>(W, bx, sa, sF, S, accf, s1f, outf);
+  dir_epilogue<D, LAST, G::FX, G::NF>(W, bx, sa, sF, S, accf, epi, outf);
 }
 
 // Brick geometry of the MUSCL pass D: faces f need cells f-2 .. f+1 (the
@@ -389,6 +424,8 @@ __global__ void __launch_bounds__(WNT) k_muscl_dir(const WenoBrick* __restrict__
   const BoxDev bx = boxes[W.box];
   const Fab S = sinf[W.box];
   const int tid = threadIdx.x;
+  EpiOps<D, LAST> epi;
+  epi_load<D, LAST>(W, bx, sa, accf, s1f, epi);
   constexpr int BA = D == 2 ? 2 : 1;
   const int xmax = bx.hi[0] + (D == 0 ? NGHOST : 0);
   const int bmax = bx.hi[BA] + (D == 0 ? 0 : NGHOST);
@@ -493,7 +530,7 @@ __global__ void __launch_bounds__(WNT) k_muscl_dir(const WenoBrick* __restrict__
   double* sF = &sq[0][0];  // [5][NF]
   store_faces<G::FX, G::NF, NFT>(F, sF, nf, rep);
   __syncthreads();
-  dir_epilogue<D, LAST, G::FX, G::NF>(W, bx, sa, sF, S, accf, s1f, outf);
+  dir_epilogue<D, LAST, G::FX, G::NF>(W, bx, sa, sF, S, accf, epi, outf);
 }
 
 }  // namespace
