@@ -1802,7 +1802,8 @@ int cfl_step(amr_boxtab_s* bt, const amr_closures_t& cls, const amr_scheme_t& sc
   hipStream_t st = bt->ctx->stream;
   // per-box partial minima straight into the policy kernel (no separate
   // per-box reduction)
-  HIPCHK(launch_estdt_boxes(bt->bt, ds->d, make_closures(cls), bt->d_partial, nullptr, st));
+  HIPCHK(launch_estdt_boxes(bt->bt, ds->d, make_closures(cls), bt->d_partial, nullptr, st, bt->d_clock, bt->batch,
+                            stop_time));
   HIPCHK(launch_dt_policy(bt->bt.nbox, bt->batch, bt->d_partial, estdt_box_blocks(bt->bt), bt->d_clock, cfl,
                           stop_time, bt->d_box_dt, st));
   return advance_impl(bt, cls, sch, state, 1.0, bt->d_box_dt);

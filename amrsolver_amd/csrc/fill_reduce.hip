@@ -222,9 +222,15 @@ __device__ __forceinline__ double block_min(double v, double* sm) {
 
 // per-box minimum: blockIdx.y = box, partial[box * gridDim.x + blockIdx.x]
 __global__ void __launch_bounds__(RBLK) k_estdt_boxes(const BoxDev* boxes, const Fab* st, Closures c,
-                                                      double* partial) {
+                                                      double* partial, const Clock* clk, int batch, double stop) {
   __shared__ double sm[RBLK];
   const int b = blockIdx.y;
+  // a problem whose clock has stopped takes step 0 whatever its estimate:
+  // its cells are not read (uniform over the block)
+  if (clk != nullptr && clock_done(clk[batch ? b : 0], stop)) {
+    if (threadIdx.x == 0) partial[size_t(b) * gridDim.x + blockIdx.x] = 1.79769313486231570815e+308;
+    return;
+  }
   const BoxDev bx = boxes[b];
   const Fab u = st[b];
   double dt = 1.79769313486231570815e+308;
@@ -419,8 +425,7 @@ hipError_t launch_estdt_scratch(const BoxTable& bt, const Closures& c, void* scr
 // problem within 1e-14 of stop_time takes no step (0); a non-positive or
 // NaN minimum marks the problem bad and stops it.
 __device__ double clock_step(Clock& k, double emin, double cfl, double stop) {
-  if (k.bad) return 0.0;
-  if (stop >= 0.0 && k.t >= stop - 1e-14) return 0.0;
+  if (clock_done(k, stop)) return 0.0;
   if (!(emin > 0.0)) {
     k.bad = 1;
     return 0.0;
@@ -618,9 +623,10 @@ int estdt_box_blocks(const BoxTable& bt) {
 }
 
 hipError_t launch_estdt_boxes(const BoxTable& bt, const Fab* st, const Closures& c, double* partial, double* result,
-                              hipStream_t s) {
+                              hipStream_t s, const Clock* clk, bool batch, double stop_time) {
   int nper = estdt_box_blocks(bt);
-  hipLaunchKernelGGL(k_estdt_boxes, dim3(nper, bt.nbox), dim3(RBLK), 0, s, bt.d_boxes, st, c, partial);
+  hipLaunchKernelGGL(k_estdt_boxes, dim3(nper, bt.nbox), dim3(RBLK), 0, s, bt.d_boxes, st, c, partial, clk,
+                     batch ? 1 : 0, stop_time);
   if (result) hipLaunchKernelGGL(k_min_boxes, dim3(bt.nbox), dim3(RBLK), 0, s, partial, nper, result);
   return hipGetLastError();
 }

@@ -298,10 +298,14 @@ int reduce_blocks(const BoxTable& bt);
 hipError_t launch_estdt(const BoxTable& bt, const Fab* st, const Closures& c, double* partial,
                         double* result, hipStream_t s);
 // per-box minimum of cns_estdt: result[b] for every box (estdt_box_blocks
-// partial minima per box in `partial`; result may be null)
+// partial minima per box in `partial`; result may be null).  With clocks
+// (clk; batch: one per box, else one for the table) the boxes of stopped
+// problems are skipped (their partials hold the largest double).
+struct Clock;
 int estdt_box_blocks(const BoxTable& bt);
 hipError_t launch_estdt_boxes(const BoxTable& bt, const Fab* st, const Closures& c, double* partial,
-                              double* result, hipStream_t s);
+                              double* result, hipStream_t s, const Clock* clk = nullptr, bool batch = false,
+                              double stop_time = -1.0);
 // Device-side time-step control: the clock of each problem of a box table
 // (the level, or each problem of a batch).  first = 1: the next step takes
 // the CFL estimate itself (computeInitialDt); bad = 1: a non-physical state
@@ -310,6 +314,11 @@ struct Clock {
   double t, dt;
   int32_t steps, first, bad, pad_;
 };
+// A problem that takes no more steps: stopped by a non-physical state, or
+// within 1e-14 of stop_time (stop_time >= 0).
+__host__ __device__ inline bool clock_done(const Clock& k, double stop) {
+  return k.bad != 0 || (stop >= 0.0 && k.t >= stop - 1e-14);
+}
 // Per box (est: nper partial minima per box of cns_estdt, without the CFL
 // factor, as launch_estdt_boxes writes them to `partial`) the time step of
 // its problem (batch: box b is problem b; else one problem) under the policy
