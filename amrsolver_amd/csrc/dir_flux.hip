@@ -32,10 +32,11 @@ namespace amr {
 
 namespace {
 
-// One thread per cell of a WX x WB brick; the faces beyond (65 x 4 in the x
-// pass, 64 x 5 in the others) take a second round on the first threads
-// (five-wave workgroups with one face per thread ran the 1D batch 18%
-// slower).
+// One thread per cell of a WX x WB brick.  The faces beyond WX x WB (65 x 4
+// in the x pass, 64 x 5 in the others) take a second round on the first
+// threads, except where the kernels avoid it (repeated rows in the y and z
+// passes; one (face, component) pair per thread in the WENO x pass);
+// five-wave workgroups with one face per thread ran the 1D batch 18% slower.
 constexpr int WX = 64, WB = 4, WNT = WX * WB;
 
 // Brick geometry of pass D: loaded cells LX x LB, faces FX x FB (x fastest);
