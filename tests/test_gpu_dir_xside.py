@@ -18,6 +18,9 @@ CASES = [
     ("sod_foextrap", (128, 8, 8), (2, 0, 0), (2, 0, 0), (0, 1, 1), (32, 8, 8), (32, 4, 8)),
     ("slipwall_reflect", (96, 8, 8), (4, 0, 0), (3, 0, 0), (0, 1, 1), (24, 8, 8), (24, 8, 4)),
     ("periodic_x", (64, 8, 8), (0, 0, 0), (0, 0, 0), (1, 1, 1), (16, 8, 8), (16, 4, 8)),
+    # a 4-deep periodic y (repeated pencil rows in the y pass) with z 8 deep, and the reverse
+    ("rows4_y", (64, 4, 8), (2, 0, 0), (3, 0, 0), (0, 1, 1), (16, 4, 8), (16, 4, 4)),
+    ("rows4_z", (64, 8, 4), (4, 0, 0), (2, 0, 0), (0, 1, 1), (16, 8, 4), (16, 4, 4)),
 ]
 
 
