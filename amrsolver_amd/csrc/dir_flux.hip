@@ -67,19 +67,20 @@ __device__ __forceinline__ void brick_ijk(int x, int b, int c, int& i, int& j, i
 
 // The epilogue operands of the thread's cell, loaded when the kernel starts
 // so that their latency hides behind the staging and the faces: the sum of
-// the earlier passes (passes y and z) and, in the last pass, s1 where the
-// stage combination reads it.
-template <int D, int LAST>
+// the earlier passes (passes y and z) and, in the last pass, s1 and (PU) the
+// stage input where the stage combination reads them (PU costs the WENO
+// pass, at 122 VGPRs without it, a wave per SIMD).
+template <int D, int LAST, bool PU>
 struct EpiOps {
   bool on;
   int i, j, k;
-  double a[5], s1[5];
+  double a[5], s1[5], u[5];
 };
 
-template <int D, int LAST>
+template <int D, int LAST, bool PU>
 __device__ __forceinline__ void epi_load(const WenoBrick& W, const BoxDev& bx, const StageArgs& sa,
-                                         const Fab* __restrict__ accf, const Fab* __restrict__ s1f,
-                                         EpiOps<D, LAST>& e) {
+                                         const Fab* __restrict__ sinf, const Fab* __restrict__ accf,
+                                         const Fab* __restrict__ s1f, EpiOps<D, LAST, PU>& e) {
   constexpr int BA = D == 2 ? 2 : 1;
   const int tid = threadIdx.x;
   const int cx = tid % WX, cb = tid / WX;
@@ -99,15 +100,20 @@ __device__ __forceinline__ void epi_load(const WenoBrick& W, const BoxDev& bx, c
 #pragma unroll
       for (int n = 0; n < 5; ++n) e.s1[n] = F(e.i, e.j, e.k, n);
     }
+    if (PU && sc.mode != 2) {
+      const Fab U = sinf[W.box];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) e.u[n] = U(e.i, e.j, e.k, n);
+    }
   }
 }
 
 // Flux difference of the thread's cell from the faces in sF ([5][NF], FX
 // faces per row) and, in the last pass, source and stage combination.
-template <int D, int LAST, int FX, int NF>
+template <int D, int LAST, int FX, int NF, bool PU>
 __device__ __forceinline__ void dir_epilogue(const WenoBrick& W, const BoxDev& bx, const StageArgs& sa,
                                              const double* sF, const Fab& S, const Fab* __restrict__ accf,
-                                             const EpiOps<D, LAST>& e, const Fab* __restrict__ outf) {
+                                             const EpiOps<D, LAST, PU>& e, const Fab* __restrict__ outf) {
   const int tid = threadIdx.x;
   // flux difference of the own cell (CNS_advance.cpp:384-390)
   if (!e.on) return;
@@ -138,7 +144,7 @@ __device__ __forceinline__ void dir_epilogue(const WenoBrick& W, const BoxDev& b
       v = L;
     } else {
       const double a = (sc.a != 0.0) ? e.s1[n] : 0.0;
-      v = stage_combine(sc, a, S(i, j, k, n), L);
+      v = stage_combine(sc, a, PU ? e.u[n] : S(i, j, k, n), L);
     }
     O(i, j, k, n) = v;
   }
@@ -245,8 +251,8 @@ __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ 
   const BoxDev bx = boxes[W.box];
   const Fab S = sinf[W.box];
   const int tid = threadIdx.x;
-  EpiOps<D, LAST> epi;
-  epi_load<D, LAST>(W, bx, sa, accf, s1f, epi);
+  EpiOps<D, LAST, false> epi;
+  epi_load<D, LAST, false>(W, bx, sa, sinf, accf, s1f, epi);
   constexpr int BA = D == 2 ? 2 : 1;  // axis of the brick's second coordinate
   const int xmax = bx.hi[0] + (D == 0 ? NGHOST : 0);
   const int bmax = bx.hi[BA] + (D == 0 ? 0 : NGHOST);
@@ -396,7 +402,7 @@ __global__ void __launch_bounds__(WNT) k_weno_dir(const WenoBrick* __restrict__ 
     __syncthreads();
   }
 
-  dir_epilogue<D, LAST, G::FX, G::NF>(W, bx, sa, sF, S, accf, epi, outf);
+  dir_epilogue<D, LAST, G::FX, G::NF, false>(W, bx, sa, sF, S, accf, epi, outf);
 }
 
 // Brick geometry of the MUSCL pass D: faces f need cells f-2 .. f+1 (the
@@ -425,8 +431,8 @@ __global__ void __launch_bounds__(WNT) k_muscl_dir(const WenoBrick* __restrict__
   const BoxDev bx = boxes[W.box];
   const Fab S = sinf[W.box];
   const int tid = threadIdx.x;
-  EpiOps<D, LAST> epi;
-  epi_load<D, LAST>(W, bx, sa, accf, s1f, epi);
+  EpiOps<D, LAST, true> epi;
+  epi_load<D, LAST, true>(W, bx, sa, sinf, accf, s1f, epi);
   constexpr int BA = D == 2 ? 2 : 1;
   const int xmax = bx.hi[0] + (D == 0 ? NGHOST : 0);
   const int bmax = bx.hi[BA] + (D == 0 ? 0 : NGHOST);
@@ -531,7 +537,7 @@ __global__ void __launch_bounds__(WNT) k_muscl_dir(const WenoBrick* __restrict__
   double* sF = &sq[0][0];  // [5][NF]
   store_faces<G::FX, G::NF, NFT>(F, sF, nf, rep);
   __syncthreads();
-  dir_epilogue<D, LAST, G::FX, G::NF>(W, bx, sa, sF, S, accf, epi, outf);
+  dir_epilogue<D, LAST, G::FX, G::NF, true>(W, bx, sa, sF, S, accf, epi, outf);
 }
 
 }  // namespace
